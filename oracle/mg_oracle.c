@@ -1,0 +1,719 @@
+/*
+ * CPU oracle for the multigrid Poisson-Helmholtz path.  TEST INFRASTRUCTURE
+ * ONLY: this file is the checker that the GPU library is compared against.
+ * Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * may load it; the product library never links or calls it.
+ *
+ * It restates, in plain C99 and with the same floating-point evaluation
+ * order, the reference algorithm on a uniform `multigrid` grid:
+ *
+ *   storage      grid/multigrid.h:16-73 (one AoS array per level, GHOSTS=2,
+ *                (2^l+4)^dim cells, i-major, the k index fastest)
+ *   traversal    grid/multigrid.h:191-241 (foreach_level / foreach order)
+ *   ghost fill   grid/multigrid.h:543-594 (box_boundary_level, BGHOSTS=1,
+ *                direction order right,left,top,bottom,front,back, full
+ *                tangential extent) and grid/multigrid.h:638-695 (periodic
+ *                copy of both ghost layers, level-0 special case)
+ *   BC formulas  common.h:957-960 (dirichlet / neumann and homogeneous forms),
+ *                grid/cartesian-common.h:464-467 (symmetry)
+ *   restriction  grid/multigrid-common.h:26-32 (restriction_average),
+ *                42-63 (face_average), 368-404 (multigrid_restriction)
+ *   prolongation grid/multigrid-common.h:124-140 (bilinear)
+ *   relax        poisson.h:271-338 (GS in place, or weighted Jacobi 2/3)
+ *   residual     poisson.h:373-389 (the !TREE "naive" form)
+ *   mg_cycle     poisson.h:32-90
+ *   mg_solve     poisson.h:139-235 (nrelax adaptation 198-204)
+ *   poisson      poisson.h:402-441
+ *
+ * One extension that the reference does not have: a red-black Gauss-Seidel
+ * smoother (colour = parity of i+j+k, red first, ghost fill between the two
+ * colours).  It is the order-independent smoother that the GPU path uses by
+ * default, so the oracle defines its exact semantics.
+ *
+ * Build once per dimension: -DDIM=1, 2 or 3, with -ffp-contract=off so that
+ * no multiply-add is fused (the reference C is compiled without FMA).
+ * Built with -fopenmp, the loops are `omp for schedule(static)` over the
+ * outer index exactly like the reference's foreach; the in-place GS smoother
+ * then races across thread chunks, as the reference's does.  Run with one
+ * thread for the deterministic oracle.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef DIM
+#error "compile with -DDIM=1, 2 or 3"
+#endif
+
+#define GHOSTS 2
+
+enum { F_A, F_B, F_RES, F_DA, F_LAMBDA, F_C, F_AX };
+#define NV (F_AX + DIM)
+
+enum { BC_SYMMETRY = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2, BC_PERIODIC = 3 };
+enum { SM_GS = 0, SM_JACOBI = 1, SM_RBGS = 2 };
+
+typedef struct {
+  int i;
+  double resb, resa, sum;
+  int nrelax, minlevel;
+} orc_mgstats;
+
+typedef struct {
+  int depth;
+  double **d;
+  double L0, X0, Y0, Z0;
+  int bc[2 * DIM];
+  double *bcv[2 * DIM];
+  int periodic[DIM];
+  int lambda_field;
+  double lambda_val;
+  int alpha_field;
+  double alpha_val;
+  int smoother;
+  double TOLERANCE;
+  int NITERMAX, NITERMIN;
+} orc_grid;
+
+#if DIM == 1
+#define POWD(n) ((size_t)(n))
+#elif DIM == 2
+#define POWD(n) ((size_t)(n) * (size_t)(n))
+#else
+#define POWD(n) ((size_t)(n) * (size_t)(n) * (size_t)(n))
+#endif
+
+static inline int msize(int l) { return (1 << l) + 2 * GHOSTS; }
+
+/* cell offset of (i,j,k) at level l, multigrid.h:59-72 */
+static inline size_t cidx(int l, int i, int j, int k) {
+  size_t m = (size_t)msize(l);
+#if DIM == 1
+  (void)m; (void)j; (void)k;
+  return (size_t)i;
+#elif DIM == 2
+  (void)k;
+  return (size_t)i * m + (size_t)j;
+#else
+  return ((size_t)i * m + (size_t)j) * m + (size_t)k;
+#endif
+}
+
+/* cell stride of a unit step along dimension d at level l */
+static inline ptrdiff_t dstride(int l, int d) {
+  ptrdiff_t m = msize(l);
+#if DIM == 1
+  (void)d; (void)m;
+  return 1;
+#elif DIM == 2
+  return d == 0 ? m : 1;
+#else
+  return d == 0 ? m * m : d == 1 ? m : 1;
+#endif
+}
+
+#define REC(g, l, i, j, k) (&(g)->d[l][cidx(l, i, j, k) * NV])
+
+static inline double delta_l(const orc_grid *g, int l) {
+  return g->L0 * (1. / (1 << l)); /* cartesian-common.h:11, multigrid.h:14 */
+}
+
+/* ---------------------------------------------------------------- setup */
+
+orc_grid *orc_new(int depth) {
+  orc_grid *g = calloc(1, sizeof(orc_grid));
+  g->depth = depth;
+  g->d = calloc(depth + 1, sizeof(double *));
+  for (int l = 0; l <= depth; l++) {
+    size_t len = POWD(msize(l)) * NV;
+    g->d[l] = calloc(len, sizeof(double)); /* reset(all, 0.), multigrid.h:758 */
+  }
+  g->L0 = 1.;
+  g->lambda_field = 0;
+  g->lambda_val = 0.;
+  g->alpha_field = 0;
+  g->alpha_val = 1.;
+  g->smoother = SM_GS;
+  g->TOLERANCE = 1e-3; /* poisson.h:102-103 */
+  g->NITERMAX = 100;
+  g->NITERMIN = 1;
+  return g;
+}
+
+void orc_free(orc_grid *g) {
+  if (!g) return;
+  for (int l = 0; l <= g->depth; l++) free(g->d[l]);
+  free(g->d);
+  for (int b = 0; b < 2 * DIM; b++) free(g->bcv[b]);
+  free(g);
+}
+
+int orc_dimension(void) { return DIM; }
+int orc_nvar(void) { return NV; }
+
+void orc_set_geometry(orc_grid *g, double L0, double X0, double Y0, double Z0) {
+  g->L0 = L0; g->X0 = X0; g->Y0 = Y0; g->Z0 = Z0;
+}
+
+void orc_set_niter(orc_grid *g, int nitermax, int nitermin) {
+  g->NITERMAX = nitermax;
+  g->NITERMIN = nitermin;
+}
+
+void orc_set_tolerance(orc_grid *g, double tol) { g->TOLERANCE = tol; }
+void orc_set_smoother(orc_grid *g, int s) { g->smoother = s; }
+void orc_set_lambda_const(orc_grid *g, double v) { g->lambda_field = 0; g->lambda_val = v; }
+void orc_use_lambda_field(orc_grid *g) { g->lambda_field = 1; }
+void orc_set_alpha_const(orc_grid *g, double v) { g->alpha_field = 0; g->alpha_val = v; }
+void orc_use_alpha_field(orc_grid *g) { g->alpha_field = 1; }
+
+/* Boundary condition of `a` on direction b (0 right, 1 left, 2 top,
+   3 bottom, 4 front, 5 back).  `vals` holds the BC expression (Dirichlet
+   value or Neumann gradient) at the finest level for every tangential
+   index over the full extent 0..n+3 of each tangential dimension, in the
+   order the tangential indices appear in (i,j,k); NULL means 0. */
+void orc_set_bc(orc_grid *g, int b, int kind, const double *vals) {
+  g->bc[b] = kind;
+  free(g->bcv[b]);
+  g->bcv[b] = NULL;
+  if (kind == BC_PERIODIC) {
+    g->periodic[b / 2] = 1;
+    g->bc[b ^ 1] = BC_PERIODIC;
+    return;
+  }
+  if (vals) {
+    size_t len = 1;
+    for (int t = 0; t < DIM - 1; t++) len *= (size_t)msize(g->depth);
+    g->bcv[b] = malloc(len * sizeof(double));
+    memcpy(g->bcv[b], vals, len * sizeof(double));
+  }
+}
+
+double *orc_level_data(orc_grid *g, int l) { return g->d[l]; }
+
+/* copy an interior SoA array (C order over i,j,k) in or out */
+static void copy_cells(orc_grid *g, int f, int l, double *buf, int in) {
+  int n = 1 << l;
+  size_t p = 0;
+#if DIM >= 2
+  int nj = n;
+#else
+  int nj = 1;
+#endif
+#if DIM >= 3
+  int nk = n;
+#else
+  int nk = 1;
+#endif
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < nj; j++)
+      for (int k = 0; k < nk; k++, p++) {
+        double *r = REC(g, l, i + GHOSTS, j + (DIM >= 2) * GHOSTS, k + (DIM >= 3) * GHOSTS);
+        if (in) r[f] = buf[p];
+        else buf[p] = r[f];
+      }
+}
+
+void orc_set_cell(orc_grid *g, int f, int l, const double *src) { copy_cells(g, f, l, (double *)src, 1); }
+void orc_get_cell(orc_grid *g, int f, int l, double *dst) { copy_cells(g, f, l, dst, 0); }
+
+/* face component c: n+1 faces along c, n along the others (C order) */
+static void copy_faces(orc_grid *g, int c, int l, double *buf, int in) {
+  int n = 1 << l;
+  int ext[3] = {1, 1, 1};
+  for (int d = 0; d < DIM; d++) ext[d] = n + (d == c);
+  size_t p = 0;
+  for (int i = 0; i < ext[0]; i++)
+    for (int j = 0; j < ext[1]; j++)
+      for (int k = 0; k < ext[2]; k++, p++) {
+        double *r = REC(g, l, i + GHOSTS, j + (DIM >= 2) * GHOSTS, k + (DIM >= 3) * GHOSTS);
+        if (in) r[F_AX + c] = buf[p];
+        else buf[p] = r[F_AX + c];
+      }
+}
+
+void orc_set_face(orc_grid *g, int c, int l, const double *src) { copy_faces(g, c, l, (double *)src, 1); }
+void orc_get_face(orc_grid *g, int c, int l, double *dst) { copy_faces(g, c, l, dst, 0); }
+
+/* ------------------------------------------------------------ ghost fill */
+
+/* box_boundary_level, multigrid.h:543-594, for one cell-centred field.
+   mode 0: the field's inhomogeneous BC (only `a` has one, and only values
+   at the finest level); mode 1: the homogeneous version. */
+static void box_boundary(orc_grid *g, int f, int l, int homogeneous, int use_bc) {
+  int n = 1 << l, m = msize(l);
+  double D = delta_l(g, l);
+  for (int b = 0; b < 2 * DIM; b++) {
+    int kind = use_bc ? g->bc[b] : BC_SYMMETRY;
+    if (kind == BC_PERIODIC) continue;
+    int d = b / 2, side = (b % 2 == 0) ? 1 : -1;
+    int plane = side > 0 ? n + GHOSTS - 1 : GHOSTS;
+    ptrdiff_t sd = dstride(l, d);
+    const double *vals = (use_bc && !homogeneous && l == g->depth) ? g->bcv[b] : NULL;
+    /* tangential dimensions in (i,j,k) order */
+    int t1 = -1, t2 = -1;
+    for (int e = 0; e < DIM; e++)
+      if (e != d) { if (t1 < 0) t1 = e; else t2 = e; }
+    int e1 = t1 >= 0 ? m : 1, e2 = t2 >= 0 ? m : 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int u = 0; u < e1; u++)
+      for (int v = 0; v < e2; v++) {
+        int idx[3] = {0, 0, 0};
+        idx[d] = plane;
+        if (t1 >= 0) idx[t1] = u;
+        if (t2 >= 0) idx[t2] = v;
+        double *in = REC(g, l, idx[0], idx[1], idx[2]);
+        double *gh = in + side * sd * NV;
+        double s = in[f], r;
+        double expr = vals ? vals[(size_t)u * e2 + v] : 0.;
+        switch (kind) {
+        case BC_DIRICHLET: r = homogeneous ? -s : 2. * (expr) - s; break;
+        case BC_NEUMANN: r = homogeneous ? s : D * (expr) + s; break;
+        default: r = s; break;
+        }
+        gh[f] = r;
+      }
+  }
+}
+
+/* periodic_boundary_level_x, multigrid.h:638-695: both ghost layers */
+static void periodic_copy(orc_grid *g, int f, int l) {
+  int any = 0;
+  for (int d = 0; d < DIM; d++) any |= g->periodic[d];
+  if (!any) return;
+  int n = 1 << l, m = msize(l);
+  if (l == 0) {
+    /* foreach_level(0) ... foreach_neighbor(): every cell of the 5^dim
+       block takes the root value (once per periodic dimension) */
+    size_t len = POWD(m);
+    double v = REC(g, 0, GHOSTS, (DIM >= 2) * GHOSTS, (DIM >= 3) * GHOSTS)[f];
+    for (size_t c = 0; c < len; c++) g->d[0][c * NV + f] = v;
+    return;
+  }
+  for (int d = 0; d < DIM; d++) {
+    if (!g->periodic[d]) continue;
+    ptrdiff_t sd = dstride(l, d);
+    int t1 = -1, t2 = -1;
+    for (int e = 0; e < DIM; e++)
+      if (e != d) { if (t1 < 0) t1 = e; else t2 = e; }
+    int e1 = t1 >= 0 ? m : 1, e2 = t2 >= 0 ? m : 1;
+    for (int u = 0; u < e1; u++)
+      for (int v = 0; v < e2; v++) {
+        int idx[3] = {0, 0, 0};
+        if (t1 >= 0) idx[t1] = u;
+        if (t2 >= 0) idx[t2] = v;
+        double *base = REC(g, l, idx[0], idx[1], idx[2]);
+        for (int i = 0; i < GHOSTS; i++)
+          base[i * sd * NV + f] = base[(i + n) * sd * NV + f];
+        for (int i = n + GHOSTS; i < n + 2 * GHOSTS; i++)
+          base[i * sd * NV + f] = base[(i - n) * sd * NV + f];
+      }
+  }
+}
+
+/* boundary_level(list, l) for one cell-centred field (box then periodic,
+   the order in which init_grid registers them, multigrid.h:733-745) */
+static void boundary_level(orc_grid *g, int f, int l, int homogeneous, int use_bc) {
+  box_boundary(g, f, l, homogeneous, use_bc);
+  periodic_copy(g, f, l);
+}
+
+void orc_boundary(orc_grid *g, int f, int l, int homogeneous) {
+  boundary_level(g, f, l, homogeneous, f == F_A || f == F_DA);
+}
+
+/* Face vectors have no box BC (cartesian-common.h:566-576).  The periodic
+   copy includes a component only when its "VT" companion carries the
+   periodic BC (multigrid.h:636-651, cartesian-common.h:903-917): that is
+   true for x in 2D and 3D and for y in 3D, not for y in 2D nor z in 3D. */
+static void periodic_copy_faces(orc_grid *g, int l) {
+  int n = 1 << l, m = msize(l);
+  for (int d = 0; d < DIM; d++) {
+    if (!g->periodic[d]) continue;
+    int copies = (DIM == 1) || d == 0 || (DIM == 3 && d == 1);
+    if (!copies) continue;
+    if (l == 0) {
+      for (int c = 0; c < DIM; c++) {
+        size_t len = POWD(m);
+        double v = REC(g, 0, GHOSTS, (DIM >= 2) * GHOSTS, (DIM >= 3) * GHOSTS)[F_AX + c];
+        for (size_t q = 0; q < len; q++) g->d[0][q * NV + F_AX + c] = v;
+      }
+      continue;
+    }
+    ptrdiff_t sd = dstride(l, d);
+    int t1 = -1, t2 = -1;
+    for (int e = 0; e < DIM; e++)
+      if (e != d) { if (t1 < 0) t1 = e; else t2 = e; }
+    int e1 = t1 >= 0 ? m : 1, e2 = t2 >= 0 ? m : 1;
+    for (int u = 0; u < e1; u++)
+      for (int v = 0; v < e2; v++) {
+        int idx[3] = {0, 0, 0};
+        if (t1 >= 0) idx[t1] = u;
+        if (t2 >= 0) idx[t2] = v;
+        double *base = REC(g, l, idx[0], idx[1], idx[2]);
+        for (int c = 0; c < DIM; c++) {
+          for (int i = 0; i < GHOSTS; i++)
+            base[i * sd * NV + F_AX + c] = base[(i + n) * sd * NV + F_AX + c];
+          for (int i = n + GHOSTS; i < n + 2 * GHOSTS; i++)
+            base[i * sd * NV + F_AX + c] = base[(i - n) * sd * NV + F_AX + c];
+        }
+      }
+  }
+}
+
+/* ---------------------------------------------------------- loop helpers */
+
+#if DIM == 1
+#define LOOP_LEVEL(l, n)                                  \
+  _Pragma("omp parallel for schedule(static)")            \
+  for (int i = GHOSTS; i < (n) + GHOSTS; i++) {           \
+    const int j = 0, k = 0; (void)j; (void)k;
+#define END_LOOP }
+#elif DIM == 2
+#define LOOP_LEVEL(l, n)                                  \
+  _Pragma("omp parallel for schedule(static)")            \
+  for (int i = GHOSTS; i < (n) + GHOSTS; i++)             \
+    for (int j = GHOSTS; j < (n) + GHOSTS; j++) {         \
+      const int k = 0; (void)k;
+#define END_LOOP }
+#else
+#define LOOP_LEVEL(l, n)                                  \
+  _Pragma("omp parallel for schedule(static)")            \
+  for (int i = GHOSTS; i < (n) + GHOSTS; i++)             \
+    for (int j = GHOSTS; j < (n) + GHOSTS; j++)           \
+      for (int k = GHOSTS; k < (n) + GHOSTS; k++) {
+#define END_LOOP }
+#endif
+
+/* ------------------------------------------------------------ restriction */
+
+/* restriction_average, multigrid-common.h:26-32: foreach_child order is
+   (i,j,k) with k fastest, multigrid.h:357-368 */
+static inline double child_sum(orc_grid *g, int f, int l, int i, int j, int k) {
+  double sum = 0.;
+  int fi = 2 * i - GHOSTS, fj = 2 * j - GHOSTS, fk = 2 * k - GHOSTS;
+#if DIM == 1
+  (void)fj; (void)fk;
+  for (int a = 0; a < 2; a++) sum += REC(g, l + 1, fi + a, 0, 0)[f];
+#elif DIM == 2
+  (void)fk;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) sum += REC(g, l + 1, fi + a, fj + b, 0)[f];
+#else
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int c = 0; c < 2; c++) sum += REC(g, l + 1, fi + a, fj + b, fk + c)[f];
+#endif
+  return sum;
+}
+
+static void restrict_level_average(orc_grid *g, int f, int l) {
+  int n = 1 << l;
+  LOOP_LEVEL(l, n)
+    REC(g, l, i, j, k)[f] = child_sum(g, f, l, i, j, k) / (1 << DIM);
+  END_LOOP
+}
+
+/* face_average, multigrid-common.h:42-58, with the tangential child order
+   that foreach_dimension's rotation gives each component (rotate.lex:163-170) */
+__attribute__((unused)) static const int face_tan[3][4][3] = {
+#if DIM == 3
+  {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}},
+  {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 1}},
+  {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}},
+#elif DIM == 2
+  {{0, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 0}},
+  {{0, 0, 0}, {1, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+  {{0}},
+#else
+  {{0}}, {{0}}, {{0}},
+#endif
+};
+
+static inline double face_avg(orc_grid *g, int c, int l, int i, int j, int k, int hi) {
+  int fi = 2 * i - GHOSTS, fj = 2 * j - GHOSTS, fk = 2 * k - GHOSTS;
+  int off[3] = {0, 0, 0};
+  off[c] = 2 * hi;
+#if DIM == 1
+  (void)fj; (void)fk;
+  return REC(g, l + 1, fi + off[0], 0, 0)[F_AX];
+#elif DIM == 2
+  (void)fk;
+  double s = 0.;
+  for (int t = 0; t < 2; t++)
+    s += REC(g, l + 1, fi + off[0] + face_tan[c][t][0], fj + off[1] + face_tan[c][t][1], 0)[F_AX + c];
+  return s / 2.;
+#else
+  double s = 0.;
+  for (int t = 0; t < 4; t++)
+    s += REC(g, l + 1, fi + off[0] + face_tan[c][t][0], fj + off[1] + face_tan[c][t][1],
+             fk + off[2] + face_tan[c][t][2])[F_AX + c];
+  return s / 4.;
+#endif
+}
+
+static void restrict_level_faces(orc_grid *g, int l) {
+  int n = 1 << l;
+  LOOP_LEVEL(l, n)
+    double *r = REC(g, l, i, j, k);
+    for (int c = 0; c < DIM; c++) {
+      r[F_AX + c] = face_avg(g, c, l, i, j, k, 0);
+      r[dstride(l, c) * NV + F_AX + c] = face_avg(g, c, l, i, j, k, 1);
+    }
+  END_LOOP
+}
+
+/* multigrid_restriction for {res}: levels depth-1..0, each followed by
+   boundary_iterate on that level (res inherits b's default symmetry BC) */
+void orc_restriction_res(orc_grid *g) {
+  for (int l = g->depth - 1; l >= 0; l--) {
+    restrict_level_average(g, F_RES, l);
+    boundary_level(g, F_RES, l, 0, 0);
+  }
+}
+
+/* poisson.h:418-420: restriction ({alpha, lambda}); lambda (listdef) is
+   restricted before alpha (listc) on each level, multigrid-common.h:388-396 */
+void orc_restriction_alpha_lambda(orc_grid *g) {
+  if (!g->alpha_field && !g->lambda_field) return;
+  for (int l = g->depth - 1; l >= 0; l--) {
+    if (g->lambda_field) restrict_level_average(g, F_LAMBDA, l);
+    if (g->alpha_field) restrict_level_faces(g, l);
+    if (g->lambda_field) boundary_level(g, F_LAMBDA, l, 0, 0);
+    if (g->alpha_field) periodic_copy_faces(g, l);
+  }
+}
+
+/* ----------------------------------------------------------- prolongation */
+
+/* bilinear, multigrid-common.h:124-140; child offsets multigrid.h:175-188 */
+static inline double bilinear(orc_grid *g, int f, int l, int i, int j, int k) {
+  int pi = (i + GHOSTS) / 2, cx = 2 * ((i + GHOSTS) % 2) - 1;
+#if DIM == 1
+  (void)j; (void)k;
+  const double *c = REC(g, l - 1, pi, 0, 0);
+  return (3. * c[f] + REC(g, l - 1, pi + cx, 0, 0)[f]) / 4.;
+#elif DIM == 2
+  (void)k;
+  int pj = (j + GHOSTS) / 2, cy = 2 * ((j + GHOSTS) % 2) - 1;
+  double c0 = REC(g, l - 1, pi, pj, 0)[f];
+  double c1 = REC(g, l - 1, pi + cx, pj, 0)[f];
+  double c2 = REC(g, l - 1, pi, pj + cy, 0)[f];
+  double c3 = REC(g, l - 1, pi + cx, pj + cy, 0)[f];
+  return (9. * c0 + 3. * (c1 + c2) + c3) / 16.;
+#else
+  int pj = (j + GHOSTS) / 2, cy = 2 * ((j + GHOSTS) % 2) - 1;
+  int pk = (k + GHOSTS) / 2, cz = 2 * ((k + GHOSTS) % 2) - 1;
+  double c000 = REC(g, l - 1, pi, pj, pk)[f];
+  double c100 = REC(g, l - 1, pi + cx, pj, pk)[f];
+  double c010 = REC(g, l - 1, pi, pj + cy, pk)[f];
+  double c001 = REC(g, l - 1, pi, pj, pk + cz)[f];
+  double c110 = REC(g, l - 1, pi + cx, pj + cy, pk)[f];
+  double c101 = REC(g, l - 1, pi + cx, pj, pk + cz)[f];
+  double c011 = REC(g, l - 1, pi, pj + cy, pk + cz)[f];
+  double c111 = REC(g, l - 1, pi + cx, pj + cy, pk + cz)[f];
+  return (27. * c000 + 9. * (c100 + c010 + c001) + 3. * (c110 + c101 + c011) + c111) / 64.;
+#endif
+}
+
+/* ------------------------------------------------------- relax / residual */
+
+static inline double lambda_at(const orc_grid *g, const double *r) {
+  return g->lambda_field ? r[F_LAMBDA] : g->lambda_val;
+}
+
+/* alpha on the low face of the cell (alpha.x[]) and on the high face
+   (alpha.x[1], stored in the next cell's record) */
+static inline double alpha_lo(const orc_grid *g, const double *r, int c) {
+  return g->alpha_field ? r[F_AX + c] : g->alpha_val;
+}
+static inline double alpha_hi(const orc_grid *g, const double *r, ptrdiff_t sd, int c) {
+  return g->alpha_field ? r[sd * NV + F_AX + c] : g->alpha_val;
+}
+
+/* the Gauss-Seidel / Jacobi value n/d of poisson.h:303-320 */
+static inline double relax_value(const orc_grid *g, int l, const double *r, int src, double D2) {
+  double n = -D2 * r[F_RES], d = -lambda_at(g, r) * D2;
+  for (int c = 0; c < DIM; c++) {
+    ptrdiff_t sd = dstride(l, c);
+    double ah = alpha_hi(g, r, sd, c), al = alpha_lo(g, r, c);
+    n += ah * r[sd * NV + src] + al * r[-sd * NV + src];
+    d += ah + al;
+  }
+  return n / d;
+}
+
+void orc_relax(orc_grid *g, int l) {
+  int n = 1 << l;
+  double D = delta_l(g, l), D2 = D * D;
+  if (g->smoother == SM_GS) {
+    LOOP_LEVEL(l, n)
+      double *r = REC(g, l, i, j, k);
+      r[F_DA] = relax_value(g, l, r, F_DA, D2);
+    END_LOOP
+  } else if (g->smoother == SM_JACOBI) {
+    LOOP_LEVEL(l, n)
+      double *r = REC(g, l, i, j, k);
+      r[F_C] = relax_value(g, l, r, F_DA, D2);
+    END_LOOP
+    LOOP_LEVEL(l, n)
+      double *r = REC(g, l, i, j, k);
+      r[F_DA] = (r[F_DA] + 2. * r[F_C]) / 3.;
+    END_LOOP
+  } else {
+    for (int colour = 0; colour < 2; colour++) {
+      if (colour == 1) boundary_level(g, F_DA, l, 1, 1);
+      LOOP_LEVEL(l, n)
+        if (((i + j + k) & 1) == colour) {
+          double *r = REC(g, l, i, j, k);
+          r[F_DA] = relax_value(g, l, r, F_DA, D2);
+        }
+      END_LOOP
+    }
+  }
+}
+
+/* residual, poisson.h:373-389 (!TREE), finest level only; the ghost fill of
+   `a` with its inhomogeneous BC comes first (stencils.h end_stencil ->
+   boundary_internal -> boundary_level(list, depth)) */
+double orc_residual(orc_grid *g) {
+  int l = g->depth, n = 1 << l;
+  double D = delta_l(g, l);
+  boundary_level(g, F_A, l, 0, 1);
+  double maxres = 0.;
+#if DIM == 1
+#pragma omp parallel for schedule(static) reduction(max : maxres)
+  for (int i = GHOSTS; i < n + GHOSTS; i++) {
+    const int j = 0, k = 0;
+#elif DIM == 2
+#pragma omp parallel for schedule(static) reduction(max : maxres)
+  for (int i = GHOSTS; i < n + GHOSTS; i++)
+    for (int j = GHOSTS; j < n + GHOSTS; j++) {
+      const int k = 0;
+#else
+#pragma omp parallel for schedule(static) reduction(max : maxres)
+  for (int i = GHOSTS; i < n + GHOSTS; i++)
+    for (int j = GHOSTS; j < n + GHOSTS; j++)
+      for (int k = GHOSTS; k < n + GHOSTS; k++) {
+#endif
+        double *r = REC(g, l, i, j, k);
+        double res = r[F_B] - lambda_at(g, r) * r[F_A];
+        for (int c = 0; c < DIM; c++) {
+          ptrdiff_t sd = dstride(l, c);
+          double a0 = r[F_A], am = r[-sd * NV + F_A], ap = r[sd * NV + F_A];
+          res += (alpha_lo(g, r, c) * ((a0 - am) / D) - alpha_hi(g, r, sd, c) * ((ap - a0) / D)) / D;
+        }
+        r[F_RES] = res;
+        if (fabs(res) > maxres) maxres = fabs(res);
+      }
+  return maxres;
+}
+
+/* ---------------------------------------------------------- mg_cycle/solve */
+
+void orc_mg_cycle(orc_grid *g, int nrelax, int minlevel, int maxlevel) {
+  orc_restriction_res(g);
+  minlevel = minlevel < maxlevel ? minlevel : maxlevel;
+  for (int l = minlevel; l <= maxlevel; l++) {
+    int n = 1 << l;
+    if (l == minlevel) {
+      LOOP_LEVEL(l, n)
+        REC(g, l, i, j, k)[F_DA] = 0.;
+      END_LOOP
+    } else {
+      LOOP_LEVEL(l, n)
+        REC(g, l, i, j, k)[F_DA] = bilinear(g, F_DA, l, i, j, k);
+      END_LOOP
+    }
+    boundary_level(g, F_DA, l, 1, 1);
+    for (int it = 0; it < nrelax; it++) {
+      orc_relax(g, l);
+      boundary_level(g, F_DA, l, 1, 1);
+    }
+  }
+  int l = g->depth, n = 1 << l;
+  LOOP_LEVEL(l, n)
+    double *r = REC(g, l, i, j, k);
+    r[F_A] += r[F_DA];
+  END_LOOP
+}
+
+double orc_sum_b(orc_grid *g) {
+  int l = g->depth, n = 1 << l;
+  double sum = 0.;
+  /* serial i-major order (the reference's reduction order with 1 thread) */
+#if DIM == 1
+  for (int i = GHOSTS; i < n + GHOSTS; i++) { const int j = 0, k = 0;
+#elif DIM == 2
+  for (int i = GHOSTS; i < n + GHOSTS; i++)
+    for (int j = GHOSTS; j < n + GHOSTS; j++) { const int k = 0;
+#else
+  for (int i = GHOSTS; i < n + GHOSTS; i++)
+    for (int j = GHOSTS; j < n + GHOSTS; j++)
+      for (int k = GHOSTS; k < n + GHOSTS; k++) {
+#endif
+        sum += REC(g, l, i, j, k)[F_B];
+      }
+  return sum;
+}
+
+int orc_mg_solve(orc_grid *g, double tolerance, int nrelax, int minlevel, orc_mgstats *out) {
+  orc_mgstats s;
+  memset(&s, 0, sizeof s);
+  s.sum = orc_sum_b(g);
+  s.nrelax = nrelax > 0 ? nrelax : 4;
+  double resb;
+  resb = s.resb = s.resa = orc_residual(g);
+  if (tolerance == 0.) tolerance = g->TOLERANCE;
+  for (s.i = 0; s.i < g->NITERMAX && (s.i < g->NITERMIN || s.resa > tolerance); s.i++) {
+    orc_mg_cycle(g, s.nrelax, minlevel, g->depth);
+    s.resa = orc_residual(g);
+    if (s.resa > tolerance) {
+      if (resb / s.resa < 1.2 && s.nrelax < 100) s.nrelax++;
+      else if (resb / s.resa > 10 && s.nrelax > 2) s.nrelax--;
+    }
+    resb = s.resa;
+  }
+  s.minlevel = minlevel;
+  if (s.resa > tolerance)
+    fprintf(stderr,
+            "WARNING: convergence for a not reached after %d iterations\n"
+            "  res: %g sum: %g nrelax: %d\n",
+            s.i, s.resa, s.sum, s.nrelax);
+  if (out) *out = s;
+  return 0;
+}
+
+/* poisson(), poisson.h:402-441 (alpha/lambda defaults are the grid's
+   constant settings; TOLERANCE is overridden for the call only) */
+int orc_poisson(orc_grid *g, double tolerance, int nrelax, int minlevel, orc_mgstats *out) {
+  orc_restriction_alpha_lambda(g);
+  double defaultol = g->TOLERANCE;
+  if (tolerance) g->TOLERANCE = tolerance;
+  int r = orc_mg_solve(g, 0., nrelax, minlevel > 1 ? minlevel : 1, out);
+  if (tolerance) g->TOLERANCE = defaultol;
+  return r;
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,264 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+``bench.py`` may import this module.  It drives ``mg_oracle.c``, the plain-C
+restatement of the reference's multigrid Poisson-Helmholtz solver
+(poisson.h:32-441, grid/multigrid.h, grid/multigrid-common.h); see the header
+of that file for the per-function citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+F_A, F_B, F_RES, F_DA, F_LAMBDA, F_C, F_AX = range(7)
+BC_SYMMETRY, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
+SMOOTHERS = {"gs": 0, "jacobi": 1, "rbgs": 2}
+GHOSTS = 2
+
+
+class MGStats(ctypes.Structure):
+    """mgstats, poisson.h:108-114."""
+    _fields_ = [("i", ctypes.c_int), ("resb", ctypes.c_double),
+                ("resa", ctypes.c_double), ("sum", ctypes.c_double),
+                ("nrelax", ctypes.c_int), ("minlevel", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build():
+    """Compile the oracle libraries (make in this directory)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_libs = {}
+
+
+def _lib(dim, omp=False):
+    key = (dim, omp)
+    if key not in _libs:
+        name = "liborc_mg%d%s.so" % (dim, "_omp" if omp else "")
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        dp = ctypes.POINTER(ctypes.c_double)
+        sigs = {
+            "orc_new": (P, [I]), "orc_free": (None, [P]),
+            "orc_set_geometry": (None, [P, D, D, D, D]),
+            "orc_set_niter": (None, [P, I, I]),
+            "orc_set_tolerance": (None, [P, D]),
+            "orc_set_smoother": (None, [P, I]),
+            "orc_set_lambda_const": (None, [P, D]),
+            "orc_use_lambda_field": (None, [P]),
+            "orc_set_alpha_const": (None, [P, D]),
+            "orc_use_alpha_field": (None, [P]),
+            "orc_set_bc": (None, [P, I, I, dp]),
+            "orc_set_cell": (None, [P, I, I, dp]),
+            "orc_get_cell": (None, [P, I, I, dp]),
+            "orc_set_face": (None, [P, I, I, dp]),
+            "orc_get_face": (None, [P, I, I, dp]),
+            "orc_level_data": (dp, [P, I]),
+            "orc_boundary": (None, [P, I, I, I]),
+            "orc_restriction_res": (None, [P]),
+            "orc_restriction_alpha_lambda": (None, [P]),
+            "orc_relax": (None, [P, I]),
+            "orc_residual": (D, [P]),
+            "orc_mg_cycle": (None, [P, I, I, I]),
+            "orc_sum_b": (D, [P]),
+            "orc_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+            "orc_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+            "orc_set_threads": (None, [I]),
+            "orc_max_threads": (I, []),
+            "orc_dimension": (I, []),
+            "orc_nvar": (I, []),
+        }
+        for name_, (res, args) in sigs.items():
+            fn = getattr(lib, name_)
+            fn.restype = res
+            fn.argtypes = args
+        _libs[key] = lib
+    return _libs[key]
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OracleGrid:
+    """A uniform multigrid hierarchy of `depth` levels in `dim` dimensions,
+    holding the fields a, b, res, da, lambda and the face vector alpha."""
+
+    def __init__(self, dim, depth, omp=False, L0=1.0, origin=(0.0, 0.0, 0.0)):
+        self.dim, self.depth = dim, depth
+        self.lib = _lib(dim, omp)
+        self.g = self.lib.orc_new(depth)
+        self.lib.orc_set_geometry(self.g, L0, *origin)
+        self.L0, self.origin = L0, tuple(origin)
+        self.nv = self.lib.orc_nvar()
+
+    def __del__(self):
+        try:
+            self.lib.orc_free(self.g)
+        except Exception:
+            pass
+
+    # -- geometry helpers (cartesian-common.h:9-34)
+    def n(self, level=None):
+        return 1 << (self.depth if level is None else level)
+
+    def delta(self, level=None):
+        return self.L0 * (1.0 / self.n(level))
+
+    def centers(self, level=None):
+        """Cell-centre coordinates per dimension (1D arrays)."""
+        n, D = self.n(level), self.delta(level)
+        return [(np.arange(n) + 0.5) * D + self.origin[d] for d in range(self.dim)]
+
+    def shape(self, level=None):
+        return (self.n(level),) * self.dim
+
+    def face_shape(self, c, level=None):
+        s = list(self.shape(level))
+        s[c] += 1
+        return tuple(s)
+
+    # -- field access
+    def set_cell(self, f, arr, level=None):
+        level = self.depth if level is None else level
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape(level))
+        self.lib.orc_set_cell(self.g, f, level, _dptr(a))
+
+    def get_cell(self, f, level=None):
+        level = self.depth if level is None else level
+        a = np.empty(self.shape(level))
+        self.lib.orc_get_cell(self.g, f, level, _dptr(a))
+        return a
+
+    def set_face(self, c, arr, level=None):
+        level = self.depth if level is None else level
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.face_shape(c, level))
+        self.lib.orc_set_face(self.g, c, level, _dptr(a))
+        self.lib.orc_use_alpha_field(self.g)
+
+    def get_face(self, c, level=None):
+        level = self.depth if level is None else level
+        a = np.empty(self.face_shape(c, level))
+        self.lib.orc_get_face(self.g, c, level, _dptr(a))
+        return a
+
+    def set_a(self, arr):
+        self.set_cell(F_A, arr)
+
+    def set_b(self, arr):
+        self.set_cell(F_B, arr)
+
+    def get_a(self):
+        return self.get_cell(F_A)
+
+    def get_res(self):
+        return self.get_cell(F_RES)
+
+    def set_alpha(self, comps):
+        for c, arr in enumerate(comps):
+            self.set_face(c, arr)
+
+    def set_lambda(self, lam):
+        if np.isscalar(lam):
+            self.lib.orc_set_lambda_const(self.g, float(lam))
+        else:
+            self.set_cell(F_LAMBDA, lam)
+            self.lib.orc_use_lambda_field(self.g)
+
+    def raw_level(self, level):
+        """The AoS storage of one level, shape (m,)*dim + (nvar,), m = n+4."""
+        m = self.n(level) + 2 * GHOSTS
+        ptr = self.lib.orc_level_data(self.g, level)
+        count = m ** self.dim * self.nv
+        buf = np.ctypeslib.as_array(ptr, shape=(count,))
+        return buf.reshape((m,) * self.dim + (self.nv,))
+
+    # -- boundary conditions of `a`
+    def set_bc(self, direction, kind, values=None):
+        """direction: 0 right, 1 left, 2 top, 3 bottom, 4 front, 5 back.
+        values: the BC expression over the full tangential extent (n+4 per
+        tangential dimension) at the finest level, or None for 0."""
+        if values is None:
+            self.lib.orc_set_bc(self.g, direction, kind, None)
+        else:
+            v = np.ascontiguousarray(values, dtype=np.float64)
+            self.lib.orc_set_bc(self.g, direction, kind, _dptr(v))
+
+    def set_bc_function(self, kind, fn, directions=None):
+        """Evaluate fn(x, y, z) on each boundary face at the finest level over
+        the full tangential extent, with the face coordinate shifted by half a
+        cell like the qcc-generated BC functions (qcc.lex:1118-1130,
+        cartesian-common.h:9-34)."""
+        dirs = range(2 * self.dim) if directions is None else directions
+        n, D, m = self.n(), self.delta(), self.n() + 2 * GHOSTS
+        idx = np.arange(m)
+        for b in dirs:
+            d, side = b // 2, (1 if b % 2 == 0 else -1)
+            plane = n + GHOSTS - 1 if side > 0 else GHOSTS
+            coords = []
+            for e in range(3):
+                if e >= self.dim:
+                    coords.append(np.zeros(1))
+                elif e == d:
+                    coords.append(np.array([(side / 2.0 + plane - GHOSTS + 0.5) * D + self.origin[e]]))
+                else:
+                    coords.append((idx - GHOSTS + 0.5) * D + self.origin[e])
+            X, Y, Z = np.meshgrid(*coords, indexing="ij")
+            vals = np.asarray(fn(X, Y, Z), dtype=np.float64) * np.ones_like(X)
+            self.set_bc(b, kind, vals.reshape(-1))
+
+    def set_periodic(self, d):
+        self.lib.orc_set_bc(self.g, 2 * d, BC_PERIODIC, None)
+
+    # -- solver
+    def set_smoother(self, name):
+        self.lib.orc_set_smoother(self.g, SMOOTHERS[name])
+
+    def set_niter(self, nitermax=100, nitermin=1):
+        self.lib.orc_set_niter(self.g, nitermax, nitermin)
+
+    def set_tolerance(self, tol):
+        self.lib.orc_set_tolerance(self.g, tol)
+
+    def residual(self):
+        return self.lib.orc_residual(self.g)
+
+    def mg_cycle(self, nrelax, minlevel, maxlevel=None):
+        self.lib.orc_mg_cycle(self.g, nrelax, minlevel,
+                              self.depth if maxlevel is None else maxlevel)
+
+    def relax(self, level):
+        self.lib.orc_relax(self.g, level)
+
+    def boundary(self, f, level, homogeneous):
+        self.lib.orc_boundary(self.g, f, level, int(homogeneous))
+
+    def restriction_res(self):
+        self.lib.orc_restriction_res(self.g)
+
+    def restriction_alpha_lambda(self):
+        self.lib.orc_restriction_alpha_lambda(self.g)
+
+    def mg_solve(self, tolerance=0.0, nrelax=0, minlevel=0):
+        s = MGStats()
+        self.lib.orc_mg_solve(self.g, tolerance, nrelax, minlevel, ctypes.byref(s))
+        return s.as_dict()
+
+    def poisson(self, tolerance=0.0, nrelax=0, minlevel=0):
+        s = MGStats()
+        self.lib.orc_poisson(self.g, tolerance, nrelax, minlevel, ctypes.byref(s))
+        return s.as_dict()
+
+
+def set_threads(n, dim=3):
+    _lib(dim, True).orc_set_threads(n)
