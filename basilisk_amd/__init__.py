@@ -1,0 +1,336 @@
+"""basilisk_amd -- MI355X multigrid Poisson-Helmholtz solver for Basilisk.
+
+The solver runs in ``libbmg.so`` (hand-written HIP for gfx950, built in-tree
+by ``python -m basilisk_amd.build``); this module is its ctypes front end.
+``Grid`` mirrors the reference's programming model: fields a, b, res, da,
+lambda and the face vector alpha on a uniform multigrid hierarchy, and the
+entry points ``poisson()`` / ``mg_solve()`` / ``mg_cycle()`` / ``residual()``
+of poisson.h (poisson.h:32-441), with boundary conditions given like
+``a[left] = dirichlet(...)`` (common.h:957-960).
+
+There is no CPU fallback: importing the package without the built library,
+or creating a Grid without a GPU, raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Grid", "MGStats", "BMGError", "library", "LIB_PATH",
+           "BC_SYMMETRY", "BC_DIRICHLET", "BC_NEUMANN", "BC_PERIODIC",
+           "RIGHT", "LEFT", "TOP", "BOTTOM", "FRONT", "BACK"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbmg.so")
+
+BC_SYMMETRY, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_AFFINE = range(5)
+RIGHT, LEFT, TOP, BOTTOM, FRONT, BACK = range(6)
+SMOOTHERS = {"gs": 0, "jacobi": 1, "rbgs": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+F_A, F_B, F_RES, F_DA, F_LAMBDA, F_ALPHA = range(6)
+PROFILE_KINDS = {"relax_fine": 0, "residual": 1, "restriction": 2,
+                 "prolongation": 3, "correction": 4, "relax_coarse": 5}
+
+# every symbol declared in include/bmg.h
+EXPORTS = [
+    "bmg_version", "bmg_last_error", "bmg_device_count", "bmg_grid_new",
+    "bmg_grid_free", "bmg_grid_info", "bmg_set_precision", "bmg_set_smoother",
+    "bmg_set_tolerance", "bmg_set_niter", "bmg_set_stream", "bmg_set_bc",
+    "bmg_set_bc_affine", "bmg_set_field", "bmg_get_field",
+    "bmg_set_alpha_const", "bmg_set_lambda_const",
+    "bmg_restriction_alpha_lambda", "bmg_residual", "bmg_relax",
+    "bmg_mg_cycle", "bmg_mg_solve", "bmg_poisson", "bmg_synchronize",
+    "bmg_profile_enable", "bmg_profile_get", "bmg_poisson_view",
+]
+
+
+class BMGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libbmg error %d: %s" % (code, msg))
+        self.code = code
+
+
+class MGStats(ctypes.Structure):
+    """mgstats, poisson.h:108-114."""
+    _fields_ = [("i", ctypes.c_int), ("resb", ctypes.c_double),
+                ("resa", ctypes.c_double), ("sum", ctypes.c_double),
+                ("nrelax", ctypes.c_int), ("minlevel", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def library():
+    """Load libbmg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libbmg.so is not built: run `python -m basilisk_amd.build` "
+                          "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    dp = ctypes.c_void_p
+    sigs = {
+        "bmg_version": (ctypes.c_char_p, []),
+        "bmg_last_error": (ctypes.c_char_p, []),
+        "bmg_device_count": (I, [ctypes.POINTER(I)]),
+        "bmg_grid_new": (I, [ctypes.POINTER(P), I, I, D, D, D, D]),
+        "bmg_grid_free": (I, [P]),
+        "bmg_grid_info": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "bmg_set_precision": (I, [P, I]),
+        "bmg_set_smoother": (I, [P, I]),
+        "bmg_set_tolerance": (I, [P, D]),
+        "bmg_set_niter": (I, [P, I, I]),
+        "bmg_set_stream": (I, [P, P]),
+        "bmg_set_bc": (I, [P, I, I, dp]),
+        "bmg_set_bc_affine": (I, [P, I, I, dp]),
+        "bmg_set_field": (I, [P, I, I, dp, I]),
+        "bmg_get_field": (I, [P, I, I, dp, I]),
+        "bmg_set_alpha_const": (I, [P, D]),
+        "bmg_set_lambda_const": (I, [P, D]),
+        "bmg_restriction_alpha_lambda": (I, [P]),
+        "bmg_residual": (I, [P, ctypes.POINTER(D)]),
+        "bmg_relax": (I, [P, I]),
+        "bmg_mg_cycle": (I, [P, I, I, I]),
+        "bmg_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_synchronize": (I, [P]),
+        "bmg_profile_enable": (I, [P, I]),
+        "bmg_profile_get": (I, [P, I, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_long)]),
+        "bmg_poisson_view": (I, [P, P, P, D, I, I, ctypes.POINTER(MGStats)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(code, allow_nan=False):
+    if code == 0 or (allow_nan and code == 5):
+        return code
+    raise BMGError(code, library().bmg_last_error().decode())
+
+
+def _host_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Grid:
+    """A uniform multigrid hierarchy (levels 0..depth) resident on the GPU.
+
+    ``dim``: 1, 2 or 3; ``depth``: finest level (2^depth cells per
+    dimension); ``L0``/``origin``: the box, like size() and origin()
+    (common.h:857-866).  ``precision``: "fp64" or "mixed";
+    ``smoother``: "rbgs" (default) or "jacobi"."""
+
+    def __init__(self, dim, depth, L0=1.0, origin=(0.0, 0.0, 0.0),
+                 precision="fp64", smoother="rbgs"):
+        lib = library()
+        self.dim, self.depth, self.L0 = dim, depth, float(L0)
+        self.origin = tuple(float(o) for o in origin) + (0.0,) * (3 - len(origin))
+        self._g = ctypes.c_void_p()
+        _check(lib.bmg_grid_new(ctypes.byref(self._g), dim, depth, self.L0, *self.origin[:3]))
+        self.lib = lib
+        self.set_smoother(smoother)
+        self.set_precision(precision)
+
+    def close(self):
+        if getattr(self, "_g", None):
+            self.lib.bmg_grid_free(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- geometry (cartesian-common.h:9-34)
+    def n(self, level=None):
+        return 1 << (self.depth if level is None else level)
+
+    def delta(self, level=None):
+        return self.L0 * (1.0 / self.n(level))
+
+    def centers(self, level=None):
+        n, D = self.n(level), self.delta(level)
+        return [(np.arange(n) + 0.5) * D + self.origin[d] for d in range(self.dim)]
+
+    def shape(self, level=None):
+        return (self.n(level),) * self.dim
+
+    def face_shape(self, c, level=None):
+        s = list(self.shape(level))
+        s[c] += 1
+        return tuple(s)
+
+    # -- configuration
+    def set_precision(self, name):
+        _check(self.lib.bmg_set_precision(self._g, PRECISIONS[name]))
+        self.precision = name
+
+    def set_smoother(self, name):
+        _check(self.lib.bmg_set_smoother(self._g, SMOOTHERS[name]))
+        self.smoother = name
+
+    def set_tolerance(self, tol):
+        _check(self.lib.bmg_set_tolerance(self._g, float(tol)))
+
+    def set_niter(self, nitermax=100, nitermin=1):
+        _check(self.lib.bmg_set_niter(self._g, nitermax, nitermin))
+
+    def set_stream(self, stream_ptr):
+        _check(self.lib.bmg_set_stream(self._g, ctypes.c_void_p(stream_ptr)))
+
+    # -- fields (numpy arrays on the host, or torch tensors on the GPU)
+    def _put(self, field, arr, level, shape):
+        level = self.depth if level is None else level
+        if hasattr(arr, "data_ptr") and getattr(arr, "is_cuda", False):
+            if tuple(arr.shape) != shape or not arr.is_contiguous() or str(arr.dtype) != "torch.float64":
+                raise ValueError("device field must be a contiguous float64 tensor of shape %s" % (shape,))
+            import torch
+            torch.cuda.current_stream().synchronize()
+            _check(self.lib.bmg_set_field(self._g, field, level, ctypes.c_void_p(arr.data_ptr()), 1))
+            return
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != int(np.prod(shape)):
+            raise ValueError("field shape %s does not match %s" % (a.shape, shape))
+        _check(self.lib.bmg_set_field(self._g, field, level, _host_ptr(a), 0))
+
+    def _get(self, field, level, shape, out=None):
+        level = self.depth if level is None else level
+        if out is not None and hasattr(out, "data_ptr"):
+            _check(self.lib.bmg_get_field(self._g, field, level, ctypes.c_void_p(out.data_ptr()), 1))
+            return out
+        a = np.empty(shape)
+        _check(self.lib.bmg_get_field(self._g, field, level, _host_ptr(a), 0))
+        return a
+
+    def set_a(self, arr):
+        self._put(F_A, arr, None, self.shape())
+
+    def set_b(self, arr):
+        self._put(F_B, arr, None, self.shape())
+
+    def get_a(self, out=None):
+        return self._get(F_A, None, self.shape(), out)
+
+    def get_b(self, out=None):
+        return self._get(F_B, None, self.shape(), out)
+
+    def get_res(self, level=None):
+        return self._get(F_RES, level, self.shape(level))
+
+    def set_res(self, arr, level=None):
+        self._put(F_RES, arr, level, self.shape(level))
+
+    def get_da(self, level=None):
+        return self._get(F_DA, level, self.shape(level))
+
+    def set_da(self, arr, level=None):
+        self._put(F_DA, arr, level, self.shape(level))
+
+    def set_alpha(self, comps, level=None):
+        if np.isscalar(comps):
+            _check(self.lib.bmg_set_alpha_const(self._g, float(comps)))
+            return
+        for c, arr in enumerate(comps):
+            self._put(F_ALPHA + c, arr, level, self.face_shape(c, level))
+
+    def get_alpha(self, c, level=None):
+        return self._get(F_ALPHA + c, level, self.face_shape(c, level))
+
+    def set_lambda(self, lam, level=None):
+        if np.isscalar(lam):
+            _check(self.lib.bmg_set_lambda_const(self._g, float(lam)))
+        else:
+            self._put(F_LAMBDA, lam, level, self.shape(level))
+
+    def get_lambda(self, level=None):
+        return self._get(F_LAMBDA, level, self.shape(level))
+
+    # -- boundary conditions of `a`
+    def set_bc(self, direction, kind, values=None):
+        """values: the BC expression on the n^(dim-1) faces of that side of
+        the finest level (C order over the tangential indices), or None."""
+        if values is None:
+            _check(self.lib.bmg_set_bc(self._g, direction, kind, None))
+            return
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if v.size == (self.n() + 4) ** (self.dim - 1) and self.dim > 1:
+            # a full-extent (ghost-inclusive) array: keep the faces of cells
+            m = self.n() + 4
+            v = v.reshape((m,) * (self.dim - 1))[(slice(2, -2),) * (self.dim - 1)].reshape(-1)
+        if v.size != self.n() ** (self.dim - 1):
+            raise ValueError("expected %d boundary values" % self.n() ** (self.dim - 1))
+        _check(self.lib.bmg_set_bc(self._g, direction, kind, _host_ptr(v)))
+
+    def set_bc_affine(self, direction, sigma, c):
+        v = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
+        _check(self.lib.bmg_set_bc_affine(self._g, direction, int(sigma), _host_ptr(v)))
+
+    def set_bc_function(self, kind, fn, directions=None):
+        """Evaluate fn(x, y, z) at the centre of each boundary face (the
+        coordinates a qcc-generated BC function sees, qcc.lex:1118-1130)."""
+        dirs = range(2 * self.dim) if directions is None else directions
+        n, D = self.n(), self.delta()
+        for b in dirs:
+            d, side = b // 2, (1 if b % 2 == 0 else -1)
+            plane = n - 1 if side > 0 else 0
+            coords = []
+            for e in range(3):
+                if e >= self.dim:
+                    coords.append(np.zeros(1))
+                elif e == d:
+                    coords.append(np.array([(side / 2.0 + plane + 0.5) * D + self.origin[e]]))
+                else:
+                    coords.append((np.arange(n) + 0.5) * D + self.origin[e])
+            X, Y, Z = np.meshgrid(*coords, indexing="ij")
+            vals = np.asarray(fn(X, Y, Z), dtype=np.float64) * np.ones_like(X)
+            self.set_bc(b, kind, vals.reshape(-1))
+
+    def set_periodic(self, d):
+        _check(self.lib.bmg_set_bc(self._g, 2 * d, BC_PERIODIC, None))
+
+    # -- solver (poisson.h)
+    def restriction_alpha_lambda(self):
+        _check(self.lib.bmg_restriction_alpha_lambda(self._g))
+
+    def residual(self):
+        m = ctypes.c_double()
+        _check(self.lib.bmg_residual(self._g, ctypes.byref(m)))
+        return m.value
+
+    def relax(self, level):
+        _check(self.lib.bmg_relax(self._g, level))
+
+    def mg_cycle(self, nrelax, minlevel, maxlevel=None):
+        _check(self.lib.bmg_mg_cycle(self._g, nrelax, minlevel,
+                                     self.depth if maxlevel is None else maxlevel))
+
+    def mg_solve(self, tolerance=0.0, nrelax=0, minlevel=0):
+        s = MGStats()
+        _check(self.lib.bmg_mg_solve(self._g, tolerance, nrelax, minlevel, ctypes.byref(s)))
+        return s.as_dict()
+
+    def poisson(self, tolerance=0.0, nrelax=0, minlevel=0):
+        s = MGStats()
+        _check(self.lib.bmg_poisson(self._g, tolerance, nrelax, minlevel, ctypes.byref(s)))
+        return s.as_dict()
+
+    def synchronize(self):
+        _check(self.lib.bmg_synchronize(self._g))
+
+    # -- kernel timing
+    def profile(self, enable=True):
+        _check(self.lib.bmg_profile_enable(self._g, int(enable)))
+
+    def profile_get(self, kind):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _check(self.lib.bmg_profile_get(self._g, PROFILE_KINDS[kind], ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

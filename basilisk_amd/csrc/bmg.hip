@@ -1,0 +1,1052 @@
+// libbmg host side: the device-resident multigrid hierarchy and the
+// reference's solver loop (poisson.h:32-441) driving the kernels of
+// bmg_kernels.h, behind the C ABI of include/bmg.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bmg.h"
+#include "bmg_kernels.h"
+
+using namespace bmg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHECK(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(BMG_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+#define TRY(call)              \
+  do {                         \
+    int r_ = (call);           \
+    if (r_ != BMG_OK) return r_; \
+  } while (0)
+
+size_t powd(size_t n, int dim) {
+  size_t r = 1;
+  for (int d = 0; d < dim; d++) r *= n;
+  return r;
+}
+
+unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// one level of the hierarchy: arrays in the cycle precision (double or
+// float) plus FP64 copies of the coefficients
+struct Level {
+  int n = 0, log2n = 0;
+  size_t ncell = 0, nwall = 0;
+  void* da[2] = {nullptr, nullptr};  // [1]: Jacobi output buffer
+  void* res = nullptr;
+  void* alo[3] = {nullptr, nullptr, nullptr};
+  void* awall[3] = {nullptr, nullptr, nullptr};
+  void* lam = nullptr;
+  double* alo64[3] = {nullptr, nullptr, nullptr};
+  double* awall64[3] = {nullptr, nullptr, nullptr};
+  double* lam64 = nullptr;
+};
+
+enum { PROF_RELAX_FINE = 0, PROF_RESIDUAL, PROF_RESTRICT, PROF_PROLONG, PROF_CORRECT, PROF_RELAX_COARSE, PROF_N };
+
+}  // namespace
+
+struct bmg_grid {
+  int dim = 0, depth = 0;
+  double L0 = 1., X0[3] = {0., 0., 0.};
+  int precision = BMG_FP64, smoother = BMG_RELAX_RBGS;
+  double TOLERANCE = 1e-3;
+  int NITERMAX = 100, NITERMIN = 1;
+  std::vector<Level> L;
+  double* a = nullptr;
+  double* b = nullptr;
+  double* res64 = nullptr;  // FP64 residual on the finest level
+  int bc_kind[6] = {0, 0, 0, 0, 0, 0};
+  int bc_sigma[6] = {1, 1, 1, 1, 1, 1};
+  double* bc_c[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int periodic[3] = {0, 0, 0};
+  bool alpha_field = false, lambda_field = false;
+  double alpha_c = 1., lambda_c = 0.;
+  bool coarse_coef_ready = false;  // FP32 copies / restricted levels valid
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  Stat* stat = nullptr;
+  double* partial = nullptr;
+  double* scalar = nullptr;
+  int npartial = 1024;
+  bool allocated_precision[2] = {false, false};
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+  double prof_ms[PROF_N] = {0};
+  long prof_n[PROF_N] = {0};
+};
+
+namespace {
+
+size_t esize(const bmg_grid* g) { return g->precision == BMG_MIXED ? sizeof(float) : sizeof(double); }
+
+int dalloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(BMG_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return BMG_OK;
+}
+
+void dfree(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <typename T>
+void dfreet(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// --- profiling events around a kernel class
+struct ProfScope {
+  bmg_grid* g;
+  int kind;
+  hipEvent_t e1 = nullptr;
+  ProfScope(bmg_grid* g_, int kind_) : g(g_), kind(kind_) {
+    if (!g->prof) return;
+    hipEvent_t e0;
+    if (g->ev_pool.size() >= 2) {
+      e0 = g->ev_pool.back(); g->ev_pool.pop_back();
+      e1 = g->ev_pool.back(); g->ev_pool.pop_back();
+    } else {
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+    }
+    (void)hipEventRecord(e0, g->stream);
+    g->ev_used.push_back({kind, {e0, e1}});
+  }
+  ~ProfScope() {
+    if (g->prof) (void)hipEventRecord(e1, g->stream);
+  }
+};
+
+void prof_collect(bmg_grid* g) {
+  if (g->ev_used.empty()) return;
+  (void)hipStreamSynchronize(g->stream);
+  for (auto& u : g->ev_used) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, u.second.first, u.second.second);
+    g->prof_ms[u.first] += ms;
+    g->prof_n[u.first] += 1;
+    g->ev_pool.push_back(u.second.first);
+    g->ev_pool.push_back(u.second.second);
+  }
+  g->ev_used.clear();
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(BMG_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+  return BMG_OK;
+}
+
+// allocate the per-level arrays of one precision
+int alloc_levels(bmg_grid* g) {
+  int p = g->precision;
+  if (g->allocated_precision[p]) return BMG_OK;
+  // arrays of the other precision are released first
+  for (auto& lv : g->L) {
+    for (int s = 0; s < 2; s++) dfree(lv.da[s]);
+    if (lv.res != (void*)g->res64) dfree(lv.res);
+    lv.res = nullptr;
+    for (int d = 0; d < 3; d++) {
+      if (lv.alo[d] != (void*)lv.alo64[d]) dfree(lv.alo[d]);
+      if (lv.awall[d] != (void*)lv.awall64[d]) dfree(lv.awall[d]);
+      lv.alo[d] = lv.awall[d] = nullptr;
+    }
+    if (lv.lam != (void*)lv.lam64) dfree(lv.lam);
+    lv.lam = nullptr;
+  }
+  g->allocated_precision[0] = g->allocated_precision[1] = false;
+  size_t es = esize(g);
+  for (int l = 0; l <= g->depth; l++) {
+    Level& lv = g->L[l];
+    TRY(dalloc(&lv.da[0], lv.ncell * es));
+    if (g->smoother == BMG_RELAX_JACOBI) TRY(dalloc(&lv.da[1], lv.ncell * es));
+    // in FP64 the finest residual is res64 itself
+    if (p == BMG_FP64 && l == g->depth) lv.res = g->res64;
+    else TRY(dalloc(&lv.res, lv.ncell * es));
+    for (int d = 0; d < g->dim; d++) {
+      if (p == BMG_FP64) {
+        lv.alo[d] = lv.alo64[d];
+        lv.awall[d] = lv.awall64[d];
+      } else {
+        TRY(dalloc(&lv.alo[d], lv.ncell * es));
+        TRY(dalloc(&lv.awall[d], lv.nwall * es));
+      }
+    }
+    if (p == BMG_FP64) lv.lam = lv.lam64;
+    else TRY(dalloc(&lv.lam, lv.ncell * es));
+  }
+  g->allocated_precision[p] = true;
+  g->coarse_coef_ready = false;
+  return BMG_OK;
+}
+
+int ensure_jacobi_buffers(bmg_grid* g) {
+  if (g->smoother != BMG_RELAX_JACOBI) return BMG_OK;
+  size_t es = esize(g);
+  for (auto& lv : g->L)
+    if (!lv.da[1]) TRY(dalloc(&lv.da[1], lv.ncell * es));
+  return BMG_OK;
+}
+
+template <int DIM>
+BCH level_bc(const bmg_grid* g, int l) {
+  BCH bc;
+  bool anyp = false;
+  for (int d = 0; d < 3; d++) {
+    bc.periodic[d] = d < g->dim ? g->periodic[d] : 0;
+    anyp |= bc.periodic[d] != 0;
+  }
+  for (int b = 0; b < 6; b++) bc.sigma[b] = g->bc_sigma[b];
+  // on level 0 the periodic copy sets every ghost to the root value
+  // (grid/multigrid.h:655-664)
+  if (l == 0 && anyp)
+    for (int b = 0; b < 6; b++) bc.sigma[b] = 1;
+  return bc;
+}
+
+template <typename T>
+Coef<T> level_coef(const bmg_grid* g, int l) {
+  const Level& lv = g->L[l];
+  Coef<T> k;
+  for (int d = 0; d < 3; d++) {
+    k.lo[d] = g->alpha_field && d < g->dim ? (const T*)lv.alo[d] : nullptr;
+    k.wall[d] = g->alpha_field && d < g->dim ? (const T*)lv.awall[d] : nullptr;
+  }
+  if (!g->alpha_field) k.lo[0] = nullptr;
+  k.alpha = (T)g->alpha_c;
+  k.lam = g->lambda_field ? (const T*)lv.lam : nullptr;
+  k.lam_c = (T)g->lambda_c;
+  return k;
+}
+
+Coef<double> fine_coef64(const bmg_grid* g) {
+  const Level& lv = g->L[g->depth];
+  Coef<double> k;
+  for (int d = 0; d < 3; d++) {
+    k.lo[d] = g->alpha_field && d < g->dim ? lv.alo64[d] : nullptr;
+    k.wall[d] = g->alpha_field && d < g->dim ? lv.awall64[d] : nullptr;
+  }
+  if (!g->alpha_field) k.lo[0] = nullptr;
+  k.alpha = g->alpha_c;
+  k.lam = g->lambda_field ? lv.lam64 : nullptr;
+  k.lam_c = g->lambda_c;
+  return k;
+}
+
+double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l)); }
+
+// --- kernels per dimension / precision --------------------------------
+
+template <int DIM, typename T>
+int relax_level(bmg_grid* g, int l) {
+  Level& lv = g->L[l];
+  Geo<DIM> geo{lv.n, lv.log2n};
+  BCH bc = level_bc<DIM>(g, l);
+  Coef<T> k = level_coef<T>(g, l);
+  double D = delta(g, l);
+  T D2 = (T)(D * D);
+  ProfScope ps(g, l == g->depth ? PROF_RELAX_FINE : PROF_RELAX_COARSE);
+  if (g->smoother == BMG_RELAX_JACOBI) {
+    hipLaunchKernelGGL((k_relax_jacobi<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                       (const T*)lv.da[0], (T*)lv.da[1], (const T*)lv.res, k, geo, bc, D2);
+    std::swap(lv.da[0], lv.da[1]);
+  } else if (g->smoother == BMG_RELAX_RBGS) {
+    size_t half = lv.n > 1 ? lv.ncell / 2 : 1;
+    for (int colour = 0; colour < 2; colour++)
+      hipLaunchKernelGGL((k_relax_rb<DIM, T>), dim3(nblocks(half)), dim3(kBlock), 0, g->stream,
+                         (T*)lv.da[0], (const T*)lv.res, k, geo, bc, D2, colour);
+  } else {
+    return fail(BMG_ERR_UNSUPPORTED, "lexicographic Gauss-Seidel is not available on the GPU path");
+  }
+  return check_launch();
+}
+
+template <int DIM, typename T>
+int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
+  // restriction (res), poisson.h:42; levels below minlevel are never read
+  // by the cycle and are not computed
+  minlevel = std::min(minlevel, maxlevel);
+  if (minlevel < 0) minlevel = 0;
+  {
+    ProfScope ps(g, PROF_RESTRICT);
+    for (int l = g->depth - 1; l >= minlevel; l--) {
+      Level& lc = g->L[l];
+      Geo<DIM> gc{lc.n, lc.log2n};
+      hipLaunchKernelGGL((k_restrict<DIM, T>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
+                         (const T*)g->L[l + 1].res, (T*)lc.res, gc);
+    }
+  }
+  TRY(check_launch());
+  for (int l = minlevel; l <= maxlevel; l++) {
+    Level& lv = g->L[l];
+    if (l == minlevel) {
+      hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, (T*)lv.da[0],
+                         lv.ncell, T(0));
+    } else {
+      ProfScope ps(g, PROF_PROLONG);
+      Geo<DIM> gf{lv.n, lv.log2n};
+      BCH bcc = level_bc<DIM>(g, l - 1);
+      hipLaunchKernelGGL((k_prolong<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                         (const T*)g->L[l - 1].da[0], (T*)lv.da[0], gf, bcc);
+    }
+    TRY(check_launch());
+    for (int it = 0; it < nrelax; it++) TRY((relax_level<DIM, T>(g, l)));
+  }
+  Level& lf = g->L[g->depth];
+  {
+    ProfScope ps(g, PROF_CORRECT);
+    hipLaunchKernelGGL((k_correct<T>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, g->a,
+                       (const T*)lf.da[0], lf.ncell);
+  }
+  return check_launch();
+}
+
+// residual on the finest level; writes res in the cycle precision (or FP64
+// when force64) and returns the max norm on the host
+template <int DIM>
+int residual_t(bmg_grid* g, double* maxres, bool force64, unsigned long long* nans, bool keep64 = false) {
+  Level& lf = g->L[g->depth];
+  Geo<DIM> geo{lf.n, lf.log2n};
+  BCA bc;
+  for (int d = 0; d < 3; d++) bc.periodic[d] = d < g->dim ? g->periodic[d] : 0;
+  for (int b = 0; b < 6; b++) {
+    bc.sigma[b] = g->bc_sigma[b];
+    bc.c[b] = g->bc_c[b];
+  }
+  Coef<double> k = fine_coef64(g);
+  double D = delta(g, g->depth);
+  hipLaunchKernelGGL(k_stat_reset, dim3(1), dim3(64), 0, g->stream, g->stat);
+  {
+    ProfScope ps(g, PROF_RESIDUAL);
+    if (g->precision == BMG_MIXED && !force64)
+      hipLaunchKernelGGL((k_residual<DIM, float>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, g->a,
+                         g->b, (float*)lf.res, keep64 ? g->res64 : (double*)nullptr, k, geo, bc, D, g->stat);
+    else
+      hipLaunchKernelGGL((k_residual<DIM, double>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, g->a,
+                         g->b, g->res64, (double*)nullptr, k, geo, bc, D, g->stat);
+  }
+  TRY(check_launch());
+  Stat st;
+  HIPCHECK(hipMemcpyAsync(&st, g->stat, sizeof st, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  double m;
+  memcpy(&m, &st.maxbits, sizeof m);
+  *maxres = m;
+  if (nans) *nans += st.nans;
+  return BMG_OK;
+}
+
+int sum_b(bmg_grid* g, double* out) {
+  Level& lf = g->L[g->depth];
+  int np = (int)std::min<size_t>((size_t)g->npartial, nblocks(lf.ncell));
+  hipLaunchKernelGGL(k_sum_partial, dim3(np), dim3(kBlock), 0, g->stream, g->b, lf.ncell, g->partial);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, g->stream, g->partial, np, g->scalar);
+  TRY(check_launch());
+  HIPCHECK(hipMemcpyAsync(out, g->scalar, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+// restriction ({alpha, lambda}), poisson.h:418-420 in FP64, then the FP32
+// copies of every level for the mixed cycle
+template <int DIM>
+int restrict_coefficients(bmg_grid* g) {
+  for (int l = g->depth - 1; l >= 0; l--) {
+    Level& lc = g->L[l];
+    Level& lf = g->L[l + 1];
+    Geo<DIM> gc{lc.n, lc.log2n};
+    if (g->lambda_field)
+      hipLaunchKernelGGL((k_restrict<DIM, double>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
+                         lf.lam64, lc.lam64, gc);
+    if (g->alpha_field) {
+      for (int d = 0; d < DIM; d++) {
+        hipLaunchKernelGGL((k_restrict_face_lo<DIM>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
+                           lf.alo64[d], lc.alo64[d], gc, d);
+        hipLaunchKernelGGL((k_restrict_face_wall<DIM>), dim3(nblocks(lc.nwall)), dim3(kBlock), 0, g->stream,
+                           lf.awall64[d], lc.awall64[d], lc.n, d);
+      }
+      // the periodic copy of face vectors reaches the components whose
+      // companion carries the periodic BC (grid/multigrid.h:636-651,
+      // grid/cartesian-common.h:903-917): x always, y only in 3D
+      // (only the normal component's high wall is ever read)
+      for (int d = 0; d < DIM; d++) {
+        bool copies = DIM == 1 || d == 0 || (DIM == 3 && d == 1);
+        if (g->periodic[d] && copies)
+          hipLaunchKernelGGL((k_wall_from_first<DIM>), dim3(nblocks(lc.nwall)), dim3(kBlock), 0, g->stream,
+                             lc.alo64[d], lc.awall64[d], gc, d);
+      }
+    }
+  }
+  TRY(check_launch());
+  g->coarse_coef_ready = false;
+  return BMG_OK;
+}
+
+// FP32 copies of the coefficients for the mixed cycle
+int refresh_coef_copies(bmg_grid* g) {
+  if (g->coarse_coef_ready) return BMG_OK;
+  if (g->precision == BMG_MIXED) {
+    for (auto& lv : g->L) {
+      for (int d = 0; d < g->dim && g->alpha_field; d++) {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                           lv.alo64[d], (float*)lv.alo[d], lv.ncell);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(lv.nwall)), dim3(kBlock), 0, g->stream,
+                           lv.awall64[d], (float*)lv.awall[d], lv.nwall);
+      }
+      if (g->lambda_field)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                           lv.lam64, (float*)lv.lam, lv.ncell);
+    }
+    TRY(check_launch());
+  }
+  g->coarse_coef_ready = true;
+  return BMG_OK;
+}
+
+// --- dispatch on dimension / precision
+int dispatch_cycle(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
+  TRY(refresh_coef_copies(g));
+  bool mixed = g->precision == BMG_MIXED;
+  switch (g->dim) {
+    case 1: return mixed ? mg_cycle_t<1, float>(g, nrelax, minlevel, maxlevel) : mg_cycle_t<1, double>(g, nrelax, minlevel, maxlevel);
+    case 2: return mixed ? mg_cycle_t<2, float>(g, nrelax, minlevel, maxlevel) : mg_cycle_t<2, double>(g, nrelax, minlevel, maxlevel);
+    default: return mixed ? mg_cycle_t<3, float>(g, nrelax, minlevel, maxlevel) : mg_cycle_t<3, double>(g, nrelax, minlevel, maxlevel);
+  }
+}
+
+int dispatch_relax(bmg_grid* g, int l) {
+  TRY(refresh_coef_copies(g));
+  bool mixed = g->precision == BMG_MIXED;
+  switch (g->dim) {
+    case 1: return mixed ? relax_level<1, float>(g, l) : relax_level<1, double>(g, l);
+    case 2: return mixed ? relax_level<2, float>(g, l) : relax_level<2, double>(g, l);
+    default: return mixed ? relax_level<3, float>(g, l) : relax_level<3, double>(g, l);
+  }
+}
+
+int dispatch_residual(bmg_grid* g, double* m, bool force64, unsigned long long* nans, bool keep64 = false) {
+  switch (g->dim) {
+    case 1: return residual_t<1>(g, m, force64, nans, keep64);
+    case 2: return residual_t<2>(g, m, force64, nans, keep64);
+    default: return residual_t<3>(g, m, force64, nans, keep64);
+  }
+}
+
+int dispatch_restrict_coef(bmg_grid* g) {
+  switch (g->dim) {
+    case 1: return restrict_coefficients<1>(g);
+    case 2: return restrict_coefficients<2>(g);
+    default: return restrict_coefficients<3>(g);
+  }
+}
+
+int check_grid(const bmg_grid* g) {
+  if (!g) return fail(BMG_ERR_ARG, "null grid");
+  return BMG_OK;
+}
+
+// copy between a user pointer and a device array
+int copy_in(bmg_grid* g, void* dst, const double* src, size_t n, int on_device) {
+  HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+// split a face component array (n+1 along d) into low faces and the wall
+int set_face(bmg_grid* g, int d, int l, const double* src, int on_device) {
+  Level& lv = g->L[l];
+  int n = lv.n;
+  size_t outer = 1, inner = 1;  // dims before d, dims after d
+  for (int e = 0; e < g->dim; e++) {
+    if (e < d) outer *= n;
+    if (e > d) inner *= n;
+  }
+  // src layout: [outer][n+1][inner]; low = [outer][n][inner]; wall = [outer][inner]
+  hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIPCHECK(hipMemcpy2DAsync(lv.alo64[d], (size_t)n * inner * sizeof(double), src, (size_t)(n + 1) * inner * sizeof(double),
+                            (size_t)n * inner * sizeof(double), outer, kind, g->stream));
+  HIPCHECK(hipMemcpy2DAsync(lv.awall64[d], inner * sizeof(double), src + (size_t)n * inner,
+                            (size_t)(n + 1) * inner * sizeof(double), inner * sizeof(double), outer, kind, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+int get_face(bmg_grid* g, int d, int l, double* dst, int on_device) {
+  Level& lv = g->L[l];
+  int n = lv.n;
+  size_t outer = 1, inner = 1;
+  for (int e = 0; e < g->dim; e++) {
+    if (e < d) outer *= n;
+    if (e > d) inner *= n;
+  }
+  hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  HIPCHECK(hipMemcpy2DAsync(dst, (size_t)(n + 1) * inner * sizeof(double), lv.alo64[d], (size_t)n * inner * sizeof(double),
+                            (size_t)n * inner * sizeof(double), outer, kind, g->stream));
+  HIPCHECK(hipMemcpy2DAsync(dst + (size_t)n * inner, (size_t)(n + 1) * inner * sizeof(double), lv.awall64[d],
+                            inner * sizeof(double), inner * sizeof(double), outer, kind, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+int mg_solve_impl(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_mgstats* out) {
+  bmg_mgstats s;
+  memset(&s, 0, sizeof s);
+  unsigned long long nans = 0;
+  TRY(sum_b(g, &s.sum));
+  s.nrelax = nrelax > 0 ? nrelax : 4;
+  double resb;
+  TRY(dispatch_residual(g, &s.resb, false, &nans));
+  resb = s.resa = s.resb;
+  if (tolerance == 0.) tolerance = g->TOLERANCE;
+  for (s.i = 0; s.i < g->NITERMAX && (s.i < g->NITERMIN || s.resa > tolerance); s.i++) {
+    TRY(dispatch_cycle(g, s.nrelax, minlevel, g->depth));
+    TRY(dispatch_residual(g, &s.resa, false, &nans));
+    // nrelax adaptation, poisson.h:198-204
+    if (s.resa > tolerance) {
+      if (resb / s.resa < 1.2 && s.nrelax < 100) s.nrelax++;
+      else if (resb / s.resa > 10 && s.nrelax > 2) s.nrelax--;
+    }
+    resb = s.resa;
+  }
+  s.minlevel = minlevel;
+  if (s.resa > tolerance)
+    fprintf(stderr,
+            "WARNING: convergence for a not reached after %d iterations\n"
+            "  res: %g sum: %g nrelax: %d\n",
+            s.i, s.resa, s.sum, s.nrelax);
+  // in mixed precision the finest-level residual is kept in FP64 too
+  if (g->precision == BMG_MIXED) {
+    double m;
+    TRY(dispatch_residual(g, &m, true, nullptr));
+  }
+  prof_collect(g);
+  if (out) *out = s;
+  if (nans) return fail(BMG_ERR_NAN, "the residual held %llu NaN values", nans);
+  return BMG_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- C ABI
+
+extern "C" {
+
+const char* bmg_version(void) { return "bmg 0.1 (gfx950)"; }
+
+const char* bmg_last_error(void) { return g_err.c_str(); }
+
+int bmg_device_count(int* count) {
+  HIPCHECK(hipGetDeviceCount(count));
+  return BMG_OK;
+}
+
+int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0, double Y0, double Z0) {
+  if (!out) return fail(BMG_ERR_ARG, "null output pointer");
+  if (dimension < 1 || dimension > 3) return fail(BMG_ERR_ARG, "dimension %d not in 1..3", dimension);
+  int maxdepth = dimension == 3 ? 10 : dimension == 2 ? 15 : 28;
+  if (depth < 0 || depth > maxdepth) return fail(BMG_ERR_ARG, "depth %d not in 0..%d", depth, maxdepth);
+  if (!(L0 > 0.)) return fail(BMG_ERR_ARG, "L0 must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(BMG_ERR_HIP, "no HIP device available: libbmg runs on the GPU only");
+  bmg_grid* g = new bmg_grid();
+  g->dim = dimension;
+  g->depth = depth;
+  g->L0 = L0;
+  g->X0[0] = X0; g->X0[1] = Y0; g->X0[2] = Z0;
+  g->L.resize(depth + 1);
+  int r = BMG_OK;
+  // a blocking stream: it orders itself after work on the legacy default
+  // stream, where callers (PyTorch included) usually prepare device inputs
+  if (hipStreamCreateWithFlags(&g->own_stream, hipStreamDefault) != hipSuccess) {
+    delete g;
+    return fail(BMG_ERR_HIP, "hipStreamCreate failed");
+  }
+  g->stream = g->own_stream;
+  for (int l = 0; l <= depth && r == BMG_OK; l++) {
+    Level& lv = g->L[l];
+    lv.n = 1 << l;
+    lv.log2n = l;
+    lv.ncell = powd(lv.n, dimension);
+    lv.nwall = powd(lv.n, dimension - 1);
+    for (int d = 0; d < dimension && r == BMG_OK; d++) {
+      r = dalloc((void**)&lv.alo64[d], lv.ncell * sizeof(double));
+      if (r == BMG_OK) r = dalloc((void**)&lv.awall64[d], lv.nwall * sizeof(double));
+    }
+    if (r == BMG_OK) r = dalloc((void**)&lv.lam64, lv.ncell * sizeof(double));
+  }
+  size_t nf = g->L[depth].ncell;
+  if (r == BMG_OK) r = dalloc((void**)&g->a, nf * sizeof(double));
+  if (r == BMG_OK) r = dalloc((void**)&g->b, nf * sizeof(double));
+  if (r == BMG_OK) r = dalloc((void**)&g->res64, nf * sizeof(double));
+  if (r == BMG_OK) r = dalloc((void**)&g->stat, sizeof(Stat));
+  if (r == BMG_OK) r = dalloc((void**)&g->partial, g->npartial * sizeof(double));
+  if (r == BMG_OK) r = dalloc((void**)&g->scalar, sizeof(double));
+  if (r == BMG_OK) r = alloc_levels(g);
+  if (r == BMG_OK) {
+    // every field starts at zero (reset (all, 0.), grid/multigrid.h:758)
+    for (auto& lv : g->L) {
+      size_t es = esize(g);
+      (void)hipMemsetAsync(lv.da[0], 0, lv.ncell * es, g->stream);
+      (void)hipMemsetAsync(lv.res, 0, lv.ncell * es, g->stream);
+    }
+    for (auto& lv : g->L) {
+      for (int d = 0; d < dimension; d++) {
+        (void)hipMemsetAsync(lv.alo64[d], 0, lv.ncell * sizeof(double), g->stream);
+        (void)hipMemsetAsync(lv.awall64[d], 0, lv.nwall * sizeof(double), g->stream);
+      }
+      (void)hipMemsetAsync(lv.lam64, 0, lv.ncell * sizeof(double), g->stream);
+    }
+    (void)hipMemsetAsync(g->a, 0, nf * sizeof(double), g->stream);
+    (void)hipMemsetAsync(g->b, 0, nf * sizeof(double), g->stream);
+    if (hipStreamSynchronize(g->stream) != hipSuccess) r = fail(BMG_ERR_HIP, "initialisation failed");
+  }
+  if (r != BMG_OK) {
+    std::string keep = g_err;
+    bmg_grid_free(g);
+    g_err = keep;
+    return r;
+  }
+  *out = g;
+  return BMG_OK;
+}
+
+int bmg_grid_free(bmg_grid* g) {
+  if (!g) return BMG_OK;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (auto& lv : g->L) {
+    for (int s = 0; s < 2; s++) dfree(lv.da[s]);
+    if (lv.res != (void*)g->res64) dfree(lv.res);
+    for (int d = 0; d < 3; d++) {
+      if (lv.alo[d] != (void*)lv.alo64[d]) dfree(lv.alo[d]);
+      if (lv.awall[d] != (void*)lv.awall64[d]) dfree(lv.awall[d]);
+      dfreet(lv.alo64[d]);
+      dfreet(lv.awall64[d]);
+    }
+    if (lv.lam != (void*)lv.lam64) dfree(lv.lam);
+    dfreet(lv.lam64);
+  }
+  dfreet(g->a);
+  dfreet(g->b);
+  dfreet(g->res64);
+  for (int b = 0; b < 6; b++) dfreet(g->bc_c[b]);
+  dfreet(g->stat);
+  dfreet(g->partial);
+  dfreet(g->scalar);
+  prof_collect(g);
+  for (auto e : g->ev_pool) (void)hipEventDestroy(e);
+  if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
+  delete g;
+  return BMG_OK;
+}
+
+int bmg_grid_info(const bmg_grid* g, int* dimension, int* depth) {
+  TRY(check_grid(g));
+  if (dimension) *dimension = g->dim;
+  if (depth) *depth = g->depth;
+  return BMG_OK;
+}
+
+int bmg_set_precision(bmg_grid* g, int precision) {
+  TRY(check_grid(g));
+  if (precision != BMG_FP64 && precision != BMG_MIXED) return fail(BMG_ERR_ARG, "unknown precision %d", precision);
+  if (precision == g->precision) return BMG_OK;
+  // the cycle fields are scratch between cycles; a and b are kept
+  g->precision = precision;
+  TRY(alloc_levels(g));
+  return BMG_OK;
+}
+
+int bmg_set_smoother(bmg_grid* g, int smoother) {
+  TRY(check_grid(g));
+  if (smoother == BMG_RELAX_GS)
+    return fail(BMG_ERR_UNSUPPORTED, "lexicographic Gauss-Seidel is not available on the GPU path; use JACOBI or RBGS");
+  if (smoother != BMG_RELAX_JACOBI && smoother != BMG_RELAX_RBGS) return fail(BMG_ERR_ARG, "unknown smoother %d", smoother);
+  g->smoother = smoother;
+  return ensure_jacobi_buffers(g);
+}
+
+int bmg_set_tolerance(bmg_grid* g, double tolerance) {
+  TRY(check_grid(g));
+  g->TOLERANCE = tolerance;
+  return BMG_OK;
+}
+
+int bmg_set_niter(bmg_grid* g, int nitermax, int nitermin) {
+  TRY(check_grid(g));
+  if (nitermax < 0 || nitermin < 0) return fail(BMG_ERR_ARG, "negative iteration count");
+  g->NITERMAX = nitermax;
+  g->NITERMIN = nitermin;
+  return BMG_OK;
+}
+
+int bmg_set_stream(bmg_grid* g, void* stream) {
+  TRY(check_grid(g));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  g->stream = stream ? (hipStream_t)stream : g->own_stream;
+  return BMG_OK;
+}
+
+static int set_bc_common(bmg_grid* g, int b, int kind, int sigma, const double* c, double scale) {
+  if (b < 0 || b >= 2 * g->dim) return fail(BMG_ERR_ARG, "direction %d out of range", b);
+  int d = b / 2;
+  dfreet(g->bc_c[b]);
+  if (kind == BMG_BC_PERIODIC) {
+    g->periodic[d] = 1;
+    g->bc_kind[2 * d] = g->bc_kind[2 * d + 1] = BMG_BC_PERIODIC;
+    dfreet(g->bc_c[b ^ 1]);
+    return BMG_OK;
+  }
+  if (g->periodic[d]) {
+    g->periodic[d] = 0;
+    g->bc_kind[b ^ 1] = BMG_BC_SYMMETRY;
+    g->bc_sigma[b ^ 1] = 1;
+  }
+  g->bc_kind[b] = kind;
+  g->bc_sigma[b] = sigma;
+  if (c) {
+    size_t nw = g->L[g->depth].nwall;
+    std::vector<double> h(nw);
+    for (size_t i = 0; i < nw; i++) h[i] = scale == 1. ? c[i] : scale * c[i];
+    TRY(dalloc((void**)&g->bc_c[b], nw * sizeof(double)));
+    HIPCHECK(hipMemcpy(g->bc_c[b], h.data(), nw * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return BMG_OK;
+}
+
+int bmg_set_bc(bmg_grid* g, int b, int kind, const double* values) {
+  TRY(check_grid(g));
+  switch (kind) {
+    case BMG_BC_SYMMETRY: return set_bc_common(g, b, kind, 1, nullptr, 1.);
+    // dirichlet: 2.*(expr) - s  ->  c = 2 v, sigma = -1 (exact: 2v is exact and
+    // c + (-1)*s rounds like c - s)
+    case BMG_BC_DIRICHLET: return set_bc_common(g, b, kind, -1, values, 2.);
+    // neumann: Delta*(expr) + s  ->  c = Delta g, sigma = +1
+    case BMG_BC_NEUMANN: return set_bc_common(g, b, kind, 1, values, delta(g, g->depth));
+    case BMG_BC_PERIODIC: return set_bc_common(g, b, kind, 1, nullptr, 1.);
+    default: return fail(BMG_ERR_ARG, "unknown BC kind %d", kind);
+  }
+}
+
+int bmg_set_bc_affine(bmg_grid* g, int b, int sigma, const double* c) {
+  TRY(check_grid(g));
+  if (sigma != 1 && sigma != -1) return fail(BMG_ERR_ARG, "sigma must be +1 or -1");
+  return set_bc_common(g, b, BMG_BC_AFFINE, sigma, c, 1.);
+}
+
+int bmg_set_alpha_const(bmg_grid* g, double alpha) {
+  TRY(check_grid(g));
+  g->alpha_field = false;
+  g->alpha_c = alpha;
+  g->coarse_coef_ready = false;
+  return BMG_OK;
+}
+
+int bmg_set_lambda_const(bmg_grid* g, double lambda) {
+  TRY(check_grid(g));
+  g->lambda_field = false;
+  g->lambda_c = lambda;
+  g->coarse_coef_ready = false;
+  return BMG_OK;
+}
+
+int bmg_set_field(bmg_grid* g, int field, int level, const double* ptr, int on_device) {
+  TRY(check_grid(g));
+  if (!ptr) return fail(BMG_ERR_ARG, "null field pointer");
+  if (level < 0 || level > g->depth) return fail(BMG_ERR_ARG, "level %d out of range", level);
+  Level& lv = g->L[level];
+  g->coarse_coef_ready = false;
+  if (field >= BMG_F_ALPHA && field < BMG_F_ALPHA + g->dim) {
+    TRY(set_face(g, field - BMG_F_ALPHA, level, ptr, on_device));
+    g->alpha_field = true;
+    return BMG_OK;
+  }
+  switch (field) {
+    case BMG_F_A:
+    case BMG_F_B:
+      if (level != g->depth) return fail(BMG_ERR_ARG, "a and b live on the finest level only");
+      return copy_in(g, field == BMG_F_A ? g->a : g->b, ptr, lv.ncell, on_device);
+    case BMG_F_LAMBDA:
+      TRY(copy_in(g, lv.lam64, ptr, lv.ncell, on_device));
+      g->lambda_field = true;
+      return BMG_OK;
+    case BMG_F_RES:
+    case BMG_F_DA: {
+      void* dst = field == BMG_F_RES ? lv.res : lv.da[0];
+      if (g->precision == BMG_FP64) return copy_in(g, dst, ptr, lv.ncell, on_device);
+      // mixed: stage through the FP64 scratch of the finest residual
+      TRY(copy_in(g, g->res64, ptr, lv.ncell, on_device));
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, g->res64,
+                         (float*)dst, lv.ncell);
+      TRY(check_launch());
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      return BMG_OK;
+    }
+    default: return fail(BMG_ERR_ARG, "unknown field %d", field);
+  }
+}
+
+int bmg_get_field(bmg_grid* g, int field, int level, double* ptr, int on_device) {
+  TRY(check_grid(g));
+  if (!ptr) return fail(BMG_ERR_ARG, "null field pointer");
+  if (level < 0 || level > g->depth) return fail(BMG_ERR_ARG, "level %d out of range", level);
+  Level& lv = g->L[level];
+  hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (field >= BMG_F_ALPHA && field < BMG_F_ALPHA + g->dim) return get_face(g, field - BMG_F_ALPHA, level, ptr, on_device);
+  const void* src = nullptr;
+  switch (field) {
+    case BMG_F_A:
+    case BMG_F_B:
+      if (level != g->depth) return fail(BMG_ERR_ARG, "a and b live on the finest level only");
+      src = field == BMG_F_A ? g->a : g->b;
+      break;
+    case BMG_F_LAMBDA: src = lv.lam64; break;
+    case BMG_F_RES:
+      if (level == g->depth) { src = g->res64; break; }
+      /* fall through */
+    case BMG_F_DA: {
+      const void* s = field == BMG_F_RES ? lv.res : lv.da[0];
+      if (g->precision == BMG_FP64) { src = s; break; }
+      double* tmp = nullptr;
+      TRY(dalloc((void**)&tmp, lv.ncell * sizeof(double)));
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                         (const float*)s, tmp, lv.ncell);
+      hipError_t e = hipMemcpyAsync(ptr, tmp, lv.ncell * sizeof(double), kind, g->stream);
+      (void)hipStreamSynchronize(g->stream);
+      (void)hipFree(tmp);
+      if (e != hipSuccess) return fail(BMG_ERR_HIP, "copy failed: %s", hipGetErrorString(e));
+      return BMG_OK;
+    }
+    default: return fail(BMG_ERR_ARG, "unknown field %d", field);
+  }
+  HIPCHECK(hipMemcpyAsync(ptr, src, lv.ncell * sizeof(double), kind, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+int bmg_restriction_alpha_lambda(bmg_grid* g) {
+  TRY(check_grid(g));
+  TRY(dispatch_restrict_coef(g));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+int bmg_residual(bmg_grid* g, double* maxres) {
+  TRY(check_grid(g));
+  double m = 0.;
+  unsigned long long nans = 0;
+  TRY(dispatch_residual(g, &m, false, &nans, true));
+  if (maxres) *maxres = m;
+  if (nans) return fail(BMG_ERR_NAN, "the residual held %llu NaN values", nans);
+  return BMG_OK;
+}
+
+int bmg_relax(bmg_grid* g, int level) {
+  TRY(check_grid(g));
+  if (level < 0 || level > g->depth) return fail(BMG_ERR_ARG, "level %d out of range", level);
+  TRY(dispatch_relax(g, level));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+int bmg_mg_cycle(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
+  TRY(check_grid(g));
+  if (maxlevel < 0 || maxlevel > g->depth) return fail(BMG_ERR_ARG, "maxlevel %d out of range", maxlevel);
+  if (nrelax < 0) return fail(BMG_ERR_ARG, "nrelax must be >= 0");
+  TRY(dispatch_cycle(g, nrelax, minlevel, maxlevel));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  prof_collect(g);
+  return BMG_OK;
+}
+
+int bmg_mg_solve(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_mgstats* stats) {
+  TRY(check_grid(g));
+  return mg_solve_impl(g, tolerance, nrelax, minlevel, stats);
+}
+
+// poisson(), poisson.h:402-441
+int bmg_poisson(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_mgstats* stats) {
+  TRY(check_grid(g));
+  TRY(dispatch_restrict_coef(g));
+  double defaultol = g->TOLERANCE;
+  if (tolerance) g->TOLERANCE = tolerance;
+  int r = mg_solve_impl(g, 0., nrelax, std::max(1, minlevel), stats);
+  if (tolerance) g->TOLERANCE = defaultol;
+  return r;
+}
+
+int bmg_synchronize(bmg_grid* g) {
+  TRY(check_grid(g));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+int bmg_profile_enable(bmg_grid* g, int enable) {
+  TRY(check_grid(g));
+  prof_collect(g);
+  g->prof = enable != 0;
+  for (int k = 0; k < PROF_N; k++) {
+    g->prof_ms[k] = 0.;
+    g->prof_n[k] = 0;
+  }
+  return BMG_OK;
+}
+
+int bmg_profile_get(bmg_grid* g, int kind, double* total_ms, long* count) {
+  TRY(check_grid(g));
+  if (kind < 0 || kind >= PROF_N) return fail(BMG_ERR_ARG, "unknown profile kind %d", kind);
+  prof_collect(g);
+  if (total_ms) *total_ms = g->prof_ms[kind];
+  if (count) *count = g->prof_n[kind];
+  return BMG_OK;
+}
+
+// --------------------------------------------------------------------------
+// poisson() on the reference's own storage.  The view is read once per call
+// (a, b, alpha, lambda on the finest level), the whole solve runs on the
+// GPU, and a (and res when asked for) is written back once.  Ghost cells of
+// the host storage are not written: the caller marks `a` dirty so that the
+// reference refills them on its next stencil.
+
+}  // extern "C"
+
+namespace {
+
+struct ViewCache {
+  bmg_grid* g = nullptr;
+  int dim = -1, depth = -1;
+  double L0 = 0., X0 = 0., Y0 = 0., Z0 = 0.;
+};
+ViewCache g_view_cache;
+
+inline const char* view_cell(const bmg_multigrid_view* v, int l, const int ijk[3]) {
+  size_t m = (size_t)(1 << l) + 4, idx;
+  if (v->dimension == 3) idx = ((size_t)(ijk[0] + 2) * m + (size_t)(ijk[1] + 2)) * m + (size_t)(ijk[2] + 2);
+  else if (v->dimension == 2) idx = (size_t)(ijk[0] + 2) * m + (size_t)(ijk[1] + 2);
+  else idx = (size_t)(ijk[0] + 2);
+  return v->d[l] + idx * v->datasize;
+}
+
+// visit the cells of a box with extents ext (C order)
+template <typename F>
+void for_box(int dim, const int ext[3], F f) {
+  int e0 = ext[0], e1 = dim >= 2 ? ext[1] : 1, e2 = dim >= 3 ? ext[2] : 1;
+  size_t p = 0;
+  for (int i = 0; i < e0; i++)
+    for (int j = 0; j < e1; j++)
+      for (int k = 0; k < e2; k++, p++) {
+        int c[3] = {i, j, k};
+        f(c, p);
+      }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_poisson_view(const bmg_multigrid_view* v, const bmg_poisson_args* p, const bmg_view_bc* bc,
+                     double TOLERANCE, int NITERMAX, int NITERMIN, bmg_mgstats* stats) {
+  if (!v || !p || !bc || !v->d) return fail(BMG_ERR_ARG, "null view argument");
+  int dim = v->dimension, depth = v->depth;
+  if (dim < 1 || dim > 3) return fail(BMG_ERR_ARG, "dimension %d not in 1..3", dim);
+  if (v->datasize % sizeof(double) || p->a < 0 || p->b < 0 || p->a >= v->nvarmax || p->b >= v->nvarmax)
+    return fail(BMG_ERR_ARG, "bad field handles or datasize");
+  ViewCache& vc = g_view_cache;
+  if (!vc.g || vc.dim != dim || vc.depth != depth || vc.L0 != v->L0 || vc.X0 != v->X0 || vc.Y0 != v->Y0 ||
+      vc.Z0 != v->Z0) {
+    bmg_grid_free(vc.g);
+    vc.g = nullptr;
+    TRY(bmg_grid_new(&vc.g, dim, depth, v->L0, v->X0, v->Y0, v->Z0));
+    vc.dim = dim; vc.depth = depth; vc.L0 = v->L0; vc.X0 = v->X0; vc.Y0 = v->Y0; vc.Z0 = v->Z0;
+  }
+  bmg_grid* g = vc.g;
+  int n = 1 << depth;
+  int ext[3] = {n, n, n};
+  size_t ncell = powd(n, dim);
+  std::vector<double> buf(ncell);
+  auto gather = [&](int h, double* out, const int* e) {
+    for_box(dim, e, [&](const int c[3], size_t q) { out[q] = ((const double*)view_cell(v, depth, c))[h]; });
+  };
+  gather(p->a, buf.data(), ext);
+  TRY(bmg_set_field(g, BMG_F_A, depth, buf.data(), 0));
+  gather(p->b, buf.data(), ext);
+  TRY(bmg_set_field(g, BMG_F_B, depth, buf.data(), 0));
+  // alpha: unset (0) or constant handles -> constant; otherwise face fields
+  bool aconst = true;
+  for (int c = 0; c < dim; c++) aconst &= p->alpha[c] == 0 || p->alpha[c] >= v->nvarmax;
+  if (aconst) {
+    double val = 1.;  // unityf
+    if (p->alpha[0] >= v->nvarmax) val = v->constant[p->alpha[0] - v->nvarmax];
+    TRY(bmg_set_alpha_const(g, val));
+  } else {
+    for (int c = 0; c < dim; c++) {
+      if (p->alpha[c] <= 0 || p->alpha[c] >= v->nvarmax)
+        return fail(BMG_ERR_UNSUPPORTED, "alpha mixes constant and variable components");
+      int fe[3] = {n, n, n};
+      fe[c] = n + 1;
+      std::vector<double> fb(powd(n, dim) / n * (n + 1));
+      gather(p->alpha[c], fb.data(), fe);
+      TRY(bmg_set_field(g, BMG_F_ALPHA + c, depth, fb.data(), 0));
+    }
+  }
+  if (p->lambda <= 0 || p->lambda >= v->nvarmax) {
+    double val = p->lambda >= v->nvarmax ? v->constant[p->lambda - v->nvarmax] : 0.;  // zeroc
+    TRY(bmg_set_lambda_const(g, val));
+  } else {
+    gather(p->lambda, buf.data(), ext);
+    TRY(bmg_set_field(g, BMG_F_LAMBDA, depth, buf.data(), 0));
+  }
+  for (int b = 0; b < 2 * dim; b++) {
+    if (bc->kind[b] == BMG_BC_PERIODIC) TRY(bmg_set_bc(g, b, BMG_BC_PERIODIC, nullptr));
+    else if (bc->kind[b] == BMG_BC_AFFINE) TRY(bmg_set_bc_affine(g, b, bc->sigma[b], bc->c[b]));
+    else return fail(BMG_ERR_ARG, "view BC kind must be AFFINE or PERIODIC");
+  }
+  TRY(bmg_set_tolerance(g, TOLERANCE));
+  TRY(bmg_set_niter(g, NITERMAX, NITERMIN));
+  bmg_mgstats s;
+  int r = bmg_poisson(g, p->tolerance, p->nrelax, p->minlevel, &s);
+  if (r != BMG_OK && r != BMG_ERR_NAN) return r;
+  if (stats) *stats = s;
+  auto scatter = [&](int h, const double* in) {
+    for_box(dim, ext, [&](const int c[3], size_t q) { ((double*)view_cell(v, depth, c))[h] = in[q]; });
+  };
+  TRY(bmg_get_field(g, BMG_F_A, depth, buf.data(), 0));
+  scatter(p->a, buf.data());
+  if (p->res >= 0 && p->res < v->nvarmax) {
+    TRY(bmg_get_field(g, BMG_F_RES, depth, buf.data(), 0));
+    scatter(p->res, buf.data());
+  }
+  return r;
+}
+
+}  // extern "C"

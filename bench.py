@@ -1,0 +1,177 @@
+"""Benchmark: time-to-1e-9 residual and HBM GB/s of the 256^3
+variable-coefficient (density ratio 1000) Poisson problem, FP64 vs mixed.
+
+One step = one poisson() solve from a = 0 to max|res| <= 1e-9 on the GPU
+(the alpha/lambda restriction of poisson() included), with b and alpha
+already resident in HBM.  Both precisions are timed; `value` is the mixed
+solver's time-to-tolerance.  A separate pass of each precision records HIP
+events around the fine-level relax sweeps for the roofline line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--depth 8]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--smoother", default="rbgs")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--precisions", default="fp64,mixed")
+    return ap.parse_args()
+
+
+def solve_once(g, zeros, tol):
+    g.set_a(zeros)
+    return g.poisson(tolerance=tol)
+
+
+def run_precision(bmg, torch, prob, prec, args):
+    dim, depth = prob["dim"], prob["depth"]
+    g = bmg.Grid(dim, depth, precision=prec, smoother=args.smoother)
+    dev = torch.device("cuda:0")
+    b = torch.from_numpy(prob["b"]).to(dev)
+    g.set_b(b)
+    g.set_alpha([torch.from_numpy(a).to(dev) for a in prob["alpha"]])
+    zeros = torch.zeros(b.shape, dtype=torch.float64, device=dev)
+    tol = prob["tolerance"]
+    for _ in range(args.warmup):
+        solve_once(g, zeros, tol)
+    g.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(args.steps):
+        stats.append(solve_once(g, zeros, tol))
+    g.synchronize()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # timing pass: HIP events on the library's stream around each kernel class
+    g.profile(True)
+    st = solve_once(g, zeros, tol)
+    kinds = {}
+    for k in ("relax_fine", "residual", "restriction", "prolongation", "correction", "relax_coarse"):
+        ms, n = g.profile_get(k)
+        kinds[k] = {"ms": ms, "count": n}
+    g.profile(False)
+    a = g.get_a()
+    g.close()
+    return {"seconds": dt, "stats": stats[-1], "profile": kinds, "profiled_stats": st, "a": a}
+
+
+def cpu_baseline(prob, args, ncycles_target):
+    """The oracle's OpenMP build (reference-default lexicographic GS,
+    AoS storage, schedule(static)) on the host cores: a bounded sample of
+    `--cpu-cycles` V-cycles of the same problem, extrapolated to the GPU
+    FP64 run's cycle count."""
+    from oracle.oracle import OracleGrid, set_threads
+    threads = min(os.cpu_count() or 1, 16)
+    set_threads(threads, prob["dim"])
+    o = OracleGrid(prob["dim"], prob["depth"], omp=True)
+    o.set_b(prob["b"])
+    o.set_alpha(prob["alpha"])
+    o.set_a(0 * prob["b"])
+    t0 = time.perf_counter()
+    o.restriction_alpha_lambda()
+    o.residual()
+    for _ in range(args.cpu_cycles):
+        o.mg_cycle(4, 1)
+        o.residual()
+    dt = (time.perf_counter() - t0) / args.cpu_cycles
+    return {"value": dt * ncycles_target * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+            "sample": "%d V-cycles (GS, nrelax 4) + residuals of the %d^3 problem on %d OpenMP threads, "
+                      "%.3f s per cycle, scaled to the GPU FP64 run's %d cycles"
+                      % (args.cpu_cycles, 1 << prob["depth"], threads, dt, ncycles_target)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import basilisk_amd as bmg
+    from basilisk_amd import configs
+
+    prob = configs.config2(depth=args.depth, seed=1)
+    N = prob["b"].size
+    res = {}
+    for prec in args.precisions.split(","):
+        res[prec] = run_precision(bmg, torch, prob, prec, args)
+        print("# %s: %.2f ms, %d cycles, resa %.3g" % (prec, res[prec]["seconds"] * 1e3, res[prec]["stats"]["i"],
+                                                      res[prec]["stats"]["resa"]), file=sys.stderr, flush=True)
+
+    main_prec = "mixed" if "mixed" in res else args.precisions.split(",")[0]
+    r = res[main_prec]
+    cycles = r["stats"]["i"]
+    w = 4 if main_prec == "mixed" else 8
+    # fine-level relax sweep: da read, res read, 3 alpha read, da write = 6w
+    # bytes per cell (the reference's relax, poisson.h:303-320)
+    prof = r["profile"]["relax_fine"]
+    sweep_ms = prof["ms"] / max(prof["count"], 1)
+    sweep_bytes = 6 * w * N
+    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0
+    # algorithmic bytes of the whole solve (per V-cycle and cell, 3D, nrelax
+    # 4: ~312 B FP64, ~192 B mixed) for the whole-solve GB/s
+    per_cycle_bytes = {"fp64": 312.0, "mixed": 192.0}[main_prec] * N
+    line = {
+        "metric": "time-to-1e-9 residual, 256^3 var-coeff Poisson (mixed precision)",
+        "value": round(r["seconds"] * 1e3, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["seconds"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32/f64" if main_prec == "mixed" else "f64",
+        "data": "synthetic (seeded spheres, density ratio 1000; seeded N(0,1) face velocity)",
+        "config": {"workload": "3D multigrid level %d (%d^3), alpha=1/rho face-averaged, symmetry walls, "
+                               "tolerance 1e-9, %s smoother" % (args.depth, 1 << args.depth, args.smoother),
+                   "cells": N, "parallelism": "single GPU"},
+        "cycles": cycles,
+        "vcycles_per_s": round(cycles / r["seconds"], 1),
+        "cells_per_s": N * cycles / r["seconds"],
+        "solve_gbs": round(per_cycle_bytes * cycles / r["seconds"] / 1e9, 1),
+        "resa": r["stats"]["resa"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "fine-level relax sweep (%s, both colours)" % main_prec,
+                     "sweep_ms": round(sweep_ms, 5), "algorithmic_bytes": sweep_bytes},
+    }
+    if "fp64" in res and "mixed" in res:
+        f, m = res["fp64"], res["mixed"]
+        line["fp64_ms"] = round(f["seconds"] * 1e3, 3)
+        line["fp64_cycles"] = f["stats"]["i"]
+        line["mixed_speedup_vs_fp64"] = round(f["seconds"] / m["seconds"], 3)
+        af, am = f["a"], m["a"]
+        af = af - af.mean()
+        am = am - am.mean()
+        line["mixed_vs_fp64_rel_maxnorm"] = float(np.abs(am - af).max() / np.abs(af).max())
+        pf = f["profile"]["relax_fine"]
+        line["fp64_sweep_gbs"] = round(6 * 8 * N / (pf["ms"] / max(pf["count"], 1) * 1e-3) / 1e9, 1)
+    line["kernel_ms"] = {k: round(v["ms"], 3) for k, v in r["profile"].items()}
+    if not args.no_cpu_baseline:
+        ncyc = res["fp64"]["stats"]["i"] if "fp64" in res else cycles
+        line["cpu_baseline"] = cpu_baseline(prob, args, ncyc)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

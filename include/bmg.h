@@ -1,0 +1,182 @@
+/*
+ * bmg.h -- C ABI of libbmg, the MI355X multigrid Poisson-Helmholtz solver.
+ *
+ * libbmg keeps a uniform multigrid hierarchy resident in GPU memory and runs
+ * the reference's solver on it: mg_cycle, mg_solve and poisson() with their
+ * relax and residual steps, restriction, prolongation and ghost handling.
+ * Every entry point returns an int status (BMG_OK on success); the message
+ * of the last failure is available from bmg_last_error().
+ *
+ * Two families of entry points:
+ *
+ *  - bmg_grid_* / bmg_set_* / bmg_poisson ... work on SoA arrays in C order
+ *    (i slowest, k fastest, the reference's storage order) passed as plain
+ *    host or device pointers.  They back the Python package and the tests.
+ *
+ *  - bmg_poisson_view reads and writes the reference's own storage
+ *    (multigrid->d[l], datasize, field index s.i, GHOSTS = 2) so that the
+ *    drop-in poisson.h can forward poisson() unchanged.
+ *
+ * Reference interfaces replaced (paths relative to the reference's src/):
+ *   mgstats                        poisson.h:108-114   -> bmg_mgstats
+ *   mg_cycle(...)                  poisson.h:32-90     -> bmg_mg_cycle
+ *   mg_solve(struct MGSolve)       poisson.h:125-235   -> bmg_mg_solve
+ *   relax / residual callbacks     poisson.h:271-391   -> bmg_relax, bmg_residual
+ *   poisson(struct Poisson)        poisson.h:258-265, 402-441 -> bmg_poisson, bmg_poisson_view
+ *   NITERMAX, NITERMIN, TOLERANCE  poisson.h:102-103   -> bmg_set_niter, bmg_set_tolerance
+ *   restriction ({alpha,lambda})   grid/multigrid-common.h:368-404 -> bmg_restriction_alpha_lambda
+ *   boundary_level (box, periodic) grid/multigrid.h:543-701 -> implicit in every kernel
+ */
+#ifndef BMG_H
+#define BMG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+  BMG_OK = 0,
+  BMG_ERR_ARG = 1,          /* invalid argument (shape, level, kind ...) */
+  BMG_ERR_HIP = 2,          /* a HIP runtime call failed */
+  BMG_ERR_UNSUPPORTED = 3,  /* valid request this build does not run */
+  BMG_ERR_NOMEM = 4,        /* device allocation failed */
+  BMG_ERR_NAN = 5           /* the residual held NaN (stats still filled) */
+};
+
+/* mgstats, poisson.h:108-114 (same layout) */
+typedef struct {
+  int i;              /* number of iterations */
+  double resb, resa;  /* maximum residual before and after the iterations */
+  double sum;         /* sum of r.h.s. */
+  int nrelax;         /* number of relaxations */
+  int minlevel;       /* minimum level of the multigrid hierarchy */
+} bmg_mgstats;
+
+/* boundary kinds of the unknown `a` on one side of the box */
+enum {
+  BMG_BC_SYMMETRY = 0,  /* ghost = s              (default, neumann(0)) */
+  BMG_BC_DIRICHLET = 1, /* ghost = 2 v - s        common.h:957 */
+  BMG_BC_NEUMANN = 2,   /* ghost = Delta g + s    common.h:959 */
+  BMG_BC_PERIODIC = 3,  /* periodic in that dimension, grid/multigrid.h:638-695 */
+  BMG_BC_AFFINE = 4     /* ghost = c + sigma s, c given per face (see bmg_set_bc_affine) */
+};
+
+/* smoothers: the reference's in-place Gauss-Seidel (poisson.h:296), its
+   -DJACOBI weighted Jacobi (poisson.h:293-294, 325-328), and red-black GS */
+enum { BMG_RELAX_GS = 0, BMG_RELAX_JACOBI = 1, BMG_RELAX_RBGS = 2 };
+
+/* arithmetic of the cycle: plain FP64, or FP32 smoothing / restriction /
+   prolongation inside FP64 residual, correction and norm */
+enum { BMG_FP64 = 0, BMG_MIXED = 1 };
+
+/* field identifiers */
+enum {
+  BMG_F_A = 0,       /* unknown (finest level) */
+  BMG_F_B = 1,       /* right-hand side (finest level) */
+  BMG_F_RES = 2,     /* residual (any level) */
+  BMG_F_DA = 3,      /* correction (any level) */
+  BMG_F_LAMBDA = 4,  /* Helmholtz coefficient field (any level) */
+  BMG_F_ALPHA = 5    /* face coefficient: BMG_F_ALPHA + c for component c (any level) */
+};
+
+typedef struct bmg_grid bmg_grid;
+
+/* library */
+const char *bmg_version(void);
+const char *bmg_last_error(void);
+int bmg_device_count(int *count);
+
+/* grid lifetime: a uniform hierarchy of levels 0..depth over the box
+   [X0, X0+L0] x ... (init_grid, grid/multigrid.h:723-759) */
+int bmg_grid_new(bmg_grid **g, int dimension, int depth, double L0,
+                 double X0, double Y0, double Z0);
+int bmg_grid_free(bmg_grid *g);
+int bmg_grid_info(const bmg_grid *g, int *dimension, int *depth);
+
+/* solver configuration */
+int bmg_set_precision(bmg_grid *g, int precision);
+int bmg_set_smoother(bmg_grid *g, int smoother);
+int bmg_set_tolerance(bmg_grid *g, double tolerance);           /* TOLERANCE */
+int bmg_set_niter(bmg_grid *g, int nitermax, int nitermin);     /* NITERMAX, NITERMIN */
+int bmg_set_stream(bmg_grid *g, void *hip_stream);              /* NULL: the grid's own */
+
+/* boundary conditions of `a`; direction 0 right, 1 left, 2 top, 3 bottom,
+   4 front, 5 back (common.h:947-951).  `values` (host memory, may be NULL
+   for 0) holds the BC expression on each boundary face of the finest level,
+   n^(dim-1) values in C order over the tangential indices.  The correction
+   field uses the homogeneous version (poisson.h:155-157). */
+int bmg_set_bc(bmg_grid *g, int direction, int kind, const double *values);
+/* general affine BC: ghost = c + sigma * s, with sigma = +1 or -1 */
+int bmg_set_bc_affine(bmg_grid *g, int direction, int sigma, const double *c);
+
+/* fields: `ptr` is host memory (on_device = 0) or device memory (1).
+   Cell fields are n^dim values; face component c is n+1 along c and n
+   along the other dimensions.  alpha and lambda are constant (1 and 0, the
+   reference's unityf and zeroc) until a field is set. */
+int bmg_set_field(bmg_grid *g, int field, int level, const double *ptr, int on_device);
+int bmg_get_field(bmg_grid *g, int field, int level, double *ptr, int on_device);
+int bmg_set_alpha_const(bmg_grid *g, double alpha);
+int bmg_set_lambda_const(bmg_grid *g, double lambda);
+
+/* the solver, on the resident fields */
+int bmg_restriction_alpha_lambda(bmg_grid *g);
+int bmg_residual(bmg_grid *g, double *maxres);
+int bmg_relax(bmg_grid *g, int level);
+int bmg_mg_cycle(bmg_grid *g, int nrelax, int minlevel, int maxlevel);
+int bmg_mg_solve(bmg_grid *g, double tolerance, int nrelax, int minlevel,
+                 bmg_mgstats *stats);
+int bmg_poisson(bmg_grid *g, double tolerance, int nrelax, int minlevel,
+                bmg_mgstats *stats);
+int bmg_synchronize(bmg_grid *g);
+
+/* kernel timing with HIP events on the grid's stream; kind 0 = fine-level
+   relax, 1 = residual, 2 = restriction, 3 = prolongation, 4 = correction,
+   5 = coarse-level relax */
+int bmg_profile_enable(bmg_grid *g, int enable);
+int bmg_profile_get(bmg_grid *g, int kind, double *total_ms, long *count);
+
+/* ---------------------------------------------------------------- views of
+   the reference's multigrid storage (grid/multigrid.h:16-73) */
+
+typedef struct {
+  int dimension;          /* 1, 2 or 3 */
+  int depth;              /* grid->depth */
+  char **d;               /* multigrid->d: level l holds (2^l+4)^dim records */
+  size_t datasize;        /* bytes per cell record */
+  int nvarmax;            /* _NVARMAX: handles >= nvarmax are constants */
+  const double *constant; /* _constant: value of handle h is constant[h - nvarmax] */
+  double L0, X0, Y0, Z0;
+} bmg_multigrid_view;
+
+/* struct Poisson, poisson.h:258-265, with field handles as ints (s.i) */
+typedef struct {
+  int a, b;
+  int alpha[3];           /* face vector components; 0 or a constant handle allowed */
+  int lambda;
+  double tolerance;
+  int nrelax, minlevel;
+  int res;                /* handle of the residual output, or -1 */
+} bmg_poisson_args;
+
+/* Boundary of `a` as the drop-in reads it from the BC function pointers:
+   per direction, kind (BMG_BC_AFFINE or BMG_BC_PERIODIC), sigma = the
+   homogeneous BC applied to s = 1, and c = the inhomogeneous BC applied to
+   s = 0 on each boundary face of the finest level (n^(dim-1) values). */
+typedef struct {
+  int kind[6];
+  int sigma[6];
+  const double *c[6];
+} bmg_view_bc;
+
+int bmg_poisson_view(const bmg_multigrid_view *v, const bmg_poisson_args *p,
+                     const bmg_view_bc *bc, double TOLERANCE, int NITERMAX,
+                     int NITERMIN, bmg_mgstats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BMG_H */
