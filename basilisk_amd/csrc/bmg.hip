@@ -338,9 +338,18 @@ int residual_t(bmg_grid* g, double* maxres, bool force64, unsigned long long* na
   Geo<DIM> geo{lf.n, lf.log2n};
   BCA bc;
   for (int d = 0; d < 3; d++) bc.periodic[d] = d < g->dim ? g->periodic[d] : 0;
+  bool anyp = false;
+  for (int d = 0; d < 3; d++) anyp |= bc.periodic[d] != 0;
   for (int b = 0; b < 6; b++) {
     bc.sigma[b] = g->bc_sigma[b];
     bc.c[b] = g->bc_c[b];
+    // a one-cell grid with a periodic dimension: the periodic copy gives
+    // every ghost the root value, box conditions included
+    // (grid/multigrid.h:655-664)
+    if (g->depth == 0 && anyp) {
+      bc.sigma[b] = 1;
+      bc.c[b] = nullptr;
+    }
   }
   Coef<double> k = fine_coef64(g);
   double D = delta(g, g->depth);
@@ -397,13 +406,21 @@ int restrict_coefficients(bmg_grid* g) {
       // the periodic copy of face vectors reaches the components whose
       // companion carries the periodic BC (grid/multigrid.h:636-651,
       // grid/cartesian-common.h:903-917): x always, y only in 3D
-      // (only the normal component's high wall is ever read)
+      // (only the normal component's high wall is ever read).  On level 0
+      // the copy sets every ghost of every component to the root's values
+      // (grid/multigrid.h:655-664), so all the walls follow.
+      bool any_copy = false;
       for (int d = 0; d < DIM; d++) {
         bool copies = DIM == 1 || d == 0 || (DIM == 3 && d == 1);
-        if (g->periodic[d] && copies)
+        any_copy |= g->periodic[d] && copies;
+        if (g->periodic[d] && copies && l > 0)
           hipLaunchKernelGGL((k_wall_from_first<DIM>), dim3(nblocks(lc.nwall)), dim3(kBlock), 0, g->stream,
                              lc.alo64[d], lc.awall64[d], gc, d);
       }
+      if (l == 0 && any_copy)
+        for (int c = 0; c < DIM; c++)
+          hipLaunchKernelGGL((k_wall_from_first<DIM>), dim3(1), dim3(kBlock), 0, g->stream, lc.alo64[c],
+                             lc.awall64[c], gc, c);
     }
   }
   TRY(check_launch());
