@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -87,6 +88,8 @@ struct bmg_grid {
   bool alpha_field = false, lambda_field = false;
   double alpha_c = 1., lambda_c = 0.;
   bool coarse_coef_ready = false;  // FP32 copies / restricted levels valid
+  bool coarse_res_fresh = false;   // the residual kernel also wrote res[depth-1]
+  Part* part = nullptr;            // per-block partial max / NaN count
   hipStream_t own_stream = nullptr, stream = nullptr;
   Stat* stat = nullptr;
   double* partial = nullptr;
@@ -95,6 +98,7 @@ struct bmg_grid {
   bool allocated_precision[2] = {false, false};
   // profiling
   bool prof = false;
+  int rb_override = -1;  // BMG_RB_KERNEL: kernel for red-black levels of >= 4096 cells
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
   double prof_ms[PROF_N] = {0};
@@ -188,7 +192,8 @@ int alloc_levels(bmg_grid* g) {
   for (int l = 0; l <= g->depth; l++) {
     Level& lv = g->L[l];
     TRY(dalloc(&lv.da[0], lv.ncell * es));
-    if (g->smoother == BMG_RELAX_JACOBI) TRY(dalloc(&lv.da[1], lv.ncell * es));
+    // second buffer for the ping-pong sweeps (Jacobi, streaming red-black)
+    TRY(dalloc(&lv.da[1], lv.ncell * es));
     // in FP64 the finest residual is res64 itself
     if (p == BMG_FP64 && l == g->depth) lv.res = g->res64;
     else TRY(dalloc(&lv.res, lv.ncell * es));
@@ -206,6 +211,7 @@ int alloc_levels(bmg_grid* g) {
   }
   g->allocated_precision[p] = true;
   g->coarse_coef_ready = false;
+  g->coarse_res_fresh = false;
   return BMG_OK;
 }
 
@@ -266,27 +272,131 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 
 // --- kernels per dimension / precision --------------------------------
 
+// relax kernel of a level: the two-launch in-place red-black, the
+// single-pass streaming red-black (3D, n >= 128) or Jacobi
+// relax kernel of a level: two-launch in-place red-black (tiny levels),
+// one-launch fused red-black, single-pass streaming red-black (3D,
+// n >= 128) or Jacobi
+enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_JACOBI };
+
+template <int DIM>
+int level_kernel(const bmg_grid* g, int l) {
+  if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
+  if (g->rb_override >= 0 && g->L[l].ncell >= 4096 &&
+      (g->rb_override != RK_STREAM_RB || (DIM == 3 && g->L[l].n >= 128)))
+    return g->rb_override;
+  // measured at 256^3: the two-launch in-place kernel is the fastest on
+  // large levels; the one-launch fused kernel halves the launches on
+  // mid-size levels, where launch latency dominates
+  if (g->L[l].ncell >= (1u << 20)) return RK_INPLACE_RB;
+  if (g->L[l].ncell >= 1024) return RK_FUSED_RB;
+  return RK_INPLACE_RB;
+}
+
+int stream_chunk(int n) { return n >= 256 ? 16 : 8; }
+
+// largest level that the single-workgroup coarse kernel takes (<= 1024 cells)
+int coarse_top(const bmg_grid* g) {
+  int top = g->dim == 3 ? 3 : g->dim == 2 ? 5 : 10;
+  return std::min(top, g->depth);
+}
+
+// one relax sweep of level l on da[src]; the ping-pong kernels write
+// da[src^1], the in-place kernel da[src].  Returns the buffer holding the
+// result.
 template <int DIM, typename T>
-int relax_level(bmg_grid* g, int l) {
+int sweep(bmg_grid* g, int l, int src, int* rc) {
   Level& lv = g->L[l];
   Geo<DIM> geo{lv.n, lv.log2n};
   BCH bc = level_bc<DIM>(g, l);
   Coef<T> k = level_coef<T>(g, l);
   double D = delta(g, l);
   T D2 = (T)(D * D);
-  ProfScope ps(g, l == g->depth ? PROF_RELAX_FINE : PROF_RELAX_COARSE);
-  if (g->smoother == BMG_RELAX_JACOBI) {
-    hipLaunchKernelGGL((k_relax_jacobi<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
-                       (const T*)lv.da[0], (T*)lv.da[1], (const T*)lv.res, k, geo, bc, D2);
-    std::swap(lv.da[0], lv.da[1]);
-  } else if (g->smoother == BMG_RELAX_RBGS) {
-    size_t half = lv.n > 1 ? lv.ncell / 2 : 1;
-    for (int colour = 0; colour < 2; colour++)
-      hipLaunchKernelGGL((k_relax_rb<DIM, T>), dim3(nblocks(half)), dim3(kBlock), 0, g->stream,
-                         (T*)lv.da[0], (const T*)lv.res, k, geo, bc, D2, colour);
-  } else {
-    return fail(BMG_ERR_UNSUPPORTED, "lexicographic Gauss-Seidel is not available on the GPU path");
+  int out = src;
+  {
+    ProfScope ps(g, l == g->depth ? PROF_RELAX_FINE : PROF_RELAX_COARSE);
+    switch (level_kernel<DIM>(g, l)) {
+      case RK_JACOBI:
+        hipLaunchKernelGGL((k_relax_jacobi<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                           (const T*)lv.da[src], (T*)lv.da[src ^ 1], (const T*)lv.res, k, geo, bc, D2);
+        out = src ^ 1;
+        break;
+      case RK_FUSED_RB:
+        hipLaunchKernelGGL((k_relax_rb_fused<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                           (const T*)lv.da[src], (T*)lv.da[src ^ 1], (const T*)lv.res, k, geo, bc, D2);
+        out = src ^ 1;
+        break;
+      case RK_STREAM_RB:
+        if constexpr (DIM == 3) {
+          int chunk = stream_chunk(lv.n);
+          unsigned nb = (unsigned)((lv.n / kTK) * (lv.n / kTJ) * (lv.n / chunk));
+          const T* din = (const T*)lv.da[src];
+          T* dout = (T*)lv.da[src ^ 1];
+          const T* r = (const T*)lv.res;
+          bool af = k.lo[0] != nullptr, lf = k.lam != nullptr;
+          if (af && lf)
+            hipLaunchKernelGGL((k_rb_stream3<T, true, true>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout, r, k,
+                               geo, bc, D2, chunk);
+          else if (af)
+            hipLaunchKernelGGL((k_rb_stream3<T, true, false>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout, r,
+                               k, geo, bc, D2, chunk);
+          else if (lf)
+            hipLaunchKernelGGL((k_rb_stream3<T, false, true>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout, r,
+                               k, geo, bc, D2, chunk);
+          else
+            hipLaunchKernelGGL((k_rb_stream3<T, false, false>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout, r,
+                               k, geo, bc, D2, chunk);
+        }
+        out = src ^ 1;
+        break;
+      default: {
+        size_t half = lv.n > 1 ? lv.ncell / 2 : 1;
+        for (int colour = 0; colour < 2; colour++)
+          hipLaunchKernelGGL((k_relax_rb<DIM, T>), dim3(nblocks(half)), dim3(kBlock), 0, g->stream,
+                             (T*)lv.da[src], (const T*)lv.res, k, geo, bc, D2, colour);
+      }
+    }
   }
+  *rc = check_launch();
+  return out;
+}
+
+// the public single relax (bmg_relax): result kept in da[0]
+template <int DIM, typename T>
+int relax_level(bmg_grid* g, int l) {
+  if (g->smoother != BMG_RELAX_JACOBI && g->smoother != BMG_RELAX_RBGS)
+    return fail(BMG_ERR_UNSUPPORTED, "lexicographic Gauss-Seidel is not available on the GPU path");
+  int rc = BMG_OK;
+  int out = sweep<DIM, T>(g, l, 0, &rc);
+  if (out == 1) std::swap(g->L[l].da[0], g->L[l].da[1]);
+  return rc;
+}
+
+template <int DIM, typename T>
+int coarse_block(bmg_grid* g, int nrelax, int minlevel, int top) {
+  static_assert(sizeof(CoarseArgs<T, DIM>) <= 4096, "kernel argument block too large");
+  if (top >= kCoarseLevels || minlevel < 0 || minlevel > top)
+    return fail(BMG_ERR_ARG, "internal: coarse levels %d..%d out of range", minlevel, top);
+  CoarseArgs<T, DIM> A;
+  memset((void*)&A, 0, sizeof A);
+  for (int l = minlevel; l <= top; l++) {
+    Level& lv = g->L[l];
+    CoarseLevel<T, DIM>& C = A.L[l];
+    C.da = (T*)lv.da[0];
+    C.da2 = (T*)lv.da[1];
+    C.res = (T*)lv.res;
+    C.k = level_coef<T>(g, l);
+    C.g = Geo<DIM>{lv.n, lv.log2n};
+    C.bc = level_bc<DIM>(g, l);
+    double D = delta(g, l);
+    C.D2 = (T)(D * D);
+  }
+  A.minlevel = minlevel;
+  A.top = top;
+  A.nrelax = nrelax;
+  A.jacobi = g->smoother == BMG_RELAX_JACOBI;
+  ProfScope ps(g, PROF_RELAX_COARSE);
+  hipLaunchKernelGGL((k_coarse_cycle<DIM, T>), dim3(1), dim3(1024), 0, g->stream, A);
   return check_launch();
 }
 
@@ -296,9 +406,14 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
   // by the cycle and are not computed
   minlevel = std::min(minlevel, maxlevel);
   if (minlevel < 0) minlevel = 0;
+  int top = std::min(coarse_top(g), maxlevel);
+  bool use_block = top >= minlevel;
+  int lowest = use_block ? top : minlevel;
+  int lstart = g->coarse_res_fresh ? g->depth - 2 : g->depth - 1;
+  g->coarse_res_fresh = false;
   {
     ProfScope ps(g, PROF_RESTRICT);
-    for (int l = g->depth - 1; l >= minlevel; l--) {
+    for (int l = lstart; l >= lowest; l--) {
       Level& lc = g->L[l];
       Geo<DIM> gc{lc.n, lc.log2n};
       hipLaunchKernelGGL((k_restrict<DIM, T>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
@@ -306,20 +421,29 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
     }
   }
   TRY(check_launch());
-  for (int l = minlevel; l <= maxlevel; l++) {
+  // the coarse levels in one workgroup; they end in da[0]
+  if (use_block) TRY((coarse_block<DIM, T>(g, nrelax, minlevel, top)));
+  for (int l = use_block ? top + 1 : minlevel; l <= maxlevel; l++) {
     Level& lv = g->L[l];
+    // ping-pong sweeps start in da[nrelax & 1] so that they end in da[0]
+    int dst = level_kernel<DIM>(g, l) == RK_INPLACE_RB ? 0 : (nrelax & 1);
     if (l == minlevel) {
-      hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, (T*)lv.da[0],
+      hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, (T*)lv.da[dst],
                          lv.ncell, T(0));
     } else {
       ProfScope ps(g, PROF_PROLONG);
       Geo<DIM> gf{lv.n, lv.log2n};
       BCH bcc = level_bc<DIM>(g, l - 1);
       hipLaunchKernelGGL((k_prolong<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
-                         (const T*)g->L[l - 1].da[0], (T*)lv.da[0], gf, bcc);
+                         (const T*)g->L[l - 1].da[0], (T*)lv.da[dst], gf, bcc);
     }
     TRY(check_launch());
-    for (int it = 0; it < nrelax; it++) TRY((relax_level<DIM, T>(g, l)));
+    int cur = dst, rc = BMG_OK;
+    for (int it = 0; it < nrelax; it++) {
+      cur = sweep<DIM, T>(g, l, cur, &rc);
+      TRY(rc);
+    }
+    if (cur != 0) return fail(BMG_ERR_ARG, "internal: sweep parity");
   }
   Level& lf = g->L[g->depth];
   {
@@ -353,10 +477,31 @@ int residual_t(bmg_grid* g, double* maxres, bool force64, unsigned long long* na
   }
   Coef<double> k = fine_coef64(g);
   double D = delta(g, g->depth);
-  hipLaunchKernelGGL(k_stat_reset, dim3(1), dim3(64), 0, g->stream, g->stat);
-  {
+  // x / D == x * (1/D) exactly when D is a power of two
+  int ex;
+  bc.pow2 = std::frexp(D, &ex) == 0.5;
+  bc.invD = 1. / D;
+  // bricks of whole 2^DIM child groups: the fused residual + restriction
+  bool fused = (DIM == 3 && lf.n >= 64) || (DIM == 2 && lf.n >= 128);
+  bool mixed = g->precision == BMG_MIXED && !force64;
+  if (fused) {
     ProfScope ps(g, PROF_RESIDUAL);
-    if (g->precision == BMG_MIXED && !force64)
+    // at most 4096 blocks (a multiple of 8), each folding several bricks
+    unsigned nb = (unsigned)std::min<size_t>(lf.ncell / kBlock, 4096);
+    void* coarse = force64 ? nullptr : g->L[g->depth - 1].res;
+    if (mixed)
+      hipLaunchKernelGGL((k_residual_restrict<DIM, float>), dim3(nb), dim3(kBlock), 0, g->stream, g->a, g->b,
+                         (float*)lf.res, (float*)coarse, keep64 ? g->res64 : (double*)nullptr, k, geo, bc, D,
+                         g->part);
+    else
+      hipLaunchKernelGGL((k_residual_restrict<DIM, double>), dim3(nb), dim3(kBlock), 0, g->stream, g->a, g->b,
+                         g->res64, (double*)coarse, (double*)nullptr, k, geo, bc, D, g->part);
+    hipLaunchKernelGGL(k_reduce_stat, dim3(1), dim3(1024), 0, g->stream, g->part, (int)nb, g->stat);
+    if (!force64) g->coarse_res_fresh = true;
+  } else {
+    hipLaunchKernelGGL(k_stat_reset, dim3(1), dim3(64), 0, g->stream, g->stat);
+    ProfScope ps(g, PROF_RESIDUAL);
+    if (mixed)
       hipLaunchKernelGGL((k_residual<DIM, float>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, g->a,
                          g->b, (float*)lf.res, keep64 ? g->res64 : (double*)nullptr, k, geo, bc, D, g->stat);
     else
@@ -602,6 +747,12 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
   g->L0 = L0;
   g->X0[0] = X0; g->X0[1] = Y0; g->X0[2] = Z0;
   g->L.resize(depth + 1);
+  // tuning knob: the red-black kernel of the larger levels
+  if (const char* e = getenv("BMG_RB_KERNEL")) {
+    if (!strcmp(e, "inplace")) g->rb_override = RK_INPLACE_RB;
+    else if (!strcmp(e, "fused")) g->rb_override = RK_FUSED_RB;
+    else if (!strcmp(e, "stream") && dimension == 3) g->rb_override = RK_STREAM_RB;
+  }
   int r = BMG_OK;
   // a blocking stream: it orders itself after work on the legacy default
   // stream, where callers (PyTorch included) usually prepare device inputs
@@ -628,6 +779,7 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
   if (r == BMG_OK) r = dalloc((void**)&g->res64, nf * sizeof(double));
   if (r == BMG_OK) r = dalloc((void**)&g->stat, sizeof(Stat));
   if (r == BMG_OK) r = dalloc((void**)&g->partial, g->npartial * sizeof(double));
+  if (r == BMG_OK) r = dalloc((void**)&g->part, (nf / kBlock + 1) * sizeof(Part));
   if (r == BMG_OK) r = dalloc((void**)&g->scalar, sizeof(double));
   if (r == BMG_OK) r = alloc_levels(g);
   if (r == BMG_OK) {
@@ -679,6 +831,7 @@ int bmg_grid_free(bmg_grid* g) {
   for (int b = 0; b < 6; b++) dfreet(g->bc_c[b]);
   dfreet(g->stat);
   dfreet(g->partial);
+  dfreet(g->part);
   dfreet(g->scalar);
   prof_collect(g);
   for (auto e : g->ev_pool) (void)hipEventDestroy(e);
@@ -803,6 +956,7 @@ int bmg_set_field(bmg_grid* g, int field, int level, const double* ptr, int on_d
   if (level < 0 || level > g->depth) return fail(BMG_ERR_ARG, "level %d out of range", level);
   Level& lv = g->L[level];
   g->coarse_coef_ready = false;
+  g->coarse_res_fresh = false;
   if (field >= BMG_F_ALPHA && field < BMG_F_ALPHA + g->dim) {
     TRY(set_face(g, field - BMG_F_ALPHA, level, ptr, on_device));
     g->alpha_field = true;

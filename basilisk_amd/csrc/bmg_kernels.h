@@ -29,82 +29,100 @@ struct BCH {
   int sigma[6];  // right, left, top, bottom, front, back
 };
 
-// inhomogeneous boundary of `a` on the finest level: ghost = c + sigma * s
+// inhomogeneous boundary of `a` on the finest level: ghost = c + sigma * s,
+// with the cell size (x / Delta is computed as x * invD when Delta is a
+// power of two, which rounds identically)
 struct BCA {
   int periodic[3];
   int sigma[6];
   const double* c[6];  // n^(DIM-1) values per side, or null for 0
+  double invD;
+  int pow2;
 };
+
+// cell indices fit in 32 bits (at most 2^30 cells per level)
+using idx_t = uint32_t;
 
 template <int DIM>
 struct Geo {
   int n, log2n;
-  __device__ __forceinline__ void decode(size_t idx, int c[3]) const {
+  __device__ __forceinline__ void decode(idx_t idx, int c[3]) const {
+    const idx_t m = (idx_t)(n - 1);
     if (DIM == 3) {
-      c[2] = (int)(idx & (size_t)(n - 1));
-      c[1] = (int)((idx >> log2n) & (size_t)(n - 1));
+      c[2] = (int)(idx & m);
+      c[1] = (int)((idx >> log2n) & m);
       c[0] = (int)(idx >> (2 * log2n));
     } else if (DIM == 2) {
       c[2] = 0;
-      c[1] = (int)(idx & (size_t)(n - 1));
+      c[1] = (int)(idx & m);
       c[0] = (int)(idx >> log2n);
     } else {
       c[0] = (int)idx;
       c[1] = c[2] = 0;
     }
   }
-  __device__ __forceinline__ size_t stride(int d) const {
-    if (DIM == 3) return d == 0 ? (size_t)n * n : d == 1 ? (size_t)n : 1;
-    if (DIM == 2) return d == 0 ? (size_t)n : 1;
-    return 1;
+  __device__ __forceinline__ idx_t stride(int d) const {
+    if (DIM == 3) return d == 0 ? (idx_t)1 << (2 * log2n) : d == 1 ? (idx_t)1 << log2n : 1u;
+    if (DIM == 2) return d == 0 ? (idx_t)1 << log2n : 1u;
+    return 1u;
   }
-  __device__ __forceinline__ size_t index(const int c[3]) const {
-    if (DIM == 3) return ((size_t)c[0] * n + c[1]) * n + c[2];
-    if (DIM == 2) return (size_t)c[0] * n + c[1];
-    return (size_t)c[0];
+  __device__ __forceinline__ idx_t index(const int c[3]) const {
+    if (DIM == 3) return ((((idx_t)c[0] << log2n) + (idx_t)c[1]) << log2n) + (idx_t)c[2];
+    if (DIM == 2) return ((idx_t)c[0] << log2n) + (idx_t)c[1];
+    return (idx_t)c[0];
   }
   // index on the wall normal to d (tangential coordinates in order)
-  __device__ __forceinline__ size_t wall(int d, const int c[3]) const {
-    if (DIM == 3) return d == 0 ? (size_t)c[1] * n + c[2] : d == 1 ? (size_t)c[0] * n + c[2] : (size_t)c[0] * n + c[1];
-    if (DIM == 2) return d == 0 ? (size_t)c[1] : (size_t)c[0];
-    return 0;
+  __device__ __forceinline__ idx_t wall(int d, const int c[3]) const {
+    if (DIM == 3)
+      return d == 0 ? ((idx_t)c[1] << log2n) + c[2] : d == 1 ? ((idx_t)c[0] << log2n) + c[2]
+                                                           : ((idx_t)c[0] << log2n) + c[1];
+    if (DIM == 2) return d == 0 ? (idx_t)c[1] : (idx_t)c[0];
+    return 0u;
   }
-  __device__ __forceinline__ size_t ncell() const {
-    return DIM == 3 ? (size_t)n * n * n : DIM == 2 ? (size_t)n * n : (size_t)n;
-  }
+  __device__ __forceinline__ idx_t ncell() const { return (idx_t)1 << (DIM * log2n); }
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous run of logical blocks so that neighbouring rows and planes of
 // a stencil share one L2.
-__device__ __forceinline__ size_t logical_block() {
-  size_t b = blockIdx.x, nb = gridDim.x;
+__device__ __forceinline__ idx_t logical_block() {
+  idx_t b = blockIdx.x, nb = gridDim.x;
   if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
   return b;
 }
 
-// value of field f at the neighbour of cell c along d (dir +1 or -1), with
-// the homogeneous ghost mapping
-template <int DIM, typename T>
-__device__ __forceinline__ T nbr_h(const T* __restrict__ f, const Geo<DIM>& g, const BCH& bc,
-                                   size_t idx, const int c[3], int d, int dir, T self) {
-  size_t s = g.stride(d);
-  if (dir > 0) {
-    if (c[d] < g.n - 1) return f[idx + s];
-    if (bc.periodic[d]) return f[idx - (size_t)(g.n - 1) * s];
-    return T(bc.sigma[2 * d]) * self;
-  } else {
-    if (c[d] > 0) return f[idx - s];
-    if (bc.periodic[d]) return f[idx + (size_t)(g.n - 1) * s];
-    return T(bc.sigma[2 * d + 1]) * self;
-  }
+// index of the neighbour of cell c along d (dir +1 or -1): the cell itself
+// when the neighbour is a non-periodic ghost.  `ghost` tells which.
+template <int DIM>
+__device__ __forceinline__ idx_t nbr_index(const Geo<DIM>& g, const int* periodic, idx_t idx, const int c[3],
+                                           int d, int dir, bool& ghost) {
+  const idx_t s = g.stride(d);
+  const bool inside = dir > 0 ? c[d] < g.n - 1 : c[d] > 0;
+  const idx_t wrapoff = (idx_t)(g.n - 1) * s;
+  const idx_t step = dir > 0 ? idx + s : idx - s;
+  const idx_t wrapped = dir > 0 ? idx - wrapoff : idx + wrapoff;
+  ghost = !inside && !periodic[d];
+  return inside ? step : (periodic[d] ? wrapped : idx);
 }
 
-// alpha on the high face of cell c along d
+// value of field f at the neighbour of cell c along d (dir +1 or -1), with
+// the homogeneous ghost mapping (branch-free: a ghost reads the cell itself
+// and scales it by sigma)
 template <int DIM, typename T>
-__device__ __forceinline__ T alpha_hi(const T* __restrict__ lo, const T* __restrict__ wall,
-                                      const Geo<DIM>& g, size_t idx, const int c[3], int d) {
-  return c[d] < g.n - 1 ? lo[idx + g.stride(d)] : wall[g.wall(d, c)];
+__device__ __forceinline__ T nbr_h(const T* __restrict__ f, const Geo<DIM>& g, const BCH& bc, idx_t idx,
+                                   const int c[3], int d, int dir, T self) {
+  bool ghost;
+  idx_t iq = nbr_index<DIM>(g, bc.periodic, idx, c, d, dir, ghost);
+  T v = ghost ? self : f[iq];
+  return ghost ? T(bc.sigma[2 * d + (dir < 0)]) * v : v;
+}
+
+// alpha on the high face of cell c along d (one load from a selected address)
+template <int DIM, typename T>
+__device__ __forceinline__ T alpha_hi(const T* __restrict__ lo, const T* __restrict__ wall, const Geo<DIM>& g,
+                                      idx_t idx, const int c[3], int d) {
+  const T* p = c[d] < g.n - 1 ? lo + idx + g.stride(d) : wall + g.wall(d, c);
+  return *p;
 }
 
 template <typename T>
@@ -118,9 +136,9 @@ struct Coef {
 
 // n/d of the relax step, poisson.h:303-319
 template <int DIM, typename T>
-__device__ __forceinline__ T relax_value(const T* __restrict__ da, const T* __restrict__ res,
-                                         const Coef<T>& k, const Geo<DIM>& g, const BCH& bc,
-                                         size_t idx, const int c[3], T self, T D2) {
+__device__ __forceinline__ T relax_value(const T* __restrict__ da, const T* __restrict__ res, const Coef<T>& k,
+                                         const Geo<DIM>& g, const BCH& bc, idx_t idx, const int c[3], T self,
+                                         T D2) {
   T lam = k.lam ? k.lam[idx] : k.lam_c;
   T nn = -D2 * res[idx], dd = -lam * D2;
 #pragma unroll
@@ -145,7 +163,7 @@ template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_relax_jacobi(const T* __restrict__ da, T* __restrict__ out,
                                                          const T* __restrict__ res, Coef<T> k,
                                                          Geo<DIM> g, BCH bc, T D2) {
-  size_t idx = logical_block() * kBlock + threadIdx.x;
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
   if (idx >= g.ncell()) return;
   int c[3];
   g.decode(idx, c);
@@ -154,23 +172,76 @@ __global__ __launch_bounds__(kBlock) void k_relax_jacobi(const T* __restrict__ d
   out[idx] = (self + T(2) * cval) / T(3);
 }
 
+// a full red-black sweep in one launch, dold -> dnew: a red cell is relaxed
+// from the old field; a black cell relaxes its red neighbours on the fly
+// (from the old field, exactly as the red pass does) and then itself from
+// those values, so the result equals the two-colour sweep bit for bit.
+// A ghost neighbour of a black cell is sigma times its own old value.
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_relax_rb_fused(const T* __restrict__ dold, T* __restrict__ dnew,
+                                                           const T* __restrict__ res, Coef<T> k, Geo<DIM> g,
+                                                           BCH bc, T D2) {
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
+  if (idx >= g.ncell()) return;
+  int c[3];
+  g.decode(idx, c);
+  T self = dold[idx];
+  if (((c[0] + c[1] + c[2]) & 1) == 0) {
+    dnew[idx] = relax_value<DIM, T>(dold, res, k, g, bc, idx, c, self, D2);
+    return;
+  }
+  T lam = k.lam ? k.lam[idx] : k.lam_c;
+  T nn = -D2 * res[idx], dd = -lam * D2;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    T v[2];
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      int dir = side == 0 ? 1 : -1;
+      int q[3] = {c[0], c[1], c[2]};
+      q[d] += dir;
+      bool ghost = false;
+      if (q[d] < 0 || q[d] >= g.n) {
+        if (bc.periodic[d]) q[d] = q[d] < 0 ? q[d] + g.n : q[d] - g.n;
+        else ghost = true;
+      }
+      if (ghost) {
+        v[side] = T(bc.sigma[2 * d + side]) * self;
+      } else {
+        idx_t iq = g.index(q);
+        v[side] = relax_value<DIM, T>(dold, res, k, g, bc, iq, q, dold[iq], D2);
+      }
+    }
+    T ah, al;
+    if (k.lo[0]) {
+      ah = alpha_hi<DIM, T>(k.lo[d], k.wall[d], g, idx, c, d);
+      al = k.lo[d][idx];
+    } else {
+      ah = al = k.alpha;
+    }
+    nn += ah * v[0] + al * v[1];
+    dd += ah + al;
+  }
+  dnew[idx] = nn / dd;
+}
+
 // one colour of a red-black Gauss-Seidel sweep, in place; one thread per
 // cell of that colour (colour = parity of i+j+k)
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_relax_rb(T* __restrict__ da, const T* __restrict__ res,
                                                      Coef<T> k, Geo<DIM> g, BCH bc, T D2, int colour) {
-  size_t t = logical_block() * kBlock + threadIdx.x;
-  size_t half = g.n > 1 ? g.ncell() / 2 : (colour == 0 ? 1 : 0);
+  idx_t t = logical_block() * kBlock + threadIdx.x;
+  idx_t half = g.n > 1 ? g.ncell() / 2 : (colour == 0 ? 1u : 0u);
   if (t >= half) return;
-  size_t idx;
+  idx_t idx;
   if (g.n > 1) {
     // rows along the last dimension hold n/2 cells of each colour
-    size_t row = t >> (g.log2n - 1);
-    int kk = (int)(t & (size_t)((g.n >> 1) - 1));
+    idx_t row = t >> (g.log2n - 1);
+    idx_t kk = t & (idx_t)((g.n >> 1) - 1);
     int c0[3];
     g.decode(row << g.log2n, c0);
     int par = DIM == 3 ? (c0[0] + c0[1]) & 1 : DIM == 2 ? c0[0] & 1 : 0;
-    idx = (row << g.log2n) + 2 * kk + ((colour + par) & 1);
+    idx = (row << g.log2n) + 2 * kk + (idx_t)((colour + par) & 1);
   } else {
     idx = 0;
   }
@@ -185,25 +256,25 @@ __global__ __launch_bounds__(kBlock) void k_relax_rb(T* __restrict__ da, const T
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_restrict(const T* __restrict__ fine, T* __restrict__ coarse,
                                                      Geo<DIM> gc) {
-  size_t idx = logical_block() * kBlock + threadIdx.x;
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
   if (idx >= gc.ncell()) return;
   int c[3];
   gc.decode(idx, c);
   Geo<DIM> gf{gc.n * 2, gc.log2n + 1};
   int f[3] = {2 * c[0], 2 * c[1], 2 * c[2]};
-  size_t b = gf.index(f);
+  idx_t b = gf.index(f);
   T sum = T(0);
   if (DIM == 1) {
     sum += fine[b];
     sum += fine[b + 1];
   } else if (DIM == 2) {
-    size_t sx = gf.stride(0);
+    idx_t sx = gf.stride(0);
     sum += fine[b];
     sum += fine[b + 1];
     sum += fine[b + sx];
     sum += fine[b + sx + 1];
   } else {
-    size_t sx = gf.stride(0), sy = gf.stride(1);
+    idx_t sx = gf.stride(0), sy = gf.stride(1);
     sum += fine[b];
     sum += fine[b + 1];
     sum += fine[b + sy];
@@ -234,50 +305,62 @@ __device__ __forceinline__ T coarse_at(const T* __restrict__ f, const Geo<DIM>& 
   return sign * f[g.index(q)];
 }
 
-// fine = bilinear interpolation of the coarse level, grid/multigrid-common.h:124-140,
-// child offsets grid/multigrid.h:175-188
+// index contributions of the two coarse cells along one dimension of the
+// bilinear stencil: the parent p and its neighbour p+o (o = -1 or +1),
+// mapped like a ghost cell (wrapped, or the parent itself with sign sigma)
+template <int DIM, typename T>
+__device__ __forceinline__ void axis_pair(const Geo<DIM>& gc, const BCH& bc, int d, int p, int o, idx_t& off0,
+                                          idx_t& off1, T& sg) {
+  const idx_t s = gc.stride(d);
+  const int q = p + o;
+  const bool out = q < 0 || q >= gc.n;
+  const int qw = q < 0 ? q + gc.n : q - gc.n;
+  const int qm = out ? (bc.periodic[d] ? qw : p) : q;
+  sg = (out && !bc.periodic[d]) ? T(bc.sigma[2 * d + (o < 0)]) : T(1);
+  off0 = (idx_t)p * s;
+  off1 = (idx_t)qm * s;
+}
+
+// bilinear value at fine cell c from the coarse level,
+// grid/multigrid-common.h:124-140, child offsets grid/multigrid.h:175-188
+template <int DIM, typename T>
+__device__ __forceinline__ T bilinear_at(const T* __restrict__ coarse, const Geo<DIM>& gc, const BCH& bcc,
+                                         const int c[3]) {
+  idx_t x0, x1, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
+  T sx, sy = T(1), sz = T(1);
+  axis_pair<DIM, T>(gc, bcc, 0, c[0] >> 1, 2 * (c[0] & 1) - 1, x0, x1, sx);
+  if (DIM >= 2) axis_pair<DIM, T>(gc, bcc, 1, c[1] >> 1, 2 * (c[1] & 1) - 1, y0, y1, sy);
+  if (DIM >= 3) axis_pair<DIM, T>(gc, bcc, 2, c[2] >> 1, 2 * (c[2] & 1) - 1, z0, z1, sz);
+  if (DIM == 1) {
+    return (T(3) * coarse[x0] + sx * coarse[x1]) / T(4);
+  } else if (DIM == 2) {
+    T c00 = coarse[x0 + y0];
+    T c10 = sx * coarse[x1 + y0];
+    T c01 = sy * coarse[x0 + y1];
+    T c11 = (sx * sy) * coarse[x1 + y1];
+    return (T(9) * c00 + T(3) * (c10 + c01) + c11) / T(16);
+  } else {
+    T c000 = coarse[x0 + y0 + z0];
+    T c100 = sx * coarse[x1 + y0 + z0];
+    T c010 = sy * coarse[x0 + y1 + z0];
+    T c001 = sz * coarse[x0 + y0 + z1];
+    T c110 = (sx * sy) * coarse[x1 + y1 + z0];
+    T c101 = (sx * sz) * coarse[x1 + y0 + z1];
+    T c011 = (sy * sz) * coarse[x0 + y1 + z1];
+    T c111 = ((sx * sy) * sz) * coarse[x1 + y1 + z1];
+    return (T(27) * c000 + T(9) * (c100 + c010 + c001) + T(3) * (c110 + c101 + c011) + c111) / T(64);
+  }
+}
+
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong(const T* __restrict__ coarse, T* __restrict__ fine,
                                                     Geo<DIM> gf, BCH bcc) {
-  size_t idx = logical_block() * kBlock + threadIdx.x;
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
   if (idx >= gf.ncell()) return;
   int c[3];
   gf.decode(idx, c);
   Geo<DIM> gc{gf.n >> 1, gf.log2n - 1};
-  int p[3] = {c[0] >> 1, c[1] >> 1, c[2] >> 1};
-  int o[3] = {2 * (c[0] & 1) - 1, 2 * (c[1] & 1) - 1, 2 * (c[2] & 1) - 1};
-  T v;
-  if (DIM == 1) {
-    int q0[3] = {p[0], 0, 0}, q1[3] = {p[0] + o[0], 0, 0};
-    v = (T(3) * coarse_at<DIM, T>(coarse, gc, bcc, q0) + coarse_at<DIM, T>(coarse, gc, bcc, q1)) / T(4);
-  } else if (DIM == 2) {
-    int q00[3] = {p[0], p[1], 0}, q10[3] = {p[0] + o[0], p[1], 0};
-    int q01[3] = {p[0], p[1] + o[1], 0}, q11[3] = {p[0] + o[0], p[1] + o[1], 0};
-    T c00 = coarse_at<DIM, T>(coarse, gc, bcc, q00);
-    T c10 = coarse_at<DIM, T>(coarse, gc, bcc, q10);
-    T c01 = coarse_at<DIM, T>(coarse, gc, bcc, q01);
-    T c11 = coarse_at<DIM, T>(coarse, gc, bcc, q11);
-    v = (T(9) * c00 + T(3) * (c10 + c01) + c11) / T(16);
-  } else {
-    int q000[3] = {p[0], p[1], p[2]};
-    int q100[3] = {p[0] + o[0], p[1], p[2]};
-    int q010[3] = {p[0], p[1] + o[1], p[2]};
-    int q001[3] = {p[0], p[1], p[2] + o[2]};
-    int q110[3] = {p[0] + o[0], p[1] + o[1], p[2]};
-    int q101[3] = {p[0] + o[0], p[1], p[2] + o[2]};
-    int q011[3] = {p[0], p[1] + o[1], p[2] + o[2]};
-    int q111[3] = {p[0] + o[0], p[1] + o[1], p[2] + o[2]};
-    T c000 = coarse_at<DIM, T>(coarse, gc, bcc, q000);
-    T c100 = coarse_at<DIM, T>(coarse, gc, bcc, q100);
-    T c010 = coarse_at<DIM, T>(coarse, gc, bcc, q010);
-    T c001 = coarse_at<DIM, T>(coarse, gc, bcc, q001);
-    T c110 = coarse_at<DIM, T>(coarse, gc, bcc, q110);
-    T c101 = coarse_at<DIM, T>(coarse, gc, bcc, q101);
-    T c011 = coarse_at<DIM, T>(coarse, gc, bcc, q011);
-    T c111 = coarse_at<DIM, T>(coarse, gc, bcc, q111);
-    v = (T(27) * c000 + T(9) * (c100 + c010 + c001) + T(3) * (c110 + c101 + c011) + c111) / T(64);
-  }
-  fine[idx] = v;
+  fine[idx] = bilinear_at<DIM, T>(coarse, gc, bcc, c);
 }
 
 template <typename T>
@@ -295,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_convert(const TI* __restrict__ in, T
 // a += da on the finest level, poisson.h:84-89
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_correct(double* __restrict__ a, const T* __restrict__ da, size_t n) {
-  size_t idx = logical_block() * kBlock + threadIdx.x;
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
   if (idx < n) a[idx] += (double)da[idx];
 }
 
@@ -341,45 +424,51 @@ __device__ __forceinline__ void block_max_to(Stat* st, double v) {
   }
 }
 
-// residual of `a` on the finest level, poisson.h:373-389 (!TREE), with the
-// inhomogeneous ghost of `a`; writes res in the cycle precision TO
+// residual of one cell of `a` on the finest level, poisson.h:373-389
+// (!TREE): b - lambda a + sum_d (alpha[] (a[] - a[-1])/D - alpha[1] (a[1] - a[])/D)/D,
+// with the inhomogeneous ghost of `a` (c + sigma a) on the walls
+template <int DIM>
+__device__ __forceinline__ double residual_cell(const double* __restrict__ a, const double* __restrict__ b,
+                                                const Coef<double>& k, const Geo<DIM>& g, const BCA& bc,
+                                                idx_t idx, const int c[3], double D) {
+  const double a0 = a[idx];
+  const double lam = k.lam ? k.lam[idx] : k.lam_c;
+  double r = b[idx] - lam * a0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    bool gp, gm;
+    const idx_t ip = nbr_index<DIM>(g, bc.periodic, idx, c, d, +1, gp);
+    const idx_t im = nbr_index<DIM>(g, bc.periodic, idx, c, d, -1, gm);
+    const double vp = a[ip], vm = a[im];
+    double cp = 0., cm = 0.;
+    if (gp && bc.c[2 * d]) cp = bc.c[2 * d][g.wall(d, c)];
+    if (gm && bc.c[2 * d + 1]) cm = bc.c[2 * d + 1][g.wall(d, c)];
+    const double ap = gp ? cp + double(bc.sigma[2 * d]) * vp : vp;
+    const double am = gm ? cm + double(bc.sigma[2 * d + 1]) * vm : vm;
+    double ah, al;
+    if (k.lo[0]) {
+      ah = alpha_hi<DIM, double>(k.lo[d], k.wall[d], g, idx, c, d);
+      al = k.lo[d][idx];
+    } else {
+      ah = al = k.alpha;
+    }
+    if (bc.pow2) r += (al * ((a0 - am) * bc.invD) - ah * ((ap - a0) * bc.invD)) * bc.invD;
+    else r += (al * ((a0 - am) / D) - ah * ((ap - a0) / D)) / D;
+  }
+  return r;
+}
+
+// residual of `a` on the finest level, writes res in the cycle precision TO
 template <int DIM, typename TO>
 __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ a, const double* __restrict__ b,
                                                      TO* __restrict__ res, double* __restrict__ res64,
                                                      Coef<double> k, Geo<DIM> g, BCA bc, double D, Stat* st) {
-  size_t idx = logical_block() * kBlock + threadIdx.x;
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
   double m = 0.;
   if (idx < g.ncell()) {
     int c[3];
     g.decode(idx, c);
-    double a0 = a[idx];
-    double lam = k.lam ? k.lam[idx] : k.lam_c;
-    double r = b[idx] - lam * a0;
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      size_t s = g.stride(d);
-      double ap, am;
-      if (c[d] < g.n - 1) ap = a[idx + s];
-      else if (bc.periodic[d]) ap = a[idx - (size_t)(g.n - 1) * s];
-      else {
-        const double* cv = bc.c[2 * d];
-        ap = (cv ? cv[g.wall(d, c)] : 0.) + double(bc.sigma[2 * d]) * a0;
-      }
-      if (c[d] > 0) am = a[idx - s];
-      else if (bc.periodic[d]) am = a[idx + (size_t)(g.n - 1) * s];
-      else {
-        const double* cv = bc.c[2 * d + 1];
-        am = (cv ? cv[g.wall(d, c)] : 0.) + double(bc.sigma[2 * d + 1]) * a0;
-      }
-      double ah, al;
-      if (k.lo[0]) {
-        ah = alpha_hi<DIM, double>(k.lo[d], k.wall[d], g, idx, c, d);
-        al = k.lo[d][idx];
-      } else {
-        ah = al = k.alpha;
-      }
-      r += (al * ((a0 - am) / D) - ah * ((ap - a0) / D)) / D;
-    }
+    double r = residual_cell<DIM>(a, b, k, g, bc, idx, c, D);
     res[idx] = (TO)r;
     if (res64) res64[idx] = r;
     if (r != r) atomicAdd(&st->nans, 1ull);
@@ -420,7 +509,7 @@ __global__ void k_sum_final(const double* __restrict__ part, int np, double* __r
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_restrict_face_lo(const double* __restrict__ fine, double* __restrict__ coarse,
                                                              Geo<DIM> gc, int d) {
-  size_t idx = logical_block() * kBlock + threadIdx.x;
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
   if (idx >= gc.ncell()) return;
   int c[3];
   gc.decode(idx, c);
@@ -486,6 +575,434 @@ __global__ __launch_bounds__(kBlock) void k_wall_from_first(const double* __rest
   }
   c[d] = 0;
   wall[idx] = lo[g.index(c)];
+}
+
+// --- residual fused with the first restriction ---------------------------
+//
+// One block covers a brick of 2 x 2 x 64 cells (3D) or 2 x 128 (2D), i.e.
+// whole groups of 2^DIM children: it computes their residuals, writes them,
+// and writes the 32 (3D) or 64 (2D) coarse residuals that restriction_average
+// makes of them, summed in foreach_child order.  The per-block max |res| and
+// NaN count go to `part`; k_reduce_stat folds them (no contended atomics).
+
+struct Part {
+  double max;
+  unsigned long long nans;
+};
+
+template <int DIM, typename TO>
+__global__ __launch_bounds__(kBlock) void k_residual_restrict(const double* __restrict__ a, const double* __restrict__ b,
+                                                              TO* __restrict__ res, TO* __restrict__ coarse,
+                                                              double* __restrict__ res64, Coef<double> k, Geo<DIM> g,
+                                                              BCA bc, double D, Part* __restrict__ part) {
+  __shared__ TO sh[kBlock];
+  __shared__ double smax[kBlock / 64];
+  __shared__ unsigned long long snan[kBlock / 64];
+  const int t = threadIdx.x;
+  const idx_t nbricks = g.ncell() / kBlock;
+  double macc = 0.;
+  unsigned long long nacc = 0;
+  // grid-stride over the bricks (the grid is capped, one partial per block)
+  for (idx_t bid = logical_block(); bid < nbricks; bid += gridDim.x) {
+    int c[3];
+    if (DIM == 3) {
+      int nkb = g.n >> 6, nj2 = g.n >> 1;
+      int bk = (int)(bid % nkb), bj = (int)((bid / nkb) % nj2), bi = (int)(bid / ((idx_t)nkb * nj2));
+      c[0] = 2 * bi + (t >> 7);
+      c[1] = 2 * bj + ((t >> 6) & 1);
+      c[2] = 64 * bk + (t & 63);
+    } else {
+      int nkb = g.n >> 7;
+      int bk = (int)(bid % nkb), bi = (int)(bid / nkb);
+      c[0] = 2 * bi + (t >> 7);
+      c[1] = 128 * bk + (t & 127);
+      c[2] = 0;
+    }
+    idx_t idx = g.index(c);
+    double r = residual_cell<DIM>(a, b, k, g, bc, idx, c, D);
+    TO ro = (TO)r;
+    res[idx] = ro;
+    if (res64) res64[idx] = r;
+    if (r != r) nacc++;
+    else macc = fmax(macc, fabs(r));
+    if (coarse) {
+      sh[t] = ro;
+      __syncthreads();
+      Geo<DIM> gc{g.n >> 1, g.log2n - 1};
+      if (DIM == 3 && t < 32) {
+        // children (a,b,c) of coarse cell t: sh[a*128 + b*64 + 2t + c]
+        TO sum = TO(0);
+        sum += sh[2 * t];
+        sum += sh[2 * t + 1];
+        sum += sh[64 + 2 * t];
+        sum += sh[64 + 2 * t + 1];
+        sum += sh[128 + 2 * t];
+        sum += sh[128 + 2 * t + 1];
+        sum += sh[192 + 2 * t];
+        sum += sh[192 + 2 * t + 1];
+        int cc[3] = {c[0] >> 1, c[1] >> 1, (c[2] & ~63) / 2 + t};
+        coarse[gc.index(cc)] = sum / TO(8);
+      } else if (DIM == 2 && t < 64) {
+        TO sum = TO(0);
+        sum += sh[2 * t];
+        sum += sh[2 * t + 1];
+        sum += sh[128 + 2 * t];
+        sum += sh[128 + 2 * t + 1];
+        int cc[3] = {c[0] >> 1, (c[1] & ~127) / 2 + t, 0};
+        coarse[gc.index(cc)] = sum / TO(4);
+      }
+      __syncthreads();
+    }
+  }
+  macc = wave_max(macc);
+  for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o, 64);
+  int lane = t & 63, w = t >> 6;
+  if (lane == 0) {
+    smax[w] = macc;
+    snan[w] = nacc;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double mm = smax[0];
+    unsigned long long ns = snan[0];
+    for (int i = 1; i < kBlock / 64; i++) {
+      mm = fmax(mm, smax[i]);
+      ns += snan[i];
+    }
+    part[blockIdx.x] = Part{mm, ns};
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_stat(const Part* __restrict__ part, int np, Stat* st) {
+  __shared__ double smax[16];
+  __shared__ unsigned long long snan[16];
+  double m = 0.;
+  unsigned long long ns = 0;
+  for (int i = threadIdx.x; i < np; i += 1024) {
+    m = fmax(m, part[i].max);
+    ns += part[i].nans;
+  }
+  m = wave_max(m);
+  for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o, 64);
+  if ((threadIdx.x & 63) == 0) {
+    smax[threadIdx.x >> 6] = m;
+    snan[threadIdx.x >> 6] = ns;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 16; i++) {
+      m = fmax(smax[0], smax[i]);
+      smax[0] = m;
+      snan[0] += snan[i];
+    }
+    st->maxbits = (unsigned long long)__double_as_longlong(smax[0]);
+    st->nans = snan[0];
+  }
+}
+
+// --- streaming red-black sweep, 3D ---------------------------------------
+//
+// One full red-black Gauss-Seidel sweep in a single pass, reading `dold` and
+// writing `dnew` (the two buffers never alias).  A block owns a tile of
+// kTJ x kTK cells in (j,k) and a run of `chunk` planes along i, which it
+// walks plane by plane keeping four planes of the field in LDS (a 2-cell
+// halo in j and k):
+//   - red cells of plane s are relaxed on the tile plus a 1-cell ring, from
+//     old black values only;
+//   - black cells of plane s-1 are then relaxed on the tile from the new red
+//     values of planes s-2, s-1 and s;
+//   - plane s-1 of the tile is written out.
+// Red cells on the ring and on the planes just outside the chunk are
+// recomputed from `dold` instead of being read from the block that owns
+// them, so blocks never exchange data and the values are exactly those of
+// the two-colour sweep (red first, ghost fill, black).
+//
+// The field plane s+2 is loaded one step ahead into registers; the
+// coefficients of a step are loaded before its first barrier.
+
+constexpr int kTJ = 8, kTK = 64, kRJ = kTJ + 4, kRK = kTK + 4;
+constexpr int kRJ1 = kTJ + 2, kRK1 = kTK + 2;
+constexpr int kNLoad = (kRJ * kRK + kBlock - 1) / kBlock;    // field values per thread
+constexpr int kNRed = (kRJ1 * kRK1 + kBlock - 1) / kBlock;   // red positions per thread
+constexpr int kNBlack = (kTJ * kTK) / kBlock;                // black positions per thread
+
+// coefficients of one cell: res, lambda and the six face alphas
+template <typename T>
+struct CellCoef {
+  T res, lam, al[3], ah[3];
+};
+
+template <typename T, bool ALPHAF, bool LAMF>
+__device__ __forceinline__ void fetch_coef(CellCoef<T>& cf, const T* __restrict__ res, const Coef<T>& k,
+                                           const Geo<3>& g, const int c[3]) {
+  size_t idx = ((size_t)c[0] * g.n + c[1]) * g.n + c[2];
+  cf.res = res[idx];
+  cf.lam = LAMF ? k.lam[idx] : k.lam_c;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (ALPHAF) {
+      cf.al[d] = k.lo[d][idx];
+      cf.ah[d] = alpha_hi<3, T>(k.lo[d], k.wall[d], g, idx, c, d);
+    } else {
+      cf.al[d] = cf.ah[d] = k.alpha;
+    }
+  }
+}
+
+template <typename T, bool ALPHAF, bool LAMF>
+__global__ __launch_bounds__(kBlock) void k_rb_stream3(const T* __restrict__ dold, T* __restrict__ dnew,
+                                                       const T* __restrict__ res, Coef<T> k, Geo<3> g, BCH bc,
+                                                       T D2, int chunk) {
+  __shared__ T buf[4][kRJ][kRK];
+  const int n = g.n, tid = threadIdx.x;
+  size_t b = logical_block();
+  const int nkt = n / kTK, njt = n / kTJ;
+  const int kt = (int)(b % nkt), jt = (int)((b / nkt) % njt), ct = (int)(b / ((size_t)nkt * njt));
+  const int i0 = ct * chunk, j0 = jt * kTJ, k0 = kt * kTK;
+  const int i1 = i0 + chunk;
+
+  auto inrange = [&](int d, int v) { return bc.periodic[d] || (v >= 0 && v < n); };
+  auto wrap = [&](int v) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+  auto slot = [&](int s) { return (s - i0 + 2) & 3; };
+
+  // field plane s over the halo region: registers <- global
+  auto fetch_plane = [&](int s, T* v) {
+#pragma unroll
+    for (int m = 0; m < kNLoad; m++) {
+      int q = tid + m * kBlock;
+      v[m] = T(0);
+      if (q < kRJ * kRK && inrange(0, s)) {
+        int pj = q / kRK, pk = q - pj * kRK;
+        int gj = j0 + pj - 2, gk = k0 + pk - 2;
+        if (inrange(1, gj) && inrange(2, gk)) v[m] = dold[((size_t)wrap(s) * n + wrap(gj)) * n + wrap(gk)];
+      }
+    }
+  };
+  auto commit_plane = [&](int s, const T* v) {
+    if (!inrange(0, s)) return;
+    T(*P)[kRK] = buf[slot(s)];
+#pragma unroll
+    for (int m = 0; m < kNLoad; m++) {
+      int q = tid + m * kBlock;
+      if (q < kRJ * kRK) {
+        int pj = q / kRK, pk = q - pj * kRK;
+        P[pj][pk] = v[m];
+      }
+    }
+  };
+  // position m of this thread in the red region (ring included) / the tile
+  auto red_pos = [&](int m, int& gj, int& gk) {
+    int q = tid + m * kBlock;
+    if (q >= kRJ1 * kRK1) return false;
+    int pj = q / kRK1, pk = q - pj * kRK1;
+    gj = j0 + pj - 1;
+    gk = k0 + pk - 1;
+    return inrange(1, gj) && inrange(2, gk);
+  };
+  auto black_pos = [&](int m, int& gj, int& gk) {
+    int q = tid + m * kBlock;
+    int pj = q / kTK, pk = q - pj * kTK;
+    gj = j0 + pj;
+    gk = k0 + pk;
+  };
+  auto fetch_red = [&](int s, CellCoef<T>* cf) {
+#pragma unroll
+    for (int m = 0; m < kNRed; m++) {
+      int gj, gk;
+      if (inrange(0, s) && red_pos(m, gj, gk) && !((s + gj + gk) & 1)) {
+        int c[3] = {wrap(s), wrap(gj), wrap(gk)};
+        fetch_coef<T, ALPHAF, LAMF>(cf[m], res, k, g, c);
+      }
+    }
+  };
+  auto fetch_black = [&](int s, CellCoef<T>* cf) {
+#pragma unroll
+    for (int m = 0; m < kNBlack; m++) {
+      int gj, gk;
+      black_pos(m, gj, gk);
+      if (s >= i0 && s < i1 && ((s + gj + gk) & 1)) {
+        int c[3] = {s, gj, gk};
+        fetch_coef<T, ALPHAF, LAMF>(cf[m], res, k, g, c);
+      }
+    }
+  };
+  // n/d at unwrapped position (s, gj, gk), neighbours from the LDS planes
+  auto relax_at = [&](int s, int gj, int gk, const CellCoef<T>& cf) -> T {
+    const int pj = gj - j0 + 2, pk = gk - k0 + 2;
+    const int sl = slot(s);
+    const T self = buf[sl][pj][pk];
+    T nn = -D2 * cf.res, dd = -cf.lam * D2;
+    T ap[3], am[3];
+    ap[0] = inrange(0, s + 1) ? buf[slot(s + 1)][pj][pk] : T(bc.sigma[0]) * self;
+    am[0] = inrange(0, s - 1) ? buf[slot(s - 1)][pj][pk] : T(bc.sigma[1]) * self;
+    ap[1] = inrange(1, gj + 1) ? buf[sl][pj + 1][pk] : T(bc.sigma[2]) * self;
+    am[1] = inrange(1, gj - 1) ? buf[sl][pj - 1][pk] : T(bc.sigma[3]) * self;
+    ap[2] = inrange(2, gk + 1) ? buf[sl][pj][pk + 1] : T(bc.sigma[4]) * self;
+    am[2] = inrange(2, gk - 1) ? buf[sl][pj][pk - 1] : T(bc.sigma[5]) * self;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      nn += cf.ah[d] * ap[d] + cf.al[d] * am[d];
+      dd += cf.ah[d] + cf.al[d];
+    }
+    return nn / dd;
+  };
+
+  T pre_da[kNLoad];
+  {
+    T tmp[kNLoad];
+    fetch_plane(i0 - 2, tmp);
+    commit_plane(i0 - 2, tmp);
+    fetch_plane(i0 - 1, tmp);
+    commit_plane(i0 - 1, tmp);
+  }
+  fetch_plane(i0, pre_da);
+  for (int s = i0 - 1; s <= i1; s++) {
+    commit_plane(s + 1, pre_da);
+    // loads: the field plane for the next step, the coefficients for this one
+    if (s + 2 <= i1 + 1) fetch_plane(s + 2, pre_da);
+    CellCoef<T> cur_red[kNRed], cur_blk[kNBlack];
+    fetch_red(s, cur_red);
+    fetch_black(s - 1, cur_blk);
+    __syncthreads();
+    // red on plane s over the tile and its 1-cell ring
+    if (inrange(0, s)) {
+#pragma unroll
+      for (int m = 0; m < kNRed; m++) {
+        int gj, gk;
+        if (red_pos(m, gj, gk) && !((s + gj + gk) & 1))
+          buf[slot(s)][gj - j0 + 2][gk - k0 + 2] = relax_at(s, gj, gk, cur_red[m]);
+      }
+    }
+    __syncthreads();
+    // black on plane s-1 over the tile, then write the plane out
+    const int q1 = s - 1;
+    if (q1 >= i0 && q1 < i1) {
+#pragma unroll
+      for (int m = 0; m < kNBlack; m++) {
+        int gj, gk;
+        black_pos(m, gj, gk);
+        T v;
+        if ((q1 + gj + gk) & 1) {
+          v = relax_at(q1, gj, gk, cur_blk[m]);
+          buf[slot(q1)][gj - j0 + 2][gk - k0 + 2] = v;
+        } else {
+          v = buf[slot(q1)][gj - j0 + 2][gk - k0 + 2];
+        }
+        dnew[((size_t)q1 * n + gj) * n + gk] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// --- all coarse levels in one workgroup ----------------------------------
+//
+// Levels minlevel..top of the cycle (each at most 4096 cells) run in one
+// block of 1024 threads: restriction of res from `top` down to minlevel,
+// then from minlevel up to top the zero / bilinear initial guess and nrelax
+// sweeps, with a block barrier between steps.  Same arithmetic as the
+// per-level kernels.
+
+// levels 0..12 at most (4096 cells in 1D)
+constexpr int kCoarseLevels = 13;
+
+template <typename T, int DIM>
+struct CoarseLevel {
+  T* da;
+  T* da2;  // Jacobi output
+  T* res;
+  Coef<T> k;
+  Geo<DIM> g;
+  BCH bc;
+  T D2;
+};
+
+template <typename T, int DIM>
+struct CoarseArgs {
+  CoarseLevel<T, DIM> L[kCoarseLevels];
+  int minlevel, top, nrelax, jacobi;
+};
+
+template <int DIM, typename T>
+__device__ void coarse_relax_cell(const CoarseLevel<T, DIM>& L, idx_t idx, int colour_or_jacobi) {
+  int c[3];
+  L.g.decode(idx, c);
+  T self = L.da[idx];
+  T v = relax_value<DIM, T>(L.da, L.res, L.k, L.g, L.bc, idx, c, self, L.D2);
+  if (colour_or_jacobi < 0) L.da2[idx] = (self + T(2) * v) / T(3);
+  else L.da[idx] = v;
+}
+
+template <int DIM, typename T>
+__global__ __launch_bounds__(1024) void k_coarse_cycle(CoarseArgs<T, DIM> A) {
+  const int t = threadIdx.x;
+  for (int l = A.top - 1; l >= A.minlevel; l--) {
+    const CoarseLevel<T, DIM>& C = A.L[l];
+    const CoarseLevel<T, DIM>& F = A.L[l + 1];
+    idx_t nc = C.g.ncell();
+    for (idx_t idx = t; idx < nc; idx += 1024) {
+      int c[3];
+      C.g.decode(idx, c);
+      int f[3] = {2 * c[0], 2 * c[1], 2 * c[2]};
+      idx_t b = F.g.index(f);
+      T sum = T(0);
+      if (DIM == 1) {
+        sum += F.res[b];
+        sum += F.res[b + 1];
+      } else if (DIM == 2) {
+        idx_t sx = F.g.stride(0);
+        sum += F.res[b];
+        sum += F.res[b + 1];
+        sum += F.res[b + sx];
+        sum += F.res[b + sx + 1];
+      } else {
+        idx_t sx = F.g.stride(0), sy = F.g.stride(1);
+        sum += F.res[b];
+        sum += F.res[b + 1];
+        sum += F.res[b + sy];
+        sum += F.res[b + sy + 1];
+        sum += F.res[b + sx];
+        sum += F.res[b + sx + 1];
+        sum += F.res[b + sx + sy];
+        sum += F.res[b + sx + sy + 1];
+      }
+      C.res[idx] = sum / T(1 << DIM);
+    }
+    __syncthreads();
+  }
+  for (int l = A.minlevel; l <= A.top; l++) {
+    const CoarseLevel<T, DIM>& L = A.L[l];
+    idx_t nl = L.g.ncell();
+    if (l == A.minlevel) {
+      for (idx_t idx = t; idx < nl; idx += 1024) L.da[idx] = T(0);
+    } else {
+      const CoarseLevel<T, DIM>& C = A.L[l - 1];
+      for (idx_t idx = t; idx < nl; idx += 1024) {
+        int c[3];
+        L.g.decode(idx, c);
+        T v = bilinear_at<DIM, T>(C.da, C.g, C.bc, c);
+        L.da[idx] = v;
+      }
+    }
+    __syncthreads();
+    for (int it = 0; it < A.nrelax; it++) {
+      if (A.jacobi) {
+        for (idx_t idx = t; idx < nl; idx += 1024) coarse_relax_cell<DIM, T>(L, idx, -1);
+        __syncthreads();
+        for (idx_t idx = t; idx < nl; idx += 1024) L.da[idx] = L.da2[idx];
+        __syncthreads();
+      } else {
+        for (int colour = 0; colour < 2; colour++) {
+          for (idx_t idx = t; idx < nl; idx += 1024) {
+            int c[3];
+            L.g.decode(idx, c);
+            if (((c[0] + c[1] + c[2]) & 1) == colour) coarse_relax_cell<DIM, T>(L, idx, colour);
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
 }
 
 }  // namespace bmg

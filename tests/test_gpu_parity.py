@@ -134,6 +134,35 @@ def test_random_problem_cycles_bitexact(bcs, dim, smoother):
     assert np.array_equal(g.get_a(), o.get_a())
 
 
+@pytest.mark.parametrize("kernel", ["default", "stream", "fused"])
+@pytest.mark.parametrize("bcs", ["mixed", "periodic_all", "periodic_x"])
+@pytest.mark.parametrize("depth", [7, 8])
+def test_large_levels_bitexact(kernel, bcs, depth, monkeypatch):
+    """Every red-black kernel of the large levels -- the two-launch in-place
+    one (default), the single-pass streaming one (plane ring in LDS, halo
+    recomputation) and the one-launch fused one (red neighbours recomputed
+    on the fly) -- and the fused residual/restriction kernel give the
+    oracle's two-colour iterates bit for bit, periodic wrap included."""
+    if kernel != "default":
+        monkeypatch.setenv("BMG_RB_KERNEL", kernel)
+    if kernel == "fused" and depth == 8:
+        pytest.skip("covered at 128^3")
+    print("case large levels", kernel, bcs, depth, flush=True)
+    g, o = pair(3, depth, "rbgs")
+    kinds = BC_MIXES[bcs]
+    lam = np.random.default_rng(8).uniform(-2.0, -0.5, (1 << depth,) * 3)
+    for grid in (g, o):
+        cases.random_problem(grid, 77, lam=lam, bcs=kinds)
+        grid.restriction_alpha_lambda()
+    assert g.residual() == o.residual()
+    for it in range(2 if depth == 8 else 3):
+        g.mg_cycle(3 - it % 2, 0)
+        o.mg_cycle(3 - it % 2, 0)
+        assert g.residual() == o.residual(), it
+    assert np.array_equal(g.get_a(), o.get_a())
+    assert np.array_equal(g.get_res(), o.get_res())
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_mg_solve_stats_bitexact(dim):
     """poisson() with a variable alpha and a lambda field: the iteration
