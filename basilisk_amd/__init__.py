@@ -33,7 +33,7 @@ PROFILE_KINDS = {"relax_fine": 0, "residual": 1, "restriction": 2,
 # every symbol declared in include/bmg.h
 EXPORTS = [
     "bmg_version", "bmg_last_error", "bmg_device_count", "bmg_grid_new",
-    "bmg_grid_free", "bmg_grid_info", "bmg_set_precision", "bmg_set_smoother",
+    "bmg_grid_free", "bmg_grid_info", "bmg_set_grid_kind", "bmg_set_precision", "bmg_set_smoother",
     "bmg_set_tolerance", "bmg_set_niter", "bmg_set_stream", "bmg_set_bc",
     "bmg_set_bc_affine", "bmg_set_field", "bmg_get_field",
     "bmg_set_alpha_const", "bmg_set_lambda_const",
@@ -80,6 +80,7 @@ def library():
         "bmg_grid_new": (I, [ctypes.POINTER(P), I, I, D, D, D, D]),
         "bmg_grid_free": (I, [P]),
         "bmg_grid_info": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "bmg_set_grid_kind": (I, [P, I]),
         "bmg_set_precision": (I, [P, I]),
         "bmg_set_smoother": (I, [P, I]),
         "bmg_set_tolerance": (I, [P, D]),
@@ -129,7 +130,7 @@ class Grid:
     ``smoother``: "rbgs" (default) or "jacobi"."""
 
     def __init__(self, dim, depth, L0=1.0, origin=(0.0, 0.0, 0.0),
-                 precision="fp64", smoother="rbgs"):
+                 precision="fp64", smoother="rbgs", kind="multigrid"):
         lib = library()
         self.dim, self.depth, self.L0 = dim, depth, float(L0)
         self.origin = tuple(float(o) for o in origin) + (0.0,) * (3 - len(origin))
@@ -138,6 +139,13 @@ class Grid:
         self.lib = lib
         self.set_smoother(smoother)
         self.set_precision(precision)
+        self.set_kind(kind)
+
+    def set_kind(self, kind):
+        """"multigrid" or "tree" (a full quadtree/octree): the ghost rule of
+        edge and corner cells."""
+        _check(self.lib.bmg_set_grid_kind(self._g, {"multigrid": 0, "tree": 1}[kind]))
+        self.kind = kind
 
     def close(self):
         if getattr(self, "_g", None):

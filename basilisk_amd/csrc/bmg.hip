@@ -99,6 +99,7 @@ struct bmg_grid {
   // profiling
   bool prof = false;
   int rb_override = -1;  // BMG_RB_KERNEL: kernel for red-black levels of >= 4096 cells
+  int grid_kind = BMG_GRID_MULTIGRID;  // ghost rule of edges and corners
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
   double prof_ms[PROF_N] = {0};
@@ -232,6 +233,7 @@ BCH level_bc(const bmg_grid* g, int l) {
     anyp |= bc.periodic[d] != 0;
   }
   for (int b = 0; b < 6; b++) bc.sigma[b] = g->bc_sigma[b];
+  bc.tree = g->grid_kind == BMG_GRID_TREE;
   // on level 0 the periodic copy sets every ghost to the root value
   // (grid/multigrid.h:655-664)
   if (l == 0 && anyp)
@@ -854,6 +856,15 @@ int bmg_set_precision(bmg_grid* g, int precision) {
   // the cycle fields are scratch between cycles; a and b are kept
   g->precision = precision;
   TRY(alloc_levels(g));
+  return BMG_OK;
+}
+
+int bmg_set_grid_kind(bmg_grid* g, int kind) {
+  TRY(check_grid(g));
+  if (kind != BMG_GRID_MULTIGRID && kind != BMG_GRID_TREE) return fail(BMG_ERR_ARG, "unknown grid kind %d", kind);
+  if (kind == BMG_GRID_TREE && (g->periodic[0] || g->periodic[1] || g->periodic[2]))
+    return fail(BMG_ERR_UNSUPPORTED, "periodic trees are not supported");
+  g->grid_kind = kind;
   return BMG_OK;
 }
 

@@ -27,6 +27,7 @@ constexpr int kBlock = 256;
 struct BCH {
   int periodic[3];
   int sigma[6];  // right, left, top, bottom, front, back
+  int tree;      // edge/corner ghosts by the tree rule (see bilinear_at)
 };
 
 // inhomogeneous boundary of `a` on the finest level: ghost = c + sigma * s,
@@ -310,15 +311,33 @@ __device__ __forceinline__ T coarse_at(const T* __restrict__ f, const Geo<DIM>& 
 // mapped like a ghost cell (wrapped, or the parent itself with sign sigma)
 template <int DIM, typename T>
 __device__ __forceinline__ void axis_pair(const Geo<DIM>& gc, const BCH& bc, int d, int p, int o, idx_t& off0,
-                                          idx_t& off1, T& sg) {
+                                          idx_t& off1, T& sg, bool& gh) {
   const idx_t s = gc.stride(d);
   const int q = p + o;
   const bool out = q < 0 || q >= gc.n;
   const int qw = q < 0 ? q + gc.n : q - gc.n;
   const int qm = out ? (bc.periodic[d] ? qw : p) : q;
-  sg = (out && !bc.periodic[d]) ? T(bc.sigma[2 * d + (o < 0)]) : T(1);
+  gh = out && !bc.periodic[d];
+  sg = gh ? T(bc.sigma[2 * d + (o < 0)]) : T(1);
   off0 = (idx_t)p * s;
   off1 = (idx_t)qm * s;
+}
+
+// a ghost cell outside along two or three dimensions, from the value v of
+// the interior cell diagonal to it.  Multigrid (grid/multigrid.h:543-594,
+// one reflection per direction): the product of the sigmas times v.  Tree
+// (grid/tree.h:1168-1258): b1 + b2 - v, or b1 + b2 + b3 - 2 v, with
+// b = sigma v per crossed side (sums of +-v, so exact in any order).
+template <typename T>
+__device__ __forceinline__ T corner2(T v, T sa, T sb, bool tree) {
+  return tree ? (sb * v + sa * v) - v : (sa * sb) * v;
+}
+template <typename T>
+__device__ __forceinline__ T corner3(T v, T sx, T sy, T sz, bool gx, bool gy, bool gz, bool tree) {
+  int k = (int)gx + (int)gy + (int)gz;
+  if (!tree || k < 2) return ((sx * sy) * sz) * v;
+  if (k == 2) return ((gz ? sz * v : T(0)) + (gy ? sy * v : T(0)) + (gx ? sx * v : T(0))) - v;
+  return ((sz * v + sy * v) + sx * v) - T(2) * v;
 }
 
 // bilinear value at fine cell c from the coarse level,
@@ -328,26 +347,28 @@ __device__ __forceinline__ T bilinear_at(const T* __restrict__ coarse, const Geo
                                          const int c[3]) {
   idx_t x0, x1, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
   T sx, sy = T(1), sz = T(1);
-  axis_pair<DIM, T>(gc, bcc, 0, c[0] >> 1, 2 * (c[0] & 1) - 1, x0, x1, sx);
-  if (DIM >= 2) axis_pair<DIM, T>(gc, bcc, 1, c[1] >> 1, 2 * (c[1] & 1) - 1, y0, y1, sy);
-  if (DIM >= 3) axis_pair<DIM, T>(gc, bcc, 2, c[2] >> 1, 2 * (c[2] & 1) - 1, z0, z1, sz);
+  bool gx, gy = false, gz = false;
+  const bool tree = bcc.tree != 0;
+  axis_pair<DIM, T>(gc, bcc, 0, c[0] >> 1, 2 * (c[0] & 1) - 1, x0, x1, sx, gx);
+  if (DIM >= 2) axis_pair<DIM, T>(gc, bcc, 1, c[1] >> 1, 2 * (c[1] & 1) - 1, y0, y1, sy, gy);
+  if (DIM >= 3) axis_pair<DIM, T>(gc, bcc, 2, c[2] >> 1, 2 * (c[2] & 1) - 1, z0, z1, sz, gz);
   if (DIM == 1) {
     return (T(3) * coarse[x0] + sx * coarse[x1]) / T(4);
   } else if (DIM == 2) {
     T c00 = coarse[x0 + y0];
     T c10 = sx * coarse[x1 + y0];
     T c01 = sy * coarse[x0 + y1];
-    T c11 = (sx * sy) * coarse[x1 + y1];
+    T c11 = (gx && gy) ? corner2(coarse[x1 + y1], sx, sy, tree) : (sx * sy) * coarse[x1 + y1];
     return (T(9) * c00 + T(3) * (c10 + c01) + c11) / T(16);
   } else {
     T c000 = coarse[x0 + y0 + z0];
     T c100 = sx * coarse[x1 + y0 + z0];
     T c010 = sy * coarse[x0 + y1 + z0];
     T c001 = sz * coarse[x0 + y0 + z1];
-    T c110 = (sx * sy) * coarse[x1 + y1 + z0];
-    T c101 = (sx * sz) * coarse[x1 + y0 + z1];
-    T c011 = (sy * sz) * coarse[x0 + y1 + z1];
-    T c111 = ((sx * sy) * sz) * coarse[x1 + y1 + z1];
+    T c110 = (gx && gy) ? corner2(coarse[x1 + y1 + z0], sx, sy, tree) : (sx * sy) * coarse[x1 + y1 + z0];
+    T c101 = (gx && gz) ? corner2(coarse[x1 + y0 + z1], sx, sz, tree) : (sx * sz) * coarse[x1 + y0 + z1];
+    T c011 = (gy && gz) ? corner2(coarse[x0 + y1 + z1], sy, sz, tree) : (sy * sz) * coarse[x0 + y1 + z1];
+    T c111 = corner3(coarse[x1 + y1 + z1], sx, sy, sz, gx, gy, gz, tree);
     return (T(27) * c000 + T(9) * (c100 + c010 + c001) + T(3) * (c110 + c101 + c011) + c111) / T(64);
   }
 }

@@ -96,7 +96,14 @@ int bmg_grid_new(bmg_grid **g, int dimension, int depth, double L0,
 int bmg_grid_free(bmg_grid *g);
 int bmg_grid_info(const bmg_grid *g, int *dimension, int *depth);
 
+/* ghost rule of edge and corner cells: the multigrid grid reflects once per
+   direction (grid/multigrid.h:543-594); a full quadtree/octree extrapolates
+   from the diagonal cell (grid/tree.h:1168-1258).  Face ghosts and all
+   other arithmetic are the same. */
+enum { BMG_GRID_MULTIGRID = 0, BMG_GRID_TREE = 1 };
+
 /* solver configuration */
+int bmg_set_grid_kind(bmg_grid *g, int kind);
 int bmg_set_precision(bmg_grid *g, int precision);
 int bmg_set_smoother(bmg_grid *g, int smoother);
 int bmg_set_tolerance(bmg_grid *g, double tolerance);           /* TOLERANCE */

@@ -76,6 +76,7 @@ typedef struct {
   int alpha_field;
   double alpha_val;
   int smoother;
+  int tree; /* full quadtree/octree semantics: Morton GS, tree corner ghosts */
   double TOLERANCE;
   int NITERMAX, NITERMIN;
 } orc_grid;
@@ -167,6 +168,7 @@ void orc_set_niter(orc_grid *g, int nitermax, int nitermin) {
 
 void orc_set_tolerance(orc_grid *g, double tol) { g->TOLERANCE = tol; }
 void orc_set_smoother(orc_grid *g, int s) { g->smoother = s; }
+void orc_set_tree(orc_grid *g, int tree) { g->tree = tree; }
 void orc_set_lambda_const(orc_grid *g, double v) { g->lambda_field = 0; g->lambda_val = v; }
 void orc_use_lambda_field(orc_grid *g) { g->lambda_field = 1; }
 void orc_set_alpha_const(orc_grid *g, double v) { g->alpha_field = 0; g->alpha_val = v; }
@@ -242,6 +244,75 @@ void orc_get_face(orc_grid *g, int c, int l, double *dst) { copy_faces(g, c, l, 
 
 /* ------------------------------------------------------------ ghost fill */
 
+/* The value of the BC of direction b applied to the interior value s at
+   tangential position (u,v) of that side (common.h:957-960). */
+static double bc_apply(orc_grid *g, int b, int l, int homogeneous, int use_bc, double s, int u, int v) {
+  int kind = use_bc ? g->bc[b] : BC_SYMMETRY;
+  int m = msize(l);
+  const double *vals = (use_bc && !homogeneous && l == g->depth) ? g->bcv[b] : NULL;
+  double expr = vals ? vals[(size_t)u * (DIM == 3 ? m : 1) + v] : 0.;
+  switch (kind) {
+  case BC_DIRICHLET: return homogeneous ? -s : 2. * (expr) - s;
+  case BC_NEUMANN: return homogeneous ? s : delta_l(g, l) * (expr) + s;
+  default: return s;
+  }
+}
+
+/* Edge and corner ghosts of a full tree (grid/tree.h:1168-1258, used by
+   box_boundary_level 1286-1363): from the diagonal interior cell n,
+     2 outside dimensions:  b1(n) + b2(n) - s[n]
+     3 outside dimensions:  b1(n) + b2(n) + b3(n) - 2 s[n]
+   where b_i is the BC of the side crossed by the i-th face neighbour of the
+   ghost; in 2D b1 is the y side, in 3D the order is z, y, x.  (The
+   multigrid fill instead reflects the cell once per direction.) */
+static void tree_corners(orc_grid *g, int f, int l, int homogeneous, int use_bc) {
+  if (DIM < 2) return;
+  int n = 1 << l;
+  for (int d = 0; d < DIM; d++)
+    if (g->periodic[d]) return; /* periodic trees are not handled */
+  int lo = GHOSTS - 1, hi = n + GHOSTS;
+#if DIM == 2
+  for (int i = lo; i <= hi; i++)
+    for (int j = lo; j <= hi; j++) {
+      int k = 0;
+#else
+  for (int i = lo; i <= hi; i++)
+    for (int j = lo; j <= hi; j++)
+      for (int k = lo; k <= hi; k++) {
+#endif
+        int c[3] = {i, j, k}, mir[3] = {i, j, k}, out[3] = {0, 0, 0}, nout = 0;
+        for (int d = 0; d < DIM; d++) {
+          if (c[d] == lo) { out[d] = 1 + 1; mir[d] = GHOSTS; nout++; }         /* left / bottom / back */
+          else if (c[d] == hi) { out[d] = 1; mir[d] = n + GHOSTS - 1; nout++; } /* right / top / front */
+        }
+        if (nout < 2) continue;
+        double s = REC(g, l, mir[0], mir[1], mir[2])[f];
+        double bv[3] = {0., 0., 0.};
+        for (int d = 0; d < DIM; d++) {
+          if (!out[d]) continue;
+          int b = 2 * d + (out[d] == 2);
+          int t[2] = {0, 0}, q = 0;
+          for (int e = 0; e < DIM; e++)
+            if (e != d) t[q++] = mir[e];
+          bv[d] = bc_apply(g, b, l, homogeneous, use_bc, s, t[0], DIM == 3 ? t[1] : 0);
+        }
+        double r;
+#if DIM == 2
+        r = bv[1] + bv[0] - s;
+#else
+        if (nout == 2) {
+          int d1 = -1, d2 = -1;
+          for (int d = 0; d < 3; d++)
+            if (out[d]) { if (d1 < 0) d1 = d; else d2 = d; }
+          r = bv[d2] + bv[d1] - s;
+        } else {
+          r = bv[2] + bv[1] + bv[0] - 2. * s;
+        }
+#endif
+        REC(g, l, i, j, k)[f] = r;
+      }
+}
+
 /* box_boundary_level, multigrid.h:543-594, for one cell-centred field.
    mode 0: the field's inhomogeneous BC (only `a` has one, and only values
    at the finest level); mode 1: the homogeneous version. */
@@ -281,6 +352,7 @@ static void box_boundary(orc_grid *g, int f, int l, int homogeneous, int use_bc)
         gh[f] = r;
       }
   }
+  if (g->tree) tree_corners(g, f, l, homogeneous, use_bc);
 }
 
 /* periodic_boundary_level_x, multigrid.h:638-695: both ghost layers */
@@ -553,7 +625,19 @@ static inline double relax_value(const orc_grid *g, int l, const double *r, int 
 void orc_relax(orc_grid *g, int l) {
   int n = 1 << l;
   double D = delta_l(g, l), D2 = D * D;
-  if (g->smoother == SM_GS) {
+  if (g->smoother == SM_GS && g->tree) {
+    /* a full tree visits level l in the pre-order of foreach_cell
+       (grid/foreach_cell.h:66-123): Morton order, x the high bit */
+    size_t ncell = POWD(n);
+    for (size_t code = 0; code < ncell; code++) {
+      int c[3] = {0, 0, 0};
+      for (int b = 0; b < l; b++)
+        for (int d = 0; d < DIM; d++)
+          c[d] |= (int)((code >> (DIM * b + DIM - 1 - d)) & 1) << b;
+      double *r = REC(g, l, c[0] + GHOSTS, (DIM >= 2) * (c[1] + GHOSTS), (DIM >= 3) * (c[2] + GHOSTS));
+      r[F_DA] = relax_value(g, l, r, F_DA, D2);
+    }
+  } else if (g->smoother == SM_GS) {
     LOOP_LEVEL(l, n)
       double *r = REC(g, l, i, j, k);
       r[F_DA] = relax_value(g, l, r, F_DA, D2);

@@ -54,6 +54,7 @@ def _lib(dim, omp=False):
             "orc_set_niter": (None, [P, I, I]),
             "orc_set_tolerance": (None, [P, D]),
             "orc_set_smoother": (None, [P, I]),
+            "orc_set_tree": (None, [P, I]),
             "orc_set_lambda_const": (None, [P, D]),
             "orc_use_lambda_field": (None, [P]),
             "orc_set_alpha_const": (None, [P, D]),
@@ -223,6 +224,11 @@ class OracleGrid:
     # -- solver
     def set_smoother(self, name):
         self.lib.orc_set_smoother(self.g, SMOOTHERS[name])
+
+    def set_tree(self, tree=True):
+        """Full quadtree/octree semantics (grid/tree.h): Morton-order
+        Gauss-Seidel and the tree's edge/corner ghost rule."""
+        self.lib.orc_set_tree(self.g, int(tree))
 
     def set_niter(self, nitermax=100, nitermin=1):
         self.lib.orc_set_niter(self.g, nitermax, nitermin)

@@ -243,3 +243,20 @@ def test_lexicographic_gs_refused():
     with pytest.raises(bmg.BMGError) as e:
         g.set_smoother("gs")
     assert e.value.code == 3
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+@pytest.mark.parametrize("dim,depth", [(2, 9), (3, 6)])
+def test_full_tree_bitexact(smoother, dim, depth):
+    """test/poisson.c on a full quadtree / octree (Dirichlet, so the tree's
+    corner ghost rule enters the prolongation): GPU equals the tree-mode
+    oracle bit for bit."""
+    print("case full tree", smoother, dim, depth, flush=True)
+    g = bmg.Grid(dim, depth, origin=(-0.5, -0.5, -0.5), smoother=smoother, kind="tree")
+    o = OracleGrid(dim, depth, origin=(-0.5, -0.5, -0.5))
+    o.set_smoother(smoother)
+    o.set_tree()
+    out_g, err_g = cases.poisson_c(g)
+    out_o, err_o = cases.poisson_c(o)
+    assert [r for _, r in out_g] == [r for _, r in out_o]
+    assert np.array_equal(g.get_a(), o.get_a())

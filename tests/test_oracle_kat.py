@@ -65,3 +65,45 @@ def test_poisson_3d_ref_final_error():
     assert "# max error %g" % err == ref[-1]
     for (i, r), line in zip(out, ref[:-1]):
         assert abs(r / float(line.split()[1]) - 1) < 0.1
+
+
+def _tree(dim, depth, origin):
+    g = OracleGrid(dim, depth, origin=origin)
+    g.set_tree()
+    return g
+
+
+@pytest.mark.parametrize("dim,depth,ref", [(2, 9, "poisson.ref"), (3, 6, "poisson.3D.ref")])
+def test_poisson_ref_full_tree(dim, depth, ref):
+    """test/poisson.c on a full quadtree / octree: Morton-order GS and the
+    tree's corner ghosts reproduce every line of the reference log."""
+    g = _tree(dim, depth, (-0.5, -0.5, -0.5))
+    out, err = cases.poisson_c(g)
+    assert fmt_cycles(out) + ["# max error %g" % err] == cases.read_ref(ref)
+
+
+def test_variable_ref_full_tree():
+    g = _tree(2, 9, (-0.5, -0.5, 0.0))
+    out, err = cases.variable_c(g)
+    assert fmt_cycles(out) + ["# max error %g" % err] == cases.read_ref("variable.ref")
+
+
+def test_poisson_cadna_digits_full_tree():
+    """test/poisson.CADNA.ref prints, per cycle, only the digits that CADNA's
+    stochastic arithmetic certifies (9 to 12 significant digits for the
+    first cycles).  The quadtree oracle reproduces all of them for cycles
+    0-8; later cycles sit at the FP64 rounding floor, where CADNA itself
+    keeps one or two digits."""
+    g = _tree(2, 9, (-0.5, -0.5, -0.5))
+    out, _ = cases.poisson_c(g)
+    checked = 0
+    for (i, r), line in zip(out, cases.read_ref("poisson.CADNA.ref")):
+        if i > 8:
+            break
+        mant, exp = line.split()[1].split("E")
+        digits = len(mant.replace("0.", "", 1))
+        ref = float(line.split()[1])
+        # within one unit of the last printed digit (CADNA truncates)
+        assert abs(r - ref) < 10.0 ** (int(exp) - digits), (i, r, line)
+        checked += digits
+    assert checked == 70  # 12+10+9+8+7+7+6+7+4 certified digits
