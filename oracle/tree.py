@@ -1,0 +1,260 @@
+"""ctypes front end of the adaptive-tree CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+``bench.py`` may import this module.  It drives ``tree_oracle.c``, the
+plain-C restatement of the reference's multigrid solver on quadtrees and
+octrees (grid/tree.h, grid/tree-common.h, poisson.h); the header of that file
+cites the reference per function.
+
+Side numbering is the reference's: right=0, left=1, top=2, bottom=3,
+front=4, back=5 (common.h:950-951).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from oracle.oracle import MGStats, build, SMOOTHERS, BC_SYMMETRY, BC_DIRICHLET, BC_NEUMANN
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+GH = 2
+T_A, T_B, T_RES, T_DA, T_LAM, T_C, T_G = range(7)
+BK_A, BK_HOM, BK_SYM = range(3)
+RIGHT, LEFT, TOP, BOTTOM, FRONT, BACK = range(6)
+
+_libs = {}
+
+
+def _lib(dim):
+    if dim not in _libs:
+        path = os.path.join(_HERE, "liborc_tree%d.so" % dim)
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+        vp = ctypes.c_void_p
+        sigs = {
+            "ot_new": (P, [D, D, D, D]), "ot_free": (None, [P]),
+            "ot_depth": (I, [P]), "ot_set_niter": (None, [P, I, I]),
+            "ot_set_tolerance": (None, [P, D]), "ot_set_smoother": (None, [P, I]),
+            "ot_set_lambda_const": (None, [P, D]), "ot_set_alpha_const": (None, [P, D]),
+            "ot_use_lambda_field": (None, [P]),
+            "ot_update": (None, [P]), "ot_nleaves": (L, [P]),
+            "ot_leaves": (None, [P, vp, vp]), "ot_refine_marked": (I, [P, vp]),
+            "ot_cache_count": (I, [P, I, I]), "ot_cache_get": (None, [P, I, I, vp]),
+            "ot_face_count": (I, [P]), "ot_face_get": (None, [P, vp]),
+            "ot_level_meta": (None, [P, I, vp, vp, vp, vp]),
+            "ot_set_leaf": (None, [P, I, vp]), "ot_get_leaf": (None, [P, I, vp]),
+            "ot_set_faces": (None, [P, vp]), "ot_set_bc": (None, [P, I, I, vp]),
+            "ot_level_field": (ctypes.POINTER(ctypes.c_double), [P, I, I]),
+            "ot_restriction_alpha_lambda": (None, [P]),
+            "ot_relax": (None, [P, I]), "ot_residual": (D, [P]),
+            "ot_mg_cycle": (None, [P, I, I, I]), "ot_sum_b": (D, [P]),
+            "ot_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+            "ot_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+            "ot_boundary": (None, [P, I, I, I]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _libs[dim] = lib
+    return _libs[dim]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleTree:
+    """An adaptive tree built like init_grid(1) followed by refine(cond).
+
+    ``refine(cond)`` takes a vectorised condition ``cond(x, y, z, level) ->
+    bool array`` over the current leaves and repeats passes until no leaf
+    satisfies it (the refine() macro, grid/tree-common.h:308-326).
+    """
+
+    def __init__(self, dim, L0=1.0, origin=(0.0, 0.0, 0.0)):
+        self.dim = dim
+        self.lib = _lib(dim)
+        o = list(origin) + [0.0] * (3 - len(origin))
+        self.L0, self.origin = L0, o[:3]
+        self.t = self.lib.ot_new(L0, *o[:3])
+
+    def __del__(self):
+        if getattr(self, "t", None):
+            self.lib.ot_free(self.t)
+            self.t = None
+
+    # -- topology
+    def depth(self):
+        return self.lib.ot_depth(self.t)
+
+    def nleaves(self):
+        return self.lib.ot_nleaves(self.t)
+
+    def leaves(self):
+        n = self.nleaves()
+        lev = np.zeros(n, np.int32)
+        xyz = np.zeros((n, 3))
+        self.lib.ot_leaves(self.t, _ptr(lev), _ptr(xyz))
+        return lev, xyz
+
+    def refine(self, cond):
+        while True:
+            lev, xyz = self.leaves()
+            mark = np.ascontiguousarray(np.asarray(cond(xyz[:, 0], xyz[:, 1], xyz[:, 2], lev), dtype=np.uint8))
+            if not mark.any():
+                return
+            self.lib.ot_refine_marked(self.t, _ptr(mark))
+
+    def cache(self, which, level):
+        """which: 'active', 'prolongation' or 'boundary' -> (n, 3) int array."""
+        w = {"active": 0, "prolongation": 1, "boundary": 2}[which]
+        n = self.lib.ot_cache_count(self.t, w, level)
+        out = np.zeros((n, 3), np.int32)
+        if n:
+            self.lib.ot_cache_get(self.t, w, level, _ptr(out))
+        return out
+
+    def faces(self):
+        n = self.lib.ot_face_count(self.t)
+        out = np.zeros((n, 5), np.int32)
+        if n:
+            self.lib.ot_face_get(self.t, _ptr(out))
+        return out
+
+    def level_meta(self, level):
+        m = (1 << level) + 2 * GH
+        shape = (m,) * self.dim
+        alloc = np.zeros(shape, np.uint8)
+        flags = np.zeros(shape, np.uint8)
+        nb = np.zeros(shape, np.uint16)
+        pid = np.zeros(shape, np.int8)
+        self.lib.ot_level_meta(self.t, level, _ptr(alloc), _ptr(flags), _ptr(nb), _ptr(pid))
+        return alloc, flags, nb, pid
+
+    def face_centres(self):
+        """Coordinates of every face value of the face cache, in the order
+        ot_set_faces consumes them (entry order, then dimension); returns
+        (xyz (n,3), dim (n,))."""
+        f = self.faces()
+        xs, ds = [], []
+        for lev, i, j, k, flags in f:
+            D = self.L0 * (1.0 / (1 << int(lev)))
+            idx = (i, j, k)
+            for d in range(self.dim):
+                if flags & (1 << d):
+                    c = []
+                    for e in range(3):
+                        if e >= self.dim:
+                            c.append(0.0)
+                            continue
+                        g = -1 if e == d else 0
+                        c.append(((g / 2.0 + (idx[e] - GH)) + 0.5) * D + self.origin[e])
+                    xs.append(c)
+                    ds.append(d)
+        return np.array(xs).reshape(-1, 3), np.array(ds, np.int32)
+
+    # -- fields
+    def set_leaf(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.nleaves()
+        self.lib.ot_set_leaf(self.t, field, _ptr(v))
+
+    def get_leaf(self, field):
+        v = np.zeros(self.nleaves())
+        self.lib.ot_get_leaf(self.t, field, _ptr(v))
+        return v
+
+    def set_a(self, v):
+        self.set_leaf(T_A, v)
+
+    def set_b(self, v):
+        self.set_leaf(T_B, v)
+
+    def get_a(self):
+        return self.get_leaf(T_A)
+
+    def get_res(self):
+        return self.get_leaf(T_RES)
+
+    def set_alpha_faces(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self.lib.ot_set_faces(self.t, _ptr(v))
+
+    def set_alpha_function(self, f):
+        """alpha.x[] = f(x, y, z, d) on the face cache (a foreach_face loop)."""
+        xyz, ds = self.face_centres()
+        self.set_alpha_faces(f(xyz[:, 0], xyz[:, 1], xyz[:, 2], ds))
+
+    def set_lambda(self, v):
+        if np.isscalar(v):
+            self.lib.ot_set_lambda_const(self.t, float(v))
+        else:
+            self.set_leaf(T_LAM, v)
+            self.lib.ot_use_lambda_field(self.t)
+
+    def set_alpha_const(self, v):
+        self.lib.ot_set_alpha_const(self.t, float(v))
+
+    def bc_face_centres(self, side):
+        """Face centres of side `side` for every level 0..depth, concatenated
+        in the order ot_set_bc reads them."""
+        d = side // 2
+        hi = side % 2 == 0
+        out = []
+        for lev in range(self.depth() + 1):
+            n = 1 << lev
+            D = self.L0 * (1.0 / n)
+            tang = [e for e in range(self.dim) if e != d]
+            grids = np.meshgrid(*[np.arange(n)] * (self.dim - 1), indexing="ij")
+            c = np.zeros((n ** (self.dim - 1), 3))
+            for t, e in enumerate(tang):
+                c[:, e] = (grids[t].ravel() + 0.5) * D + self.origin[e]
+            # face between the interior cell and its ghost: ig = +-1
+            i_cell = n - 1 if hi else 0
+            g = 1 if hi else -1
+            c[:, d] = ((g / 2.0 + i_cell) + 0.5) * D + self.origin[d]
+            out.append(c)
+        return np.concatenate(out)
+
+    def set_bc(self, side, kind, values=None):
+        v = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        self.lib.ot_set_bc(self.t, side, kind, None if v is None else _ptr(v))
+        self._keep = v
+
+    def set_bc_function(self, side, kind, f):
+        c = self.bc_face_centres(side)
+        self.set_bc(side, kind, f(c[:, 0], c[:, 1], c[:, 2]))
+
+    def set_smoother(self, name):
+        self.lib.ot_set_smoother(self.t, SMOOTHERS[name])
+
+    def set_niter(self, nmax, nmin=1):
+        self.lib.ot_set_niter(self.t, nmax, nmin)
+
+    def set_tolerance(self, tol):
+        self.lib.ot_set_tolerance(self.t, tol)
+
+    # -- solver
+    def residual(self):
+        return self.lib.ot_residual(self.t)
+
+    def mg_cycle(self, nrelax=4, minlevel=0, maxlevel=None):
+        if maxlevel is None:
+            maxlevel = self.depth()
+        self.lib.ot_mg_cycle(self.t, nrelax, minlevel, maxlevel)
+
+    def restriction_alpha_lambda(self):
+        self.lib.ot_restriction_alpha_lambda(self.t)
+
+    def mg_solve(self, tolerance=0.0, nrelax=4, minlevel=0):
+        s = MGStats()
+        self.lib.ot_mg_solve(self.t, tolerance, nrelax, minlevel, ctypes.byref(s))
+        return s.as_dict()
+
+    def poisson(self, tolerance=0.0, nrelax=4, minlevel=0):
+        s = MGStats()
+        self.lib.ot_poisson(self.t, tolerance, nrelax, minlevel, ctypes.byref(s))
+        return s.as_dict()
