@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Grid", "MGStats", "BMGError", "library", "LIB_PATH",
+__all__ = ["Grid", "Tree", "MGStats", "BMGError", "library", "LIB_PATH",
            "BC_SYMMETRY", "BC_DIRICHLET", "BC_NEUMANN", "BC_PERIODIC",
            "RIGHT", "LEFT", "TOP", "BOTTOM", "FRONT", "BACK"]
 
@@ -40,6 +40,13 @@ EXPORTS = [
     "bmg_restriction_alpha_lambda", "bmg_residual", "bmg_relax",
     "bmg_mg_cycle", "bmg_mg_solve", "bmg_poisson", "bmg_synchronize",
     "bmg_profile_enable", "bmg_profile_get", "bmg_poisson_view",
+    "bmg_tree_new", "bmg_tree_free", "bmg_tree_info", "bmg_tree_leaves",
+    "bmg_tree_refine", "bmg_tree_cache", "bmg_tree_faces", "bmg_tree_set_precision",
+    "bmg_tree_set_smoother", "bmg_tree_set_tolerance", "bmg_tree_set_niter",
+    "bmg_tree_set_bc", "bmg_tree_set_leaf_field", "bmg_tree_get_leaf_field",
+    "bmg_tree_set_alpha_faces", "bmg_tree_set_alpha_const", "bmg_tree_set_lambda_const",
+    "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
+    "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
 ]
 
 
@@ -102,6 +109,31 @@ def library():
         "bmg_profile_enable": (I, [P, I]),
         "bmg_profile_get": (I, [P, I, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_long)]),
         "bmg_poisson_view": (I, [P, P, P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_tree_new": (I, [ctypes.POINTER(P), I, D, D, D, D]),
+        "bmg_tree_free": (I, [P]),
+        "bmg_tree_info": (I, [P, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_long),
+                              ctypes.POINTER(ctypes.c_long)]),
+        "bmg_tree_leaves": (I, [P, dp, dp]),
+        "bmg_tree_refine": (I, [P, dp, ctypes.POINTER(ctypes.c_long)]),
+        "bmg_tree_cache": (I, [P, I, I, ctypes.POINTER(I), dp]),
+        "bmg_tree_faces": (I, [P, ctypes.POINTER(ctypes.c_long), dp]),
+        "bmg_tree_set_precision": (I, [P, I]),
+        "bmg_tree_set_smoother": (I, [P, I]),
+        "bmg_tree_set_tolerance": (I, [P, D]),
+        "bmg_tree_set_niter": (I, [P, I, I]),
+        "bmg_tree_set_bc": (I, [P, I, I, dp]),
+        "bmg_tree_set_leaf_field": (I, [P, I, dp, I]),
+        "bmg_tree_get_leaf_field": (I, [P, I, dp, I]),
+        "bmg_tree_set_alpha_faces": (I, [P, dp, I]),
+        "bmg_tree_set_alpha_const": (I, [P, D]),
+        "bmg_tree_set_lambda_const": (I, [P, D]),
+        "bmg_tree_restriction_alpha_lambda": (I, [P]),
+        "bmg_tree_residual": (I, [P, ctypes.POINTER(D)]),
+        "bmg_tree_relax": (I, [P, I]),
+        "bmg_tree_mg_cycle": (I, [P, I, I, I]),
+        "bmg_tree_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_tree_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_tree_synchronize": (I, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -342,3 +374,245 @@ class Grid:
         ms, n = ctypes.c_double(), ctypes.c_long()
         _check(self.lib.bmg_profile_get(self._g, PROFILE_KINDS[kind], ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+GHOSTS = 2
+
+
+class Tree:
+    """An adaptive quadtree (dim 2) or octree (dim 3) over the box
+    [origin, origin + L0]^dim, refined on the host like the reference's
+    init_grid(1) + refine(cond) (grid/tree.h:1488-1584,
+    grid/tree-common.h:21-67, 308-326), solved on the GPU.
+
+    Leaf fields are numpy arrays in leaf (cache) order, or float64 torch CUDA
+    tensors.  The solver entry points mirror poisson.h on a tree grid.
+    ``precision``: "fp64" or "mixed"; ``smoother``: "rbgs" or "jacobi"."""
+
+    def __init__(self, dim, L0=1.0, origin=(0.0, 0.0, 0.0), precision="fp64", smoother="rbgs"):
+        lib = library()
+        self.dim, self.L0 = dim, float(L0)
+        self.origin = tuple(float(o) for o in origin) + (0.0,) * (3 - len(origin))
+        self._t = ctypes.c_void_p()
+        _check(lib.bmg_tree_new(ctypes.byref(self._t), dim, self.L0, *self.origin[:3]))
+        self.lib = lib
+        self.set_precision(precision)
+        self.set_smoother(smoother)
+
+    def close(self):
+        if getattr(self, "_t", None):
+            self.lib.bmg_tree_free(self._t)
+            self._t = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- topology
+    def info(self):
+        d, nl, nc = ctypes.c_int(), ctypes.c_long(), ctypes.c_long()
+        _check(self.lib.bmg_tree_info(self._t, ctypes.byref(d), ctypes.byref(nl), ctypes.byref(nc)))
+        return d.value, nl.value, nc.value
+
+    def depth(self):
+        return self.info()[0]
+
+    def nleaves(self):
+        return self.info()[1]
+
+    def leaves(self):
+        n = self.nleaves()
+        lev = np.zeros(n, np.int32)
+        xyz = np.zeros((n, 3))
+        _check(self.lib.bmg_tree_leaves(self._t, _host_ptr(lev), _host_ptr(xyz)))
+        return lev, xyz
+
+    def refine_marked(self, mark):
+        m = np.ascontiguousarray(np.asarray(mark, dtype=np.uint8))
+        if m.size != self.nleaves():
+            raise ValueError("one mark per leaf")
+        nr = ctypes.c_long()
+        _check(self.lib.bmg_tree_refine(self._t, _host_ptr(m), ctypes.byref(nr)))
+        return nr.value
+
+    def refine(self, cond):
+        """refine(cond): passes until no leaf satisfies
+        ``cond(x, y, z, level) -> bool array`` (vectorised over the leaves)."""
+        while True:
+            lev, xyz = self.leaves()
+            mark = np.asarray(cond(xyz[:, 0], xyz[:, 1], xyz[:, 2], lev), dtype=np.uint8)
+            if not mark.any():
+                return
+            self.refine_marked(mark)
+
+    def cache(self, which, level):
+        w = {"active": 0, "prolongation": 1, "boundary": 2}[which]
+        n = ctypes.c_int()
+        _check(self.lib.bmg_tree_cache(self._t, w, level, ctypes.byref(n), None))
+        out = np.zeros((n.value, 3), np.int32)
+        if n.value:
+            _check(self.lib.bmg_tree_cache(self._t, w, level, ctypes.byref(n), _host_ptr(out)))
+        if self.dim == 2:
+            out[:, 2] = 0
+        return out
+
+    def faces(self):
+        n = ctypes.c_long()
+        _check(self.lib.bmg_tree_faces(self._t, ctypes.byref(n), None))
+        out = np.zeros((n.value, 5), np.int32)
+        if n.value:
+            _check(self.lib.bmg_tree_faces(self._t, ctypes.byref(n), _host_ptr(out)))
+        return out
+
+    def face_centres(self):
+        """Centres of the face values in the order set_alpha_faces reads them
+        (x = (ig/2. + _I + 0.5)*Delta + X0 with ig = -1 along the normal)."""
+        f = self.faces()
+        rows, ds = [], []
+        for d in range(self.dim):
+            sel = (f[:, 4] & (1 << d)) != 0
+            rows.append(np.nonzero(sel)[0])
+            ds.append(np.full(sel.sum(), d))
+        order = np.argsort(np.concatenate(rows) * 4 + np.concatenate(ds), kind="stable")
+        idx = np.concatenate(rows)[order]
+        dd = np.concatenate(ds)[order]
+        lev = f[idx, 0]
+        D = self.L0 * (1.0 / (1 << lev.astype(np.int64)))
+        xyz = np.zeros((len(idx), 3))
+        for e in range(self.dim):
+            g = np.where(dd == e, -1, 0)
+            xyz[:, e] = ((g / 2.0 + (f[idx, 1 + e] - GHOSTS)) + 0.5) * D + self.origin[e]
+        return xyz, dd.astype(np.int32)
+
+    # -- settings
+    def set_precision(self, p):
+        _check(self.lib.bmg_tree_set_precision(self._t, PRECISIONS[p]))
+        self.precision = p
+
+    def set_smoother(self, s):
+        _check(self.lib.bmg_tree_set_smoother(self._t, SMOOTHERS[s]))
+
+    def set_tolerance(self, tol):
+        _check(self.lib.bmg_tree_set_tolerance(self._t, tol))
+
+    def set_niter(self, nmax, nmin=1):
+        _check(self.lib.bmg_tree_set_niter(self._t, nmax, nmin))
+
+    def bc_face_centres(self, side):
+        """Centres of the boundary faces of `side`, every level 0..depth,
+        concatenated in the order set_bc reads them."""
+        d, hi = side // 2, side % 2 == 0
+        out = []
+        for lev in range(self.depth() + 1):
+            n = 1 << lev
+            D = self.L0 * (1.0 / n)
+            tang = [e for e in range(self.dim) if e != d]
+            grids = np.meshgrid(*[np.arange(n)] * (self.dim - 1), indexing="ij")
+            c = np.zeros((n ** (self.dim - 1), 3))
+            for t, e in enumerate(tang):
+                c[:, e] = (grids[t].ravel() + 0.5) * D + self.origin[e]
+            g = 1 if hi else -1
+            c[:, d] = ((g / 2.0 + (n - 1 if hi else 0)) + 0.5) * D + self.origin[d]
+            out.append(c)
+        return np.concatenate(out)
+
+    def set_bc(self, side, kind, values=None):
+        v = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        _check(self.lib.bmg_tree_set_bc(self._t, side, kind, None if v is None else _host_ptr(v)))
+
+    def set_bc_function(self, side, kind, f):
+        c = self.bc_face_centres(side)
+        self.set_bc(side, kind, f(c[:, 0], c[:, 1], c[:, 2]))
+
+    # -- fields
+    def _put(self, field, v):
+        if hasattr(v, "data_ptr") and getattr(v, "is_cuda", False):
+            if not v.is_contiguous() or str(v.dtype) != "torch.float64" or v.numel() != self.nleaves():
+                raise ValueError("device leaf field must be a contiguous float64 tensor, one value per leaf")
+            import torch
+            torch.cuda.current_stream().synchronize()
+            _check(self.lib.bmg_tree_set_leaf_field(self._t, field, ctypes.c_void_p(v.data_ptr()), 1))
+            return
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        if a.size != self.nleaves():
+            raise ValueError("one value per leaf (%d), got %d" % (self.nleaves(), a.size))
+        _check(self.lib.bmg_tree_set_leaf_field(self._t, field, _host_ptr(a), 0))
+
+    def _get(self, field, out=None):
+        if out is not None and hasattr(out, "data_ptr"):
+            _check(self.lib.bmg_tree_get_leaf_field(self._t, field, ctypes.c_void_p(out.data_ptr()), 1))
+            return out
+        a = np.empty(self.nleaves())
+        _check(self.lib.bmg_tree_get_leaf_field(self._t, field, _host_ptr(a), 0))
+        return a
+
+    def set_a(self, v):
+        self._put(F_A, v)
+
+    def set_b(self, v):
+        self._put(F_B, v)
+
+    def get_a(self, out=None):
+        return self._get(F_A, out)
+
+    def get_b(self):
+        return self._get(F_B)
+
+    def get_res(self):
+        return self._get(F_RES)
+
+    def get_da(self):
+        return self._get(F_DA)
+
+    def set_lambda(self, v):
+        if np.isscalar(v):
+            _check(self.lib.bmg_tree_set_lambda_const(self._t, float(v)))
+        else:
+            self._put(F_LAMBDA, v)
+
+    def set_alpha_const(self, v):
+        _check(self.lib.bmg_tree_set_alpha_const(self._t, float(v)))
+
+    def set_alpha_faces(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        f = self.faces()
+        n = int(sum(((f[:, 4] >> d) & 1).sum() for d in range(self.dim)))
+        if a.size != n:
+            raise ValueError("one alpha value per face value (%d), got %d" % (n, a.size))
+        _check(self.lib.bmg_tree_set_alpha_faces(self._t, _host_ptr(a), 0))
+
+    def set_alpha_function(self, f):
+        """alpha.x[] = f(x, y, z, d) on every face of the face cache."""
+        xyz, dd = self.face_centres()
+        self.set_alpha_faces(f(xyz[:, 0], xyz[:, 1], xyz[:, 2], dd))
+
+    # -- solver (poisson.h)
+    def restriction_alpha_lambda(self):
+        _check(self.lib.bmg_tree_restriction_alpha_lambda(self._t))
+
+    def residual(self):
+        m = ctypes.c_double()
+        _check(self.lib.bmg_tree_residual(self._t, ctypes.byref(m)), allow_nan=True)
+        return m.value
+
+    def relax(self, level):
+        _check(self.lib.bmg_tree_relax(self._t, level))
+
+    def mg_cycle(self, nrelax=4, minlevel=0, maxlevel=None):
+        _check(self.lib.bmg_tree_mg_cycle(self._t, nrelax, minlevel,
+                                          self.depth() if maxlevel is None else maxlevel))
+
+    def mg_solve(self, tolerance=0.0, nrelax=0, minlevel=0):
+        s = MGStats()
+        _check(self.lib.bmg_tree_mg_solve(self._t, tolerance, nrelax, minlevel, ctypes.byref(s)))
+        return s.as_dict()
+
+    def poisson(self, tolerance=0.0, nrelax=0, minlevel=0):
+        s = MGStats()
+        _check(self.lib.bmg_tree_poisson(self._t, tolerance, nrelax, minlevel, ctypes.byref(s)))
+        return s.as_dict()
+
+    def synchronize(self):
+        _check(self.lib.bmg_tree_synchronize(self._t))

@@ -26,7 +26,7 @@ FLAGS = [
 
 
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, "bmg.hip")]
+    srcs = [os.path.join(CSRC, "bmg.hip"), os.path.join(CSRC, "bmg_tree.hip")]
     tmp = LIB + ".tmp"
     cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
     if verbose:

@@ -12,15 +12,16 @@
 #include <vector>
 
 #include "../../include/bmg.h"
+#include "bmg_internal.h"
 #include "bmg_kernels.h"
 
 using namespace bmg;
 
 namespace {
-
 thread_local std::string g_err;
+}
 
-int fail(int code, const char* fmt, ...) {
+int bmg::fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -30,19 +31,7 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHECK(call)                                                                  \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(BMG_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
-                  __FILE__, __LINE__);                                                  \
-  } while (0)
-
-#define TRY(call)              \
-  do {                         \
-    int r_ = (call);           \
-    if (r_ != BMG_OK) return r_; \
-  } while (0)
+namespace {
 
 size_t powd(size_t n, int dim) {
   size_t r = 1;

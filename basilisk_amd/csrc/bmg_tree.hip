@@ -1,0 +1,1561 @@
+// libbmg adaptive-tree path: the host-side quadtree/octree (2:1-balanced
+// refinement and the reference's caches), its compilation into per-level
+// slot tables in GPU memory, and the solver loop over the kernels of
+// bmg_tree_kernels.h, behind the bmg_tree_* entry points of include/bmg.h.
+//
+// Reference semantics (paths relative to the reference's src/):
+//   Cell, flags, neighbours      grid/tree.h:24-52, 398-399
+//   init_grid                    grid/tree.h:1488-1584
+//   refine_cell, refine()        grid/tree-common.h:21-67, 308-326
+//   increment_neighbors          grid/tree.h:1032-1041, alloc_children 929-972
+//   update_cache_f (FBOUNDARY)   grid/tree.h:556-684, foreach_cell.h:66-135
+//   foreach_level_or_leaf        grid/tree.h:735-740
+//   tree_boundary_level          grid/tree-common.h:587-716
+//   box_boundary_level           grid/tree.h:1139-1363
+//   halo_face, refine_face_x     grid/tree-common.h:373-409, 464-511
+//   relax, residual (TREE)       poisson.h:271-338, 355-372
+//   mg_cycle, mg_solve, poisson  poisson.h:32-90, 139-235, 402-441
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/bmg.h"
+#include "bmg_internal.h"
+#include "bmg_tree_kernels.h"
+
+using namespace bmgt;
+
+namespace {
+
+constexpr int GH = 2;
+constexpr int kMaxDepth = 16;
+enum : uint8_t { C_ACTIVE = 1, C_LEAF = 2, C_MARK = 4, C_ALLOC = 8 };
+
+struct Cell {
+  uint8_t flags = 0;
+  int8_t pid = 0;
+  uint16_t nb = 0;
+  int32_t slot = -1;
+};
+
+struct IJK {
+  int i, j, k;
+  int& operator[](int d) { return d == 0 ? i : d == 1 ? j : k; }
+};
+
+// one level of the tree: a dense directory of 8^dim bricks of cells
+struct TLevel {
+  int dim = 2, n = 0, len = 0, nbd = 0;
+  std::vector<int32_t> dir;
+  std::vector<std::unique_ptr<Cell[]>> bricks;
+  void init(int l, int d) {
+    dim = d;
+    n = 1 << l;
+    len = n + 2 * GH;
+    nbd = (len + 7) >> 3;
+    size_t nd = (size_t)nbd * nbd * (dim == 3 ? nbd : 1);
+    dir.assign(nd, -1);
+  }
+  size_t bsize() const { return dim == 3 ? 512 : 64; }
+  size_t bindex(int i, int j, int k) const {
+    return dim == 3 ? ((size_t)(i >> 3) * nbd + (j >> 3)) * nbd + (k >> 3) : (size_t)(i >> 3) * nbd + (j >> 3);
+  }
+  size_t cindex(int i, int j, int k) const {
+    return dim == 3 ? (size_t)(((i & 7) << 6) | ((j & 7) << 3) | (k & 7)) : (size_t)(((i & 7) << 3) | (j & 7));
+  }
+  bool inside(int i, int j, int k) const {
+    return i >= 0 && i < len && j >= 0 && j < len && (dim == 2 || (k >= 0 && k < len));
+  }
+  Cell* find(int i, int j, int k) {
+    if (!inside(i, j, k)) return nullptr;
+    int32_t b = dir[bindex(i, j, k)];
+    if (b < 0) return nullptr;
+    Cell* c = &bricks[b][cindex(i, j, k)];
+    return (c->flags & C_ALLOC) ? c : nullptr;
+  }
+  Cell& make(int i, int j, int k) {
+    size_t bi = bindex(i, j, k);
+    if (dir[bi] < 0) {
+      dir[bi] = (int32_t)bricks.size();
+      bricks.emplace_back(new Cell[bsize()]);
+    }
+    return bricks[dir[bi]][cindex(i, j, k)];
+  }
+};
+
+struct Face {
+  int level;
+  IJK c;
+  int flags;
+};
+
+// host-side topology, following the reference's tree
+struct Topo {
+  int dim = 2, depth = 0;
+  std::vector<TLevel> L;
+  bool dirty = true;
+  // caches
+  std::vector<std::vector<IJK>> active, prolong, bnd;
+  std::vector<int> leaf_level;
+  std::vector<IJK> leaf_c;
+  std::vector<Face> faces;
+
+  Cell& at(int l, int i, int j, int k) {
+    Cell* c = L[l].find(i, j, k);
+    if (!c) {
+      static Cell none;
+      none = Cell();
+      return none;
+    }
+    return *c;
+  }
+  bool alloc(int l, int i, int j, int k) { return L[l].find(i, j, k) != nullptr; }
+
+  void init(int d) {
+    dim = d;
+    depth = 0;
+    L.clear();
+    L.resize(1);
+    L[0].init(0, dim);
+    int kmax = dim == 3 ? 2 * GH : 0;
+    for (int i = 0; i <= 2 * GH; i++)
+      for (int j = 0; j <= 2 * GH; j++)
+        for (int k = 0; k <= kmax; k++) {
+          Cell& c = L[0].make(i, j, k);
+          c.flags = C_ALLOC;
+          int a = i - GH, b = j - GH, e = dim == 3 ? k - GH : 0;
+          // right=0, left=1, top=2, bottom=3, front=4, back=5
+          c.pid = (int8_t)(a > 0 ? -1 : a < 0 ? -2 : b > 0 ? -3 : b < 0 ? -4 : e > 0 ? -5 : e < 0 ? -6 : 0);
+        }
+    Cell& root = L[0].make(GH, GH, dim == 3 ? GH : 0);
+    root.flags = C_ALLOC | C_LEAF | C_ACTIVE;
+    root.pid = 0;
+    dirty = true;
+  }
+
+  void add_level() {
+    L.emplace_back();
+    depth++;
+    L[depth].init(depth, dim);
+  }
+
+  int kext() const { return dim == 3 ? 2 : 1; }
+
+  void alloc_children(int l, IJK c) {
+    if (l == depth)
+      add_level();
+    else if (alloc(l + 1, 2 * c.i - GH, 2 * c.j - GH, dim == 3 ? 2 * c.k - GH : 0))
+      return;
+    int8_t pid = at(l, c.i, c.j, c.k).pid;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < kext(); e++) {
+          Cell& ch = L[l + 1].make(2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0);
+          ch.flags = C_ALLOC;
+          ch.nb = 0;
+          ch.pid = pid;
+        }
+  }
+
+  void increment_neighbors(int l, IJK c) {
+    dirty = true;
+    if (at(l, c.i, c.j, c.k).nb++ == 0) alloc_children(l, c);
+    int e0 = dim == 3 ? -1 : 0, e1 = dim == 3 ? 1 : 0;
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        for (int e = e0; e <= e1; e++) {
+          IJK n = {c.i + a, c.j + b, c.k + e};
+          if (at(l, n.i, n.j, n.k).nb++ == 0) alloc_children(l, n);
+        }
+    at(l, c.i, c.j, c.k).nb--;
+  }
+
+  void refine_cell(int l, IJK c) {
+    if (l > 0) {
+      int ch[3] = {2 * ((c.i + GH) % 2) - 1, 2 * ((c.j + GH) % 2) - 1, dim == 3 ? 2 * ((c.k + GH) % 2) - 1 : 1};
+      int pc[3] = {(c.i + GH) / 2, (c.j + GH) / 2, dim == 3 ? (c.k + GH) / 2 : 0};
+      for (int a = 0; a != 2 * ch[0]; a += ch[0])
+        for (int b = 0; b != 2 * ch[1]; b += ch[1])
+          for (int e = 0; e != (dim == 3 ? 2 * ch[2] : 1); e += (dim == 3 ? ch[2] : 1)) {
+            IJK p = {pc[0] + a, pc[1] + b, dim == 3 ? pc[2] + e : 0};
+            Cell& pcell = at(l - 1, p.i, p.j, p.k);
+            if (pcell.pid >= 0 && (pcell.flags & C_LEAF)) refine_cell(l - 1, p);
+          }
+    }
+    increment_neighbors(l, c);
+    Cell& cell = at(l, c.i, c.j, c.k);
+    uint8_t cflag = (cell.flags & C_ACTIVE) ? (C_ACTIVE | C_LEAF) : C_LEAF;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < kext(); e++)
+          at(l + 1, 2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0).flags |= cflag;
+    at(l, c.i, c.j, c.k).flags &= (uint8_t)~C_LEAF;
+  }
+
+  bool is_boundary(const Cell& c) const { return c.pid < 0; }
+  bool is_leaf(const Cell& c) const { return c.flags & C_LEAF; }
+  bool is_refined(const Cell& c) const { return !(c.flags & C_LEAF) && c.nb && c.pid >= 0; }
+  bool is_prolongation(const Cell& c) const { return !(c.flags & C_LEAF) && !c.nb && c.pid >= 0; }
+
+  void visit(int l, IJK c) {
+    Cell* cp = L[l].find(c.i, c.j, c.k);
+    if (!cp) return;
+    Cell& cell = *cp;
+    if (cell.pid == 0 && (cell.flags & C_ACTIVE)) active[l].push_back(c);
+    if (!is_boundary(cell)) {
+      int e0 = dim == 3 ? -1 : 0, e1 = dim == 3 ? 1 : 0;
+      for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+          for (int e = e0; e <= e1; e++) {
+            Cell* n = L[l].find(c.i + a, c.j + b, c.k + e);
+            if (n && is_boundary(*n) && !(n->flags & C_MARK)) {
+              bnd[l].push_back({c.i + a, c.j + b, c.k + e});
+              n->flags |= C_MARK;
+            }
+          }
+    }
+    if (is_leaf(cell)) {
+      if (cell.pid == 0) {
+        leaf_level.push_back(l);
+        leaf_c.push_back(c);
+        int flags = 0;
+        for (int d = 0; d < dim; d++) {
+          IJK m = c;
+          m[d]--;
+          Cell& mc = at(l, m.i, m.j, m.k);
+          if (is_boundary(mc) || is_prolongation(mc) || is_leaf(mc)) flags |= 1 << d;
+        }
+        if (flags) faces.push_back({l, c, flags});
+        for (int d = 0; d < dim; d++) {
+          IJK p = c;
+          p[d]++;
+          Cell& pc = at(l, p.i, p.j, p.k);
+          if (is_boundary(pc) || is_prolongation(pc) || (pc.pid != 0 && is_leaf(pc)))
+            faces.push_back({l, p, 1 << d});
+        }
+        if (cell.nb > 0) prolong[l].push_back(c);
+      }
+      return;
+    }
+    if (l < depth)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          for (int e = 0; e < kext(); e++)
+            visit(l + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0});
+  }
+
+  void update() {
+    if (!dirty) return;
+    active.assign(depth + 1, {});
+    prolong.assign(depth + 1, {});
+    bnd.assign(depth + 1, {});
+    leaf_level.clear();
+    leaf_c.clear();
+    faces.clear();
+    visit(0, {GH, GH, dim == 3 ? GH : 0});
+    for (int l = 0; l <= depth; l++)
+      for (auto& c : bnd[l]) at(l, c.i, c.j, c.k).flags &= (uint8_t)~C_MARK;
+    dirty = false;
+  }
+
+  bool full() {
+    update();
+    return (long)leaf_c.size() == (1L << (depth * dim));
+  }
+};
+
+// the compiled tree in GPU memory
+struct DevIndex {
+  int depth = 0, dim = 2;
+  long nslot = 0, nleaf = 0;
+  std::vector<int> off, nA, abase;                 // per level
+  std::vector<int> rec_off, n_nonleaf, n_prolong;  // parent records per level
+  std::vector<int> box_off, box_n;                 // box rules per level
+  std::vector<int> face_off, face_n;               // residual face ops per level
+  std::vector<int> fr_off, fr_n;                   // alpha face restriction per level
+  std::vector<int> fp_off, fp_n;                   // refine_face_x ops per level
+  std::vector<int> fc_off, fc_n;                   // ghost tangential face copies per level
+  std::vector<int> leaf_off, leaf_n;               // leaves of each level (slot lists)
+  int nhf = 0;                                     // halo_face ops
+  long nfaceval = 0;
+  int *nbr = nullptr, *rec_self = nullptr, *rec_child0 = nullptr, *rec_ring = nullptr;
+  uint8_t* colour = nullptr;
+  int *leaf_gs = nullptr, *lvl_leaves = nullptr;
+  BoxRule* box = nullptr;
+  FaceOp* fops = nullptr;
+  FaceAvg *hf = nullptr, *fr = nullptr;
+  FaceProlong* fp = nullptr;
+  FaceCopy* fc = nullptr;
+  int *fv_gs = nullptr, *fv_d = nullptr;
+  double* fc_tmp = nullptr;
+  int maxfc = 0;
+  std::vector<int> bc_off;  // per level offset into each side's face table
+};
+
+void dfree_(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+template <typename T>
+int upload(T** dst, const std::vector<T>& v) {
+  size_t bytes = std::max<size_t>(v.size() * sizeof(T), 8);
+  hipError_t e = hipMalloc((void**)dst, bytes);
+  if (e != hipSuccess) return bmg::fail(BMG_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  if (!v.empty()) HIPCHECK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return BMG_OK;
+}
+
+unsigned nb_(long n) { return (unsigned)((n + kB - 1) / kB); }
+
+}  // namespace
+
+struct bmg_tree {
+  int dim = 2;
+  double L0 = 1., X0[3] = {0., 0., 0.};
+  Topo topo;
+  // solver settings
+  int precision = BMG_FP64, smoother = BMG_RELAX_RBGS;
+  double TOLERANCE = 1e-3;
+  int NITERMAX = 100, NITERMIN = 1;
+  int bc_kind[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> bc_host[6];  // face values of every level, concatenated
+  bool alpha_field = false, lambda_field = false;
+  double alpha_c = 1., lambda_c = 0.;
+  // device
+  bool compiled = false;
+  DevIndex ix;
+  double *a = nullptr, *b = nullptr, *res64 = nullptr, *g[3] = {nullptr, nullptr, nullptr};
+  double *al64[3] = {nullptr, nullptr, nullptr}, *lam64 = nullptr;
+  void *da = nullptr, *res = nullptr, *tmp = nullptr, *alT[3] = {nullptr, nullptr, nullptr}, *lamT = nullptr;
+  double* bcv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool coef_ready = false;
+  TStat* stat = nullptr;
+  double *part = nullptr, *scalar = nullptr;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  // field values kept across a recompilation, in leaf order
+  std::vector<double> keep_a, keep_b, keep_lam;
+};
+
+namespace {
+
+size_t tsize(const bmg_tree* t) { return t->precision == BMG_MIXED ? sizeof(float) : sizeof(double); }
+
+void free_device(bmg_tree* t) {
+  DevIndex& x = t->ix;
+  for (void* p : {(void*)x.nbr, (void*)x.rec_self, (void*)x.rec_child0, (void*)x.rec_ring, (void*)x.colour,
+                  (void*)x.leaf_gs, (void*)x.lvl_leaves, (void*)x.box, (void*)x.fops, (void*)x.hf, (void*)x.fr,
+                  (void*)x.fp, (void*)x.fc, (void*)x.fv_gs, (void*)x.fv_d, (void*)x.fc_tmp})
+    dfree_(p);
+  x = DevIndex();
+  bool mixed = t->precision == BMG_MIXED;
+  dfree_(t->a); dfree_(t->b); dfree_(t->res64);
+  for (int d = 0; d < 3; d++) {
+    dfree_(t->g[d]);
+    dfree_(t->al64[d]);
+    if (mixed) dfree_(t->alT[d]);
+    t->g[d] = t->al64[d] = nullptr;
+    t->alT[d] = nullptr;
+  }
+  dfree_(t->lam64);
+  if (mixed) { dfree_(t->lamT); dfree_(t->res); }
+  dfree_(t->da); dfree_(t->tmp);
+  for (int s = 0; s < 6; s++) { dfree_(t->bcv[s]); t->bcv[s] = nullptr; }
+  dfree_(t->stat); dfree_(t->part); dfree_(t->scalar);
+  t->a = t->b = t->res64 = t->lam64 = nullptr;
+  t->da = t->res = t->tmp = t->lamT = nullptr;
+  t->stat = nullptr; t->part = t->scalar = nullptr;
+  t->compiled = false;
+  t->coef_ready = false;
+}
+
+int dalloc_(void** p, size_t bytes) {
+  if (!bytes) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return bmg::fail(BMG_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return hipMemset(*p, 0, bytes) == hipSuccess ? BMG_OK : bmg::fail(BMG_ERR_HIP, "hipMemset failed");
+}
+
+// tangential index of the face of interior cell c on side s at level l
+int bc_ti(int dim, int l, int side, const int c[3]) {
+  int d = side / 2, n = 1 << l, ti = 0;
+  for (int e = 0; e < dim; e++)
+    if (e != d) ti = ti * n + (c[e] - GH);
+  return ti;
+}
+
+int bid(const Cell& c) { return -c.pid - 1; }
+
+// the tangential child offsets of face averages per component
+const int kFtan[3][4][3] = {
+    {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}},
+    {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {1, 0, 1}},
+    {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}},
+};
+const int kFtan2[2][2][3] = {{{0, 0, 0}, {0, 1, 0}}, {{0, 0, 0}, {1, 0, 0}}};
+
+int ftan(int dim, int d, int m, int e) { return dim == 3 ? kFtan[d][m][e] : kFtan2[d][m][e]; }
+
+// compile the topology into slot tables and allocate the fields
+int compile(bmg_tree* t) {
+  Topo& T = t->topo;
+  T.update();
+  const int dim = t->dim, D = T.depth;
+  if (D + 1 > kSeg) return bmg::fail(BMG_ERR_UNSUPPORTED, "tree depth %d exceeds %d levels", D, kSeg - 1);
+  // keep leaf values across recompilation
+  free_device(t);
+  DevIndex& x = t->ix;
+  x.depth = D;
+  x.dim = dim;
+  x.off.assign(D + 2, 0);
+  x.nA.assign(D + 1, 0);
+  x.abase.assign(D + 1, 0);
+  // slot assignment: active, halo children, ghost blocks
+  std::vector<std::vector<IJK>> ghosts(D + 1);
+  long next = 0;
+  for (int l = 0; l <= D; l++) {
+    TLevel& Lv = T.L[l];
+    x.off[l] = (int)next;
+    x.abase[l] = l ? x.abase[l - 1] + x.nA[l - 1] : 0;
+    for (auto& c : T.active[l]) Lv.find(c.i, c.j, c.k)->slot = (int)next++;
+    x.nA[l] = (int)T.active[l].size();
+    if (l > 0) {
+      for (auto& p : T.prolong[l - 1])
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++)
+            for (int e = 0; e < T.kext(); e++) {
+              Cell* ch = Lv.find(2 * p.i - GH + a, 2 * p.j - GH + b, dim == 3 ? 2 * p.k - GH + e : 0);
+              if (!ch) return bmg::fail(BMG_ERR_ARG, "tree: missing halo child");
+              ch->slot = (int)next++;
+            }
+      for (auto& gp : ghosts[l - 1]) {
+        Cell* gc = T.L[l - 1].find(gp.i, gp.j, gp.k);
+        if (gc->nb == 0) continue;
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++)
+            for (int e = 0; e < T.kext(); e++) {
+              IJK c = {2 * gp.i - GH + a, 2 * gp.j - GH + b, dim == 3 ? 2 * gp.k - GH + e : 0};
+              Cell* ch = Lv.find(c.i, c.j, c.k);
+              if (!ch) return bmg::fail(BMG_ERR_ARG, "tree: missing ghost child");
+              ch->slot = (int)next++;
+              ghosts[l].push_back(c);
+            }
+      }
+    } else {
+      int kmax = dim == 3 ? 2 * GH : 0;
+      for (int i = 0; i <= 2 * GH; i++)
+        for (int j = 0; j <= 2 * GH; j++)
+          for (int k = 0; k <= kmax; k++) {
+            Cell* c = Lv.find(i, j, k);
+            if (c->pid < 0) {
+              c->slot = (int)next++;
+              ghosts[0].push_back({i, j, k});
+            }
+          }
+    }
+    // every allocated cell has a slot
+    long count = 0;
+    for (auto& br : Lv.bricks)
+      for (size_t q = 0; q < Lv.bsize(); q++)
+        if (br[q].flags & C_ALLOC) {
+          count++;
+          if (br[q].slot < x.off[l] || br[q].slot >= next)
+            return bmg::fail(BMG_ERR_ARG, "tree: unslotted cell at level %d", l);
+        }
+    if (count != next - x.off[l]) return bmg::fail(BMG_ERR_ARG, "tree: slot count mismatch at level %d", l);
+  }
+  x.off[D + 1] = (int)next;
+  x.nslot = next;
+  const long NS = next;
+  auto S = [&](int l, int i, int j, int k) -> int {
+    Cell* c = T.L[l].find(i, j, k);
+    return c ? c->slot : -1;
+  };
+  std::vector<int> bad;  // any -1 in a table is an error
+  bool ok = true;
+  auto chk = [&](int v) {
+    if (v < 0 || v >= NS) ok = false;
+    return v;
+  };
+  // active neighbours and colours
+  long NA = x.abase[D] + x.nA[D];
+  std::vector<int> nbr((size_t)NA * 2 * dim);
+  std::vector<uint8_t> colour(NA);
+  for (int l = 0; l <= D; l++)
+    for (int q = 0; q < x.nA[l]; q++) {
+      IJK c = T.active[l][q];
+      size_t ai = (size_t)x.abase[l] + q;
+      for (int d = 0; d < dim; d++) {
+        int o[3] = {0, 0, 0};
+        o[d] = 1;
+        nbr[ai * 2 * dim + 2 * d] = chk(S(l, c.i - o[0], c.j - o[1], c.k - o[2]));
+        nbr[ai * 2 * dim + 2 * d + 1] = chk(S(l, c.i + o[0], c.j + o[1], c.k + o[2]));
+      }
+      colour[ai] = (uint8_t)((c.i + c.j + c.k) & 1);
+    }
+  if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: active cell without neighbours");
+  // parent records: non-leaf active cells, then prolongation leaves
+  std::vector<int> rself, rchild, rring;
+  x.rec_off.assign(D + 1, 0);
+  x.n_nonleaf.assign(D + 1, 0);
+  x.n_prolong.assign(D + 1, 0);
+  auto add_rec = [&](int l, IJK c) {
+    rself.push_back(chk(S(l, c.i, c.j, c.k)));
+    int c0 = chk(S(l + 1, 2 * c.i - GH, 2 * c.j - GH, dim == 3 ? 2 * c.k - GH : 0));
+    rchild.push_back(c0);
+    // the kernels address the children as child0 + (foreach_child index)
+    int m = 0;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < T.kext(); e++, m++)
+          if (S(l + 1, 2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0) != c0 + m) ok = false;
+    int e0 = dim == 3 ? -1 : 0, e1 = dim == 3 ? 1 : 0;
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        for (int e = e0; e <= e1; e++) rring.push_back(chk(S(l, c.i + a, c.j + b, c.k + e)));
+  };
+  for (int l = 0; l <= D; l++) {
+    x.rec_off[l] = (int)rself.size();
+    for (auto& c : T.active[l])
+      if (!T.is_leaf(T.at(l, c.i, c.j, c.k))) {
+        add_rec(l, c);
+        x.n_nonleaf[l]++;
+      }
+    for (auto& c : T.prolong[l]) {
+      add_rec(l, c);
+      x.n_prolong[l]++;
+    }
+  }
+  if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: parent record with an unallocated ring");
+  // leaves: global cache order and per level
+  x.nleaf = (long)T.leaf_c.size();
+  std::vector<int> leaf_gs(x.nleaf);
+  std::vector<std::vector<int>> lv(D + 1);
+  for (long n = 0; n < x.nleaf; n++) {
+    IJK c = T.leaf_c[n];
+    leaf_gs[n] = chk(S(T.leaf_level[n], c.i, c.j, c.k));
+    lv[T.leaf_level[n]].push_back(leaf_gs[n]);
+  }
+  std::vector<int> lvl_leaves;
+  x.leaf_off.assign(D + 1, 0);
+  x.leaf_n.assign(D + 1, 0);
+  for (int l = 0; l <= D; l++) {
+    x.leaf_off[l] = (int)lvl_leaves.size();
+    x.leaf_n[l] = (int)lv[l].size();
+    lvl_leaves.insert(lvl_leaves.end(), lv[l].begin(), lv[l].end());
+  }
+  // box rules (boundary cache order)
+  x.bc_off.assign(D + 2, 0);
+  for (int l = 0; l <= D; l++) {
+    long m = 1;
+    for (int e = 1; e < dim; e++) m *= (1L << l);
+    x.bc_off[l + 1] = x.bc_off[l] + (int)m;
+  }
+  std::vector<BoxRule> box;
+  x.box_off.assign(D + 1, 0);
+  x.box_n.assign(D + 1, 0);
+  for (int l = 0; l <= D; l++) {
+    x.box_off[l] = (int)box.size();
+    for (auto& g : T.bnd[l]) {
+      BoxRule r;
+      memset(&r, 0, sizeof r);
+      r.ghost = chk(S(l, g.i, g.j, g.k));
+      r.src = -1;
+      const int gc[3] = {g.i, g.j, g.k};
+      Cell& gcell = T.at(l, g.i, g.j, g.k);
+      bool done = false;
+      for (int d = 0; d < dim && !done; d++)
+        for (int s = -1; s <= 1 && !done; s += 2) {
+          int c[3] = {gc[0], gc[1], gc[2]};
+          c[d] += s;
+          Cell* p = T.L[l].find(c[0], c[1], c[2]);
+          if (p && p->pid >= 0) {
+            r.nterm = 1;
+            r.src = p->slot;
+            r.side[0] = bid(gcell);
+            r.ti[0] = x.bc_off[l] + bc_ti(dim, l, r.side[0], c);
+            done = true;
+          }
+        }
+      for (int pl = 0; pl < (dim == 3 ? 3 : 1) && !done; pl++) {
+        int da = pl, db = (pl + 1) % dim;
+        for (int i = -1; i <= 1 && !done; i += 2)
+          for (int j = -1; j <= 1 && !done; j += 2) {
+            int nn[3] = {gc[0], gc[1], gc[2]}, n1[3] = {gc[0], gc[1], gc[2]}, n2[3] = {gc[0], gc[1], gc[2]};
+            nn[da] += i; nn[db] += j; n1[da] += i; n2[db] += j;
+            Cell* pn = T.L[l].find(nn[0], nn[1], nn[2]);
+            Cell* p1 = T.L[l].find(n1[0], n1[1], n1[2]);
+            Cell* p2 = T.L[l].find(n2[0], n2[1], n2[2]);
+            if (pn && pn->pid >= 0 && p1 && p1->pid < 0 && p2 && p2->pid < 0) {
+              r.nterm = 2;
+              r.src = pn->slot;
+              r.side[0] = bid(*p1);
+              r.side[1] = bid(*p2);
+              r.ti[0] = x.bc_off[l] + bc_ti(dim, l, r.side[0], nn);
+              r.ti[1] = x.bc_off[l] + bc_ti(dim, l, r.side[1], nn);
+              done = true;
+            }
+          }
+      }
+      if (dim == 3)
+        for (int i = -1; i <= 1 && !done; i += 2)
+          for (int j = -1; j <= 1 && !done; j += 2)
+            for (int k = -1; k <= 1 && !done; k += 2) {
+              Cell* p0 = T.L[l].find(gc[0] + i, gc[1] + j, gc[2] + k);
+              Cell& p1 = T.at(l, gc[0] + i, gc[1] + j, gc[2]);
+              Cell& p2 = T.at(l, gc[0] + i, gc[1], gc[2] + k);
+              Cell& p3 = T.at(l, gc[0], gc[1] + j, gc[2] + k);
+              if (p0 && p0->pid >= 0 && p1.pid < 0 && p2.pid < 0 && p3.pid < 0) {
+                int n0[3] = {gc[0] + i, gc[1] + j, gc[2] + k};
+                r.nterm = 3;
+                r.src = p0->slot;
+                r.side[0] = bid(p1);
+                r.side[1] = bid(p2);
+                r.side[2] = bid(p3);
+                for (int m = 0; m < 3; m++) r.ti[m] = x.bc_off[l] + bc_ti(dim, l, r.side[m], n0);
+                done = true;
+              }
+            }
+      if (!done) r.nterm = 0;
+      if (r.nterm) chk(r.src);
+      box.push_back(r);
+    }
+    x.box_n[l] = (int)box.size() - x.box_off[l];
+  }
+  // residual face ops per level, and the face-cache value map
+  std::vector<std::vector<FaceOp>> fl(D + 1);
+  std::vector<int> fv_gs, fv_d;
+  for (auto& f : T.faces) {
+    int l = f.level;
+    const int fc[3] = {f.c.i, f.c.j, f.c.k};
+    for (int d = 0; d < dim; d++)
+      if (f.flags & (1 << d)) {
+        int m[3] = {fc[0], fc[1], fc[2]};
+        m[d]--;
+        FaceOp op = {chk(S(l, fc[0], fc[1], fc[2])), chk(S(l, m[0], m[1], m[2])), d};
+        fl[l].push_back(op);
+        fv_gs.push_back(op.gs);
+        fv_d.push_back(d);
+      }
+  }
+  x.nfaceval = (long)fv_gs.size();
+  std::vector<FaceOp> fops;
+  x.face_off.assign(D + 1, 0);
+  x.face_n.assign(D + 1, 0);
+  for (int l = 0; l <= D; l++) {
+    x.face_off[l] = (int)fops.size();
+    x.face_n[l] = (int)fl[l].size();
+    fops.insert(fops.end(), fl[l].begin(), fl[l].end());
+  }
+  // fine-face averages: halo_face (all levels in one launch) and the face
+  // restriction of alpha (per level)
+  auto fine_avg = [&](int l, const int c[3], int d, int hi, int target) {
+    FaceAvg fa;
+    memset(&fa, 0, sizeof fa);
+    fa.target = target;
+    fa.d = d;
+    int base[3] = {2 * c[0] - GH, 2 * c[1] - GH, dim == 3 ? 2 * c[2] - GH : 0};
+    base[d] += 2 * hi;
+    for (int m = 0; m < (1 << (dim - 1)); m++)
+      fa.fine[m] = chk(S(l + 1, base[0] + ftan(dim, d, m, 0), base[1] + ftan(dim, d, m, 1),
+                         dim == 3 ? base[2] + ftan(dim, d, m, 2) : 0));
+    return fa;
+  };
+  std::vector<FaceAvg> hf;
+  for (int l = D - 1; l >= 0; l--)
+    for (auto& p : T.prolong[l]) {
+      const int c[3] = {p.i, p.j, p.k};
+      for (int d = 0; d < dim; d++) {
+        int m[3] = {c[0], c[1], c[2]}, q[3] = {c[0], c[1], c[2]};
+        m[d]--;
+        q[d]++;
+        if (T.is_refined(T.at(l, m[0], m[1], m[2]))) hf.push_back(fine_avg(l, c, d, 0, chk(S(l, c[0], c[1], c[2]))));
+        if (T.is_refined(T.at(l, q[0], q[1], q[2]))) hf.push_back(fine_avg(l, c, d, 1, chk(S(l, q[0], q[1], q[2]))));
+      }
+    }
+  x.nhf = (int)hf.size();
+  std::vector<FaceAvg> fr;
+  x.fr_off.assign(D + 1, 0);
+  x.fr_n.assign(D + 1, 0);
+  for (int l = 0; l <= D; l++) {
+    x.fr_off[l] = (int)fr.size();
+    for (auto& p : T.active[l]) {
+      if (T.is_leaf(T.at(l, p.i, p.j, p.k))) continue;
+      const int c[3] = {p.i, p.j, p.k};
+      for (int d = 0; d < dim; d++) {
+        int q[3] = {c[0], c[1], c[2]};
+        q[d]++;
+        fr.push_back(fine_avg(l, c, d, 0, chk(S(l, c[0], c[1], c[2]))));
+        fr.push_back(fine_avg(l, c, d, 1, chk(S(l, q[0], q[1], q[2]))));
+      }
+    }
+    x.fr_n[l] = (int)fr.size() - x.fr_off[l];
+  }
+  // refine_face_x of the prolongation leaves
+  std::vector<FaceProlong> fp;
+  x.fp_off.assign(D + 1, 0);
+  x.fp_n.assign(D + 1, 0);
+  for (int l = 0; l < D; l++) {
+    x.fp_off[l] = (int)fp.size();
+    for (auto& p : T.prolong[l]) {
+      const int c[3] = {p.i, p.j, p.k};
+      for (int d = 0; d < dim; d++) {
+        int t1 = (d + 1) % dim, t2 = (d + 2) % dim;
+        auto sl = [&](int od, int o1, int o2) {
+          int q[3] = {c[0], c[1], c[2]};
+          q[d] += od;
+          q[t1] += o1;
+          if (dim == 3) q[t2] += o2;
+          return chk(S(l, q[0], q[1], q[2]));
+        };
+        FaceProlong r;
+        memset(&r, 0, sizeof r);
+        r.d = d;
+        int m[3] = {c[0], c[1], c[2]}, q[3] = {c[0], c[1], c[2]};
+        m[d]--;
+        q[d]++;
+        Cell& cm = T.at(l, m[0], m[1], m[2]);
+        Cell& cq = T.at(l, q[0], q[1], q[2]);
+        r.flags = (!T.is_refined(cm) ? 1 : 0) | ((!T.is_refined(cq) && cq.nb) ? 2 : 0);
+        r.self = sl(0, 0, 0);
+        r.t1m = sl(0, -1, 0);
+        r.t1p = sl(0, 1, 0);
+        r.t2m = dim == 3 ? sl(0, 0, -1) : r.self;
+        r.t2p = dim == 3 ? sl(0, 0, 1) : r.self;
+        r.next = sl(1, 0, 0);
+        r.nt1m = sl(1, -1, 0);
+        r.nt1p = sl(1, 1, 0);
+        r.nt2m = dim == 3 ? sl(1, 0, -1) : r.next;
+        r.nt2p = dim == 3 ? sl(1, 0, 1) : r.next;
+        int base[3] = {2 * c[0] - GH, 2 * c[1] - GH, dim == 3 ? 2 * c[2] - GH : 0};
+        for (int o = 0; o < 3; o++) {
+          if ((o == 0 && !(r.flags & 1)) || (o == 2 && !(r.flags & 2))) continue;
+          for (int j = 0; j < 2; j++)
+            for (int k = 0; k < (dim == 3 ? 2 : 1); k++) {
+              int f[3] = {base[0], base[1], base[2]};
+              f[d] += o;
+              f[t1] += j;
+              if (dim == 3) f[t2] += k;
+              r.out[o][dim == 3 ? j * 2 + k : j] = chk(S(l + 1, f[0], f[1], f[2]));
+            }
+        }
+        fp.push_back(r);
+      }
+    }
+    x.fp_n[l] = (int)fp.size() - x.fp_off[l];
+  }
+  // ghost tangential faces: sequential semantics resolved to pre-pass sources
+  std::vector<FaceCopy> fc;
+  x.fc_off.assign(D + 1, 0);
+  x.fc_n.assign(D + 1, 0);
+  for (int l = 0; l <= D; l++) {
+    x.fc_off[l] = (int)fc.size();
+    std::vector<std::vector<int>> written(dim);  // slot -> source for this pass
+    std::vector<std::vector<std::pair<int, int>>> w(dim);
+    for (auto& g : T.bnd[l]) {
+      const int gc[3] = {g.i, g.j, g.k};
+      int gs = chk(S(l, gc[0], gc[1], gc[2]));
+      for (int d = 0; d < dim; d++) {
+        int m[3] = {gc[0], gc[1], gc[2]};
+        m[d]--;
+        Cell* pm = T.L[l].find(m[0], m[1], m[2]);
+        if (pm && pm->pid >= 0) continue;
+        int t1 = (d + 1) % dim, t2 = (d + 2) % dim;
+        int found = -1;
+        for (int j = -1; j <= 1 && found < 0; j += 2) {
+          for (int pass = 0; pass < (dim == 3 ? 2 : 1) && found < 0; pass++) {
+            int tt = pass == 0 ? t1 : t2;
+            int a[3] = {gc[0], gc[1], gc[2]};
+            a[tt] += j;
+            int bb[3] = {a[0], a[1], a[2]};
+            bb[d]--;
+            Cell* pa = T.L[l].find(a[0], a[1], a[2]);
+            Cell* pb = T.L[l].find(bb[0], bb[1], bb[2]);
+            if ((pa && pa->pid >= 0) || (pb && pb->pid >= 0)) found = chk(S(l, a[0], a[1], a[2]));
+          }
+        }
+        int src = found;
+        if (found >= 0)
+          for (auto& pr : w[d])
+            if (pr.first == found) src = pr.second;  // written earlier in this pass
+        // later writes of the same target replace earlier ones
+        bool replaced = false;
+        for (auto& pr : w[d])
+          if (pr.first == gs) {
+            pr.second = src;
+            replaced = true;
+          }
+        if (!replaced) w[d].push_back({gs, src});
+      }
+    }
+    for (int d = 0; d < dim; d++)
+      for (auto& pr : w[d]) fc.push_back({pr.first, pr.second, d});
+    x.fc_n[l] = (int)fc.size() - x.fc_off[l];
+    x.maxfc = std::max(x.maxfc, x.fc_n[l]);
+  }
+  if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: index table refers to an unallocated cell");
+  // upload
+  TRY(upload(&x.nbr, nbr));
+  TRY(upload(&x.colour, colour));
+  TRY(upload(&x.rec_self, rself));
+  TRY(upload(&x.rec_child0, rchild));
+  TRY(upload(&x.rec_ring, rring));
+  TRY(upload(&x.leaf_gs, leaf_gs));
+  TRY(upload(&x.lvl_leaves, lvl_leaves));
+  TRY(upload(&x.box, box));
+  TRY(upload(&x.fops, fops));
+  TRY(upload(&x.hf, hf));
+  TRY(upload(&x.fr, fr));
+  TRY(upload(&x.fp, fp));
+  TRY(upload(&x.fc, fc));
+  TRY(upload(&x.fv_gs, fv_gs));
+  TRY(upload(&x.fv_d, fv_d));
+  TRY(dalloc_((void**)&x.fc_tmp, sizeof(double) * std::max(1, x.maxfc)));
+  // fields
+  bool mixed = t->precision == BMG_MIXED;
+  size_t ts = tsize(t);
+  TRY(dalloc_((void**)&t->a, NS * 8));
+  TRY(dalloc_((void**)&t->b, NS * 8));
+  TRY(dalloc_((void**)&t->res64, NS * 8));
+  TRY(dalloc_((void**)&t->lam64, NS * 8));
+  for (int d = 0; d < dim; d++) {
+    TRY(dalloc_((void**)&t->g[d], NS * 8));
+    TRY(dalloc_((void**)&t->al64[d], NS * 8));
+    if (mixed)
+      TRY(dalloc_(&t->alT[d], NS * ts));
+    else
+      t->alT[d] = t->al64[d];
+  }
+  if (mixed) {
+    TRY(dalloc_(&t->lamT, NS * ts));
+    TRY(dalloc_(&t->res, NS * ts));
+  } else {
+    t->lamT = t->lam64;
+    t->res = t->res64;
+  }
+  TRY(dalloc_(&t->da, NS * ts));
+  TRY(dalloc_(&t->tmp, NS * ts));
+  for (int s = 0; s < 2 * dim; s++) {
+    TRY(dalloc_((void**)&t->bcv[s], sizeof(double) * x.bc_off[D + 1]));
+    if ((int)t->bc_host[s].size() >= x.bc_off[D + 1])
+      HIPCHECK(hipMemcpy(t->bcv[s], t->bc_host[s].data(), sizeof(double) * x.bc_off[D + 1], hipMemcpyHostToDevice));
+  }
+  TRY(dalloc_((void**)&t->stat, sizeof(TStat)));
+  TRY(dalloc_((void**)&t->part, sizeof(double) * 1024));
+  TRY(dalloc_((void**)&t->scalar, sizeof(double)));
+  t->compiled = true;
+  // restore leaf values set before the mesh changed
+  auto restore = [&](std::vector<double>& v, double* f) -> int {
+    if ((long)v.size() != x.nleaf) return BMG_OK;
+    double* tmp = nullptr;
+    TRY(dalloc_((void**)&tmp, v.size() * 8));
+    HIPCHECK(hipMemcpy(tmp, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+    k_t_leaf_scatter<<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, tmp, f);
+    HIPCHECK(hipStreamSynchronize(t->stream));
+    dfree_(tmp);
+    return BMG_OK;
+  };
+  TRY(restore(t->keep_a, t->a));
+  TRY(restore(t->keep_b, t->b));
+  TRY(restore(t->keep_lam, t->lam64));
+  return BMG_OK;
+}
+
+// the tree's topology lives on the host; the GPU is needed from the first
+// solver call on (no CPU execution path)
+int ensure(bmg_tree* t) {
+  if (!t->stream) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      return bmg::fail(BMG_ERR_HIP, "no HIP device available");
+    HIPCHECK(hipStreamCreateWithFlags(&t->own_stream, hipStreamDefault));
+    t->stream = t->own_stream;
+  }
+  if (t->topo.dirty || !t->compiled) return compile(t);
+  return BMG_OK;
+}
+
+// ------------------------------------------------------------ launches
+
+template <typename T>
+T* F(void* p) {
+  return (T*)p;
+}
+
+// segments: the active cells of level l then the leaves of levels l-1..0
+Segs level_or_leaf(const bmg_tree* t, int l) {
+  const DevIndex& x = t->ix;
+  Segs S;
+  memset(&S, 0, sizeof S);
+  int blocks = 0;
+  for (int m = l; m >= 0; m--) {
+    int s = S.n++;
+    S.first_block[s] = blocks;
+    S.abase[s] = x.abase[m];
+    S.lvoff[s] = x.off[m];
+    S.D[s] = t->L0 * (1. / (1 << m));
+    if (m == l) {
+      S.start[s] = x.off[m];
+      S.count[s] = x.nA[m];
+      S.list[s] = nullptr;
+    } else {
+      S.count[s] = x.leaf_n[m];
+      S.list[s] = x.lvl_leaves + x.leaf_off[m];
+    }
+    blocks += (int)nb_(S.count[s]);
+  }
+  S.first_block[S.n] = blocks;
+  return S;
+}
+
+// segments: the leaves of every level
+Segs all_leaves(const bmg_tree* t) {
+  const DevIndex& x = t->ix;
+  Segs S;
+  memset(&S, 0, sizeof S);
+  int blocks = 0;
+  for (int m = 0; m <= x.depth; m++) {
+    int s = S.n++;
+    S.first_block[s] = blocks;
+    S.abase[s] = x.abase[m];
+    S.lvoff[s] = x.off[m];
+    S.D[s] = t->L0 * (1. / (1 << m));
+    S.count[s] = x.leaf_n[m];
+    S.list[s] = x.lvl_leaves + x.leaf_off[m];
+    blocks += (int)nb_(S.count[s]);
+  }
+  S.first_block[S.n] = blocks;
+  return S;
+}
+
+double delta(const bmg_tree* t, int l) { return t->L0 * (1. / (1 << l)); }
+
+BCT bct(const bmg_tree* t) {
+  BCT b;
+  for (int s = 0; s < 6; s++) {
+    b.kind[s] = t->bc_kind[s];
+    b.v[s] = t->bcv[s];
+  }
+  return b;
+}
+
+template <typename T>
+void box(bmg_tree* t, T* f, int bk, int l) {
+  const DevIndex& x = t->ix;
+  int n = x.box_n[l];
+  if (n) k_t_box<T><<<nb_(n), kB, 0, t->stream>>>(x.box + x.box_off[l], n, f, bct(t), bk, delta(t, l));
+}
+
+template <typename T, int DIM>
+void restrict_level(bmg_tree* t, T* f, int l) {
+  const DevIndex& x = t->ix;
+  int n = x.n_nonleaf[l];
+  if (n)
+    k_t_restrict<T, DIM>
+        <<<nb_(n), kB, 0, t->stream>>>(x.rec_self + x.rec_off[l], x.rec_child0 + x.rec_off[l], n, f);
+}
+
+template <typename T, int DIM>
+void prolong_records(bmg_tree* t, T* f, int l, bool halo) {
+  const DevIndex& x = t->ix;
+  int r0 = x.rec_off[l] + (halo ? x.n_nonleaf[l] : 0);
+  int n = halo ? x.n_prolong[l] : x.n_nonleaf[l];
+  const int NR = DIM == 3 ? 27 : 9;
+  if (n)
+    k_t_prolong<T, DIM><<<nb_(n), kB, 0, t->stream>>>(x.rec_ring + (size_t)r0 * NR, x.rec_child0 + r0, n, f);
+}
+
+// tree_boundary_level ({f}, l) (l < 0: depth)
+template <typename T, int DIM>
+void tree_boundary(bmg_tree* t, T* f, int bk, int l) {
+  int depth = l < 0 ? t->ix.depth : l;
+  if (t->topo.full()) {
+    box<T>(t, f, bk, depth);
+    return;
+  }
+  for (int m = depth - 1; m >= 0; m--) restrict_level<T, DIM>(t, f, m);
+  box<T>(t, f, bk, 0);
+  for (int i = 0; i < depth; i++) {
+    prolong_records<T, DIM>(t, f, i, true);
+    box<T>(t, f, bk, i + 1);
+  }
+}
+
+template <typename T>
+RelaxCoef<T> relax_coef(bmg_tree* t) {
+  RelaxCoef<T> k;
+  for (int d = 0; d < 3; d++) k.alpha[d] = t->alpha_field && d < t->dim ? F<T>(t->alT[d]) : nullptr;
+  k.alpha_c = (T)t->alpha_c;
+  k.lam = t->lambda_field ? F<T>(t->lamT) : nullptr;
+  k.lam_c = (T)t->lambda_c;
+  return k;
+}
+
+template <typename T, int DIM>
+int relax(bmg_tree* t, int l) {
+  const DevIndex& x = t->ix;
+  Segs S = level_or_leaf(t, l);
+  unsigned nbk = (unsigned)S.first_block[S.n];
+  if (!nbk) return BMG_OK;
+  RelaxCoef<T> k = relax_coef<T>(t);
+  T* da = F<T>(t->da);
+  T* res = F<T>(t->res);
+  if (t->smoother == BMG_RELAX_JACOBI) {
+    k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, F<T>(t->tmp), res, x.nbr, x.colour, k, -1);
+    k_t_jacobi<T><<<nbk, kB, 0, t->stream>>>(S, da, F<T>(t->tmp));
+  } else if (t->smoother == BMG_RELAX_RBGS) {
+    k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, res, x.nbr, x.colour, k, 0);
+    for (int m = 0; m <= l; m++) box<T>(t, da, BK_HOM, m);
+    k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, res, x.nbr, x.colour, k, 1);
+  } else
+    return bmg::fail(BMG_ERR_UNSUPPORTED, "the in-place Gauss-Seidel smoother is sequential; use JACOBI or RBGS");
+  return BMG_OK;
+}
+
+// restriction (res) of mg_cycle (tree_restriction)
+template <typename T, int DIM>
+void restriction_res(bmg_tree* t) {
+  T* r = F<T>(t->res);
+  int D = t->ix.depth;
+  if (t->topo.full()) {
+    box<T>(t, r, BK_SYM, D);
+    for (int l = D - 1; l >= 0; l--) {
+      restrict_level<T, DIM>(t, r, l);
+      box<T>(t, r, BK_SYM, l);
+    }
+  } else
+    tree_boundary<T, DIM>(t, r, BK_SYM, -1);
+}
+
+// residual (TREE form) in FP64; res in the cycle precision as well
+template <typename T, int DIM>
+int residual(bmg_tree* t, double* maxres) {
+  const DevIndex& x = t->ix;
+  tree_boundary<double, DIM>(t, t->a, BK_A, -1);
+  const double* al[3] = {nullptr, nullptr, nullptr};
+  if (t->alpha_field)
+    for (int d = 0; d < DIM; d++) al[d] = t->al64[d];
+  for (int l = 0; l <= x.depth; l++) {
+    int n = x.face_n[l];
+    if (n)
+      k_t_faces<<<nb_(n), kB, 0, t->stream>>>(x.fops + x.face_off[l], n, t->a, t->g[0], t->g[1], t->g[2], al[0],
+                                              al[1], al[2], t->alpha_c, delta(t, l));
+  }
+  if (x.nhf) k_t_face_avg<DIM><<<nb_(x.nhf), kB, 0, t->stream>>>(x.hf, x.nhf, t->g[0], t->g[1], t->g[2]);
+  k_t_stat_reset<<<1, 64, 0, t->stream>>>(t->stat);
+  Segs S = all_leaves(t);
+  const double* lam = t->lambda_field ? t->lam64 : nullptr;
+  k_t_leaf_res<DIM, T><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(
+      S, t->a, t->b, t->g[0], t->g[1], t->g[2], x.nbr, lam, t->lambda_c, t->res64, F<T>(t->res), t->stat);
+  TStat st;
+  HIPCHECK(hipMemcpyAsync(&st, t->stat, sizeof st, hipMemcpyDeviceToHost, t->stream));
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  HIPCHECK(hipGetLastError());
+  double m;
+  memcpy(&m, &st.maxbits, sizeof m);
+  *maxres = m;
+  return st.nans ? bmg::fail(BMG_ERR_NAN, "residual: %llu NaN values", st.nans) : BMG_OK;
+}
+
+// restriction ({alpha, lambda}) of poisson() on the tree, in FP64, then the
+// copies in the cycle precision
+template <typename T, int DIM>
+int coefficients(bmg_tree* t) {
+  const DevIndex& x = t->ix;
+  if (t->coef_ready) return BMG_OK;
+  bool full = t->topo.full();
+  int D = x.depth;
+  auto faces_restrict = [&](int l) {
+    int n = x.fr_n[l];
+    if (n) k_t_face_avg<DIM><<<nb_(n), kB, 0, t->stream>>>(x.fr + x.fr_off[l], n, t->al64[0], t->al64[1], t->al64[2]);
+  };
+  auto faces_box = [&](int l) {
+    int n = x.fc_n[l];
+    if (!n) return;
+    k_t_face_gather<<<nb_(n), kB, 0, t->stream>>>(x.fc + x.fc_off[l], n, t->al64[0], t->al64[1], t->al64[2],
+                                                  x.fc_tmp);
+    k_t_face_scatter<<<nb_(n), kB, 0, t->stream>>>(x.fc + x.fc_off[l], n, t->al64[0], t->al64[1], t->al64[2],
+                                                   x.fc_tmp);
+  };
+  if (t->alpha_field && x.nhf)
+    k_t_face_avg<DIM><<<nb_(x.nhf), kB, 0, t->stream>>>(x.hf, x.nhf, t->al64[0], t->al64[1], t->al64[2]);
+  if (full) {
+    if (t->lambda_field) box<double>(t, t->lam64, BK_SYM, D);
+    if (t->alpha_field) faces_box(D);
+    for (int l = D - 1; l >= 0; l--) {
+      if (t->lambda_field) restrict_level<double, DIM>(t, t->lam64, l);
+      if (t->alpha_field) faces_restrict(l);
+      if (t->lambda_field) box<double>(t, t->lam64, BK_SYM, l);
+      if (t->alpha_field) faces_box(l);
+    }
+  } else {
+    for (int l = D - 1; l >= 0; l--) {
+      if (t->lambda_field) restrict_level<double, DIM>(t, t->lam64, l);
+      if (t->alpha_field) faces_restrict(l);
+    }
+    if (t->lambda_field) box<double>(t, t->lam64, BK_SYM, 0);
+    if (t->alpha_field) faces_box(0);
+    for (int i = 0; i < D; i++) {
+      if (t->lambda_field) prolong_records<double, DIM>(t, t->lam64, i, true);
+      if (t->alpha_field && x.fp_n[i])
+        k_t_face_prolong<DIM><<<nb_(x.fp_n[i]), kB, 0, t->stream>>>(x.fp + x.fp_off[i], x.fp_n[i], t->al64[0],
+                                                                    t->al64[1], t->al64[2]);
+      if (t->lambda_field) box<double>(t, t->lam64, BK_SYM, i + 1);
+      if (t->alpha_field) faces_box(i + 1);
+    }
+  }
+  if (t->precision == BMG_MIXED) {
+    long NS = x.nslot;
+    if (t->alpha_field)
+      for (int d = 0; d < DIM; d++)
+        k_t_convert<double, float><<<nb_(NS), kB, 0, t->stream>>>(t->al64[d], F<float>(t->alT[d]), NS);
+    if (t->lambda_field) k_t_convert<double, float><<<nb_(NS), kB, 0, t->stream>>>(t->lam64, F<float>(t->lamT), NS);
+  }
+  HIPCHECK(hipGetLastError());
+  t->coef_ready = true;
+  return BMG_OK;
+}
+
+template <typename T, int DIM>
+int mg_cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
+  const DevIndex& x = t->ix;
+  TRY(coefficients<T, DIM>(t));
+  restriction_res<T, DIM>(t);
+  minlevel = std::min(minlevel, maxlevel);
+  T* da = F<T>(t->da);
+  for (int l = minlevel; l <= maxlevel; l++) {
+    if (l == minlevel) {
+      Segs S = level_or_leaf(t, l);
+      if (S.first_block[S.n]) k_t_fill<T><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(S, da, T(0));
+    } else
+      prolong_records<T, DIM>(t, da, l - 1, false);
+    tree_boundary<T, DIM>(t, da, BK_HOM, l);
+    for (int i = 0; i < nrelax; i++) {
+      TRY(relax<T, DIM>(t, l));
+      tree_boundary<T, DIM>(t, da, BK_HOM, l);
+    }
+  }
+  k_t_correct<T><<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, t->a, da);
+  HIPCHECK(hipGetLastError());
+  return BMG_OK;
+}
+
+template <int DIM>
+int dispatch_cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
+  return t->precision == BMG_MIXED ? mg_cycle<float, DIM>(t, nrelax, minlevel, maxlevel)
+                                   : mg_cycle<double, DIM>(t, nrelax, minlevel, maxlevel);
+}
+
+int cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
+  return t->dim == 3 ? dispatch_cycle<3>(t, nrelax, minlevel, maxlevel)
+                     : dispatch_cycle<2>(t, nrelax, minlevel, maxlevel);
+}
+
+int resid(bmg_tree* t, double* m) {
+  if (t->dim == 3)
+    return t->precision == BMG_MIXED ? residual<float, 3>(t, m) : residual<double, 3>(t, m);
+  return t->precision == BMG_MIXED ? residual<float, 2>(t, m) : residual<double, 2>(t, m);
+}
+
+int coefs(bmg_tree* t) {
+  if (t->dim == 3)
+    return t->precision == BMG_MIXED ? coefficients<float, 3>(t) : coefficients<double, 3>(t);
+  return t->precision == BMG_MIXED ? coefficients<float, 2>(t) : coefficients<double, 2>(t);
+}
+
+int sum_b(bmg_tree* t, double* out) {
+  const DevIndex& x = t->ix;
+  int np = (int)std::min<long>(1024, std::max<long>(1, (long)nb_(x.nleaf)));
+  k_t_sum_partial<<<np, kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, t->b, t->part);
+  k_t_sum_final<<<1, 64, 0, t->stream>>>(t->part, np, t->scalar);
+  HIPCHECK(hipMemcpyAsync(out, t->scalar, sizeof(double), hipMemcpyDeviceToHost, t->stream));
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  return BMG_OK;
+}
+
+int mg_solve_impl(bmg_tree* t, double tolerance, int nrelax, int minlevel, bmg_mgstats* out) {
+  bmg_mgstats s;
+  memset(&s, 0, sizeof s);
+  TRY(sum_b(t, &s.sum));
+  s.nrelax = nrelax > 0 ? nrelax : 4;
+  double resb;
+  int rc = resid(t, &resb);
+  if (rc != BMG_OK && rc != BMG_ERR_NAN) return rc;
+  int nan = rc == BMG_ERR_NAN;
+  s.resb = s.resa = resb;
+  if (tolerance == 0.) tolerance = t->TOLERANCE;
+  for (s.i = 0; s.i < t->NITERMAX && (s.i < t->NITERMIN || s.resa > tolerance); s.i++) {
+    TRY(cycle(t, s.nrelax, minlevel, t->ix.depth));
+    rc = resid(t, &s.resa);
+    if (rc != BMG_OK && rc != BMG_ERR_NAN) return rc;
+    nan |= rc == BMG_ERR_NAN;
+    if (s.resa > tolerance) {
+      if (resb / s.resa < 1.2 && s.nrelax < 100)
+        s.nrelax++;
+      else if (resb / s.resa > 10 && s.nrelax > 2)
+        s.nrelax--;
+    }
+    resb = s.resa;
+    if (nan) break;
+  }
+  s.minlevel = minlevel;
+  if (out) *out = s;
+  return nan ? bmg::fail(BMG_ERR_NAN, "residual is NaN") : BMG_OK;
+}
+
+int set_leaf_values(bmg_tree* t, double* f, const double* v, int on_device) {
+  const DevIndex& x = t->ix;
+  const double* src = v;
+  double* tmp = nullptr;
+  if (!on_device) {
+    TRY(dalloc_((void**)&tmp, x.nleaf * 8));
+    HIPCHECK(hipMemcpy(tmp, v, x.nleaf * 8, hipMemcpyHostToDevice));
+    src = tmp;
+  }
+  k_t_leaf_scatter<<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, src, f);
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  dfree_(tmp);
+  return BMG_OK;
+}
+
+template <typename T>
+int get_leaf_values(bmg_tree* t, const T* f, double* v, int on_device) {
+  const DevIndex& x = t->ix;
+  double* dst = v;
+  double* tmp = nullptr;
+  if (!on_device) {
+    TRY(dalloc_((void**)&tmp, x.nleaf * 8));
+    dst = tmp;
+  }
+  k_t_leaf_gather<T><<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, f, dst);
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  if (!on_device) HIPCHECK(hipMemcpy(v, tmp, x.nleaf * 8, hipMemcpyDeviceToHost));
+  dfree_(tmp);
+  return BMG_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ C ABI
+
+extern "C" {
+
+int bmg_tree_new(bmg_tree** out, int dimension, double L0, double X0, double Y0, double Z0) {
+  if (!out) return bmg::fail(BMG_ERR_ARG, "null handle pointer");
+  *out = nullptr;
+  if (dimension != 2 && dimension != 3) return bmg::fail(BMG_ERR_ARG, "trees are 2D or 3D (got %d)", dimension);
+  if (!(L0 > 0.)) return bmg::fail(BMG_ERR_ARG, "L0 must be positive");
+  bmg_tree* t = new bmg_tree();
+  t->dim = dimension;
+  t->L0 = L0;
+  t->X0[0] = X0;
+  t->X0[1] = Y0;
+  t->X0[2] = Z0;
+  t->topo.init(dimension);
+  *out = t;
+  return BMG_OK;
+}
+
+int bmg_tree_free(bmg_tree* t) {
+  if (!t) return BMG_OK;
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  free_device(t);
+  if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
+  delete t;
+  return BMG_OK;
+}
+
+int bmg_tree_info(bmg_tree* t, int* depth, long* nleaves, long* ncells) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  t->topo.update();
+  if (depth) *depth = t->topo.depth;
+  if (nleaves) *nleaves = (long)t->topo.leaf_c.size();
+  if (ncells) {
+    long n = 0;
+    for (auto& L : t->topo.L)
+      for (auto& br : L.bricks)
+        for (size_t q = 0; q < L.bsize(); q++) n += (br[q].flags & C_ALLOC) ? 1 : 0;
+    *ncells = n;
+  }
+  return BMG_OK;
+}
+
+int bmg_tree_leaves(bmg_tree* t, int* level, double* xyz) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  Topo& T = t->topo;
+  T.update();
+  for (size_t n = 0; n < T.leaf_c.size(); n++) {
+    int l = T.leaf_level[n];
+    double D = t->L0 * (1. / (1 << l));
+    if (level) level[n] = l;
+    if (xyz) {
+      xyz[3 * n] = (T.leaf_c[n].i - GH + 0.5) * D + t->X0[0];
+      xyz[3 * n + 1] = (T.leaf_c[n].j - GH + 0.5) * D + t->X0[1];
+      xyz[3 * n + 2] = t->dim == 3 ? (T.leaf_c[n].k - GH + 0.5) * D + t->X0[2] : 0.;
+    }
+  }
+  return BMG_OK;
+}
+
+int bmg_tree_refine(bmg_tree* t, const unsigned char* mark, long* nrefined) {
+  if (!t || !mark) return bmg::fail(BMG_ERR_ARG, "null argument");
+  Topo& T = t->topo;
+  T.update();
+  // keep the leaf fields across the mesh change only when nothing changes
+  std::vector<int> ll = T.leaf_level;
+  std::vector<IJK> lc = T.leaf_c;
+  long nr = 0;
+  for (size_t q = 0; q < lc.size(); q++) {
+    if (!mark[q]) continue;
+    Cell& c = T.at(ll[q], lc[q].i, lc[q].j, lc[q].k);
+    if (!(c.flags & C_LEAF)) continue;
+    if (ll[q] + 1 >= kSeg || ll[q] + 1 > kMaxDepth)
+      return bmg::fail(BMG_ERR_UNSUPPORTED, "refinement beyond level %d", std::min(kSeg - 1, kMaxDepth));
+    T.refine_cell(ll[q], lc[q]);
+    nr++;
+  }
+  if (nr) {
+    T.dirty = true;
+    t->compiled = false;
+    t->coef_ready = false;
+    t->keep_a.clear();
+    t->keep_b.clear();
+    t->keep_lam.clear();
+  }
+  if (nrefined) *nrefined = nr;
+  return BMG_OK;
+}
+
+int bmg_tree_cache(bmg_tree* t, int which, int level, int* n, int* ijk) {
+  if (!t || !n) return bmg::fail(BMG_ERR_ARG, "null argument");
+  Topo& T = t->topo;
+  T.update();
+  if (level < 0 || level > T.depth || which < 0 || which > 2) return bmg::fail(BMG_ERR_ARG, "bad cache request");
+  auto& c = which == 0 ? T.active[level] : which == 1 ? T.prolong[level] : T.bnd[level];
+  if (ijk)
+    for (size_t q = 0; q < c.size(); q++) {
+      ijk[3 * q] = c[q].i;
+      ijk[3 * q + 1] = c[q].j;
+      ijk[3 * q + 2] = c[q].k;
+    }
+  *n = (int)c.size();
+  return BMG_OK;
+}
+
+int bmg_tree_faces(bmg_tree* t, long* n, int* out) {
+  if (!t || !n) return bmg::fail(BMG_ERR_ARG, "null argument");
+  Topo& T = t->topo;
+  T.update();
+  if (out)
+    for (size_t q = 0; q < T.faces.size(); q++) {
+      out[5 * q] = T.faces[q].level;
+      out[5 * q + 1] = T.faces[q].c.i;
+      out[5 * q + 2] = T.faces[q].c.j;
+      out[5 * q + 3] = T.faces[q].c.k;
+      out[5 * q + 4] = T.faces[q].flags;
+    }
+  *n = (long)T.faces.size();
+  return BMG_OK;
+}
+
+int bmg_tree_set_precision(bmg_tree* t, int precision) {
+  if (!t || (precision != BMG_FP64 && precision != BMG_MIXED)) return bmg::fail(BMG_ERR_ARG, "bad precision");
+  if (precision != t->precision) {
+    // keep the leaf values of a, b and lambda across the reallocation
+    if (t->compiled) {
+      t->keep_a.resize(t->ix.nleaf);
+      t->keep_b.resize(t->ix.nleaf);
+      TRY(get_leaf_values<double>(t, t->a, t->keep_a.data(), 0));
+      TRY(get_leaf_values<double>(t, t->b, t->keep_b.data(), 0));
+      if (t->lambda_field) {
+        t->keep_lam.resize(t->ix.nleaf);
+        TRY(get_leaf_values<double>(t, t->lam64, t->keep_lam.data(), 0));
+      }
+      if (t->alpha_field) return bmg::fail(BMG_ERR_ARG, "set the precision before alpha");
+      free_device(t);
+    }
+    t->precision = precision;
+  }
+  return BMG_OK;
+}
+
+int bmg_tree_set_smoother(bmg_tree* t, int smoother) {
+  if (!t || smoother < BMG_RELAX_GS || smoother > BMG_RELAX_RBGS) return bmg::fail(BMG_ERR_ARG, "bad smoother");
+  if (smoother == BMG_RELAX_GS)
+    return bmg::fail(BMG_ERR_UNSUPPORTED, "the in-place Gauss-Seidel smoother is sequential; use JACOBI or RBGS");
+  t->smoother = smoother;
+  return BMG_OK;
+}
+
+int bmg_tree_set_tolerance(bmg_tree* t, double tol) {
+  if (!t || !(tol >= 0.)) return bmg::fail(BMG_ERR_ARG, "bad tolerance");
+  t->TOLERANCE = tol;
+  return BMG_OK;
+}
+
+int bmg_tree_set_niter(bmg_tree* t, int nitermax, int nitermin) {
+  if (!t || nitermax < 0 || nitermin < 0) return bmg::fail(BMG_ERR_ARG, "bad iteration limits");
+  t->NITERMAX = nitermax;
+  t->NITERMIN = nitermin;
+  return BMG_OK;
+}
+
+int bmg_tree_set_bc(bmg_tree* t, int side, int kind, const double* values) {
+  if (!t || side < 0 || side >= 2 * t->dim) return bmg::fail(BMG_ERR_ARG, "bad side");
+  if (kind != BMG_BC_SYMMETRY && kind != BMG_BC_DIRICHLET && kind != BMG_BC_NEUMANN)
+    return bmg::fail(BMG_ERR_UNSUPPORTED, "trees take symmetry, Dirichlet or Neumann sides");
+  Topo& T = t->topo;
+  T.update();
+  long m = 0;
+  for (int l = 0; l <= T.depth; l++) {
+    long p = 1;
+    for (int e = 1; e < t->dim; e++) p *= (1L << l);
+    m += p;
+  }
+  t->bc_kind[side] = kind;
+  t->bc_host[side].assign(m, 0.);
+  if (values) std::copy(values, values + m, t->bc_host[side].begin());
+  if (t->compiled) {
+    if ((long)t->ix.bc_off[t->ix.depth + 1] != m) return bmg::fail(BMG_ERR_ARG, "tree changed");
+    HIPCHECK(hipMemcpy(t->bcv[side], t->bc_host[side].data(), sizeof(double) * m, hipMemcpyHostToDevice));
+  }
+  return BMG_OK;
+}
+
+int bmg_tree_set_leaf_field(bmg_tree* t, int field, const double* values, int on_device) {
+  if (!t || !values) return bmg::fail(BMG_ERR_ARG, "null argument");
+  TRY(ensure(t));
+  if (field == BMG_F_A) return set_leaf_values(t, t->a, values, on_device);
+  if (field == BMG_F_B) return set_leaf_values(t, t->b, values, on_device);
+  if (field == BMG_F_LAMBDA) {
+    t->lambda_field = true;
+    t->coef_ready = false;
+    return set_leaf_values(t, t->lam64, values, on_device);
+  }
+  return bmg::fail(BMG_ERR_ARG, "field %d cannot be set on a tree", field);
+}
+
+int bmg_tree_get_leaf_field(bmg_tree* t, int field, double* values, int on_device) {
+  if (!t || !values) return bmg::fail(BMG_ERR_ARG, "null argument");
+  TRY(ensure(t));
+  bool mixed = t->precision == BMG_MIXED;
+  switch (field) {
+    case BMG_F_A: return get_leaf_values<double>(t, t->a, values, on_device);
+    case BMG_F_B: return get_leaf_values<double>(t, t->b, values, on_device);
+    case BMG_F_RES: return get_leaf_values<double>(t, t->res64, values, on_device);
+    case BMG_F_DA:
+      return mixed ? get_leaf_values<float>(t, F<float>(t->da), values, on_device)
+                   : get_leaf_values<double>(t, F<double>(t->da), values, on_device);
+    case BMG_F_LAMBDA: return get_leaf_values<double>(t, t->lam64, values, on_device);
+  }
+  return bmg::fail(BMG_ERR_ARG, "field %d cannot be read on a tree", field);
+}
+
+int bmg_tree_set_alpha_faces(bmg_tree* t, const double* values, int on_device) {
+  if (!t || !values) return bmg::fail(BMG_ERR_ARG, "null argument");
+  TRY(ensure(t));
+  const DevIndex& x = t->ix;
+  const double* src = values;
+  double* tmp = nullptr;
+  if (!on_device) {
+    TRY(dalloc_((void**)&tmp, std::max<long>(1, x.nfaceval) * 8));
+    HIPCHECK(hipMemcpy(tmp, values, x.nfaceval * 8, hipMemcpyHostToDevice));
+    src = tmp;
+  }
+  if (x.nfaceval)
+    k_t_face_set<<<nb_(x.nfaceval), kB, 0, t->stream>>>(x.fv_gs, x.fv_d, x.nfaceval, src, t->al64[0], t->al64[1],
+                                                        t->al64[2]);
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  dfree_(tmp);
+  t->alpha_field = true;
+  t->coef_ready = false;
+  return BMG_OK;
+}
+
+int bmg_tree_set_alpha_const(bmg_tree* t, double alpha) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  t->alpha_field = false;
+  t->alpha_c = alpha;
+  t->coef_ready = false;
+  return BMG_OK;
+}
+
+int bmg_tree_set_lambda_const(bmg_tree* t, double lambda) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  t->lambda_field = false;
+  t->lambda_c = lambda;
+  t->coef_ready = false;
+  return BMG_OK;
+}
+
+int bmg_tree_restriction_alpha_lambda(bmg_tree* t) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  TRY(ensure(t));
+  t->coef_ready = false;
+  TRY(coefs(t));
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  return BMG_OK;
+}
+
+int bmg_tree_residual(bmg_tree* t, double* maxres) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  TRY(ensure(t));
+  double m = 0.;
+  int rc = resid(t, &m);
+  if (maxres) *maxres = m;
+  return rc;
+}
+
+int bmg_tree_relax(bmg_tree* t, int level) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  TRY(ensure(t));
+  if (level < 0 || level > t->ix.depth) return bmg::fail(BMG_ERR_ARG, "bad level %d", level);
+  TRY(coefs(t));
+  int rc = t->dim == 3 ? (t->precision == BMG_MIXED ? relax<float, 3>(t, level) : relax<double, 3>(t, level))
+                       : (t->precision == BMG_MIXED ? relax<float, 2>(t, level) : relax<double, 2>(t, level));
+  TRY(rc);
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  HIPCHECK(hipGetLastError());
+  return BMG_OK;
+}
+
+int bmg_tree_mg_cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  TRY(ensure(t));
+  if (minlevel < 0 || maxlevel < 0 || maxlevel > t->ix.depth) return bmg::fail(BMG_ERR_ARG, "bad levels");
+  TRY(cycle(t, nrelax, minlevel, maxlevel));
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  return BMG_OK;
+}
+
+int bmg_tree_mg_solve(bmg_tree* t, double tolerance, int nrelax, int minlevel, bmg_mgstats* stats) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  TRY(ensure(t));
+  if (minlevel < 0) return bmg::fail(BMG_ERR_ARG, "bad minlevel");
+  return mg_solve_impl(t, tolerance, nrelax, minlevel, stats);
+}
+
+int bmg_tree_poisson(bmg_tree* t, double tolerance, int nrelax, int minlevel, bmg_mgstats* stats) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  TRY(ensure(t));
+  t->coef_ready = false;
+  TRY(coefs(t));
+  double defaultol = t->TOLERANCE;
+  if (tolerance) t->TOLERANCE = tolerance;
+  int rc = mg_solve_impl(t, 0., nrelax, std::max(1, minlevel), stats);
+  if (tolerance) t->TOLERANCE = defaultol;
+  return rc;
+}
+
+int bmg_tree_synchronize(bmg_tree* t) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  return BMG_OK;
+}
+
+}  // extern "C"

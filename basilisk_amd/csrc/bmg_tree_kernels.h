@@ -1,0 +1,450 @@
+// Device kernels of the adaptive-tree path of libbmg.
+//
+// A tree level is stored as a compact array of "slots": the active cells of
+// the level in the reference's cache order (pre-order / Morton), then the
+// halo cells (children of the prolongation leaves of the level above), then
+// every allocated boundary (ghost) cell.  All levels are concatenated, so a
+// field is one array indexed by the global slot.  The host builds index
+// tables (neighbours, children, 3^dim rings, boundary rules, face lists) once
+// per mesh; every kernel is a gather over those tables.
+//
+// Arithmetic follows the reference expression by expression (relax
+// poisson.h:303-320, residual 355-372, bilinear multigrid-common.h:124-140,
+// restriction_average 26-32, face_average 42-58, halo_face
+// tree-common.h:373-409, refine_face_x 464-511, box rules tree.h:1139-1258)
+// and libbmg is compiled with -ffp-contract=off, so the FP64 path equals the
+// CPU oracle bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bmgt {
+
+constexpr int kB = 256;
+constexpr int kSeg = 24;
+
+// A launch over several groups of slots ("segments"): each segment is a
+// contiguous slot range [start, start + count) when list is null, otherwise
+// the slots list[0 .. count).  Blocks are assigned to segments by prefix.
+struct Segs {
+  int n;
+  int first_block[kSeg + 1];
+  int start[kSeg];
+  int count[kSeg];
+  int abase[kSeg];  // active-table index of slot lvoff (the level's first slot)
+  int lvoff[kSeg];
+  double D[kSeg];   // Delta of the segment's level
+  const int* list[kSeg];
+};
+
+__device__ __forceinline__ int seg_of(const Segs& S) {
+  int s = 0;
+  while (s + 1 < S.n && (int)blockIdx.x >= S.first_block[s + 1]) s++;
+  return s;
+}
+
+// slot handled by this thread, or -1
+__device__ __forceinline__ int seg_slot(const Segs& S, int s) {
+  int t = ((int)blockIdx.x - S.first_block[s]) * kB + (int)threadIdx.x;
+  if (t >= S.count[s]) return -1;
+  return S.list[s] ? S.list[s][t] : S.start[s] + t;
+}
+
+// one boundary (ghost) cell: normal rule (nterm 1), 2D diagonal (2), 3D
+// diagonal (3) or no neighbour (0, the reference's `undefined`)
+struct BoxRule {
+  int ghost, src;
+  int nterm;
+  int side[3];
+  int ti[3];  // index of the face value in the side's table
+};
+
+// boundary kinds of the field being filled
+enum { BK_A = 0, BK_HOM = 1, BK_SYM = 2 };
+enum { K_SYMMETRY = 0, K_DIRICHLET = 1, K_NEUMANN = 2 };
+
+struct BCT {
+  int kind[6];
+  const double* v[6];
+};
+
+template <typename T>
+__device__ __forceinline__ T bc_term(const BCT& bc, int bk, int side, int ti, double D, T s) {
+  int kind = bk == BK_SYM ? K_SYMMETRY : bc.kind[side];
+  if (kind == K_SYMMETRY) return s;
+  if (bk == BK_HOM) return kind == K_DIRICHLET ? -s : s;
+  double v = bc.v[side] ? bc.v[side][ti] : 0.;
+  if (kind == K_DIRICHLET) return (T)(2. * v - (double)s);
+  return (T)(D * v + (double)s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_t_box(const BoxRule* __restrict__ R, int n, T* __restrict__ f, BCT bc, int bk,
+                                              double D) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const BoxRule q = R[r];
+  T s = q.nterm ? f[q.src] : T(0);
+  T v;
+  if (q.nterm == 1)
+    v = bc_term<T>(bc, bk, q.side[0], q.ti[0], D, s);
+  else if (q.nterm == 2)
+    v = bc_term<T>(bc, bk, q.side[0], q.ti[0], D, s) + bc_term<T>(bc, bk, q.side[1], q.ti[1], D, s) - s;
+  else if (q.nterm == 3)
+    v = bc_term<T>(bc, bk, q.side[0], q.ti[0], D, s) + bc_term<T>(bc, bk, q.side[1], q.ti[1], D, s) +
+        bc_term<T>(bc, bk, q.side[2], q.ti[2], D, s) - T(2) * s;
+  else
+    v = (T)__builtin_nan("");
+  f[q.ghost] = v;
+}
+
+// restriction_average over the 2^dim children (child block in foreach_child
+// order) of each record
+template <typename T, int DIM>
+__global__ __launch_bounds__(kB) void k_t_restrict(const int* __restrict__ self, const int* __restrict__ child0, int n,
+                                                   T* __restrict__ f) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const T* c = f + child0[r];
+  T sum = T(0);
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); m++) sum += c[m];
+  f[self[r]] = sum / T(1 << DIM);
+}
+
+// bilinear into the 2^dim children of each record from its 3^dim ring
+template <typename T, int DIM>
+__global__ __launch_bounds__(kB) void k_t_prolong(const int* __restrict__ ring, const int* __restrict__ child0, int n,
+                                                  T* __restrict__ f) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  constexpr int NR = DIM == 2 ? 9 : 27;
+  const int* g = ring + (size_t)r * NR;
+  T* out = f + child0[r];
+  if (DIM == 2) {
+    T v[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) v[m] = f[g[m]];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        int cx = 2 * a - 1, cy = 2 * b - 1;
+        T c0 = v[4], c1 = v[(1 + cx) * 3 + 1], c2 = v[3 + 1 + cy], c3 = v[(1 + cx) * 3 + 1 + cy];
+        out[a * 2 + b] = (T(9) * c0 + T(3) * (c1 + c2) + c3) / T(16);
+      }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          int cx = 2 * a - 1, cy = 2 * b - 1, cz = 2 * e - 1;
+#define RG(x, y, z) f[g[((1 + (x)) * 3 + 1 + (y)) * 3 + 1 + (z)]]
+          T c000 = RG(0, 0, 0), c100 = RG(cx, 0, 0), c010 = RG(0, cy, 0), c001 = RG(0, 0, cz);
+          T c110 = RG(cx, cy, 0), c101 = RG(cx, 0, cz), c011 = RG(0, cy, cz), c111 = RG(cx, cy, cz);
+#undef RG
+          out[(a * 2 + b) * 2 + e] =
+              (T(27) * c000 + T(9) * (c100 + c010 + c001) + T(3) * (c110 + c101 + c011) + c111) / T(64);
+        }
+  }
+}
+
+// per-dimension coefficient pointers of relax (null = constant)
+template <typename T>
+struct RelaxCoef {
+  const T* alpha[3];
+  T alpha_c;
+  const T* lam;
+  T lam_c;
+};
+
+// the value n/d of poisson.h:303-320 at slot gs (active index ai)
+template <typename T, int DIM>
+__device__ __forceinline__ T t_relax_value(const T* __restrict__ da, const T* __restrict__ res,
+                                           const int* __restrict__ nbr, const RelaxCoef<T>& k, int gs, int ai,
+                                           T D2) {
+  const int* nb = nbr + (size_t)ai * 2 * DIM;
+  T n = -D2 * res[gs], d = -(k.lam ? k.lam[gs] : k.lam_c) * D2;
+#pragma unroll
+  for (int c = 0; c < DIM; c++) {
+    int lo = nb[2 * c], hi = nb[2 * c + 1];
+    T ah = k.alpha[c] ? k.alpha[c][hi] : k.alpha_c, al = k.alpha[c] ? k.alpha[c][gs] : k.alpha_c;
+    n += ah * da[hi] + al * da[lo];
+    d += ah + al;
+  }
+  return n / d;
+}
+
+// relaxation of the cells of foreach_level_or_leaf(l).  colour < 0: all
+// cells, value written to out (Jacobi's c); colour 0/1: in place, cells of
+// that colour only
+template <typename T, int DIM>
+__global__ __launch_bounds__(kB) void k_t_relax(Segs S, T* __restrict__ da, T* __restrict__ out,
+                                                const T* __restrict__ res, const int* __restrict__ nbr,
+                                                const uint8_t* __restrict__ colour, RelaxCoef<T> k, int sel) {
+  int s = seg_of(S);
+  int gs = seg_slot(S, s);
+  if (gs < 0) return;
+  int ai = S.abase[s] + (gs - S.lvoff[s]);
+  if (sel >= 0 && colour[ai] != sel) return;
+  T D = (T)S.D[s];
+  T v = t_relax_value<T, DIM>(da, res, nbr, k, gs, ai, D * D);
+  if (sel < 0)
+    out[gs] = v;
+  else
+    da[gs] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_t_jacobi(Segs S, T* __restrict__ da, const T* __restrict__ c) {
+  int s = seg_of(S);
+  int gs = seg_slot(S, s);
+  if (gs < 0) return;
+  da[gs] = (da[gs] + T(2) * c[gs]) / T(3);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_t_fill(Segs S, T* __restrict__ f, T v) {
+  int s = seg_of(S);
+  int gs = seg_slot(S, s);
+  if (gs < 0) return;
+  f[gs] = v;
+}
+
+// face fluxes of the residual, g.x[] = alpha.x[] * ((a[] - a[-1]) / Delta)
+struct FaceOp {
+  int gs, gm, d;
+};
+
+__global__ __launch_bounds__(kB) void k_t_faces(const FaceOp* __restrict__ F, int n, const double* __restrict__ a,
+                                                double* g0, double* g1, double* g2, const double* al0,
+                                                const double* al1, const double* al2, double alpha_c, double D) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  FaceOp q = F[r];
+  double* g = q.d == 0 ? g0 : q.d == 1 ? g1 : g2;
+  const double* al = q.d == 0 ? al0 : q.d == 1 ? al1 : al2;
+  double alpha = al ? al[q.gs] : alpha_c;
+  g[q.gs] = alpha * ((a[q.gs] - a[q.gm]) / D);
+}
+
+// mean of 2 (2D) or 4 (3D) fine faces into one coarse face (halo_face,
+// face_average); the fine list is in the reference's summation order
+struct FaceAvg {
+  int target, d;
+  int fine[4];
+};
+
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_t_face_avg(const FaceAvg* __restrict__ F, int n, double* f0, double* f1,
+                                                   double* f2) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  FaceAvg q = F[r];
+  double* f = q.d == 0 ? f0 : q.d == 1 ? f1 : f2;
+  double s = 0.;
+#pragma unroll
+  for (int m = 0; m < (1 << (DIM - 1)); m++) s += f[q.fine[m]];
+  f[q.target] = s / (double)(1 << (DIM - 1));
+}
+
+// refine_face_x for one prolongation leaf and one component d.  v[...] are
+// slots at the leaf's level: self, self -+ t1, self -+ t2, next (= self + d),
+// next -+ t1, next -+ t2; out[] the children faces (o, j, k) with o the
+// position along d (0, 1, 2)
+struct FaceProlong {
+  int d;
+  int flags;  // 1: low side written, 2: high side written
+  int self, t1m, t1p, t2m, t2p;
+  int next, nt1m, nt1p, nt2m, nt2p;
+  int out[3][4];
+};
+
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_t_face_prolong(const FaceProlong* __restrict__ F, int n, double* f0,
+                                                       double* f1, double* f2) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const FaceProlong& q = F[r];
+  double* v = q.d == 0 ? f0 : q.d == 1 ? f1 : f2;
+  double vs = v[q.self], vn = v[q.next];
+  if (q.flags & 1) {
+    double g1 = (v[q.t1p] - v[q.t1m]) / 8.;
+    if (DIM == 3) {
+      double g2 = (v[q.t2p] - v[q.t2m]) / 8.;
+      for (int j = 0; j <= 1; j++)
+        for (int k = 0; k <= 1; k++) v[q.out[0][j * 2 + k]] = vs + (2 * j - 1) * g1 + (2 * k - 1) * g2;
+    } else
+      for (int j = 0; j <= 1; j++) v[q.out[0][j]] = vs + (2 * j - 1) * g1;
+  }
+  if (q.flags & 2) {
+    double g1 = (v[q.nt1p] - v[q.nt1m]) / 8.;
+    if (DIM == 3) {
+      double g2 = (v[q.nt2p] - v[q.nt2m]) / 8.;
+      for (int j = 0; j <= 1; j++)
+        for (int k = 0; k <= 1; k++) v[q.out[2][j * 2 + k]] = vn + (2 * j - 1) * g1 + (2 * k - 1) * g2;
+    } else
+      for (int j = 0; j <= 1; j++) v[q.out[2][j]] = vn + (2 * j - 1) * g1;
+  }
+  {
+    double g1 = (v[q.t1p] - v[q.t1m] + v[q.nt1p] - v[q.nt1m]) / 16.;
+    if (DIM == 3) {
+      double g2 = (v[q.t2p] - v[q.t2m] + v[q.nt2p] - v[q.nt2m]) / 16.;
+      for (int j = 0; j <= 1; j++)
+        for (int k = 0; k <= 1; k++)
+          v[q.out[1][j * 2 + k]] = (vs + vn) / 2. + (2 * j - 1) * g1 + (2 * k - 1) * g2;
+    } else
+      for (int j = 0; j <= 1; j++) v[q.out[1][j]] = (vs + vn) / 2. + (2 * j - 1) * g1;
+  }
+}
+
+// tangential faces of ghost cells (the face-vector part of box_boundary_level
+// with symmetry BCs): a gather of pre-pass values, then a scatter
+struct FaceCopy {
+  int target, src, d;  // src < 0: zero
+};
+
+__global__ __launch_bounds__(kB) void k_t_face_gather(const FaceCopy* __restrict__ F, int n, const double* f0,
+                                                      const double* f1, const double* f2, double* __restrict__ tmp) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  FaceCopy q = F[r];
+  const double* f = q.d == 0 ? f0 : q.d == 1 ? f1 : f2;
+  tmp[r] = q.src >= 0 ? f[q.src] : 0.;
+}
+
+__global__ __launch_bounds__(kB) void k_t_face_scatter(const FaceCopy* __restrict__ F, int n, double* f0, double* f1,
+                                                       double* f2, const double* __restrict__ tmp) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  FaceCopy q = F[r];
+  double* f = q.d == 0 ? f0 : q.d == 1 ? f1 : f2;
+  f[q.target] = tmp[r];
+}
+
+// values given in face-cache order (a foreach_face assignment)
+__global__ __launch_bounds__(kB) void k_t_face_set(const int* __restrict__ gs, const int* __restrict__ dd, long n,
+                                                   const double* __restrict__ v, double* f0, double* f1, double* f2) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  double* f = dd[r] == 0 ? f0 : dd[r] == 1 ? f1 : f2;
+  f[gs[r]] = v[r];
+}
+
+// statistics of the residual
+struct TStat {
+  unsigned long long maxbits;
+  unsigned long long nans;
+};
+
+__global__ void k_t_stat_reset(TStat* s) {
+  if (threadIdx.x == 0) {
+    s->maxbits = 0ull;
+    s->nans = 0ull;
+  }
+}
+
+__device__ __forceinline__ double t_wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// leaf residual: res = b - lambda a - sum_d (g.x[1] - g.x[]) / Delta
+template <int DIM, typename TO>
+__global__ __launch_bounds__(kB) void k_t_leaf_res(Segs S, const double* __restrict__ a, const double* __restrict__ b,
+                                                   const double* g0, const double* g1, const double* g2,
+                                                   const int* __restrict__ nbr, const double* __restrict__ lam,
+                                                   double lam_c, double* __restrict__ res64, TO* __restrict__ res,
+                                                   TStat* st) {
+  __shared__ double smax[kB / 64];
+  __shared__ unsigned long long snan;
+  if (threadIdx.x == 0) snan = 0ull;
+  __syncthreads();
+  int s = seg_of(S);
+  int gs = seg_slot(S, s);
+  double m = 0.;
+  if (gs >= 0) {
+    int ai = S.abase[s] + (gs - S.lvoff[s]);
+    const int* nb = nbr + (size_t)ai * 2 * DIM;
+    double D = S.D[s];
+    double r = b[gs] - (lam ? lam[gs] : lam_c) * a[gs];
+    const double* g[3] = {g0, g1, g2};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) r -= (g[d][nb[2 * d + 1]] - g[d][gs]) / D;
+    res64[gs] = r;
+    res[gs] = (TO)r;
+    if (r != r)
+      atomicAdd(&snan, 1ull);
+    else
+      m = fabs(r);
+  }
+  m = t_wave_max(m);
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = smax[0];
+    for (int i = 1; i < kB / 64; i++) mm = fmax(mm, smax[i]);
+    if (mm > 0.) atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(mm));
+    if (snan) atomicAdd(&st->nans, snan);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_t_correct(const int* __restrict__ leaves, long n, double* __restrict__ a,
+                                                  const T* __restrict__ da) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  int gs = leaves[r];
+  a[gs] += (double)da[gs];
+}
+
+// leaf values in cache order <-> slots
+__global__ __launch_bounds__(kB) void k_t_leaf_scatter(const int* __restrict__ leaves, long n,
+                                                       const double* __restrict__ v, double* __restrict__ f) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r < n) f[leaves[r]] = v[r];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kB) void k_t_leaf_gather(const int* __restrict__ leaves, long n, const T* __restrict__ f,
+                                                      double* __restrict__ v) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r < n) v[r] = (double)f[leaves[r]];
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kB) void k_t_convert(const TI* __restrict__ in, TO* __restrict__ out, long n) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r < n) out[r] = (TO)in[r];
+}
+
+// sum of b over the leaves: per-block partial sums, then one block
+__global__ __launch_bounds__(kB) void k_t_sum_partial(const int* __restrict__ leaves, long n,
+                                                      const double* __restrict__ b, double* __restrict__ part) {
+  __shared__ double sp[kB / 64];
+  double s = 0.;
+  for (long r = (long)blockIdx.x * kB + threadIdx.x; r < n; r += (long)gridDim.x * kB) s += b[leaves[r]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sp[0];
+    for (int i = 1; i < kB / 64; i++) t += sp[i];
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_t_sum_final(const double* __restrict__ part, int np, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.;
+    for (int i = 0; i < np; i++) s += part[i];
+    *out = s;
+  }
+}
+
+}  // namespace bmgt

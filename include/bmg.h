@@ -182,6 +182,70 @@ int bmg_poisson_view(const bmg_multigrid_view *v, const bmg_poisson_args *p,
                      const bmg_view_bc *bc, double TOLERANCE, int NITERMAX,
                      int NITERMIN, bmg_mgstats *stats);
 
+/* ---------------------------------------------------------------- adaptive
+   quadtrees and octrees (grid/tree.h, grid/tree-common.h)
+
+   A bmg_tree starts as the root cell of the box [X0, X0+L0]^dim
+   (init_grid(1), grid/tree.h:1488-1584) and is refined on the host with the
+   reference's 2:1-balanced refine_cell (grid/tree-common.h:21-67).  Its
+   leaves, faces and level lists are the reference's caches in the same
+   (pre-order) order (update_cache_f, grid/tree.h:556-684).  Before the first
+   solver call after a refinement the tree is compiled into per-level slot
+   tables in GPU memory; the solver then runs the reference's tree algorithm
+   (tree_boundary_level with halo prolongation, halo_face, the TREE residual
+   poisson.h:355-372) on the GPU.  Leaf fields are passed in leaf (cache)
+   order.  Sides: 0 right, 1 left, 2 top, 3 bottom, 4 front, 5 back. */
+
+typedef struct bmg_tree bmg_tree;
+
+int bmg_tree_new(bmg_tree **t, int dimension, double L0, double X0, double Y0, double Z0);
+int bmg_tree_free(bmg_tree *t);
+/* depth, number of leaves and of allocated cells (ghosts and halos included) */
+int bmg_tree_info(bmg_tree *t, int *depth, long *nleaves, long *ncells);
+/* leaf levels and cell centres (3 doubles per leaf), leaf order */
+int bmg_tree_leaves(bmg_tree *t, int *level, double *xyz);
+/* one pass of refine() (grid/tree-common.h:308-326): refine every leaf
+   whose mark (leaf order) is non-zero, with 2:1 balancing; repeat passes
+   until no leaf is marked.  Fields set before are discarded. */
+int bmg_tree_refine(bmg_tree *t, const unsigned char *mark, long *nrefined);
+/* the cache lists of a level (which: 0 active, 1 prolongation, 2 boundary)
+   as (i,j,k) triples with the reference's GHOSTS = 2 offset; ijk may be
+   NULL to get the count */
+int bmg_tree_cache(bmg_tree *t, int which, int level, int *n, int *ijk);
+/* the face cache: (level, i, j, k, flags) per entry, flags bit d = face
+   normal to dimension d (foreach_face order); out may be NULL */
+int bmg_tree_faces(bmg_tree *t, long *n, int *out);
+
+int bmg_tree_set_precision(bmg_tree *t, int precision);
+int bmg_tree_set_smoother(bmg_tree *t, int smoother);
+int bmg_tree_set_tolerance(bmg_tree *t, double tolerance);
+int bmg_tree_set_niter(bmg_tree *t, int nitermax, int nitermin);
+/* boundary condition of `a` on one side (symmetry, Dirichlet or Neumann):
+   the expression on every boundary face of every level 0..depth,
+   concatenated, n_l^(dim-1) values per level in C order of the tangential
+   indices (may be NULL for 0) */
+int bmg_tree_set_bc(bmg_tree *t, int side, int kind, const double *values);
+/* leaf fields (BMG_F_A, BMG_F_B, BMG_F_LAMBDA to set; also BMG_F_RES and
+   BMG_F_DA to read), one value per leaf in leaf order */
+int bmg_tree_set_leaf_field(bmg_tree *t, int field, const double *values, int on_device);
+int bmg_tree_get_leaf_field(bmg_tree *t, int field, double *values, int on_device);
+/* alpha on the faces of the face cache: one value per set flag bit, entry
+   order then dimension (a foreach_face() assignment) */
+int bmg_tree_set_alpha_faces(bmg_tree *t, const double *values, int on_device);
+int bmg_tree_set_alpha_const(bmg_tree *t, double alpha);
+int bmg_tree_set_lambda_const(bmg_tree *t, double lambda);
+/* restriction ({alpha, lambda}) through tree_restriction (halo_face, face
+   averages, refine_face_x into the halos) */
+int bmg_tree_restriction_alpha_lambda(bmg_tree *t);
+int bmg_tree_residual(bmg_tree *t, double *maxres);
+int bmg_tree_relax(bmg_tree *t, int level);
+int bmg_tree_mg_cycle(bmg_tree *t, int nrelax, int minlevel, int maxlevel);
+int bmg_tree_mg_solve(bmg_tree *t, double tolerance, int nrelax, int minlevel,
+                      bmg_mgstats *stats);
+int bmg_tree_poisson(bmg_tree *t, double tolerance, int nrelax, int minlevel,
+                     bmg_mgstats *stats);
+int bmg_tree_synchronize(bmg_tree *t);
+
 #ifdef __cplusplus
 }
 #endif

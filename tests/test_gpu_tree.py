@@ -1,0 +1,126 @@
+"""The GPU adaptive-tree path against the tree oracle.
+
+FP64 with the order-independent smoothers (weighted Jacobi and red-black GS)
+is compared bit for bit: every residual norm, the residual field, the
+solution and the solver statistics.  The red-black GPU solve also reproduces
+the final-error lines of the reference's own logs (circle.ref,
+circle-variable.ref, circle.3D.ref), which depend only on the discretisation
+once the cycles have converged.
+"""
+import numpy as np
+import pytest
+
+import basilisk_amd as bmg
+from oracle.tree import OracleTree
+from tests import cases, tree_cases
+
+pytestmark = pytest.mark.gpu
+
+ORIGIN = (-0.5, -0.5, -0.5)
+
+
+def _pair(dim, smoother, precision="fp64", origin=ORIGIN):
+    g = bmg.Tree(dim, 1.0, origin, precision=precision, smoother=smoother)
+    o = OracleTree(dim, 1.0, origin)
+    o.set_smoother(smoother)
+    return g, o
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+@pytest.mark.parametrize("dim,depth,variable", [(2, 7, False), (2, 7, True), (3, 5, False), (3, 5, True)])
+def test_circle_cycles_bitexact(dim, depth, variable, smoother):
+    print("circle", dim, depth, variable, smoother, flush=True)
+    g, o = _pair(dim, smoother)
+    tree_cases.circle_setup(g, depth, variable)
+    tree_cases.circle_setup(o, depth, variable)
+    assert g.residual() == o.residual()
+    assert np.array_equal(g.get_res(), o.get_res())
+    for _ in range(4):
+        g.mg_cycle(4, 0, depth)
+        o.mg_cycle(4, 0, depth)
+        assert g.residual() == o.residual()
+    assert np.array_equal(g.get_a(), o.get_a())
+    assert np.array_equal(g.get_res(), o.get_res())
+
+
+@pytest.mark.parametrize("dim,depth,variable,ref", [
+    (2, 7, False, "circle.ref"), (2, 9, False, "circle.ref"), (2, 8, True, "circle-variable.ref"),
+    (3, 5, False, "circle.3D.ref"), (3, 6, False, "circle.3D.ref")])
+def test_circle_ref_final_error(dim, depth, variable, ref):
+    """GPU red-black solve of test/circle.c: the converged error line equals
+    the reference log's."""
+    print("circle ref", dim, depth, variable, flush=True)
+    g = bmg.Tree(dim, 1.0, ORIGIN, smoother="rbgs")
+    lines = tree_cases.circle_lines(g, depth, variable)
+    want = [l for l in cases.read_ref(ref) if l.startswith("max error %d " % depth)]
+    assert lines[-1] == want[0]
+
+
+def test_full_tree_bitexact():
+    """A uniformly refined tree takes the reference's tree_is_full() paths."""
+    print("full tree", flush=True)
+    g, o = _pair(2, "rbgs")
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: lev < 6)
+    assert g.nleaves() == 4 ** 6
+    for t in (g, o):
+        _, xyz = t.leaves()
+        t.set_b(np.cos(3 * xyz[:, 0]) * np.sin(2 * xyz[:, 1]))
+        t.set_a(np.zeros(len(xyz)))
+        t.set_bc_function(bmg.LEFT, bmg.BC_DIRICHLET, lambda x, y, z: np.sin(4 * y))
+    assert g.residual() == o.residual()
+    for _ in range(3):
+        g.mg_cycle(4, 0, 6)
+        o.mg_cycle(4, 0, 6)
+        assert g.residual() == o.residual()
+    assert np.array_equal(g.get_a(), o.get_a())
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 8), (3, 6)])
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+def test_poisson_spheres_bitexact(dim, depth, smoother):
+    """poisson() with alpha = 1/rho faces, a lambda field and mixed BCs on a
+    tree refined around spheres: statistics and solution equal the oracle's."""
+    print("spheres", dim, depth, smoother, flush=True)
+    g, o = _pair(dim, smoother, origin=(0.0, 0.0, 0.0))
+    tree_cases.spheres_problem(g, depth, 7)
+    tree_cases.spheres_problem(o, depth, 7)
+    for t in (g, o):
+        t.set_niter(30)
+    sg = g.poisson(tolerance=1e-6)
+    so = o.poisson(tolerance=1e-6)
+    for k in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert sg[k] == so[k], k
+    assert abs(sg["sum"] - so["sum"]) <= 1e-12 * max(1.0, abs(so["sum"])) * g.nleaves()
+    assert np.array_equal(g.get_a(), o.get_a())
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 8), (3, 6)])
+def test_mixed_precision_tree(dim, depth):
+    """FP32 smoothing in FP64 defect correction reaches the same tolerance as
+    FP64 and the same solution to 1e-9 (relative max norm)."""
+    print("mixed", dim, depth, flush=True)
+    ref = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0), precision="fp64")
+    mix = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0), precision="mixed")
+    for t in (ref, mix):
+        tree_cases.spheres_problem(t, depth, 3)
+        t.set_niter(60)
+    tol = 1e-9
+    s64 = ref.poisson(tolerance=tol)
+    s32 = mix.poisson(tolerance=tol)
+    assert s64["resa"] <= tol and s32["resa"] <= tol
+    assert s32["i"] <= s64["i"] + 2
+    a64, a32 = ref.get_a(), mix.get_a()
+    assert np.abs(a32 - a64).max() <= 1e-9 * np.abs(a64).max()
+
+
+def test_tree_errors():
+    g = bmg.Tree(2)
+    with pytest.raises(bmg.BMGError):
+        g.set_smoother("gs")
+    with pytest.raises(ValueError):
+        g.set_a(np.zeros(5))
+    g.refine(lambda x, y, z, lev: lev < 3)
+    v = np.arange(g.nleaves(), dtype=float)
+    g.set_b(v)
+    assert np.array_equal(g.get_b(), v)

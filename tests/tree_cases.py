@@ -71,11 +71,11 @@ def circle_lines(t, depth, variable=False):
 def spheres_tree(t, depth, minlevel, nspheres, seed, rmin, rmax, band=2.0):
     """Refines around the interfaces of seeded spheres (circles in 2D) down to
     `depth` within `band` finest cells of each interface, at least `minlevel`
-    everywhere (the configurations of the adaptive benchmarks).  Returns the
-    centres and radii."""
+    everywhere (the configurations of the adaptive benchmarks), on the unit
+    box [0,1]^dim (origin 0).  Returns the centres and radii."""
     rng = np.random.default_rng(seed)
     dim = t.dim
-    c = rng.uniform(0.1, 0.9, (nspheres, dim)) - 0.5
+    c = rng.uniform(0.1, 0.9, (nspheres, dim))
     r = rng.uniform(rmin, rmax, nspheres)
     h = 1.0 / (1 << depth)
 
@@ -89,4 +89,48 @@ def spheres_tree(t, depth, minlevel, nspheres, seed, rmin, rmax, band=2.0):
             near |= dist < band * h + size * np.sqrt(dim)
         return want | ((lev < depth) & near)
     t.refine(cond)
+    return c, r
+
+
+def spheres_problem(t, depth, seed, nspheres=6, minlevel=3, ratio=1000.0, lam=True):
+    """A variable-density Poisson-Helmholtz problem on a tree refined around
+    seeded spheres: alpha = 1/rho on faces (rho = ratio inside the spheres,
+    1 outside, arithmetic face average of the two sides), optional negative
+    lambda field, Dirichlet left / Neumann right / symmetry elsewhere, and a
+    seeded N(0,1) right-hand side at the leaves."""
+    c, r = spheres_tree(t, depth, minlevel, nspheres, seed, 0.05, 0.15)
+    dim = t.dim
+    rng = np.random.default_rng(seed + 100)
+
+    def rho(p):
+        inside = np.zeros(len(p), bool)
+        for s in range(len(r)):
+            inside |= ((p - c[s]) ** 2).sum(1) < r[s] ** 2
+        return np.where(inside, ratio, 1.0)
+
+    _, xyz = t.leaves()
+    n = len(xyz)
+    t.set_b(rng.standard_normal(n))
+    t.set_a(np.zeros(n))
+    if lam:
+        t.set_lambda(-rng.uniform(0.5, 2.0, n))
+    else:
+        t.set_lambda(0.0)
+    fxyz, fd = t.face_centres()
+    f = t.faces()
+    nper = np.zeros(len(f), np.int64)
+    for d in range(dim):
+        nper += (f[:, 4] >> d) & 1
+    h = np.repeat(1.0 / (1 << f[:, 0].astype(np.int64)), nper)
+    p = fxyz[:, :dim]
+    left = p.copy()
+    right = p.copy()
+    left[np.arange(len(p)), fd] -= h / 2
+    right[np.arange(len(p)), fd] += h / 2
+    alpha = 2.0 / (rho(left) + rho(right))
+    t.set_alpha_faces(alpha)
+    cb = t.bc_face_centres(1)
+    t.set_bc(1, BC_DIRICHLET, np.sin(3 * cb[:, 1]))
+    cb = t.bc_face_centres(0)
+    t.set_bc(0, 2, np.cos(2 * cb[:, 1]))
     return c, r
