@@ -20,6 +20,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM bytes moved per fine-level red-black sweep (two launches of the
+# in-place colour kernel), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+# this workload and scaled by the counters' calibration on a copy kernel of
+# known size (fetch x2.0, write x1.0); the derivation is in
+# profiles/hbm_traffic_poisson256_mixed.md.  Keyed by (precision, depth,
+# red-black kernel).
+MEASURED_SWEEP_TRAFFIC = {("mixed", 8, "inplace"): 2 * (333867.0 + 65536.0) * 1024}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -129,6 +137,8 @@ def main():
     # algorithmic bytes of the whole solve (per V-cycle and cell, 3D, nrelax
     # 4: ~312 B FP64, ~192 B mixed) for the whole-solve GB/s
     per_cycle_bytes = {"fp64": 312.0, "mixed": 192.0}[main_prec] * N
+    rb_kernel = os.environ.get("BMG_RB_KERNEL", "inplace")
+    traffic = MEASURED_SWEEP_TRAFFIC.get((main_prec, args.depth, rb_kernel))
     line = {
         "metric": "time-to-1e-9 residual, 256^3 var-coeff Poisson (mixed precision)",
         "value": round(r["seconds"] * 1e3, 3),
@@ -151,7 +161,7 @@ def main():
         "solve_gbs": round(per_cycle_bytes * cycles / r["seconds"] / 1e9, 1),
         "resa": r["stats"]["resa"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "fine-level relax sweep (%s, both colours)" % main_prec,
                      "sweep_ms": round(sweep_ms, 5), "algorithmic_bytes": sweep_bytes},
     }
