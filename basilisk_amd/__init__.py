@@ -47,6 +47,7 @@ EXPORTS = [
     "bmg_tree_set_alpha_faces", "bmg_tree_set_alpha_const", "bmg_tree_set_lambda_const",
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
+    "bmg_coef_gradient",
 ]
 
 
@@ -134,6 +135,7 @@ def library():
         "bmg_tree_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_synchronize": (I, [P]),
+        "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -365,6 +367,32 @@ class Grid:
 
     def synchronize(self):
         _check(self.lib.bmg_synchronize(self._g))
+
+    def coef_gradient(self, psi, out_alpha=None, out_lambda=None, want_lambda=True):
+        """dJ/dalpha (per face component) and dJ/dlambda for the adjoint state
+        psi, at the grid's current `a` (see bmg_coef_gradient).  With torch
+        CUDA tensors (psi and preallocated outputs) everything stays on the
+        device; with numpy input new host arrays are returned."""
+        if hasattr(psi, "data_ptr") and getattr(psi, "is_cuda", False):
+            import torch
+            torch.cuda.current_stream().synchronize()
+            ptrs = [None, None, None]
+            for c in range(self.dim):
+                if out_alpha is not None and out_alpha[c] is not None:
+                    ptrs[c] = ctypes.c_void_p(out_alpha[c].data_ptr())
+            lam = ctypes.c_void_p(out_lambda.data_ptr()) if out_lambda is not None else None
+            _check(self.lib.bmg_coef_gradient(self._g, ctypes.c_void_p(psi.data_ptr()), ptrs[0], ptrs[1],
+                                              ptrs[2], lam, 1))
+            return out_alpha, out_lambda
+        p = np.ascontiguousarray(psi, dtype=np.float64)
+        if p.size != int(np.prod(self.shape())):
+            raise ValueError("psi must have one value per finest-level cell")
+        ga = [np.empty(self.face_shape(c)) for c in range(self.dim)]
+        gl = np.empty(self.shape()) if want_lambda else None
+        ptrs = [_host_ptr(x) for x in ga] + [None] * (3 - self.dim)
+        _check(self.lib.bmg_coef_gradient(self._g, _host_ptr(p), ptrs[0], ptrs[1], ptrs[2],
+                                          _host_ptr(gl) if gl is not None else None, 0))
+        return ga, gl
 
     # -- kernel timing
     def profile(self, enable=True):

@@ -1,0 +1,116 @@
+"""Differentiable Poisson-Helmholtz solves for PyTorch.
+
+``PoissonSolver`` wraps two device-resident hierarchies of libbmg: the
+forward one (the BCs of the unknown) and the adjoint one (the same BC kinds,
+homogeneous).  Calling it solves
+
+    div(alpha grad phi) + lambda phi = b      (poisson.h:402-441, poisson())
+
+and returns phi as a float64 CUDA tensor.  The backward pass uses that the
+discrete operator is self-adjoint: given g = dJ/dphi it solves the same
+operator with homogeneous BCs and right-hand side g for the adjoint state
+psi, then
+
+    dJ/db      = psi
+    dJ/dalpha  = grad(psi) . grad(phi) on each face (wall faces use the ghost
+                 of phi), computed by one fused kernel on the device
+    dJ/dlambda = -psi phi
+
+(bmg_coef_gradient).  The gradients are FP64 whatever the solver precision.
+With ``recompute=True`` the forward solution is not kept between forward and
+backward; the backward solves the forward problem again first.
+"""
+import torch
+
+import basilisk_amd as bmg
+
+
+class PoissonSolver:
+    """dim, depth: the uniform grid (2^depth cells per dimension) over
+    [origin, origin + L0]^dim.  ``bc``: {side: (kind, values or None)} for the
+    sides that are not symmetry (values: numpy array of the boundary
+    expression on the finest-level faces, as bmg.Grid.set_bc takes).
+    ``precision``: "fp64" or "mixed"; ``tolerance``: max |residual|."""
+
+    def __init__(self, dim, depth, bc=None, L0=1.0, origin=(0.0, 0.0, 0.0), precision="mixed",
+                 smoother="rbgs", tolerance=1e-9, nitermax=100, recompute=False):
+        self.dim, self.depth = dim, depth
+        self.tolerance = tolerance
+        self.recompute = recompute
+        self.fwd = bmg.Grid(dim, depth, L0, origin, precision=precision, smoother=smoother)
+        self.adj = bmg.Grid(dim, depth, L0, origin, precision=precision, smoother=smoother)
+        for g in (self.fwd, self.adj):
+            g.set_niter(nitermax, 1)
+        for side, (kind, values) in (bc or {}).items():
+            if kind == bmg.BC_PERIODIC:
+                raise ValueError("periodic boundaries are not differentiable here")
+            self.fwd.set_bc(side, kind, values)
+            self.adj.set_bc(side, kind, None)
+        self.shape = self.fwd.shape()
+        self.last_stats = None
+
+    def _coefficients(self, g, alpha, lam):
+        if alpha is None:
+            g.set_alpha(1.0)
+        else:
+            g.set_alpha([a.detach().contiguous() for a in alpha])
+        if lam is None:
+            g.set_lambda(0.0)
+        elif isinstance(lam, (int, float)):
+            g.set_lambda(float(lam))
+        else:
+            g.set_lambda(lam.detach().contiguous())
+
+    def _solve(self, g, rhs, alpha, lam):
+        self._coefficients(g, alpha, lam)
+        g.set_b(rhs.detach().contiguous())
+        g.set_a(torch.zeros(self.shape, dtype=torch.float64, device=rhs.device))
+        st = g.poisson(tolerance=self.tolerance)
+        out = torch.empty(self.shape, dtype=torch.float64, device=rhs.device)
+        g.get_a(out)
+        return out, st
+
+    def __call__(self, b, alpha=None, lam=None):
+        alpha = list(alpha) if alpha is not None else []
+        return _PoissonFunction.apply(self, b, lam, *alpha)
+
+
+class _PoissonFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, solver, b, lam, *alpha):
+        alpha_l = list(alpha) if alpha else None
+        phi, st = solver._solve(solver.fwd, b, alpha_l, lam)
+        solver.last_stats = st
+        ctx.solver = solver
+        ctx.has_alpha = bool(alpha)
+        ctx.lam_const = not torch.is_tensor(lam)
+        ctx.lam_value = lam if ctx.lam_const else None
+        tensors = [b] + ([lam] if torch.is_tensor(lam) else []) + list(alpha)
+        if not solver.recompute:
+            tensors.append(phi)
+        ctx.save_for_backward(*tensors)
+        return phi
+
+    @staticmethod
+    def backward(ctx, gphi):
+        solver = ctx.solver
+        saved = list(ctx.saved_tensors)
+        b = saved.pop(0)
+        lam = ctx.lam_value if ctx.lam_const else saved.pop(0)
+        phi = None if solver.recompute else saved.pop()
+        alpha = saved if ctx.has_alpha else None
+        if phi is None:
+            phi, _ = solver._solve(solver.fwd, b, alpha, lam)
+        psi, _ = solver._solve(solver.adj, gphi.contiguous(), alpha, lam)
+        g = solver.fwd
+        g.set_a(phi.detach().contiguous())
+        need_a = ctx.has_alpha and any(ctx.needs_input_grad[3:])
+        need_l = (not ctx.lam_const) and ctx.needs_input_grad[2]
+        ga = [torch.empty(g.face_shape(c), dtype=torch.float64, device=phi.device) for c in range(solver.dim)] \
+            if need_a else None
+        gl = torch.empty(solver.shape, dtype=torch.float64, device=phi.device) if need_l else None
+        if need_a or need_l:
+            g.coef_gradient(psi, ga if need_a else [None] * 3, gl)
+            torch.cuda.synchronize()
+        grads_alpha = tuple(ga) if need_a else tuple(None for _ in range(len(alpha) if alpha else 0))
+        return (None, psi if ctx.needs_input_grad[1] else None, gl) + grads_alpha

@@ -99,8 +99,11 @@ def config2(depth=8, seed=1, nspheres=24, ratio=1000.0):
 
 
 def config4(depth=10, seed=4):
-    """2D adjoint problem: log-normal face alpha (sigma 0.5), a target
-    phi_t from a seeded 32-Gaussian source, symmetry walls."""
+    """2D adjoint problem: log-normal face alpha (sigma 0.5) and a seeded
+    32-Gaussian source on homogeneous Dirichlet walls (well posed, so J is
+    defined without a free constant).  The target phi_t of
+    J = 0.5 ||phi - phi_t||^2 is the solution for the same source with
+    alpha = 1."""
     rng = np.random.default_rng(seed)
     n = 1 << depth
     alpha = [np.exp(0.5 * rng.standard_normal((n + 1, n) if c == 0 else (n, n + 1))) for c in range(2)]
@@ -113,4 +116,5 @@ def config4(depth=10, seed=4):
     for k in range(32):
         src += A[k] * np.exp(-((X - c[k, 0]) ** 2 + (Y - c[k, 1]) ** 2) / w[k] ** 2)
     src -= src.mean()
-    return {"dim": 2, "depth": depth, "alpha": alpha, "source": src, "tolerance": 1e-9}
+    return {"dim": 2, "depth": depth, "alpha": alpha, "source": src, "tolerance": 1e-9,
+            "bc": {side: (1, None) for side in range(4)}}

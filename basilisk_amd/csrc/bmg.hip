@@ -708,6 +708,59 @@ int mg_solve_impl(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_m
   return BMG_OK;
 }
 
+template <int DIM>
+int coef_gradient_t(bmg_grid* g, const double* psi, double* const* galpha, double* glambda, int on_device) {
+  Level& lf = g->L[g->depth];
+  Geo<DIM> geo{lf.n, lf.log2n};
+  BCA bc;
+  memset(&bc, 0, sizeof bc);
+  for (int b = 0; b < 6; b++) {
+    bc.sigma[b] = g->bc_sigma[b];
+    bc.c[b] = g->bc_c[b];
+  }
+  double D = delta(g, g->depth);
+  size_t nface = (size_t)(lf.n + 1) * powd(lf.n, DIM - 1);
+  const double* dpsi = psi;
+  double *tmp_psi = nullptr, *tmp_out = nullptr;
+  if (!on_device) {
+    TRY(dalloc((void**)&tmp_psi, lf.ncell * sizeof(double)));
+    HIPCHECK(hipMemcpy(tmp_psi, psi, lf.ncell * sizeof(double), hipMemcpyHostToDevice));
+    dpsi = tmp_psi;
+    TRY(dalloc((void**)&tmp_out, nface * sizeof(double)));
+  }
+  int rc = BMG_OK;
+  for (int c = 0; c < DIM && rc == BMG_OK; c++) {
+    if (!galpha[c]) continue;
+    double* out = on_device ? galpha[c] : tmp_out;
+    hipLaunchKernelGGL((k_coef_grad<DIM>), dim3(nblocks(nface)), dim3(kBlock), 0, g->stream, g->a, dpsi, out, geo,
+                       bc, c, D);
+    if (!on_device && hipMemcpyAsync(galpha[c], tmp_out, nface * sizeof(double), hipMemcpyDeviceToHost,
+                                     g->stream) != hipSuccess)
+      rc = fail(BMG_ERR_HIP, "copy of the alpha gradient failed");
+    if (hipStreamSynchronize(g->stream) != hipSuccess) rc = fail(BMG_ERR_HIP, "coefficient gradient failed");
+  }
+  if (rc == BMG_OK && glambda) {
+    double* out = on_device ? glambda : tmp_out;
+    if (!on_device) {
+      dfreet(tmp_out);
+      rc = dalloc((void**)&tmp_out, lf.ncell * sizeof(double));
+      out = tmp_out;
+    }
+    if (rc == BMG_OK) {
+      hipLaunchKernelGGL(k_lambda_grad, dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, g->a, dpsi, out,
+                         lf.ncell);
+      if (!on_device && hipMemcpyAsync(glambda, tmp_out, lf.ncell * sizeof(double), hipMemcpyDeviceToHost,
+                                       g->stream) != hipSuccess)
+        rc = fail(BMG_ERR_HIP, "copy of the lambda gradient failed");
+      if (hipStreamSynchronize(g->stream) != hipSuccess) rc = fail(BMG_ERR_HIP, "lambda gradient failed");
+    }
+  }
+  dfreet(tmp_psi);
+  dfreet(tmp_out);
+  TRY(rc);
+  return check_launch();
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- C ABI
@@ -1074,6 +1127,18 @@ int bmg_poisson(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_mgs
   int r = mg_solve_impl(g, 0., nrelax, std::max(1, minlevel), stats);
   if (tolerance) g->TOLERANCE = defaultol;
   return r;
+}
+
+int bmg_coef_gradient(bmg_grid* g, const double* psi, double* galpha_x, double* galpha_y, double* galpha_z,
+                      double* glambda, int on_device) {
+  TRY(check_grid(g));
+  if (!psi) return fail(BMG_ERR_ARG, "null adjoint state");
+  for (int d = 0; d < g->dim; d++)
+    if (g->periodic[d]) return fail(BMG_ERR_UNSUPPORTED, "coefficient gradients on periodic boundaries");
+  double* ga[3] = {galpha_x, g->dim > 1 ? galpha_y : nullptr, g->dim > 2 ? galpha_z : nullptr};
+  if (g->dim == 3) return coef_gradient_t<3>(g, psi, ga, glambda, on_device);
+  if (g->dim == 2) return coef_gradient_t<2>(g, psi, ga, glambda, on_device);
+  return coef_gradient_t<1>(g, psi, ga, glambda, on_device);
 }
 
 int bmg_synchronize(bmg_grid* g) {

@@ -403,6 +403,56 @@ __global__ __launch_bounds__(kBlock) void k_correct(double* __restrict__ a, cons
   if (idx < n) a[idx] += (double)da[idx];
 }
 
+// --- coefficient gradients (adjoint) ----------------------------------
+//
+// With L(a) = lambda a + div(alpha grad a) (ghosts from the BCs of `a`) and
+// the adjoint state psi solving the same operator with homogeneous BCs and
+// right-hand side dJ/da, dJ/dp = -psi^T dL/dp at fixed a:
+//   interior face between cells l and r:  (psi_r - psi_l)/D * (a_r - a_l)/D
+//   wall face of cell i:                  -psi_i * ((ghost(a_i) - a_i)/D)/D
+//   lambda of cell i:                     -psi_i a_i
+// One thread per face of component c, in the user layout (n+1 along c).
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_coef_grad(const double* __restrict__ a, const double* __restrict__ psi,
+                                                      double* __restrict__ out, Geo<DIM> g, BCA bc, int c,
+                                                      double D) {
+  size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  size_t nf = (size_t)(g.n + 1);
+  for (int d = 1; d < DIM; d++) nf *= (size_t)g.n;
+  if (t >= nf) return;
+  int q[3] = {0, 0, 0};
+  size_t r = t;
+  for (int d = DIM - 1; d >= 0; d--) {
+    size_t m = (size_t)(d == c ? g.n + 1 : g.n);
+    q[d] = (int)(r % m);
+    r /= m;
+  }
+  const int ic = q[c];
+  int cl[3] = {q[0], q[1], q[2]}, cr[3] = {q[0], q[1], q[2]};
+  cl[c] = ic - 1;
+  double v;
+  if (ic > 0 && ic < g.n) {
+    idx_t il = g.index(cl), ir = g.index(cr);
+    v = ((psi[ir] - psi[il]) / D) * ((a[ir] - a[il]) / D);
+  } else {
+    const bool low = ic == 0;
+    const int* cc = low ? cr : cl;
+    idx_t i = g.index(cc);
+    int side = 2 * c + (low ? 1 : 0);
+    const double* cv = bc.c[side];
+    double ghost = (cv ? cv[g.wall(c, cc)] : 0.) + (double)bc.sigma[side] * a[i];
+    v = -psi[i] * (((ghost - a[i]) / D) / D);
+  }
+  out[t] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lambda_grad(const double* __restrict__ a, const double* __restrict__ psi,
+                                                        double* __restrict__ out, size_t n) {
+  size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t < n) out[t] = -psi[t] * a[t];
+}
+
 // --- reductions -------------------------------------------------------
 
 // statistics slots written by the residual kernel

@@ -139,6 +139,20 @@ int bmg_poisson(bmg_grid *g, double tolerance, int nrelax, int minlevel,
                 bmg_mgstats *stats);
 int bmg_synchronize(bmg_grid *g);
 
+/* Gradient of an objective J(a) with respect to the coefficients (no
+   counterpart in the reference: the adjoint behind the torch.autograd
+   binding).  L(a) = lambda a + div(alpha grad a) is self-adjoint, so the
+   adjoint state psi is one more solve of the same operator with homogeneous
+   BCs and right-hand side dJ/da, and dJ/db = psi.  Given psi (finest level,
+   n^dim values) and the grid's current `a` and BCs, this writes
+     dJ/dalpha_f = (grad psi . grad a)_f on interior faces,
+                   -psi_i (ghost(a_i) - a_i)/Delta^2 on wall faces,
+     dJ/dlambda  = -psi a,
+   in the layout of bmg_set_field (face component c: n+1 along c).  NULL
+   outputs are skipped; periodic boundaries are refused. */
+int bmg_coef_gradient(bmg_grid *g, const double *psi, double *galpha_x, double *galpha_y,
+                      double *galpha_z, double *glambda, int on_device);
+
 /* kernel timing with HIP events on the grid's stream; kind 0 = fine-level
    relax, 1 = residual, 2 = restriction, 3 = prolongation, 4 = correction,
    5 = coarse-level relax */

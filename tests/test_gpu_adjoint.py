@@ -1,0 +1,120 @@
+"""GPU adjoint gradients through the torch.autograd binding.
+
+- FP64: the gradient w.r.t. b, alpha and lambda equals the CPU adjoint (the
+  oracle's red-black solves and the face formula) bit for bit;
+- mixed precision: within 1e-7 of the FP64 gradient;
+- the 1024^2 adjoint workload: central finite differences along 64 seeded
+  directions agree with <grad, delta> to 1e-6 of the largest directional
+  derivative (a per-direction ratio is meaningless for the directions whose
+  derivative happens to be near zero).
+"""
+import numpy as np
+import pytest
+import torch
+
+import basilisk_amd as bmg
+from basilisk_amd import configs
+from basilisk_amd.autograd import PoissonSolver
+from oracle.oracle import OracleGrid
+from tests import adjoint_cases as ac
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _torch(x, grad=True):
+    return torch.tensor(x, dtype=torch.float64, device=DEV, requires_grad=grad)
+
+
+def _gpu_grad(prob, precision, tol, recompute=False):
+    bc = {side: (kind, v) for side, (kind, v) in prob["bcs"].items()}
+    solver = PoissonSolver(prob["dim"], prob["depth"], bc=bc, precision=precision, smoother="rbgs",
+                           tolerance=tol, nitermax=200, recompute=recompute)
+    b = _torch(prob["b"])
+    alpha = [_torch(a) for a in prob["alpha"]]
+    lam = _torch(prob["lam"])
+    phi = solver(b, alpha, lam)
+    J = 0.5 * ((phi - torch.tensor(prob["phit"], device=DEV)) ** 2).sum()
+    J.backward()
+    return (phi.detach().cpu().numpy(), b.grad.cpu().numpy(), [a.grad.cpu().numpy() for a in alpha],
+            lam.grad.cpu().numpy())
+
+
+def _cpu_adjoint(prob, tol):
+    def solve(rhs, homogeneous):
+        g = OracleGrid(prob["dim"], prob["depth"])
+        g.set_smoother("rbgs")
+        ac.setup(g, prob, rhs, homogeneous)
+        g.set_niter(200, 1)
+        g.poisson(tolerance=tol)
+        return g.get_a()
+    phi = solve(prob["b"], False)
+    psi = solve(phi - prob["phit"], True)
+    ga, gl = ac.face_gradient(prob, phi, psi)
+    return phi, psi, ga, gl
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 6), (3, 4)])
+@pytest.mark.parametrize("recompute", [False, True])
+def test_fp64_gradient_equals_cpu_adjoint(dim, depth, recompute):
+    print("adjoint fp64", dim, depth, recompute, flush=True)
+    prob = ac.problem(dim, depth, seed=21)
+    phi, gb, ga, gl = _gpu_grad(prob, "fp64", 1e-10, recompute)
+    cphi, cpsi, cga, cgl = _cpu_adjoint(prob, 1e-10)
+    assert np.array_equal(phi, cphi)
+    assert np.array_equal(gb, cpsi)
+    for g, c in zip(ga, cga):
+        assert np.array_equal(g, c)
+    assert np.array_equal(gl, cgl)
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 7), (3, 5)])
+def test_mixed_gradient_close_to_fp64(dim, depth):
+    print("adjoint mixed", dim, depth, flush=True)
+    prob = ac.problem(dim, depth, seed=22)
+    _, gb64, ga64, gl64 = _gpu_grad(prob, "fp64", 1e-10)
+    _, gb32, ga32, gl32 = _gpu_grad(prob, "mixed", 1e-10)
+    assert np.abs(gb32 - gb64).max() <= 1e-7 * np.abs(gb64).max()
+    for g32, g64 in zip(ga32, ga64):
+        assert np.abs(g32 - g64).max() <= 1e-7 * np.abs(g64).max()
+    assert np.abs(gl32 - gl64).max() <= 1e-7 * np.abs(gl64).max()
+
+
+def test_adjoint_workload_finite_differences():
+    """The 1024^2 adjoint workload: log-normal alpha, 32-Gaussian source,
+    Dirichlet walls; gradients from the mixed-precision solver, J from FP64
+    solves; 64 seeded directions in (b, alpha)."""
+    print("adjoint workload fd", flush=True)
+    p = configs.config4(depth=10, seed=4)
+    src = torch.tensor(p["source"], device=DEV)
+    target = PoissonSolver(2, 10, bc=p["bc"], precision="fp64", tolerance=1e-11)
+    with torch.no_grad():
+        phit = target(src, None, None)
+    mixed = PoissonSolver(2, 10, bc=p["bc"], precision="mixed", tolerance=1e-10)
+    b = src.clone().requires_grad_(True)
+    alpha = [_torch(a) for a in p["alpha"]]
+    phi = mixed(b, alpha, None)
+    J = 0.5 * ((phi - phit) ** 2).sum()
+    J.backward()
+    gb, ga = b.grad, [a.grad for a in alpha]
+    fp64 = PoissonSolver(2, 10, bc=p["bc"], precision="fp64", tolerance=1e-11, nitermax=60)
+
+    def objective(bb, aa):
+        with torch.no_grad():
+            ph = fp64(bb, aa, None)
+            return 0.5 * float(((ph - phit) ** 2).sum())
+
+    gen = torch.Generator(device="cpu").manual_seed(64)
+    eps = 1e-6
+    fds, wants = [], []
+    for k in range(64):
+        db = torch.randn(b.shape, generator=gen, dtype=torch.float64).to(DEV)
+        da = [torch.randn(a.shape, generator=gen, dtype=torch.float64).to(DEV) for a in alpha]
+        wants.append(float((gb * db).sum()) + sum(float((g * d).sum()) for g, d in zip(ga, da)))
+        jp = objective(b.detach() + eps * db, [a.detach() + eps * d for a, d in zip(alpha, da)])
+        jm = objective(b.detach() - eps * db, [a.detach() - eps * d for a, d in zip(alpha, da)])
+        fds.append((jp - jm) / (2 * eps))
+    fds, wants = np.array(fds), np.array(wants)
+    err = np.abs(fds - wants).max() / np.abs(wants).max()
+    print("FD error %.3g of the largest directional derivative %.3g" % (err, np.abs(wants).max()), flush=True)
+    assert err <= 1e-6
