@@ -118,3 +118,151 @@ def config4(depth=10, seed=4):
     src -= src.mean()
     return {"dim": 2, "depth": depth, "alpha": alpha, "source": src, "tolerance": 1e-9,
             "bc": {side: (1, None) for side in range(4)}}
+
+
+# --- adaptive trees (configs 3 and 5) --------------------------------------
+
+def shell_refine(dim, depth, minlevel, centres, radii, band=2.0):
+    """refine() condition of the adaptive workloads: every cell up to
+    `minlevel`, and down to `depth` every cell whose centre lies within
+    band finest cells plus its own half-diagonal of a sphere's surface.
+    The cells of each level are enumerated per sphere over its bounding box,
+    so one pass costs O(shell cells), not O(leaves x spheres)."""
+    h = 1.0 / (1 << depth)
+    marks = {}
+
+    def level_keys(lev):
+        if lev in marks:
+            return marks[lev]
+        n = 1 << lev
+        size = 1.0 / n
+        tol = band * h + size * np.sqrt(dim)
+        keys = []
+        for c, r in zip(centres, radii):
+            lo = np.maximum(np.floor((c - r - tol) * n - 0.5).astype(np.int64), 0)
+            hi = np.minimum(np.ceil((c + r + tol) * n - 0.5).astype(np.int64), n - 1)
+            axes = [np.arange(lo[d], hi[d] + 1) for d in range(dim)]
+            grids = np.meshgrid(*axes, indexing="ij")
+            d2 = np.zeros(grids[0].shape)
+            for d in range(dim):
+                d2 += ((grids[d] + 0.5) * size - c[d]) ** 2
+            sel = np.abs(np.sqrt(d2) - r) < tol
+            key = np.zeros(int(sel.sum()), np.int64)
+            for d in range(dim):
+                key = key * n + grids[d][sel]
+            keys.append(key)
+        marks[lev] = np.unique(np.concatenate(keys)) if keys else np.zeros(0, np.int64)
+        return marks[lev]
+
+    def cond(x, y, z, lev):
+        lev = np.asarray(lev)
+        out = lev < minlevel
+        p = [x, y, z][:dim]
+        for l in np.unique(lev):
+            if l >= depth or l < minlevel:
+                continue
+            sel = lev == l
+            n = 1 << int(l)
+            key = np.zeros(int(sel.sum()), np.int64)
+            for d in range(dim):
+                key = key * n + np.floor(p[d][sel] * n).astype(np.int64)
+            k = level_keys(int(l))
+            pos = np.searchsorted(k, key)
+            pos[pos >= len(k)] = 0
+            out[np.nonzero(sel)[0]] = (len(k) > 0) & (k[pos] == key)
+        return out
+    return cond
+
+
+def inside_spheres(q, centres, radii, nbins=16):
+    """Whether each point of q (n, dim) lies inside any sphere; the points
+    are binned on an nbins^dim grid so each sphere tests only the points of
+    the bins its bounding box covers."""
+    dim = q.shape[1]
+    b = np.clip((q * nbins).astype(np.int64), 0, nbins - 1)
+    lin = np.zeros(len(q), np.int64)
+    for d in range(dim):
+        lin = lin * nbins + b[:, d]
+    order = np.argsort(lin, kind="stable")
+    bounds = np.searchsorted(lin[order], np.arange(nbins ** dim + 1))
+    inside = np.zeros(len(q), bool)
+    for c, r in zip(centres, radii):
+        lo = np.clip(((c - r) * nbins).astype(np.int64), 0, nbins - 1)
+        hi = np.clip(((c + r) * nbins).astype(np.int64), 0, nbins - 1)
+        for cell in np.ndindex(*[hi[d] - lo[d] + 1 for d in range(dim)]):
+            key = 0
+            for d in range(dim):
+                key = key * nbins + lo[d] + cell[d]
+            idx = order[bounds[key]:bounds[key + 1]]
+            if len(idx):
+                inside[idx] |= ((q[idx] - c) ** 2).sum(1) < r * r
+    return inside
+
+
+def _tree_rho_alpha(t, centres, radii, ratio):
+    """alpha = 1/rho on the tree's faces (face-cache order), arithmetic
+    average of rho on the two sides of each face, rho = ratio inside the
+    spheres and 1 outside."""
+    dim = t.dim
+    xyz, dd = t.face_centres()
+    f = t.faces()
+    nper = np.zeros(len(f), np.int64)
+    for d in range(dim):
+        nper += (f[:, 4] >> d) & 1
+    h = np.repeat(1.0 / (1 << f[:, 0].astype(np.int64)), nper)
+    p = xyz[:, :dim]
+
+    def rho(q):
+        return np.where(inside_spheres(q, centres, radii), ratio, 1.0)
+    left = p.copy()
+    right = p.copy()
+    idx = np.arange(len(p))
+    left[idx, dd] -= h / 2
+    right[idx, dd] += h / 2
+    return 1.0 / ((rho(left) + rho(right)) / 2.0)
+
+
+def config3_tree(t, seed=3, depth=11, minlevel=5, ncircles=40, ratio=1000.0, band=3.0):
+    """2D quadtree refined to `depth` around the interfaces of seeded
+    circles (radii U[0.01, 0.05]) from level `minlevel`; alpha = 1/rho
+    (density ratio 1000), lambda = 0; b = the 16 seeded Gaussians of config 1
+    at the leaf centres, minus their volume-weighted mean (symmetry walls).
+    `t` is an empty tree of the unit square (basilisk_amd.Tree or the
+    oracle's)."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.1, 0.9, (ncircles, 2))
+    radii = rng.uniform(0.01, 0.05, ncircles)
+    t.refine(shell_refine(2, depth, minlevel, centres, radii, band))
+    lev, xyz = t.leaves()
+    g1 = np.random.default_rng(1)
+    c = g1.uniform(0.1, 0.9, (16, 2))
+    w = g1.uniform(0.02, 0.05, 16)
+    A = g1.standard_normal(16)
+    b = np.zeros(len(lev))
+    for k in range(16):
+        b += A[k] * np.exp(-((xyz[:, 0] - c[k, 0]) ** 2 + (xyz[:, 1] - c[k, 1]) ** 2) / w[k] ** 2)
+    vol = (1.0 / (1 << lev.astype(np.int64))) ** 2
+    b -= (b * vol).sum() / vol.sum()
+    t.set_b(b)
+    t.set_a(np.zeros(len(b)))
+    t.set_lambda(0.0)
+    t.set_alpha_faces(_tree_rho_alpha(t, centres, radii, ratio))
+    return {"tolerance": 1e-9, "leaves": len(b)}
+
+
+def config5_tree(t, seed=5, depth=10, minlevel=5, nspheres=200, ratio=1000.0, band=2.0):
+    """3D octree refined to `depth` around seeded spheres (radii
+    U[0.005, 0.03]) from level `minlevel`; alpha = 1/rho (density ratio
+    1000); lambda = -1e-2 / h_min^2 (the Helmholtz shift of an implicit
+    step); b seeded N(0,1) at the leaves; symmetry walls."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.1, 0.9, (nspheres, 3))
+    radii = rng.uniform(0.005, 0.03, nspheres)
+    t.refine(shell_refine(3, depth, minlevel, centres, radii, band))
+    n = t.nleaves()
+    t.set_b(rng.standard_normal(n))
+    t.set_a(np.zeros(n))
+    hmin = 1.0 / (1 << depth)
+    t.set_lambda(-1e-2 / hmin ** 2)
+    t.set_alpha_faces(_tree_rho_alpha(t, centres, radii, ratio))
+    return {"tolerance": 1e-9, "leaves": n}

@@ -88,6 +88,7 @@ struct bmg_grid {
   // profiling
   bool prof = false;
   int rb_override = -1;  // BMG_RB_KERNEL: kernel for red-black levels of >= 4096 cells
+  int rb_reverse = 1;    // black passes walk the blocks backwards (BMG_RB_REVERSE=0: forwards)
   int grid_kind = BMG_GRID_MULTIGRID;  // ghost rule of edges and corners
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
@@ -268,13 +269,13 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // relax kernel of a level: two-launch in-place red-black (tiny levels),
 // one-launch fused red-black, single-pass streaming red-black (3D,
 // n >= 128) or Jacobi
-enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_JACOBI };
+enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_JACOBI };
 
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
   if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
   if (g->rb_override >= 0 && g->L[l].ncell >= 4096 &&
-      (g->rb_override != RK_STREAM_RB || (DIM == 3 && g->L[l].n >= 128)))
+      ((g->rb_override != RK_STREAM_RB && g->rb_override != RK_STREAM2_RB) || (DIM == 3 && g->L[l].n >= 128)))
     return g->rb_override;
   // measured at 256^3: the two-launch in-place kernel is the fastest on
   // large levels; the one-launch fused kernel halves the launches on
@@ -340,11 +341,34 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
         }
         out = src ^ 1;
         break;
+      case RK_STREAM2_RB:
+        if constexpr (DIM == 3) {
+          int chunk = stream_chunk(lv.n);
+          unsigned nb = (unsigned)((lv.n / kTK) * (lv.n / kTJ) * (lv.n / chunk));
+          const T* din = (const T*)lv.da[src];
+          T* dout = (T*)lv.da[src ^ 1];
+          const T* r = (const T*)lv.res;
+          bool af = k.lo[0] != nullptr, lf = k.lam != nullptr;
+          if (af && lf)
+            hipLaunchKernelGGL((k_rb_stream3b<T, true, true>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout, r,
+                               k, geo, bc, D2, chunk);
+          else if (af)
+            hipLaunchKernelGGL((k_rb_stream3b<T, true, false>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout, r,
+                               k, geo, bc, D2, chunk);
+          else if (lf)
+            hipLaunchKernelGGL((k_rb_stream3b<T, false, true>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout, r,
+                               k, geo, bc, D2, chunk);
+          else
+            hipLaunchKernelGGL((k_rb_stream3b<T, false, false>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout,
+                               r, k, geo, bc, D2, chunk);
+        }
+        out = src ^ 1;
+        break;
       default: {
         size_t half = lv.n > 1 ? lv.ncell / 2 : 1;
         for (int colour = 0; colour < 2; colour++)
           hipLaunchKernelGGL((k_relax_rb<DIM, T>), dim3(nblocks(half)), dim3(kBlock), 0, g->stream,
-                             (T*)lv.da[src], (const T*)lv.res, k, geo, bc, D2, colour);
+                             (T*)lv.da[src], (const T*)lv.res, k, geo, bc, D2, colour, colour & g->rb_reverse);
       }
     }
   }
@@ -796,7 +820,9 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
     if (!strcmp(e, "inplace")) g->rb_override = RK_INPLACE_RB;
     else if (!strcmp(e, "fused")) g->rb_override = RK_FUSED_RB;
     else if (!strcmp(e, "stream") && dimension == 3) g->rb_override = RK_STREAM_RB;
+    else if (!strcmp(e, "stream2") && dimension == 3) g->rb_override = RK_STREAM2_RB;
   }
+  if (const char* e = getenv("BMG_RB_REVERSE")) g->rb_reverse = e[0] != '0';
   int r = BMG_OK;
   // a blocking stream: it orders itself after work on the legacy default
   // stream, where callers (PyTorch included) usually prepare device inputs
@@ -880,6 +906,7 @@ int bmg_grid_free(bmg_grid* g) {
   prof_collect(g);
   for (auto e : g->ev_pool) (void)hipEventDestroy(e);
   if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
+  (void)hipGetLastError();  // teardown is best effort; leave no stale status
   delete g;
   return BMG_OK;
 }

@@ -230,8 +230,13 @@ __global__ __launch_bounds__(kBlock) void k_relax_rb_fused(const T* __restrict__
 // cell of that colour (colour = parity of i+j+k)
 template <int DIM, typename T>
 __global__ __launch_bounds__(kBlock) void k_relax_rb(T* __restrict__ da, const T* __restrict__ res,
-                                                     Coef<T> k, Geo<DIM> g, BCH bc, T D2, int colour) {
-  idx_t t = logical_block() * kBlock + threadIdx.x;
+                                                     Coef<T> k, Geo<DIM> g, BCH bc, T D2, int colour,
+                                                     int reverse) {
+  // reverse: walk the blocks from the far end, so that a pass starts on the
+  // data the previous pass touched last (still in the memory-side cache)
+  idx_t lb = logical_block();
+  if (reverse) lb = gridDim.x - 1 - lb;
+  idx_t t = lb * kBlock + threadIdx.x;
   idx_t half = g.n > 1 ? g.ncell() / 2 : (colour == 0 ? 1u : 0u);
   if (t >= half) return;
   idx_t idx;
@@ -963,6 +968,171 @@ __global__ __launch_bounds__(kBlock) void k_rb_stream3(const T* __restrict__ dol
       }
     }
     __syncthreads();
+  }
+}
+
+// --- streaming red-black sweep, 3D, colour-mapped threads ---------------
+//
+// The schedule and the arithmetic of k_rb_stream3 (red of plane s over the
+// tile and its 1-cell ring from old black values, black of plane s-1 over
+// the tile from the new red values), with two changes:
+//   - every thread owns cells of one colour: in the tile of 8 x 64 cells the
+//     pair (j, 2m), (j, 2m+1) -- one black, one red -- written out with one
+//     vector store, and up to two of the 330 red cells of the ring region;
+//     no thread slot is predicated off by the colour;
+//   - the coefficients of step s+1 are loaded into registers while step s is
+//     computed, so their latency overlaps the LDS work.
+
+constexpr int kRedRow = kRK1 / 2;                 // red cells per ring row (33)
+constexpr int kRedCells = kRJ1 * kRedRow;         // 330
+constexpr int kNRed2 = (kRedCells + kBlock - 1) / kBlock;  // 2
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+template <typename T, bool ALPHAF, bool LAMF>
+__global__ __launch_bounds__(kBlock) void k_rb_stream3b(const T* __restrict__ dold, T* __restrict__ dnew,
+                                                        const T* __restrict__ res, Coef<T> k, Geo<3> g, BCH bc,
+                                                        T D2, int chunk) {
+  __shared__ T buf[4][kRJ][kRK];
+  const int n = g.n, tid = threadIdx.x;
+  size_t b = logical_block();
+  const int nkt = n / kTK, njt = n / kTJ;
+  const int kt = (int)(b % nkt), jt = (int)((b / nkt) % njt), ct = (int)(b / ((size_t)nkt * njt));
+  const int i0 = ct * chunk, j0 = jt * kTJ, k0 = kt * kTK;
+  const int i1 = i0 + chunk;
+
+  auto inrange = [&](int d, int v) { return bc.periodic[d] || (v >= 0 && v < n); };
+  auto wrap = [&](int v) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+  auto slot = [&](int s) { return (s - i0 + 2) & 3; };
+
+  auto fetch_plane = [&](int s, T* v) {
+#pragma unroll
+    for (int m = 0; m < kNLoad; m++) {
+      int q = tid + m * kBlock;
+      v[m] = T(0);
+      if (q < kRJ * kRK && inrange(0, s)) {
+        int pj = q / kRK, pk = q - pj * kRK;
+        int gj = j0 + pj - 2, gk = k0 + pk - 2;
+        if (inrange(1, gj) && inrange(2, gk)) v[m] = dold[((size_t)wrap(s) * n + wrap(gj)) * n + wrap(gk)];
+      }
+    }
+  };
+  auto commit_plane = [&](int s, const T* v) {
+    if (!inrange(0, s)) return;
+    T(*P)[kRK] = buf[slot(s)];
+#pragma unroll
+    for (int m = 0; m < kNLoad; m++) {
+      int q = tid + m * kBlock;
+      if (q < kRJ * kRK) {
+        int pj = q / kRK, pk = q - pj * kRK;
+        P[pj][pk] = v[m];
+      }
+    }
+  };
+  // red cell m of this thread on plane s: row of the ring region and the k
+  // of the red parity (s + j + k even); false when outside the domain
+  auto red_cell = [&](int s, int m, int& gj, int& gk) {
+    int q = tid + m * kBlock;
+    if (q >= kRedCells) return false;
+    int r = q / kRedRow, mm = q - r * kRedRow;
+    gj = j0 - 1 + r;
+    const int kb = k0 - 1;
+    gk = kb + 2 * mm + ((s + gj + kb) & 1);
+    return inrange(1, gj) && inrange(2, gk);
+  };
+  // the pair of tile cells of this thread: row j, cells k and k+1
+  const int pr = tid / (kTK / 2), pm = tid - pr * (kTK / 2);
+  const int bj = j0 + pr, bk = k0 + 2 * pm;
+  // which of the pair is black on plane q: (q + bj + bk + e) odd
+  auto black_e = [&](int q) { return (q + bj + bk + 1) & 1; };
+
+  auto fetch_red = [&](int s, CellCoef<T>* cf) {
+#pragma unroll
+    for (int m = 0; m < kNRed2; m++) {
+      int gj, gk;
+      if (inrange(0, s) && red_cell(s, m, gj, gk)) {
+        int c[3] = {wrap(s), wrap(gj), wrap(gk)};
+        fetch_coef<T, ALPHAF, LAMF>(cf[m], res, k, g, c);
+      }
+    }
+  };
+  auto fetch_black = [&](int s, CellCoef<T>& cf) {
+    if (s >= i0 && s < i1) {
+      int c[3] = {s, bj, bk + black_e(s)};
+      fetch_coef<T, ALPHAF, LAMF>(cf, res, k, g, c);
+    }
+  };
+  auto relax_at = [&](int s, int gj, int gk, const CellCoef<T>& cf) -> T {
+    const int pj = gj - j0 + 2, pk = gk - k0 + 2;
+    const int sl = slot(s);
+    const T self = buf[sl][pj][pk];
+    T nn = -D2 * cf.res, dd = -cf.lam * D2;
+    T ap[3], am[3];
+    ap[0] = inrange(0, s + 1) ? buf[slot(s + 1)][pj][pk] : T(bc.sigma[0]) * self;
+    am[0] = inrange(0, s - 1) ? buf[slot(s - 1)][pj][pk] : T(bc.sigma[1]) * self;
+    ap[1] = inrange(1, gj + 1) ? buf[sl][pj + 1][pk] : T(bc.sigma[2]) * self;
+    am[1] = inrange(1, gj - 1) ? buf[sl][pj - 1][pk] : T(bc.sigma[3]) * self;
+    ap[2] = inrange(2, gk + 1) ? buf[sl][pj][pk + 1] : T(bc.sigma[4]) * self;
+    am[2] = inrange(2, gk - 1) ? buf[sl][pj][pk - 1] : T(bc.sigma[5]) * self;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      nn += cf.ah[d] * ap[d] + cf.al[d] * am[d];
+      dd += cf.ah[d] + cf.al[d];
+    }
+    return nn / dd;
+  };
+
+  T pre_da[kNLoad];
+  {
+    T tmp[kNLoad];
+    fetch_plane(i0 - 2, tmp);
+    commit_plane(i0 - 2, tmp);
+    fetch_plane(i0 - 1, tmp);
+    commit_plane(i0 - 1, tmp);
+  }
+  fetch_plane(i0, pre_da);
+  CellCoef<T> cur_red[kNRed2], cur_blk, nxt_red[kNRed2], nxt_blk;
+  fetch_red(i0 - 1, cur_red);
+  fetch_black(i0 - 2, cur_blk);
+  for (int s = i0 - 1; s <= i1; s++) {
+    commit_plane(s + 1, pre_da);
+    if (s + 2 <= i1 + 1) fetch_plane(s + 2, pre_da);
+    fetch_red(s + 1, nxt_red);
+    fetch_black(s, nxt_blk);
+    __syncthreads();
+    if (inrange(0, s)) {
+#pragma unroll
+      for (int m = 0; m < kNRed2; m++) {
+        int gj, gk;
+        if (red_cell(s, m, gj, gk)) buf[slot(s)][gj - j0 + 2][gk - k0 + 2] = relax_at(s, gj, gk, cur_red[m]);
+      }
+    }
+    __syncthreads();
+    const int q1 = s - 1;
+    if (q1 >= i0 && q1 < i1) {
+      const int e = black_e(q1);
+      T v[2];
+      v[e] = relax_at(q1, bj, bk + e, cur_blk);
+      v[1 - e] = buf[slot(q1)][bj - j0 + 2][bk + (1 - e) - k0 + 2];
+      using V2 = typename Vec2<T>::type;
+      V2 out;
+      out.x = v[0];
+      out.y = v[1];
+      *reinterpret_cast<V2*>(dnew + ((size_t)q1 * n + bj) * n + bk) = out;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kNRed2; m++) cur_red[m] = nxt_red[m];
+    cur_blk = nxt_blk;
   }
 }
 

@@ -21,8 +21,11 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <vector>
 
 #include "../../include/bmg.h"
@@ -340,13 +343,28 @@ struct bmg_tree {
   hipStream_t own_stream = nullptr, stream = nullptr;
   // field values kept across a recompilation, in leaf order
   std::vector<double> keep_a, keep_b, keep_lam;
+  // HIP graphs of the cycle (per nrelax, minlevel, maxlevel) and of the
+  // residual's kernels: the topology and every kernel argument are fixed
+  // until a setting or the mesh changes, when they are dropped
+  bool use_graphs = true;
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> cycle_graphs;
+  hipGraphExec_t residual_graph = nullptr;
 };
 
 namespace {
 
 size_t tsize(const bmg_tree* t) { return t->precision == BMG_MIXED ? sizeof(float) : sizeof(double); }
 
+void drop_graphs(bmg_tree* t) {
+  for (auto& kv : t->cycle_graphs)
+    if (kv.second) (void)hipGraphExecDestroy(kv.second);
+  t->cycle_graphs.clear();
+  if (t->residual_graph) (void)hipGraphExecDestroy(t->residual_graph);
+  t->residual_graph = nullptr;
+}
+
 void free_device(bmg_tree* t) {
+  drop_graphs(t);
   DevIndex& x = t->ix;
   for (void* p : {(void*)x.nbr, (void*)x.rec_self, (void*)x.rec_child0, (void*)x.rec_ring, (void*)x.colour,
                   (void*)x.leaf_gs, (void*)x.lvl_leaves, (void*)x.box, (void*)x.fops, (void*)x.hf, (void*)x.fr,
@@ -372,6 +390,9 @@ void free_device(bmg_tree* t) {
   t->stat = nullptr; t->part = t->scalar = nullptr;
   t->compiled = false;
   t->coef_ready = false;
+  // the frees above are best effort; do not leave their status behind for
+  // the next launch check
+  (void)hipGetLastError();
 }
 
 int dalloc_(void** p, size_t bytes) {
@@ -1034,7 +1055,7 @@ void restriction_res(bmg_tree* t) {
 
 // residual (TREE form) in FP64; res in the cycle precision as well
 template <typename T, int DIM>
-int residual(bmg_tree* t, double* maxres) {
+void residual_kernels(bmg_tree* t) {
   const DevIndex& x = t->ix;
   tree_boundary<double, DIM>(t, t->a, BK_A, -1);
   const double* al[3] = {nullptr, nullptr, nullptr};
@@ -1052,6 +1073,39 @@ int residual(bmg_tree* t, double* maxres) {
   const double* lam = t->lambda_field ? t->lam64 : nullptr;
   k_t_leaf_res<DIM, T><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(
       S, t->a, t->b, t->g[0], t->g[1], t->g[2], x.nbr, lam, t->lambda_c, t->res64, F<T>(t->res), t->stat);
+}
+
+// runs `launches` (kernel launches on t->stream only) as a captured graph,
+// instantiated on first use and replayed afterwards
+template <typename F>
+int run_graph(bmg_tree* t, hipGraphExec_t* slot, F launches) {
+  if (!t->use_graphs) return launches();
+  if (!*slot) {
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launches();
+    hipError_t e = hipStreamEndCapture(t->stream, &graph);
+    if (rc != BMG_OK) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return bmg::fail(BMG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return bmg::fail(BMG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    *slot = exec;
+  }
+  HIPCHECK(hipGraphLaunch(*slot, t->stream));
+  return BMG_OK;
+}
+
+template <typename T, int DIM>
+int residual(bmg_tree* t, double* maxres) {
+  TRY(run_graph(t, &t->residual_graph, [&]() {
+    residual_kernels<T, DIM>(t);
+    return BMG_OK;
+  }));
   TStat st;
   HIPCHECK(hipMemcpyAsync(&st, t->stat, sizeof st, hipMemcpyDeviceToHost, t->stream));
   HIPCHECK(hipStreamSynchronize(t->stream));
@@ -1122,9 +1176,8 @@ int coefficients(bmg_tree* t) {
 }
 
 template <typename T, int DIM>
-int mg_cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
+int cycle_kernels(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
   const DevIndex& x = t->ix;
-  TRY(coefficients<T, DIM>(t));
   restriction_res<T, DIM>(t);
   minlevel = std::min(minlevel, maxlevel);
   T* da = F<T>(t->da);
@@ -1143,6 +1196,14 @@ int mg_cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
   k_t_correct<T><<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, t->a, da);
   HIPCHECK(hipGetLastError());
   return BMG_OK;
+}
+
+template <typename T, int DIM>
+int mg_cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
+  TRY(coefficients<T, DIM>(t));
+  if (!t->use_graphs) return cycle_kernels<T, DIM>(t, nrelax, minlevel, maxlevel);
+  hipGraphExec_t& slot = t->cycle_graphs[std::make_tuple(nrelax, minlevel, maxlevel)];
+  return run_graph(t, &slot, [&]() { return cycle_kernels<T, DIM>(t, nrelax, minlevel, maxlevel); });
 }
 
 template <int DIM>
@@ -1257,6 +1318,8 @@ int bmg_tree_new(bmg_tree** out, int dimension, double L0, double X0, double Y0,
   t->X0[1] = Y0;
   t->X0[2] = Z0;
   t->topo.init(dimension);
+  const char* gr = getenv("BMG_GRAPHS");
+  t->use_graphs = !(gr && gr[0] == '0');
   *out = t;
   return BMG_OK;
 }
@@ -1266,6 +1329,7 @@ int bmg_tree_free(bmg_tree* t) {
   if (t->stream) (void)hipStreamSynchronize(t->stream);
   free_device(t);
   if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
+  (void)hipGetLastError();
   delete t;
   return BMG_OK;
 }
@@ -1389,6 +1453,7 @@ int bmg_tree_set_smoother(bmg_tree* t, int smoother) {
   if (smoother == BMG_RELAX_GS)
     return bmg::fail(BMG_ERR_UNSUPPORTED, "the in-place Gauss-Seidel smoother is sequential; use JACOBI or RBGS");
   t->smoother = smoother;
+  drop_graphs(t);
   return BMG_OK;
 }
 
@@ -1418,6 +1483,7 @@ int bmg_tree_set_bc(bmg_tree* t, int side, int kind, const double* values) {
     m += p;
   }
   t->bc_kind[side] = kind;
+  drop_graphs(t);
   t->bc_host[side].assign(m, 0.);
   if (values) std::copy(values, values + m, t->bc_host[side].begin());
   if (t->compiled) {
@@ -1435,6 +1501,7 @@ int bmg_tree_set_leaf_field(bmg_tree* t, int field, const double* values, int on
   if (field == BMG_F_LAMBDA) {
     t->lambda_field = true;
     t->coef_ready = false;
+    drop_graphs(t);
     return set_leaf_values(t, t->lam64, values, on_device);
   }
   return bmg::fail(BMG_ERR_ARG, "field %d cannot be set on a tree", field);
@@ -1474,6 +1541,7 @@ int bmg_tree_set_alpha_faces(bmg_tree* t, const double* values, int on_device) {
   dfree_(tmp);
   t->alpha_field = true;
   t->coef_ready = false;
+  drop_graphs(t);
   return BMG_OK;
 }
 
@@ -1482,6 +1550,7 @@ int bmg_tree_set_alpha_const(bmg_tree* t, double alpha) {
   t->alpha_field = false;
   t->alpha_c = alpha;
   t->coef_ready = false;
+  drop_graphs(t);
   return BMG_OK;
 }
 
@@ -1490,6 +1559,7 @@ int bmg_tree_set_lambda_const(bmg_tree* t, double lambda) {
   t->lambda_field = false;
   t->lambda_c = lambda;
   t->coef_ready = false;
+  drop_graphs(t);
   return BMG_OK;
 }
 

@@ -39,7 +39,78 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
+                    help="2: the 256^3 multigrid problem (default); 3: 2D adaptive quadtree; "
+                         "5: 3D adaptive octree")
     return ap.parse_args()
+
+
+def tree_bench(args):
+    """The adaptive-tree workloads: one step = one poisson() solve to the
+    tolerance on the tree (alpha/lambda restriction included), after the
+    tree and its slot tables are built (once per mesh, outside the timed
+    region)."""
+    import numpy as np
+    import torch
+    import basilisk_amd as bmg
+    from basilisk_amd import configs
+    make = configs.config3_tree if args.config == 3 else configs.config5_tree
+    dim = 2 if args.config == 3 else 3
+    res = {}
+    for prec in args.precisions.split(","):
+        t = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0), precision=prec, smoother=args.smoother)
+        t0 = time.perf_counter()
+        info = make(t)
+        t.set_niter(200, 1)
+        zeros = torch.zeros(info["leaves"], dtype=torch.float64, device="cuda:0")
+        t.set_a(zeros)
+        st = t.poisson(tolerance=info["tolerance"])  # first call also compiles the slot tables
+        build = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            t.set_a(zeros)
+            t.poisson(tolerance=info["tolerance"])
+        t.synchronize()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            t.set_a(zeros)
+            st = t.poisson(tolerance=info["tolerance"])
+        t.synchronize()
+        dt = (time.perf_counter() - t1) / args.steps
+        d, nl, nc = t.info()
+        res[prec] = {"seconds": dt, "stats": st, "a": t.get_a(), "build_s": build, "depth": d, "cells": nc,
+                     "leaves": nl}
+        print("# %s: %.2f ms, %d cycles, resa %.3g, %d leaves, setup %.1f s" % (prec, dt * 1e3, st["i"], st["resa"],
+              nl, build), file=sys.stderr, flush=True)
+        t.close()
+    main_prec = "mixed" if "mixed" in res else args.precisions.split(",")[0]
+    r = res[main_prec]
+    name = {3: "2D quadtree levels 5-11 around 40 circles", 5: "3D octree levels 5-10 around 200 spheres"}
+    line = {
+        "metric": "time-to-1e-9 residual, %s (density ratio 1000)" % name[args.config],
+        "value": round(r["seconds"] * 1e3, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r["seconds"] * 1e3, 3), "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32/f64" if main_prec == "mixed" else "f64",
+        "data": "synthetic (seeded circles/spheres, density ratio 1000)",
+        "config": {"workload": "%s, %d leaves, %d allocated cells, tolerance 1e-9, %s smoother"
+                               % (name[args.config], r["leaves"], r["cells"], args.smoother),
+                   "leaves": r["leaves"], "parallelism": "single GPU"},
+        "cycles": r["stats"]["i"],
+        "vcycles_per_s": round(r["stats"]["i"] / r["seconds"], 1),
+        "leaves_per_s": r["leaves"] * r["stats"]["i"] / r["seconds"],
+        "resa": r["stats"]["resa"],
+        "setup_s": round(r["build_s"], 2),
+        "roofline": None,
+    }
+    if "fp64" in res and "mixed" in res:
+        f, m = res["fp64"], res["mixed"]
+        line["fp64_ms"] = round(f["seconds"] * 1e3, 3)
+        line["fp64_cycles"] = f["stats"]["i"]
+        line["mixed_speedup_vs_fp64"] = round(f["seconds"] / m["seconds"], 3)
+        af, am = f["a"] - f["a"].mean(), m["a"] - m["a"].mean()
+        line["mixed_vs_fp64_rel_maxnorm"] = float(np.abs(am - af).max() / np.abs(af).max())
+    print(json.dumps(line), flush=True)
 
 
 def solve_once(g, zeros, tol):
@@ -111,6 +182,9 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.config != 2:
+        tree_bench(args)
+        return
     import numpy as np
     import torch
     import basilisk_amd as bmg

@@ -134,13 +134,14 @@ def test_random_problem_cycles_bitexact(bcs, dim, smoother):
     assert np.array_equal(g.get_a(), o.get_a())
 
 
-@pytest.mark.parametrize("kernel", ["default", "stream", "fused"])
+@pytest.mark.parametrize("kernel", ["default", "stream", "stream2", "fused"])
 @pytest.mark.parametrize("bcs", ["mixed", "periodic_all", "periodic_x"])
 @pytest.mark.parametrize("depth", [7, 8])
 def test_large_levels_bitexact(kernel, bcs, depth, monkeypatch):
     """Every red-black kernel of the large levels -- the two-launch in-place
-    one (default), the single-pass streaming one (plane ring in LDS, halo
-    recomputation) and the one-launch fused one (red neighbours recomputed
+    one (default), the single-pass streaming ones (plane ring in LDS, halo
+    recomputation; "stream2" with colour-mapped threads and prefetched
+    coefficients) and the one-launch fused one (red neighbours recomputed
     on the fly) -- and the fused residual/restriction kernel give the
     oracle's two-colour iterates bit for bit, periodic wrap included."""
     if kernel != "default":

@@ -124,3 +124,32 @@ def test_tree_errors():
     v = np.arange(g.nleaves(), dtype=float)
     g.set_b(v)
     assert np.array_equal(g.get_b(), v)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_match_eager(monkeypatch, precision):
+    """The cycle and the residual run as captured HIP graphs by default;
+    every replay (with eager host work in between) equals the eager launch
+    sequence bit for bit, also after the graph key (nrelax, levels) changes."""
+    print("graphs", precision, flush=True)
+    trees = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("BMG_GRAPHS", flag)
+        g = bmg.Tree(2, 1.0, ORIGIN, smoother="rbgs", precision=precision)
+        tree_cases.circle_setup(g, 7, True)
+        trees.append(g)
+    ga, ge = trees
+    assert ga.residual() == ge.residual()
+    for i in range(5):
+        ga.mg_cycle(4, 0, 7)
+        ge.mg_cycle(4, 0, 7)
+        assert ga.residual() == ge.residual()
+        assert np.array_equal(ga.get_a(), ge.get_a())
+    for nrelax, minlevel in ((3, 1), (4, 0), (2, 2)):
+        ga.mg_cycle(nrelax, minlevel, 7)
+        ge.mg_cycle(nrelax, minlevel, 7)
+        assert ga.residual() == ge.residual()
+    sa = ga.poisson(tolerance=1e-8)
+    se = ge.poisson(tolerance=1e-8)
+    assert sa["i"] == se["i"] and sa["resa"] == se["resa"]
+    assert np.array_equal(ga.get_a(), ge.get_a())

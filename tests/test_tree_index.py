@@ -63,3 +63,29 @@ def test_refine_marks_validated():
         g.refine_marked(np.ones(3, np.uint8))
     assert g.refine_marked(np.ones(1, np.uint8)) == 1
     assert g.nleaves() == 4
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 8), (3, 6)])
+def test_shell_refine_equals_direct_condition(dim, depth):
+    """The fast refinement condition of the adaptive workloads (cells
+    enumerated per sphere) builds the same tree as the direct per-leaf test
+    |dist - r| < band h + size sqrt(dim)."""
+    from basilisk_amd import configs
+    rng = np.random.default_rng(9)
+    c = rng.uniform(0.1, 0.9, (7, dim))
+    r = rng.uniform(0.02, 0.1, 7)
+    h = 1.0 / (1 << depth)
+
+    def direct(x, y, z, lev):
+        p = np.stack([x, y, z][:dim], axis=1)
+        size = 1.0 / (1 << lev.astype(np.int64))
+        near = np.zeros(len(x), bool)
+        for s in range(len(r)):
+            near |= np.abs(np.sqrt(((p - c[s]) ** 2).sum(1)) - r[s]) < 2.0 * h + size * np.sqrt(dim)
+        return (lev < 3) | ((lev < depth) & near)
+    a = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    b = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    a.refine(direct)
+    b.refine(configs.shell_refine(dim, depth, 3, c, r, 2.0))
+    assert a.nleaves() == b.nleaves()
+    assert np.array_equal(a.leaves()[1], b.leaves()[1])
