@@ -285,7 +285,15 @@ int level_kernel(const bmg_grid* g, int l) {
   return RK_INPLACE_RB;
 }
 
-int stream_chunk(int n) { return n >= 256 ? 16 : 8; }
+// planes per workgroup of the streaming kernels (BMG_STREAM_CHUNK overrides,
+// a power of two dividing n)
+int stream_chunk(int n) {
+  if (const char* e = getenv("BMG_STREAM_CHUNK")) {
+    int c = atoi(e);
+    if (c >= 4 && c <= n && (c & (c - 1)) == 0) return c;
+  }
+  return n >= 256 ? 16 : 8;
+}
 
 // largest level that the single-workgroup coarse kernel takes (<= 1024 cells)
 int coarse_top(const bmg_grid* g) {

@@ -298,6 +298,7 @@ struct DevIndex {
   int *fv_gs = nullptr, *fv_d = nullptr;
   double* fc_tmp = nullptr;
   int maxfc = 0;
+  TLevelInfo* lvinfo = nullptr;  // per-level extents, for the batched kernel
   std::vector<int> bc_off;  // per level offset into each side's face table
 };
 
@@ -347,6 +348,7 @@ struct bmg_tree {
   // residual's kernels: the topology and every kernel argument are fixed
   // until a setting or the mesh changes, when they are dropped
   bool use_graphs = true;
+  bool batching = true;  // small operations run batched in one workgroup (BMG_TREE_BATCH=0: off)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> cycle_graphs;
   hipGraphExec_t residual_graph = nullptr;
 };
@@ -368,7 +370,7 @@ void free_device(bmg_tree* t) {
   DevIndex& x = t->ix;
   for (void* p : {(void*)x.nbr, (void*)x.rec_self, (void*)x.rec_child0, (void*)x.rec_ring, (void*)x.colour,
                   (void*)x.leaf_gs, (void*)x.lvl_leaves, (void*)x.box, (void*)x.fops, (void*)x.hf, (void*)x.fr,
-                  (void*)x.fp, (void*)x.fc, (void*)x.fv_gs, (void*)x.fv_d, (void*)x.fc_tmp})
+                  (void*)x.fp, (void*)x.fc, (void*)x.fv_gs, (void*)x.fv_d, (void*)x.fc_tmp, (void*)x.lvinfo})
     dfree_(p);
   x = DevIndex();
   bool mixed = t->precision == BMG_MIXED;
@@ -836,6 +838,13 @@ int compile(bmg_tree* t) {
   TRY(upload(&x.fc, fc));
   TRY(upload(&x.fv_gs, fv_gs));
   TRY(upload(&x.fv_d, fv_d));
+  {
+    std::vector<TLevelInfo> li(D + 1);
+    for (int l = 0; l <= D; l++)
+      li[l] = {x.off[l], x.nA[l], x.abase[l], x.rec_off[l], x.n_nonleaf[l], x.n_prolong[l],
+               x.box_off[l], x.box_n[l], x.leaf_off[l], x.leaf_n[l]};
+    TRY(upload(&x.lvinfo, li));
+  }
   TRY(dalloc_((void**)&x.fc_tmp, sizeof(double) * std::max(1, x.maxfc)));
   // fields
   bool mixed = t->precision == BMG_MIXED;
@@ -991,22 +1000,6 @@ void prolong_records(bmg_tree* t, T* f, int l, bool halo) {
     k_t_prolong<T, DIM><<<nb_(n), kB, 0, t->stream>>>(x.rec_ring + (size_t)r0 * NR, x.rec_child0 + r0, n, f);
 }
 
-// tree_boundary_level ({f}, l) (l < 0: depth)
-template <typename T, int DIM>
-void tree_boundary(bmg_tree* t, T* f, int bk, int l) {
-  int depth = l < 0 ? t->ix.depth : l;
-  if (t->topo.full()) {
-    box<T>(t, f, bk, depth);
-    return;
-  }
-  for (int m = depth - 1; m >= 0; m--) restrict_level<T, DIM>(t, f, m);
-  box<T>(t, f, bk, 0);
-  for (int i = 0; i < depth; i++) {
-    prolong_records<T, DIM>(t, f, i, true);
-    box<T>(t, f, bk, i + 1);
-  }
-}
-
 template <typename T>
 RelaxCoef<T> relax_coef(bmg_tree* t) {
   RelaxCoef<T> k;
@@ -1038,26 +1031,149 @@ int relax(bmg_tree* t, int l) {
   return BMG_OK;
 }
 
-// restriction (res) of mg_cycle (tree_restriction)
+// Operations of at most kSmallOp elements are queued and run by one
+// workgroup (k_t_ops); a larger one first flushes the queue, then runs as its
+// own kernel, so the order of the chain is kept.
+constexpr long kSmallOp = 2048;
+
 template <typename T, int DIM>
-void restriction_res(bmg_tree* t) {
-  T* r = F<T>(t->res);
-  int D = t->ix.depth;
-  if (t->topo.full()) {
-    box<T>(t, r, BK_SYM, D);
-    for (int l = D - 1; l >= 0; l--) {
-      restrict_level<T, DIM>(t, r, l);
-      box<T>(t, r, BK_SYM, l);
+struct Batcher {
+  bmg_tree* t;
+  const DevIndex& x;
+  TOps P;
+  TreeArgs<T> A;
+  Batcher(bmg_tree* t_, T* f0, T* f1, T* f2) : t(t_), x(t_->ix) {
+    P.n = 0;
+    memset(&A, 0, sizeof A);
+    A.lv = x.lvinfo;
+    A.rec_self = x.rec_self;
+    A.rec_child0 = x.rec_child0;
+    A.rec_ring = x.rec_ring;
+    A.nbr = x.nbr;
+    A.lvl_leaves = x.lvl_leaves;
+    A.colour = x.colour;
+    A.box = x.box;
+    A.fld[0] = f0;
+    A.fld[1] = f1;
+    A.fld[2] = f2;
+    A.res = f1;
+    A.k = relax_coef<T>(t);
+    A.bc = bct(t);
+    A.L0 = t->L0;
+  }
+  bool small(long n) const { return t->batching && n <= kSmallOp; }
+  void flush() {
+    if (P.n) k_t_ops<T, DIM><<<1, kOpsBlock, 0, t->stream>>>(P, A);
+    P.n = 0;
+  }
+  void push(int code, int level, int arg, int field) {
+    if (P.n == kMaxOps) flush();
+    P.op[P.n++] = TOp{(unsigned char)code, (unsigned char)field, (signed char)arg, (unsigned char)level};
+  }
+  void box(int f, int bk, int l) {
+    int n = x.box_n[l];
+    if (!n) return;
+    if (small(n)) return push(TOP_BOX, l, bk, f);
+    flush();
+    k_t_box<T><<<nb_(n), kB, 0, t->stream>>>(x.box + x.box_off[l], n, A.fld[f], A.bc, bk, delta(t, l));
+  }
+  void restrict_level(int f, int l) {
+    int n = x.n_nonleaf[l];
+    if (!n) return;
+    if (small(n)) return push(TOP_RESTRICT, l, 0, f);
+    flush();
+    k_t_restrict<T, DIM><<<nb_(n), kB, 0, t->stream>>>(x.rec_self + x.rec_off[l], x.rec_child0 + x.rec_off[l], n,
+                                                       A.fld[f]);
+  }
+  void prolong(int f, int l, bool halo) {
+    int r0 = x.rec_off[l] + (halo ? x.n_nonleaf[l] : 0);
+    int n = halo ? x.n_prolong[l] : x.n_nonleaf[l];
+    if (!n) return;
+    if (small(n)) return push(TOP_PROLONG, l, halo ? 1 : 0, f);
+    flush();
+    const int NR = DIM == 3 ? 27 : 9;
+    k_t_prolong<T, DIM><<<nb_(n), kB, 0, t->stream>>>(x.rec_ring + (size_t)r0 * NR, x.rec_child0 + r0, n, A.fld[f]);
+  }
+  // tree_boundary_level ({f}, l) (l < 0: depth)
+  void tree_boundary(int f, int bk, int l) {
+    int depth = l < 0 ? x.depth : l;
+    if (t->topo.full()) {
+      box(f, bk, depth);
+      return;
     }
-  } else
-    tree_boundary<T, DIM>(t, r, BK_SYM, -1);
-}
+    for (int m = depth - 1; m >= 0; m--) restrict_level(f, m);
+    box(f, bk, 0);
+    for (int i = 0; i < depth; i++) {
+      prolong(f, i, true);
+      box(f, bk, i + 1);
+    }
+  }
+  long cells(int l) const {
+    long n = x.nA[l];
+    for (int m = 0; m < l; m++) n += x.leaf_n[m];
+    return n;
+  }
+  void fill(int l) {
+    if (small(cells(l))) return push(TOP_FILL, l, 0, 0);
+    flush();
+    Segs S = level_or_leaf(t, l);
+    if (S.first_block[S.n]) k_t_fill<T><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(S, A.fld[0], T(0));
+  }
+  int relax(int l) {
+    if (t->smoother != BMG_RELAX_JACOBI && t->smoother != BMG_RELAX_RBGS)
+      return bmg::fail(BMG_ERR_UNSUPPORTED,
+                       "the in-place Gauss-Seidel smoother is sequential; use JACOBI or RBGS");
+    const bool jac = t->smoother == BMG_RELAX_JACOBI;
+    if (small(cells(l))) {
+      if (jac) {
+        push(TOP_RELAX, l, -1, 0);
+        push(TOP_JACOBI, l, 0, 0);
+      } else {
+        push(TOP_RELAX, l, 0, 0);
+        for (int m = 0; m <= l; m++) box(0, BK_HOM, m);
+        push(TOP_RELAX, l, 1, 0);
+      }
+      return BMG_OK;
+    }
+    flush();
+    Segs S = level_or_leaf(t, l);
+    unsigned nbk = (unsigned)S.first_block[S.n];
+    if (!nbk) return BMG_OK;
+    T* da = A.fld[0];
+    if (jac) {
+      k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, A.fld[2], A.res, x.nbr, x.colour, A.k, -1);
+      k_t_jacobi<T><<<nbk, kB, 0, t->stream>>>(S, da, A.fld[2]);
+    } else {
+      k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, A.res, x.nbr, x.colour, A.k, 0);
+      for (int m = 0; m <= l; m++) box(0, BK_HOM, m);
+      flush();
+      k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, A.res, x.nbr, x.colour, A.k, 1);
+    }
+    return BMG_OK;
+  }
+  // restriction (res) of mg_cycle (tree_restriction), on field f
+  void restriction_res(int f) {
+    int D = x.depth;
+    if (t->topo.full()) {
+      box(f, BK_SYM, D);
+      for (int l = D - 1; l >= 0; l--) {
+        restrict_level(f, l);
+        box(f, BK_SYM, l);
+      }
+    } else
+      tree_boundary(f, BK_SYM, -1);
+  }
+};
 
 // residual (TREE form) in FP64; res in the cycle precision as well
 template <typename T, int DIM>
 void residual_kernels(bmg_tree* t) {
   const DevIndex& x = t->ix;
-  tree_boundary<double, DIM>(t, t->a, BK_A, -1);
+  {
+    Batcher<double, DIM> B(t, t->a, nullptr, nullptr);
+    B.tree_boundary(0, BK_A, -1);
+    B.flush();
+  }
   const double* al[3] = {nullptr, nullptr, nullptr};
   if (t->alpha_field)
     for (int d = 0; d < DIM; d++) al[d] = t->al64[d];
@@ -1178,21 +1294,22 @@ int coefficients(bmg_tree* t) {
 template <typename T, int DIM>
 int cycle_kernels(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
   const DevIndex& x = t->ix;
-  restriction_res<T, DIM>(t);
-  minlevel = std::min(minlevel, maxlevel);
   T* da = F<T>(t->da);
+  Batcher<T, DIM> B(t, da, F<T>(t->res), F<T>(t->tmp));
+  B.restriction_res(1);
+  minlevel = std::min(minlevel, maxlevel);
   for (int l = minlevel; l <= maxlevel; l++) {
-    if (l == minlevel) {
-      Segs S = level_or_leaf(t, l);
-      if (S.first_block[S.n]) k_t_fill<T><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(S, da, T(0));
-    } else
-      prolong_records<T, DIM>(t, da, l - 1, false);
-    tree_boundary<T, DIM>(t, da, BK_HOM, l);
+    if (l == minlevel)
+      B.fill(l);
+    else
+      B.prolong(0, l - 1, false);
+    B.tree_boundary(0, BK_HOM, l);
     for (int i = 0; i < nrelax; i++) {
-      TRY(relax<T, DIM>(t, l));
-      tree_boundary<T, DIM>(t, da, BK_HOM, l);
+      TRY(B.relax(l));
+      B.tree_boundary(0, BK_HOM, l);
     }
   }
+  B.flush();
   k_t_correct<T><<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, t->a, da);
   HIPCHECK(hipGetLastError());
   return BMG_OK;
@@ -1320,6 +1437,8 @@ int bmg_tree_new(bmg_tree** out, int dimension, double L0, double X0, double Y0,
   t->topo.init(dimension);
   const char* gr = getenv("BMG_GRAPHS");
   t->use_graphs = !(gr && gr[0] == '0');
+  const char* tb = getenv("BMG_TREE_BATCH");
+  t->batching = !(tb && tb[0] == '0');
   *out = t;
   return BMG_OK;
 }

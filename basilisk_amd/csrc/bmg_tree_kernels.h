@@ -80,11 +80,7 @@ __device__ __forceinline__ T bc_term(const BCT& bc, int bk, int side, int ti, do
 }
 
 template <typename T>
-__global__ __launch_bounds__(kB) void k_t_box(const BoxRule* __restrict__ R, int n, T* __restrict__ f, BCT bc, int bk,
-                                              double D) {
-  int r = blockIdx.x * kB + threadIdx.x;
-  if (r >= n) return;
-  const BoxRule q = R[r];
+__device__ __forceinline__ void t_box_one(const BoxRule& q, T* __restrict__ f, const BCT& bc, int bk, double D) {
   T s = q.nterm ? f[q.src] : T(0);
   T v;
   if (q.nterm == 1)
@@ -99,29 +95,38 @@ __global__ __launch_bounds__(kB) void k_t_box(const BoxRule* __restrict__ R, int
   f[q.ghost] = v;
 }
 
+template <typename T>
+__global__ __launch_bounds__(kB) void k_t_box(const BoxRule* __restrict__ R, int n, T* __restrict__ f, BCT bc, int bk,
+                                              double D) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const BoxRule q = R[r];
+  t_box_one<T>(q, f, bc, bk, D);
+}
+
 // restriction_average over the 2^dim children (child block in foreach_child
 // order) of each record
+template <typename T, int DIM>
+__device__ __forceinline__ void t_restrict_one(T* __restrict__ f, int self, int child0) {
+  const T* c = f + child0;
+  T sum = T(0);
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); m++) sum += c[m];
+  f[self] = sum / T(1 << DIM);
+}
+
 template <typename T, int DIM>
 __global__ __launch_bounds__(kB) void k_t_restrict(const int* __restrict__ self, const int* __restrict__ child0, int n,
                                                    T* __restrict__ f) {
   int r = blockIdx.x * kB + threadIdx.x;
   if (r >= n) return;
-  const T* c = f + child0[r];
-  T sum = T(0);
-#pragma unroll
-  for (int m = 0; m < (1 << DIM); m++) sum += c[m];
-  f[self[r]] = sum / T(1 << DIM);
+  t_restrict_one<T, DIM>(f, self[r], child0[r]);
 }
 
 // bilinear into the 2^dim children of each record from its 3^dim ring
 template <typename T, int DIM>
-__global__ __launch_bounds__(kB) void k_t_prolong(const int* __restrict__ ring, const int* __restrict__ child0, int n,
-                                                  T* __restrict__ f) {
-  int r = blockIdx.x * kB + threadIdx.x;
-  if (r >= n) return;
-  constexpr int NR = DIM == 2 ? 9 : 27;
-  const int* g = ring + (size_t)r * NR;
-  T* out = f + child0[r];
+__device__ __forceinline__ void t_prolong_one(const int* __restrict__ g, int child0, T* __restrict__ f) {
+  T* out = f + child0;
   if (DIM == 2) {
     T v[9];
 #pragma unroll
@@ -150,6 +155,15 @@ __global__ __launch_bounds__(kB) void k_t_prolong(const int* __restrict__ ring, 
               (T(27) * c000 + T(9) * (c100 + c010 + c001) + T(3) * (c110 + c101 + c011) + c111) / T(64);
         }
   }
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(kB) void k_t_prolong(const int* __restrict__ ring, const int* __restrict__ child0, int n,
+                                                  T* __restrict__ f) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  constexpr int NR = DIM == 2 ? 9 : 27;
+  t_prolong_one<T, DIM>(ring + (size_t)r * NR, child0[r], f);
 }
 
 // per-dimension coefficient pointers of relax (null = constant)
@@ -444,6 +458,110 @@ __global__ void k_t_sum_final(const double* __restrict__ part, int np, double* _
     double s = 0.;
     for (int i = 0; i < np; i++) s += part[i];
     *out = s;
+  }
+}
+
+// --- small operations batched into one workgroup --------------------------
+//
+// A tree cycle is a long chain of dependent operations, most of them on a
+// few hundred cells (the ghost cells of a level, the coarse levels).  Each
+// one as its own launch costs a couple of microseconds whatever its size, so
+// consecutive small operations are queued and run by one 1024-thread
+// workgroup, in order, with a barrier between them.  The operations and the
+// arithmetic are those of the standalone kernels above.
+
+enum { TOP_RESTRICT = 0, TOP_PROLONG = 1, TOP_BOX = 2, TOP_RELAX = 3, TOP_JACOBI = 4, TOP_FILL = 5 };
+constexpr int kOpsBlock = 1024;
+constexpr int kMaxOps = 250;
+
+struct TOp {
+  unsigned char code, field;
+  signed char arg;  // prolong: halo; box: bc kind; relax: colour (-1: Jacobi value to tmp)
+  unsigned char level;
+};
+
+struct TOps {
+  int n;
+  TOp op[kMaxOps];
+};
+
+// per-level extents of the index tables
+struct TLevelInfo {
+  int off, nA, abase, rec_off, n_nonleaf, n_prolong, box_off, box_n, leaf_off, leaf_n;
+};
+
+template <typename T>
+struct TreeArgs {
+  const TLevelInfo* lv;
+  const int *rec_self, *rec_child0, *rec_ring, *nbr, *lvl_leaves;
+  const uint8_t* colour;
+  const BoxRule* box;
+  T* fld[3];  // fields an operation can name (da, res, tmp; or `a` alone)
+  const T* res;
+  RelaxCoef<T> k;
+  BCT bc;
+  double L0;
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ void t_ops_cells(const TreeArgs<T>& A, const TOp& op) {
+  const int l = op.level;
+  T* da = A.fld[0];
+  for (int m = l; m >= 0; m--) {
+    const TLevelInfo L = A.lv[m];
+    const int n = m == l ? L.nA : L.leaf_n;
+    const double Dm = A.L0 * (1. / (1 << m));
+    for (int i = threadIdx.x; i < n; i += kOpsBlock) {
+      const int gs = m == l ? L.off + i : A.lvl_leaves[L.leaf_off + i];
+      const int ai = L.abase + (gs - L.off);
+      if (op.code == TOP_FILL) {
+        A.fld[op.field][gs] = T(0);
+      } else if (op.code == TOP_JACOBI) {
+        da[gs] = (da[gs] + T(2) * A.fld[2][gs]) / T(3);
+      } else {
+        if (op.arg >= 0 && A.colour[ai] != op.arg) continue;
+        const T D = (T)Dm;
+        T v = t_relax_value<T, DIM>(da, A.res, A.nbr, A.k, gs, ai, D * D);
+        if (op.arg < 0)
+          A.fld[2][gs] = v;
+        else
+          da[gs] = v;
+      }
+    }
+  }
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(kOpsBlock) void k_t_ops(TOps P, TreeArgs<T> A) {
+  constexpr int NR = DIM == 2 ? 9 : 27;
+  for (int o = 0; o < P.n; o++) {
+    const TOp op = P.op[o];
+    const TLevelInfo L = A.lv[op.level];
+    T* f = A.fld[op.field];
+    switch (op.code) {
+      case TOP_RESTRICT:
+        for (int r = threadIdx.x; r < L.n_nonleaf; r += kOpsBlock)
+          t_restrict_one<T, DIM>(f, A.rec_self[L.rec_off + r], A.rec_child0[L.rec_off + r]);
+        break;
+      case TOP_PROLONG: {
+        const int r0 = L.rec_off + (op.arg ? L.n_nonleaf : 0);
+        const int n = op.arg ? L.n_prolong : L.n_nonleaf;
+        for (int r = threadIdx.x; r < n; r += kOpsBlock)
+          t_prolong_one<T, DIM>(A.rec_ring + (size_t)(r0 + r) * NR, A.rec_child0[r0 + r], f);
+        break;
+      }
+      case TOP_BOX: {
+        const double D = A.L0 * (1. / (1 << op.level));
+        for (int r = threadIdx.x; r < L.box_n; r += kOpsBlock) {
+          const BoxRule q = A.box[L.box_off + r];
+          t_box_one<T>(q, f, A.bc, op.arg, D);
+        }
+        break;
+      }
+      default:
+        t_ops_cells<T, DIM>(A, op);
+    }
+    __syncthreads();
   }
 }
 
