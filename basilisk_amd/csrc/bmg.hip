@@ -269,7 +269,7 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // relax kernel of a level: two-launch in-place red-black (tiny levels),
 // one-launch fused red-black, single-pass streaming red-black (3D,
 // n >= 128) or Jacobi
-enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_JACOBI };
+enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_PAIR_RB, RK_JACOBI };
 
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
@@ -295,9 +295,14 @@ int stream_chunk(int n) {
   return n >= 256 ? 16 : 8;
 }
 
-// largest level that the single-workgroup coarse kernel takes (<= 1024 cells)
+// largest level that the single-workgroup coarse kernel takes (at most 4096
+// cells; BMG_COARSE_TOP overrides within that limit)
 int coarse_top(const bmg_grid* g) {
   int top = g->dim == 3 ? 3 : g->dim == 2 ? 5 : 10;
+  if (const char* e = getenv("BMG_COARSE_TOP")) {
+    int v = atoi(e), lim = g->dim == 3 ? 4 : g->dim == 2 ? 6 : 12;
+    if (v >= 0 && v <= lim) top = v;
+  }
   return std::min(top, g->depth);
 }
 
@@ -371,6 +376,11 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
                                r, k, geo, bc, D2, chunk);
         }
         out = src ^ 1;
+        break;
+      case RK_PAIR_RB:
+        for (int colour = 0; colour < 2; colour++)
+          hipLaunchKernelGGL((k_relax_rb_pair<DIM, T>), dim3(nblocks(lv.ncell / 2)), dim3(kBlock), 0, g->stream,
+                             (T*)lv.da[src], (const T*)lv.res, k, geo, bc, D2, colour, colour & g->rb_reverse);
         break;
       default: {
         size_t half = lv.n > 1 ? lv.ncell / 2 : 1;
@@ -449,7 +459,8 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
   for (int l = use_block ? top + 1 : minlevel; l <= maxlevel; l++) {
     Level& lv = g->L[l];
     // ping-pong sweeps start in da[nrelax & 1] so that they end in da[0]
-    int dst = level_kernel<DIM>(g, l) == RK_INPLACE_RB ? 0 : (nrelax & 1);
+    const int rk = level_kernel<DIM>(g, l);
+    int dst = (rk == RK_INPLACE_RB || rk == RK_PAIR_RB) ? 0 : (nrelax & 1);
     if (l == minlevel) {
       hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, (T*)lv.da[dst],
                          lv.ncell, T(0));
@@ -829,6 +840,7 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
     else if (!strcmp(e, "fused")) g->rb_override = RK_FUSED_RB;
     else if (!strcmp(e, "stream") && dimension == 3) g->rb_override = RK_STREAM_RB;
     else if (!strcmp(e, "stream2") && dimension == 3) g->rb_override = RK_STREAM2_RB;
+    else if (!strcmp(e, "pair") && dimension >= 2) g->rb_override = RK_PAIR_RB;
   }
   if (const char* e = getenv("BMG_RB_REVERSE")) g->rb_reverse = e[0] != '0';
   int r = BMG_OK;

@@ -44,6 +44,19 @@ struct BCA {
 // cell indices fit in 32 bits (at most 2^30 cells per level)
 using idx_t = uint32_t;
 
+// two adjacent values as one vector load/store
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+
 template <int DIM>
 struct Geo {
   int n, log2n;
@@ -255,6 +268,69 @@ __global__ __launch_bounds__(kBlock) void k_relax_rb(T* __restrict__ da, const T
   g.decode(idx, c);
   T self = da[idx];
   da[idx] = relax_value<DIM, T>(da, res, k, g, bc, idx, c, self, D2);
+}
+
+// In-place colour pass over pairs of adjacent cells along the last
+// dimension: every thread owns the aligned pair (k, k+1) of a row, loads the
+// pair and the neighbouring rows' pairs as vectors, and updates the one cell
+// of the pair that has the pass colour.  The arithmetic is relax_value's, in
+// the same order; only the memory access differs from k_relax_rb (contiguous
+// vector loads instead of every other element).
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_relax_rb_pair(T* __restrict__ da, const T* __restrict__ res,
+                                                          Coef<T> k, Geo<DIM> g, BCH bc, T D2, int colour,
+                                                          int reverse) {
+  using V2 = typename Vec2<T>::type;
+  idx_t lb = logical_block();
+  if (reverse) lb = gridDim.x - 1 - lb;
+  const idx_t p = lb * kBlock + threadIdx.x;
+  if (p >= g.ncell() / 2) return;
+  const idx_t i0 = 2 * p;
+  int c[3];
+  g.decode(i0, c);
+  const int e = (colour + c[0] + (DIM == 3 ? c[1] : 0)) & 1;  // the last coordinate of i0 is even
+  const idx_t t = i0 + e;
+  c[DIM - 1] += e;
+  const int n = g.n;
+  auto pick = [&](const T* f, idx_t at) {
+    V2 v = *reinterpret_cast<const V2*>(f + at);
+    return e ? v.y : v.x;
+  };
+  const V2 pr = *reinterpret_cast<const V2*>(da + i0);
+  const T self = e ? pr.y : pr.x;
+  const T lam = k.lam ? k.lam[t] : k.lam_c;
+  T nn = -D2 * pick(res, i0), dd = -lam * D2;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const idx_t s = g.stride(d);
+    const bool last = d == DIM - 1;
+    T ah, al;
+    if (k.lo[0]) {
+      if (!last) {
+        al = pick(k.lo[d], i0);
+        ah = c[d] < n - 1 ? pick(k.lo[d], i0 + s) : k.wall[d][g.wall(d, c)];
+      } else {
+        const V2 lp = *reinterpret_cast<const V2*>(k.lo[d] + i0);
+        al = e ? lp.y : lp.x;
+        ah = e == 0 ? lp.y : (c[d] < n - 1 ? k.lo[d][t + 1] : k.wall[d][g.wall(d, c)]);
+      }
+    } else {
+      ah = al = k.alpha;
+    }
+    T ap, am;
+    const T sp = T(bc.sigma[2 * d]) * self, sm = T(bc.sigma[2 * d + 1]) * self;
+    const idx_t wrap = (idx_t)(n - 1) * s;
+    if (!last) {
+      ap = c[d] < n - 1 ? pick(da, i0 + s) : (bc.periodic[d] ? pick(da, i0 - wrap) : sp);
+      am = c[d] > 0 ? pick(da, i0 - s) : (bc.periodic[d] ? pick(da, i0 + wrap) : sm);
+    } else {
+      ap = e == 0 ? pr.y : (c[d] < n - 1 ? da[t + 1] : (bc.periodic[d] ? da[t - wrap] : sp));
+      am = e == 1 ? pr.x : (c[d] > 0 ? da[t - 1] : (bc.periodic[d] ? da[t + wrap] : sm));
+    }
+    nn += ah * ap + al * am;
+    dd += ah + al;
+  }
+  da[t] = nn / dd;
 }
 
 // coarse = mean of the 2^DIM children, restriction_average
@@ -986,17 +1062,6 @@ __global__ __launch_bounds__(kBlock) void k_rb_stream3(const T* __restrict__ dol
 constexpr int kRedRow = kRK1 / 2;                 // red cells per ring row (33)
 constexpr int kRedCells = kRJ1 * kRedRow;         // 330
 constexpr int kNRed2 = (kRedCells + kBlock - 1) / kBlock;  // 2
-
-template <typename T>
-struct Vec2;
-template <>
-struct Vec2<float> {
-  using type = float2;
-};
-template <>
-struct Vec2<double> {
-  using type = double2;
-};
 
 template <typename T, bool ALPHAF, bool LAMF>
 __global__ __launch_bounds__(kBlock) void k_rb_stream3b(const T* __restrict__ dold, T* __restrict__ dnew,

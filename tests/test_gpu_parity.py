@@ -134,15 +134,16 @@ def test_random_problem_cycles_bitexact(bcs, dim, smoother):
     assert np.array_equal(g.get_a(), o.get_a())
 
 
-@pytest.mark.parametrize("kernel", ["default", "stream", "stream2", "fused"])
+@pytest.mark.parametrize("kernel", ["default", "stream", "stream2", "pair", "fused"])
 @pytest.mark.parametrize("bcs", ["mixed", "periodic_all", "periodic_x"])
 @pytest.mark.parametrize("depth", [7, 8])
 def test_large_levels_bitexact(kernel, bcs, depth, monkeypatch):
     """Every red-black kernel of the large levels -- the two-launch in-place
     one (default), the single-pass streaming ones (plane ring in LDS, halo
     recomputation; "stream2" with colour-mapped threads and prefetched
-    coefficients) and the one-launch fused one (red neighbours recomputed
-    on the fly) -- and the fused residual/restriction kernel give the
+    coefficients), the in-place one over vector pairs ("pair") and the
+    one-launch fused one (red neighbours recomputed on the fly) -- and the
+    fused residual/restriction kernel give the
     oracle's two-colour iterates bit for bit, periodic wrap included."""
     if kernel != "default":
         monkeypatch.setenv("BMG_RB_KERNEL", kernel)
@@ -162,6 +163,29 @@ def test_large_levels_bitexact(kernel, bcs, depth, monkeypatch):
         assert g.residual() == o.residual(), it
     assert np.array_equal(g.get_a(), o.get_a())
     assert np.array_equal(g.get_res(), o.get_res())
+
+
+@pytest.mark.parametrize("kernel", ["default", "pair"])
+@pytest.mark.parametrize("bcs", ["dirichlet", "periodic"])
+def test_large_levels_2d_bitexact(kernel, bcs, monkeypatch):
+    """2D levels above 4096 cells with the in-place kernels (one cell per
+    thread, or pairs of cells with vector loads) against the oracle."""
+    if kernel != "default":
+        monkeypatch.setenv("BMG_RB_KERNEL", kernel)
+    print("case large levels 2d", kernel, bcs, flush=True)
+    g, o = pair(2, 10, "rbgs")
+    lam = np.random.default_rng(4).uniform(-2.0, -0.5, (1 << 10,) * 2)
+    for grid in (g, o):
+        cases.random_problem(grid, 41, lam=lam, bcs=[cases.BC_DIRICHLET, cases.BC_NEUMANN] * 2)
+        if bcs == "periodic":
+            grid.set_periodic(1)
+        grid.restriction_alpha_lambda()
+    assert g.residual() == o.residual()
+    for it in range(2):
+        g.mg_cycle(2, 0)
+        o.mg_cycle(2, 0)
+        assert g.residual() == o.residual(), it
+    assert np.array_equal(g.get_a(), o.get_a())
 
 
 @pytest.mark.parametrize("dim", [2, 3])

@@ -153,3 +153,30 @@ def test_graph_replays_match_eager(monkeypatch, precision):
     se = ge.poisson(tolerance=1e-8)
     assert sa["i"] == se["i"] and sa["resa"] == se["resa"]
     assert np.array_equal(ga.get_a(), ge.get_a())
+
+
+TINY = {
+    "root": lambda x, y, z, lev: np.zeros(len(lev), bool),
+    "one_level": lambda x, y, z, lev: lev < 1,
+    "corner": lambda x, y, z, lev: (lev < 1) | ((lev < 3) & (x < 0.3) & (y < 0.3)),
+}
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("shape", list(TINY))
+def test_tiny_trees_bitexact(dim, shape):
+    """Degenerate trees: the root alone (depth 0), one full level, and one
+    refined corner three levels deep next to unrefined cells of level 1."""
+    print("tiny", dim, shape, flush=True)
+    g, o = _pair(dim, "rbgs", origin=(0.0, 0.0, 0.0))
+    stats = []
+    for t in (g, o):
+        t.refine(TINY[shape])
+        _, xyz = t.leaves()
+        t.set_b(np.sin(5 * xyz[:, 0]) + xyz[:, 1])
+        t.set_a(np.zeros(len(xyz)))
+        t.set_bc_function(bmg.LEFT, bmg.BC_DIRICHLET, lambda x, y, z: x + y)
+        stats.append(t.poisson(tolerance=1e-10))
+    for k in ("i", "resb", "resa", "nrelax"):
+        assert stats[0][k] == stats[1][k], k
+    assert np.array_equal(g.get_a(), o.get_a())
