@@ -269,17 +269,24 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // relax kernel of a level: two-launch in-place red-black (tiny levels),
 // one-launch fused red-black, single-pass streaming red-black (3D,
 // n >= 128) or Jacobi
-enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_PAIR_RB, RK_JACOBI };
+enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI };
 
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
   if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
+  // the column-streamed single-pass kernel: 3D levels of 256^3 cells and
+  // more (fewer planes give too few blocks), walls in every direction
+  const bool walls = !g->periodic[0] && !g->periodic[1] && !g->periodic[2];
+  const bool stream3 = DIM == 3 && g->L[l].n >= 256 && walls;
   if (g->rb_override >= 0 && g->L[l].ncell >= 4096 &&
-      ((g->rb_override != RK_STREAM_RB && g->rb_override != RK_STREAM2_RB) || (DIM == 3 && g->L[l].n >= 128)))
+      ((g->rb_override != RK_STREAM_RB && g->rb_override != RK_STREAM2_RB && g->rb_override != RK_STREAM3_RB) ||
+       (g->rb_override == RK_STREAM3_RB ? stream3 : DIM == 3 && g->L[l].n >= 128)))
     return g->rb_override;
-  // measured at 256^3: the two-launch in-place kernel is the fastest on
-  // large levels; the one-launch fused kernel halves the launches on
-  // mid-size levels, where launch latency dominates
+  // measured at 256^3: the single-pass column-streamed kernel reads each
+  // operand once and beats the two-launch in-place kernel, which is the
+  // fastest elsewhere on large levels; the one-launch fused kernel halves
+  // the launches on mid-size levels, where launch latency dominates
+  if (stream3 && g->rb_override < 0) return RK_STREAM3_RB;
   if (g->L[l].ncell >= (1u << 20)) return RK_INPLACE_RB;
   if (g->L[l].ncell >= 1024) return RK_FUSED_RB;
   return RK_INPLACE_RB;
@@ -293,6 +300,16 @@ int stream_chunk(int n) {
     if (c >= 4 && c <= n && (c & (c - 1)) == 0) return c;
   }
   return n >= 256 ? 16 : 8;
+}
+
+// plane chunks of the column-streamed kernel (BMG_STREAM_CHUNKS overrides,
+// 1..n/4): about two blocks per CU in FP32, one in FP64 (register bound)
+int stream_chunks(int n, size_t es) {
+  if (const char* e = getenv("BMG_STREAM_CHUNKS")) {
+    int c = atoi(e);
+    if (c >= 1 && c <= n / 4) return c;
+  }
+  return std::max(1, es == 4 ? n / 32 : n / 64);
 }
 
 // largest level that the single-workgroup coarse kernel takes (at most 4096
@@ -374,6 +391,33 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
           else
             hipLaunchKernelGGL((k_rb_stream3b<T, false, false>), dim3(nb), dim3(kBlock), 0, g->stream, din, dout,
                                r, k, geo, bc, D2, chunk);
+        }
+        out = src ^ 1;
+        break;
+      case RK_STREAM3_RB:
+        if constexpr (DIM == 3) {
+          const int nch = stream_chunks(lv.n, sizeof(T));
+          if (lv.n % kSTK || g->periodic[0] || g->periodic[1] || g->periodic[2]) {
+            *rc = fail(BMG_ERR_ARG, "internal: column-streamed sweep on level %d", l);
+            return src;
+          }
+          unsigned nb = (unsigned)((lv.n / kSTK) * (lv.n / kSTJ) * nch);
+          const T* din = (const T*)lv.da[src];
+          T* dout = (T*)lv.da[src ^ 1];
+          const T* r = (const T*)lv.res;
+          bool af = k.lo[0] != nullptr, lf = k.lam != nullptr;
+          if (af && lf)
+            hipLaunchKernelGGL((k_rb_stream3c<T, true, true>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout, r,
+                               k, geo, bc, D2, nch);
+          else if (af)
+            hipLaunchKernelGGL((k_rb_stream3c<T, true, false>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout,
+                               r, k, geo, bc, D2, nch);
+          else if (lf)
+            hipLaunchKernelGGL((k_rb_stream3c<T, false, true>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout,
+                               r, k, geo, bc, D2, nch);
+          else
+            hipLaunchKernelGGL((k_rb_stream3c<T, false, false>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout,
+                               r, k, geo, bc, D2, nch);
         }
         out = src ^ 1;
         break;
@@ -841,6 +885,7 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
     else if (!strcmp(e, "stream") && dimension == 3) g->rb_override = RK_STREAM_RB;
     else if (!strcmp(e, "stream2") && dimension == 3) g->rb_override = RK_STREAM2_RB;
     else if (!strcmp(e, "pair") && dimension >= 2) g->rb_override = RK_PAIR_RB;
+    else if (!strcmp(e, "stream3") && dimension == 3) g->rb_override = RK_STREAM3_RB;
   }
   if (const char* e = getenv("BMG_RB_REVERSE")) g->rb_reverse = e[0] != '0';
   int r = BMG_OK;

@@ -1201,6 +1201,340 @@ __global__ __launch_bounds__(kBlock) void k_rb_stream3b(const T* __restrict__ do
   }
 }
 
+// --- raw buffer access ---------------------------------------------------
+//
+// A buffer resource (base address, size in bytes) per array or plane; the
+// hardware range check makes a load at an offset outside [0, size) return
+// zero without touching memory, so loads need neither clamping nor branches,
+// and the per-lane address is one 32-bit offset shared by every array.
+
+using Rsrc = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// dword 3 of a gfx9 buffer resource: 32-bit data format
+constexpr int kBufWord3 = 0x00020000;
+// an offset past the end of every resource
+constexpr uint32_t kOOB = 0x80000000u;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufWord3);
+}
+template <typename T>
+__device__ __forceinline__ T bld1(Rsrc r, uint32_t off);
+template <>
+__device__ __forceinline__ float bld1<float>(Rsrc r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double bld1<double>(Rsrc r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type bld2(Rsrc r, uint32_t off);
+template <>
+__device__ __forceinline__ float2 bld2<float>(Rsrc r, uint32_t off) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  float2 o;
+  o.x = __builtin_bit_cast(float, v.x);
+  o.y = __builtin_bit_cast(float, v.y);
+  return o;
+}
+template <>
+__device__ __forceinline__ double2 bld2<double>(Rsrc r, uint32_t off) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+  double2 o;
+  o.x = __builtin_bit_cast(double, u32x2{v.x, v.y});
+  o.y = __builtin_bit_cast(double, u32x2{v.z, v.w});
+  return o;
+}
+__device__ __forceinline__ void bst2(Rsrc r, uint32_t off, float2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(u32x2{__builtin_bit_cast(unsigned, v.x), __builtin_bit_cast(unsigned, v.y)},
+                                        r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst2(Rsrc r, uint32_t off, double2 v) {
+  const u32x2 a = __builtin_bit_cast(u32x2, v.x), b = __builtin_bit_cast(u32x2, v.y);
+  __builtin_amdgcn_raw_buffer_store_b128(u32x4{a.x, a.y, b.x, b.y}, r, (int)off, 0, 0);
+}
+// bitwise or: combines two loads of which at most one was in range
+__device__ __forceinline__ float bor(float a, float b) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, a) | __builtin_bit_cast(unsigned, b));
+}
+__device__ __forceinline__ double bor(double a, double b) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, a) | __builtin_bit_cast(unsigned long long, b));
+}
+// --- end of raw buffer access
+
+// --- streaming red-black sweep, 3D, coefficients streamed per column ---
+//
+// One full red-black sweep in one pass, dold -> dnew, for levels with walls
+// in every direction (no periodic wrap).  The schedule is k_rb_stream3's:
+// step s relaxes the red cells of plane s, then the black cells of plane
+// s-1, from the field planes held in LDS (five slots, so that loading a plane
+// needs no barrier of its own).  The tile is 16 x 64 cells with one thread
+// per pair of cells (j, k), (j, k+1), k even.  On plane s the red cell of a
+// pair and on plane s-1 its black cell sit at the same k, so the coefficients
+// of a pair on plane p -- res, lambda and the faces, loaded as vectors --
+// serve the red cell on step p and the black cell on step p+1.  The red cells
+// of the 1-cell ring around the tile, which the black cells on the tile's
+// edge need, are relaxed by threads spread over all waves, with the
+// coefficients relax_value reads; like k_rb_stream3, blocks recompute them
+// instead of exchanging data.  Same arithmetic as relax_value.
+//
+// Everything a step reads from global memory is loaded during the step
+// before, through buffer resources of the plane (a plane outside the level
+// gets an empty resource), and the three coefficient records rotate their
+// roles through a loop unrolled by three, so that no register copy and no
+// branch makes the compiler wait for a load early.
+
+constexpr int kSTJ = 16, kSTK = 64, kSBlock = 512;
+constexpr int kSRJ = kSTJ + 4, kSRK = kSTK + 4;          // field region, 2-cell halo
+constexpr int kSNRing = kSTJ + kSTK + 2;                  // red ring cells per plane (82)
+constexpr int kSRingLanes = (kSNRing + kSBlock / 64 - 1) / (kSBlock / 64);  // ring cells per wave (11)
+constexpr int kSRegion2 = kSRJ * kSRK / 2;                // vector slots of the field region (680)
+constexpr int kSNLoad = (kSRegion2 + kSBlock - 1) / kSBlock;  // vector field loads per thread (2)
+static_assert(kSTJ * kSTK == 2 * kSBlock, "one thread per pair of tile cells");
+
+// coefficients of a pair of cells (k, k+1) on one plane p
+template <typename T>
+struct PairCoef {
+  typename Vec2<T>::type res, lam, ay, ayh, az, axl, axh;  // axh: x faces of plane p+1 (or the wall)
+  T azh;  // high z face of cell k+1: low face of cell k+2, or the wall
+};
+
+// buffer resources of the coefficient arrays on one plane
+struct PlaneRs {
+  Rsrc res, lam, l0, l1, l2, w1, w2, xh;
+};
+
+template <typename T>
+__device__ __forceinline__ T relax_nd(T D2, T res, T lam, const T al[3], const T ah[3], const T ap[3],
+                                      const T am[3]) {
+  T nn = -D2 * res, dd = -lam * D2;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    nn += ah[d] * ap[d] + al[d] * am[d];
+    dd += ah[d] + al[d];
+  }
+  return nn / dd;
+}
+
+// FP32: two blocks per CU (at most 128 registers); FP64: one
+template <typename T, bool ALPHAF, bool LAMF>
+__global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __restrict__ res, Coef<T> k,
+                   Geo<3> g, BCH bc, T D2, int nchunks) {
+  using V2 = typename Vec2<T>::type;
+  __shared__ __align__(16) T buf[5][kSRJ][kSRK];
+  const int n = g.n, tid = threadIdx.x;
+  const size_t b = logical_block();
+  const int nkt = n / kSTK, njt = n / kSTJ;
+  const int kt = (int)(b % nkt), jt = (int)((b / nkt) % njt), ct = (int)(b / ((size_t)nkt * njt));
+  const int i0 = (int)((size_t)ct * n / nchunks), i1 = (int)((size_t)(ct + 1) * n / nchunks);
+  const int j0 = jt * kSTJ, k0 = kt * kSTK;
+  // the pair of this thread: row bj, cells bk and bk+1
+  const int pr = tid >> 5, pm = tid & 31;
+  const int bj = j0 + pr, bk = k0 + 2 * pm;
+  const int pj = pr + 2, pk = 2 * pm + 2;  // LDS position of cell bk
+  auto slot = [&](int p) { return (p - i0 + 10) % 5; };
+  auto in = [&](int v) { return v >= 0 && v < n; };
+  constexpr uint32_t es = sizeof(T);
+  const size_t plane = (size_t)n * n;
+  const uint32_t pbytes = (uint32_t)(plane * es), rbytes = (uint32_t)n * es;
+  const uint32_t voff = ((uint32_t)bj * n + bk) * es;  // the pair within a plane
+  // resource of plane p of an n^3 array (empty outside the level)
+  auto prs = [&](const T* a, int p) { return make_rsrc(a + (in(p) ? p : 0) * plane, in(p) ? pbytes : 0u); };
+  // resource of row p of an n^2 wall array
+  auto wrs = [&](const T* a, int p) { return make_rsrc(a + (size_t)(in(p) ? p : 0) * n, in(p) ? rbytes : 0u); };
+  auto plane_rs = [&](int p) {
+    PlaneRs r;
+    r.res = prs(res, p);
+    if (LAMF) r.lam = prs(k.lam, p);
+    if (ALPHAF) {
+      r.l0 = prs(k.lo[0], p);
+      r.l1 = prs(k.lo[1], p);
+      r.l2 = prs(k.lo[2], p);
+      r.w1 = wrs(k.wall[1], p);
+      r.w2 = wrs(k.wall[2], p);
+      // high x faces: the next plane's low faces, or the wall (n x n, (j,k))
+      r.xh = p + 1 < n ? prs(k.lo[0], p + 1) : make_rsrc(k.wall[0], in(p) ? pbytes : 0u);
+    }
+    return r;
+  };
+
+  // field plane p over the tile and its 2-cell halo (zero outside the level)
+  auto fetch_plane = [&](int p, V2* v) {
+    const Rsrc r = prs(dold, p);
+#pragma unroll
+    for (int m = 0; m < kSNLoad; m++) {
+      const int q = tid + m * kSBlock;
+      const int row = q / (kSRK / 2), c2 = q - row * (kSRK / 2);
+      const int gj = j0 - 2 + row, gk = k0 - 2 + 2 * c2;
+      const bool ok = q < kSRegion2 && in(gj) && in(gk);
+      v[m] = bld2<T>(r, ok ? ((uint32_t)gj * n + gk) * es : kOOB);
+    }
+  };
+  auto commit_plane = [&](int p, const V2* v) {
+    if (!in(p)) return;
+    T(*P)[kSRK] = buf[slot(p)];
+#pragma unroll
+    for (int m = 0; m < kSNLoad; m++) {
+      const int q = tid + m * kSBlock;
+      if (q < kSRegion2) {
+        const int row = q / (kSRK / 2), c2 = q - row * (kSRK / 2);
+        *reinterpret_cast<V2*>(&P[row][2 * c2]) = v[m];
+      }
+    }
+  };
+  // coefficients of the pair on plane p
+  auto fetch_pair = [&](const PlaneRs& r, PairCoef<T>& c) {
+    c.res = bld2<T>(r.res, voff);
+    if (LAMF) c.lam = bld2<T>(r.lam, voff);
+    if (ALPHAF) {
+      const bool lastj = bj == n - 1, lastk = bk + 2 == n;
+      c.ay = bld2<T>(r.l1, voff);
+      const V2 yl = bld2<T>(r.l1, lastj ? kOOB : voff + rbytes), yw = bld2<T>(r.w1, lastj ? bk * es : kOOB);
+      c.ayh.x = bor(yl.x, yw.x);
+      c.ayh.y = bor(yl.y, yw.y);
+      c.az = bld2<T>(r.l2, voff);
+      c.azh = bor(bld1<T>(r.l2, lastk ? kOOB : voff + 2 * es), bld1<T>(r.w2, lastk ? bj * es : kOOB));
+      c.axl = bld2<T>(r.l0, voff);
+      c.axh = bld2<T>(r.xh, voff);
+    }
+  };
+  // ring cell of this thread on plane s (lanes 0..10 of every wave)
+  const int rq = (tid & 63) < kSRingLanes ? (tid & 63) * (kSBlock / 64) + (tid >> 6) : kSNRing;
+  auto ring_cell = [&](int s, int& gj, int& gk) {
+    if (rq < 2 * (kSTK / 2 + 1)) {
+      const int half = kSTK / 2 + 1;  // 33 red cells on a ring row
+      gj = rq < half ? j0 - 1 : j0 + kSTJ;
+      const int m = rq < half ? rq : rq - half;
+      gk = k0 - 1 + 2 * m + ((s + gj + k0 - 1) & 1);
+    } else {
+      const int qq = rq - 2 * (kSTK / 2 + 1);
+      gk = qq < kSTJ / 2 ? k0 - 1 : k0 + kSTK;
+      const int m = qq < kSTJ / 2 ? qq : qq - kSTJ / 2;
+      gj = j0 + 2 * m + ((s + j0 + gk) & 1);
+    }
+    return rq < kSNRing && in(gj) && in(gk);
+  };
+  // coefficients of this thread's ring cell on plane s (relax_value's reads)
+  auto fetch_ring = [&](int s, const PlaneRs& r, CellCoef<T>& cf) {
+    int gj, gk;
+    const bool ok = ring_cell(s, gj, gk);
+    const uint32_t off = ok ? ((uint32_t)gj * n + gk) * es : kOOB;
+    cf.res = bld1<T>(r.res, off);
+    cf.lam = LAMF ? bld1<T>(r.lam, off) : k.lam_c;
+    if (ALPHAF) {
+      cf.al[0] = bld1<T>(r.l0, off);
+      cf.ah[0] = bld1<T>(r.xh, off);
+      cf.al[1] = bld1<T>(r.l1, off);
+      cf.ah[1] = bor(bld1<T>(r.l1, ok && gj < n - 1 ? off + rbytes : kOOB),
+                     bld1<T>(r.w1, ok && gj == n - 1 ? gk * es : kOOB));
+      cf.al[2] = bld1<T>(r.l2, off);
+      cf.ah[2] = bor(bld1<T>(r.l2, ok && gk < n - 1 ? off + es : kOOB),
+                     bld1<T>(r.w2, ok && gk == n - 1 ? gj * es : kOOB));
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; d++) cf.al[d] = cf.ah[d] = k.alpha;
+    }
+  };
+  // neighbours of the cell at LDS (qj, qk) of plane s (global gj, gk); the
+  // LDS reads are unconditional, a ghost takes sigma times the cell
+  auto neighbours = [&](int s, int gj, int gk, int qj, int qk, T self, T* ap, T* am) {
+    const int sl = slot(s);
+    const T xp = buf[slot(s + 1)][qj][qk], xm = buf[slot(s - 1)][qj][qk];
+    const T yp = buf[sl][qj + 1][qk], ym = buf[sl][qj - 1][qk];
+    const T zp = buf[sl][qj][qk + 1], zm = buf[sl][qj][qk - 1];
+    ap[0] = in(s + 1) ? xp : T(bc.sigma[0]) * self;
+    am[0] = in(s - 1) ? xm : T(bc.sigma[1]) * self;
+    ap[1] = in(gj + 1) ? yp : T(bc.sigma[2]) * self;
+    am[1] = in(gj - 1) ? ym : T(bc.sigma[3]) * self;
+    ap[2] = in(gk + 1) ? zp : T(bc.sigma[4]) * self;
+    am[2] = in(gk - 1) ? zm : T(bc.sigma[5]) * self;
+  };
+  auto pick = [](V2 v, int e) { return e ? v.y : v.x; };
+  // relax cell e of the pair on plane s with the coefficients cp of plane s
+  auto relax_pair_cell = [&](int s, int e, const PairCoef<T>& cp) {
+    const int qk = pk + e;
+    const T self = buf[slot(s)][pj][qk];
+    T ap[3], am[3], al[3], ah[3];
+    neighbours(s, bj, bk + e, pj, qk, self, ap, am);
+    if (ALPHAF) {
+      al[0] = pick(cp.axl, e);
+      ah[0] = pick(cp.axh, e);
+      al[1] = pick(cp.ay, e);
+      ah[1] = pick(cp.ayh, e);
+      al[2] = pick(cp.az, e);
+      ah[2] = e ? cp.azh : cp.az.y;
+    } else {
+      al[0] = al[1] = al[2] = ah[0] = ah[1] = ah[2] = k.alpha;
+    }
+    return relax_nd<T>(D2, pick(cp.res, e), LAMF ? pick(cp.lam, e) : k.lam_c, al, ah, ap, am);
+  };
+
+  V2 dreg[kSNLoad];
+  // step s: red of plane s (coefficients cur, ring rcur), black of plane
+  // s-1 (coefficients prev) when `black`; loads plane s+2 of the field and
+  // the coefficients of plane s+1 (next, rnext)
+  auto step = [&](int s, bool black, const PairCoef<T>& prev, const PairCoef<T>& cur, PairCoef<T>& next,
+                  const CellCoef<T>& rcur, CellCoef<T>& rnext) {
+    commit_plane(s + 1, dreg);
+    fetch_plane(s + 2, dreg);
+    {
+      const PlaneRs r = plane_rs(s + 1);
+      fetch_pair(r, next);
+      fetch_ring(s + 1, r, rnext);
+    }
+    __syncthreads();
+    const int e = (s + bj + bk) & 1;
+    if (in(s)) {
+      buf[slot(s)][pj][pk + e] = relax_pair_cell(s, e, cur);
+      int gj, gk;
+      if (s >= i0 && s < i1 && ring_cell(s, gj, gk)) {
+        const int qj = gj - j0 + 2, qk = gk - k0 + 2;
+        const T self = buf[slot(s)][qj][qk];
+        T ap[3], am[3];
+        neighbours(s, gj, gk, qj, qk, self, ap, am);
+        buf[slot(s)][qj][qk] = relax_nd<T>(D2, rcur.res, rcur.lam, rcur.al, rcur.ah, ap, am);
+      }
+    }
+    __syncthreads();
+    if (black) {
+      // black of plane s-1 at the k of this step's red cell; the pair is
+      // written out with the red value of plane s-1
+      const int q1 = s - 1;
+      V2 out;
+      const T v = relax_pair_cell(q1, e, prev);
+      const T r = buf[slot(q1)][pj][pk + 1 - e];
+      out.x = e ? r : v;
+      out.y = e ? v : r;
+      bst2(make_rsrc(dnew + (size_t)q1 * plane, pbytes), voff, out);
+    }
+  };
+
+  // prologue: planes i0-2 and i0-1 in LDS, plane i0 in registers
+  {
+    V2 tmp[kSNLoad];
+    fetch_plane(i0 - 2, tmp);
+    commit_plane(i0 - 2, tmp);
+    fetch_plane(i0 - 1, tmp);
+    commit_plane(i0 - 1, tmp);
+  }
+  fetch_plane(i0, dreg);
+  PairCoef<T> A{}, B{}, C{};
+  CellCoef<T> RA{}, RB{}, RC{};
+  fetch_pair(plane_rs(i0 - 1), A);
+  step(i0 - 1, false, C, A, B, RC, RA);  // RA <- ring of plane i0
+  step(i0, false, A, B, C, RA, RB);
+  for (int s = i0 + 1; s <= i1; s += 3) {
+    step(s, true, B, C, A, RB, RC);
+    if (s + 1 > i1) break;
+    step(s + 1, true, C, A, B, RC, RA);
+    if (s + 2 > i1) break;
+    step(s + 2, true, A, B, C, RA, RB);
+  }
+}
+
 // --- all coarse levels in one workgroup ----------------------------------
 //
 // Levels minlevel..top of the cycle (each at most 4096 cells) run in one

@@ -134,14 +134,17 @@ def test_random_problem_cycles_bitexact(bcs, dim, smoother):
     assert np.array_equal(g.get_a(), o.get_a())
 
 
-@pytest.mark.parametrize("kernel", ["default", "stream", "stream2", "pair", "fused"])
+@pytest.mark.parametrize("kernel", ["default", "inplace", "stream", "stream2", "stream3", "pair", "fused"])
 @pytest.mark.parametrize("bcs", ["mixed", "periodic_all", "periodic_x"])
 @pytest.mark.parametrize("depth", [7, 8])
 def test_large_levels_bitexact(kernel, bcs, depth, monkeypatch):
     """Every red-black kernel of the large levels -- the two-launch in-place
-    one (default), the single-pass streaming ones (plane ring in LDS, halo
+    one ("inplace", the default below 256^3 and with periodic directions),
+    the single-pass streaming ones (plane ring in LDS, halo
     recomputation; "stream2" with colour-mapped threads and prefetched
-    coefficients), the in-place one over vector pairs ("pair") and the
+    coefficients; "stream3" with per-column coefficient vectors, walls
+    only, so the periodic cases run the default kernel), the in-place one
+    over vector pairs ("pair") and the
     one-launch fused one (red neighbours recomputed on the fly) -- and the
     fused residual/restriction kernel give the
     oracle's two-colour iterates bit for bit, periodic wrap included."""
@@ -251,6 +254,29 @@ def test_mixed_precision_converges_to_fp64_solution(dim):
     scale = np.abs(ref).max()
     assert np.abs(g64.get_a() - ref).max() == 0.0
     assert np.abs(gmx.get_a() - ref).max() <= 1e-9 * scale
+
+
+@pytest.mark.parametrize("kernel", ["inplace", "pair"])
+def test_mixed_precision_kernels_agree(kernel, monkeypatch):
+    """The FP32 smoother of the mixed path at 256^3: the default kernel (the
+    single-pass column-streamed sweep) and the two-colour in-place kernels
+    perform the same FP32 operations, so whole mixed-precision solves agree
+    bit for bit, statistics included."""
+    print("case mixed kernels", kernel, flush=True)
+    lam = np.random.default_rng(12).uniform(-2.0, -0.5, (256,) * 3)
+    out = []
+    for k in ("default", kernel):
+        if k != "default":
+            monkeypatch.setenv("BMG_RB_KERNEL", k)
+        g = bmg.Grid(3, 8, smoother="rbgs", precision="mixed")
+        cases.random_problem(g, 21, lam=lam, bcs=BC_MIXES["mixed"])
+        g.set_a(np.zeros(g.shape()))
+        st = g.poisson(tolerance=1e-9)
+        out.append((st, g.get_a()))
+    (s0, a0), (s1, a1) = out
+    for key in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert s0[key] == s1[key], key
+    assert np.array_equal(a0, a1)
 
 
 def test_nan_is_reported():
