@@ -68,6 +68,8 @@ struct bmg_grid {
   int NITERMAX = 100, NITERMIN = 1;
   std::vector<Level> L;
   double* a = nullptr;
+  double* a2 = nullptr;     // second buffer of a for the fused correction (allocated on first use)
+  bool pending_correct = false;  // the last cycle left a += da to the next residual
   double* b = nullptr;
   double* res64 = nullptr;  // FP64 residual on the finest level
   int bc_kind[6] = {0, 0, 0, 0, 0, 0};
@@ -277,7 +279,8 @@ int level_kernel(const bmg_grid* g, int l) {
   // the column-streamed single-pass kernel: 3D levels of 256^3 cells and
   // more (fewer planes give too few blocks), walls in every direction
   const bool walls = !g->periodic[0] && !g->periodic[1] && !g->periodic[2];
-  const bool stream3 = DIM == 3 && g->L[l].n >= 256 && walls;
+  // (buffer resources cover at most 2^31 bytes: up to 512^3 cells)
+  const bool stream3 = DIM == 3 && g->L[l].n >= 256 && g->L[l].ncell <= (1u << 27) && walls;
   if (g->rb_override >= 0 && g->L[l].ncell >= 4096 &&
       ((g->rb_override != RK_STREAM_RB && g->rb_override != RK_STREAM2_RB && g->rb_override != RK_STREAM3_RB) ||
        (g->rb_override == RK_STREAM3_RB ? stream3 : DIM == 3 && g->L[l].n >= 128)))
@@ -477,8 +480,18 @@ int coarse_block(bmg_grid* g, int nrelax, int minlevel, int top) {
   return check_launch();
 }
 
+// the residual kernel that also applies the correction and restricts:
+// 3D levels of whole 16 x 64 tiles (BMG_FUSED_RESIDUAL=0 turns it off)
+bool fused_residual(const bmg_grid* g) {
+  const Level& lf = g->L[g->depth];
+  if (g->dim != 3 || lf.n < 128 || lf.n % kSTK || lf.ncell > (1u << 27)) return false;
+  const char* e = getenv("BMG_FUSED_RESIDUAL");
+  return !(e && e[0] == '0');
+}
+
+// defer: leave a += da to the next residual (fused_residual grids only)
 template <int DIM, typename T>
-int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
+int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) {
   // restriction (res), poisson.h:42; levels below minlevel are never read
   // by the cycle and are not computed
   minlevel = std::min(minlevel, maxlevel);
@@ -524,12 +537,30 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
     if (cur != 0) return fail(BMG_ERR_ARG, "internal: sweep parity");
   }
   Level& lf = g->L[g->depth];
+  if (defer && maxlevel == g->depth) {
+    g->pending_correct = true;
+    return BMG_OK;
+  }
   {
     ProfScope ps(g, PROF_CORRECT);
     hipLaunchKernelGGL((k_correct<T>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, g->a,
                        (const T*)lf.da[0], lf.ncell);
   }
   return check_launch();
+}
+
+// launch k_corr_residual3 for the coefficient kinds
+template <typename TO, typename TD>
+void corr_residual3(bmg_grid* g, bool af, bool lf, unsigned nb, const TD* da, TO* res, double* r64, TO* coarse,
+                    const Coef<double>& k, const Geo<3>& g3, const BCA& bc, double D, int nch) {
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(kSBlock), 0, g->stream, g->a, da, g->a2, g->b, res, r64, coarse, k, g3,
+                       bc, D, nch, g->part);
+  };
+  if (af && lf) go(k_corr_residual3<TO, TD, true, true>);
+  else if (af) go(k_corr_residual3<TO, TD, true, false>);
+  else if (lf) go(k_corr_residual3<TO, TD, false, true>);
+  else go(k_corr_residual3<TO, TD, false, false>);
 }
 
 // residual on the finest level; writes res in the cycle precision (or FP64
@@ -559,9 +590,53 @@ int residual_t(bmg_grid* g, double* maxres, bool force64, unsigned long long* na
   int ex;
   bc.pow2 = std::frexp(D, &ex) == 0.5;
   bc.invD = 1. / D;
+  bool mixed = g->precision == BMG_MIXED && !force64;
+  if constexpr (DIM == 3) {
+    if (fused_residual(g)) {
+      // the pending correction, the residual and the first restriction in
+      // one streaming pass; a + da goes to a2, which then becomes a
+      const bool corr = g->pending_correct;
+      if (corr && !g->a2) TRY(dalloc((void**)&g->a2, lf.ncell * sizeof(double)));
+      // one block per CU (the kernel needs more than 128 registers): chunks
+      // of 64 planes at 256^3, of 8 planes at 128^3
+      const int nch = lf.n >= 256 ? lf.n / 64 : lf.n / 8;
+      const unsigned nb = (unsigned)((lf.n / kSTK) * (lf.n / kSTJ) * nch);
+      Geo<3> g3{lf.n, lf.log2n};
+      {
+        ProfScope ps(g, PROF_RESIDUAL);
+        void* coarse = force64 ? nullptr : g->L[g->depth - 1].res;
+        const bool af = k.lo[0] != nullptr, lamf = k.lam != nullptr;
+        if (mixed) {
+          const float* da = corr ? (const float*)lf.da[0] : nullptr;
+          double* r64 = keep64 ? g->res64 : nullptr;
+          corr_residual3<float, float>(g, af, lamf, nb, da, (float*)lf.res, r64, (float*)coarse, k, g3, bc, D, nch);
+        } else {
+          // FP64 cycle, or the FP64 residual of a mixed solve (no pending
+          // correction then: the cycles' residuals consumed it)
+          if (corr && g->precision == BMG_MIXED) return fail(BMG_ERR_ARG, "internal: pending FP32 correction");
+          const double* da = corr ? (const double*)lf.da[0] : nullptr;
+          corr_residual3<double, double>(g, af, lamf, nb, da, g->res64, nullptr, (double*)coarse, k, g3, bc, D, nch);
+        }
+        hipLaunchKernelGGL(k_reduce_stat, dim3(1), dim3(1024), 0, g->stream, g->part, (int)nb, g->stat);
+      }
+      TRY(check_launch());
+      if (corr) {
+        std::swap(g->a, g->a2);
+        g->pending_correct = false;
+      }
+      if (!force64) g->coarse_res_fresh = true;
+      Stat st;
+      HIPCHECK(hipMemcpyAsync(&st, g->stat, sizeof st, hipMemcpyDeviceToHost, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      double m;
+      memcpy(&m, &st.maxbits, sizeof m);
+      *maxres = m;
+      if (nans) *nans += st.nans;
+      return BMG_OK;
+    }
+  }
   // bricks of whole 2^DIM child groups: the fused residual + restriction
   bool fused = (DIM == 3 && lf.n >= 64) || (DIM == 2 && lf.n >= 128);
-  bool mixed = g->precision == BMG_MIXED && !force64;
   if (fused) {
     ProfScope ps(g, PROF_RESIDUAL);
     // at most 4096 blocks (a multiple of 8), each folding several bricks
@@ -673,13 +748,14 @@ int refresh_coef_copies(bmg_grid* g) {
 }
 
 // --- dispatch on dimension / precision
-int dispatch_cycle(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
+int dispatch_cycle(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer = false) {
   TRY(refresh_coef_copies(g));
   bool mixed = g->precision == BMG_MIXED;
+  defer = defer && fused_residual(g);
   switch (g->dim) {
-    case 1: return mixed ? mg_cycle_t<1, float>(g, nrelax, minlevel, maxlevel) : mg_cycle_t<1, double>(g, nrelax, minlevel, maxlevel);
-    case 2: return mixed ? mg_cycle_t<2, float>(g, nrelax, minlevel, maxlevel) : mg_cycle_t<2, double>(g, nrelax, minlevel, maxlevel);
-    default: return mixed ? mg_cycle_t<3, float>(g, nrelax, minlevel, maxlevel) : mg_cycle_t<3, double>(g, nrelax, minlevel, maxlevel);
+    case 1: return mixed ? mg_cycle_t<1, float>(g, nrelax, minlevel, maxlevel, false) : mg_cycle_t<1, double>(g, nrelax, minlevel, maxlevel, false);
+    case 2: return mixed ? mg_cycle_t<2, float>(g, nrelax, minlevel, maxlevel, false) : mg_cycle_t<2, double>(g, nrelax, minlevel, maxlevel, false);
+    default: return mixed ? mg_cycle_t<3, float>(g, nrelax, minlevel, maxlevel, defer) : mg_cycle_t<3, double>(g, nrelax, minlevel, maxlevel, defer);
   }
 }
 
@@ -769,7 +845,8 @@ int mg_solve_impl(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_m
   resb = s.resa = s.resb;
   if (tolerance == 0.) tolerance = g->TOLERANCE;
   for (s.i = 0; s.i < g->NITERMAX && (s.i < g->NITERMIN || s.resa > tolerance); s.i++) {
-    TRY(dispatch_cycle(g, s.nrelax, minlevel, g->depth));
+    // the cycle's correction of a is applied by the residual kernel
+    TRY(dispatch_cycle(g, s.nrelax, minlevel, g->depth, true));
     TRY(dispatch_residual(g, &s.resa, false, &nans));
     // nrelax adaptation, poisson.h:198-204
     if (s.resa > tolerance) {
@@ -961,6 +1038,7 @@ int bmg_grid_free(bmg_grid* g) {
     dfreet(lv.lam64);
   }
   dfreet(g->a);
+  dfreet(g->a2);
   dfreet(g->b);
   dfreet(g->res64);
   for (int b = 0; b < 6; b++) dfreet(g->bc_c[b]);

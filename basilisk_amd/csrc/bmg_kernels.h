@@ -1209,8 +1209,10 @@ __global__ __launch_bounds__(kBlock) void k_rb_stream3b(const T* __restrict__ do
 // and the per-lane address is one 32-bit offset shared by every array.
 
 using Rsrc = __amdgpu_buffer_rsrc_t;
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// the builtins' own vector types (GCC vector_size, not ext_vector_type):
+// values are moved in and out with bit casts only
+typedef unsigned int bv2 __attribute__((vector_size(8)));
+typedef unsigned int bv4 __attribute__((vector_size(16)));
 // dword 3 of a gfx9 buffer resource: 32-bit data format
 constexpr int kBufWord3 = 0x00020000;
 // an offset past the end of every resource
@@ -1219,41 +1221,33 @@ constexpr uint32_t kOOB = 0x80000000u;
 __device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufWord3);
 }
+// loads and stores at byte offset off (per lane) + soff (wave-uniform, e.g.
+// a plane); a resource covers at most 2^31 bytes, so kOOB + soff never wraps
 template <typename T>
-__device__ __forceinline__ T bld1(Rsrc r, uint32_t off);
+__device__ __forceinline__ T bld1(Rsrc r, uint32_t off, uint32_t soff = 0);
 template <>
-__device__ __forceinline__ float bld1<float>(Rsrc r, uint32_t off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+__device__ __forceinline__ float bld1<float>(Rsrc r, uint32_t off, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
 }
 template <>
-__device__ __forceinline__ double bld1<double>(Rsrc r, uint32_t off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+__device__ __forceinline__ double bld1<double>(Rsrc r, uint32_t off, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
 }
 template <typename T>
-__device__ __forceinline__ typename Vec2<T>::type bld2(Rsrc r, uint32_t off);
+__device__ __forceinline__ typename Vec2<T>::type bld2(Rsrc r, uint32_t off, uint32_t soff = 0);
 template <>
-__device__ __forceinline__ float2 bld2<float>(Rsrc r, uint32_t off) {
-  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
-  float2 o;
-  o.x = __builtin_bit_cast(float, v.x);
-  o.y = __builtin_bit_cast(float, v.y);
-  return o;
+__device__ __forceinline__ float2 bld2<float>(Rsrc r, uint32_t off, uint32_t soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
 }
 template <>
-__device__ __forceinline__ double2 bld2<double>(Rsrc r, uint32_t off) {
-  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
-  double2 o;
-  o.x = __builtin_bit_cast(double, u32x2{v.x, v.y});
-  o.y = __builtin_bit_cast(double, u32x2{v.z, v.w});
-  return o;
+__device__ __forceinline__ double2 bld2<double>(Rsrc r, uint32_t off, uint32_t soff) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0));
 }
-__device__ __forceinline__ void bst2(Rsrc r, uint32_t off, float2 v) {
-  __builtin_amdgcn_raw_buffer_store_b64(u32x2{__builtin_bit_cast(unsigned, v.x), __builtin_bit_cast(unsigned, v.y)},
-                                        r, (int)off, 0, 0);
+__device__ __forceinline__ void bst2(Rsrc r, uint32_t off, uint32_t soff, float2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bv2, v), r, (int)off, (int)soff, 0);
 }
-__device__ __forceinline__ void bst2(Rsrc r, uint32_t off, double2 v) {
-  const u32x2 a = __builtin_bit_cast(u32x2, v.x), b = __builtin_bit_cast(u32x2, v.y);
-  __builtin_amdgcn_raw_buffer_store_b128(u32x4{a.x, a.y, b.x, b.y}, r, (int)off, 0, 0);
+__device__ __forceinline__ void bst2(Rsrc r, uint32_t off, uint32_t soff, double2 v) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(bv4, v), r, (int)off, (int)soff, 0);
 }
 // bitwise or: combines two loads of which at most one was in range
 __device__ __forceinline__ float bor(float a, float b) {
@@ -1269,14 +1263,17 @@ __device__ __forceinline__ double bor(double a, double b) {
 // One full red-black sweep in one pass, dold -> dnew, for levels with walls
 // in every direction (no periodic wrap).  The schedule is k_rb_stream3's:
 // step s relaxes the red cells of plane s, then the black cells of plane
-// s-1, from the field planes held in LDS (five slots, so that loading a plane
-// needs no barrier of its own).  The tile is 16 x 64 cells with one thread
+// s-1, from the field planes held in LDS (eight slots, so that loading a
+// plane needs no barrier of its own).  The halo cells outside the level hold
+// their ghost values, sigma times the old value of the cell they mirror --
+// the value relax_value gives a ghost neighbour -- so that every neighbour is
+// a plain LDS read.  The tile is 16 x 64 cells with one thread
 // per pair of cells (j, k), (j, k+1), k even.  On plane s the red cell of a
 // pair and on plane s-1 its black cell sit at the same k, so the coefficients
 // of a pair on plane p -- res, lambda and the faces, loaded as vectors --
 // serve the red cell on step p and the black cell on step p+1.  The red cells
 // of the 1-cell ring around the tile, which the black cells on the tile's
-// edge need, are relaxed by threads spread over all waves, with the
+// edge need, are relaxed by the first 82 threads (two waves), with the
 // coefficients relax_value reads; like k_rb_stream3, blocks recompute them
 // instead of exchanging data.  Same arithmetic as relax_value.
 //
@@ -1289,7 +1286,8 @@ __device__ __forceinline__ double bor(double a, double b) {
 constexpr int kSTJ = 16, kSTK = 64, kSBlock = 512;
 constexpr int kSRJ = kSTJ + 4, kSRK = kSTK + 4;          // field region, 2-cell halo
 constexpr int kSNRing = kSTJ + kSTK + 2;                  // red ring cells per plane (82)
-constexpr int kSRingLanes = (kSNRing + kSBlock / 64 - 1) / (kSBlock / 64);  // ring cells per wave (11)
+constexpr int kSRingThreads = 128;                        // the waves that relax the ring
+static_assert(kSNRing <= kSRingThreads, "ring cells fit two waves");
 constexpr int kSRegion2 = kSRJ * kSRK / 2;                // vector slots of the field region (680)
 constexpr int kSNLoad = (kSRegion2 + kSBlock - 1) / kSBlock;  // vector field loads per thread (2)
 static_assert(kSTJ * kSTK == 2 * kSBlock, "one thread per pair of tile cells");
@@ -1301,10 +1299,6 @@ struct PairCoef {
   T azh;  // high z face of cell k+1: low face of cell k+2, or the wall
 };
 
-// buffer resources of the coefficient arrays on one plane
-struct PlaneRs {
-  Rsrc res, lam, l0, l1, l2, w1, w2, xh;
-};
 
 template <typename T>
 __device__ __forceinline__ T relax_nd(T D2, T res, T lam, const T al[3], const T ah[3], const T ap[3],
@@ -1324,7 +1318,7 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(sizeof(
 void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __restrict__ res, Coef<T> k,
                    Geo<3> g, BCH bc, T D2, int nchunks) {
   using V2 = typename Vec2<T>::type;
-  __shared__ __align__(16) T buf[5][kSRJ][kSRK];
+  __shared__ __align__(16) T buf[8][kSRJ][kSRK];
   const int n = g.n, tid = threadIdx.x;
   const size_t b = logical_block();
   const int nkt = n / kSTK, njt = n / kSTJ;
@@ -1335,74 +1329,89 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
   const int pr = tid >> 5, pm = tid & 31;
   const int bj = j0 + pr, bk = k0 + 2 * pm;
   const int pj = pr + 2, pk = 2 * pm + 2;  // LDS position of cell bk
-  auto slot = [&](int p) { return (p - i0 + 10) % 5; };
+  auto slot = [&](int p) { return (p - i0 + 16) & 7; };
   auto in = [&](int v) { return v >= 0 && v < n; };
+  auto clampn = [&](int v) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); };
   constexpr uint32_t es = sizeof(T);
   const size_t plane = (size_t)n * n;
   const uint32_t pbytes = (uint32_t)(plane * es), rbytes = (uint32_t)n * es;
   const uint32_t voff = ((uint32_t)bj * n + bk) * es;  // the pair within a plane
-  // resource of plane p of an n^3 array (empty outside the level)
-  auto prs = [&](const T* a, int p) { return make_rsrc(a + (in(p) ? p : 0) * plane, in(p) ? pbytes : 0u); };
-  // resource of row p of an n^2 wall array
-  auto wrs = [&](const T* a, int p) { return make_rsrc(a + (size_t)(in(p) ? p : 0) * n, in(p) ? rbytes : 0u); };
-  auto plane_rs = [&](int p) {
-    PlaneRs r;
-    r.res = prs(res, p);
-    if (LAMF) r.lam = prs(k.lam, p);
-    if (ALPHAF) {
-      r.l0 = prs(k.lo[0], p);
-      r.l1 = prs(k.lo[1], p);
-      r.l2 = prs(k.lo[2], p);
-      r.w1 = wrs(k.wall[1], p);
-      r.w2 = wrs(k.wall[2], p);
-      // high x faces: the next plane's low faces, or the wall (n x n, (j,k))
-      r.xh = p + 1 < n ? prs(k.lo[0], p + 1) : make_rsrc(k.wall[0], in(p) ? pbytes : 0u);
-    }
-    return r;
-  };
+  // one resource per array; a plane is selected by the uniform offset
+  const uint32_t vbytes = (uint32_t)((size_t)n * plane * es);
+  const Rsrc Rold = make_rsrc(dold, vbytes), Rnew = make_rsrc(dnew, vbytes), Rres = make_rsrc(res, vbytes);
+  const Rsrc Rlam = make_rsrc(LAMF ? k.lam : nullptr, LAMF ? vbytes : 0u);
+  const Rsrc Rl0 = make_rsrc(ALPHAF ? k.lo[0] : nullptr, ALPHAF ? vbytes : 0u);
+  const Rsrc Rl1 = make_rsrc(ALPHAF ? k.lo[1] : nullptr, ALPHAF ? vbytes : 0u);
+  const Rsrc Rl2 = make_rsrc(ALPHAF ? k.lo[2] : nullptr, ALPHAF ? vbytes : 0u);
+  const Rsrc Rw0 = make_rsrc(ALPHAF ? k.wall[0] : nullptr, ALPHAF ? pbytes : 0u);
+  const Rsrc Rw1 = make_rsrc(ALPHAF ? k.wall[1] : nullptr, ALPHAF ? pbytes : 0u);
+  const Rsrc Rw2 = make_rsrc(ALPHAF ? k.wall[2] : nullptr, ALPHAF ? pbytes : 0u);
+  // offsets of plane p (clamped into the level) in the n^3 arrays and of its
+  // row in the y and z walls
+  auto pso = [&](int p) { return (uint32_t)clampn(p) * pbytes; };
+  auto wso = [&](int p) { return (uint32_t)clampn(p) * rbytes; };
 
-  // field plane p over the tile and its 2-cell halo (zero outside the level)
+  // field plane p over the tile and its 2-cell halo; outside the level, the
+  // cells the ghosts mirror (plane, row and pair clamped into the level)
   auto fetch_plane = [&](int p, V2* v) {
-    const Rsrc r = prs(dold, p);
+    const uint32_t so = pso(p);
 #pragma unroll
     for (int m = 0; m < kSNLoad; m++) {
       const int q = tid + m * kSBlock;
       const int row = q / (kSRK / 2), c2 = q - row * (kSRK / 2);
-      const int gj = j0 - 2 + row, gk = k0 - 2 + 2 * c2;
-      const bool ok = q < kSRegion2 && in(gj) && in(gk);
-      v[m] = bld2<T>(r, ok ? ((uint32_t)gj * n + gk) * es : kOOB);
+      const int gj = clampn(j0 - 2 + row), gk = k0 - 2 + 2 * c2;
+      const int mk = gk < 0 ? 0 : (gk > n - 2 ? n - 2 : gk);
+      v[m] = bld2<T>(Rold, q < kSRegion2 ? ((uint32_t)gj * n + mk) * es : kOOB, so);
     }
   };
+  // ... stored with the ghost rule: sigma times the mirrored cell
   auto commit_plane = [&](int p, const V2* v) {
-    if (!in(p)) return;
     T(*P)[kSRK] = buf[slot(p)];
 #pragma unroll
     for (int m = 0; m < kSNLoad; m++) {
       const int q = tid + m * kSBlock;
       if (q < kSRegion2) {
         const int row = q / (kSRK / 2), c2 = q - row * (kSRK / 2);
-        *reinterpret_cast<V2*>(&P[row][2 * c2]) = v[m];
+        const int gj = j0 - 2 + row, gk = k0 - 2 + 2 * c2;
+        V2 w = v[m];
+        if (p < 0 || p >= n || gj < 0 || gj >= n || gk < 0 || gk >= n) {
+          const int sx = p < 0 ? 1 : 0, sy = gj < 0 ? 3 : 2;
+          if (p < 0 || p >= n) {
+            w.x = T(bc.sigma[sx]) * w.x;
+            w.y = T(bc.sigma[sx]) * w.y;
+          } else if (gj < 0 || gj >= n) {
+            w.x = T(bc.sigma[sy]) * w.x;
+            w.y = T(bc.sigma[sy]) * w.y;
+          } else if (gk < 0) {
+            w.y = T(bc.sigma[5]) * w.x;  // cell -1 mirrors cell 0
+          } else {
+            w.x = T(bc.sigma[4]) * w.y;  // cell n mirrors cell n-1
+          }
+        }
+        *reinterpret_cast<V2*>(&P[row][2 * c2]) = w;
       }
     }
   };
-  // coefficients of the pair on plane p
-  auto fetch_pair = [&](const PlaneRs& r, PairCoef<T>& c) {
-    c.res = bld2<T>(r.res, voff);
-    if (LAMF) c.lam = bld2<T>(r.lam, voff);
+  // coefficients of the pair on plane p (clamped into the level); the high
+  // x faces of the last plane are the wall (n x n, (j,k))
+  auto fetch_pair = [&](int p, PairCoef<T>& c) {
+    const uint32_t so = pso(p), sw = wso(p);
+    c.res = bld2<T>(Rres, voff, so);
+    if (LAMF) c.lam = bld2<T>(Rlam, voff, so);
     if (ALPHAF) {
-      const bool lastj = bj == n - 1, lastk = bk + 2 == n;
-      c.ay = bld2<T>(r.l1, voff);
-      const V2 yl = bld2<T>(r.l1, lastj ? kOOB : voff + rbytes), yw = bld2<T>(r.w1, lastj ? bk * es : kOOB);
+      const bool lastj = bj == n - 1, lastk = bk + 2 == n, lastp = clampn(p) == n - 1;
+      c.ay = bld2<T>(Rl1, voff, so);
+      const V2 yl = bld2<T>(Rl1, lastj ? kOOB : voff + rbytes, so), yw = bld2<T>(Rw1, lastj ? bk * es : kOOB, sw);
       c.ayh.x = bor(yl.x, yw.x);
       c.ayh.y = bor(yl.y, yw.y);
-      c.az = bld2<T>(r.l2, voff);
-      c.azh = bor(bld1<T>(r.l2, lastk ? kOOB : voff + 2 * es), bld1<T>(r.w2, lastk ? bj * es : kOOB));
-      c.axl = bld2<T>(r.l0, voff);
-      c.axh = bld2<T>(r.xh, voff);
+      c.az = bld2<T>(Rl2, voff, so);
+      c.azh = bor(bld1<T>(Rl2, lastk ? kOOB : voff + 2 * es, so), bld1<T>(Rw2, lastk ? bj * es : kOOB, sw));
+      c.axl = bld2<T>(Rl0, voff, so);
+      c.axh = bld2<T>(lastp ? Rw0 : Rl0, voff, lastp ? 0u : so + pbytes);
     }
   };
-  // ring cell of this thread on plane s (lanes 0..10 of every wave)
-  const int rq = (tid & 63) < kSRingLanes ? (tid & 63) * (kSBlock / 64) + (tid >> 6) : kSNRing;
+  // ring cell of this thread on plane s (threads 0..81)
+  const int rq = tid < kSNRing ? tid : kSNRing;
   auto ring_cell = [&](int s, int& gj, int& gk) {
     if (rq < 2 * (kSTK / 2 + 1)) {
       const int half = kSTK / 2 + 1;  // 33 red cells on a ring row
@@ -1418,47 +1427,44 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
     return rq < kSNRing && in(gj) && in(gk);
   };
   // coefficients of this thread's ring cell on plane s (relax_value's reads)
-  auto fetch_ring = [&](int s, const PlaneRs& r, CellCoef<T>& cf) {
+  auto fetch_ring = [&](int s, CellCoef<T>& cf) {
     int gj, gk;
     const bool ok = ring_cell(s, gj, gk);
     const uint32_t off = ok ? ((uint32_t)gj * n + gk) * es : kOOB;
-    cf.res = bld1<T>(r.res, off);
-    cf.lam = LAMF ? bld1<T>(r.lam, off) : k.lam_c;
+    const uint32_t so = pso(s), sw = wso(s);
+    const bool lastp = clampn(s) == n - 1;
+    cf.res = bld1<T>(Rres, off, so);
+    cf.lam = LAMF ? bld1<T>(Rlam, off, so) : k.lam_c;
     if (ALPHAF) {
-      cf.al[0] = bld1<T>(r.l0, off);
-      cf.ah[0] = bld1<T>(r.xh, off);
-      cf.al[1] = bld1<T>(r.l1, off);
-      cf.ah[1] = bor(bld1<T>(r.l1, ok && gj < n - 1 ? off + rbytes : kOOB),
-                     bld1<T>(r.w1, ok && gj == n - 1 ? gk * es : kOOB));
-      cf.al[2] = bld1<T>(r.l2, off);
-      cf.ah[2] = bor(bld1<T>(r.l2, ok && gk < n - 1 ? off + es : kOOB),
-                     bld1<T>(r.w2, ok && gk == n - 1 ? gj * es : kOOB));
+      cf.al[0] = bld1<T>(Rl0, off, so);
+      cf.ah[0] = bld1<T>(lastp ? Rw0 : Rl0, off, lastp ? 0u : so + pbytes);
+      cf.al[1] = bld1<T>(Rl1, off, so);
+      cf.ah[1] = bor(bld1<T>(Rl1, ok && gj < n - 1 ? off + rbytes : kOOB, so),
+                     bld1<T>(Rw1, ok && gj == n - 1 ? gk * es : kOOB, sw));
+      cf.al[2] = bld1<T>(Rl2, off, so);
+      cf.ah[2] = bor(bld1<T>(Rl2, ok && gk < n - 1 ? off + es : kOOB, so),
+                     bld1<T>(Rw2, ok && gk == n - 1 ? gj * es : kOOB, sw));
     } else {
 #pragma unroll
       for (int d = 0; d < 3; d++) cf.al[d] = cf.ah[d] = k.alpha;
     }
   };
-  // neighbours of the cell at LDS (qj, qk) of plane s (global gj, gk); the
-  // LDS reads are unconditional, a ghost takes sigma times the cell
-  auto neighbours = [&](int s, int gj, int gk, int qj, int qk, T self, T* ap, T* am) {
+  // neighbours of the cell at LDS (qj, qk) of plane s (ghosts included)
+  auto neighbours = [&](int s, int qj, int qk, T* ap, T* am) {
     const int sl = slot(s);
-    const T xp = buf[slot(s + 1)][qj][qk], xm = buf[slot(s - 1)][qj][qk];
-    const T yp = buf[sl][qj + 1][qk], ym = buf[sl][qj - 1][qk];
-    const T zp = buf[sl][qj][qk + 1], zm = buf[sl][qj][qk - 1];
-    ap[0] = in(s + 1) ? xp : T(bc.sigma[0]) * self;
-    am[0] = in(s - 1) ? xm : T(bc.sigma[1]) * self;
-    ap[1] = in(gj + 1) ? yp : T(bc.sigma[2]) * self;
-    am[1] = in(gj - 1) ? ym : T(bc.sigma[3]) * self;
-    ap[2] = in(gk + 1) ? zp : T(bc.sigma[4]) * self;
-    am[2] = in(gk - 1) ? zm : T(bc.sigma[5]) * self;
+    ap[0] = buf[slot(s + 1)][qj][qk];
+    am[0] = buf[slot(s - 1)][qj][qk];
+    ap[1] = buf[sl][qj + 1][qk];
+    am[1] = buf[sl][qj - 1][qk];
+    ap[2] = buf[sl][qj][qk + 1];
+    am[2] = buf[sl][qj][qk - 1];
   };
   auto pick = [](V2 v, int e) { return e ? v.y : v.x; };
   // relax cell e of the pair on plane s with the coefficients cp of plane s
   auto relax_pair_cell = [&](int s, int e, const PairCoef<T>& cp) {
     const int qk = pk + e;
-    const T self = buf[slot(s)][pj][qk];
     T ap[3], am[3], al[3], ah[3];
-    neighbours(s, bj, bk + e, pj, qk, self, ap, am);
+    neighbours(s, pj, qk, ap, am);
     if (ALPHAF) {
       al[0] = pick(cp.axl, e);
       ah[0] = pick(cp.axh, e);
@@ -1480,21 +1486,17 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
                   const CellCoef<T>& rcur, CellCoef<T>& rnext) {
     commit_plane(s + 1, dreg);
     fetch_plane(s + 2, dreg);
-    {
-      const PlaneRs r = plane_rs(s + 1);
-      fetch_pair(r, next);
-      fetch_ring(s + 1, r, rnext);
-    }
+    fetch_pair(s + 1, next);
+    if (tid < kSRingThreads) fetch_ring(s + 1, rnext);
     __syncthreads();
     const int e = (s + bj + bk) & 1;
     if (in(s)) {
       buf[slot(s)][pj][pk + e] = relax_pair_cell(s, e, cur);
       int gj, gk;
-      if (s >= i0 && s < i1 && ring_cell(s, gj, gk)) {
+      if (tid < kSRingThreads && s >= i0 && s < i1 && ring_cell(s, gj, gk)) {
         const int qj = gj - j0 + 2, qk = gk - k0 + 2;
-        const T self = buf[slot(s)][qj][qk];
         T ap[3], am[3];
-        neighbours(s, gj, gk, qj, qk, self, ap, am);
+        neighbours(s, qj, qk, ap, am);
         buf[slot(s)][qj][qk] = relax_nd<T>(D2, rcur.res, rcur.lam, rcur.al, rcur.ah, ap, am);
       }
     }
@@ -1508,7 +1510,7 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
       const T r = buf[slot(q1)][pj][pk + 1 - e];
       out.x = e ? r : v;
       out.y = e ? v : r;
-      bst2(make_rsrc(dnew + (size_t)q1 * plane, pbytes), voff, out);
+      bst2(Rnew, voff, (uint32_t)q1 * pbytes, out);
     }
   };
 
@@ -1523,7 +1525,7 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
   fetch_plane(i0, dreg);
   PairCoef<T> A{}, B{}, C{};
   CellCoef<T> RA{}, RB{}, RC{};
-  fetch_pair(plane_rs(i0 - 1), A);
+  fetch_pair(i0 - 1, A);
   step(i0 - 1, false, C, A, B, RC, RA);  // RA <- ring of plane i0
   step(i0, false, A, B, C, RA, RB);
   for (int s = i0 + 1; s <= i1; s += 3) {
@@ -1532,6 +1534,267 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
     step(s + 1, true, C, A, B, RC, RA);
     if (s + 2 > i1) break;
     step(s + 2, true, A, B, C, RA, RB);
+  }
+}
+
+// --- correction, residual and first restriction in one pass, 3D --------
+//
+// a_new = a + da on the finest level (the correction that ends mg_cycle,
+// poisson.h:84-89; none when da is null), the residual of a_new (as
+// residual_cell, poisson.h:373-389) and its restriction to the next level
+// (restriction_average, children in foreach_child order), streaming along i
+// like k_rb_stream3c: a block owns a 16 x 64 tile of (j,k) and a chunk of an
+// even number of planes.  a_new of planes p-1..p+1 sits in LDS with a 1-cell
+// halo, recomputed from a and da by every block that needs it, so blocks never
+// exchange data; a_new goes to a second array, never in place.  Periodic
+// directions wrap the halo.  Partial max |res| and NaN counts per block go to
+// `part` as in k_residual_restrict.
+
+constexpr int kRRJ = kSTJ + 2, kRRK = kSTK + 4;           // a_new region: 1-row halo, pairs from k0-2
+constexpr int kRRegion2 = kRRJ * kRRK / 2;                 // vector slots (612)
+constexpr int kRNLoad = (kRRegion2 + kSBlock - 1) / kSBlock;  // 2
+
+// what the residual of a pair of cells (k, k+1) on one plane reads besides a
+// (the x-wall values are read on the two wall planes only)
+struct ResCoef {
+  double2 b, lam, axl, axh, ay, ayh, az;
+  double azh;
+  double2 cy, cz;  // values of the y and z walls the cells touch (0 elsewhere)
+};
+
+template <typename TO>
+struct Vec2Of;
+template <>
+struct Vec2Of<float> {
+  using type = float2;
+};
+template <>
+struct Vec2Of<double> {
+  using type = double2;
+};
+
+template <typename TO, typename TD, bool ALPHAF, bool LAMF>
+__global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_corr_residual3(const double* __restrict__ a, const TD* __restrict__ da, double* __restrict__ anew,
+                      const double* __restrict__ b, TO* __restrict__ res, double* __restrict__ res64,
+                      TO* __restrict__ coarse, Coef<double> k, Geo<3> g, BCA bc, double D, int nchunks,
+                      Part* __restrict__ part) {
+  using TD2 = typename Vec2<TD>::type;
+  using TO2 = typename Vec2Of<TO>::type;
+  __shared__ __align__(16) double A[4][kRRJ][kRRK];
+  __shared__ double smax[kSBlock / 64];
+  __shared__ unsigned long long snan[kSBlock / 64];
+  const int n = g.n, tid = threadIdx.x;
+  const size_t blk = logical_block();
+  const int nkt = n / kSTK, njt = n / kSTJ;
+  const int kt = (int)(blk % nkt), jt = (int)((blk / nkt) % njt), ct = (int)(blk / ((size_t)nkt * njt));
+  // chunks of whole coarse planes
+  const int i0 = 2 * (int)((size_t)ct * (n / 2) / nchunks), i1 = 2 * (int)((size_t)(ct + 1) * (n / 2) / nchunks);
+  const int j0 = jt * kSTJ, k0 = kt * kSTK;
+  const int pr = tid >> 5, pm = tid & 31;
+  const int bj = j0 + pr, bk = k0 + 2 * pm;
+  const int qj = pr + 1, qk = 2 * pm + 2;  // LDS position of cell bk
+  const bool corr = da != nullptr;
+  const bool px = bc.periodic[0] != 0, py = bc.periodic[1] != 0, pz = bc.periodic[2] != 0;
+  auto slot = [&](int p) { return (p - i0 + 8) & 3; };
+  auto wrapv = [&](int v) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+  const size_t plane = (size_t)n * n;
+  const uint32_t pb8 = (uint32_t)(plane * 8), rb8 = (uint32_t)n * 8, vb8 = (uint32_t)(plane * n * 8);
+  const uint32_t cell0 = (uint32_t)bj * n + bk;  // the pair within a plane
+  auto clampn = [&](int v) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); };
+  // one resource per array (empty for a null array); a plane is selected by
+  // the uniform offset of plane p clamped into the level
+  auto arr3 = [&](const void* arr, uint32_t es) { return make_rsrc(arr, arr ? (uint32_t)(plane * n * es) : 0u); };
+  auto arr2 = [&](const double* arr) { return make_rsrc(arr, arr ? pb8 : 0u); };
+  const Rsrc Ra = arr3(a, 8), Rd = arr3(corr ? (const void*)da : nullptr, sizeof(TD)), Rb = arr3(b, 8);
+  const Rsrc Rlam = arr3(LAMF ? k.lam : nullptr, 8);
+  const Rsrc Rl0 = arr3(ALPHAF ? k.lo[0] : nullptr, 8), Rl1 = arr3(ALPHAF ? k.lo[1] : nullptr, 8);
+  const Rsrc Rl2 = arr3(ALPHAF ? k.lo[2] : nullptr, 8);
+  const Rsrc Rw0 = arr2(ALPHAF ? k.wall[0] : nullptr), Rw1 = arr2(ALPHAF ? k.wall[1] : nullptr);
+  const Rsrc Rw2 = arr2(ALPHAF ? k.wall[2] : nullptr);
+  const Rsrc Rc2 = arr2(py ? nullptr : bc.c[2]), Rc3 = arr2(py ? nullptr : bc.c[3]);
+  const Rsrc Rc4 = arr2(pz ? nullptr : bc.c[4]), Rc5 = arr2(pz ? nullptr : bc.c[5]);
+  (void)vb8;
+
+  // a (and da) of plane p over the tile and its 1-cell halo
+  auto fetch_region = [&](int p, double2* va, TD2* vd) {
+    const uint32_t pp = (uint32_t)clampn(px ? wrapv(p) : p);
+#pragma unroll
+    for (int m = 0; m < kRNLoad; m++) {
+      const int q = tid + m * kSBlock;
+      const int row = q / (kRRK / 2), c2 = q - row * (kRRK / 2);
+      const int gj = j0 - 1 + row, gk = k0 - 2 + 2 * c2;
+      const bool ok = q < kRRegion2 && (py || (gj >= 0 && gj < n)) && (pz || (gk >= 0 && gk < n));
+      const uint32_t c = (uint32_t)wrapv(gj) * n + wrapv(gk);
+      va[m] = bld2<double>(Ra, ok ? c * 8 : kOOB, pp * pb8);
+      vd[m] = bld2<TD>(Rd, ok ? c * (uint32_t)sizeof(TD) : kOOB, pp * (uint32_t)(plane * sizeof(TD)));
+    }
+  };
+  auto commit_region = [&](int p, const double2* va, const TD2* vd) {
+    if (!px && (p < 0 || p >= n)) return;
+    double(*P)[kRRK] = A[slot(p)];
+#pragma unroll
+    for (int m = 0; m < kRNLoad; m++) {
+      const int q = tid + m * kSBlock;
+      if (q < kRRegion2) {
+        const int row = q / (kRRK / 2), c2 = q - row * (kRRK / 2);
+        double2 v = va[m];
+        if (corr) {
+          v.x += (double)vd[m].x;
+          v.y += (double)vd[m].y;
+        }
+        *reinterpret_cast<double2*>(&P[row][2 * c2]) = v;
+      }
+    }
+  };
+  // the coefficients of the pair on plane p (clamped into the level)
+  auto fetch_coef3 = [&](int p, ResCoef& c) {
+    const uint32_t off = cell0 * 8, so = (uint32_t)clampn(p) * pb8, sw = (uint32_t)clampn(p) * rb8;
+    c.b = bld2<double>(Rb, off, so);
+    if (LAMF) c.lam = bld2<double>(Rlam, off, so);
+    if (ALPHAF) {
+      const bool lastj = bj == n - 1, lastk = bk + 2 == n, lastp = clampn(p) == n - 1;
+      c.axl = bld2<double>(Rl0, off, so);
+      c.axh = bld2<double>(lastp ? Rw0 : Rl0, off, lastp ? 0u : so + pb8);
+      c.ay = bld2<double>(Rl1, off, so);
+      const double2 yl = bld2<double>(Rl1, lastj ? kOOB : off + rb8, so);
+      const double2 yw = bld2<double>(Rw1, lastj ? bk * 8 : kOOB, sw);
+      c.ayh.x = bor(yl.x, yw.x);
+      c.ayh.y = bor(yl.y, yw.y);
+      c.az = bld2<double>(Rl2, off, so);
+      c.azh = bor(bld1<double>(Rl2, lastk ? kOOB : off + 16, so), bld1<double>(Rw2, lastk ? bj * 8 : kOOB, sw));
+    }
+    // boundary values: y walls are rows of lanes, z walls single cells
+    const bool yp = !py && bj == n - 1, ym = !py && bj == 0;
+    const double2 y1 = bld2<double>(Rc2, yp ? bk * 8 : kOOB, sw);
+    const double2 y2 = bld2<double>(Rc3, ym ? bk * 8 : kOOB, sw);
+    c.cy.x = bor(y1.x, y2.x);
+    c.cy.y = bor(y1.y, y2.y);
+    c.cz.x = bld1<double>(Rc5, !pz && bk == 0 ? bj * 8 : kOOB, sw);
+    c.cz.y = bld1<double>(Rc4, !pz && bk + 1 == n - 1 ? bj * 8 : kOOB, sw);
+  };
+  auto pick = [](double2 v, int e) { return e ? v.y : v.x; };
+  auto term = [&](double r, double al, double ah, double a0, double am, double ap) {
+    if (bc.pow2) return r + (al * ((a0 - am) * bc.invD) - ah * ((ap - a0) * bc.invD)) * bc.invD;
+    return r + (al * ((a0 - am) / D) - ah * ((ap - a0) / D)) / D;
+  };
+
+  double macc = 0.;
+  unsigned long long nacc = 0;
+  TO acc = TO(0);
+  double2 ra[kRNLoad];
+  TD2 rd[kRNLoad];
+  // step p: residual of plane p (coefficients cur); loads the region of
+  // plane p+2 and the coefficients of plane p+1 (next)
+  auto step = [&](int p, const ResCoef& cur, ResCoef& next) {
+    commit_region(p + 1, ra, rd);
+    fetch_region(p + 2, ra, rd);
+    fetch_coef3(p + 1, next);
+    // the x-wall values, on the two wall planes only
+    double2 cx;
+    cx.x = cx.y = 0.;
+    if (!px && (p == 0 || p == n - 1)) cx = bld2<double>(arr2(p == n - 1 ? bc.c[0] : bc.c[1]), cell0 * 8);
+    __syncthreads();
+    const int s = slot(p), sp = slot(p + 1), sm = slot(p - 1);
+    TO ro[2];
+    double a0s[2];
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int kk = bk + e;
+      const double a0 = A[s][qj][qk + e];
+      a0s[e] = a0;
+      const double lam = LAMF ? pick(cur.lam, e) : k.lam_c;
+      double r = pick(cur.b, e) - lam * a0;
+      {
+        const bool gp = !px && p == n - 1, gm = !px && p == 0;
+        const double vp = gp ? a0 : A[sp][qj][qk + e], vm = gm ? a0 : A[sm][qj][qk + e];
+        const double ap = gp ? pick(cx, e) + double(bc.sigma[0]) * vp : vp;
+        const double am = gm ? pick(cx, e) + double(bc.sigma[1]) * vm : vm;
+        const double ah = ALPHAF ? pick(cur.axh, e) : k.alpha, al = ALPHAF ? pick(cur.axl, e) : k.alpha;
+        r = term(r, al, ah, a0, am, ap);
+      }
+      {
+        const bool gp = !py && bj == n - 1, gm = !py && bj == 0;
+        const double vp = gp ? a0 : A[s][qj + 1][qk + e], vm = gm ? a0 : A[s][qj - 1][qk + e];
+        const double ap = gp ? pick(cur.cy, e) + double(bc.sigma[2]) * vp : vp;
+        const double am = gm ? pick(cur.cy, e) + double(bc.sigma[3]) * vm : vm;
+        const double ah = ALPHAF ? pick(cur.ayh, e) : k.alpha, al = ALPHAF ? pick(cur.ay, e) : k.alpha;
+        r = term(r, al, ah, a0, am, ap);
+      }
+      {
+        const bool gp = !pz && kk == n - 1, gm = !pz && kk == 0;
+        const double vp = gp ? a0 : A[s][qj][qk + e + 1], vm = gm ? a0 : A[s][qj][qk + e - 1];
+        const double ap = gp ? pick(cur.cz, e) + double(bc.sigma[4]) * vp : vp;
+        const double am = gm ? pick(cur.cz, e) + double(bc.sigma[5]) * vm : vm;
+        const double ah = ALPHAF ? (e ? cur.azh : cur.az.y) : k.alpha, al = ALPHAF ? pick(cur.az, e) : k.alpha;
+        r = term(r, al, ah, a0, am, ap);
+      }
+      ro[e] = (TO)r;
+      if (r != r) nacc++;
+      else macc = fmax(macc, fabs(r));
+      if (res64) res64[(size_t)p * plane + cell0 + e] = r;
+    }
+    TO2 o;
+    o.x = ro[0];
+    o.y = ro[1];
+    *reinterpret_cast<TO2*>(res + (size_t)p * plane + cell0) = o;
+    if (corr) {
+      double2 w;
+      w.x = a0s[0];
+      w.y = a0s[1];
+      *reinterpret_cast<double2*>(anew + (size_t)p * plane + cell0) = w;
+    }
+    if (coarse) {
+      // children (p&1, row parity, k parity) summed in foreach_child order;
+      // the odd row of a pair of rows is 32 lanes up
+      const TO u0 = __shfl_down(ro[0], 32, 64), u1 = __shfl_down(ro[1], 32, 64);
+      if ((pr & 1) == 0) {
+        if ((p & 1) == 0) acc = TO(0);
+        acc += ro[0];
+        acc += ro[1];
+        acc += u0;
+        acc += u1;
+        if (p & 1) {
+          const int nc = n / 2;
+          coarse[((size_t)(p / 2) * nc + bj / 2) * nc + bk / 2] = acc / TO(8);
+        }
+      }
+    }
+  };
+
+  // prologue: planes i0-1 and i0 in LDS, plane i0+1 in registers
+  {
+    double2 ta[kRNLoad];
+    TD2 td[kRNLoad];
+    fetch_region(i0 - 1, ta, td);
+    commit_region(i0 - 1, ta, td);
+    fetch_region(i0, ta, td);
+    commit_region(i0, ta, td);
+  }
+  fetch_region(i0 + 1, ra, rd);
+  ResCoef C0{}, C1{};
+  fetch_coef3(i0, C0);
+  for (int p = i0; p < i1; p += 2) {
+    step(p, C0, C1);
+    step(p + 1, C1, C0);
+  }
+  // block partials of max |res| and the NaN count
+  macc = wave_max(macc);
+  for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o, 64);
+  const int lane = tid & 63, w = tid >> 6;
+  if (lane == 0) {
+    smax[w] = macc;
+    snan[w] = nacc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double mm = smax[0];
+    unsigned long long ns = snan[0];
+    for (int i = 1; i < kSBlock / 64; i++) {
+      mm = fmax(mm, smax[i]);
+      ns += snan[i];
+    }
+    part[blockIdx.x] = Part{mm, ns};
   }
 }
 

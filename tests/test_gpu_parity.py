@@ -279,6 +279,29 @@ def test_mixed_precision_kernels_agree(kernel, monkeypatch):
     assert np.array_equal(a0, a1)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_correction_residual_agrees(precision, monkeypatch):
+    """At 256^3 mg_solve leaves each cycle's correction a += da to the
+    streaming residual kernel, which also restricts the residual: the solve
+    equals, bit for bit, the one with the separate correction, residual and
+    restriction kernels (BMG_FUSED_RESIDUAL=0)."""
+    print("case fused residual", precision, flush=True)
+    lam = np.random.default_rng(13).uniform(-2.0, -0.5, (256,) * 3)
+    out = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("BMG_FUSED_RESIDUAL", fused)
+        g = bmg.Grid(3, 8, smoother="rbgs", precision=precision)
+        cases.random_problem(g, 23, lam=lam, bcs=BC_MIXES["mixed"])
+        g.set_a(np.zeros(g.shape()))
+        st = g.poisson(tolerance=1e-9)
+        out.append((st, g.get_a(), g.get_res()))
+    (s0, a0, r0), (s1, a1, r1) = out
+    for key in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert s0[key] == s1[key], key
+    assert np.array_equal(a0, a1)
+    assert np.array_equal(r0, r1)
+
+
 def test_nan_is_reported():
     g = bmg.Grid(2, 5)
     b = np.zeros(g.shape())
