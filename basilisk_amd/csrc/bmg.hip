@@ -273,6 +273,28 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // n >= 128) or Jacobi
 enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI };
 
+// plane chunks of the column-streamed kernel (BMG_STREAM_CHUNKS overrides,
+// 1..n/4): about two blocks per CU in FP32, one in FP64 (register bound),
+// with chunks of at least 4 planes
+int stream_chunks(int n, size_t es) {
+  if (const char* e = getenv("BMG_STREAM_CHUNKS")) {
+    int c = atoi(e);
+    if (c >= 1 && c <= n / 4) return c;
+  }
+  const int tiles = (n / kSTK) * (n / kSTJ), target = es == 4 ? 512 : 256;
+  return std::max(1, std::min(n / 4, target / tiles));
+}
+
+// smallest level edge that takes the column-streamed kernel (BMG_STREAM3_MIN
+// overrides, 128 or more)
+int stream3_min() {
+  if (const char* e = getenv("BMG_STREAM3_MIN")) {
+    int v = atoi(e);
+    if (v >= 128) return v;
+  }
+  return 256;
+}
+
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
   if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
@@ -280,7 +302,7 @@ int level_kernel(const bmg_grid* g, int l) {
   // more (fewer planes give too few blocks), walls in every direction
   const bool walls = !g->periodic[0] && !g->periodic[1] && !g->periodic[2];
   // (buffer resources cover at most 2^31 bytes: up to 512^3 cells)
-  const bool stream3 = DIM == 3 && g->L[l].n >= 256 && g->L[l].ncell <= (1u << 27) && walls;
+  const bool stream3 = DIM == 3 && g->L[l].n >= stream3_min() && g->L[l].ncell <= (1u << 27) && walls;
   if (g->rb_override >= 0 && g->L[l].ncell >= 4096 &&
       ((g->rb_override != RK_STREAM_RB && g->rb_override != RK_STREAM2_RB && g->rb_override != RK_STREAM3_RB) ||
        (g->rb_override == RK_STREAM3_RB ? stream3 : DIM == 3 && g->L[l].n >= 128)))
@@ -305,15 +327,6 @@ int stream_chunk(int n) {
   return n >= 256 ? 16 : 8;
 }
 
-// plane chunks of the column-streamed kernel (BMG_STREAM_CHUNKS overrides,
-// 1..n/4): about two blocks per CU in FP32, one in FP64 (register bound)
-int stream_chunks(int n, size_t es) {
-  if (const char* e = getenv("BMG_STREAM_CHUNKS")) {
-    int c = atoi(e);
-    if (c >= 1 && c <= n / 4) return c;
-  }
-  return std::max(1, es == 4 ? n / 32 : n / 64);
-}
 
 // largest level that the single-workgroup coarse kernel takes (at most 4096
 // cells; BMG_COARSE_TOP overrides within that limit)
@@ -525,8 +538,13 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) 
       ProfScope ps(g, PROF_PROLONG);
       Geo<DIM> gf{lv.n, lv.log2n};
       BCH bcc = level_bc<DIM>(g, l - 1);
-      hipLaunchKernelGGL((k_prolong<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
-                         (const T*)g->L[l - 1].da[0], (T*)lv.da[dst], gf, bcc);
+      // one thread per coarse cell on the large 2D/3D levels
+      if (DIM >= 2 && lv.ncell >= (1u << 15))
+        hipLaunchKernelGGL((k_prolong_children<DIM, T>), dim3(nblocks(g->L[l - 1].ncell)), dim3(kBlock), 0,
+                           g->stream, (const T*)g->L[l - 1].da[0], (T*)lv.da[dst], gf, bcc);
+      else
+        hipLaunchKernelGGL((k_prolong<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                           (const T*)g->L[l - 1].da[0], (T*)lv.da[dst], gf, bcc);
     }
     TRY(check_launch());
     int cur = dst, rc = BMG_OK;

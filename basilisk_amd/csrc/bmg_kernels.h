@@ -465,6 +465,37 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const T* __restrict__ coarse
   fine[idx] = bilinear_at<DIM, T>(coarse, gc, bcc, c);
 }
 
+// the same prolongation, one thread per coarse cell: its 2^DIM children
+// (bilinear_at each, sharing the coarse loads), the two children along the
+// last dimension written as one vector
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_prolong_children(const T* __restrict__ coarse, T* __restrict__ fine,
+                                                             Geo<DIM> gf, BCH bcc) {
+  using V2 = typename Vec2<T>::type;
+  Geo<DIM> gc{gf.n >> 1, gf.log2n - 1};
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
+  if (idx >= gc.ncell()) return;
+  int cc[3];
+  gc.decode(idx, cc);
+  const int nb = DIM == 3 ? 4 : 2;  // children groups along the other dimensions
+#pragma unroll
+  for (int m = 0; m < nb; m++) {
+    int c[3] = {2 * cc[0], 2 * cc[1], 2 * cc[2]};
+    if (DIM == 3) {
+      c[0] += m >> 1;
+      c[1] += m & 1;
+    } else {
+      c[0] += m;
+    }
+    V2 v;
+    v.x = bilinear_at<DIM, T>(coarse, gc, bcc, c);
+    c[DIM - 1] += 1;
+    v.y = bilinear_at<DIM, T>(coarse, gc, bcc, c);
+    c[DIM - 1] -= 1;
+    *reinterpret_cast<V2*>(fine + gf.index(c)) = v;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ f, size_t n, T v) {
   size_t idx = (size_t)blockIdx.x * kBlock + threadIdx.x;
