@@ -8,7 +8,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/bmg.h"
@@ -92,6 +94,11 @@ struct bmg_grid {
   int rb_override = -1;  // BMG_RB_KERNEL: kernel for red-black levels of >= 4096 cells
   int rb_reverse = 1;    // black passes walk the blocks backwards (BMG_RB_REVERSE=0: forwards)
   int grid_kind = BMG_GRID_MULTIGRID;  // ghost rule of edges and corners
+  // the cycle as a HIP graph per (nrelax, minlevel, maxlevel, deferred
+  // correction, fresh coarse res); dropped whenever a kernel argument of the
+  // cycle could change (BMG_GRAPHS=0: plain launches)
+  bool use_graphs = true;
+  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> cycle_graphs;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
   double prof_ms[PROF_N] = {0};
@@ -164,7 +171,14 @@ int check_launch() {
 }
 
 // allocate the per-level arrays of one precision
+void drop_graphs(bmg_grid* g) {
+  for (auto& kv : g->cycle_graphs)
+    if (kv.second) (void)hipGraphExecDestroy(kv.second);
+  g->cycle_graphs.clear();
+}
+
 int alloc_levels(bmg_grid* g) {
+  drop_graphs(g);
   int p = g->precision;
   if (g->allocated_precision[p]) return BMG_OK;
   // arrays of the other precision are released first
@@ -766,14 +780,45 @@ int refresh_coef_copies(bmg_grid* g) {
 }
 
 // --- dispatch on dimension / precision
+template <int DIM, typename T>
+int cycle_graph(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) {
+  // profiling records events around the kernels: plain launches then
+  if (!g->use_graphs || g->prof) return mg_cycle_t<DIM, T>(g, nrelax, minlevel, maxlevel, defer);
+  hipGraphExec_t& slot = g->cycle_graphs[std::make_tuple(nrelax, minlevel, maxlevel, (int)defer,
+                                                         (int)g->coarse_res_fresh)];
+  if (!slot) {
+    // capturing runs the host side of the cycle (its flags) but no kernel
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = mg_cycle_t<DIM, T>(g, nrelax, minlevel, maxlevel, defer);
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc != BMG_OK) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(BMG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(BMG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    slot = exec;
+  } else {
+    // the host side of mg_cycle_t
+    g->coarse_res_fresh = false;
+    if (defer && maxlevel == g->depth) g->pending_correct = true;
+  }
+  HIPCHECK(hipGraphLaunch(slot, g->stream));
+  return BMG_OK;
+}
+
 int dispatch_cycle(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer = false) {
   TRY(refresh_coef_copies(g));
   bool mixed = g->precision == BMG_MIXED;
   defer = defer && fused_residual(g);
   switch (g->dim) {
     case 1: return mixed ? mg_cycle_t<1, float>(g, nrelax, minlevel, maxlevel, false) : mg_cycle_t<1, double>(g, nrelax, minlevel, maxlevel, false);
-    case 2: return mixed ? mg_cycle_t<2, float>(g, nrelax, minlevel, maxlevel, false) : mg_cycle_t<2, double>(g, nrelax, minlevel, maxlevel, false);
-    default: return mixed ? mg_cycle_t<3, float>(g, nrelax, minlevel, maxlevel, defer) : mg_cycle_t<3, double>(g, nrelax, minlevel, maxlevel, defer);
+    case 2: return mixed ? cycle_graph<2, float>(g, nrelax, minlevel, maxlevel, false) : cycle_graph<2, double>(g, nrelax, minlevel, maxlevel, false);
+    default: return mixed ? cycle_graph<3, float>(g, nrelax, minlevel, maxlevel, defer) : cycle_graph<3, double>(g, nrelax, minlevel, maxlevel, defer);
   }
 }
 
@@ -983,6 +1028,7 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
     else if (!strcmp(e, "stream3") && dimension == 3) g->rb_override = RK_STREAM3_RB;
   }
   if (const char* e = getenv("BMG_RB_REVERSE")) g->rb_reverse = e[0] != '0';
+  if (const char* e = getenv("BMG_GRAPHS")) g->use_graphs = e[0] != '0';
   int r = BMG_OK;
   // a blocking stream: it orders itself after work on the legacy default
   // stream, where callers (PyTorch included) usually prepare device inputs
@@ -1055,6 +1101,7 @@ int bmg_grid_free(bmg_grid* g) {
     if (lv.lam != (void*)lv.lam64) dfree(lv.lam);
     dfreet(lv.lam64);
   }
+  drop_graphs(g);
   dfreet(g->a);
   dfreet(g->a2);
   dfreet(g->b);
@@ -1085,7 +1132,7 @@ int bmg_set_precision(bmg_grid* g, int precision) {
   if (precision == g->precision) return BMG_OK;
   // the cycle fields are scratch between cycles; a and b are kept
   g->precision = precision;
-  TRY(alloc_levels(g));
+  TRY(alloc_levels(g));  // drops the graphs
   return BMG_OK;
 }
 
@@ -1095,6 +1142,7 @@ int bmg_set_grid_kind(bmg_grid* g, int kind) {
   if (kind == BMG_GRID_TREE && (g->periodic[0] || g->periodic[1] || g->periodic[2]))
     return fail(BMG_ERR_UNSUPPORTED, "periodic trees are not supported");
   g->grid_kind = kind;
+  drop_graphs(g);
   return BMG_OK;
 }
 
@@ -1104,6 +1152,7 @@ int bmg_set_smoother(bmg_grid* g, int smoother) {
     return fail(BMG_ERR_UNSUPPORTED, "lexicographic Gauss-Seidel is not available on the GPU path; use JACOBI or RBGS");
   if (smoother != BMG_RELAX_JACOBI && smoother != BMG_RELAX_RBGS) return fail(BMG_ERR_ARG, "unknown smoother %d", smoother);
   g->smoother = smoother;
+  drop_graphs(g);
   return ensure_jacobi_buffers(g);
 }
 
@@ -1130,6 +1179,7 @@ int bmg_set_stream(bmg_grid* g, void* stream) {
 
 static int set_bc_common(bmg_grid* g, int b, int kind, int sigma, const double* c, double scale) {
   if (b < 0 || b >= 2 * g->dim) return fail(BMG_ERR_ARG, "direction %d out of range", b);
+  drop_graphs(g);
   int d = b / 2;
   dfreet(g->bc_c[b]);
   if (kind == BMG_BC_PERIODIC) {
@@ -1177,6 +1227,7 @@ int bmg_set_bc_affine(bmg_grid* g, int b, int sigma, const double* c) {
 
 int bmg_set_alpha_const(bmg_grid* g, double alpha) {
   TRY(check_grid(g));
+  if (g->alpha_field || g->alpha_c != alpha) drop_graphs(g);
   g->alpha_field = false;
   g->alpha_c = alpha;
   g->coarse_coef_ready = false;
@@ -1185,6 +1236,7 @@ int bmg_set_alpha_const(bmg_grid* g, double alpha) {
 
 int bmg_set_lambda_const(bmg_grid* g, double lambda) {
   TRY(check_grid(g));
+  if (g->lambda_field || g->lambda_c != lambda) drop_graphs(g);
   g->lambda_field = false;
   g->lambda_c = lambda;
   g->coarse_coef_ready = false;
@@ -1200,6 +1252,7 @@ int bmg_set_field(bmg_grid* g, int field, int level, const double* ptr, int on_d
   g->coarse_res_fresh = false;
   if (field >= BMG_F_ALPHA && field < BMG_F_ALPHA + g->dim) {
     TRY(set_face(g, field - BMG_F_ALPHA, level, ptr, on_device));
+    if (!g->alpha_field) drop_graphs(g);
     g->alpha_field = true;
     return BMG_OK;
   }
@@ -1210,6 +1263,7 @@ int bmg_set_field(bmg_grid* g, int field, int level, const double* ptr, int on_d
       return copy_in(g, field == BMG_F_A ? g->a : g->b, ptr, lv.ncell, on_device);
     case BMG_F_LAMBDA:
       TRY(copy_in(g, lv.lam64, ptr, lv.ncell, on_device));
+      if (!g->lambda_field) drop_graphs(g);
       g->lambda_field = true;
       return BMG_OK;
     case BMG_F_RES:

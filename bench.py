@@ -20,13 +20,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# HBM bytes moved per fine-level red-black sweep (two launches of the
-# in-place colour kernel), measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
-# this workload and scaled by the counters' calibration on a copy kernel of
-# known size (fetch x2.0, write x1.0); the derivation is in
-# profiles/hbm_traffic_poisson256_mixed.md.  Keyed by (precision, depth,
-# red-black kernel).
-MEASURED_SWEEP_TRAFFIC = {("mixed", 8, "inplace"): 2 * (333867.0 + 65536.0) * 1024}
+# HBM bytes moved per fine-level red-black sweep, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE on this workload (KiB per dispatch) and scaled by
+# the counters' calibration on a copy kernel of known size (fetch x2.0,
+# write x1.0); the derivation is in profiles/hbm_traffic_poisson256_mixed.md.
+# Keyed by (precision, depth, red-black kernel): "stream3" is the default
+# single-pass sweep at 256^3, "inplace" the two launches of the colour kernel.
+MEASURED_SWEEP_TRAFFIC = {
+    ("mixed", 8, "stream3"): (2 * 203205.6 + 65536.0) * 1024,
+    ("mixed", 8, "inplace"): 2 * (333867.0 + 65536.0) * 1024,
+}
 
 
 def parse():
@@ -211,7 +214,7 @@ def main():
     # algorithmic bytes of the whole solve (per V-cycle and cell, 3D, nrelax
     # 4: ~312 B FP64, ~192 B mixed) for the whole-solve GB/s
     per_cycle_bytes = {"fp64": 312.0, "mixed": 192.0}[main_prec] * N
-    rb_kernel = os.environ.get("BMG_RB_KERNEL", "inplace")
+    rb_kernel = os.environ.get("BMG_RB_KERNEL", "stream3")
     traffic = MEASURED_SWEEP_TRAFFIC.get((main_prec, args.depth, rb_kernel))
     line = {
         "metric": "time-to-1e-9 residual, 256^3 var-coeff Poisson (mixed precision)",

@@ -302,6 +302,37 @@ def test_fused_correction_residual_agrees(precision, monkeypatch):
     assert np.array_equal(r0, r1)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("depth", [7, 8])
+def test_cycle_graph_replays_equal_plain_launches(precision, depth, monkeypatch):
+    """The cycle runs as a captured HIP graph, replayed for every cycle with
+    the same (nrelax, minlevel, maxlevel): each replay, with eager residual
+    kernels and synchronizations in between, equals the plain launches bit
+    for bit, and so does a whole solve."""
+    print("case graphs", precision, depth, flush=True)
+    lam = np.random.default_rng(14).uniform(-2.0, -0.5, (1 << depth,) * 3)
+    runs = []
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("BMG_GRAPHS", graphs)
+        g = bmg.Grid(3, depth, smoother="rbgs", precision=precision)
+        cases.random_problem(g, 31, lam=lam, bcs=BC_MIXES["mixed"])
+        g.restriction_alpha_lambda()
+        res = [g.residual()]
+        for it in range(4):
+            g.mg_cycle(3, 1)
+            res.append(g.residual())
+        a = g.get_a()
+        g.set_a(np.zeros(g.shape()))
+        st = g.poisson(tolerance=1e-9)
+        runs.append((res, a, st, g.get_a()))
+    (r0, a0, s0, f0), (r1, a1, s1, f1) = runs
+    assert r0 == r1
+    assert np.array_equal(a0, a1)
+    for key in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert s0[key] == s1[key], key
+    assert np.array_equal(f0, f1)
+
+
 def test_nan_is_reported():
     g = bmg.Grid(2, 5)
     b = np.zeros(g.shape())
