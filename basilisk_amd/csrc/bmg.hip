@@ -309,6 +309,12 @@ int stream3_min() {
   return 256;
 }
 
+// a cell-count threshold with an environment override (tuning knob)
+size_t size_knob(const char* name, size_t dflt) {
+  const char* e = getenv(name);
+  return e && atoll(e) > 0 ? (size_t)atoll(e) : dflt;
+}
+
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
   if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
@@ -323,11 +329,14 @@ int level_kernel(const bmg_grid* g, int l) {
     return g->rb_override;
   // measured at 256^3: the single-pass column-streamed kernel reads each
   // operand once and beats the two-launch in-place kernel, which is the
-  // fastest elsewhere on large levels; the one-launch fused kernel halves
-  // the launches on mid-size levels, where launch latency dominates
+  // fastest on every other level now that the cycle replays as a graph (the
+  // one-launch fused kernel, which recomputes red neighbours, only paid off
+  // while launches were issued one by one; BMG_FUSED_MIN brings it back for
+  // levels of at least that many cells, below BMG_INPLACE_MIN)
   if (stream3 && g->rb_override < 0) return RK_STREAM3_RB;
-  if (g->L[l].ncell >= (1u << 20)) return RK_INPLACE_RB;
-  if (g->L[l].ncell >= 1024) return RK_FUSED_RB;
+  const size_t nc = g->L[l].ncell;
+  if (nc >= size_knob("BMG_FUSED_MIN", ~(size_t)0) && nc < size_knob("BMG_INPLACE_MIN", 1u << 20))
+    return RK_FUSED_RB;
   return RK_INPLACE_RB;
 }
 
