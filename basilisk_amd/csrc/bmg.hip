@@ -11,6 +11,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bmg.h"
@@ -287,6 +288,13 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // n >= 128) or Jacobi
 enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI };
 
+// tile rows of the column-streamed sweep: 32 in FP32 (1024 threads), 16 in
+// FP64 (512 threads; BMG_STREAM_TJ=16 selects them in FP32 too)
+int stream_tj(size_t es) {
+  const char* e = getenv("BMG_STREAM_TJ");
+  return es == 4 && !(e && atoi(e) == 16) ? kSTJ32 : kSTJ;
+}
+
 // plane chunks of the column-streamed kernel (BMG_STREAM_CHUNKS overrides,
 // 1..n/4): about two blocks per CU in FP32, one in FP64 (register bound),
 // with chunks of at least 4 planes
@@ -295,7 +303,8 @@ int stream_chunks(int n, size_t es) {
     int c = atoi(e);
     if (c >= 1 && c <= n / 4) return c;
   }
-  const int tiles = (n / kSTK) * (n / kSTJ), target = es == 4 ? 512 : 256;
+  const int tj = stream_tj(es), tiles = (n / kSTK) * (n / tj);
+  const int target = tj == kSTJ32 ? 256 : (es == 4 ? 512 : 256);
   return std::max(1, std::min(n / 4, target / tiles));
 }
 
@@ -435,28 +444,38 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
         break;
       case RK_STREAM3_RB:
         if constexpr (DIM == 3) {
-          const int nch = stream_chunks(lv.n, sizeof(T));
+          const int nch = stream_chunks(lv.n, sizeof(T)), tj = stream_tj(sizeof(T));
           if (lv.n % kSTK || g->periodic[0] || g->periodic[1] || g->periodic[2]) {
             *rc = fail(BMG_ERR_ARG, "internal: column-streamed sweep on level %d", l);
             return src;
           }
-          unsigned nb = (unsigned)((lv.n / kSTK) * (lv.n / kSTJ) * nch);
+          const unsigned nb = (unsigned)((lv.n / kSTK) * (lv.n / tj) * nch);
           const T* din = (const T*)lv.da[src];
           T* dout = (T*)lv.da[src ^ 1];
           const T* r = (const T*)lv.res;
-          bool af = k.lo[0] != nullptr, lf = k.lam != nullptr;
-          if (af && lf)
-            hipLaunchKernelGGL((k_rb_stream3c<T, true, true>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout, r,
-                               k, geo, bc, D2, nch);
-          else if (af)
-            hipLaunchKernelGGL((k_rb_stream3c<T, true, false>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout,
-                               r, k, geo, bc, D2, nch);
-          else if (lf)
-            hipLaunchKernelGGL((k_rb_stream3c<T, false, true>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout,
-                               r, k, geo, bc, D2, nch);
-          else
-            hipLaunchKernelGGL((k_rb_stream3c<T, false, false>), dim3(nb), dim3(kSBlock), 0, g->stream, din, dout,
-                               r, k, geo, bc, D2, nch);
+          const bool af = k.lo[0] != nullptr, lf = k.lam != nullptr;
+          auto go = [&](auto tag) {
+            constexpr int TJ = decltype(tag)::value;
+            const dim3 blk(TJ * kSTK / 2);
+            if (af && lf)
+              hipLaunchKernelGGL((k_rb_stream3c<T, true, true, TJ>), dim3(nb), blk, 0, g->stream, din, dout, r, k,
+                                 geo, bc, D2, nch);
+            else if (af)
+              hipLaunchKernelGGL((k_rb_stream3c<T, true, false, TJ>), dim3(nb), blk, 0, g->stream, din, dout, r, k,
+                                 geo, bc, D2, nch);
+            else if (lf)
+              hipLaunchKernelGGL((k_rb_stream3c<T, false, true, TJ>), dim3(nb), blk, 0, g->stream, din, dout, r, k,
+                                 geo, bc, D2, nch);
+            else
+              hipLaunchKernelGGL((k_rb_stream3c<T, false, false, TJ>), dim3(nb), blk, 0, g->stream, din, dout, r, k,
+                                 geo, bc, D2, nch);
+          };
+          if constexpr (sizeof(T) == 4) {
+            if (tj == kSTJ32) go(std::integral_constant<int, kSTJ32>());
+            else go(std::integral_constant<int, kSTJ>());
+          } else {
+            go(std::integral_constant<int, kSTJ>());
+          }
         }
         out = src ^ 1;
         break;

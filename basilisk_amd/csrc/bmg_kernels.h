@@ -1298,13 +1298,14 @@ __device__ __forceinline__ double bor(double a, double b) {
 // plane needs no barrier of its own).  The halo cells outside the level hold
 // their ghost values, sigma times the old value of the cell they mirror --
 // the value relax_value gives a ghost neighbour -- so that every neighbour is
-// a plain LDS read.  The tile is 16 x 64 cells with one thread
-// per pair of cells (j, k), (j, k+1), k even.  On plane s the red cell of a
+// a plain LDS read.  The tile is TJ x 64 cells (TJ = 32 in FP32, 16 in
+// FP64, where registers allow only 512 threads) with one thread per pair of
+// cells (j, k), (j, k+1), k even.  On plane s the red cell of a
 // pair and on plane s-1 its black cell sit at the same k, so the coefficients
 // of a pair on plane p -- res, lambda and the faces, loaded as vectors --
 // serve the red cell on step p and the black cell on step p+1.  The red cells
 // of the 1-cell ring around the tile, which the black cells on the tile's
-// edge need, are relaxed by the first 82 threads (two waves), with the
+// edge need, are relaxed by the first TJ + 66 threads (two waves), with the
 // coefficients relax_value reads; like k_rb_stream3, blocks recompute them
 // instead of exchanging data.  Same arithmetic as relax_value.
 //
@@ -1314,14 +1315,11 @@ __device__ __forceinline__ double bor(double a, double b) {
 // roles through a loop unrolled by three, so that no register copy and no
 // branch makes the compiler wait for a load early.
 
+// tiles of the streaming kernels: 16 x 64 cells, 512 threads (the residual
+// kernel below; the sweep's FP64 tile), and the sweep's FP32 tile of 32 rows
 constexpr int kSTJ = 16, kSTK = 64, kSBlock = 512;
-constexpr int kSRJ = kSTJ + 4, kSRK = kSTK + 4;          // field region, 2-cell halo
-constexpr int kSNRing = kSTJ + kSTK + 2;                  // red ring cells per plane (82)
-constexpr int kSRingThreads = 128;                        // the waves that relax the ring
-static_assert(kSNRing <= kSRingThreads, "ring cells fit two waves");
-constexpr int kSRegion2 = kSRJ * kSRK / 2;                // vector slots of the field region (680)
-constexpr int kSNLoad = (kSRegion2 + kSBlock - 1) / kSBlock;  // vector field loads per thread (2)
-static_assert(kSTJ * kSTK == 2 * kSBlock, "one thread per pair of tile cells");
+constexpr int kSTJ32 = 32;
+constexpr int kSRingThreads = 128;  // the waves that relax the sweep's ring
 
 // coefficients of a pair of cells (k, k+1) on one plane p
 template <typename T>
@@ -1343,12 +1341,19 @@ __device__ __forceinline__ T relax_nd(T D2, T res, T lam, const T al[3], const T
   return nn / dd;
 }
 
-// FP32: two blocks per CU (at most 128 registers); FP64: one
-template <typename T, bool ALPHAF, bool LAMF>
-__global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+// at most 128 registers: 16 waves per CU (FP32, one block of 1024 threads);
+// FP64: one block of 512 threads
+template <typename T, bool ALPHAF, bool LAMF, int TJ>
+__global__ __launch_bounds__(TJ * kSTK / 2) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __restrict__ res, Coef<T> k,
                    Geo<3> g, BCH bc, T D2, int nchunks) {
   using V2 = typename Vec2<T>::type;
+  constexpr int kSTJ = TJ, kSBlock = TJ * kSTK / 2;
+  constexpr int kSRJ = kSTJ + 4, kSRK = kSTK + 4;              // field region, 2-cell halo
+  constexpr int kSNRing = kSTJ + kSTK + 2;                      // red ring cells per plane
+  static_assert(kSNRing <= kSRingThreads, "ring cells fit two waves");
+  constexpr int kSRegion2 = kSRJ * kSRK / 2;                    // vector slots of the field region
+  constexpr int kSNLoad = (kSRegion2 + kSBlock - 1) / kSBlock;  // vector field loads per thread
   __shared__ __align__(16) T buf[8][kSRJ][kSRK];
   const int n = g.n, tid = threadIdx.x;
   const size_t b = logical_block();
