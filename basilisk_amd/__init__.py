@@ -47,7 +47,7 @@ EXPORTS = [
     "bmg_tree_set_alpha_faces", "bmg_tree_set_alpha_const", "bmg_tree_set_lambda_const",
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
-    "bmg_coef_gradient",
+    "bmg_coef_gradient", "bmg_project",
 ]
 
 
@@ -78,6 +78,14 @@ def library():
     if not os.path.exists(LIB_PATH):
         raise ImportError("libbmg.so is not built: run `python -m basilisk_amd.build` "
                           "(there is no CPU fallback)")
+    # PyTorch, when installed, is loaded first: it carries its own copy of
+    # the HIP runtime, and a process in which this library has already
+    # initialised the GPU through the system runtime finds no device for
+    # torch afterwards.  Loaded first, torch's runtime serves both.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     dp = ctypes.c_void_p
@@ -136,6 +144,7 @@ def library():
         "bmg_tree_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_synchronize": (I, [P]),
         "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
+        "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -393,6 +402,36 @@ class Grid:
         _check(self.lib.bmg_coef_gradient(self._g, _host_ptr(p), ptrs[0], ptrs[1], ptrs[2],
                                           _host_ptr(gl) if gl is not None else None, 0))
         return ga, gl
+
+    def project(self, uf, dt=1.0, nrelax=0):
+        """project(), poisson.h:469-509: makes the face velocity `uf` (one
+        array per component, n+1 faces along its own direction) discretely
+        divergence-free with p = the grid's `a` (initial guess in, pressure
+        out) and the grid's alpha; returns the poisson() statistics.  torch
+        CUDA tensors are updated on the device, numpy float64 arrays in
+        place through host copies."""
+        if len(uf) != self.dim:
+            raise ValueError("need %d face components" % self.dim)
+        for c, u in enumerate(uf):
+            if tuple(u.shape) != self.face_shape(c):
+                raise ValueError("face component %d: shape %s, expected %s" % (c, tuple(u.shape), self.face_shape(c)))
+        s = MGStats()
+        if all(getattr(u, "is_cuda", False) for u in uf):
+            import torch
+            for u in uf:
+                if u.dtype != torch.float64 or not u.is_contiguous():
+                    raise ValueError("face components must be contiguous float64 tensors")
+            torch.cuda.current_stream().synchronize()
+            ptrs = [ctypes.c_void_p(u.data_ptr()) for u in uf] + [None] * (3 - self.dim)
+            _check(self.lib.bmg_project(self._g, ptrs[0], ptrs[1], ptrs[2], float(dt), int(nrelax), 1,
+                                        ctypes.byref(s)))
+            return s.as_dict()
+        for u in uf:
+            if not isinstance(u, np.ndarray) or u.dtype != np.float64 or not u.flags.c_contiguous:
+                raise ValueError("face components must be C-contiguous float64 numpy arrays (or CUDA tensors)")
+        ptrs = [_host_ptr(u) for u in uf] + [None] * (3 - self.dim)
+        _check(self.lib.bmg_project(self._g, ptrs[0], ptrs[1], ptrs[2], float(dt), int(nrelax), 0, ctypes.byref(s)))
+        return s.as_dict()
 
     # -- kernel timing
     def profile(self, enable=True):

@@ -95,7 +95,7 @@ def config2(depth=8, seed=1, nspheres=24, ratio=1000.0):
         u.append(uc)
     b = divergence_rhs(u, delta)
     return {"dim": 3, "depth": depth, "b": b, "alpha": alpha, "tolerance": 1e-9, "nrelax": 4,
-            "rho_inside_fraction": float(inside.mean())}
+            "rho_inside_fraction": float(inside.mean()), "u": u}
 
 
 def config4(depth=10, seed=4):

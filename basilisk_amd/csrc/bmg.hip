@@ -1016,6 +1016,11 @@ int coef_gradient_t(bmg_grid* g, const double* psi, double* const* galpha, doubl
   return check_launch();
 }
 
+// project(), poisson.h:469-509 on the finest level: divergence of uf into
+// b, poisson() for a with TOLERANCE/sq(dt), face update of uf
+template <int DIM>
+int project_t(bmg_grid* g, double* const* uf, double dt, int nrelax, bmg_mgstats* stats);
+
 }  // namespace
 
 // ------------------------------------------------------------------- C ABI
@@ -1399,6 +1404,48 @@ int bmg_poisson(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_mgs
   return r;
 }
 
+int bmg_project(bmg_grid* g, double* uf_x, double* uf_y, double* uf_z, double dt, int nrelax, int on_device,
+                bmg_mgstats* stats) {
+  TRY(check_grid(g));
+  double* in[3] = {uf_x, uf_y, uf_z};
+  for (int d = 0; d < g->dim; d++)
+    if (!in[d]) return fail(BMG_ERR_ARG, "null face velocity component %d", d);
+  // the reference's defaults: dt = 0 means 1, nrelax = 0 means 4
+  if (dt == 0.) dt = 1.;
+  if (nrelax == 0) nrelax = 4;
+  if (!(dt > 0.) || nrelax < 0) return fail(BMG_ERR_ARG, "dt must be positive and nrelax non-negative");
+  const Level& lf = g->L[g->depth];
+  const size_t nface = (size_t)(lf.n + 1) * powd(lf.n, g->dim - 1);
+  double* dev[3] = {nullptr, nullptr, nullptr};
+  int rc = BMG_OK;
+  if (!on_device) {
+    for (int d = 0; d < g->dim && rc == BMG_OK; d++) {
+      rc = dalloc((void**)&dev[d], nface * sizeof(double));
+      if (rc == BMG_OK && hipMemcpy(dev[d], in[d], nface * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(BMG_ERR_HIP, "copy of the face velocity failed");
+    }
+  } else {
+    for (int d = 0; d < g->dim; d++) dev[d] = in[d];
+  }
+  if (rc == BMG_OK) {
+    switch (g->dim) {
+      case 1: rc = project_t<1>(g, dev, dt, nrelax, stats); break;
+      case 2: rc = project_t<2>(g, dev, dt, nrelax, stats); break;
+      default: rc = project_t<3>(g, dev, dt, nrelax, stats); break;
+    }
+  }
+  if (!on_device) {
+    // the face update ran even when the solve reported a problem
+    for (int d = 0; d < g->dim; d++) {
+      if (dev[d] && hipMemcpy(in[d], dev[d], nface * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess &&
+          rc == BMG_OK)
+        rc = fail(BMG_ERR_HIP, "copy of the face velocity failed");
+      dfreet(dev[d]);
+    }
+  }
+  return rc;
+}
+
 int bmg_coef_gradient(bmg_grid* g, const double* psi, double* galpha_x, double* galpha_y, double* galpha_z,
                       double* glambda, int on_device) {
   TRY(check_grid(g));
@@ -1556,3 +1603,37 @@ int bmg_poisson_view(const bmg_multigrid_view* v, const bmg_poisson_args* p, con
 }
 
 }  // extern "C"
+
+namespace {
+
+template <int DIM>
+int project_t(bmg_grid* g, double* const* uf, double dt, int nrelax, bmg_mgstats* stats) {
+  Level& lf = g->L[g->depth];
+  Geo<DIM> geo{lf.n, lf.log2n};
+  const double D = delta(g, g->depth);
+  hipLaunchKernelGGL((k_divergence<DIM>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, uf[0],
+                     DIM >= 2 ? uf[1] : nullptr, DIM >= 3 ? uf[2] : nullptr, g->b, geo, dt * D);
+  TRY(check_launch());
+  g->coarse_res_fresh = false;
+  const int rc = bmg_poisson(g, g->TOLERANCE / (dt * dt), nrelax, 0, stats);
+  if (rc != BMG_OK && rc != BMG_ERR_NAN) return rc;
+  BCA bc;
+  memset(&bc, 0, sizeof bc);
+  for (int d = 0; d < 3; d++) bc.periodic[d] = d < g->dim ? g->periodic[d] : 0;
+  for (int b = 0; b < 6; b++) {
+    bc.sigma[b] = g->bc_sigma[b];
+    bc.c[b] = g->bc_c[b];
+  }
+  const size_t nface = (size_t)(lf.n + 1) * powd(lf.n, DIM - 1);
+  for (int d = 0; d < DIM; d++) {
+    const double* alo = g->alpha_field ? lf.alo64[d] : nullptr;
+    const double* aw = g->alpha_field ? lf.awall64[d] : nullptr;
+    hipLaunchKernelGGL((k_face_update<DIM>), dim3(nblocks(nface)), dim3(kBlock), 0, g->stream, uf[d], g->a, alo,
+                       aw, g->alpha_c, geo, bc, d, dt, D);
+  }
+  TRY(check_launch());
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return rc;
+}
+
+}  // namespace

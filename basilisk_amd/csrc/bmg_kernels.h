@@ -684,6 +684,86 @@ __global__ void k_sum_final(const double* __restrict__ part, int np, double* __r
   }
 }
 
+// --- project(), poisson.h:469-509 ----------------------------------------
+//
+// Face velocities in the user layout: component d has n+1 faces along d,
+// C order.
+
+template <int DIM>
+__device__ __forceinline__ size_t uface(int n, int d, const int c[3]) {
+  const size_t e1 = DIM >= 2 ? (size_t)n + (d == 1) : 1, e2 = DIM >= 3 ? (size_t)n + (d == 2) : 1;
+  return ((size_t)c[0] * e1 + c[1]) * e2 + c[2];
+}
+
+// div = 0; div += uf.x[1] - uf.x[] per dimension (x, y, z); div /= dt*Delta
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_divergence(const double* __restrict__ u0, const double* __restrict__ u1,
+                                                       const double* __restrict__ u2, double* __restrict__ b,
+                                                       Geo<DIM> g, double dtD) {
+  const idx_t idx = logical_block() * kBlock + threadIdx.x;
+  if (idx >= g.ncell()) return;
+  int c[3];
+  g.decode(idx, c);
+  const double* u[3] = {u0, u1, u2};
+  double div = 0.;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    int h[3] = {c[0], c[1], c[2]};
+    h[d] += 1;
+    div += u[d][uface<DIM>(g.n, d, h)] - u[d][uface<DIM>(g.n, d, c)];
+  }
+  b[idx] = div / dtD;
+}
+
+// uf.x[] -= dt*alpha.x[]*face_gradient_x(p, 0) on every face of component
+// d, walls included; face_gradient_x(a,i) = (a[i] - a[i-1])/Delta
+// (common.h:1288) with the ghosts of p = a from its boundary conditions
+// (c + sigma s, or the periodic copy)
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_face_update(double* __restrict__ u, const double* __restrict__ a,
+                                                        const double* __restrict__ alo,
+                                                        const double* __restrict__ awall, double alpha_c,
+                                                        Geo<DIM> g, BCA bc, int d, double dt, double D) {
+  const int n = g.n;
+  const size_t e1 = DIM >= 2 ? (size_t)n + (d == 1) : 1, e2 = DIM >= 3 ? (size_t)n + (d == 2) : 1;
+  const size_t e0 = (size_t)n + (d == 0);
+  const size_t f = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (f >= e0 * e1 * e2) return;
+  int c[3] = {(int)(f / (e1 * e2)), (int)((f / e2) % e1), (int)(f % e2)};
+  const int i = c[d];
+  int q[3] = {c[0], c[1], c[2]};
+  q[d] = i < n ? i : n - 1;
+  const double ahi = a[g.index(q)];
+  q[d] = i > 0 ? i - 1 : 0;
+  const double alow = a[g.index(q)];
+  const idx_t w = g.wall(d, c);
+  double ph, pl;
+  if (i < n) {
+    ph = ahi;
+  } else if (bc.periodic[d]) {
+    q[d] = 0;
+    ph = a[g.index(q)];
+  } else {
+    const double cv = bc.c[2 * d] ? bc.c[2 * d][w] : 0.;
+    ph = cv + double(bc.sigma[2 * d]) * alow;
+  }
+  if (i > 0) {
+    pl = alow;
+  } else if (bc.periodic[d]) {
+    q[d] = n - 1;
+    pl = a[g.index(q)];
+  } else {
+    const double cv = bc.c[2 * d + 1] ? bc.c[2 * d + 1][w] : 0.;
+    pl = cv + double(bc.sigma[2 * d + 1]) * ahi;
+  }
+  double alpha = alpha_c;
+  if (alo) {
+    q[d] = i;
+    alpha = i < n ? alo[g.index(q)] : awall[w];
+  }
+  u[f] = u[f] - dt * alpha * ((ph - pl) / D);
+}
+
 // --- face coefficients (FP64, once per poisson() call) ------------------
 
 // coarse low faces of component d: face_average, grid/multigrid-common.h:42-58,

@@ -122,6 +122,7 @@ def solve_once(g, zeros, tol):
 
 
 def run_precision(bmg, torch, prob, prec, args):
+    import numpy as np
     dim, depth = prob["dim"], prob["depth"]
     g = bmg.Grid(dim, depth, precision=prec, smoother=args.smoother)
     dev = torch.device("cuda:0")
@@ -150,8 +151,29 @@ def run_precision(bmg, torch, prob, prec, args):
         kinds[k] = {"ms": ms, "count": n}
     g.profile(False)
     a = g.get_a()
+    proj = None
+    if prob.get("u") is not None:
+        # project() of the face velocity whose divergence is b (dt = 1, the
+        # same tolerance): divergence, poisson() and face update on the
+        # device; the velocity is reset from a pristine copy between steps
+        u0 = [torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in prob["u"]]
+        u = [x.clone() for x in u0]
+        g.set_tolerance(tol)
+        ptimes = []
+        for it in range(args.warmup + args.steps):
+            for x, y in zip(u, u0):
+                x.copy_(y)
+            g.set_a(zeros)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            pst = g.project(u, dt=1.0)
+            g.synchronize()
+            if it >= args.warmup:
+                ptimes.append(time.perf_counter() - t1)
+        proj = {"seconds": sum(ptimes) / len(ptimes), "stats": pst}
+        del u, u0
     g.close()
-    return {"seconds": dt, "stats": stats[-1], "profile": kinds, "profiled_stats": st, "a": a}
+    return {"seconds": dt, "stats": stats[-1], "profile": kinds, "profiled_stats": st, "a": a, "project": proj}
 
 
 def cpu_baseline(prob, args, ncycles_target):
@@ -254,6 +276,11 @@ def main():
         pf = f["profile"]["relax_fine"]
         line["fp64_sweep_gbs"] = round(6 * 8 * N / (pf["ms"] / max(pf["count"], 1) * 1e-3) / 1e9, 1)
     line["kernel_ms"] = {k: round(v["ms"], 3) for k, v in r["profile"].items()}
+    if r.get("project"):
+        # project() (poisson.h:469-509) of the config's face velocity: the
+        # solver's caller, divergence and face update included
+        line["project_ms"] = round(r["project"]["seconds"] * 1e3, 3)
+        line["project_cycles"] = r["project"]["stats"]["i"]
     if not args.no_cpu_baseline:
         ncyc = res["fp64"]["stats"]["i"] if "fp64" in res else cycles
         line["cpu_baseline"] = cpu_baseline(prob, args, ncyc)

@@ -139,6 +139,20 @@ int bmg_poisson(bmg_grid *g, double tolerance, int nrelax, int minlevel,
                 bmg_mgstats *stats);
 int bmg_synchronize(bmg_grid *g);
 
+/* project(), poisson.h:469-509 (struct Project, poisson.h:460-466): the
+   divergence of the face velocity uf, divided by dt*Delta, becomes b; p is
+   the grid's `a` (initial guess in, pressure out) with its boundary
+   conditions, alpha the grid's face coefficients (unityf unless set);
+   poisson() runs with tolerance TOLERANCE/sq(dt); then every face of uf,
+   walls included, gets uf -= dt*alpha*(p[] - p[-1])/Delta.  uf_x/y/z are
+   the face components in the user layout (n+1 faces along their own
+   direction, C order), device pointers when on_device, else host arrays
+   (copied in and out).  dt = 0 means 1 and nrelax = 0 means 4, as in the
+   reference.  Returns the poisson() status; the face update is done even
+   when that status is BMG_ERR_NAN. */
+int bmg_project(bmg_grid *g, double *uf_x, double *uf_y, double *uf_z, double dt, int nrelax, int on_device,
+                bmg_mgstats *stats);
+
 /* Gradient of an objective J(a) with respect to the coefficients (no
    counterpart in the reference: the adjoint behind the torch.autograd
    binding).  L(a) = lambda a + div(alpha grad a) is self-adjoint, so the

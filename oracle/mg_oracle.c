@@ -786,6 +786,56 @@ int orc_poisson(orc_grid *g, double tolerance, int nrelax, int minlevel, orc_mgs
   return r;
 }
 
+/* project(), poisson.h:469-509: div = sum over dimensions of
+   (uf.x[1] - uf.x[]), divided by dt*Delta; poisson() for p = a with
+   tolerance TOLERANCE/sq(dt); then on every face (walls included, the
+   multigrid foreach_face, grid/multigrid.h:255-279, 353-355)
+   uf.x[] -= dt*alpha.x[]*face_gradient_x(p, 0), with
+   face_gradient_x(a,i) = (a[i] - a[i-1])/Delta (common.h:1288) and the
+   ghosts of p from its (inhomogeneous) boundary conditions.  uf[c] holds
+   face component c in the user layout: n+1 faces along c, C order. */
+static inline size_t uface(int n, int c, int i, int j, int k) {
+  int ext[3] = {1, 1, 1};
+  for (int d = 0; d < DIM; d++) ext[d] = n + (d == c);
+  return ((size_t)i * ext[1] + j) * ext[2] + k;
+}
+
+int orc_project(orc_grid *g, double *const *uf, double dt, int nrelax, orc_mgstats *out) {
+  int l = g->depth, n = 1 << l;
+  if (dt == 0.) dt = 1.;        /* poisson.h:475 */
+  if (nrelax == 0) nrelax = 4;  /* poisson.h:476 */
+  double D = delta_l(g, l);
+  LOOP_LEVEL(l, n)
+    int ci = i - GHOSTS, cj = (DIM >= 2) ? j - GHOSTS : 0, ck = (DIM >= 3) ? k - GHOSTS : 0;
+    double div = 0.;
+    for (int c = 0; c < DIM; c++) {
+      int hi[3] = {ci, cj, ck};
+      hi[c] += 1;
+      div += uf[c][uface(n, c, hi[0], hi[1], hi[2])] - uf[c][uface(n, c, ci, cj, ck)];
+    }
+    div /= dt * D;
+    REC(g, l, i, j, k)[F_B] = div;
+  END_LOOP
+  int r = orc_poisson(g, g->TOLERANCE / (dt * dt), nrelax, 0, out);
+  /* the stencil read of p below fills its ghosts first */
+  boundary_level(g, F_A, l, 0, 1);
+  for (int c = 0; c < DIM; c++) {
+    int ext[3] = {1, 1, 1};
+    for (int d = 0; d < DIM; d++) ext[d] = n + (d == c);
+    ptrdiff_t sd = dstride(l, c);
+    for (int i = 0; i < ext[0]; i++)
+      for (int j = 0; j < ext[1]; j++)
+        for (int k = 0; k < ext[2]; k++) {
+          const double *rc = REC(g, l, i + GHOSTS, j + (DIM >= 2) * GHOSTS, k + (DIM >= 3) * GHOSTS);
+          double alpha = g->alpha_field ? rc[F_AX + c] : g->alpha_val;
+          double grad = (rc[F_A] - rc[-sd * NV + F_A]) / D;
+          double *u = &uf[c][uface(n, c, i, j, k)];
+          *u -= dt * alpha * grad;
+        }
+  }
+  return r;
+}
+
 void orc_set_threads(int n) {
 #ifdef _OPENMP
   omp_set_num_threads(n);

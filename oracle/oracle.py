@@ -74,6 +74,7 @@ def _lib(dim, omp=False):
             "orc_sum_b": (D, [P]),
             "orc_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "orc_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+            "orc_project": (I, [P, ctypes.POINTER(dp), D, I, ctypes.POINTER(MGStats)]),
             "orc_set_threads": (None, [I]),
             "orc_max_threads": (I, []),
             "orc_dimension": (I, []),
@@ -258,6 +259,21 @@ class OracleGrid:
     def mg_solve(self, tolerance=0.0, nrelax=0, minlevel=0):
         s = MGStats()
         self.lib.orc_mg_solve(self.g, tolerance, nrelax, minlevel, ctypes.byref(s))
+        return s.as_dict()
+
+    def project(self, uf, dt=1.0, nrelax=0):
+        """project(), poisson.h:469-509, with p = a and the grid's alpha:
+        updates the face components `uf` (float64 arrays, n+1 faces along
+        their own direction) in place and returns the poisson() stats."""
+        arrs = []
+        for c, u in enumerate(uf):
+            if u.dtype != np.float64 or not u.flags.c_contiguous or u.shape != self.face_shape(c):
+                raise ValueError("face component %d: need a C-contiguous float64 array of shape %s"
+                                 % (c, self.face_shape(c)))
+            arrs.append(u)
+        ptrs = (ctypes.POINTER(ctypes.c_double) * self.dim)(*[_dptr(u) for u in arrs])
+        s = MGStats()
+        self.lib.orc_project(self.g, ptrs, float(dt), int(nrelax), ctypes.byref(s))
         return s.as_dict()
 
     def poisson(self, tolerance=0.0, nrelax=0, minlevel=0):
