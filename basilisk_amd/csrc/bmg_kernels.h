@@ -1389,8 +1389,9 @@ __device__ __forceinline__ double bor(double a, double b) {
 // coefficients relax_value reads; like k_rb_stream3, blocks recompute them
 // instead of exchanging data.  Same arithmetic as relax_value.
 //
-// Everything a step reads from global memory is loaded during the step
-// before, through buffer resources of the plane (a plane outside the level
+// One barrier per step (between the loads and the red cells).  Everything
+// a step reads from global memory is loaded during the step before, through
+// buffer resources of the plane (a plane outside the level
 // gets an empty resource), and the three coefficient records rotate their
 // roles through a loop unrolled by three, so that no register copy and no
 // branch makes the compiler wait for a load early.
@@ -1616,7 +1617,10 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
         buf[slot(s)][qj][qk] = relax_nd<T>(D2, rcur.res, rcur.lam, rcur.al, rcur.ah, ap, am);
       }
     }
-    __syncthreads();
+    // no barrier here: the black cell of plane s-1 reads the red cells of
+    // planes s and s-2 at its own (j, k), which this thread wrote itself, and
+    // the red cells of plane s-1 around it, written during the step before
+    // (ordered by this step's barrier)
     if (black) {
       // black of plane s-1 at the k of this step's red cell; the pair is
       // written out with the red value of plane s-1
