@@ -288,6 +288,12 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // n >= 128) or Jacobi
 enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI };
 
+// a cell-count threshold with an environment override (tuning knob)
+size_t size_knob(const char* name, size_t dflt) {
+  const char* e = getenv(name);
+  return e && atoll(e) > 0 ? (size_t)atoll(e) : dflt;
+}
+
 // tile rows of the column-streamed sweep: 32 in FP32 (1024 threads), 16 in
 // FP64 (512 threads; BMG_STREAM_TJ=16 selects them in FP32 too)
 int stream_tj(size_t es) {
@@ -305,29 +311,25 @@ int stream_chunks(int n, size_t es) {
   }
   const int tj = stream_tj(es), tiles = (n / kSTK) * (n / tj);
   const int target = tj == kSTJ32 ? 256 : (es == 4 ? 512 : 256);
-  return std::max(1, std::min(n / 4, target / tiles));
+  const int minplanes = (int)size_knob("BMG_STREAM_MINPLANES", 4);
+  return std::max(1, std::min(n / std::max(1, minplanes), target / tiles));
 }
 
 // smallest level edge that takes the column-streamed kernel (BMG_STREAM3_MIN
-// overrides, 128 or more)
+// overrides, 128 or more); at 128^3 its chunks of 4 planes still beat the
+// two-launch in-place kernel
 int stream3_min() {
   if (const char* e = getenv("BMG_STREAM3_MIN")) {
     int v = atoi(e);
     if (v >= 128) return v;
   }
-  return 256;
-}
-
-// a cell-count threshold with an environment override (tuning knob)
-size_t size_knob(const char* name, size_t dflt) {
-  const char* e = getenv(name);
-  return e && atoll(e) > 0 ? (size_t)atoll(e) : dflt;
+  return 128;
 }
 
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
   if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
-  // the column-streamed single-pass kernel: 3D levels of 256^3 cells and
+  // the column-streamed single-pass kernel: 3D levels of 128^3 cells and
   // more (fewer planes give too few blocks), walls in every direction
   const bool walls = !g->periodic[0] && !g->periodic[1] && !g->periodic[2];
   // (buffer resources cover at most 2^31 bytes: up to 512^3 cells)
