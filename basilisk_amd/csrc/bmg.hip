@@ -533,7 +533,10 @@ int coarse_block(bmg_grid* g, int nrelax, int minlevel, int top) {
   A.nrelax = nrelax;
   A.jacobi = g->smoother == BMG_RELAX_JACOBI;
   ProfScope ps(g, PROF_RELAX_COARSE);
-  hipLaunchKernelGGL((k_coarse_cycle<DIM, T>), dim3(1), dim3(1024), 0, g->stream, A);
+  // threads of the workgroup (BMG_COARSE_BLOCK: a multiple of 64 up to 1024)
+  size_t nt = size_knob("BMG_COARSE_BLOCK", 1024);
+  if (nt > 1024 || nt % 64) nt = 1024;
+  hipLaunchKernelGGL((k_coarse_cycle<DIM, T>), dim3(1), dim3((unsigned)nt), 0, g->stream, A);
   return check_launch();
 }
 

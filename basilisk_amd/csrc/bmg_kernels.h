@@ -1963,7 +1963,7 @@ __global__ __launch_bounds__(1024) void k_coarse_cycle(CoarseArgs<T, DIM> A) {
     const CoarseLevel<T, DIM>& C = A.L[l];
     const CoarseLevel<T, DIM>& F = A.L[l + 1];
     idx_t nc = C.g.ncell();
-    for (idx_t idx = t; idx < nc; idx += 1024) {
+    for (idx_t idx = t; idx < nc; idx += (idx_t)blockDim.x) {
       int c[3];
       C.g.decode(idx, c);
       int f[3] = {2 * c[0], 2 * c[1], 2 * c[2]};
@@ -1997,10 +1997,10 @@ __global__ __launch_bounds__(1024) void k_coarse_cycle(CoarseArgs<T, DIM> A) {
     const CoarseLevel<T, DIM>& L = A.L[l];
     idx_t nl = L.g.ncell();
     if (l == A.minlevel) {
-      for (idx_t idx = t; idx < nl; idx += 1024) L.da[idx] = T(0);
+      for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) L.da[idx] = T(0);
     } else {
       const CoarseLevel<T, DIM>& C = A.L[l - 1];
-      for (idx_t idx = t; idx < nl; idx += 1024) {
+      for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) {
         int c[3];
         L.g.decode(idx, c);
         T v = bilinear_at<DIM, T>(C.da, C.g, C.bc, c);
@@ -2010,13 +2010,13 @@ __global__ __launch_bounds__(1024) void k_coarse_cycle(CoarseArgs<T, DIM> A) {
     __syncthreads();
     for (int it = 0; it < A.nrelax; it++) {
       if (A.jacobi) {
-        for (idx_t idx = t; idx < nl; idx += 1024) coarse_relax_cell<DIM, T>(L, idx, -1);
+        for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) coarse_relax_cell<DIM, T>(L, idx, -1);
         __syncthreads();
-        for (idx_t idx = t; idx < nl; idx += 1024) L.da[idx] = L.da2[idx];
+        for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) L.da[idx] = L.da2[idx];
         __syncthreads();
       } else {
         for (int colour = 0; colour < 2; colour++) {
-          for (idx_t idx = t; idx < nl; idx += 1024) {
+          for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) {
             int c[3];
             L.g.decode(idx, c);
             if (((c[0] + c[1] + c[2]) & 1) == colour) coarse_relax_cell<DIM, T>(L, idx, colour);

@@ -348,6 +348,7 @@ struct bmg_tree {
   // residual's kernels: the topology and every kernel argument are fixed
   // until a setting or the mesh changes, when they are dropped
   bool use_graphs = true;
+  int ops_block = kOpsBlock;  // threads of the batched small-operation kernel (BMG_OPS_BLOCK)
   bool batching = true;  // small operations run batched in one workgroup (BMG_TREE_BATCH=0: off)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> cycle_graphs;
   hipGraphExec_t residual_graph = nullptr;
@@ -1063,7 +1064,7 @@ struct Batcher {
   }
   bool small(long n) const { return t->batching && n <= kSmallOp; }
   void flush() {
-    if (P.n) k_t_ops<T, DIM><<<1, kOpsBlock, 0, t->stream>>>(P, A);
+    if (P.n) k_t_ops<T, DIM><<<1, t->ops_block, 0, t->stream>>>(P, A);
     P.n = 0;
   }
   void push(int code, int level, int arg, int field) {
@@ -1437,6 +1438,10 @@ int bmg_tree_new(bmg_tree** out, int dimension, double L0, double X0, double Y0,
   t->topo.init(dimension);
   const char* gr = getenv("BMG_GRAPHS");
   t->use_graphs = !(gr && gr[0] == '0');
+  if (const char* ob = getenv("BMG_OPS_BLOCK")) {
+    int v = atoi(ob);
+    if (v >= 64 && v <= kOpsBlock && v % 64 == 0) t->ops_block = v;
+  }
   const char* tb = getenv("BMG_TREE_BATCH");
   t->batching = !(tb && tb[0] == '0');
   *out = t;

@@ -511,7 +511,7 @@ __device__ __forceinline__ void t_ops_cells(const TreeArgs<T>& A, const TOp& op)
     const TLevelInfo L = A.lv[m];
     const int n = m == l ? L.nA : L.leaf_n;
     const double Dm = A.L0 * (1. / (1 << m));
-    for (int i = threadIdx.x; i < n; i += kOpsBlock) {
+    for (int i = threadIdx.x; i < n; i += (int)blockDim.x) {
       const int gs = m == l ? L.off + i : A.lvl_leaves[L.leaf_off + i];
       const int ai = L.abase + (gs - L.off);
       if (op.code == TOP_FILL) {
@@ -540,19 +540,19 @@ __global__ __launch_bounds__(kOpsBlock) void k_t_ops(TOps P, TreeArgs<T> A) {
     T* f = A.fld[op.field];
     switch (op.code) {
       case TOP_RESTRICT:
-        for (int r = threadIdx.x; r < L.n_nonleaf; r += kOpsBlock)
+        for (int r = threadIdx.x; r < L.n_nonleaf; r += (int)blockDim.x)
           t_restrict_one<T, DIM>(f, A.rec_self[L.rec_off + r], A.rec_child0[L.rec_off + r]);
         break;
       case TOP_PROLONG: {
         const int r0 = L.rec_off + (op.arg ? L.n_nonleaf : 0);
         const int n = op.arg ? L.n_prolong : L.n_nonleaf;
-        for (int r = threadIdx.x; r < n; r += kOpsBlock)
+        for (int r = threadIdx.x; r < n; r += (int)blockDim.x)
           t_prolong_one<T, DIM>(A.rec_ring + (size_t)(r0 + r) * NR, A.rec_child0[r0 + r], f);
         break;
       }
       case TOP_BOX: {
         const double D = A.L0 * (1. / (1 << op.level));
-        for (int r = threadIdx.x; r < L.box_n; r += kOpsBlock) {
+        for (int r = threadIdx.x; r < L.box_n; r += (int)blockDim.x) {
           const BoxRule q = A.box[L.box_off + r];
           t_box_one<T>(q, f, A.bc, op.arg, D);
         }
