@@ -19,6 +19,11 @@ from oracle.oracle import MGStats, build, SMOOTHERS, BC_SYMMETRY, BC_DIRICHLET, 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 GH = 2
 T_A, T_B, T_RES, T_DA, T_LAM, T_C, T_G = range(7)
+
+
+def t_s(dim):
+    """Index of the user scalar field (after the face fields g and alpha)."""
+    return T_G + 2 * dim
 BK_A, BK_HOM, BK_SYM = range(3)
 RIGHT, LEFT, TOP, BOTTOM, FRONT, BACK = range(6)
 
@@ -53,6 +58,7 @@ def _lib(dim):
             "ot_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "ot_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "ot_boundary": (None, [P, I, I, I]),
+            "ot_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I)]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -258,3 +264,21 @@ class OracleTree:
         s = MGStats()
         self.lib.ot_poisson(self.t, tolerance, nrelax, minlevel, ctypes.byref(s))
         return s.as_dict()
+
+    # -- adaptation
+    def set_s(self, v):
+        """Leaf values of the user scalar that adapt_wavelet estimates."""
+        self.set_leaf(t_s(self.dim), v)
+
+    def get_s(self):
+        return self.get_leaf(t_s(self.dim))
+
+    def get_b(self):
+        return self.get_leaf(T_B)
+
+    def adapt_wavelet(self, max, maxlevel, minlevel=1):
+        """adapt_wavelet ({s}, {max}, maxlevel, minlevel), updating s, a and
+        b (grid/tree-common.h:165-303); returns (nf, nc)."""
+        nf = ctypes.c_int(0)
+        nc = self.lib.ot_adapt_wavelet(self.t, float(max), int(maxlevel), int(minlevel), ctypes.byref(nf))
+        return nf.value, nc

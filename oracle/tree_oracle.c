@@ -39,6 +39,10 @@
  *                  gets halo_face; `res` gets the full tree boundary at the
  *                  restriction(res) of mg_cycle (tree_restriction).
  *
+ *   adaptation     grid/tree-common.h:98-131, 165-303 (coarsen_cell,
+ *                  adapt_wavelet's refinement and coarsening passes),
+ *                  grid/tree.h:990-1059 (free_children, decrement_neighbors)
+ *
  * A tree whose leaves are all on one level ("full") takes the reference's
  * tree_is_full() shortcuts: box BCs only at the given level, and
  * multigrid_restriction for restriction().
@@ -72,11 +76,16 @@
 #define GH 2
 #define NCH (1 << DIM)
 
-enum { C_ACTIVE = 1, C_LEAF = 2, C_FB = 4 };
-/* fields, per level, structure of arrays */
+/* cell flags; the last four are adapt_wavelet's user flags
+   (grid/tree-common.h:197-198, 201, 218: refined, too_fine, too_coarse,
+   just_fine) */
+enum { C_ACTIVE = 1, C_LEAF = 2, C_FB = 4, C_REFINED = 8, C_TOO_FINE = 16, C_TOO_COARSE = 32, C_JUST_FINE = 64 };
+/* fields, per level, structure of arrays; T_S is a user scalar (the one
+   adapt_wavelet estimates) */
 enum { T_A, T_B, T_RES, T_DA, T_LAM, T_C, T_G };
 #define T_AL (T_G + DIM)
-#define NVT (T_AL + DIM)
+#define T_S (T_AL + DIM)
+#define NVT (T_S + 1)
 
 enum { BC_SYMMETRY = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2 };
 enum { SM_GS = 0, SM_JACOBI = 1, SM_RBGS = 2 };
@@ -90,6 +99,7 @@ typedef struct {
   unsigned short *nb;
   signed char *pid;
   double *f[NVT];
+  long nc; /* allocated groups of children on this level (Layer.nc) */
 } otl;
 
 typedef struct {
@@ -121,6 +131,7 @@ typedef struct {
   /* problem */
   int bc[2 * DIM];
   double **bcv[2 * DIM]; /* [side][level] -> n^(DIM-1) values */
+  int bcv_n[2 * DIM];    /* levels given; deeper levels read zero */
   int lambda_field;
   double lambda_val;
   int alpha_field;
@@ -195,6 +206,7 @@ static void level_init(otl *L, int l) {
   L->nb = calloc(L->ncell, sizeof(unsigned short));
   L->pid = calloc(L->ncell, 1);
   for (int f = 0; f < NVT; f++) L->f[f] = calloc(L->ncell, sizeof(double));
+  L->nc = 0;
 }
 
 static void level_free(otl *L) {
@@ -260,9 +272,10 @@ ot_tree *ot_new(double L0, double X0, double Y0, double Z0) {
 
 static void free_bcv(ot_tree *t, int b) {
   if (!t->bcv[b]) return;
-  for (int l = 0; l <= t->depth; l++) free(t->bcv[b][l]);
+  for (int l = 0; l < t->bcv_n[b]; l++) free(t->bcv[b][l]);
   free(t->bcv[b]);
   t->bcv[b] = NULL;
+  t->bcv_n[b] = 0;
 }
 
 void ot_free(ot_tree *t) {
@@ -293,6 +306,7 @@ static void alloc_children(ot_tree *t, int l, const int c[3]) {
   int fi = 2 * c[0] - GH, fj = 2 * c[1] - GH, fk = 2 * c[2] - GH;
   if (F->alloc[lidx(F, fi, fj, DIM == 3 ? fk : 0)]) return;
   signed char pid = t->L[l].pid[lidx(&t->L[l], c[0], c[1], c[2])];
+  F->nc++;
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
@@ -301,6 +315,8 @@ static void alloc_children(ot_tree *t, int l, const int c[3]) {
         F->flags[q] = 0;
         F->nb[q] = 0;
         F->pid[q] = pid;
+        /* mempool_alloc0: a new cell's fields start at zero */
+        for (int f = 0; f < NVT; f++) F->f[f][q] = 0.;
       }
 }
 
@@ -324,8 +340,13 @@ static void increment_neighbors(ot_tree *t, int l, const int c[3]) {
   L->nb[q0]--;
 }
 
-/* refine_cell, grid/tree-common.h:21-67 (no MPI, no refined cache) */
-static int refine_cell(ot_tree *t, int l, const int c[3]) {
+static void halo_bilinear(ot_tree *t, int f, int l, size_t q);
+
+/* refine_cell, grid/tree-common.h:21-67 (no MPI, no refined cache): the
+   coarser neighbours the 2:1 rule needs are refined first and get `flag`;
+   the fields of `list` are prolongated into the new children with
+   refine_bilinear (multigrid-common.h:142-146) */
+static int refine_cell_f(ot_tree *t, int l, const int c[3], int flag, const int *list, int nlist) {
   int nr = 0;
   if (l > 0) {
     int ch[3], pc[3];
@@ -340,7 +361,10 @@ static int refine_cell(ot_tree *t, int l, const int c[3]) {
           int p[3] = {pc[0] + a, pc[1] + b, DIM == 3 ? pc[2] + e : 0};
           otl *P = &t->L[l - 1];
           size_t q = lidx(P, p[0], p[1], p[2]);
-          if (P->pid[q] >= 0 && (P->flags[q] & C_LEAF)) nr += refine_cell(t, l - 1, p);
+          if (P->pid[q] >= 0 && (P->flags[q] & C_LEAF)) {
+            nr += refine_cell_f(t, l - 1, p, flag, list, nlist);
+            t->L[l - 1].flags[q] |= flag;
+          }
         }
   }
   increment_neighbors(t, l, c);
@@ -352,8 +376,84 @@ static int refine_cell(ot_tree *t, int l, const int c[3]) {
     for (int b = 0; b < 2; b++)
       for (int e = 0; e < (DIM == 3 ? 2 : 1); e++)
         F->flags[lidx(F, 2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0)] |= cflag;
+  for (int m = 0; m < nlist; m++) halo_bilinear(t, list[m], l, q);
   L->flags[q] &= ~C_LEAF;
   return nr + 1;
+}
+
+static int refine_cell(ot_tree *t, int l, const int c[3]) { return refine_cell_f(t, l, c, 0, NULL, 0); }
+
+/* free_children, grid/tree.h:990-1004: the layer goes when its last group
+   of children does (update_depth (-1)) */
+static void free_children(ot_tree *t, int l, const int c[3]) {
+  otl *F = &t->L[l + 1];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+        size_t q = lidx(F, 2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0);
+        F->alloc[q] = 0;
+        F->flags[q] = 0;
+        F->nb[q] = 0;
+      }
+  if (--F->nc == 0) {
+    if (l + 1 != t->depth) {
+      fprintf(stderr, "tree oracle: layer %d emptied below the depth %d\n", l + 1, t->depth);
+      abort();
+    }
+    level_free(F);
+    memset(F, 0, sizeof(otl));
+    t->depth--;
+    t->dirty = 1;
+  }
+}
+
+/* decrement_neighbors, grid/tree.h:1043-1059 */
+static void decrement_neighbors(ot_tree *t, int l, const int c[3]) {
+  t->dirty = 1;
+  for (int a = -1; a <= 1; a++)
+    for (int b = -1; b <= 1; b++)
+      for (int e = (DIM == 3 ? -1 : 0); e <= (DIM == 3 ? 1 : 0); e++) {
+        int n[3] = {c[0] + a, c[1] + b, c[2] + e};
+        otl *L = &t->L[l];
+        size_t q = lidx(L, n[0], n[1], n[2]);
+        if (L->alloc[q]) {
+          L->nb[q]--;
+          if (L->nb[q] == 0) free_children(t, l, n);
+        }
+      }
+  otl *L = &t->L[l];
+  size_t q0 = lidx(L, c[0], c[1], c[2]);
+  if (L->nb[q0]) {
+    otl *F = &t->L[l + 1];
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+          size_t q = lidx(F, 2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0);
+          F->flags[q] = 0;
+          F->pid[q] = L->pid[q0];
+        }
+  }
+}
+
+static double child_avg(ot_tree *t, int f, int l, size_t q);
+
+/* coarsen_cell, grid/tree-common.h:98-131 (no MPI): refused when a child
+   has a refined neighbour; the fields of `list` take the children's average
+   (restriction_average) */
+static int coarsen_cell(ot_tree *t, int l, const int c[3], const int *list, int nlist) {
+  otl *L = &t->L[l], *F = &t->L[l + 1];
+  size_t q = lidx(L, c[0], c[1], c[2]);
+  int pid = L->pid[q];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+        size_t p = lidx(F, 2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0);
+        if (F->nb[p] || (F->pid[p] < 0 && F->pid[p] != pid)) return 0;
+      }
+  for (int m = 0; m < nlist; m++) L->f[list[m]][q] = child_avg(t, list[m], l, q);
+  L->flags[q] |= C_LEAF;
+  decrement_neighbors(t, l, c);
+  return 1;
 }
 
 /* ---------------------------------------------------------- caches */
@@ -544,6 +644,7 @@ void ot_set_bc(ot_tree *t, int b, int kind, const double *vals) {
   t->bc[b] = kind;
   if (!vals) return;
   t->bcv[b] = calloc(t->depth + 1, sizeof(double *));
+  t->bcv_n[b] = t->depth + 1;
   size_t off = 0;
   for (int l = 0; l <= t->depth; l++) {
     size_t m = 1;
@@ -566,7 +667,7 @@ static inline double bc_value(const ot_tree *t, int bk, int b, int l, const int 
   size_t ti = 0;
   for (int e = 0; e < DIM; e++)
     if (e != d) ti = ti * n + (size_t)(c[e] - GH);
-  double v = t->bcv[b] ? t->bcv[b][l][ti] : 0.;
+  double v = t->bcv[b] && l < t->bcv_n[b] ? t->bcv[b][l][ti] : 0.;
   if (kind == BC_DIRICHLET) return 2. * v - s;
   return delta_l(t, l) * v + s;
 }
@@ -1072,4 +1173,139 @@ int ot_poisson(ot_tree *t, double tolerance, int nrelax, int minlevel, ot_mgstat
 void ot_boundary(ot_tree *t, int f, int bk, int l) {
   ot_update(t);
   tree_boundary_scalar(t, f, bk, l);
+}
+
+/* ---------------------------------------------------------- adaptation */
+
+/* the fields adapt_wavelet keeps up to date (its `list`): the estimated
+   scalar s and the solver's a and b, with their boundary conditions */
+static const int adapt_list[3] = {T_S, T_A, T_B};
+static const int adapt_bk[3] = {BK_SYM, BK_A, BK_SYM};
+
+/* the refinement pass of adapt_wavelet (grid/tree-common.h:196-265): a
+   pre-order foreach_cell() in which `continue` skips a cell's children */
+static void adapt_refine_visit(ot_tree *t, int l, const int c[3], double max, int maxlevel, int minlevel, int *nf) {
+  otl *L = &t->L[l];
+  size_t q = lidx(L, c[0], c[1], c[2]);
+  if (!L->alloc[q] || !(L->flags[q] & C_ACTIVE)) return;
+  if (L->flags[q] & C_LEAF) {
+    if (L->flags[q] & C_TOO_COARSE) {
+      L->flags[q] &= ~C_TOO_COARSE;
+      refine_cell_f(t, l, c, C_REFINED, adapt_list, 3);
+      (*nf)++;
+    }
+    return;
+  }
+  if (L->flags[q] & C_REFINED) { /* refined earlier in this pass: skip its children */
+    L->flags[q] &= ~C_TOO_COARSE;
+    return;
+  }
+  {
+    otl *F = &t->L[l + 1];
+    size_t ch[NCH];
+    double sc[NCH];
+    int m = 0;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++, m++) {
+          ch[m] = lidx(F, 2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0);
+          sc[m] = F->f[T_S][ch[m]];
+        }
+    halo_bilinear(t, T_S, l, q); /* s.prolongation (point, s) */
+    const int lev = l + 1;
+    for (m = 0; m < NCH; m++) {
+      unsigned char *fl = &F->flags[ch[m]];
+      double e = fabs(sc[m] - F->f[T_S][ch[m]]);
+      if (e > max && lev < maxlevel) {
+        *fl &= ~C_TOO_FINE;
+        *fl |= C_TOO_COARSE;
+      } else if ((e <= max / 1.5 || lev > maxlevel) && !(*fl & (C_TOO_COARSE | C_JUST_FINE))) {
+        if (lev >= minlevel) *fl |= C_TOO_FINE;
+      } else if (!(*fl & C_TOO_COARSE)) {
+        *fl &= ~C_TOO_FINE;
+        *fl |= C_JUST_FINE;
+      }
+      F->f[T_S][ch[m]] = sc[m];
+    }
+    for (m = 0; m < NCH; m++) {
+      unsigned char *fl = &F->flags[ch[m]];
+      *fl &= ~C_JUST_FINE;
+      if (!(*fl & C_LEAF)) {
+        *fl &= ~C_TOO_COARSE;
+        if (lev >= maxlevel) *fl |= C_TOO_FINE;
+      } else if (!(*fl & C_ACTIVE))
+        *fl &= ~C_TOO_COARSE;
+    }
+  }
+  if (l < t->depth)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+          int cc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
+          adapt_refine_visit(t, l + 1, cc, max, maxlevel, minlevel, nf);
+        }
+}
+
+/* one level l of the coarsening passes (grid/tree-common.h:268-294) */
+static void adapt_coarsen_visit(ot_tree *t, int m, const int c[3], int l, int *nc) {
+  if (m > t->depth) return;
+  otl *L = &t->L[m];
+  size_t q = lidx(L, c[0], c[1], c[2]);
+  if (!L->alloc[q]) return;
+  if (!is_boundary(L, q)) {
+    if (m == l) {
+      if (!(L->flags[q] & C_LEAF)) {
+        if (L->flags[q] & C_REFINED)
+          L->flags[q] &= ~(C_REFINED | C_TOO_FINE);
+        else if (L->flags[q] & C_TOO_FINE) {
+          if (coarsen_cell(t, m, c, adapt_list, 3)) (*nc)++;
+          L = &t->L[m];
+          L->flags[q] &= ~C_TOO_FINE; /* do not coarsen the parent */
+        }
+      }
+      if (L->flags[q] & C_TOO_FINE)
+        L->flags[q] &= ~C_TOO_FINE;
+      else if (m > 0) {
+        otl *P = &t->L[m - 1];
+        size_t p = lidx(P, (c[0] + GH) / 2, (c[1] + GH) / 2, DIM == 3 ? (c[2] + GH) / 2 : 0);
+        if (P->flags[p] & C_TOO_FINE) P->flags[p] &= ~C_TOO_FINE;
+      }
+      return;
+    } else if (L->flags[q] & C_LEAF)
+      return;
+  }
+  if (m < t->depth)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+          int cc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
+          adapt_coarsen_visit(t, m + 1, cc, l, nc);
+        }
+}
+
+/* adapt_wavelet ({s}, {max}, maxlevel, minlevel) on a Cartesian tree
+   (grid/tree-common.h:165-303, no MPI): s is the field T_S, and the fields
+   it refines and coarsens are s, a and b.  Returns the number of coarsened
+   cells; *nf receives the number of refined ones. */
+int ot_adapt_wavelet(ot_tree *t, double max, int maxlevel, int minlevel, int *nf) {
+  ot_update(t);
+  /* boundary (list); restriction ({s}) */
+  if (is_full(t)) {
+    for (int m = 0; m < 3; m++) box_scalar(t, adapt_list[m], adapt_bk[m], t->depth);
+    for (int l = t->depth - 1; l >= 0; l--) { /* multigrid_restriction */
+      restrict_level(t, T_S, l);
+      box_scalar(t, T_S, BK_SYM, l);
+    }
+  } else
+    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+  if (minlevel < 1) minlevel = 1;
+  int nrf = 0, ncs = 0;
+  const int root[3] = {GH, GH, DIM == 3 ? GH : 0};
+  adapt_refine_visit(t, 0, root, max, maxlevel, minlevel, &nrf);
+  for (int l = t->depth; l >= 0; l--) adapt_coarsen_visit(t, 0, root, l, &ncs);
+  t->dirty = 1;
+  if (nrf || ncs) /* mpi_boundary_update (list) */
+    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+  if (nf) *nf = nrf;
+  return ncs;
 }
