@@ -27,6 +27,7 @@ RIGHT, LEFT, TOP, BOTTOM, FRONT, BACK = range(6)
 SMOOTHERS = {"gs": 0, "jacobi": 1, "rbgs": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 F_A, F_B, F_RES, F_DA, F_LAMBDA, F_ALPHA = range(6)
+F_S = 8  # trees: the scalar adapt_wavelet estimates
 PROFILE_KINDS = {"relax_fine": 0, "residual": 1, "restriction": 2,
                  "prolongation": 3, "correction": 4, "relax_coarse": 5}
 
@@ -47,6 +48,7 @@ EXPORTS = [
     "bmg_tree_set_alpha_faces", "bmg_tree_set_alpha_const", "bmg_tree_set_lambda_const",
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
+    "bmg_tree_adapt_wavelet",
     "bmg_coef_gradient", "bmg_project",
 ]
 
@@ -143,6 +145,7 @@ def library():
         "bmg_tree_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_synchronize": (I, [P]),
+        "bmg_tree_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
         "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
     }
@@ -632,6 +635,23 @@ class Tree:
 
     def get_da(self):
         return self._get(F_DA)
+
+    # -- adaptation (grid/tree-common.h)
+    def set_s(self, v):
+        """Leaf values of the scalar that adapt_wavelet estimates."""
+        self._put(F_S, v)
+
+    def get_s(self, out=None):
+        return self._get(F_S, out)
+
+    def adapt_wavelet(self, max, maxlevel, minlevel=1):
+        """adapt_wavelet ({s}, {max}, maxlevel, minlevel) with s, a and b
+        following the mesh; returns (nf, nc), the refined and coarsened cell
+        counts.  alpha and a lambda field must be set again afterwards."""
+        nf, nc = ctypes.c_int(0), ctypes.c_int(0)
+        _check(self.lib.bmg_tree_adapt_wavelet(self._t, float(max), int(maxlevel), int(minlevel),
+                                               ctypes.byref(nf), ctypes.byref(nc)))
+        return nf.value, nc.value
 
     def set_lambda(self, v):
         if np.isscalar(v):

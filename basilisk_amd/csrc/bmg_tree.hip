@@ -38,7 +38,12 @@ namespace {
 
 constexpr int GH = 2;
 constexpr int kMaxDepth = 16;
-enum : uint8_t { C_ACTIVE = 1, C_LEAF = 2, C_MARK = 4, C_ALLOC = 8 };
+// cell flags; the last four are adapt_wavelet's (grid/tree-common.h:197-198,
+// 201, 218: refined, too_fine, too_coarse, just_fine)
+enum : uint8_t {
+  C_ACTIVE = 1, C_LEAF = 2, C_MARK = 4, C_ALLOC = 8,
+  C_REFINED = 16, C_TOO_FINE = 32, C_TOO_COARSE = 64, C_JUST_FINE = 128
+};
 
 struct Cell {
   uint8_t flags = 0;
@@ -98,10 +103,20 @@ struct Face {
   int flags;
 };
 
+// cells refined by adapt_wavelet, each with the slots (in the layout before
+// the mesh change) of its 3^dim ring, which refine_bilinear reads
+struct RefineLog {
+  std::vector<int> level;
+  std::vector<IJK> cell;
+  std::vector<int> ring;
+  bool ok = true;
+};
+
 // host-side topology, following the reference's tree
 struct Topo {
   int dim = 2, depth = 0;
   std::vector<TLevel> L;
+  std::vector<long> nc;  // allocated groups of children per level (Layer.nc)
   bool dirty = true;
   // caches
   std::vector<std::vector<IJK>> active, prolong, bnd;
@@ -126,6 +141,7 @@ struct Topo {
     L.clear();
     L.resize(1);
     L[0].init(0, dim);
+    nc.assign(1, 0);
     int kmax = dim == 3 ? 2 * GH : 0;
     for (int i = 0; i <= 2 * GH; i++)
       for (int j = 0; j <= 2 * GH; j++)
@@ -144,6 +160,7 @@ struct Topo {
 
   void add_level() {
     L.emplace_back();
+    nc.push_back(0);
     depth++;
     L[depth].init(depth, dim);
   }
@@ -156,6 +173,7 @@ struct Topo {
     else if (alloc(l + 1, 2 * c.i - GH, 2 * c.j - GH, dim == 3 ? 2 * c.k - GH : 0))
       return;
     int8_t pid = at(l, c.i, c.j, c.k).pid;
+    nc[l + 1]++;
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++)
         for (int e = 0; e < kext(); e++) {
@@ -163,6 +181,7 @@ struct Topo {
           ch.flags = C_ALLOC;
           ch.nb = 0;
           ch.pid = pid;
+          ch.slot = -1;
         }
   }
 
@@ -179,7 +198,9 @@ struct Topo {
     at(l, c.i, c.j, c.k).nb--;
   }
 
-  void refine_cell(int l, IJK c) {
+  // refine_cell with the 2:1 rule; the coarser neighbours it refines get
+  // `flag`; with a log, every refined cell is recorded with its ring's slots
+  void refine_cell(int l, IJK c, uint8_t flag = 0, RefineLog* log = nullptr) {
     if (l > 0) {
       int ch[3] = {2 * ((c.i + GH) % 2) - 1, 2 * ((c.j + GH) % 2) - 1, dim == 3 ? 2 * ((c.k + GH) % 2) - 1 : 1};
       int pc[3] = {(c.i + GH) / 2, (c.j + GH) / 2, dim == 3 ? (c.k + GH) / 2 : 0};
@@ -188,10 +209,26 @@ struct Topo {
           for (int e = 0; e != (dim == 3 ? 2 * ch[2] : 1); e += (dim == 3 ? ch[2] : 1)) {
             IJK p = {pc[0] + a, pc[1] + b, dim == 3 ? pc[2] + e : 0};
             Cell& pcell = at(l - 1, p.i, p.j, p.k);
-            if (pcell.pid >= 0 && (pcell.flags & C_LEAF)) refine_cell(l - 1, p);
+            if (pcell.pid >= 0 && (pcell.flags & C_LEAF)) {
+              refine_cell(l - 1, p, flag, log);
+              at(l - 1, p.i, p.j, p.k).flags |= flag;
+            }
           }
     }
     increment_neighbors(l, c);
+    if (log) {
+      log->level.push_back(l);
+      log->cell.push_back(c);
+      int e0 = dim == 3 ? -1 : 0, e1 = dim == 3 ? 1 : 0;
+      for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+          for (int e = e0; e <= e1; e++) {
+            Cell* n = L[l].find(c.i + a, c.j + b, c.k + e);
+            int s = n ? n->slot : -1;
+            if (s < 0) log->ok = false;
+            log->ring.push_back(s);
+          }
+    }
     Cell& cell = at(l, c.i, c.j, c.k);
     uint8_t cflag = (cell.flags & C_ACTIVE) ? (C_ACTIVE | C_LEAF) : C_LEAF;
     for (int a = 0; a < 2; a++)
@@ -199,6 +236,168 @@ struct Topo {
         for (int e = 0; e < kext(); e++)
           at(l + 1, 2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0).flags |= cflag;
     at(l, c.i, c.j, c.k).flags &= (uint8_t)~C_LEAF;
+  }
+
+  // free_children, grid/tree.h:990-1004: the deepest layer goes with its last
+  // group of children (update_depth (-1))
+  bool free_children(int l, IJK c) {
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < kext(); e++) {
+          Cell* ch = L[l + 1].find(2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0);
+          if (!ch) return false;
+          ch->flags = 0;
+          ch->nb = 0;
+        }
+    if (--nc[l + 1] == 0) {
+      if (l + 1 != depth) return false;
+      L.pop_back();
+      nc.pop_back();
+      depth--;
+    }
+    return true;
+  }
+
+  // decrement_neighbors, grid/tree.h:1043-1059
+  bool decrement_neighbors(int l, IJK c) {
+    dirty = true;
+    bool ok = true;
+    int e0 = dim == 3 ? -1 : 0, e1 = dim == 3 ? 1 : 0;
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        for (int e = e0; e <= e1; e++) {
+          IJK n = {c.i + a, c.j + b, c.k + e};
+          Cell* p = L[l].find(n.i, n.j, n.k);
+          if (p && --p->nb == 0) ok = free_children(l, n) && ok;
+        }
+    Cell& cc = at(l, c.i, c.j, c.k);
+    if (cc.nb)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          for (int e = 0; e < kext(); e++) {
+            Cell& ch = at(l + 1, 2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0);
+            ch.flags = C_ALLOC;
+            ch.pid = cc.pid;
+          }
+    return ok;
+  }
+
+  // coarsen_cell, grid/tree-common.h:98-131 (the fields are handled on the
+  // GPU): refused when a child has a refined neighbour
+  int coarsen_cell(int l, IJK c) {
+    Cell& cc = at(l, c.i, c.j, c.k);
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < kext(); e++) {
+          Cell* ch = L[l + 1].find(2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0);
+          if (!ch) return -1;
+          if (ch->nb || (ch->pid < 0 && ch->pid != cc.pid)) return 0;
+        }
+    cc.flags |= C_LEAF;
+    return decrement_neighbors(l, c) ? 1 : -1;
+  }
+
+  // adapt_wavelet's passes over the topology (grid/tree-common.h:196-294);
+  // err[slot] is the wavelet estimate of each child of a refined cell
+  struct Adapt {
+    const std::vector<double>* err = nullptr;
+    double max = 0.;
+    int maxlevel = 0, minlevel = 1;
+    long nf = 0, nc = 0;
+    RefineLog log;
+    bool ok = true;
+  };
+
+  // the refinement pass: a pre-order foreach_cell() in which `continue`
+  // skips a cell's children
+  void adapt_refine(int l, IJK c, Adapt& A) {
+    Cell* cp = L[l].find(c.i, c.j, c.k);
+    if (!cp || !(cp->flags & C_ACTIVE)) return;
+    if (cp->flags & C_LEAF) {
+      if (cp->flags & C_TOO_COARSE) {
+        cp->flags &= (uint8_t)~C_TOO_COARSE;
+        refine_cell(l, c, C_REFINED, &A.log);
+        A.nf++;
+      }
+      return;
+    }
+    if (cp->flags & C_REFINED) {  // refined earlier in this pass: skip its children
+      cp->flags &= (uint8_t)~C_TOO_COARSE;
+      return;
+    }
+    const int lev = l + 1;
+    Cell* ch[8];
+    int m = 0;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < kext(); e++, m++) {
+          ch[m] = L[l + 1].find(2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0);
+          if (!ch[m] || ch[m]->slot < 0 || ch[m]->slot >= (long)A.err->size()) {
+            A.ok = false;
+            return;
+          }
+        }
+    for (m = 0; m < (1 << dim); m++) {
+      uint8_t& fl = ch[m]->flags;
+      const double e = (*A.err)[ch[m]->slot];
+      if (e > A.max && lev < A.maxlevel) {
+        fl &= (uint8_t)~C_TOO_FINE;
+        fl |= C_TOO_COARSE;
+      } else if ((e <= A.max / 1.5 || lev > A.maxlevel) && !(fl & (C_TOO_COARSE | C_JUST_FINE))) {
+        if (lev >= A.minlevel) fl |= C_TOO_FINE;
+      } else if (!(fl & C_TOO_COARSE)) {
+        fl &= (uint8_t)~C_TOO_FINE;
+        fl |= C_JUST_FINE;
+      }
+    }
+    for (m = 0; m < (1 << dim); m++) {
+      uint8_t& fl = ch[m]->flags;
+      fl &= (uint8_t)~C_JUST_FINE;
+      if (!(fl & C_LEAF)) {
+        fl &= (uint8_t)~C_TOO_COARSE;
+        if (lev >= A.maxlevel) fl |= C_TOO_FINE;
+      } else if (!(fl & C_ACTIVE))
+        fl &= (uint8_t)~C_TOO_COARSE;
+    }
+    if (l < depth)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          for (int e = 0; e < kext(); e++)
+            adapt_refine(l + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0}, A);
+  }
+
+  // one level l of the coarsening passes
+  void adapt_coarsen(int m, IJK c, int l, Adapt& A) {
+    if (m > depth) return;
+    Cell* cp = L[m].find(c.i, c.j, c.k);
+    if (!cp) return;
+    if (!is_boundary(*cp)) {
+      if (m == l) {
+        if (!(cp->flags & C_LEAF)) {
+          if (cp->flags & C_REFINED)
+            cp->flags &= (uint8_t)~(C_REFINED | C_TOO_FINE);
+          else if (cp->flags & C_TOO_FINE) {
+            int r = coarsen_cell(m, c);
+            if (r < 0) A.ok = false;
+            if (r > 0) A.nc++;
+            cp->flags &= (uint8_t)~C_TOO_FINE;  // do not coarsen the parent
+          }
+        }
+        if (cp->flags & C_TOO_FINE)
+          cp->flags &= (uint8_t)~C_TOO_FINE;
+        else if (m > 0) {
+          Cell* pp = L[m - 1].find((c.i + GH) / 2, (c.j + GH) / 2, dim == 3 ? (c.k + GH) / 2 : 0);
+          if (pp && (pp->flags & C_TOO_FINE)) pp->flags &= (uint8_t)~C_TOO_FINE;
+        }
+        return;
+      } else if (cp->flags & C_LEAF)
+        return;
+    }
+    if (m < depth)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          for (int e = 0; e < kext(); e++)
+            adapt_coarsen(m + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0}, l, A);
   }
 
   bool is_boundary(const Cell& c) const { return c.pid < 0; }
@@ -335,6 +534,8 @@ struct bmg_tree {
   bool compiled = false;
   DevIndex ix;
   double *a = nullptr, *b = nullptr, *res64 = nullptr, *g[3] = {nullptr, nullptr, nullptr};
+  double* s = nullptr;  // the user scalar adapt_wavelet estimates (BMG_F_S)
+  bool s_set = false;
   double *al64[3] = {nullptr, nullptr, nullptr}, *lam64 = nullptr;
   void *da = nullptr, *res = nullptr, *tmp = nullptr, *alT[3] = {nullptr, nullptr, nullptr}, *lamT = nullptr;
   double* bcv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -343,7 +544,9 @@ struct bmg_tree {
   double *part = nullptr, *scalar = nullptr;
   hipStream_t own_stream = nullptr, stream = nullptr;
   // field values kept across a recompilation, in leaf order
-  std::vector<double> keep_a, keep_b, keep_lam;
+  std::vector<double> keep_a, keep_b, keep_lam, keep_s;
+  // when set, compile() records each new slot's slot in the previous layout
+  std::vector<int>* prev_slots = nullptr;
   // HIP graphs of the cycle (per nrelax, minlevel, maxlevel) and of the
   // residual's kernels: the topology and every kernel argument are fixed
   // until a setting or the mesh changes, when they are dropped
@@ -375,7 +578,8 @@ void free_device(bmg_tree* t) {
     dfree_(p);
   x = DevIndex();
   bool mixed = t->precision == BMG_MIXED;
-  dfree_(t->a); dfree_(t->b); dfree_(t->res64);
+  dfree_(t->a); dfree_(t->b); dfree_(t->res64); dfree_(t->s);
+  t->s = nullptr;
   for (int d = 0; d < 3; d++) {
     dfree_(t->g[d]);
     dfree_(t->al64[d]);
@@ -442,11 +646,16 @@ int compile(bmg_tree* t) {
   // slot assignment: active, halo children, ghost blocks
   std::vector<std::vector<IJK>> ghosts(D + 1);
   long next = 0;
+  if (t->prev_slots) t->prev_slots->clear();
+  auto take = [&](Cell* c) {
+    if (t->prev_slots) t->prev_slots->push_back(c->slot);
+    c->slot = (int)next++;
+  };
   for (int l = 0; l <= D; l++) {
     TLevel& Lv = T.L[l];
     x.off[l] = (int)next;
     x.abase[l] = l ? x.abase[l - 1] + x.nA[l - 1] : 0;
-    for (auto& c : T.active[l]) Lv.find(c.i, c.j, c.k)->slot = (int)next++;
+    for (auto& c : T.active[l]) take(Lv.find(c.i, c.j, c.k));
     x.nA[l] = (int)T.active[l].size();
     if (l > 0) {
       for (auto& p : T.prolong[l - 1])
@@ -455,7 +664,7 @@ int compile(bmg_tree* t) {
             for (int e = 0; e < T.kext(); e++) {
               Cell* ch = Lv.find(2 * p.i - GH + a, 2 * p.j - GH + b, dim == 3 ? 2 * p.k - GH + e : 0);
               if (!ch) return bmg::fail(BMG_ERR_ARG, "tree: missing halo child");
-              ch->slot = (int)next++;
+              take(ch);
             }
       for (auto& gp : ghosts[l - 1]) {
         Cell* gc = T.L[l - 1].find(gp.i, gp.j, gp.k);
@@ -466,7 +675,7 @@ int compile(bmg_tree* t) {
               IJK c = {2 * gp.i - GH + a, 2 * gp.j - GH + b, dim == 3 ? 2 * gp.k - GH + e : 0};
               Cell* ch = Lv.find(c.i, c.j, c.k);
               if (!ch) return bmg::fail(BMG_ERR_ARG, "tree: missing ghost child");
-              ch->slot = (int)next++;
+              take(ch);
               ghosts[l].push_back(c);
             }
       }
@@ -477,7 +686,7 @@ int compile(bmg_tree* t) {
           for (int k = 0; k <= kmax; k++) {
             Cell* c = Lv.find(i, j, k);
             if (c->pid < 0) {
-              c->slot = (int)next++;
+              take(c);
               ghosts[0].push_back({i, j, k});
             }
           }
@@ -852,6 +1061,7 @@ int compile(bmg_tree* t) {
   size_t ts = tsize(t);
   TRY(dalloc_((void**)&t->a, NS * 8));
   TRY(dalloc_((void**)&t->b, NS * 8));
+  TRY(dalloc_((void**)&t->s, NS * 8));
   TRY(dalloc_((void**)&t->res64, NS * 8));
   TRY(dalloc_((void**)&t->lam64, NS * 8));
   for (int d = 0; d < dim; d++) {
@@ -873,8 +1083,9 @@ int compile(bmg_tree* t) {
   TRY(dalloc_(&t->tmp, NS * ts));
   for (int s = 0; s < 2 * dim; s++) {
     TRY(dalloc_((void**)&t->bcv[s], sizeof(double) * x.bc_off[D + 1]));
-    if ((int)t->bc_host[s].size() >= x.bc_off[D + 1])
-      HIPCHECK(hipMemcpy(t->bcv[s], t->bc_host[s].data(), sizeof(double) * x.bc_off[D + 1], hipMemcpyHostToDevice));
+    // the levels given; deeper ones (a tree grown since) read zero
+    size_t nv = std::min<size_t>(t->bc_host[s].size(), (size_t)x.bc_off[D + 1]);
+    if (nv) HIPCHECK(hipMemcpy(t->bcv[s], t->bc_host[s].data(), sizeof(double) * nv, hipMemcpyHostToDevice));
   }
   TRY(dalloc_((void**)&t->stat, sizeof(TStat)));
   TRY(dalloc_((void**)&t->part, sizeof(double) * 1024));
@@ -893,6 +1104,7 @@ int compile(bmg_tree* t) {
   };
   TRY(restore(t->keep_a, t->a));
   TRY(restore(t->keep_b, t->b));
+  TRY(restore(t->keep_s, t->s));
   TRY(restore(t->keep_lam, t->lam64));
   return BMG_OK;
 }
@@ -1418,6 +1630,105 @@ int get_leaf_values(bmg_tree* t, const T* f, double* v, int on_device) {
   return BMG_OK;
 }
 
+// ------------------------------------------------------------ adapt_wavelet
+
+// adapt_wavelet's boundary (list) and restriction ({s}) for list = {s, a, b}:
+// every level of the three fields restricted and its ghosts and halos filled
+// (a and b are also restricted on a full tree, where the reference leaves
+// their coarse levels alone: coarsen_cell recomputes the same averages)
+template <int DIM>
+void adapt_boundary(bmg_tree* t) {
+  Batcher<double, DIM> B(t, t->s, t->a, t->b);
+  const int bk[3] = {BK_SYM, BK_A, BK_SYM};
+  if (t->topo.full()) {
+    const int D = t->ix.depth;
+    for (int f = 0; f < 3; f++) {
+      B.box(f, bk[f], D);
+      for (int l = D - 1; l >= 0; l--) {
+        B.restrict_level(f, l);
+        B.box(f, bk[f], l);
+      }
+    }
+  } else {
+    for (int f = 0; f < 3; f++) B.tree_boundary(f, bk[f], -1);
+  }
+  B.flush();
+}
+
+// the wavelet estimate of every child of a refined cell, by slot
+template <int DIM>
+int adapt_estimate(bmg_tree* t, std::vector<double>& err_h) {
+  const DevIndex& x = t->ix;
+  double* err = nullptr;
+  TRY(dalloc_((void**)&err, x.nslot * 8));
+  const int NR = DIM == 3 ? 27 : 9;
+  for (int l = 0; l <= x.depth; l++) {
+    int n = x.n_nonleaf[l];
+    if (n)
+      k_t_wavelet<DIM><<<nb_(n), kB, 0, t->stream>>>(x.rec_ring + (size_t)x.rec_off[l] * NR,
+                                                     x.rec_child0 + x.rec_off[l], n, t->s, err);
+  }
+  err_h.resize(x.nslot);
+  hipError_t e = hipMemcpyAsync(err_h.data(), err, x.nslot * 8, hipMemcpyDeviceToHost, t->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+  dfree_(err);
+  if (e != hipSuccess) return bmg::fail(BMG_ERR_HIP, "wavelet estimate: %s", hipGetErrorString(e));
+  return BMG_OK;
+}
+
+// s, a and b in the new layout: the old value of every cell that stays, the
+// bilinear of the old ring in the children of the refined cells, then the
+// tree boundary (mpi_boundary_update (list))
+template <int DIM>
+int adapt_remap(bmg_tree* t, double* const olds[3], long old_nslot, const std::vector<int>& prev,
+                const RefineLog& log) {
+  const DevIndex& x = t->ix;
+  Topo& T = t->topo;
+  // every slot the kernels read lies in the old arrays
+  if ((long)prev.size() != x.nslot) return bmg::fail(BMG_ERR_ARG, "tree adaptation: slot map size");
+  for (int p : prev)
+    if (p >= old_nslot) return bmg::fail(BMG_ERR_ARG, "tree adaptation: slot map out of range");
+  for (int p : log.ring)
+    if (p < 0 || p >= old_nslot) return bmg::fail(BMG_ERR_ARG, "tree adaptation: ring outside the old layout");
+  const size_t nr = log.level.size();
+  std::vector<int> child0(nr);
+  for (size_t r = 0; r < nr; r++) {
+    const int l = log.level[r];
+    const IJK c = log.cell[r];
+    int m = 0;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < T.kext(); e++, m++) {
+          Cell* ch = T.L[l + 1].find(2 * c.i - GH + a, 2 * c.j - GH + b, DIM == 3 ? 2 * c.k - GH + e : 0);
+          if (!ch || (m > 0 && ch->slot != child0[r] + m))
+            return bmg::fail(BMG_ERR_ARG, "tree adaptation: children of a refined cell not in one block");
+          if (m == 0) child0[r] = ch->slot;
+        }
+  }
+  int *dprev = nullptr, *dring = nullptr, *dchild = nullptr;
+  int rc = upload(&dprev, prev);
+  if (rc == BMG_OK) rc = upload(&dring, log.ring);
+  if (rc == BMG_OK) rc = upload(&dchild, child0);
+  if (rc == BMG_OK) {
+    double* news[3] = {t->s, t->a, t->b};
+    for (int f = 0; f < 3; f++) {
+      k_t_remap<<<nb_(x.nslot), kB, 0, t->stream>>>(dprev, x.nslot, olds[f], news[f]);
+      if (nr) k_t_refine_remap<DIM><<<nb_((long)nr), kB, 0, t->stream>>>(dring, dchild, (int)nr, olds[f], news[f]);
+    }
+    Batcher<double, DIM> B(t, t->s, t->a, t->b);
+    B.tree_boundary(0, BK_SYM, -1);
+    B.tree_boundary(1, BK_A, -1);
+    B.tree_boundary(2, BK_SYM, -1);
+    B.flush();
+    hipError_t e = hipStreamSynchronize(t->stream);
+    if (e != hipSuccess) rc = bmg::fail(BMG_ERR_HIP, "tree adaptation: %s", hipGetErrorString(e));
+  }
+  dfree_(dprev);
+  dfree_(dring);
+  dfree_(dchild);
+  return rc;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ C ABI
@@ -1514,8 +1825,64 @@ int bmg_tree_refine(bmg_tree* t, const unsigned char* mark, long* nrefined) {
     t->keep_a.clear();
     t->keep_b.clear();
     t->keep_lam.clear();
+    t->keep_s.clear();
+    t->s_set = false;
   }
   if (nrefined) *nrefined = nr;
+  return BMG_OK;
+}
+
+int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, int* nf, int* nc) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  if (!(max >= 0.)) return bmg::fail(BMG_ERR_ARG, "max must be a non-negative number");
+  const int lim = std::min(kSeg - 1, kMaxDepth);
+  if (maxlevel < 0 || maxlevel > lim) return bmg::fail(BMG_ERR_ARG, "maxlevel %d outside 0..%d", maxlevel, lim);
+  TRY(ensure(t));
+  if (!t->s_set) return bmg::fail(BMG_ERR_ARG, "set the estimated scalar (BMG_F_S) first");
+  Topo& T = t->topo;
+  std::vector<double> err;
+  if (t->dim == 2) {
+    adapt_boundary<2>(t);
+    TRY(adapt_estimate<2>(t, err));
+  } else {
+    adapt_boundary<3>(t);
+    TRY(adapt_estimate<3>(t, err));
+  }
+  Topo::Adapt A;
+  A.err = &err;
+  A.max = max;
+  A.maxlevel = maxlevel;
+  A.minlevel = minlevel < 1 ? 1 : minlevel;
+  const IJK root = {GH, GH, t->dim == 3 ? GH : 0};
+  T.adapt_refine(0, root, A);
+  for (int l = T.depth; l >= 0; l--) T.adapt_coarsen(0, root, l, A);
+  if (!A.ok || !A.log.ok) {
+    t->compiled = false;
+    return bmg::fail(BMG_ERR_ARG, "tree adaptation met an inconsistent tree");
+  }
+  if (A.nf || A.nc) {
+    T.dirty = true;
+    // the three fields survive the recompilation in their old layout
+    double* olds[3] = {t->s, t->a, t->b};
+    const long old_nslot = t->ix.nslot;
+    t->s = t->a = t->b = nullptr;
+    t->keep_a.clear();
+    t->keep_b.clear();
+    t->keep_lam.clear();
+    t->keep_s.clear();
+    std::vector<int> prev;
+    t->prev_slots = &prev;
+    int rc = compile(t);
+    t->prev_slots = nullptr;
+    if (rc == BMG_OK)
+      rc = t->dim == 2 ? adapt_remap<2>(t, olds, old_nslot, prev, A.log)
+                       : adapt_remap<3>(t, olds, old_nslot, prev, A.log);
+    for (double* p : olds) dfree_(p);
+    if (rc != BMG_OK) return rc;
+    t->coef_ready = false;
+  }
+  if (nf) *nf = (int)A.nf;
+  if (nc) *nc = (int)A.nc;
   return BMG_OK;
 }
 
@@ -1560,6 +1927,10 @@ int bmg_tree_set_precision(bmg_tree* t, int precision) {
       t->keep_b.resize(t->ix.nleaf);
       TRY(get_leaf_values<double>(t, t->a, t->keep_a.data(), 0));
       TRY(get_leaf_values<double>(t, t->b, t->keep_b.data(), 0));
+      if (t->s_set) {
+        t->keep_s.resize(t->ix.nleaf);
+        TRY(get_leaf_values<double>(t, t->s, t->keep_s.data(), 0));
+      }
       if (t->lambda_field) {
         t->keep_lam.resize(t->ix.nleaf);
         TRY(get_leaf_values<double>(t, t->lam64, t->keep_lam.data(), 0));
@@ -1622,6 +1993,10 @@ int bmg_tree_set_leaf_field(bmg_tree* t, int field, const double* values, int on
   TRY(ensure(t));
   if (field == BMG_F_A) return set_leaf_values(t, t->a, values, on_device);
   if (field == BMG_F_B) return set_leaf_values(t, t->b, values, on_device);
+  if (field == BMG_F_S) {
+    t->s_set = true;
+    return set_leaf_values(t, t->s, values, on_device);
+  }
   if (field == BMG_F_LAMBDA) {
     t->lambda_field = true;
     t->coef_ready = false;
@@ -1638,6 +2013,7 @@ int bmg_tree_get_leaf_field(bmg_tree* t, int field, double* values, int on_devic
   switch (field) {
     case BMG_F_A: return get_leaf_values<double>(t, t->a, values, on_device);
     case BMG_F_B: return get_leaf_values<double>(t, t->b, values, on_device);
+    case BMG_F_S: return get_leaf_values<double>(t, t->s, values, on_device);
     case BMG_F_RES: return get_leaf_values<double>(t, t->res64, values, on_device);
     case BMG_F_DA:
       return mixed ? get_leaf_values<float>(t, F<float>(t->da), values, on_device)

@@ -123,21 +123,21 @@ __global__ __launch_bounds__(kB) void k_t_restrict(const int* __restrict__ self,
   t_restrict_one<T, DIM>(f, self[r], child0[r]);
 }
 
-// bilinear into the 2^dim children of each record from its 3^dim ring
+// bilinear (grid/multigrid-common.h:124-140) of the 2^dim children of a
+// record, in foreach_child order, from its 3^dim ring in src
 template <typename T, int DIM>
-__device__ __forceinline__ void t_prolong_one(const int* __restrict__ g, int child0, T* __restrict__ f) {
-  T* out = f + child0;
+__device__ __forceinline__ void t_bilinear_children(const int* __restrict__ g, const T* src, T* v) {
   if (DIM == 2) {
-    T v[9];
+    T r[9];
 #pragma unroll
-    for (int m = 0; m < 9; m++) v[m] = f[g[m]];
+    for (int m = 0; m < 9; m++) r[m] = src[g[m]];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int b = 0; b < 2; b++) {
         int cx = 2 * a - 1, cy = 2 * b - 1;
-        T c0 = v[4], c1 = v[(1 + cx) * 3 + 1], c2 = v[3 + 1 + cy], c3 = v[(1 + cx) * 3 + 1 + cy];
-        out[a * 2 + b] = (T(9) * c0 + T(3) * (c1 + c2) + c3) / T(16);
+        T c0 = r[4], c1 = r[(1 + cx) * 3 + 1], c2 = r[3 + 1 + cy], c3 = r[(1 + cx) * 3 + 1 + cy];
+        v[a * 2 + b] = (T(9) * c0 + T(3) * (c1 + c2) + c3) / T(16);
       }
   } else {
 #pragma unroll
@@ -147,14 +147,23 @@ __device__ __forceinline__ void t_prolong_one(const int* __restrict__ g, int chi
 #pragma unroll
         for (int e = 0; e < 2; e++) {
           int cx = 2 * a - 1, cy = 2 * b - 1, cz = 2 * e - 1;
-#define RG(x, y, z) f[g[((1 + (x)) * 3 + 1 + (y)) * 3 + 1 + (z)]]
+#define RG(x, y, z) src[g[((1 + (x)) * 3 + 1 + (y)) * 3 + 1 + (z)]]
           T c000 = RG(0, 0, 0), c100 = RG(cx, 0, 0), c010 = RG(0, cy, 0), c001 = RG(0, 0, cz);
           T c110 = RG(cx, cy, 0), c101 = RG(cx, 0, cz), c011 = RG(0, cy, cz), c111 = RG(cx, cy, cz);
 #undef RG
-          out[(a * 2 + b) * 2 + e] =
+          v[(a * 2 + b) * 2 + e] =
               (T(27) * c000 + T(9) * (c100 + c010 + c001) + T(3) * (c110 + c101 + c011) + c111) / T(64);
         }
   }
+}
+
+// bilinear into the 2^dim children of each record from its 3^dim ring
+template <typename T, int DIM>
+__device__ __forceinline__ void t_prolong_one(const int* __restrict__ g, int child0, T* f) {
+  T v[1 << DIM];
+  t_bilinear_children<T, DIM>(g, f, v);
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); m++) f[child0 + m] = v[m];
 }
 
 template <typename T, int DIM>
@@ -563,6 +572,50 @@ __global__ __launch_bounds__(kOpsBlock) void k_t_ops(TOps P, TreeArgs<T> A) {
     }
     __syncthreads();
   }
+}
+
+// --- adapt_wavelet -------------------------------------------------------
+
+// the wavelet estimate of adapt_wavelet (grid/tree-common.h:215-223): for
+// every child of a refined cell, |value - bilinear prolongation of the
+// parent's ring|
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_t_wavelet(const int* __restrict__ ring, const int* __restrict__ child0, int n,
+                                                  const double* __restrict__ f, double* __restrict__ err) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  constexpr int NR = DIM == 2 ? 9 : 27;
+  double v[1 << DIM];
+  t_bilinear_children<double, DIM>(ring + (size_t)r * NR, f, v);
+  const int c0 = child0[r];
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); m++) err[c0 + m] = fabs(f[c0 + m] - v[m]);
+}
+
+// a field carried across a mesh change: each slot of the new layout takes
+// the value of the same cell in the old one (prev < 0: a new cell, zero)
+__global__ __launch_bounds__(kB) void k_t_remap(const int* __restrict__ prev, long n, const double* __restrict__ old,
+                                                double* __restrict__ out) {
+  long s = (long)blockIdx.x * kB + threadIdx.x;
+  if (s >= n) return;
+  const int p = prev[s];
+  out[s] = p >= 0 ? old[p] : 0.0;
+}
+
+// the children of the cells refined by adapt_wavelet: refine_bilinear of the
+// parent's ring in the old layout (ring: old slots) into the new slots
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_t_refine_remap(const int* __restrict__ ring, const int* __restrict__ child0,
+                                                       int n, const double* __restrict__ old,
+                                                       double* __restrict__ out) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  constexpr int NR = DIM == 2 ? 9 : 27;
+  double v[1 << DIM];
+  t_bilinear_children<double, DIM>(ring + (size_t)r * NR, old, v);
+  const int c0 = child0[r];
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); m++) out[c0 + m] = v[m];
 }
 
 }  // namespace bmgt

@@ -79,7 +79,8 @@ enum {
   BMG_F_RES = 2,     /* residual (any level) */
   BMG_F_DA = 3,      /* correction (any level) */
   BMG_F_LAMBDA = 4,  /* Helmholtz coefficient field (any level) */
-  BMG_F_ALPHA = 5    /* face coefficient: BMG_F_ALPHA + c for component c (any level) */
+  BMG_F_ALPHA = 5,   /* face coefficient: BMG_F_ALPHA + c for component c (any level) */
+  BMG_F_S = 8        /* trees: the user scalar adapt_wavelet estimates (leaf field) */
 };
 
 typedef struct bmg_grid bmg_grid;
@@ -236,6 +237,18 @@ int bmg_tree_leaves(bmg_tree *t, int *level, double *xyz);
    whose mark (leaf order) is non-zero, with 2:1 balancing; repeat passes
    until no leaf is marked.  Fields set before are discarded. */
 int bmg_tree_refine(bmg_tree *t, const unsigned char *mark, long *nrefined);
+/* adapt_wavelet ({s}, {max}, maxlevel, minlevel) (grid/tree-common.h:165-303)
+   for the leaf scalar s (BMG_F_S, set first): the children of every refined
+   cell whose |value - bilinear prolongation| exceeds max are refined (below
+   maxlevel), cells whose children all stay within max/1.5 are coarsened (at
+   minlevel and above; minlevel < 1 counts as 1), keeping 2:1 balance.  The
+   estimate and the field updates run on the GPU, the passes over the tree
+   on the host.  s, a and b follow the mesh (refine_bilinear into new leaves,
+   the children's average into coarsened ones); alpha and a lambda field must
+   be set again, and boundary values of `a` for levels the tree grew into
+   read zero until set again.  nf and nc (may be NULL) receive the numbers of
+   refined and coarsened cells (astats). */
+int bmg_tree_adapt_wavelet(bmg_tree *t, double max, int maxlevel, int minlevel, int *nf, int *nc);
 /* the cache lists of a level (which: 0 active, 1 prolongation, 2 boundary)
    as (i,j,k) triples with the reference's GHOSTS = 2 offset; ijk may be
    NULL to get the count */
@@ -253,8 +266,8 @@ int bmg_tree_set_niter(bmg_tree *t, int nitermax, int nitermin);
    concatenated, n_l^(dim-1) values per level in C order of the tangential
    indices (may be NULL for 0) */
 int bmg_tree_set_bc(bmg_tree *t, int side, int kind, const double *values);
-/* leaf fields (BMG_F_A, BMG_F_B, BMG_F_LAMBDA to set; also BMG_F_RES and
-   BMG_F_DA to read), one value per leaf in leaf order */
+/* leaf fields (BMG_F_A, BMG_F_B, BMG_F_S, BMG_F_LAMBDA to set; also
+   BMG_F_RES and BMG_F_DA to read), one value per leaf in leaf order */
 int bmg_tree_set_leaf_field(bmg_tree *t, int field, const double *values, int on_device);
 int bmg_tree_get_leaf_field(bmg_tree *t, int field, double *values, int on_device);
 /* alpha on the faces of the face cache: one value per set flag bit, entry

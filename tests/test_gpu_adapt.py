@@ -1,0 +1,156 @@
+"""adapt_wavelet on the GPU tree against the tree oracle.
+
+The wavelet estimate, the field updates (refine_bilinear into new leaves, the
+children's average into coarsened cells) and the tree boundary run on the
+GPU; the refinement and coarsening passes over the topology on the host.
+After every call the adapted tree (depth, leaves, every cache) and the leaf
+values of s, a and b equal the oracle's bit for bit, and so do the refined /
+coarsened counts.  The oracle's adaptation is pinned by halo.ref
+(test_adapt_oracle.py); the GPU tree reproduces its x, y, level columns too.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import basilisk_amd as bmg
+from oracle.tree import GH, OracleTree
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(__file__)
+
+
+def _same_tree(g, o):
+    assert g.depth() == o.depth()
+    assert g.nleaves() == o.nleaves()
+    la, xa = g.leaves()
+    lb, xb = o.leaves()
+    assert np.array_equal(la, lb) and np.array_equal(xa, xb)
+    for which in ("active", "prolongation", "boundary"):
+        for l in range(o.depth() + 1):
+            assert np.array_equal(g.cache(which, l), o.cache(which, l)), (which, l)
+
+
+def _same_fields(g, o):
+    assert np.array_equal(g.get_s(), o.get_s())
+    assert np.array_equal(g.get_a(), o.get_a())
+    assert np.array_equal(g.get_b(), o.get_b())
+
+
+def test_halo_ref_gpu():
+    print("halo.ref on the GPU tree", flush=True)
+    g = bmg.Tree(2, 1.0, (-0.5, -0.5))
+    o = OracleTree(2, 1.0, (-0.5, -0.5))
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: lev < 5)
+        _, xyz = t.leaves()
+        t.set_s(np.exp(-(xyz[:, 0] ** 2 + xyz[:, 1] ** 2) / 0.01))
+    rounds = 0
+    while True:
+        sg = g.adapt_wavelet(1e-2, 5)
+        so = o.adapt_wavelet(1e-2, 5)
+        assert sg == so
+        _same_tree(g, o)
+        _same_fields(g, o)
+        rounds += 1
+        assert rounds < 20
+        if not sg[1]:
+            break
+    with open(os.path.join(HERE, "golden", "ref", "halo.ref")) as f:
+        ref = [ln.split()[:3] for ln in f.read().split("\n") if ln]
+    got = []
+    for l in range(g.depth()):
+        D = 1.0 / (1 << l)
+        for i, j, _ in g.cache("prolongation", l):
+            got.append(["%g" % ((i - GH + 0.5) * D - 0.5), "%g" % ((j - GH + 0.5) * D - 0.5), "%d" % l])
+    assert got == ref
+
+
+def _bumps(xyz, centres, w):
+    v = np.zeros(len(xyz))
+    for c in centres:
+        v += np.exp(-((xyz - c) ** 2).sum(1) / (w * w))
+    return v
+
+
+@pytest.mark.parametrize("dim,base,maxlevel,minlevel", [(2, 4, 9, 2), (2, 5, 7, 1), (3, 3, 6, 2)])
+def test_adapt_rounds_bitexact(dim, base, maxlevel, minlevel):
+    """A moving pair of bumps: every call refines ahead and coarsens behind;
+    s, a (with a Dirichlet side) and b follow the mesh."""
+    print("adapt rounds", dim, base, maxlevel, minlevel, flush=True)
+    rng = np.random.default_rng(11 * dim + base)
+    g = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0))
+    o = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: lev < base)
+    totf = totc = 0
+    for rnd in range(5):
+        c = rng.uniform(0.2, 0.8, (2, 3))[:, :dim]
+        for t in (g, o):
+            _, xyz = t.leaves()
+            p = xyz[:, :dim]
+            t.set_s(_bumps(p, c, 0.06))
+            if rnd == 0:
+                t.set_a(np.sin(3 * p[:, 0]) + p[:, 1] ** 2)
+                t.set_b(np.cos(2 * p.sum(1)))
+            t.set_bc_function(bmg.LEFT, bmg.BC_DIRICHLET, lambda x, y, z: 1.0 + y)
+        sg = g.adapt_wavelet(3e-3, maxlevel, minlevel)
+        so = o.adapt_wavelet(3e-3, maxlevel, minlevel)
+        print("  round", rnd, sg, g.nleaves(), g.depth(), flush=True)
+        assert sg == so
+        totf += sg[0]
+        totc += sg[1]
+        _same_tree(g, o)
+        _same_fields(g, o)
+    assert totf > 0 and totc > 0
+
+
+@pytest.mark.parametrize("dim,base,maxlevel", [(2, 5, 8), (3, 3, 6)])
+def test_poisson_after_adapt_bitexact(dim, base, maxlevel):
+    """The solver runs on the adapted tree: poisson() with Dirichlet BCs,
+    alpha from a function and a lambda constant equals the oracle's."""
+    print("poisson after adapt", dim, base, maxlevel, flush=True)
+    g = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0), smoother="rbgs")
+    o = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    o.set_smoother("rbgs")
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: lev < base)
+    for rnd in range(3):
+        for t in (g, o):
+            _, xyz = t.leaves()
+            t.set_s(_bumps(xyz[:, :dim], [np.full(dim, 0.35 + 0.1 * rnd)], 0.05))
+        assert g.adapt_wavelet(2e-3, maxlevel) == o.adapt_wavelet(2e-3, maxlevel)
+    _same_tree(g, o)
+    for t in (g, o):
+        _, xyz = t.leaves()
+        p = xyz[:, :dim]
+        t.set_b(np.cos(3 * p[:, 0]) * np.sin(2 * p[:, 1]))
+        t.set_a(np.zeros(len(p)))
+        for side in range(2 * dim):
+            t.set_bc_function(side, bmg.BC_DIRICHLET, lambda x, y, z: x - y)
+        t.set_alpha_function(lambda x, y, z, d: 1.0 + 0.5 * x + 0.25 * d)
+        t.set_lambda(-2.0)
+        t.set_niter(30)
+    sg = g.poisson(tolerance=1e-7)
+    so = o.poisson(tolerance=1e-7)
+    for k in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert sg[k] == so[k], k
+    assert np.array_equal(g.get_a(), o.get_a())
+
+
+def test_adapt_errors():
+    print("adapt errors", flush=True)
+    g = bmg.Tree(2, 1.0, (0.0, 0.0, 0.0))
+    g.refine(lambda x, y, z, lev: lev < 3)
+    with pytest.raises(bmg.BMGError):
+        g.adapt_wavelet(1e-2, 5)  # s not set
+    g.set_s(np.zeros(g.nleaves()))
+    with pytest.raises(bmg.BMGError):
+        g.adapt_wavelet(1e-2, 99)
+    with pytest.raises(bmg.BMGError):
+        g.adapt_wavelet(-1.0, 5)
+    # a constant field coarsens everything down to minlevel
+    nf, nc = g.adapt_wavelet(1e-2, 5, 2)
+    assert nf == 0 and nc > 0
+    assert g.depth() == 2 and g.nleaves() == 16
