@@ -85,6 +85,20 @@ def tree_bench(args):
                      "leaves": nl}
         print("# %s: %.2f ms, %d cycles, resa %.3g, %d leaves, setup %.1f s" % (prec, dt * 1e3, st["i"], st["resa"],
               nl, build), file=sys.stderr, flush=True)
+        if prec == args.precisions.split(",")[-1]:
+            # one adapt_wavelet (grid/tree-common.h:165-303) on the solution:
+            # estimate, host passes, recompilation and field remap
+            a = res[prec]["a"]
+            t.set_s(a)
+            tol = 1e-3 * float(np.abs(a).max())
+            t.synchronize()
+            t2 = time.perf_counter()
+            nf, ncs = t.adapt_wavelet(tol, d)
+            t.synchronize()
+            adapt = {"ms": round((time.perf_counter() - t2) * 1e3, 1), "max": tol, "maxlevel": d, "nf": nf,
+                     "nc": ncs, "leaves_after": t.nleaves()}
+            print("# adapt_wavelet: %.1f ms, %d refined, %d coarsened, %d leaves" % (adapt["ms"], nf, ncs,
+                  adapt["leaves_after"]), file=sys.stderr, flush=True)
         t.close()
     main_prec = "mixed" if "mixed" in res else args.precisions.split(",")[0]
     r = res[main_prec]
@@ -104,6 +118,7 @@ def tree_bench(args):
         "leaves_per_s": r["leaves"] * r["stats"]["i"] / r["seconds"],
         "resa": r["stats"]["resa"],
         "setup_s": round(r["build_s"], 2),
+        "adapt_wavelet": adapt,
         "roofline": None,
     }
     if "fp64" in res and "mixed" in res:
