@@ -1,0 +1,268 @@
+"""CPU oracle of viscosity() on a uniform multigrid grid.  TEST
+INFRASTRUCTURE ONLY: only tests/, the smoke test and the benchmark's CPU leg
+may import this module; the product library never calls it.
+
+A numpy restatement, in the reference's evaluation order (elementwise
+operations only, so every value is rounded exactly as the C expressions
+round it), of
+
+  viscosity()            viscosity.h:163-178 (r = u, restriction ({mu, rho}),
+                         mg_solve with minlevel 0 and TOLERANCE)
+  relax_viscosity        viscosity.h:27-96, the JACOBI variant (w from the
+                         old iterate, then u = (u + 2 w)/3), non-AXI,
+                         lambda = 1
+  residual_viscosity     viscosity.h:98-160, the "naive" (!TREE) form
+  mg_solve, mg_cycle     poisson.h:32-90, 139-235 (nrelax default 4, the
+                         nrelax adaptation, NITERMIN/NITERMAX)
+  restriction            restriction_average and face_average,
+                         grid/multigrid-common.h:26-63 (children summed in
+                         foreach_child order; the faces in the tangential
+                         order the dimension rotation gives each component)
+  bilinear               grid/multigrid-common.h:124-140
+  box ghosts             grid/multigrid.h:543-594: one ghost layer filled
+                         direction by direction (right, left, top, bottom,
+                         front, back) over the full tangential extent, so an
+                         edge or corner ghost is the product of the sides'
+                         factors times the diagonal interior cell; every
+                         component's ghost is sigma * the mirrored cell
+                         (antisymmetry -1 / symmetry +1: the reference's
+                         vector BCs at walls at rest)
+
+Fields are (n,)*dim arrays in C order (i slowest); face field d has n+1
+entries along d.  Periodic boundaries are not covered.
+"""
+import numpy as np
+
+
+def default_sigma(dim):
+    """The reference's default vector BCs: the normal component is
+    antisymmetric, the tangential ones symmetric
+    (grid/cartesian-common.h:536-563)."""
+    s = np.ones((dim, 2 * dim), np.int64)
+    for c in range(dim):
+        s[c, 2 * c] = s[c, 2 * c + 1] = -1
+    return s
+
+
+def _pad(f, sig, dim):
+    """One ghost layer, filled direction by direction."""
+    n = f.shape[0]
+    p = np.zeros((n + 2,) * dim)
+    p[(slice(1, n + 1),) * dim] = f
+    for d in range(dim):
+        hi = [slice(None)] * dim
+        lo = [slice(None)] * dim
+        hi_src = [slice(None)] * dim
+        lo_src = [slice(None)] * dim
+        hi[d], hi_src[d] = n + 1, n
+        lo[d], lo_src[d] = 0, 1
+        p[tuple(hi)] = float(sig[2 * d]) * p[tuple(hi_src)]
+        p[tuple(lo)] = float(sig[2 * d + 1]) * p[tuple(lo_src)]
+    return p
+
+
+def _sh(p, off, n, dim):
+    """View of the padded array p shifted by `off` (one entry per dim)."""
+    return p[tuple(slice(1 + off[d], 1 + off[d] + n) for d in range(dim))]
+
+
+def _face(mu, d, hi, n, dim):
+    sl = [slice(None)] * dim
+    sl[d] = slice(1, n + 1) if hi else slice(0, n)
+    return mu[tuple(sl)]
+
+
+def _unit(dim, *pairs):
+    o = [0] * dim
+    for d, v in pairs:
+        o[d] += v
+    return o
+
+
+def _others(c, dim):
+    return [(c + 1) % dim, (c + 2) % dim][:dim - 1]
+
+
+class Viscosity:
+    """The viscous problem on a 2^depth grid of the box [0, L0]^dim."""
+
+    def __init__(self, dim, depth, L0=1.0):
+        self.dim, self.depth, self.L0 = dim, depth, L0
+        self.sigma = default_sigma(dim)
+        self.TOLERANCE = 1e-3
+        self.NITERMAX, self.NITERMIN = 100, 1
+
+    def delta(self, l):
+        return self.L0 / (1 << l)
+
+    # -- coefficients on every level (restriction ({mu, rho}))
+    def set_coefficients(self, mu, rho):
+        """mu: list of dim face arrays or a constant; rho: cell array or a
+        constant.  Restricted to every level."""
+        dim, D = self.dim, self.depth
+        self.mu_c = None if isinstance(mu, (list, tuple)) else float(mu)
+        self.rho_c = None if isinstance(rho, np.ndarray) else float(rho)
+        self.mu = [None] * (D + 1)
+        self.rho = [None] * (D + 1)
+        if self.mu_c is None:
+            self.mu[D] = [np.asarray(m, np.float64) for m in mu]
+            for l in range(D - 1, -1, -1):
+                self.mu[l] = [self._restrict_face(self.mu[l + 1][d], d) for d in range(dim)]
+        if self.rho_c is None:
+            self.rho[D] = np.asarray(rho, np.float64)
+            for l in range(D - 1, -1, -1):
+                self.rho[l] = self._restrict(self.rho[l + 1])
+
+    def _restrict(self, f):
+        """restriction_average: children in foreach_child order."""
+        dim = self.dim
+        s = np.zeros(tuple(m // 2 for m in f.shape))
+        for ch in np.ndindex(*(2,) * dim):
+            s = s + f[tuple(slice(o, None, 2) for o in ch)]
+        return s / float(1 << dim)
+
+    def _restrict_face(self, f, d):
+        """face_average for component d (n+1 along d)."""
+        dim = self.dim
+        if dim == 2:
+            t = 1 - d
+            order = [(0,), (1,)]
+            tdims = [t]
+        else:
+            tdims = {0: [1, 2], 1: [2, 0], 2: [0, 1]}[d]
+            order = [(0, 0), (1, 0), (0, 1), (1, 1)]
+        s = None
+        for o in order:
+            sl = [slice(None)] * dim
+            sl[d] = slice(0, None, 2)
+            for td, v in zip(tdims, o):
+                sl[td] = slice(v, None, 2)
+            term = f[tuple(sl)]
+            s = term if s is None else s + term
+        return s / float(len(order))
+
+    def _mu(self, l, d, hi, n):
+        if self.mu_c is not None:
+            return self.mu_c
+        return _face(self.mu[l][d], d, hi, n, self.dim)
+
+    def _dtr(self, l, dt):
+        return dt / (self.rho_c if self.rho_c is not None else self.rho[l])
+
+    # -- the stencils
+    def _terms(self, P, c, l, n, residual):
+        """The bracket of relax_viscosity (numerator terms, residual=False)
+        or of residual_viscosity (residual=True) for component c; P are the
+        padded components."""
+        dim = self.dim
+        U = lambda comp, *pairs: _sh(P[comp], _unit(dim, *pairs), n, dim)
+        u0 = U(c)
+        mch, mcl = self._mu(l, c, True, n), self._mu(l, c, False, n)
+        if residual:
+            S = 2. * mch * (U(c, (c, 1)) - u0) - 2. * mcl * (u0 - U(c, (c, -1)))
+        else:
+            S = 2. * mch * U(c, (c, 1)) + 2. * mcl * U(c, (c, -1))
+        for e in _others(c, dim):
+            meh, mel = self._mu(l, e, True, n), self._mu(l, e, False, n)
+            up = (U(e, (c, 1)) + U(e, (c, 1), (e, 1))) / 4.
+            um = (U(e, (c, -1)) + U(e, (c, -1), (e, 1))) / 4.
+            lp = (U(e, (c, 1), (e, -1)) + U(e, (c, 1))) / 4.
+            lm = (U(e, (c, -1), (e, -1)) + U(e, (c, -1))) / 4.
+            if residual:
+                S = S + meh * (U(c, (e, 1)) - u0 + up - um)
+                S = S - mel * (u0 - U(c, (e, -1)) + lp - lm)
+            else:
+                S = S + meh * (U(c, (e, 1)) + up - um)
+                S = S - mel * (-U(c, (e, -1)) + lp - lm)
+        return S
+
+    def _diag(self, c, l, n):
+        dim = self.dim
+        s = 2. * self._mu(l, c, True, n) + 2. * self._mu(l, c, False, n)
+        for e in _others(c, dim):
+            s = s + self._mu(l, e, True, n) + self._mu(l, e, False, n)
+        return s
+
+    def relax(self, da, res, l, dt):
+        """relax_viscosity, JACOBI: returns the new components."""
+        dim = self.dim
+        n = 1 << l
+        D = self.delta(l)
+        D2 = D * D
+        P = [_pad(da[c], self.sigma[c], dim) for c in range(dim)]
+        dtr = self._dtr(l, dt)
+        out = []
+        for c in range(dim):
+            w = (dtr * self._terms(P, c, l, n, False) + res[c] * D2) / (D2 * 1. + dtr * self._diag(c, l, n))
+            out.append(w)
+        return [(da[c] + 2. * out[c]) / 3. for c in range(dim)]
+
+    def residual(self, u, r, dt):
+        """residual_viscosity on the finest level: (res list, max |res|)."""
+        dim, l = self.dim, self.depth
+        n = 1 << l
+        D = self.delta(l)
+        P = [_pad(u[c], self.sigma[c], dim) for c in range(dim)]
+        dtr = self._dtr(l, dt)
+        res = []
+        m = 0.
+        for c in range(dim):
+            rc = (r[c] - 1. * u[c]) + dtr * self._terms(P, c, l, n, True) / (D * D)
+            res.append(rc)
+            m = max(m, float(np.abs(rc).max()))
+        return res, m
+
+    def _bilinear(self, coarse, sig):
+        dim = self.dim
+        nc = coarse.shape[0]
+        P = _pad(coarse, sig, dim)
+        n = 2 * nc
+        idx = np.indices((n,) * dim)
+        p = [(idx[d] >> 1) + 1 for d in range(dim)]
+        o = [2 * (idx[d] & 1) - 1 for d in range(dim)]
+        g = lambda *sh: P[tuple(p[d] + (o[d] if sh[d] else 0) for d in range(dim))]
+        if dim == 2:
+            return (9. * g(0, 0) + 3. * (g(1, 0) + g(0, 1)) + g(1, 1)) / 16.
+        return (27. * g(0, 0, 0) + 9. * (g(1, 0, 0) + g(0, 1, 0) + g(0, 0, 1))
+                + 3. * (g(1, 1, 0) + g(1, 0, 1) + g(0, 1, 1)) + g(1, 1, 1)) / 64.
+
+    def mg_cycle(self, u, res, dt, nrelax, minlevel=0):
+        dim, D = self.dim, self.depth
+        # restriction (res) to every level
+        rl = [None] * (D + 1)
+        rl[D] = res
+        for l in range(D - 1, -1, -1):
+            rl[l] = [self._restrict(rl[l + 1][c]) for c in range(dim)]
+        da = None
+        for l in range(min(minlevel, D), D + 1):
+            n = 1 << l
+            if l == min(minlevel, D):
+                da = [np.zeros((n,) * dim) for _ in range(dim)]
+            else:
+                da = [self._bilinear(da[c], self.sigma[c]) for c in range(dim)]
+            for _ in range(nrelax):
+                da = self.relax(da, rl[l], l, dt)
+        return [u[c] + da[c] for c in range(dim)]
+
+    def solve(self, u, dt, nrelax=0, tolerance=0.0):
+        """viscosity(): returns (u, stats, residuals after each cycle)."""
+        u = [np.array(x, np.float64) for x in u]
+        r = [x.copy() for x in u]
+        s = {"i": 0, "sum": float(sum(x.sum() for x in r)), "nrelax": nrelax if nrelax > 0 else 4,
+             "minlevel": 0}
+        res, resb = self.residual(u, r, dt)
+        s["resb"] = s["resa"] = resb
+        tol = tolerance if tolerance else self.TOLERANCE
+        hist = []
+        while s["i"] < self.NITERMAX and (s["i"] < self.NITERMIN or s["resa"] > tol):
+            u = self.mg_cycle(u, res, dt, s["nrelax"])
+            res, s["resa"] = self.residual(u, r, dt)
+            hist.append(s["resa"])
+            if s["resa"] > tol:
+                if resb / s["resa"] < 1.2 and s["nrelax"] < 100:
+                    s["nrelax"] += 1
+                elif resb / s["resa"] > 10 and s["nrelax"] > 2:
+                    s["nrelax"] -= 1
+            resb = s["resa"]
+            s["i"] += 1
+        return u, s, hist
