@@ -49,7 +49,7 @@ EXPORTS = [
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
     "bmg_tree_adapt_wavelet",
-    "bmg_coef_gradient", "bmg_project",
+    "bmg_coef_gradient", "bmg_project", "bmg_viscosity",
 ]
 
 
@@ -148,6 +148,7 @@ def library():
         "bmg_tree_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
         "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_viscosity": (I, [P, dp, dp, dp, dp, dp, dp, D, dp, D, D, dp, I, I, ctypes.POINTER(MGStats)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -434,6 +435,70 @@ class Grid:
                 raise ValueError("face components must be C-contiguous float64 numpy arrays (or CUDA tensors)")
         ptrs = [_host_ptr(u) for u in uf] + [None] * (3 - self.dim)
         _check(self.lib.bmg_project(self._g, ptrs[0], ptrs[1], ptrs[2], float(dt), int(nrelax), 0, ctypes.byref(s)))
+        return s.as_dict()
+
+    def viscosity(self, u, mu=1.0, rho=1.0, dt=1.0, sigma=None, nrelax=0):
+        """viscosity(), viscosity.h:163-178: the implicit viscous step
+        u - dt/rho div(2 mu D(u)) = u_old for the cell components `u` (one
+        (n,)*dim array per component, updated in place), with relax_viscosity
+        in its JACOBI variant; mu: one face array per component (n+1 along
+        it) or a constant; rho: a cell array or a constant; sigma: (dim, 2
+        dim) wall factors (default: -1 for the normal component, +1 for the
+        tangential ones).  torch CUDA tensors stay on the device, numpy
+        float64 arrays are updated through host copies.  Returns the
+        mg_solve statistics."""
+        if len(u) != self.dim:
+            raise ValueError("need %d velocity components" % self.dim)
+        shape = self.shape()
+        for c, x in enumerate(u):
+            if tuple(x.shape) != shape:
+                raise ValueError("component %d: shape %s, expected %s" % (c, tuple(x.shape), shape))
+        dev = all(getattr(x, "is_cuda", False) for x in u)
+        keep = []
+
+        def ptr(x, what):
+            if x is None:
+                return None
+            if dev:
+                import torch
+                if not getattr(x, "is_cuda", False) or x.dtype != torch.float64 or not x.is_contiguous():
+                    raise ValueError("%s: contiguous float64 CUDA tensor expected" % what)
+                return ctypes.c_void_p(x.data_ptr())
+            if not isinstance(x, np.ndarray) or x.dtype != np.float64 or not x.flags.c_contiguous:
+                raise ValueError("%s: C-contiguous float64 numpy array expected" % what)
+            keep.append(x)
+            return _host_ptr(x)
+
+        up = [ptr(x, "u") for x in u] + [None] * (3 - self.dim)
+        if np.isscalar(mu):
+            mp, mu_c = [None] * 3, float(mu)
+        else:
+            if len(mu) != self.dim:
+                raise ValueError("need %d mu components" % self.dim)
+            for c, m in enumerate(mu):
+                if tuple(m.shape) != self.face_shape(c):
+                    raise ValueError("mu component %d: shape %s, expected %s" % (c, tuple(m.shape),
+                                                                                self.face_shape(c)))
+            mp, mu_c = [ptr(m, "mu") for m in mu] + [None] * (3 - self.dim), 0.0
+        if np.isscalar(rho):
+            rp, rho_c = None, float(rho)
+        else:
+            if tuple(rho.shape) != shape:
+                raise ValueError("rho: shape %s, expected %s" % (tuple(rho.shape), shape))
+            rp, rho_c = ptr(rho, "rho"), 0.0
+        sp = None
+        if sigma is not None:
+            sg = np.ones((3, 6), np.int32)
+            sa = np.asarray(sigma, np.int32)
+            sg[:sa.shape[0], :sa.shape[1]] = sa
+            keep.append(sg)
+            sp = _host_ptr(sg)
+        if dev:
+            import torch
+            torch.cuda.current_stream().synchronize()
+        s = MGStats()
+        _check(self.lib.bmg_viscosity(self._g, up[0], up[1], up[2], mp[0], mp[1], mp[2], mu_c, rp, rho_c,
+                                      float(dt), sp, int(nrelax), 1 if dev else 0, ctypes.byref(s)))
         return s.as_dict()
 
     # -- kernel timing

@@ -17,6 +17,7 @@
 #include "../../include/bmg.h"
 #include "bmg_internal.h"
 #include "bmg_kernels.h"
+#include "bmg_visc_kernels.h"
 
 using namespace bmg;
 
@@ -104,6 +105,11 @@ struct bmg_grid {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
   double prof_ms[PROF_N] = {0};
   long prof_n[PROF_N] = {0};
+  // viscosity() workspace, allocated on first use: per level kViscSlots
+  // arrays (da, the Jacobi output and res per component, mu low and wall
+  // faces per component, rho), and r = u_old on the finest level
+  std::vector<double*> visc;
+  double* visc_r[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -1149,6 +1155,8 @@ int bmg_grid_free(bmg_grid* g) {
   dfreet(g->partial);
   dfreet(g->part);
   dfreet(g->scalar);
+  for (double*& p : g->visc) dfreet(p);
+  for (int c = 0; c < 3; c++) dfreet(g->visc_r[c]);
   prof_collect(g);
   for (auto e : g->ev_pool) (void)hipEventDestroy(e);
   if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
@@ -1642,3 +1650,278 @@ int project_t(bmg_grid* g, double* const* uf, double dt, int nrelax, bmg_mgstats
 }
 
 }  // namespace
+
+// ------------------------------------------------------------ viscosity()
+
+namespace {
+
+constexpr int kViscSlots = 16;
+enum { VS_DA = 0, VS_DN = 3, VS_RES = 6, VS_MLO = 9, VS_MWALL = 12, VS_RHO = 15 };
+
+double*& vbuf(bmg_grid* g, int l, int slot) { return g->visc[(size_t)l * kViscSlots + slot]; }
+
+int visc_alloc(bmg_grid* g) {
+  if (!g->visc.empty()) return BMG_OK;
+  g->visc.assign((size_t)(g->depth + 1) * kViscSlots, nullptr);
+  for (int l = 0; l <= g->depth; l++) {
+    const Level& lv = g->L[l];
+    for (int c = 0; c < g->dim; c++) {
+      TRY(dalloc((void**)&vbuf(g, l, VS_DA + c), lv.ncell * sizeof(double)));
+      TRY(dalloc((void**)&vbuf(g, l, VS_DN + c), lv.ncell * sizeof(double)));
+      TRY(dalloc((void**)&vbuf(g, l, VS_RES + c), lv.ncell * sizeof(double)));
+      TRY(dalloc((void**)&vbuf(g, l, VS_MLO + c), lv.ncell * sizeof(double)));
+      TRY(dalloc((void**)&vbuf(g, l, VS_MWALL + c), std::max<size_t>(lv.nwall, 1) * sizeof(double)));
+    }
+    TRY(dalloc((void**)&vbuf(g, l, VS_RHO), lv.ncell * sizeof(double)));
+  }
+  for (int c = 0; c < g->dim; c++) TRY(dalloc((void**)&g->visc_r[c], g->L[g->depth].ncell * sizeof(double)));
+  return BMG_OK;
+}
+
+// the viscous problem of one call: u (device, finest level), constant or
+// field mu and rho, the components' wall sigmas
+template <int DIM>
+struct Visc {
+  bmg_grid* g;
+  double* u[3];
+  bool mu_field, rho_field;
+  double mu_c, rho_c, dt;
+  int sig[3][6];
+
+  ViscArgs args(int l, double* const* in, double* const* out, double* const* r) const {
+    ViscArgs A;
+    memset(&A, 0, sizeof A);
+    for (int c = 0; c < DIM; c++) {
+      A.u[c] = in[c];
+      A.out[c] = out[c];
+      A.r[c] = r[c];
+      if (mu_field) {
+        A.mlo[c] = vbuf(g, l, VS_MLO + c);
+        A.mwall[c] = vbuf(g, l, VS_MWALL + c);
+      }
+      for (int b = 0; b < 6; b++) A.sig[c][b] = sig[c][b];
+    }
+    A.mu_c = mu_c;
+    A.rho = rho_field ? vbuf(g, l, VS_RHO) : nullptr;
+    A.rho_c = rho_c;
+    A.dt = dt;
+    A.D = delta(g, l);
+    return A;
+  }
+  double* const* lv(int l, int slot) const { return &g->visc[(size_t)l * kViscSlots + slot]; }
+
+  // restriction ({mu, rho}) (viscosity.h:173-175)
+  void restrict_coefficients() {
+    for (int l = g->depth - 1; l >= 0; l--) {
+      const Level& lc = g->L[l];
+      Geo<DIM> gc{lc.n, lc.log2n};
+      if (rho_field)
+        hipLaunchKernelGGL((k_restrict<DIM, double>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
+                           vbuf(g, l + 1, VS_RHO), vbuf(g, l, VS_RHO), gc);
+      if (mu_field)
+        for (int d = 0; d < DIM; d++) {
+          hipLaunchKernelGGL((k_restrict_face_lo<DIM>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
+                             vbuf(g, l + 1, VS_MLO + d), vbuf(g, l, VS_MLO + d), gc, d);
+          hipLaunchKernelGGL((k_restrict_face_wall<DIM>), dim3(nblocks(std::max<size_t>(lc.nwall, 1))),
+                             dim3(kBlock), 0, g->stream, vbuf(g, l + 1, VS_MWALL + d), vbuf(g, l, VS_MWALL + d),
+                             lc.n, d);
+        }
+    }
+  }
+
+  int residual(double* maxres, unsigned long long* nans) {
+    const Level& lf = g->L[g->depth];
+    Geo<DIM> geo{lf.n, lf.log2n};
+    hipLaunchKernelGGL(k_stat_reset, dim3(1), dim3(64), 0, g->stream, g->stat);
+    hipLaunchKernelGGL((k_visc_residual<DIM>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream,
+                       args(g->depth, u, lv(g->depth, VS_RES), g->visc_r), geo, g->stat);
+    TRY(check_launch());
+    Stat st;
+    HIPCHECK(hipMemcpyAsync(&st, g->stat, sizeof st, hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    memcpy(maxres, &st.maxbits, sizeof(double));
+    *nans += st.nans;
+    return BMG_OK;
+  }
+
+  // mg_cycle (poisson.h:32-90) with minlevel 0 and relax_viscosity (JACOBI)
+  int cycle(int nrelax) {
+    const int D = g->depth;
+    for (int l = D - 1; l >= 0; l--) {
+      Geo<DIM> gc{g->L[l].n, g->L[l].log2n};
+      for (int c = 0; c < DIM; c++)
+        hipLaunchKernelGGL((k_restrict<DIM, double>), dim3(nblocks(g->L[l].ncell)), dim3(kBlock), 0, g->stream,
+                           vbuf(g, l + 1, VS_RES + c), vbuf(g, l, VS_RES + c), gc);
+    }
+    for (int l = 0; l <= D; l++) {
+      const Level& lv_ = g->L[l];
+      Geo<DIM> gf{lv_.n, lv_.log2n};
+      for (int c = 0; c < DIM; c++) {
+        if (l == 0) {
+          hipLaunchKernelGGL((k_fill<double>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream,
+                             vbuf(g, l, VS_DA + c), lv_.ncell, 0.);
+        } else {
+          BCH bcc;
+          memset(&bcc, 0, sizeof bcc);
+          for (int b = 0; b < 6; b++) bcc.sigma[b] = sig[c][b];
+          hipLaunchKernelGGL((k_prolong<DIM, double>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream,
+                             vbuf(g, l - 1, VS_DA + c), vbuf(g, l, VS_DA + c), gf, bcc);
+        }
+      }
+      for (int i = 0; i < nrelax; i++) {
+        hipLaunchKernelGGL((k_visc_relax<DIM>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream,
+                           args(l, lv(l, VS_DA), lv(l, VS_DN), lv(l, VS_RES)), gf);
+        for (int c = 0; c < DIM; c++) std::swap(vbuf(g, l, VS_DA + c), vbuf(g, l, VS_DN + c));
+      }
+    }
+    const size_t n = g->L[D].ncell;
+    for (int c = 0; c < DIM; c++)
+      hipLaunchKernelGGL((k_correct<double>), dim3(nblocks(n)), dim3(kBlock), 0, g->stream, u[c],
+                         (const double*)vbuf(g, D, VS_DA + c), n);
+    return check_launch();
+  }
+
+  // mg_solve (poisson.h:139-235) around residual_viscosity / relax_viscosity
+  int solve(int nrelax, bmg_mgstats* out) {
+    const Level& lf = g->L[g->depth];
+    // r = u (viscosity.h:166-169) and the sum of b
+    bmg_mgstats s;
+    memset(&s, 0, sizeof s);
+    for (int c = 0; c < DIM; c++) {
+      HIPCHECK(hipMemcpyAsync(g->visc_r[c], u[c], lf.ncell * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+      int np = (int)std::min<size_t>((size_t)g->npartial, nblocks(lf.ncell));
+      hipLaunchKernelGGL(k_sum_partial, dim3(np), dim3(kBlock), 0, g->stream, g->visc_r[c], lf.ncell, g->partial);
+      hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, g->stream, g->partial, np, g->scalar);
+      TRY(check_launch());
+      double sc = 0.;
+      HIPCHECK(hipMemcpyAsync(&sc, g->scalar, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      s.sum += sc;
+    }
+    restrict_coefficients();
+    TRY(check_launch());
+    s.nrelax = nrelax > 0 ? nrelax : 4;
+    unsigned long long nans = 0;
+    double resb;
+    TRY(residual(&s.resb, &nans));
+    resb = s.resa = s.resb;
+    const double tolerance = g->TOLERANCE;
+    for (s.i = 0; s.i < g->NITERMAX && (s.i < g->NITERMIN || s.resa > tolerance); s.i++) {
+      TRY(cycle(s.nrelax));
+      TRY(residual(&s.resa, &nans));
+      if (s.resa > tolerance) {
+        if (resb / s.resa < 1.2 && s.nrelax < 100)
+          s.nrelax++;
+        else if (resb / s.resa > 10 && s.nrelax > 2)
+          s.nrelax--;
+      }
+      resb = s.resa;
+    }
+    s.minlevel = 0;
+    if (out) *out = s;
+    return nans ? fail(BMG_ERR_NAN, "viscous residual is NaN") : BMG_OK;
+  }
+};
+
+// copy of a face field in the user layout (n+1 along d) into low faces and
+// wall faces (as set_face does for alpha)
+int visc_set_face(bmg_grid* g, int d, const double* src, int on_device) {
+  const int l = g->depth;
+  const Level& lv = g->L[l];
+  const int n = lv.n;
+  size_t outer = 1, inner = 1;
+  for (int e = 0; e < g->dim; e++) {
+    if (e < d) outer *= n;
+    if (e > d) inner *= n;
+  }
+  hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIPCHECK(hipMemcpy2DAsync(vbuf(g, l, VS_MLO + d), (size_t)n * inner * sizeof(double), src,
+                            (size_t)(n + 1) * inner * sizeof(double), (size_t)n * inner * sizeof(double), outer, kind,
+                            g->stream));
+  HIPCHECK(hipMemcpy2DAsync(vbuf(g, l, VS_MWALL + d), inner * sizeof(double), src + (size_t)n * inner,
+                            (size_t)(n + 1) * inner * sizeof(double), inner * sizeof(double), outer, kind, g->stream));
+  return BMG_OK;
+}
+
+template <int DIM>
+int viscosity_t(bmg_grid* g, double* const* u, const double* const* mu, double mu_c, const double* rho, double rho_c,
+                double dt, const int (*sig)[6], int nrelax, int on_device, bmg_mgstats* stats) {
+  TRY(visc_alloc(g));
+  Visc<DIM> V;
+  V.g = g;
+  for (int c = 0; c < 3; c++) V.u[c] = c < DIM ? u[c] : nullptr;
+  V.mu_field = mu[0] != nullptr;
+  V.rho_field = rho != nullptr;
+  V.mu_c = mu_c;
+  V.rho_c = rho_c;
+  V.dt = dt;
+  for (int c = 0; c < 3; c++)
+    for (int b = 0; b < 6; b++) V.sig[c][b] = sig[c][b];
+  const Level& lf = g->L[g->depth];
+  if (V.mu_field)
+    for (int d = 0; d < DIM; d++) TRY(visc_set_face(g, d, mu[d], on_device));
+  if (V.rho_field)
+    HIPCHECK(hipMemcpyAsync(vbuf(g, g->depth, VS_RHO), rho, lf.ncell * sizeof(double),
+                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, g->stream));
+  const int rc = V.solve(nrelax, stats);
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const double* mu_x, const double* mu_y,
+                  const double* mu_z, double mu_c, const double* rho, double rho_c, double dt, const int* sigma,
+                  int nrelax, int on_device, bmg_mgstats* stats) {
+  TRY(check_grid(g));
+  if (g->dim < 2) return fail(BMG_ERR_UNSUPPORTED, "viscosity() is built for 2D and 3D grids");
+  for (int d = 0; d < g->dim; d++)
+    if (g->periodic[d]) return fail(BMG_ERR_UNSUPPORTED, "viscosity() on periodic grids is not built");
+  double* u[3] = {u_x, u_y, u_z};
+  const double* mu[3] = {mu_x, mu_y, mu_z};
+  int nmu = 0;
+  for (int d = 0; d < g->dim; d++) {
+    if (!u[d]) return fail(BMG_ERR_ARG, "null velocity component %d", d);
+    nmu += mu[d] != nullptr;
+  }
+  if (nmu != 0 && nmu != g->dim) return fail(BMG_ERR_ARG, "mu: every component or none");
+  if (!(dt >= 0.) || nrelax < 0) return fail(BMG_ERR_ARG, "dt must be non-negative and nrelax non-negative");
+  if (!rho && !(rho_c > 0.)) return fail(BMG_ERR_ARG, "rho must be positive");
+  if (!nmu && !(mu_c >= 0.)) return fail(BMG_ERR_ARG, "mu must be non-negative");
+  int sig[3][6];
+  for (int c = 0; c < 3; c++)
+    for (int b = 0; b < 6; b++) {
+      // walls at rest by default: the normal component antisymmetric
+      int v = sigma ? sigma[c * 6 + b] : (b / 2 == c ? -1 : 1);
+      if (c < g->dim && b < 2 * g->dim && v != 1 && v != -1) return fail(BMG_ERR_ARG, "sigma must be +1 or -1");
+      sig[c][b] = v == -1 ? -1 : 1;
+    }
+  const Level& lf = g->L[g->depth];
+  double* dev[3] = {nullptr, nullptr, nullptr};
+  int rc = BMG_OK;
+  if (!on_device) {
+    for (int d = 0; d < g->dim && rc == BMG_OK; d++) {
+      rc = dalloc((void**)&dev[d], lf.ncell * sizeof(double));
+      if (rc == BMG_OK && hipMemcpy(dev[d], u[d], lf.ncell * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(BMG_ERR_HIP, "copy of the velocity failed");
+    }
+  } else {
+    for (int d = 0; d < g->dim; d++) dev[d] = u[d];
+  }
+  if (rc == BMG_OK)
+    rc = g->dim == 2 ? viscosity_t<2>(g, dev, mu, mu_c, rho, rho_c, dt, sig, nrelax, on_device, stats)
+                     : viscosity_t<3>(g, dev, mu, mu_c, rho, rho_c, dt, sig, nrelax, on_device, stats);
+  if (!on_device) {
+    for (int d = 0; d < g->dim; d++) {
+      if (dev[d] && (rc == BMG_OK || rc == BMG_ERR_NAN) &&
+          hipMemcpy(u[d], dev[d], lf.ncell * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(BMG_ERR_HIP, "copy of the velocity failed");
+      dfreet(dev[d]);
+    }
+  }
+  return rc;
+}
+
+}  // extern "C"

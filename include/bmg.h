@@ -154,6 +154,25 @@ int bmg_synchronize(bmg_grid *g);
 int bmg_project(bmg_grid *g, double *uf_x, double *uf_y, double *uf_z, double dt, int nrelax, int on_device,
                 bmg_mgstats *stats);
 
+/* viscosity(), viscosity.h:163-178 (struct Viscosity, viscosity.h:3-10):
+   the implicit viscous step u - dt/rho div(2 mu D(u)) = u_old, solved by
+   mg_solve's loop (poisson.h:139-235: minlevel 0, tolerance TOLERANCE,
+   NITERMAX/NITERMIN, nrelax = 0 meaning 4) around residual_viscosity (the
+   multigrid grid's "naive" form, viscosity.h:134-157) and relax_viscosity
+   (viscosity.h:27-96) in its JACOBI variant, non-AXI, after restriction
+   ({mu, rho}).  u_x/u_y/u_z: the cell components (n^dim, C order), updated
+   in place; mu_x/y/z: face fields in the layout of bmg_set_field (n+1 along
+   their component), or all NULL for the constant mu_c; rho: a cell field,
+   or NULL for the constant rho_c.  Walls at rest: each component's ghost is
+   sigma times the mirrored cell, sigma[6*c + side] = +1 or -1 (NULL: the
+   reference's default vector BCs, -1 for the normal component and +1 for the
+   tangential ones; -1 everywhere is no slip).  2D and 3D; periodic grids are
+   refused.  Device pointers when on_device, else host arrays.  FP64,
+   whatever the grid's precision. */
+int bmg_viscosity(bmg_grid *g, double *u_x, double *u_y, double *u_z, const double *mu_x, const double *mu_y,
+                  const double *mu_z, double mu_c, const double *rho, double rho_c, double dt, const int *sigma,
+                  int nrelax, int on_device, bmg_mgstats *stats);
+
 /* Gradient of an objective J(a) with respect to the coefficients (no
    counterpart in the reference: the adjoint behind the torch.autograd
    binding).  L(a) = lambda a + div(alpha grad a) is self-adjoint, so the

@@ -1,0 +1,88 @@
+"""viscosity() on the GPU (bmg_viscosity, viscosity.h:163-178) against the
+oracle's restatement (oracle/viscosity.py): with the Jacobi relax_viscosity
+the solver statistics and every velocity component equal the oracle's bit
+for bit, for variable and constant mu and rho, the default wall BCs and no
+slip, in 2D and 3D.  The oracle is pinned by test_viscosity_oracle.py."""
+import numpy as np
+import pytest
+
+import basilisk_amd as bmg
+from oracle.viscosity import Viscosity
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(dim, depth, seed, variable):
+    rng = np.random.default_rng(seed)
+    n = 1 << depth
+    u = [np.ascontiguousarray(rng.standard_normal((n,) * dim)) for _ in range(dim)]
+    if variable:
+        mu = [np.ascontiguousarray(rng.uniform(0.2, 3.0, tuple(n + (1 if e == d else 0) for e in range(dim))))
+              for d in range(dim)]
+        rho = np.ascontiguousarray(np.where(rng.uniform(size=(n,) * dim) < 0.3, 10.0, 1.0))
+    else:
+        mu, rho = 0.8, 1.5
+    return u, mu, rho
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 6), (2, 7), (3, 4), (3, 5)])
+@pytest.mark.parametrize("variable", [True, False])
+@pytest.mark.parametrize("noslip", [False, True])
+def test_viscosity_bitexact(dim, depth, variable, noslip):
+    print("case viscosity", dim, depth, variable, noslip, flush=True)
+    u, mu, rho = _problem(dim, depth, 100 * dim + depth, variable)
+    dt = 0.004
+    sigma = -np.ones((dim, 2 * dim), np.int64) if noslip else None
+    g = bmg.Grid(dim, depth)
+    g.set_tolerance(1e-8)
+    o = Viscosity(dim, depth)
+    o.TOLERANCE = 1e-8
+    if noslip:
+        o.sigma = sigma
+    o.set_coefficients(mu, rho)
+    uo, so, _ = o.solve(u, dt)
+    ug = [x.copy() for x in u]
+    sg = g.viscosity(ug, mu=mu, rho=rho, dt=dt, sigma=sigma)
+    for k in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert sg[k] == so[k], (k, sg, so)
+    assert abs(sg["sum"] - so["sum"]) <= 1e-9 * (1.0 + abs(so["sum"]))
+    for c in range(dim):
+        assert np.array_equal(ug[c], uo[c]), c
+
+
+def test_viscosity_device_tensors():
+    torch = pytest.importorskip("torch")
+    print("case viscosity tensors", flush=True)
+    dim, depth = 3, 5
+    u, mu, rho = _problem(dim, depth, 7, True)
+    g = bmg.Grid(dim, depth)
+    g.set_tolerance(1e-9)
+    uh = [x.copy() for x in u]
+    sh = g.viscosity(uh, mu=mu, rho=rho, dt=0.01)
+    ud = [torch.tensor(x, device="cuda:0") for x in u]
+    md = [torch.tensor(m, device="cuda:0") for m in mu]
+    rd = torch.tensor(rho, device="cuda:0")
+    sd = g.viscosity(ud, mu=md, rho=rd, dt=0.01)
+    assert sh == sd
+    for c in range(dim):
+        assert np.array_equal(ud[c].cpu().numpy(), uh[c])
+
+
+def test_viscosity_zero_dt_is_identity_and_errors():
+    print("case viscosity errors", flush=True)
+    g = bmg.Grid(2, 5)
+    u, mu, rho = _problem(2, 5, 3, True)
+    ug = [x.copy() for x in u]
+    s = g.viscosity(ug, mu=mu, rho=rho, dt=0.0)
+    assert s["i"] == 1 and s["resb"] == 0.0
+    for c in range(2):
+        assert np.array_equal(ug[c], u[c])
+    with pytest.raises(bmg.BMGError):
+        g.viscosity([x.copy() for x in u], mu=mu, rho=rho, dt=0.1, sigma=[[2, 1, 1, 1], [1, 1, 1, 1]])
+    with pytest.raises(bmg.BMGError):
+        g.viscosity([x.copy() for x in u], mu=1.0, rho=-1.0, dt=0.1)
+    p = bmg.Grid(2, 5)
+    p.set_bc(bmg.LEFT, bmg.BC_PERIODIC)
+    p.set_bc(bmg.RIGHT, bmg.BC_PERIODIC)
+    with pytest.raises(bmg.BMGError):
+        p.viscosity([x.copy() for x in u], mu=1.0, rho=1.0, dt=0.1)
