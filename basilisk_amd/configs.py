@@ -95,7 +95,37 @@ def config2(depth=8, seed=1, nspheres=24, ratio=1000.0):
         u.append(uc)
     b = divergence_rhs(u, delta)
     return {"dim": 3, "depth": depth, "b": b, "alpha": alpha, "tolerance": 1e-9, "nrelax": 4,
-            "rho_inside_fraction": float(inside.mean()), "u": u}
+            "rho_inside_fraction": float(inside.mean()), "u": u, "inside": inside}
+
+
+def config2_viscous(prob, seed=2, mu_in=1.0, mu_out=0.01, dt=1e-3):
+    """The implicit viscous step of the config-2 two-phase field: cell
+    velocity N(0,1) per component, rho = 1000 in the spheres and 1 outside,
+    mu = mu_in / mu_out arithmetically averaged on faces (walls: the cell's
+    value), dt = 1e-3."""
+    inside = prob["inside"]
+    n = inside.shape[0]
+    rng = np.random.default_rng(seed)
+    muc = np.where(inside, mu_in, mu_out)
+    mu = []
+    for c in range(3):
+        shape = [n, n, n]
+        shape[c] = n + 1
+        f = np.empty(shape)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[c], hi[c] = slice(0, n - 1), slice(1, n)
+        mid = [slice(None)] * 3
+        mid[c] = slice(1, n)
+        f[tuple(mid)] = (muc[tuple(lo)] + muc[tuple(hi)]) / 2.0
+        e0, e1, c0, c1 = [slice(None)] * 3, [slice(None)] * 3, [slice(None)] * 3, [slice(None)] * 3
+        e0[c], e1[c], c0[c], c1[c] = 0, n, 0, n - 1
+        f[tuple(e0)] = muc[tuple(c0)]
+        f[tuple(e1)] = muc[tuple(c1)]
+        mu.append(f)
+    u = [rng.standard_normal((n, n, n)) for _ in range(3)]
+    rho = np.where(inside, 1000.0, 1.0)
+    return {"u": u, "mu": mu, "rho": rho, "dt": dt}
 
 
 def config4(depth=10, seed=4):

@@ -1768,11 +1768,13 @@ struct Visc {
                              vbuf(g, l - 1, VS_DA + c), vbuf(g, l, VS_DA + c), gf, bcc);
         }
       }
-      for (int i = 0; i < nrelax; i++) {
-        hipLaunchKernelGGL((k_visc_relax<DIM>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream,
-                           args(l, lv(l, VS_DA), lv(l, VS_DN), lv(l, VS_RES)), gf);
-        for (int c = 0; c < DIM; c++) std::swap(vbuf(g, l, VS_DA + c), vbuf(g, l, VS_DN + c));
-      }
+      const bool jacobi = g->smoother == BMG_RELAX_JACOBI;
+      for (int i = 0; i < nrelax; i++)
+        for (int colour = jacobi ? -1 : 0; colour < (jacobi ? 0 : 2); colour++) {
+          hipLaunchKernelGGL((k_visc_relax<DIM>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream,
+                             args(l, lv(l, VS_DA), lv(l, VS_DN), lv(l, VS_RES)), gf, colour);
+          for (int c = 0; c < DIM; c++) std::swap(vbuf(g, l, VS_DA + c), vbuf(g, l, VS_DN + c));
+        }
     }
     const size_t n = g->L[D].ncell;
     for (int c = 0; c < DIM; c++)

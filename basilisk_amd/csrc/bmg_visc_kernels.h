@@ -1,9 +1,14 @@
 // libbmg: kernels of viscosity() on the uniform multigrid grid, the second
 // client of mg_solve (viscosity.h:27-178), FP64.
 //
-//   relax_viscosity     viscosity.h:27-96, the JACOBI variant: w from the
-//                       old iterate for every component, then
-//                       u = (u + 2 w)/3 (non-AXI, lambda = 1)
+//   relax_viscosity     viscosity.h:27-96 (non-AXI, lambda = 1): the JACOBI
+//                       variant, w from the old iterate for every component,
+//                       then u = (u + 2 w)/3; or red-black Gauss-Seidel (an
+//                       extension standing in for the sequential in-place
+//                       default): the cells with i+j+k of one parity take w,
+//                       read from the state before the pass, the others
+//                       keep their value (one pass per colour, out of place:
+//                       the cross terms read same-colour diagonal cells)
 //   residual_viscosity  viscosity.h:98-160, the "naive" (!TREE) form
 //
 // Component c couples to the other components e through the cross terms of
@@ -111,21 +116,28 @@ __device__ __forceinline__ double visc_diag(const ViscArgs& A, const Geo<DIM>& g
   return s;
 }
 
-// one Jacobi sweep of relax_viscosity over a level: out = (u + 2 w)/3
+// one sweep of relax_viscosity over a level: colour < 0, Jacobi,
+// out = (u + 2 w)/3; colour 0 or 1, the red-black pass of that parity,
+// out = w there and u elsewhere
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_visc_relax(ViscArgs A, Geo<DIM> g) {
+__global__ __launch_bounds__(kBlock) void k_visc_relax(ViscArgs A, Geo<DIM> g, int colour) {
   const idx_t idx = logical_block() * kBlock + threadIdx.x;
   if (idx >= g.ncell()) return;
   int x[3];
   g.decode(idx, x);
+  const bool upd = colour < 0 || ((x[0] + x[1] + x[2]) & 1) == colour;
   const double D2 = A.D * A.D;
   const double dtr = A.dt / (A.rho ? A.rho[idx] : A.rho_c);
 #pragma unroll
   for (int c = 0; c < DIM; c++) {
     const double u0 = A.u[c][idx];
-    const double w = (dtr * visc_terms<DIM, false>(A, g, idx, x, c, u0) + A.r[c][idx] * D2) /
-                     (D2 * 1. + dtr * visc_diag<DIM>(A, g, idx, x, c));
-    A.out[c][idx] = (u0 + 2. * w) / 3.;
+    double v = u0;
+    if (upd) {
+      const double w = (dtr * visc_terms<DIM, false>(A, g, idx, x, c, u0) + A.r[c][idx] * D2) /
+                       (D2 * 1. + dtr * visc_diag<DIM>(A, g, idx, x, c));
+      v = colour < 0 ? (u0 + 2. * w) / 3. : w;
+    }
+    A.out[c][idx] = v;
   }
 }
 

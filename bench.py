@@ -187,8 +187,32 @@ def run_precision(bmg, torch, prob, prec, args):
                 ptimes.append(time.perf_counter() - t1)
         proj = {"seconds": sum(ptimes) / len(ptimes), "stats": pst}
         del u, u0
+    visc = None
+    if prob.get("inside") is not None and prec == "fp64":
+        # viscosity() (viscosity.h:163-178) of a two-phase velocity field on
+        # the same grid: the second mg_solve client, FP64, Jacobi
+        from basilisk_amd import configs
+        vp = configs.config2_viscous(prob)
+        v0 = [torch.from_numpy(x).to(dev) for x in vp["u"]]
+        mu = [torch.from_numpy(np.ascontiguousarray(m)).to(dev) for m in vp["mu"]]
+        rho = torch.from_numpy(vp["rho"]).to(dev)
+        del vp["u"], vp["mu"], vp["rho"]
+        uv = [x.clone() for x in v0]
+        vtimes = []
+        for it in range(args.warmup + args.steps):
+            for x, y in zip(uv, v0):
+                x.copy_(y)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            vst = g.viscosity(uv, mu=mu, rho=rho, dt=vp["dt"])
+            g.synchronize()
+            if it >= args.warmup:
+                vtimes.append(time.perf_counter() - t1)
+        visc = {"seconds": sum(vtimes) / len(vtimes), "stats": vst}
+        del uv, v0, mu, rho
     g.close()
-    return {"seconds": dt, "stats": stats[-1], "profile": kinds, "profiled_stats": st, "a": a, "project": proj}
+    return {"seconds": dt, "stats": stats[-1], "profile": kinds, "profiled_stats": st, "a": a, "project": proj,
+            "viscosity": visc}
 
 
 def cpu_baseline(prob, args, ncycles_target):
@@ -296,6 +320,13 @@ def main():
         # solver's caller, divergence and face update included
         line["project_ms"] = round(r["project"]["seconds"] * 1e3, 3)
         line["project_cycles"] = r["project"]["stats"]["i"]
+    rv = res.get("fp64", {}).get("viscosity")
+    if rv:
+        # viscosity() of a two-phase velocity (rho 1000/1, mu 1/0.01, dt 1e-3)
+        # to the same tolerance: FP64, weighted-Jacobi relax_viscosity
+        line["viscosity_ms"] = round(rv["seconds"] * 1e3, 3)
+        line["viscosity_cycles"] = rv["stats"]["i"]
+        line["viscosity_resa"] = rv["stats"]["resa"]
     if not args.no_cpu_baseline:
         ncyc = res["fp64"]["stats"]["i"] if "fp64" in res else cycles
         line["cpu_baseline"] = cpu_baseline(prob, args, ncyc)

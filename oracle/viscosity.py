@@ -8,9 +8,14 @@ round it), of
 
   viscosity()            viscosity.h:163-178 (r = u, restriction ({mu, rho}),
                          mg_solve with minlevel 0 and TOLERANCE)
-  relax_viscosity        viscosity.h:27-96, the JACOBI variant (w from the
-                         old iterate, then u = (u + 2 w)/3), non-AXI,
-                         lambda = 1
+  relax_viscosity        viscosity.h:27-96, non-AXI, lambda = 1: the JACOBI
+                         variant (w from the old iterate, then
+                         u = (u + 2 w)/3), or red-black Gauss-Seidel (an
+                         extension standing in for the sequential in-place
+                         default): the cells with i+j+k even, then the odd
+                         ones, take w computed from the state before their
+                         pass (the cross terms read same-colour diagonal
+                         neighbours, which keep their old values)
   residual_viscosity     viscosity.h:98-160, the "naive" (!TREE) form
   mg_solve, mg_cycle     poisson.h:32-90, 139-235 (nrelax default 4, the
                          nrelax adaptation, NITERMIN/NITERMAX)
@@ -91,6 +96,7 @@ class Viscosity:
         self.sigma = default_sigma(dim)
         self.TOLERANCE = 1e-3
         self.NITERMAX, self.NITERMIN = 100, 1
+        self.smoother = "jacobi"
 
     def delta(self, l):
         return self.L0 / (1 << l)
@@ -183,19 +189,27 @@ class Viscosity:
             s = s + self._mu(l, e, True, n) + self._mu(l, e, False, n)
         return s
 
-    def relax(self, da, res, l, dt):
-        """relax_viscosity, JACOBI: returns the new components."""
+    def _target(self, da, res, l, dt):
+        """The value w of relax_viscosity for every cell and component."""
         dim = self.dim
         n = 1 << l
         D = self.delta(l)
         D2 = D * D
         P = [_pad(da[c], self.sigma[c], dim) for c in range(dim)]
         dtr = self._dtr(l, dt)
-        out = []
-        for c in range(dim):
-            w = (dtr * self._terms(P, c, l, n, False) + res[c] * D2) / (D2 * 1. + dtr * self._diag(c, l, n))
-            out.append(w)
-        return [(da[c] + 2. * out[c]) / 3. for c in range(dim)]
+        return [(dtr * self._terms(P, c, l, n, False) + res[c] * D2) / (D2 * 1. + dtr * self._diag(c, l, n))
+                for c in range(dim)]
+
+    def relax(self, da, res, l, dt):
+        """relax_viscosity: returns the new components."""
+        if self.smoother == "jacobi":
+            w = self._target(da, res, l, dt)
+            return [(da[c] + 2. * w[c]) / 3. for c in range(self.dim)]
+        parity = np.indices((1 << l,) * self.dim).sum(0) & 1
+        for colour in (0, 1):
+            w = self._target(da, res, l, dt)
+            da = [np.where(parity == colour, w[c], da[c]) for c in range(self.dim)]
+        return da
 
     def residual(self, u, r, dt):
         """residual_viscosity on the finest level: (res list, max |res|)."""

@@ -1,8 +1,10 @@
 """viscosity() on the GPU (bmg_viscosity, viscosity.h:163-178) against the
 oracle's restatement (oracle/viscosity.py): with the Jacobi relax_viscosity
-the solver statistics and every velocity component equal the oracle's bit
-for bit, for variable and constant mu and rho, the default wall BCs and no
-slip, in 2D and 3D.  The oracle is pinned by test_viscosity_oracle.py."""
+(the reference's JACOBI variant) and with the red-black Gauss-Seidel
+extension, the solver statistics and every velocity component equal the
+oracle's bit for bit, for variable and constant mu and rho, the default wall
+BCs and no slip, in 2D and 3D.  The oracle is pinned by
+test_viscosity_oracle.py."""
 import numpy as np
 import pytest
 
@@ -28,14 +30,16 @@ def _problem(dim, depth, seed, variable):
 @pytest.mark.parametrize("dim,depth", [(2, 6), (2, 7), (3, 4), (3, 5)])
 @pytest.mark.parametrize("variable", [True, False])
 @pytest.mark.parametrize("noslip", [False, True])
-def test_viscosity_bitexact(dim, depth, variable, noslip):
-    print("case viscosity", dim, depth, variable, noslip, flush=True)
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+def test_viscosity_bitexact(dim, depth, variable, noslip, smoother):
+    print("case viscosity", dim, depth, variable, noslip, smoother, flush=True)
     u, mu, rho = _problem(dim, depth, 100 * dim + depth, variable)
     dt = 0.004
     sigma = -np.ones((dim, 2 * dim), np.int64) if noslip else None
-    g = bmg.Grid(dim, depth)
+    g = bmg.Grid(dim, depth, smoother=smoother)
     g.set_tolerance(1e-8)
     o = Viscosity(dim, depth)
+    o.smoother = smoother
     o.TOLERANCE = 1e-8
     if noslip:
         o.sigma = sigma

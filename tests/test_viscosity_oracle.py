@@ -8,7 +8,8 @@ logs come from whole Navier-Stokes runs), so:
 - with constant mu the operator, walls included, is symmetric;
 - relax_viscosity and residual_viscosity describe the same operator: the
   Jacobi target satisfies w - u = Delta^2 res / diag;
-- mg_solve converges to the direct sparse solution of the same operator.
+- mg_solve converges to the direct sparse solution of the same operator,
+  with the Jacobi relax and with the red-black Gauss-Seidel extension.
 The per-cycle iterates have no reference output and are parity unpinned
 beyond these checks.
 """
@@ -91,8 +92,10 @@ def test_relax_matches_residual(dim, depth):
 
 
 @pytest.mark.parametrize("dim,depth,sigma", [(2, 5, None), (2, 4, "noslip"), (3, 3, None)])
-def test_solve_matches_direct(dim, depth, sigma):
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+def test_solve_matches_direct(dim, depth, sigma, smoother):
     V = Viscosity(dim, depth)
+    V.smoother = smoother
     n = 1 << depth
     rng = np.random.default_rng(3 * dim + depth)
     if sigma == "noslip":
