@@ -1771,8 +1771,14 @@ struct Visc {
       const bool jacobi = g->smoother == BMG_RELAX_JACOBI;
       for (int i = 0; i < nrelax; i++)
         for (int colour = jacobi ? -1 : 0; colour < (jacobi ? 0 : 2); colour++) {
-          hipLaunchKernelGGL((k_visc_relax<DIM>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream,
-                             args(l, lv(l, VS_DA), lv(l, VS_DN), lv(l, VS_RES)), gf, colour);
+          const ViscArgs A = args(l, lv(l, VS_DA), lv(l, VS_DN), lv(l, VS_RES));
+          if (lv_.n < 2)
+            hipLaunchKernelGGL((k_visc_one<DIM>), dim3(1), dim3(64), 0, g->stream, A, gf, colour);
+          else if (jacobi)
+            hipLaunchKernelGGL((k_visc_jacobi<DIM>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream, A, gf);
+          else
+            hipLaunchKernelGGL((k_visc_rb<DIM>), dim3(nblocks(lv_.ncell / 2)), dim3(kBlock), 0, g->stream, A, gf,
+                               colour);
           for (int c = 0; c < DIM; c++) std::swap(vbuf(g, l, VS_DA + c), vbuf(g, l, VS_DN + c));
         }
     }

@@ -43,44 +43,93 @@ struct ViscArgs {
   double dt, D;
 };
 
-// component field f at cell x + o (|o_d| <= 1), with the wall ghosts
+// the values a cell's stencils read: component m at offset o (|o_d| <= 1),
+// slot (o_x+1)*9 + (o_y+1)*3 + (o_z+1) in 3D, (o_x+1)*3 + (o_y+1) in 2D.
+// Component m is read at the centre and its face neighbours (as the
+// component being relaxed) and, for every other component c, at o_c = +-1
+// with o_m in {-1, 0, 1} (the cross terms): 15 values in 3D, 7 in 2D.
 template <int DIM>
-__device__ __forceinline__ double v_at(const double* __restrict__ f, const Geo<DIM>& g, const int* sig,
-                                       const int x[3], const int o[3]) {
-  int q[3] = {x[0] + o[0], x[1] + o[1], x[2] + o[2]};
-  double s = 1.0;
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    if (q[d] < 0) {
-      q[d] = 0;
-      s *= (double)sig[2 * d + 1];
-    } else if (q[d] >= g.n) {
-      q[d] = g.n - 1;
-      s *= (double)sig[2 * d];
-    }
+struct VNb {
+  static constexpr int NS = DIM == 3 ? 27 : 9;
+  double v[DIM][NS];
+  static __device__ __forceinline__ constexpr int slot(int ox, int oy, int oz) {
+    return DIM == 3 ? (ox + 1) * 9 + (oy + 1) * 3 + (oz + 1) : (ox + 1) * 3 + (oy + 1);
   }
-  return s * f[g.index(q)];
-}
+  static __device__ __forceinline__ constexpr bool needed(int m, int ox, int oy, int oz) {
+    const int o[3] = {ox, oy, oz};
+    int nz = 0;
+    for (int d = 0; d < 3; d++) nz += o[d] != 0;
+    if (nz <= 1) return true;  // centre and face neighbours
+    if (nz > 2 || o[m] == 0) return false;
+    return true;  // o_m = +-1 and one other axis +-1
+  }
+};
+
+// per-axis addressing of the 3^dim neighbourhood of cell x: the clamped
+// offset along each axis and, per component, the wall factor of a ghost
+template <int DIM>
+struct VAxes {
+  idx_t off[3][3];
+  double sg[3][3][3];  // [component][axis][o+1]
+  __device__ __forceinline__ void init(const ViscArgs& A, const Geo<DIM>& g, const int x[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int o = -1; o <= 1; o++) {
+        int q = d < DIM ? x[d] + o : 0;
+        const bool lo = d < DIM && q < 0, hi = d < DIM && q >= g.n;
+        q = lo ? 0 : hi ? g.n - 1 : q;
+        off[d][o + 1] = d < DIM ? (idx_t)q * g.stride(d) : 0u;
+#pragma unroll
+        for (int m = 0; m < 3; m++) sg[m][d][o + 1] = lo ? (double)A.sig[m][2 * d + 1] : hi ? (double)A.sig[m][2 * d] : 1.0;
+      }
+  }
+};
 
 template <int DIM>
-__device__ __forceinline__ double mu_face(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3], int d,
-                                          bool hi) {
-  if (!A.mlo[d]) return A.mu_c;
-  return hi ? alpha_hi<DIM, double>(A.mlo[d], A.mwall[d], g, idx, x, d) : A.mlo[d][idx];
+__device__ __forceinline__ void visc_load(const ViscArgs& A, const VAxes<DIM>& X, VNb<DIM>& N) {
+#pragma unroll
+  for (int m = 0; m < DIM; m++)
+#pragma unroll
+    for (int ox = -1; ox <= 1; ox++)
+#pragma unroll
+      for (int oy = -1; oy <= 1; oy++)
+#pragma unroll
+        for (int oz = (DIM == 3 ? -1 : 0); oz <= (DIM == 3 ? 1 : 0); oz++)
+          if (VNb<DIM>::needed(m, ox, oy, oz)) {
+            const idx_t a = X.off[0][ox + 1] + X.off[1][oy + 1] + X.off[2][oz + 1];
+            const double s = (X.sg[m][0][ox + 1] * X.sg[m][1][oy + 1]) * X.sg[m][2][oz + 1];
+            N.v[m][VNb<DIM>::slot(ox, oy, oz)] = s * A.u[m][a];
+          }
+}
+
+// mu on the low (0) and high (1) face of cell idx along each axis
+template <int DIM>
+__device__ __forceinline__ void visc_mu(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3],
+                                        double mu[3][2]) {
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    if (A.mlo[d]) {
+      mu[d][0] = A.mlo[d][idx];
+      mu[d][1] = alpha_hi<DIM, double>(A.mlo[d], A.mwall[d], g, idx, x, d);
+    } else {
+      mu[d][0] = mu[d][1] = A.mu_c;
+    }
+  }
 }
 
 // the bracket of relax_viscosity (RES = false: the neighbour terms of the
 // numerator) or of residual_viscosity (RES = true) for component c
 template <int DIM, bool RES>
-__device__ __forceinline__ double visc_terms(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3],
-                                             int c, double u0) {
+__device__ __forceinline__ double visc_terms(const VNb<DIM>& N, const double mu[3][2], int c) {
   auto U = [&](int comp, int d1, int s1, int d2, int s2) {
     int o[3] = {0, 0, 0};
     o[d1] += s1;
     o[d2] += s2;
-    return v_at<DIM>(A.u[comp], g, A.sig[comp], x, o);
+    return N.v[comp][VNb<DIM>::slot(o[0], o[1], o[2])];
   };
-  const double mch = mu_face<DIM>(A, g, idx, x, c, true), mcl = mu_face<DIM>(A, g, idx, x, c, false);
+  const double u0 = N.v[c][VNb<DIM>::slot(0, 0, 0)];
+  const double mch = mu[c][1], mcl = mu[c][0];
   double S;
   if (RES)
     S = 2. * mch * (U(c, c, 1, c, 0) - u0) - 2. * mcl * (u0 - U(c, c, -1, c, 0));
@@ -89,7 +138,7 @@ __device__ __forceinline__ double visc_terms(const ViscArgs& A, const Geo<DIM>& 
 #pragma unroll
   for (int k = 1; k < DIM; k++) {
     const int e = (c + k) % DIM;
-    const double meh = mu_face<DIM>(A, g, idx, x, e, true), mel = mu_face<DIM>(A, g, idx, x, e, false);
+    const double meh = mu[e][1], mel = mu[e][0];
     const double up = (U(e, c, 1, e, 0) + U(e, c, 1, e, 1)) / 4.;
     const double um = (U(e, c, -1, e, 0) + U(e, c, -1, e, 1)) / 4.;
     const double lp = (U(e, c, 1, e, -1) + U(e, c, 1, e, 0)) / 4.;
@@ -106,38 +155,81 @@ __device__ __forceinline__ double visc_terms(const ViscArgs& A, const Geo<DIM>& 
 }
 
 template <int DIM>
-__device__ __forceinline__ double visc_diag(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3], int c) {
-  double s = 2. * mu_face<DIM>(A, g, idx, x, c, true) + 2. * mu_face<DIM>(A, g, idx, x, c, false);
+__device__ __forceinline__ double visc_diag(const double mu[3][2], int c) {
+  double s = 2. * mu[c][1] + 2. * mu[c][0];
 #pragma unroll
   for (int k = 1; k < DIM; k++) {
     const int e = (c + k) % DIM;
-    s = s + mu_face<DIM>(A, g, idx, x, e, true) + mu_face<DIM>(A, g, idx, x, e, false);
+    s = s + mu[e][1] + mu[e][0];
   }
   return s;
 }
 
-// one sweep of relax_viscosity over a level: colour < 0, Jacobi,
-// out = (u + 2 w)/3; colour 0 or 1, the red-black pass of that parity,
-// out = w there and u elsewhere
+// w of relax_viscosity for every component of cell idx
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_visc_relax(ViscArgs A, Geo<DIM> g, int colour) {
+__device__ __forceinline__ void visc_target(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3],
+                                            double w[3]) {
+  VAxes<DIM> X;
+  X.init(A, g, x);
+  VNb<DIM> N;
+  visc_load<DIM>(A, X, N);
+  double mu[3][2];
+  visc_mu<DIM>(A, g, idx, x, mu);
+  const double D2 = A.D * A.D;
+  const double dtr = A.dt / (A.rho ? A.rho[idx] : A.rho_c);
+#pragma unroll
+  for (int c = 0; c < DIM; c++)
+    w[c] = (dtr * visc_terms<DIM, false>(N, mu, c) + A.r[c][idx] * D2) / (D2 * 1. + dtr * visc_diag<DIM>(mu, c));
+}
+
+// one Jacobi sweep of relax_viscosity over a level: out = (u + 2 w)/3
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_visc_jacobi(ViscArgs A, Geo<DIM> g) {
   const idx_t idx = logical_block() * kBlock + threadIdx.x;
   if (idx >= g.ncell()) return;
   int x[3];
   g.decode(idx, x);
-  const bool upd = colour < 0 || ((x[0] + x[1] + x[2]) & 1) == colour;
-  const double D2 = A.D * A.D;
-  const double dtr = A.dt / (A.rho ? A.rho[idx] : A.rho_c);
+  double w[3];
+  visc_target<DIM>(A, g, idx, x, w);
+#pragma unroll
+  for (int c = 0; c < DIM; c++) A.out[c][idx] = (A.u[c][idx] + 2. * w[c]) / 3.;
+}
+
+// one red-black pass: the cells whose i+j+k has the parity `colour` take w,
+// read from the pass input, the others are copied.  One thread per pair of
+// cells along the last dimension (one of each parity, so no wave diverges);
+// levels with n >= 2.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_visc_rb(ViscArgs A, Geo<DIM> g, int colour) {
+  const idx_t t = logical_block() * kBlock + threadIdx.x;
+  if (t >= g.ncell() / 2) return;
+  const idx_t pair = 2 * t;  // the pair's first cell (k even)
+  int x[3];
+  g.decode(pair, x);
+  const int par = (x[0] + x[1] + x[2]) & 1;
+  const idx_t idx = par == colour ? pair : pair + 1;
+  const idx_t other = par == colour ? pair + 1 : pair;
+  x[DIM - 1] += par == colour ? 0 : 1;
+  double w[3];
+  visc_target<DIM>(A, g, idx, x, w);
 #pragma unroll
   for (int c = 0; c < DIM; c++) {
-    const double u0 = A.u[c][idx];
-    double v = u0;
-    if (upd) {
-      const double w = (dtr * visc_terms<DIM, false>(A, g, idx, x, c, u0) + A.r[c][idx] * D2) /
-                       (D2 * 1. + dtr * visc_diag<DIM>(A, g, idx, x, c));
-      v = colour < 0 ? (u0 + 2. * w) / 3. : w;
-    }
-    A.out[c][idx] = v;
+    A.out[c][idx] = w[c];
+    A.out[c][other] = A.u[c][other];
+  }
+}
+
+// the single cell of level 0: either parity, the same arithmetic
+template <int DIM>
+__global__ void k_visc_one(ViscArgs A, Geo<DIM> g, int colour) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int x[3] = {0, 0, 0};
+  double w[3];
+  visc_target<DIM>(A, g, 0, x, w);
+#pragma unroll
+  for (int c = 0; c < DIM; c++) {
+    const double u0 = A.u[c][0];
+    A.out[c][0] = colour < 0 ? (u0 + 2. * w[c]) / 3. : colour == 0 ? w[c] : u0;
   }
 }
 
@@ -150,11 +242,17 @@ __global__ __launch_bounds__(kBlock) void k_visc_residual(ViscArgs A, Geo<DIM> g
   if (idx < g.ncell()) {
     int x[3];
     g.decode(idx, x);
+    VAxes<DIM> X;
+    X.init(A, g, x);
+    VNb<DIM> N;
+    visc_load<DIM>(A, X, N);
+    double mu[3][2];
+    visc_mu<DIM>(A, g, idx, x, mu);
     const double dtr = A.dt / (A.rho ? A.rho[idx] : A.rho_c);
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
-      const double u0 = A.u[c][idx];
-      const double v = (A.r[c][idx] - 1. * u0) + dtr * visc_terms<DIM, true>(A, g, idx, x, c, u0) / (A.D * A.D);
+      const double u0 = N.v[c][VNb<DIM>::slot(0, 0, 0)];
+      const double v = (A.r[c][idx] - 1. * u0) + dtr * visc_terms<DIM, true>(N, mu, c) / (A.D * A.D);
       A.out[c][idx] = v;
       if (v != v) atomicAdd(&st->nans, 1ull);
       else m = fmax(m, fabs(v));
