@@ -48,7 +48,7 @@ EXPORTS = [
     "bmg_tree_set_alpha_faces", "bmg_tree_set_alpha_const", "bmg_tree_set_lambda_const",
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
-    "bmg_tree_adapt_wavelet",
+    "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore",
     "bmg_coef_gradient", "bmg_project", "bmg_viscosity",
 ]
 
@@ -146,6 +146,8 @@ def library():
         "bmg_tree_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_synchronize": (I, [P]),
         "bmg_tree_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "bmg_tree_dump": (I, [P, ctypes.c_char_p, dp, dp, I, D, I]),
+        "bmg_tree_restore": (I, [P, ctypes.c_char_p, dp, dp, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
         "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
         "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_viscosity": (I, [P, dp, dp, dp, dp, dp, dp, D, dp, D, D, dp, I, I, ctypes.POINTER(MGStats)]),
@@ -708,6 +710,35 @@ class Tree:
 
     def get_s(self, out=None):
         return self._get(F_S, out)
+
+    @staticmethod
+    def _dump_fields(names):
+        fmap = {"a": F_A, "b": F_B, "s": F_S}
+        items = list(names.items())
+        for k, _ in items:
+            if k not in fmap:
+                raise ValueError("dump fields are a, b and s (got %r)" % k)
+        ids = (ctypes.c_int * max(1, len(items)))(*[fmap[k] for k, _ in items])
+        nm = (ctypes.c_char_p * max(1, len(items)))(*[str(v).encode() for _, v in items])
+        return ids, nm, len(items)
+
+    def dump(self, path, names=None, t=0.0, i=0):
+        """dump() (output.h:1043-1118): the tree and the leaf fields in the
+        reference's file format; `names` maps "a", "b", "s" to the names
+        written (default: a and b under their own names)."""
+        ids, nm, n = self._dump_fields(names if names is not None else {"a": "a", "b": "b"})
+        _check(self.lib.bmg_tree_dump(self._t, str(path).encode(), ctypes.cast(ids, ctypes.c_void_p),
+                                      ctypes.cast(nm, ctypes.c_void_p), n, float(t), int(i)))
+
+    def restore(self, path, names=None):
+        """restore() (output.h:1221-1390): rebuilds the tree from the file and
+        sets "a", "b", "s" from the columns named in `names`; returns the
+        header's (t, i)."""
+        ids, nm, n = self._dump_fields(names if names is not None else {"a": "a", "b": "b"})
+        tt, ii = ctypes.c_double(0.0), ctypes.c_int(0)
+        _check(self.lib.bmg_tree_restore(self._t, str(path).encode(), ctypes.cast(ids, ctypes.c_void_p),
+                                         ctypes.cast(nm, ctypes.c_void_p), n, ctypes.byref(tt), ctypes.byref(ii)))
+        return tt.value, ii.value
 
     def adapt_wavelet(self, max, maxlevel, minlevel=1):
         """adapt_wavelet ({s}, {max}, maxlevel, minlevel) with s, a and b

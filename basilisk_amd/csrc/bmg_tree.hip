@@ -25,6 +25,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -1729,6 +1730,78 @@ int adapt_remap(bmg_tree* t, double* const olds[3], long old_nslot, const std::v
   return rc;
 }
 
+// ------------------------------------------------------------ dump / restore
+
+// struct DumpHeader, output.h:1049-1053 (coord = 3 doubles)
+struct DumpHeaderC {
+  double t;
+  long len;
+  int i, depth, npe, version;
+  double n[3];
+};
+static_assert(sizeof(DumpHeaderC) == 56, "the reference's DumpHeader layout");
+constexpr int kDumpVersion = 170901;
+constexpr unsigned kDumpLeaf = 2;  // the `leaf` flag, grid/tree.h:31-34
+
+// the active cells in foreach_cell() order (pre-order, stopping at leaves)
+void dump_preorder(Topo& T, int l, IJK c, std::vector<Cell*>& out) {
+  Cell* cp = T.L[l].find(c.i, c.j, c.k);
+  if (!cp) return;
+  out.push_back(cp);
+  if (T.is_leaf(*cp) || l >= T.depth) return;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int e = 0; e < T.kext(); e++)
+        dump_preorder(T, l + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, T.dim == 3 ? 2 * c.k - GH + e : 0}, out);
+}
+
+// subtree sizes in pre-order (subtree_size (size, false),
+// grid/multigrid-common.h:422-445: a leaf 1, a parent 1 + its children)
+size_t dump_sizes(const std::vector<Cell*>& cells, int dim, size_t q, std::vector<double>& size) {
+  if (cells[q]->flags & C_LEAF) {
+    size[q] = 1.;
+    return q + 1;
+  }
+  double sum = 1.;
+  size_t r = q + 1;
+  for (int m = 0; m < (1 << dim); m++) {
+    const size_t nxt = dump_sizes(cells, dim, r, size);
+    sum += size[r];
+    r = nxt;
+  }
+  size[q] = sum;
+  return r;
+}
+
+double* field_ptr(bmg_tree* t, int field) {
+  return field == BMG_F_A ? t->a : field == BMG_F_B ? t->b : field == BMG_F_S ? t->s : nullptr;
+}
+
+// restore(): the pre-order traversal of the growing tree, one record per
+// visited cell, refining a leaf whose record is not a leaf; each cell's
+// record index is kept in its slot until the tree is compiled
+int restore_visit(Topo& T, int l, IJK c, FILE* fp, size_t reclen, std::vector<double>& rec, long& nrec,
+                  const std::vector<int>& cols, std::vector<std::vector<double>>& vals) {
+  Cell* cp = T.L[l].find(c.i, c.j, c.k);
+  if (!cp) return BMG_OK;
+  unsigned flags;
+  if (fread(&flags, sizeof flags, 1, fp) != 1 || fread(rec.data(), sizeof(double), reclen, fp) != reclen)
+    return bmg::fail(BMG_ERR_ARG, "restore: the file ends before the tree does");
+  cp->slot = (int32_t)nrec++;
+  for (size_t m = 0; m < cols.size(); m++) vals[m].push_back(cols[m] >= 0 ? rec[cols[m]] : 0.);
+  if (!(flags & kDumpLeaf) && T.is_leaf(*cp)) {
+    if (l + 1 >= kSeg || l + 1 > kMaxDepth) return bmg::fail(BMG_ERR_UNSUPPORTED, "restore: tree deeper than supported");
+    T.refine_cell(l, c);
+  }
+  if (T.is_leaf(*cp) || l >= T.depth) return BMG_OK;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int e = 0; e < T.kext(); e++)
+        TRY(restore_visit(T, l + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, T.dim == 3 ? 2 * c.k - GH + e : 0}, fp,
+                          reclen, rec, nrec, cols, vals));
+  return BMG_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ C ABI
@@ -1883,6 +1956,149 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
   }
   if (nf) *nf = (int)A.nf;
   if (nc) *nc = (int)A.nc;
+  return BMG_OK;
+}
+
+int bmg_tree_dump(bmg_tree* t, const char* path, const int* fields, const char* const* names, int nfields,
+                  double time, int iter) {
+  if (!t || !path || (nfields && (!fields || !names)) || nfields < 0)
+    return bmg::fail(BMG_ERR_ARG, "null argument");
+  for (int m = 0; m < nfields; m++)
+    if (!names[m] || (fields[m] != BMG_F_A && fields[m] != BMG_F_B && fields[m] != BMG_F_S))
+      return bmg::fail(BMG_ERR_ARG, "dump: field %d is not a, b or s", m);
+  TRY(ensure(t));
+  // non-leaf cells hold the restriction of their leaves
+  if (t->dim == 2)
+    adapt_boundary<2>(t);
+  else
+    adapt_boundary<3>(t);
+  const DevIndex& x = t->ix;
+  std::vector<std::vector<double>> host(nfields, std::vector<double>(x.nslot));
+  for (int m = 0; m < nfields; m++)
+    HIPCHECK(hipMemcpyAsync(host[m].data(), field_ptr(t, fields[m]), x.nslot * 8, hipMemcpyDeviceToHost, t->stream));
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  Topo& T = t->topo;
+  std::vector<Cell*> cells;
+  dump_preorder(T, 0, {GH, GH, t->dim == 3 ? GH : 0}, cells);
+  std::vector<double> size(cells.size());
+  if (dump_sizes(cells, t->dim, 0, size) != cells.size()) return bmg::fail(BMG_ERR_ARG, "dump: inconsistent tree");
+  std::string tmp = std::string(path) + "~";
+  FILE* fp = fopen(tmp.c_str(), "wb");
+  if (!fp) return bmg::fail(BMG_ERR_ARG, "dump: cannot write %s", tmp.c_str());
+  DumpHeaderC h;
+  memset(&h, 0, sizeof h);
+  h.t = time;
+  h.len = nfields + 1;
+  h.i = iter;
+  h.depth = T.depth;
+  h.npe = 1;
+  h.version = kDumpVersion;
+  bool ok = fwrite(&h, sizeof h, 1, fp) == 1;
+  auto name = [&](const char* nm) {
+    unsigned len = (unsigned)strlen(nm);
+    ok = ok && fwrite(&len, sizeof len, 1, fp) == 1 && fwrite(nm, 1, len, fp) == len;
+  };
+  name("size");
+  for (int m = 0; m < nfields; m++) name(names[m]);
+  const double o[4] = {t->X0[0], t->X0[1], t->X0[2], t->L0};
+  ok = ok && fwrite(o, sizeof(double), 4, fp) == 4;
+  const size_t rl = 4 + 8 * (size_t)(nfields + 1);
+  std::vector<char> buf;
+  buf.reserve(rl * 65536);
+  for (size_t q = 0; q < cells.size() && ok; q++) {
+    const unsigned flags = (cells[q]->flags & C_LEAF) ? kDumpLeaf : 0u;
+    const char* fb = reinterpret_cast<const char*>(&flags);
+    buf.insert(buf.end(), fb, fb + 4);
+    const char* sb = reinterpret_cast<const char*>(&size[q]);
+    buf.insert(buf.end(), sb, sb + 8);
+    for (int m = 0; m < nfields; m++) {
+      const char* vb = reinterpret_cast<const char*>(&host[m][cells[q]->slot]);
+      buf.insert(buf.end(), vb, vb + 8);
+    }
+    if (buf.size() >= rl * 65536 || q + 1 == cells.size()) {
+      ok = fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+      buf.clear();
+    }
+  }
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) return bmg::fail(BMG_ERR_ARG, "dump: write error on %s", tmp.c_str());
+  if (rename(tmp.c_str(), path) != 0) return bmg::fail(BMG_ERR_ARG, "dump: cannot rename %s", tmp.c_str());
+  return BMG_OK;
+}
+
+int bmg_tree_restore(bmg_tree* t, const char* path, const int* fields, const char* const* names, int nfields,
+                     double* time, int* iter) {
+  if (!t || !path || (nfields && (!fields || !names)) || nfields < 0)
+    return bmg::fail(BMG_ERR_ARG, "null argument");
+  for (int m = 0; m < nfields; m++)
+    if (!names[m] || (fields[m] != BMG_F_A && fields[m] != BMG_F_B && fields[m] != BMG_F_S))
+      return bmg::fail(BMG_ERR_ARG, "restore: field %d is not a, b or s", m);
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return bmg::fail(BMG_ERR_ARG, "restore: cannot open %s", path);
+  std::unique_ptr<FILE, int (*)(FILE*)> guard(fp, fclose);
+  DumpHeaderC h;
+  if (fread(&h, sizeof h, 1, fp) != 1) return bmg::fail(BMG_ERR_ARG, "restore: expecting header");
+  if (h.version != kDumpVersion)
+    return bmg::fail(BMG_ERR_UNSUPPORTED, "restore: file version %d (reads %d)", h.version, kDumpVersion);
+  if (h.len < 1 || h.len > 100000) return bmg::fail(BMG_ERR_ARG, "restore: bad field count %ld", h.len);
+  std::vector<std::string> fnames;
+  for (long m = 0; m < h.len; m++) {
+    unsigned len;
+    if (fread(&len, sizeof len, 1, fp) != 1 || len > 4096) return bmg::fail(BMG_ERR_ARG, "restore: expecting len");
+    std::string nm(len, '\0');
+    if (len && fread(&nm[0], 1, len, fp) != len) return bmg::fail(BMG_ERR_ARG, "restore: expecting a name");
+    fnames.push_back(nm);
+  }
+  double o[4];
+  if (fread(o, sizeof(double), 4, fp) != 4) return bmg::fail(BMG_ERR_ARG, "restore: expecting coordinates");
+  if (!(o[3] > 0.)) return bmg::fail(BMG_ERR_ARG, "restore: bad box size");
+  // column of each requested field (the subtree size, column 0, is skipped;
+  // a name the file lacks restores as zero)
+  std::vector<int> cols(nfields, -1);
+  for (int m = 0; m < nfields; m++)
+    for (long q = 1; q < h.len; q++)
+      if (fnames[q] == names[m]) cols[m] = (int)q;
+  // a new tree from init_grid (1)
+  free_device(t);
+  t->topo.init(t->dim);
+  t->L0 = o[3];
+  for (int d = 0; d < 3; d++) t->X0[d] = o[d];
+  t->keep_a.clear();
+  t->keep_b.clear();
+  t->keep_lam.clear();
+  t->keep_s.clear();
+  t->coef_ready = false;
+  Topo& T = t->topo;
+  std::vector<double> rec(h.len);
+  std::vector<std::vector<double>> vals(nfields);
+  long nrec = 0;
+  int rc = restore_visit(T, 0, {GH, GH, t->dim == 3 ? GH : 0}, fp, (size_t)h.len, rec, nrec, cols, vals);
+  if (rc != BMG_OK) {
+    t->topo.init(t->dim);
+    return rc;
+  }
+  char extra;
+  if (fread(&extra, 1, 1, fp) == 1) {
+    t->topo.init(t->dim);
+    return bmg::fail(BMG_ERR_ARG, "restore: the file holds more records than the tree");
+  }
+  T.dirty = true;
+  T.update();
+  std::vector<int> recs(T.leaf_c.size());
+  for (size_t n = 0; n < T.leaf_c.size(); n++) {
+    const IJK c = T.leaf_c[n];
+    recs[n] = T.at(T.leaf_level[n], c.i, c.j, c.k).slot;
+  }
+  TRY(ensure(t));
+  t->s_set = false;
+  std::vector<double> leafv(recs.size());
+  for (int m = 0; m < nfields; m++) {
+    for (size_t n = 0; n < recs.size(); n++) leafv[n] = vals[m][recs[n]];
+    TRY(set_leaf_values(t, field_ptr(t, fields[m]), leafv.data(), 0));
+    if (fields[m] == BMG_F_S) t->s_set = true;
+  }
+  if (time) *time = h.t;
+  if (iter) *iter = h.i;
   return BMG_OK;
 }
 

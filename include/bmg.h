@@ -268,6 +268,24 @@ int bmg_tree_refine(bmg_tree *t, const unsigned char *mark, long *nrefined);
    read zero until set again.  nf and nc (may be NULL) receive the numbers of
    refined and coarsened cells (astats). */
 int bmg_tree_adapt_wavelet(bmg_tree *t, double max, int maxlevel, int minlevel, int *nf, int *nc);
+/* dump() (output.h:1043-1118, the non-MPI branch, file version 170901) of
+   the leaf fields `fields` (BMG_F_A, BMG_F_B, BMG_F_S) under `names`: the
+   header (struct DumpHeader: t, the field count with the subtree "size"
+   first, iter, depth, npe = 1, version), the names, {X0, Y0, Z0, L0}, then
+   every active cell in foreach_cell() order: unsigned flags (2 for a leaf),
+   the subtree size and the field values (non-leaf cells: the restriction
+   of their leaves).  Written to `path`~ and renamed, as the reference does. */
+int bmg_tree_dump(bmg_tree *t, const char *path, const int *fields, const char *const *names, int nfields,
+                  double time, int iter);
+/* restore() (output.h:1221-1390) of such a file: the tree is rebuilt from
+   init_grid(1) by the same traversal, refining every leaf whose record is
+   not a leaf (refine_cell with 2:1 balancing); the origin and box size come
+   from the file; `fields` take the leaf values of the columns named `names`
+   (zero when the file has no such column, like the reference's reset).
+   alpha faces and a lambda field must be set again.  time and iter (may be
+   NULL) receive the header's t and i. */
+int bmg_tree_restore(bmg_tree *t, const char *path, const int *fields, const char *const *names, int nfields,
+                     double *time, int *iter);
 /* the cache lists of a level (which: 0 active, 1 prolongation, 2 boundary)
    as (i,j,k) triples with the reference's GHOSTS = 2 offset; ijk may be
    NULL to get the count */

@@ -59,6 +59,7 @@ def _lib(dim):
             "ot_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "ot_boundary": (None, [P, I, I, I]),
             "ot_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I)]),
+            "ot_refine_cell": (I, [P, I, I, I, I]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -282,3 +283,7 @@ class OracleTree:
         nf = ctypes.c_int(0)
         nc = self.lib.ot_adapt_wavelet(self.t, float(max), int(maxlevel), int(minlevel), ctypes.byref(nf))
         return nf.value, nc
+
+    def refine_cell_at(self, level, c):
+        """refine_cell of the leaf (level, i, j, k) with the 2:1 cascade."""
+        return self.lib.ot_refine_cell(self.t, int(level), int(c[0]), int(c[1]), int(c[2]) if self.dim == 3 else 0)

@@ -1309,3 +1309,16 @@ int ot_adapt_wavelet(ot_tree *t, double max, int maxlevel, int minlevel, int *nf
   if (nf) *nf = nrf;
   return ncs;
 }
+
+/* refine one leaf (refine_cell with the 2:1 cascade, no fields): restore()
+   rebuilding a dumped tree (output.h:1340-1360) */
+int ot_refine_cell(ot_tree *t, int l, int i, int j, int k) {
+  if (l < 0 || l > t->depth) return 0;
+  otl *L = &t->L[l];
+  size_t q = lidx(L, i, j, k);
+  if (!L->alloc[q] || !(L->flags[q] & C_LEAF)) return 0;
+  int c[3] = {i, j, k};
+  refine_cell(t, l, c);
+  t->dirty = 1;
+  return 1;
+}
