@@ -99,6 +99,24 @@ def tree_bench(args):
                      "nc": ncs, "leaves_after": t.nleaves()}
             print("# adapt_wavelet: %.1f ms, %d refined, %d coarsened, %d leaves" % (adapt["ms"], nf, ncs,
                   adapt["leaves_after"]), file=sys.stderr, flush=True)
+            # dump() and restore() (output.h:1043-1390) of the adapted tree
+            # with a, b and s, through a file in a scratch directory
+            import tempfile
+            with tempfile.TemporaryDirectory() as td:
+                path = os.path.join(td, "tree.dump")
+                names = {"a": "a", "b": "b", "s": "s"}
+                t3 = time.perf_counter()
+                t.dump(path, names=names)
+                t4 = time.perf_counter()
+                nbytes = os.path.getsize(path)
+                t.restore(path, names=names)
+                t.synchronize()
+                t5 = time.perf_counter()
+            adapt["dump_ms"] = round((t4 - t3) * 1e3, 1)
+            adapt["restore_ms"] = round((t5 - t4) * 1e3, 1)
+            adapt["dump_mb"] = round(nbytes / 1e6, 1)
+            print("# dump: %.1f ms, restore %.1f ms, %.1f MB" % (adapt["dump_ms"], adapt["restore_ms"],
+                  adapt["dump_mb"]), file=sys.stderr, flush=True)
         t.close()
     main_prec = "mixed" if "mixed" in res else args.precisions.split(",")[0]
     r = res[main_prec]
