@@ -411,7 +411,10 @@ struct Topo {
     if (!cp) return;
     Cell& cell = *cp;
     if (cell.pid == 0 && (cell.flags & C_ACTIVE)) active[l].push_back(c);
-    if (!is_boundary(cell)) {
+    // only a cell next to the box has ghost cells in its ring
+    const int hi = (1 << l) + GH - 1;
+    const bool edge = c.i <= GH || c.i >= hi || c.j <= GH || c.j >= hi || (dim == 3 && (c.k <= GH || c.k >= hi));
+    if (edge && !is_boundary(cell)) {
       int e0 = dim == 3 ? -1 : 0, e1 = dim == 3 ? 1 : 0;
       for (int a = -1; a <= 1; a++)
         for (int b = -1; b <= 1; b++)
