@@ -49,7 +49,7 @@ EXPORTS = [
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
     "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore",
-    "bmg_coef_gradient", "bmg_project", "bmg_viscosity",
+    "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
 ]
 
 
@@ -151,6 +151,8 @@ def library():
         "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
         "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_viscosity": (I, [P, dp, dp, dp, dp, dp, dp, D, dp, D, D, dp, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_grid_dump": (I, [P, ctypes.c_char_p, dp, dp, I, D, I]),
+        "bmg_grid_restore": (I, [P, ctypes.c_char_p, dp, dp, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -502,6 +504,33 @@ class Grid:
         _check(self.lib.bmg_viscosity(self._g, up[0], up[1], up[2], mp[0], mp[1], mp[2], mu_c, rp, rho_c,
                                       float(dt), sp, int(nrelax), 1 if dev else 0, ctypes.byref(s)))
         return s.as_dict()
+
+    @staticmethod
+    def _dump_fields(names):
+        fmap = {"a": F_A, "b": F_B}
+        items = list(names.items())
+        for k, _ in items:
+            if k not in fmap:
+                raise ValueError("dump fields are a and b (got %r)" % k)
+        ids = (ctypes.c_int * max(1, len(items)))(*[fmap[k] for k, _ in items])
+        nm = (ctypes.c_char_p * max(1, len(items)))(*[str(v).encode() for _, v in items])
+        return ids, nm, len(items)
+
+    def dump(self, path, names=None, t=0.0, i=0):
+        """dump() (output.h:1043-1118) of every level in the reference's file
+        format; `names` maps "a" and "b" to the names written."""
+        ids, nm, n = self._dump_fields(names if names is not None else {"a": "a", "b": "b"})
+        _check(self.lib.bmg_grid_dump(self._g, str(path).encode(), ctypes.cast(ids, ctypes.c_void_p),
+                                      ctypes.cast(nm, ctypes.c_void_p), n, float(t), int(i)))
+
+    def restore(self, path, names=None):
+        """restore() (output.h:1221-1390) of the finest level; returns the
+        header's (t, i)."""
+        ids, nm, n = self._dump_fields(names if names is not None else {"a": "a", "b": "b"})
+        tt, ii = ctypes.c_double(0.0), ctypes.c_int(0)
+        _check(self.lib.bmg_grid_restore(self._g, str(path).encode(), ctypes.cast(ids, ctypes.c_void_p),
+                                         ctypes.cast(nm, ctypes.c_void_p), n, ctypes.byref(tt), ctypes.byref(ii)))
+        return tt.value, ii.value
 
     # -- kernel timing
     def profile(self, enable=True):

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -1930,6 +1931,200 @@ int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const doub
     }
   }
   return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ dump / restore
+
+namespace {
+
+// struct DumpHeader, output.h:1049-1053 (coord = 3 doubles)
+struct GridDumpHeader {
+  double t;
+  long len;
+  int i, depth, npe, version;
+  double n[3];
+};
+static_assert(sizeof(GridDumpHeader) == 56, "the reference's DumpHeader layout");
+constexpr int kGridDumpVersion = 170901;
+
+// a finest-level field restricted to every level on the GPU
+// (restriction_average, children in foreach_child order), all levels copied
+// to the host
+template <int DIM>
+int dump_levels(bmg_grid* g, const double* fine, std::vector<std::vector<double>>& host) {
+  const int D = g->depth;
+  std::vector<double*> dl(D + 1, nullptr);
+  dl[D] = const_cast<double*>(fine);
+  int rc = BMG_OK;
+  for (int l = D - 1; l >= 0 && rc == BMG_OK; l--) {
+    rc = dalloc((void**)&dl[l], g->L[l].ncell * sizeof(double));
+    if (rc == BMG_OK) {
+      Geo<DIM> gc{g->L[l].n, g->L[l].log2n};
+      hipLaunchKernelGGL((k_restrict<DIM, double>), dim3(nblocks(g->L[l].ncell)), dim3(kBlock), 0, g->stream,
+                         dl[l + 1], dl[l], gc);
+      rc = check_launch();
+    }
+  }
+  host.assign(D + 1, {});
+  for (int l = 0; l <= D && rc == BMG_OK; l++) {
+    host[l].resize(g->L[l].ncell);
+    if (hipMemcpyAsync(host[l].data(), dl[l], g->L[l].ncell * sizeof(double), hipMemcpyDeviceToHost, g->stream) !=
+        hipSuccess)
+      rc = fail(BMG_ERR_HIP, "dump: copy failed");
+  }
+  if (rc == BMG_OK && hipStreamSynchronize(g->stream) != hipSuccess) rc = fail(BMG_ERR_HIP, "dump: sync failed");
+  for (int l = 0; l < D; l++) dfreet(dl[l]);
+  return rc;
+}
+
+size_t grid_index(int dim, int l, const int c[3]) {
+  size_t n = (size_t)1 << l, idx = 0;
+  for (int d = 0; d < dim; d++) idx = idx * n + (size_t)c[d];
+  return idx;
+}
+
+// foreach_cell() on the multigrid: every cell of every level in pre-order
+template <typename F>
+int grid_preorder(int dim, int depth, int l, const int c[3], F&& f) {
+  TRY(f(l, c));
+  if (l == depth) return BMG_OK;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < (dim >= 2 ? 2 : 1); b++)
+      for (int e = 0; e < (dim == 3 ? 2 : 1); e++) {
+        const int ch[3] = {2 * c[0] + a, dim >= 2 ? 2 * c[1] + b : 0, dim == 3 ? 2 * c[2] + e : 0};
+        TRY(grid_preorder(dim, depth, l + 1, ch, f));
+      }
+  return BMG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_grid_dump(bmg_grid* g, const char* path, const int* fields, const char* const* names, int nfields,
+                  double time, int iter) {
+  TRY(check_grid(g));
+  if (!path || nfields < 0 || (nfields && (!fields || !names))) return fail(BMG_ERR_ARG, "null argument");
+  for (int m = 0; m < nfields; m++)
+    if (!names[m] || (fields[m] != BMG_F_A && fields[m] != BMG_F_B))
+      return fail(BMG_ERR_ARG, "dump: field %d is not a or b", m);
+  const int D = g->depth, dim = g->dim;
+  std::vector<std::vector<std::vector<double>>> lv(nfields);
+  for (int m = 0; m < nfields; m++) {
+    const double* f = fields[m] == BMG_F_A ? g->a : g->b;
+    TRY(dim == 1 ? dump_levels<1>(g, f, lv[m]) : dim == 2 ? dump_levels<2>(g, f, lv[m]) : dump_levels<3>(g, f, lv[m]));
+  }
+  std::vector<double> size(D + 1, 1.);
+  for (int l = D - 1; l >= 0; l--) size[l] = 1. + (double)(1 << dim) * size[l + 1];
+  std::string tmp = std::string(path) + "~";
+  FILE* fp = fopen(tmp.c_str(), "wb");
+  if (!fp) return fail(BMG_ERR_ARG, "dump: cannot write %s", tmp.c_str());
+  GridDumpHeader h;
+  memset(&h, 0, sizeof h);
+  h.t = time;
+  h.len = nfields + 1;
+  h.i = iter;
+  h.depth = D;
+  h.npe = 1;
+  h.version = kGridDumpVersion;
+  bool ok = fwrite(&h, sizeof h, 1, fp) == 1;
+  auto name = [&](const char* nm) {
+    unsigned len = (unsigned)strlen(nm);
+    ok = ok && fwrite(&len, sizeof len, 1, fp) == 1 && fwrite(nm, 1, len, fp) == len;
+  };
+  name("size");
+  for (int m = 0; m < nfields; m++) name(names[m]);
+  const double o[4] = {g->X0[0], g->X0[1], g->X0[2], g->L0};
+  ok = ok && fwrite(o, sizeof(double), 4, fp) == 4;
+  std::vector<char> buf;
+  const size_t rl = 4 + 8 * (size_t)(nfields + 1);
+  buf.reserve(rl * 65536 + rl);
+  const int root[3] = {0, 0, 0};
+  (void)grid_preorder(dim, D, 0, root, [&](int l, const int c[3]) {
+    const unsigned flags = l == D ? 2u : 0u;
+    const char* fb = reinterpret_cast<const char*>(&flags);
+    buf.insert(buf.end(), fb, fb + 4);
+    const char* sb = reinterpret_cast<const char*>(&size[l]);
+    buf.insert(buf.end(), sb, sb + 8);
+    const size_t idx = grid_index(dim, l, c);
+    for (int m = 0; m < nfields; m++) {
+      const char* vb = reinterpret_cast<const char*>(&lv[m][l][idx]);
+      buf.insert(buf.end(), vb, vb + 8);
+    }
+    if (buf.size() >= rl * 65536) {
+      ok = ok && fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+      buf.clear();
+    }
+    return BMG_OK;
+  });
+  ok = ok && (buf.empty() || fwrite(buf.data(), 1, buf.size(), fp) == buf.size());
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) return fail(BMG_ERR_ARG, "dump: write error on %s", tmp.c_str());
+  if (rename(tmp.c_str(), path) != 0) return fail(BMG_ERR_ARG, "dump: cannot rename %s", tmp.c_str());
+  return BMG_OK;
+}
+
+int bmg_grid_restore(bmg_grid* g, const char* path, const int* fields, const char* const* names, int nfields,
+                     double* time, int* iter) {
+  TRY(check_grid(g));
+  if (!path || nfields < 0 || (nfields && (!fields || !names))) return fail(BMG_ERR_ARG, "null argument");
+  for (int m = 0; m < nfields; m++)
+    if (!names[m] || (fields[m] != BMG_F_A && fields[m] != BMG_F_B))
+      return fail(BMG_ERR_ARG, "restore: field %d is not a or b", m);
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(BMG_ERR_ARG, "restore: cannot open %s", path);
+  std::unique_ptr<FILE, int (*)(FILE*)> guard(fp, fclose);
+  GridDumpHeader h;
+  if (fread(&h, sizeof h, 1, fp) != 1) return fail(BMG_ERR_ARG, "restore: expecting header");
+  if (h.version != kGridDumpVersion)
+    return fail(BMG_ERR_UNSUPPORTED, "restore: file version %d (reads %d)", h.version, kGridDumpVersion);
+  if (h.depth != g->depth) return fail(BMG_ERR_ARG, "restore: file depth %d, grid depth %d", h.depth, g->depth);
+  if (h.len < 1 || h.len > 100000) return fail(BMG_ERR_ARG, "restore: bad field count %ld", h.len);
+  std::vector<std::string> fnames;
+  for (long m = 0; m < h.len; m++) {
+    unsigned len;
+    if (fread(&len, sizeof len, 1, fp) != 1 || len > 4096) return fail(BMG_ERR_ARG, "restore: expecting len");
+    std::string nm(len, '\0');
+    if (len && fread(&nm[0], 1, len, fp) != len) return fail(BMG_ERR_ARG, "restore: expecting a name");
+    fnames.push_back(nm);
+  }
+  double o[4];
+  if (fread(o, sizeof(double), 4, fp) != 4) return fail(BMG_ERR_ARG, "restore: expecting coordinates");
+  if (!(o[3] > 0.)) return fail(BMG_ERR_ARG, "restore: bad box size");
+  std::vector<int> cols(nfields, -1);
+  for (int m = 0; m < nfields; m++)
+    for (long q = 1; q < h.len; q++)
+      if (fnames[q] == names[m]) cols[m] = (int)q;
+  const int D = g->depth, dim = g->dim;
+  const size_t nf = g->L[D].ncell;
+  std::vector<std::vector<double>> fine(nfields, std::vector<double>(nf, 0.));
+  std::vector<double> rec(h.len);
+  const int root[3] = {0, 0, 0};
+  TRY(grid_preorder(dim, D, 0, root, [&](int l, const int c[3]) -> int {
+    unsigned flags;
+    if (fread(&flags, sizeof flags, 1, fp) != 1 || fread(rec.data(), sizeof(double), h.len, fp) != (size_t)h.len)
+      return fail(BMG_ERR_ARG, "restore: the file ends before the grid does");
+    if (((flags & 2u) != 0) != (l == D)) return fail(BMG_ERR_ARG, "restore: leaf flags do not match a multigrid");
+    if (l == D) {
+      const size_t idx = grid_index(dim, l, c);
+      for (int m = 0; m < nfields; m++)
+        if (cols[m] >= 0) fine[m][idx] = rec[cols[m]];
+    }
+    return BMG_OK;
+  }));
+  char extra;
+  if (fread(&extra, 1, 1, fp) == 1) return fail(BMG_ERR_ARG, "restore: the file holds more records than the grid");
+  if (o[3] != g->L0 || o[0] != g->X0[0] || o[1] != g->X0[1] || o[2] != g->X0[2]) {
+    g->L0 = o[3];
+    for (int d = 0; d < 3; d++) g->X0[d] = o[d];
+    drop_graphs(g);
+  }
+  for (int m = 0; m < nfields; m++) TRY(bmg_set_field(g, fields[m], D, fine[m].data(), 0));
+  if (time) *time = h.t;
+  if (iter) *iter = h.i;
+  return BMG_OK;
 }
 
 }  // extern "C"

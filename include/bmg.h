@@ -173,6 +173,18 @@ int bmg_viscosity(bmg_grid *g, double *u_x, double *u_y, double *u_z, const doub
                   const double *mu_z, double mu_c, const double *rho, double rho_c, double dt, const int *sigma,
                   int nrelax, int on_device, bmg_mgstats *stats);
 
+/* dump() / restore() (output.h:1043-1390, non-MPI, file version 170901) on
+   the multigrid grid: foreach_cell() visits every cell of every level in
+   pre-order; the leaves are the finest level; a coarse cell's values are
+   the restriction of the finest level (computed on the GPU).  `fields`
+   (BMG_F_A, BMG_F_B) are written / read under `names`; restore needs the
+   file's depth to equal the grid's, takes the origin and box size from the
+   file and sets the finest level (zero for a name the file lacks). */
+int bmg_grid_dump(bmg_grid *g, const char *path, const int *fields, const char *const *names, int nfields,
+                  double time, int iter);
+int bmg_grid_restore(bmg_grid *g, const char *path, const int *fields, const char *const *names, int nfields,
+                     double *time, int *iter);
+
 /* Gradient of an objective J(a) with respect to the coefficients (no
    counterpart in the reference: the adjoint behind the torch.autograd
    binding).  L(a) = lambda a + div(alpha grad a) is self-adjoint, so the

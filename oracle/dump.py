@@ -201,3 +201,48 @@ def restore_tree(data, dim):
     for m, nm in enumerate(names[1:], start=1):
         fields[nm] = np.array([cellvals[k][m] for k in keys])
     return t, h, names, fields
+
+
+def grid_dump_bytes(fields, L0=1.0, origin=(0.0, 0.0, 0.0), t=0.0, i=0):
+    """The dump of a uniform multigrid grid (output.h:1088-1118 on
+    grid/multigrid.h): `fields` is a list of (name, finest-level array of
+    shape (n,)*dim).  foreach_cell() visits every cell of every level in
+    pre-order; the leaves are the finest level; a coarse cell holds the
+    restriction_average of its children (as multigrid_restriction leaves
+    it)."""
+    a0 = np.asarray(fields[0][1])
+    dim, n = a0.ndim, a0.shape[0]
+    depth = n.bit_length() - 1
+    levels = []
+    for _, f in fields:
+        lv = [None] * (depth + 1)
+        lv[depth] = np.asarray(f, np.float64)
+        for l in range(depth - 1, -1, -1):
+            s = np.zeros((1 << l,) * dim)
+            for ch in np.ndindex(*(2,) * dim):
+                s = s + lv[l + 1][tuple(slice(o, None, 2) for o in ch)]
+            lv[l] = s / float(1 << dim)
+        levels.append(lv)
+    size = [0.0] * (depth + 1)
+    size[depth] = 1.0
+    for l in range(depth - 1, -1, -1):
+        size[l] = 1.0 + (1 << dim) * size[l + 1]
+    names = ["size"] + [nm for nm, _ in fields]
+    h = DumpHeader(t, len(names), i, depth, 1, DUMP_VERSION, (0.0, 0.0, 0.0))
+    out = bytearray(bytes(h))
+    for nm in names:
+        b = nm.encode()
+        out += struct.pack("=I", len(b)) + b
+    out += struct.pack("=4d", *(list(origin[:3]) + [L0]))
+    fmt = "=I%dd" % len(names)
+
+    def visit(l, idx):
+        vals = [size[l]] + [lv[l][idx] for lv in levels]
+        out.extend(struct.pack(fmt, LEAF if l == depth else 0, *vals))
+        if l == depth:
+            return
+        for ch in np.ndindex(*(2,) * dim):
+            visit(l + 1, tuple(2 * idx[d] + ch[d] for d in range(dim)))
+
+    visit(0, (0,) * dim)
+    return bytes(out)

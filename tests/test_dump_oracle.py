@@ -104,3 +104,39 @@ def test_truncated_file_refused():
         dump.read_dump(data[:-3], 2)
     h = struct.unpack_from("=i", data, 28)[0]
     assert h == 170901
+
+
+@pytest.mark.parametrize("dim,depth", [(1, 5), (2, 4), (3, 3)])
+def test_grid_records_invariants(dim, depth):
+    """The multigrid dump: one record per cell of every level, leaves
+    exactly the finest level, subtree sizes 1 + 2^dim x the child's, each
+    coarse value the average of its children, the finest values in C order
+    of a pre-order walk."""
+    n = 1 << depth
+    a = np.random.default_rng(dim).standard_normal((n,) * dim)
+    data = dump.grid_dump_bytes([("a", a)], L0=3.0, origin=(1.0, 2.0, 3.0), t=0.5, i=2)
+    h, names, origin, recs = dump.read_dump(data, dim)
+    assert (h.depth, h.len, h.version, h.i) == (depth, 2, 170901, 2)
+    assert origin == (1.0, 2.0, 3.0, 3.0)
+    assert len(recs) == sum((1 << (dim * l)) for l in range(depth + 1))
+    assert sum(1 for f, _ in recs if f == 2) == n ** dim
+    assert recs[0][1][0] == len(recs)
+    nch = 1 << dim
+
+    def check(q, l):
+        f, v = recs[q]
+        if l == depth:
+            assert f == 2 and v[0] == 1.0
+            return q + 1, [v[1]]
+        r, leaves, s = q + 1, [], 0.0
+        for _ in range(nch):
+            s = s + recs[r][1][1]
+            r, lv = check(r, l + 1)
+            leaves += lv
+        assert v[1] == s / nch
+        return r, leaves
+
+    end, leaves = check(0, 0)
+    assert end == len(recs)
+    # the pre-order leaf sequence is a permutation of the finest level
+    assert sorted(leaves) == sorted(a.reshape(-1).tolist())
