@@ -66,23 +66,29 @@ struct VNb {
 };
 
 // per-axis addressing of the 3^dim neighbourhood of cell x: the clamped
-// offset along each axis and, per component, the wall factor of a ghost
+// offset along each axis, and which of its -1 / +1 neighbours are ghosts
+// (their factor, a component's sigma of that side, is applied at the load)
 template <int DIM>
 struct VAxes {
   idx_t off[3][3];
-  double sg[3][3][3];  // [component][axis][o+1]
-  __device__ __forceinline__ void init(const ViscArgs& A, const Geo<DIM>& g, const int x[3]) {
+  int lo, hi;  // bit d: the -1 / +1 neighbour along d is a ghost
+  __device__ __forceinline__ void init(const Geo<DIM>& g, const int x[3]) {
+    lo = hi = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++)
 #pragma unroll
       for (int o = -1; o <= 1; o++) {
         int q = d < DIM ? x[d] + o : 0;
-        const bool lo = d < DIM && q < 0, hi = d < DIM && q >= g.n;
-        q = lo ? 0 : hi ? g.n - 1 : q;
+        const bool l = d < DIM && q < 0, h = d < DIM && q >= g.n;
+        if (l) lo |= 1 << d;
+        if (h) hi |= 1 << d;
+        q = l ? 0 : h ? g.n - 1 : q;
         off[d][o + 1] = d < DIM ? (idx_t)q * g.stride(d) : 0u;
-#pragma unroll
-        for (int m = 0; m < 3; m++) sg[m][d][o + 1] = lo ? (double)A.sig[m][2 * d + 1] : hi ? (double)A.sig[m][2 * d] : 1.0;
       }
+  }
+  // the ghost factor of component m at offset o along d
+  __device__ __forceinline__ int factor(const ViscArgs& A, int m, int d, int o) const {
+    return o < 0 ? ((lo >> d) & 1 ? A.sig[m][2 * d + 1] : 1) : o > 0 ? ((hi >> d) & 1 ? A.sig[m][2 * d] : 1) : 1;
   }
 };
 
@@ -98,8 +104,8 @@ __device__ __forceinline__ void visc_load(const ViscArgs& A, const VAxes<DIM>& X
         for (int oz = (DIM == 3 ? -1 : 0); oz <= (DIM == 3 ? 1 : 0); oz++)
           if (VNb<DIM>::needed(m, ox, oy, oz)) {
             const idx_t a = X.off[0][ox + 1] + X.off[1][oy + 1] + X.off[2][oz + 1];
-            const double s = (X.sg[m][0][ox + 1] * X.sg[m][1][oy + 1]) * X.sg[m][2][oz + 1];
-            N.v[m][VNb<DIM>::slot(ox, oy, oz)] = s * A.u[m][a];
+            const int s = X.factor(A, m, 0, ox) * X.factor(A, m, 1, oy) * X.factor(A, m, 2, oz);
+            N.v[m][VNb<DIM>::slot(ox, oy, oz)] = (double)s * A.u[m][a];
           }
 }
 
@@ -170,7 +176,7 @@ template <int DIM>
 __device__ __forceinline__ void visc_target(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3],
                                             double w[3]) {
   VAxes<DIM> X;
-  X.init(A, g, x);
+  X.init(g, x);
   VNb<DIM> N;
   visc_load<DIM>(A, X, N);
   double mu[3][2];
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_residual(ViscArgs A, Geo<DIM> g
     int x[3];
     g.decode(idx, x);
     VAxes<DIM> X;
-    X.init(A, g, x);
+    X.init(g, x);
     VNb<DIM> N;
     visc_load<DIM>(A, X, N);
     double mu[3][2];
