@@ -493,6 +493,10 @@ struct DevIndex {
   int *nbr = nullptr, *rec_self = nullptr, *rec_child0 = nullptr, *rec_ring = nullptr;
   uint8_t* colour = nullptr;
   int *leaf_gs = nullptr, *lvl_leaves = nullptr;
+  // per level and colour: the active slots, then the leaf slots (red-black
+  // passes launch over one colour only)
+  int* col_list = nullptr;
+  std::vector<int> col_off, col_n, lcol_off, lcol_n;  // [2 * level + colour]
   BoxRule* box = nullptr;
   FaceOp* fops = nullptr;
   FaceAvg *hf = nullptr, *fr = nullptr;
@@ -577,7 +581,7 @@ void free_device(bmg_tree* t) {
   drop_graphs(t);
   DevIndex& x = t->ix;
   for (void* p : {(void*)x.nbr, (void*)x.rec_self, (void*)x.rec_child0, (void*)x.rec_ring, (void*)x.colour,
-                  (void*)x.leaf_gs, (void*)x.lvl_leaves, (void*)x.box, (void*)x.fops, (void*)x.hf, (void*)x.fr,
+                  (void*)x.leaf_gs, (void*)x.lvl_leaves, (void*)x.col_list, (void*)x.box, (void*)x.fops, (void*)x.hf, (void*)x.fr,
                   (void*)x.fp, (void*)x.fc, (void*)x.fv_gs, (void*)x.fv_d, (void*)x.fc_tmp, (void*)x.lvinfo})
     dfree_(p);
   x = DevIndex();
@@ -1044,6 +1048,27 @@ int compile(bmg_tree* t) {
   TRY(upload(&x.rec_ring, rring));
   TRY(upload(&x.leaf_gs, leaf_gs));
   TRY(upload(&x.lvl_leaves, lvl_leaves));
+  {
+    std::vector<int> cl;
+    x.col_off.assign(2 * (D + 1), 0);
+    x.col_n.assign(2 * (D + 1), 0);
+    x.lcol_off.assign(2 * (D + 1), 0);
+    x.lcol_n.assign(2 * (D + 1), 0);
+    for (int l = 0; l <= D; l++)
+      for (int c = 0; c < 2; c++) {
+        x.col_off[2 * l + c] = (int)cl.size();
+        for (int q = 0; q < x.nA[l]; q++)
+          if (colour[(size_t)x.abase[l] + q] == c) cl.push_back(x.off[l] + q);
+        x.col_n[2 * l + c] = (int)cl.size() - x.col_off[2 * l + c];
+        x.lcol_off[2 * l + c] = (int)cl.size();
+        for (int q = 0; q < x.leaf_n[l]; q++) {
+          const int gs = lvl_leaves[(size_t)x.leaf_off[l] + q];
+          if (colour[(size_t)x.abase[l] + (gs - x.off[l])] == c) cl.push_back(gs);
+        }
+        x.lcol_n[2 * l + c] = (int)cl.size() - x.lcol_off[2 * l + c];
+      }
+    TRY(upload(&x.col_list, cl));
+  }
   TRY(upload(&x.box, box));
   TRY(upload(&x.fops, fops));
   TRY(upload(&x.hf, hf));
@@ -1160,6 +1185,31 @@ Segs level_or_leaf(const bmg_tree* t, int l) {
   return S;
 }
 
+// the same cells of one colour only (lists built at compilation)
+Segs level_or_leaf_colour(const bmg_tree* t, int l, int c) {
+  const DevIndex& x = t->ix;
+  Segs S;
+  memset(&S, 0, sizeof S);
+  int blocks = 0;
+  for (int m = l; m >= 0; m--) {
+    int s = S.n++;
+    S.first_block[s] = blocks;
+    S.abase[s] = x.abase[m];
+    S.lvoff[s] = x.off[m];
+    S.D[s] = t->L0 * (1. / (1 << m));
+    if (m == l) {
+      S.count[s] = x.col_n[2 * m + c];
+      S.list[s] = x.col_list + x.col_off[2 * m + c];
+    } else {
+      S.count[s] = x.lcol_n[2 * m + c];
+      S.list[s] = x.col_list + x.lcol_off[2 * m + c];
+    }
+    blocks += (int)nb_(S.count[s]);
+  }
+  S.first_block[S.n] = blocks;
+  return S;
+}
+
 // segments: the leaves of every level
 Segs all_leaves(const bmg_tree* t) {
   const DevIndex& x = t->ix;
@@ -1240,9 +1290,12 @@ int relax(bmg_tree* t, int l) {
     k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, F<T>(t->tmp), res, x.nbr, x.colour, k, -1);
     k_t_jacobi<T><<<nbk, kB, 0, t->stream>>>(S, da, F<T>(t->tmp));
   } else if (t->smoother == BMG_RELAX_RBGS) {
-    k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, res, x.nbr, x.colour, k, 0);
-    for (int m = 0; m <= l; m++) box<T>(t, da, BK_HOM, m);
-    k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, res, x.nbr, x.colour, k, 1);
+    for (int c = 0; c < 2; c++) {
+      if (c) for (int m = 0; m <= l; m++) box<T>(t, da, BK_HOM, m);
+      Segs SC = level_or_leaf_colour(t, l, c);
+      unsigned nbc = (unsigned)SC.first_block[SC.n];
+      if (nbc) k_t_relax<T, DIM><<<nbc, kB, 0, t->stream>>>(SC, da, nullptr, res, x.nbr, x.colour, k, 2);
+    }
   } else
     return bmg::fail(BMG_ERR_UNSUPPORTED, "the in-place Gauss-Seidel smoother is sequential; use JACOBI or RBGS");
   return BMG_OK;
@@ -1361,10 +1414,15 @@ struct Batcher {
       k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, A.fld[2], A.res, x.nbr, x.colour, A.k, -1);
       k_t_jacobi<T><<<nbk, kB, 0, t->stream>>>(S, da, A.fld[2]);
     } else {
-      k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, A.res, x.nbr, x.colour, A.k, 0);
-      for (int m = 0; m <= l; m++) box(0, BK_HOM, m);
-      flush();
-      k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, nullptr, A.res, x.nbr, x.colour, A.k, 1);
+      for (int c = 0; c < 2; c++) {
+        if (c) {
+          for (int m = 0; m <= l; m++) box(0, BK_HOM, m);
+          flush();
+        }
+        Segs SC = level_or_leaf_colour(t, l, c);
+        unsigned nbc = (unsigned)SC.first_block[SC.n];
+        if (nbc) k_t_relax<T, DIM><<<nbc, kB, 0, t->stream>>>(SC, da, nullptr, A.res, x.nbr, x.colour, A.k, 2);
+      }
     }
     return BMG_OK;
   }

@@ -203,7 +203,8 @@ __device__ __forceinline__ T t_relax_value(const T* __restrict__ da, const T* __
 
 // relaxation of the cells of foreach_level_or_leaf(l).  colour < 0: all
 // cells, value written to out (Jacobi's c); colour 0/1: in place, cells of
-// that colour only
+// that colour only; 2: in place, every cell of the segments (lists of one
+// colour)
 template <typename T, int DIM>
 __global__ __launch_bounds__(kB) void k_t_relax(Segs S, T* __restrict__ da, T* __restrict__ out,
                                                 const T* __restrict__ res, const int* __restrict__ nbr,
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(kB) void k_t_relax(Segs S, T* __restrict__ da, T* _
   int gs = seg_slot(S, s);
   if (gs < 0) return;
   int ai = S.abase[s] + (gs - S.lvoff[s]);
-  if (sel >= 0 && colour[ai] != sel) return;
+  if ((sel == 0 || sel == 1) && colour[ai] != sel) return;
   T D = (T)S.D[s];
   T v = t_relax_value<T, DIM>(da, res, nbr, k, gs, ai, D * D);
   if (sel < 0)
