@@ -141,3 +141,23 @@ def test_grid_dump_equals_oracle_and_restores(dim, depth, tmp_path):
     assert (tmp_path / "again.dump").read_bytes() == ref
     with pytest.raises(bmg.BMGError):
         bmg.Grid(dim, depth - 1).restore(path)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_root_only_tree_dump(dim, tmp_path):
+    """A tree of one leaf: one record, size 1, flags leaf."""
+    print("case dump root", dim, flush=True)
+    g = bmg.Tree(dim)
+    o = OracleTree(dim)
+    for t in (g, o):
+        t.set_a(np.array([2.5]))
+        t.set_b(np.array([-1.0]))
+        t.set_s(np.array([0.25]))
+    path = tmp_path / "root.dump"
+    g.dump(path, names={"a": "a", "b": "b", "s": "s"})
+    assert path.read_bytes() == dump.dump_bytes(o, [("a", T_A), ("b", T_B), ("s", t_s(dim))])
+    h = bmg.Tree(dim)
+    h.refine(lambda x, y, z, lev: lev < 2)
+    h.restore(path, names={"a": "a", "s": "s"})
+    assert h.nleaves() == 1 and h.depth() == 0
+    assert h.get_a()[0] == 2.5 and h.get_s()[0] == 0.25

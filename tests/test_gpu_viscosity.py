@@ -90,3 +90,25 @@ def test_viscosity_zero_dt_is_identity_and_errors():
     p.set_bc(bmg.RIGHT, bmg.BC_PERIODIC)
     with pytest.raises(bmg.BMGError):
         p.viscosity([x.copy() for x in u], mu=1.0, rho=1.0, dt=0.1)
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 1), (2, 2), (3, 1), (3, 2)])
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+def test_viscosity_tiny_grids_bitexact(dim, depth, smoother):
+    """Grids of 2 and 4 cells per side: the single-cell level 0 and pairs
+    that touch both walls."""
+    print("case viscosity tiny", dim, depth, smoother, flush=True)
+    u, mu, rho = _problem(dim, depth, 7 * dim + depth, True)
+    g = bmg.Grid(dim, depth, smoother=smoother)
+    g.set_tolerance(1e-10)
+    o = Viscosity(dim, depth)
+    o.smoother = smoother
+    o.TOLERANCE = 1e-10
+    o.set_coefficients(mu, rho)
+    uo, so, _ = o.solve(u, 0.05)
+    ug = [x.copy() for x in u]
+    sg = g.viscosity(ug, mu=mu, rho=rho, dt=0.05)
+    for k in ("i", "resb", "resa", "nrelax"):
+        assert sg[k] == so[k], (k, sg, so)
+    for c in range(dim):
+        assert np.array_equal(ug[c], uo[c]), c
