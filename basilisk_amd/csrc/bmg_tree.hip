@@ -25,7 +25,9 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -637,6 +639,21 @@ const int kFtan2[2][2][3] = {{{0, 0, 0}, {0, 1, 0}}, {{0, 0, 0}, {1, 0, 0}}};
 
 int ftan(int dim, int d, int m, int e) { return dim == 3 ? kFtan[d][m][e] : kFtan2[d][m][e]; }
 
+// f(begin, end) over [0, n) on up to 16 host threads (large n only); f
+// reads the topology through find() only, which writes nothing
+template <typename F>
+void par_for(size_t n, F&& f) {
+  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  nt = (unsigned)std::min<size_t>(nt, n / 32768);
+  if (nt <= 1) {
+    f((size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned k = 0; k < nt; k++) th.emplace_back([&, k] { f(n * k / nt, n * (k + 1) / nt); });
+  for (auto& x : th) x.join();
+}
+
 // compile the topology into slot tables and allocate the fields
 int compile(bmg_tree* t) {
   Topo& T = t->topo;
@@ -717,10 +734,11 @@ int compile(bmg_tree* t) {
     Cell* c = T.L[l].find(i, j, k);
     return c ? c->slot : -1;
   };
-  std::vector<int> bad;  // any -1 in a table is an error
-  bool ok = true;
+  // any slot out of range in a table is an error (tables are filled by
+  // several host threads)
+  std::atomic<bool> ok{true};
   auto chk = [&](int v) {
-    if (v < 0 || v >= NS) ok = false;
+    if (v < 0 || v >= NS) ok.store(false, std::memory_order_relaxed);
     return v;
   };
   // active neighbours and colours
@@ -728,49 +746,65 @@ int compile(bmg_tree* t) {
   std::vector<int> nbr((size_t)NA * 2 * dim);
   std::vector<uint8_t> colour(NA);
   for (int l = 0; l <= D; l++)
-    for (int q = 0; q < x.nA[l]; q++) {
-      IJK c = T.active[l][q];
-      size_t ai = (size_t)x.abase[l] + q;
-      for (int d = 0; d < dim; d++) {
-        int o[3] = {0, 0, 0};
-        o[d] = 1;
-        nbr[ai * 2 * dim + 2 * d] = chk(S(l, c.i - o[0], c.j - o[1], c.k - o[2]));
-        nbr[ai * 2 * dim + 2 * d + 1] = chk(S(l, c.i + o[0], c.j + o[1], c.k + o[2]));
+    par_for((size_t)x.nA[l], [&](size_t q0, size_t q1) {
+      for (size_t q = q0; q < q1; q++) {
+        IJK c = T.active[l][q];
+        size_t ai = (size_t)x.abase[l] + q;
+        for (int d = 0; d < dim; d++) {
+          int o[3] = {0, 0, 0};
+          o[d] = 1;
+          nbr[ai * 2 * dim + 2 * d] = chk(S(l, c.i - o[0], c.j - o[1], c.k - o[2]));
+          nbr[ai * 2 * dim + 2 * d + 1] = chk(S(l, c.i + o[0], c.j + o[1], c.k + o[2]));
+        }
+        colour[ai] = (uint8_t)((c.i + c.j + c.k) & 1);
       }
-      colour[ai] = (uint8_t)((c.i + c.j + c.k) & 1);
-    }
+    });
   if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: active cell without neighbours");
+  // the non-leaf active cells of each level (indices into active[l])
+  std::vector<std::vector<int>> nonleaf(D + 1);
+  for (int l = 0; l <= D; l++)
+    for (int q = 0; q < x.nA[l]; q++) {
+      const IJK& c = T.active[l][q];
+      if (!T.is_leaf(*T.L[l].find(c.i, c.j, c.k))) nonleaf[l].push_back(q);
+    }
   // parent records: non-leaf active cells, then prolongation leaves
-  std::vector<int> rself, rchild, rring;
+  const int NR = dim == 3 ? 27 : 9;
   x.rec_off.assign(D + 1, 0);
   x.n_nonleaf.assign(D + 1, 0);
   x.n_prolong.assign(D + 1, 0);
-  auto add_rec = [&](int l, IJK c) {
-    rself.push_back(chk(S(l, c.i, c.j, c.k)));
+  size_t nrec = 0;
+  for (int l = 0; l <= D; l++) {
+    x.rec_off[l] = (int)nrec;
+    x.n_nonleaf[l] = (int)nonleaf[l].size();
+    x.n_prolong[l] = (int)T.prolong[l].size();
+    nrec += (size_t)x.n_nonleaf[l] + x.n_prolong[l];
+  }
+  std::vector<int> rself(nrec), rchild(nrec), rring(nrec * NR);
+  auto add_rec = [&](size_t r, int l, IJK c) {
+    rself[r] = chk(S(l, c.i, c.j, c.k));
     int c0 = chk(S(l + 1, 2 * c.i - GH, 2 * c.j - GH, dim == 3 ? 2 * c.k - GH : 0));
-    rchild.push_back(c0);
+    rchild[r] = c0;
     // the kernels address the children as child0 + (foreach_child index)
     int m = 0;
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++)
         for (int e = 0; e < T.kext(); e++, m++)
-          if (S(l + 1, 2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0) != c0 + m) ok = false;
+          if (S(l + 1, 2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0) != c0 + m)
+            ok.store(false, std::memory_order_relaxed);
     int e0 = dim == 3 ? -1 : 0, e1 = dim == 3 ? 1 : 0;
+    size_t w = r * NR;
     for (int a = -1; a <= 1; a++)
       for (int b = -1; b <= 1; b++)
-        for (int e = e0; e <= e1; e++) rring.push_back(chk(S(l, c.i + a, c.j + b, c.k + e)));
+        for (int e = e0; e <= e1; e++) rring[w++] = chk(S(l, c.i + a, c.j + b, c.k + e));
   };
   for (int l = 0; l <= D; l++) {
-    x.rec_off[l] = (int)rself.size();
-    for (auto& c : T.active[l])
-      if (!T.is_leaf(T.at(l, c.i, c.j, c.k))) {
-        add_rec(l, c);
-        x.n_nonleaf[l]++;
-      }
-    for (auto& c : T.prolong[l]) {
-      add_rec(l, c);
-      x.n_prolong[l]++;
-    }
+    const size_t r0 = x.rec_off[l], np = x.n_nonleaf[l];
+    par_for(np, [&](size_t k0, size_t k1) {
+      for (size_t k = k0; k < k1; k++) add_rec(r0 + k, l, T.active[l][nonleaf[l][k]]);
+    });
+    par_for(T.prolong[l].size(), [&](size_t k0, size_t k1) {
+      for (size_t k = k0; k < k1; k++) add_rec(r0 + np + k, l, T.prolong[l][k]);
+    });
   }
   if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: parent record with an unallocated ring");
   // leaves: global cache order and per level
@@ -868,31 +902,47 @@ int compile(bmg_tree* t) {
     }
     x.box_n[l] = (int)box.size() - x.box_off[l];
   }
-  // residual face ops per level, and the face-cache value map
-  std::vector<std::vector<FaceOp>> fl(D + 1);
-  std::vector<int> fv_gs, fv_d;
-  for (auto& f : T.faces) {
-    int l = f.level;
-    const int fc[3] = {f.c.i, f.c.j, f.c.k};
-    for (int d = 0; d < dim; d++)
-      if (f.flags & (1 << d)) {
-        int m[3] = {fc[0], fc[1], fc[2]};
-        m[d]--;
-        FaceOp op = {chk(S(l, fc[0], fc[1], fc[2])), chk(S(l, m[0], m[1], m[2])), d};
-        fl[l].push_back(op);
-        fv_gs.push_back(op.gs);
-        fv_d.push_back(d);
-      }
-  }
-  x.nfaceval = (long)fv_gs.size();
-  std::vector<FaceOp> fops;
+  // residual face ops per level, and the face-cache value map (entry order
+  // then dimension); the positions come from prefix counts of the flag bits
+  const size_t NF = T.faces.size();
+  std::vector<size_t> gpos(NF), lpos(NF);
   x.face_off.assign(D + 1, 0);
   x.face_n.assign(D + 1, 0);
-  for (int l = 0; l <= D; l++) {
-    x.face_off[l] = (int)fops.size();
-    x.face_n[l] = (int)fl[l].size();
-    fops.insert(fops.end(), fl[l].begin(), fl[l].end());
+  {
+    size_t g = 0;
+    for (size_t e = 0; e < NF; e++) {
+      gpos[e] = g;
+      const int nb = __builtin_popcount((unsigned)T.faces[e].flags & ((1u << dim) - 1));
+      g += nb;
+      x.face_n[T.faces[e].level] += nb;
+    }
+    x.nfaceval = (long)g;
+    for (int l = 1; l <= D; l++) x.face_off[l] = x.face_off[l - 1] + x.face_n[l - 1];
+    std::vector<size_t> cur(x.face_off.begin(), x.face_off.end());
+    for (size_t e = 0; e < NF; e++) {
+      lpos[e] = cur[T.faces[e].level];
+      cur[T.faces[e].level] += __builtin_popcount((unsigned)T.faces[e].flags & ((1u << dim) - 1));
+    }
   }
+  std::vector<int> fv_gs(x.nfaceval), fv_d(x.nfaceval);
+  std::vector<FaceOp> fops(x.nfaceval);
+  par_for(NF, [&](size_t e0, size_t e1) {
+    for (size_t e = e0; e < e1; e++) {
+      const Face& f = T.faces[e];
+      int l = f.level;
+      const int fc[3] = {f.c.i, f.c.j, f.c.k};
+      size_t g = gpos[e], lp = lpos[e];
+      for (int d = 0; d < dim; d++)
+        if (f.flags & (1 << d)) {
+          int m[3] = {fc[0], fc[1], fc[2]};
+          m[d]--;
+          FaceOp op = {chk(S(l, fc[0], fc[1], fc[2])), chk(S(l, m[0], m[1], m[2])), d};
+          fops[lp++] = op;
+          fv_gs[g] = op.gs;
+          fv_d[g++] = d;
+        }
+    }
+  });
   // fine-face averages: halo_face (all levels in one launch) and the face
   // restriction of alpha (per level)
   auto fine_avg = [&](int l, const int c[3], int d, int hi, int target) {
@@ -920,30 +970,43 @@ int compile(bmg_tree* t) {
       }
     }
   x.nhf = (int)hf.size();
-  std::vector<FaceAvg> fr;
   x.fr_off.assign(D + 1, 0);
   x.fr_n.assign(D + 1, 0);
+  size_t nfr = 0;
   for (int l = 0; l <= D; l++) {
-    x.fr_off[l] = (int)fr.size();
-    for (auto& p : T.active[l]) {
-      if (T.is_leaf(T.at(l, p.i, p.j, p.k))) continue;
-      const int c[3] = {p.i, p.j, p.k};
-      for (int d = 0; d < dim; d++) {
-        int q[3] = {c[0], c[1], c[2]};
-        q[d]++;
-        fr.push_back(fine_avg(l, c, d, 0, chk(S(l, c[0], c[1], c[2]))));
-        fr.push_back(fine_avg(l, c, d, 1, chk(S(l, q[0], q[1], q[2]))));
-      }
-    }
-    x.fr_n[l] = (int)fr.size() - x.fr_off[l];
+    x.fr_off[l] = (int)nfr;
+    x.fr_n[l] = (int)nonleaf[l].size() * 2 * dim;
+    nfr += x.fr_n[l];
   }
+  std::vector<FaceAvg> fr(nfr);
+  for (int l = 0; l <= D; l++)
+    par_for(nonleaf[l].size(), [&](size_t k0, size_t k1) {
+      for (size_t k = k0; k < k1; k++) {
+        const IJK& p = T.active[l][nonleaf[l][k]];
+        const int c[3] = {p.i, p.j, p.k};
+        size_t w = x.fr_off[l] + k * 2 * dim;
+        for (int d = 0; d < dim; d++) {
+          int q[3] = {c[0], c[1], c[2]};
+          q[d]++;
+          fr[w++] = fine_avg(l, c, d, 0, chk(S(l, c[0], c[1], c[2])));
+          fr[w++] = fine_avg(l, c, d, 1, chk(S(l, q[0], q[1], q[2])));
+        }
+      }
+    });
   // refine_face_x of the prolongation leaves
-  std::vector<FaceProlong> fp;
   x.fp_off.assign(D + 1, 0);
   x.fp_n.assign(D + 1, 0);
+  size_t nfp = 0;
   for (int l = 0; l < D; l++) {
-    x.fp_off[l] = (int)fp.size();
-    for (auto& p : T.prolong[l]) {
+    x.fp_off[l] = (int)nfp;
+    x.fp_n[l] = (int)T.prolong[l].size() * dim;
+    nfp += x.fp_n[l];
+  }
+  std::vector<FaceProlong> fp(nfp);
+  for (int l = 0; l < D; l++)
+    par_for(T.prolong[l].size(), [&](size_t k0, size_t k1) {
+    for (size_t kp = k0; kp < k1; kp++) {
+      const IJK& p = T.prolong[l][kp];
       const int c[3] = {p.i, p.j, p.k};
       for (int d = 0; d < dim; d++) {
         int t1 = (d + 1) % dim, t2 = (d + 2) % dim;
@@ -960,9 +1023,10 @@ int compile(bmg_tree* t) {
         int m[3] = {c[0], c[1], c[2]}, q[3] = {c[0], c[1], c[2]};
         m[d]--;
         q[d]++;
-        Cell& cm = T.at(l, m[0], m[1], m[2]);
-        Cell& cq = T.at(l, q[0], q[1], q[2]);
-        r.flags = (!T.is_refined(cm) ? 1 : 0) | ((!T.is_refined(cq) && cq.nb) ? 2 : 0);
+        const Cell* cm = T.L[l].find(m[0], m[1], m[2]);
+        const Cell* cq = T.L[l].find(q[0], q[1], q[2]);
+        const bool rm = cm && T.is_refined(*cm), rq = cq && T.is_refined(*cq);
+        r.flags = (!rm ? 1 : 0) | ((!rq && cq && cq->nb) ? 2 : 0);
         r.self = sl(0, 0, 0);
         r.t1m = sl(0, -1, 0);
         r.t1p = sl(0, 1, 0);
@@ -985,11 +1049,10 @@ int compile(bmg_tree* t) {
               r.out[o][dim == 3 ? j * 2 + k : j] = chk(S(l + 1, f[0], f[1], f[2]));
             }
         }
-        fp.push_back(r);
+        fp[(size_t)x.fp_off[l] + kp * dim + d] = r;
       }
     }
-    x.fp_n[l] = (int)fp.size() - x.fp_off[l];
-  }
+    });
   // ghost tangential faces: sequential semantics resolved to pre-pass sources
   std::vector<FaceCopy> fc;
   x.fc_off.assign(D + 1, 0);
