@@ -90,3 +90,40 @@ def test_project_device_tensors_and_mixed():
     sh = g.project(uf, dt=0.25)
     assert sh == s64
     assert all(np.array_equal(a, b) for a, b in zip(uf, u64))
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 7), (3, 5)])
+def test_project_time_steps_with_nrelax_feedback(dim, depth):
+    """The projection step of navier-stokes/centered.h over several time
+    steps: each step adds a seeded perturbation to the face velocity and
+    calls project() with the previous step's mgstats.nrelax (and the previous
+    pressure as the initial guess), as `mgp = project (uf, p, alpha, dt,
+    mgp.nrelax)` does; every step equals the oracle's bit for bit."""
+    print("case project steps", dim, depth, flush=True)
+    g = bmg.Grid(dim, depth, smoother="rbgs")
+    o = OracleGrid(dim, depth)
+    o.set_smoother("rbgs")
+    ug = _setup(g, dim, "symmetry", 31, True)
+    uo = _setup(o, dim, "symmetry", 31, True)
+    for grid in (g, o):
+        grid.set_tolerance(1e-7)
+    rng = np.random.default_rng(77)
+    ng = no = 0
+    for step in range(4):
+        for c in range(dim):
+            pert = 0.1 * rng.standard_normal(ug[c].shape)
+            idx = [slice(None)] * dim
+            idx[c] = 0
+            pert[tuple(idx)] = 0.0
+            idx[c] = -1
+            pert[tuple(idx)] = 0.0
+            ug[c] += pert
+            uo[c] += pert
+        sg = g.project(ug, dt=0.05, nrelax=ng)
+        so = o.project(uo, dt=0.05, nrelax=no)
+        for k in ("i", "resb", "resa", "nrelax"):
+            assert sg[k] == so[k], (step, k, sg, so)
+        ng, no = sg["nrelax"], so["nrelax"]
+        assert np.array_equal(g.get_a(), o.get_a())
+        for c in range(dim):
+            assert np.array_equal(ug[c], uo[c]), (step, c)
