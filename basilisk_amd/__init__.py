@@ -48,7 +48,7 @@ EXPORTS = [
     "bmg_tree_set_alpha_faces", "bmg_tree_set_alpha_const", "bmg_tree_set_lambda_const",
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
-    "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore",
+    "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
     "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
 ]
 
@@ -147,6 +147,8 @@ def library():
         "bmg_tree_synchronize": (I, [P]),
         "bmg_tree_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "bmg_tree_dump": (I, [P, ctypes.c_char_p, dp, dp, I, D, I]),
+        "bmg_tree_nonleaf": (I, [P, ctypes.POINTER(ctypes.c_long), dp, dp]),
+        "bmg_tree_unrefine": (I, [P, dp, ctypes.POINTER(ctypes.c_long)]),
         "bmg_tree_restore": (I, [P, ctypes.c_char_p, dp, dp, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
         "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
         "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
@@ -739,6 +741,27 @@ class Tree:
 
     def get_s(self, out=None):
         return self._get(F_S, out)
+
+    def nonleaf(self):
+        """Levels and centres of the non-leaf cells foreach_cell() visits."""
+        n = ctypes.c_long()
+        _check(self.lib.bmg_tree_nonleaf(self._t, ctypes.byref(n), None, None))
+        lev = np.zeros(n.value, np.int32)
+        xyz = np.zeros((n.value, 3))
+        if n.value:
+            _check(self.lib.bmg_tree_nonleaf(self._t, ctypes.byref(n), _host_ptr(lev), _host_ptr(xyz)))
+        return lev, xyz
+
+    def unrefine(self, cond):
+        """unrefine (cond) (grid/tree-common.h:348-371): cond(x, y, z, level)
+        on the non-leaf cells; returns the number of coarsened cells."""
+        lev, xyz = self.nonleaf()
+        m = np.ascontiguousarray(cond(xyz[:, 0], xyz[:, 1], xyz[:, 2], lev), dtype=np.uint8)
+        if m.size == 0:
+            m = np.zeros(1, np.uint8)
+        nc = ctypes.c_long()
+        _check(self.lib.bmg_tree_unrefine(self._t, _host_ptr(m), ctypes.byref(nc)))
+        return nc.value
 
     @staticmethod
     def _dump_fields(names):

@@ -403,6 +403,41 @@ struct Topo {
             adapt_coarsen(m + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0}, l, A);
   }
 
+  // the non-leaf cells foreach_cell() visits (pre-order, not descending
+  // into leaves)
+  void nonleaf(int l, IJK c, std::vector<int>& lev, std::vector<IJK>& cells) {
+    Cell* cp = L[l].find(c.i, c.j, c.k);
+    if (!cp || (cp->flags & C_LEAF)) return;
+    lev.push_back(l);
+    cells.push_back(c);
+    if (l < depth)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          for (int e = 0; e < kext(); e++)
+            nonleaf(l + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0}, lev, cells);
+  }
+
+  // one level l of unrefine()'s coarsening passes (grid/tree-common.h:356-369)
+  void unrefine_pass(int m, IJK c, int l, long& nc, bool& ok) {
+    if (m > depth) return;
+    Cell* cp = L[m].find(c.i, c.j, c.k);
+    if (!cp || (cp->flags & C_LEAF)) return;
+    if (m == l) {
+      if (cp->flags & C_TOO_FINE) {
+        int r = coarsen_cell(m, c);
+        if (r < 0) ok = false;
+        if (r > 0) nc++;
+        cp->flags &= (uint8_t)~C_TOO_FINE;
+      }
+      return;
+    }
+    if (m < depth)
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          for (int e = 0; e < kext(); e++)
+            unrefine_pass(m + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0}, l, nc, ok);
+  }
+
   bool is_boundary(const Cell& c) const { return c.pid < 0; }
   bool is_leaf(const Cell& c) const { return c.flags & C_LEAF; }
   bool is_refined(const Cell& c) const { return !(c.flags & C_LEAF) && c.nb && c.pid >= 0; }
@@ -2080,6 +2115,72 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
   }
   if (nf) *nf = (int)A.nf;
   if (nc) *nc = (int)A.nc;
+  return BMG_OK;
+}
+
+int bmg_tree_nonleaf(bmg_tree* t, long* n, int* level, double* xyz) {
+  if (!t || !n) return bmg::fail(BMG_ERR_ARG, "null argument");
+  Topo& T = t->topo;
+  std::vector<int> lev;
+  std::vector<IJK> cells;
+  T.nonleaf(0, {GH, GH, t->dim == 3 ? GH : 0}, lev, cells);
+  *n = (long)cells.size();
+  for (size_t q = 0; q < cells.size(); q++) {
+    const double D = t->L0 * (1. / (1 << lev[q]));
+    if (level) level[q] = lev[q];
+    if (xyz) {
+      xyz[3 * q] = (cells[q].i - GH + 0.5) * D + t->X0[0];
+      xyz[3 * q + 1] = (cells[q].j - GH + 0.5) * D + t->X0[1];
+      xyz[3 * q + 2] = t->dim == 3 ? (cells[q].k - GH + 0.5) * D + t->X0[2] : 0.;
+    }
+  }
+  return BMG_OK;
+}
+
+int bmg_tree_unrefine(bmg_tree* t, const unsigned char* mark, long* ncoarsened) {
+  if (!t || !mark) return bmg::fail(BMG_ERR_ARG, "null argument");
+  TRY(ensure(t));
+  Topo& T = t->topo;
+  std::vector<int> lev;
+  std::vector<IJK> cells;
+  const IJK root = {GH, GH, t->dim == 3 ? GH : 0};
+  T.nonleaf(0, root, lev, cells);
+  // the fields hold their restriction where coarsen_cell reads it
+  if (t->dim == 2)
+    adapt_boundary<2>(t);
+  else
+    adapt_boundary<3>(t);
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  for (size_t q = 0; q < cells.size(); q++)
+    if (mark[q]) T.at(lev[q], cells[q].i, cells[q].j, cells[q].k).flags |= C_TOO_FINE;
+  long nc = 0;
+  bool ok = true;
+  for (int l = T.depth; l >= 0; l--) T.unrefine_pass(0, root, l, nc, ok);
+  if (!ok) {
+    t->compiled = false;
+    return bmg::fail(BMG_ERR_ARG, "unrefine met an inconsistent tree");
+  }
+  if (nc) {
+    T.dirty = true;
+    double* olds[3] = {t->s, t->a, t->b};
+    const long old_nslot = t->ix.nslot;
+    t->s = t->a = t->b = nullptr;
+    t->keep_a.clear();
+    t->keep_b.clear();
+    t->keep_lam.clear();
+    t->keep_s.clear();
+    std::vector<int> prev;
+    t->prev_slots = &prev;
+    int rc = compile(t);
+    t->prev_slots = nullptr;
+    RefineLog none;
+    if (rc == BMG_OK)
+      rc = t->dim == 2 ? adapt_remap<2>(t, olds, old_nslot, prev, none) : adapt_remap<3>(t, olds, old_nslot, prev, none);
+    for (double* p : olds) dfree_(p);
+    if (rc != BMG_OK) return rc;
+    t->coef_ready = false;
+  }
+  if (ncoarsened) *ncoarsened = nc;
   return BMG_OK;
 }
 

@@ -280,6 +280,15 @@ int bmg_tree_refine(bmg_tree *t, const unsigned char *mark, long *nrefined);
    read zero until set again.  nf and nc (may be NULL) receive the numbers of
    refined and coarsened cells (astats). */
 int bmg_tree_adapt_wavelet(bmg_tree *t, double max, int maxlevel, int minlevel, int *nf, int *nc);
+/* the non-leaf cells that foreach_cell() visits, in its pre-order: level
+   and centre (3 doubles) each; level/xyz may be NULL to get the count */
+int bmg_tree_nonleaf(bmg_tree *t, long *n, int *level, double *xyz);
+/* unrefine (cond) (grid/tree-common.h:348-371) with cond given as one mark
+   per cell of bmg_tree_nonleaf's list: the marked cells are coarsened from
+   the depth up (coarsen_cell, which the 2:1 rule may refuse); s, a and b
+   take the children's average; alpha and a lambda field must be set again.
+   ncoarsened (may be NULL) receives the number of coarsened cells. */
+int bmg_tree_unrefine(bmg_tree *t, const unsigned char *mark, long *ncoarsened);
 /* dump() (output.h:1043-1118, the non-MPI branch, file version 170901) of
    the leaf fields `fields` (BMG_F_A, BMG_F_B, BMG_F_S) under `names`: the
    header (struct DumpHeader: t, the field count with the subtree "size"

@@ -60,6 +60,7 @@ def _lib(dim):
             "ot_boundary": (None, [P, I, I, I]),
             "ot_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I)]),
             "ot_refine_cell": (I, [P, I, I, I, I]),
+            "ot_nonleaf": (L, [P, vp, vp]), "ot_unrefine": (I, [P, vp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(lib, name)
@@ -287,3 +288,19 @@ class OracleTree:
     def refine_cell_at(self, level, c):
         """refine_cell of the leaf (level, i, j, k) with the 2:1 cascade."""
         return self.lib.ot_refine_cell(self.t, int(level), int(c[0]), int(c[1]), int(c[2]) if self.dim == 3 else 0)
+
+    def nonleaf(self):
+        """Levels and centres of the non-leaf cells foreach_cell() visits."""
+        n = self.lib.ot_nonleaf(self.t, None, None)
+        lev = np.zeros(n, np.int32)
+        xyz = np.zeros((n, 3))
+        if n:
+            self.lib.ot_nonleaf(self.t, _ptr(lev), _ptr(xyz))
+        return lev, xyz
+
+    def unrefine(self, cond):
+        """unrefine (cond), grid/tree-common.h:348-371: cond(x, y, z, level)
+        on the non-leaf cells; returns the number of coarsened cells."""
+        lev, xyz = self.nonleaf()
+        mark = np.ascontiguousarray(cond(xyz[:, 0], xyz[:, 1], xyz[:, 2], lev), dtype=np.uint8)
+        return self.lib.ot_unrefine(self.t, _ptr(mark))

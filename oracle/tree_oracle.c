@@ -1322,3 +1322,94 @@ int ot_refine_cell(ot_tree *t, int l, int i, int j, int k) {
   t->dirty = 1;
   return 1;
 }
+
+/* ---------------------------------------------------------- unrefine() */
+
+/* the non-leaf cells foreach_cell() visits (pre-order, not descending into
+   leaves): level and centre; returns their number (arrays may be NULL) */
+static long nonleaf_visit(ot_tree *t, int l, const int c[3], int *level, double *xyz, long n) {
+  otl *L = &t->L[l];
+  size_t q = lidx(L, c[0], c[1], c[2]);
+  if (!L->alloc[q] || (L->flags[q] & C_LEAF)) return n;
+  if (level) level[n] = l;
+  if (xyz) {
+    double D = delta_l(t, l);
+    xyz[3 * n] = (c[0] - GH + 0.5) * D + t->X0;
+    xyz[3 * n + 1] = (c[1] - GH + 0.5) * D + t->Y0;
+    xyz[3 * n + 2] = DIM == 3 ? (c[2] - GH + 0.5) * D + t->Z0 : 0.;
+  }
+  n++;
+  if (l < t->depth)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+          int cc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
+          n = nonleaf_visit(t, l + 1, cc, level, xyz, n);
+        }
+  return n;
+}
+
+long ot_nonleaf(ot_tree *t, int *level, double *xyz) {
+  const int root[3] = {GH, GH, DIM == 3 ? GH : 0};
+  return nonleaf_visit(t, 0, root, level, xyz, 0);
+}
+
+static long unrefine_mark(ot_tree *t, int l, const int c[3], const unsigned char *mark, long n) {
+  otl *L = &t->L[l];
+  size_t q = lidx(L, c[0], c[1], c[2]);
+  if (!L->alloc[q] || (L->flags[q] & C_LEAF)) return n;
+  if (mark[n++]) L->flags[q] |= C_TOO_FINE;
+  if (l < t->depth)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+          int cc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
+          n = unrefine_mark(t, l + 1, cc, mark, n);
+        }
+  return n;
+}
+
+static void unrefine_pass(ot_tree *t, int m, const int c[3], int l, int *nc) {
+  if (m > t->depth) return;
+  otl *L = &t->L[m];
+  size_t q = lidx(L, c[0], c[1], c[2]);
+  if (!L->alloc[q] || (L->flags[q] & C_LEAF)) return;
+  if (m == l) {
+    if (L->flags[q] & C_TOO_FINE) {
+      if (coarsen_cell(t, m, c, adapt_list, 3)) (*nc)++;
+      L = &t->L[m];
+      L->flags[q] &= ~C_TOO_FINE;
+    }
+    return;
+  }
+  if (m < t->depth)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+          int cc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
+          unrefine_pass(t, m + 1, cc, l, nc);
+        }
+}
+
+/* unrefine (cond), grid/tree-common.h:348-371, with cond given as one mark
+   per cell of ot_nonleaf's list: marked cells are coarsened from the depth
+   up (coarsen_cell, which the 2:1 rule may refuse), s, a and b taking the
+   children's average; then the tree boundary of the three fields.  Returns
+   the number of coarsened cells. */
+int ot_unrefine(ot_tree *t, const unsigned char *mark) {
+  ot_update(t);
+  /* the fields hold their restriction where coarsen_cell reads it */
+  if (is_full(t)) {
+    for (int m = 0; m < 3; m++)
+      for (int l = t->depth - 1; l >= 0; l--) restrict_level(t, adapt_list[m], l);
+  } else
+    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+  const int root[3] = {GH, GH, DIM == 3 ? GH : 0};
+  unrefine_mark(t, 0, root, mark, 0);
+  int nc = 0;
+  for (int l = t->depth; l >= 0; l--) unrefine_pass(t, 0, root, l, &nc);
+  t->dirty = 1;
+  if (nc)
+    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+  return nc;
+}

@@ -146,3 +146,39 @@ def test_linear_field_carried_exactly(dim):
         assert ok.sum() > 100
         np.testing.assert_allclose(a[ok], exact[ok], rtol=0, atol=1e-12)
     assert totf > 0 and totc > 0
+
+
+def test_coarsen_ref():
+    """test/coarsen.c: init_grid (64), refine (level < 8), unrefine (level >
+    5), tree_check (): coarsen.ref's last line is `depth: 6 leaves: 4096`
+    (its `mem` figures are the reference's allocator and not compared)."""
+    t = OracleTree(2, 1.0, (0.0, 0.0, 0.0))
+    t.refine(lambda x, y, z, lev: lev < 6)
+    assert t.depth() == 6
+    t.refine(lambda x, y, z, lev: lev < 8)
+    assert t.depth() == 8
+    nc = t.unrefine(lambda x, y, z, lev: lev > 5)
+    tree_check(t)
+    with open(os.path.join(HERE, "golden", "ref", "coarsen.ref")) as f:
+        last = [ln for ln in f.read().split("\n") if ln][-1]
+    assert last.startswith("depth: %d leaves: %d " % (t.depth(), t.nleaves()))
+    assert nc == 4 ** 6 + 4 ** 7
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_unrefine_region_keeps_tree_consistent(dim):
+    """unrefine of a region of an adaptive tree: the 2:1 rule refuses some
+    cells, the tree stays consistent and a linear field stays exact."""
+    t = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    t.refine(lambda x, y, z, lev: (lev < 3) | ((lev < (7 if dim == 2 else 5)) & (x < 0.6)))
+    _, xyz = t.leaves()
+    coef = np.array([1.5, -2.0, 0.75])[:dim]
+    t.set_a(0.5 + xyz[:, :dim] @ coef)
+    nc = t.unrefine(lambda x, y, z, lev: (x > 0.3) & (lev > 2))
+    assert nc > 0
+    tree_check(t)
+    lev, xyz = t.leaves()
+    a = t.get_a()
+    h = 0.5 ** lev.astype(float)
+    ok = np.all((xyz[:, :dim] > 4 * h[:, None]) & (xyz[:, :dim] < 1 - 4 * h[:, None]), axis=1)
+    np.testing.assert_allclose(a[ok], 0.5 + xyz[ok][:, :dim] @ coef, rtol=0, atol=1e-12)

@@ -154,3 +154,51 @@ def test_adapt_errors():
     nf, nc = g.adapt_wavelet(1e-2, 5, 2)
     assert nf == 0 and nc > 0
     assert g.depth() == 2 and g.nleaves() == 16
+
+
+def test_coarsen_ref_gpu():
+    """test/coarsen.c on the GPU tree (init_grid (64), refine (level < 8),
+    unrefine (level > 5)): depth 6 and 4096 leaves as in coarsen.ref, and
+    the oracle's caches."""
+    print("case coarsen.ref", flush=True)
+    g = bmg.Tree(2, 1.0, (0.0, 0.0, 0.0))
+    o = OracleTree(2, 1.0, (0.0, 0.0, 0.0))
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: lev < 8)
+        _, xyz = t.leaves()
+        t.set_a(xyz[:, 0] - xyz[:, 1])
+        t.set_s(np.zeros(len(xyz)))
+    ng = g.unrefine(lambda x, y, z, lev: lev > 5)
+    no = o.unrefine(lambda x, y, z, lev: lev > 5)
+    assert ng == no
+    assert (g.depth(), g.nleaves()) == (6, 4096)
+    _same_tree(g, o)
+    _same_fields(g, o)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_unrefine_region_bitexact(dim):
+    print("case unrefine", dim, flush=True)
+    g = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0))
+    o = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    deep = 7 if dim == 2 else 5
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: (lev < 3) | ((lev < deep) & (x < 0.6)))
+        _, xyz = t.leaves()
+        t.set_a(np.sin(4 * xyz[:, 0]) + xyz[:, 1])
+        t.set_b(np.cos(3 * xyz[:, 1]))
+        t.set_s(xyz[:, 0] ** 2)
+    cond = lambda x, y, z, lev: (x > 0.3) & (lev > 2)
+    ng, no = g.unrefine(cond), o.unrefine(cond)
+    assert ng == no and ng > 0
+    _same_tree(g, o)
+    _same_fields(g, o)
+    # and the solver runs on the coarsened tree
+    for t in (g, o):
+        t.set_alpha_const(1.0)
+        t.set_lambda(-1.0)
+    o.set_smoother("rbgs")
+    sg, so = g.poisson(tolerance=1e-7), o.poisson(tolerance=1e-7)
+    for k in ("i", "resb", "resa", "nrelax"):
+        assert sg[k] == so[k], k
+    assert np.array_equal(g.get_a(), o.get_a())
