@@ -89,3 +89,22 @@ def test_shell_refine_equals_direct_condition(dim, depth):
     b.refine(configs.shell_refine(dim, depth, 3, c, r, 2.0))
     assert a.nleaves() == b.nleaves()
     assert np.array_equal(a.leaves()[1], b.leaves()[1])
+
+
+def test_faces_ref():
+    """test/faces.c: init_grid (4), refine (level == 2 && x*x + y*y <
+    sq(0.25)) on [-0.5, 0.5]^2, then the face centres of foreach_face(x),
+    foreach_face(y) and foreach_face(): faces.ref line for line, from the
+    library's face cache and from the oracle's."""
+    import os
+    ref_path = os.path.join(os.path.dirname(__file__), "golden", "ref", "faces.ref")
+    with open(ref_path) as f:
+        ref = [ln for ln in f.read().split("\n") if ln]
+    for t in (bmg.Tree(2, 1.0, (-0.5, -0.5)), OracleTree(2, 1.0, (-0.5, -0.5))):
+        t.refine(lambda x, y, z, lev: lev < 2)
+        t.refine(lambda x, y, z, lev: (lev == 2) & (x * x + y * y < 0.25 ** 2))
+        xyz, dd = t.face_centres()
+        line = lambda p: "%g %g" % (p[0], p[1])
+        got = ([line(p) for p, d in zip(xyz, dd) if d == 0] + [line(p) for p, d in zip(xyz, dd) if d == 1]
+               + [line(p) for p in xyz])
+        assert got == ref
