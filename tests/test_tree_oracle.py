@@ -150,3 +150,29 @@ def test_refine_order_independent():
         for l in range(a.depth() + 1):
             assert np.array_equal(a.cache(which, l), b.cache(which, l))
     assert np.array_equal(a.faces(), b.faces())
+
+
+def test_restriction_c_face_vector_invariant():
+    """test/restriction.c (assert-only): every face of a non-trivial tree set
+    to 1, then the restriction of the face vector (face_average below the
+    leaves, refine_face_x into the halos, halo_face at resolution changes):
+    every active cell of every level sees 1 on its four faces."""
+    from oracle.tree import GH, OracleTree, T_G
+    import numpy as np
+    depth = 4
+    t = OracleTree(2, 1.0, (-0.7, -0.7))
+    t.refine(lambda x, y, z, lev: (lev < depth - 2)
+             | (lev <= depth * (1.0 - np.sqrt((x - 0.1) ** 2 + (y - 0.1) ** 2))))
+    assert t.depth() > 2 and t.nleaves() < 4 ** t.depth()
+    t.set_alpha_function(lambda x, y, z, d: np.ones(len(x)))
+    t.restriction_alpha_lambda()
+    checked = 0
+    for l in range(t.depth() + 1):
+        n = (1 << l) + 2 * GH
+        ax = np.ctypeslib.as_array(t.lib.ot_level_field(t.t, l, T_G + 2), shape=(n, n))
+        ay = np.ctypeslib.as_array(t.lib.ot_level_field(t.t, l, T_G + 3), shape=(n, n))
+        for i, j, _ in t.cache("active", l):
+            assert ax[i, j] == 1.0 and ax[i + 1, j] == 1.0, (l, i, j)
+            assert ay[i, j] == 1.0 and ay[i, j + 1] == 1.0, (l, i, j)
+            checked += 1
+    assert checked > 50
