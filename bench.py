@@ -252,10 +252,19 @@ def cpu_baseline(prob, args, ncycles_target):
         o.mg_cycle(4, 1)
         o.residual()
     dt = (time.perf_counter() - t0) / args.cpu_cycles
-    return {"value": dt * ncycles_target * 1e3, "unit": "ms", "cores": threads, "kind": "port",
+    base = {"value": dt * ncycles_target * 1e3, "unit": "ms", "cores": threads, "kind": "port",
             "sample": "%d V-cycles (GS, nrelax 4) + residuals of the %d^3 problem on %d OpenMP threads, "
                       "%.3f s per cycle, scaled to the GPU FP64 run's %d cycles"
                       % (args.cpu_cycles, 1 << prob["depth"], threads, dt, ncycles_target)}
+    # the oracle's own solve to 1e-11: the reference solution the GPU
+    # solutions are measured against (relative max norm after mean removal)
+    o.set_a(0 * prob["b"])
+    t1 = time.perf_counter()
+    st = o.poisson(tolerance=1e-11)
+    ref = o.get_a()
+    oracle = {"tolerance": 1e-11, "cycles": st["i"], "resa": st["resa"],
+              "seconds": round(time.perf_counter() - t1, 2), "a": ref - ref.mean()}
+    return base, oracle
 
 
 def main():
@@ -347,7 +356,13 @@ def main():
         line["viscosity_resa"] = rv["stats"]["resa"]
     if not args.no_cpu_baseline:
         ncyc = res["fp64"]["stats"]["i"] if "fp64" in res else cycles
-        line["cpu_baseline"] = cpu_baseline(prob, args, ncyc)
+        line["cpu_baseline"], oracle = cpu_baseline(prob, args, ncyc)
+        ref = oracle.pop("a")
+        scale = float(np.abs(ref).max())
+        for prec, rr in res.items():
+            a = rr["a"] - rr["a"].mean()
+            oracle["%s_rel_maxnorm" % prec] = float(np.abs(a.reshape(ref.shape) - ref).max() / scale)
+        line["error_vs_oracle"] = oracle
     print(json.dumps(line), flush=True)
 
 
