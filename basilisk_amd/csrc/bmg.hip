@@ -763,16 +763,22 @@ int restrict_coefficients(bmg_grid* g) {
     Level& lc = g->L[l];
     Level& lf = g->L[l + 1];
     Geo<DIM> gc{lc.n, lc.log2n};
-    if (g->lambda_field)
-      hipLaunchKernelGGL((k_restrict<DIM, double>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
-                         lf.lam64, lc.lam64, gc);
-    if (g->alpha_field) {
+    // lambda, the low faces and the high walls of the level in one launch
+    CoefLevelArgs A{};
+    if (g->lambda_field) {
+      A.flam = lf.lam64;
+      A.clam = lc.lam64;
+    }
+    if (g->alpha_field)
       for (int d = 0; d < DIM; d++) {
-        hipLaunchKernelGGL((k_restrict_face_lo<DIM>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream,
-                           lf.alo64[d], lc.alo64[d], gc, d);
-        hipLaunchKernelGGL((k_restrict_face_wall<DIM>), dim3(nblocks(lc.nwall)), dim3(kBlock), 0, g->stream,
-                           lf.awall64[d], lc.awall64[d], lc.n, d);
+        A.flo[d] = lf.alo64[d];
+        A.clo[d] = lc.alo64[d];
+        A.fwall[d] = lf.awall64[d];
+        A.cwall[d] = lc.awall64[d];
       }
+    if (g->lambda_field || g->alpha_field)
+      hipLaunchKernelGGL((k_restrict_coef_level<DIM>), dim3(nblocks(lc.ncell)), dim3(kBlock), 0, g->stream, A, gc);
+    if (g->alpha_field) {
       // the periodic copy of face vectors reaches the components whose
       // companion carries the periodic BC (grid/multigrid.h:636-651,
       // grid/cartesian-common.h:903-917): x always, y only in 3D
@@ -802,17 +808,33 @@ int restrict_coefficients(bmg_grid* g) {
 int refresh_coef_copies(bmg_grid* g) {
   if (g->coarse_coef_ready) return BMG_OK;
   if (g->precision == BMG_MIXED) {
+    // every level's arrays, kConvSegs per launch
+    ConvBatch B{};
+    size_t blocks = 0;
+    auto flush = [&]() {
+      if (B.nseg) hipLaunchKernelGGL(k_convert_batch, dim3((unsigned)blocks), dim3(kBlock), 0, g->stream, B);
+      B = ConvBatch{};
+      blocks = 0;
+    };
+    auto add = [&](const double* src, void* dst, size_t n) {
+      if (!n) return;
+      if (B.nseg == kConvSegs || blocks + nblocks(n) > 0xffffffffu) flush();
+      B.src[B.nseg] = src;
+      B.dst[B.nseg] = (float*)dst;
+      B.n[B.nseg] = (unsigned)n;
+      B.first[B.nseg] = (unsigned)blocks;
+      blocks += nblocks(n);
+      B.first[++B.nseg] = (unsigned)blocks;
+    };
     for (auto& lv : g->L) {
+      if (lv.ncell > 0xffffffffu) return fail(BMG_ERR_ARG, "level of %zu cells is too large", (size_t)lv.ncell);
       for (int d = 0; d < g->dim && g->alpha_field; d++) {
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
-                           lv.alo64[d], (float*)lv.alo[d], lv.ncell);
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(lv.nwall)), dim3(kBlock), 0, g->stream,
-                           lv.awall64[d], (float*)lv.awall[d], lv.nwall);
+        add(lv.alo64[d], lv.alo[d], lv.ncell);
+        add(lv.awall64[d], lv.awall[d], lv.nwall);
       }
-      if (g->lambda_field)
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
-                           lv.lam64, (float*)lv.lam, lv.ncell);
+      if (g->lambda_field) add(lv.lam64, lv.lam, lv.ncell);
     }
+    flush();
     TRY(check_launch());
   }
   g->coarse_coef_ready = true;

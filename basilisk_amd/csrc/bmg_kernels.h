@@ -770,10 +770,8 @@ __global__ __launch_bounds__(kBlock) void k_face_update(double* __restrict__ u, 
 // with the tangential order that the reference's dimension rotation gives
 // each component
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_restrict_face_lo(const double* __restrict__ fine, double* __restrict__ coarse,
-                                                             Geo<DIM> gc, int d) {
-  idx_t idx = logical_block() * kBlock + threadIdx.x;
-  if (idx >= gc.ncell()) return;
+__device__ __forceinline__ void restrict_face_lo_at(const double* __restrict__ fine, double* __restrict__ coarse,
+                                                    const Geo<DIM>& gc, int d, idx_t idx) {
   int c[3];
   gc.decode(idx, c);
   Geo<DIM> gf{gc.n * 2, gc.log2n + 1};
@@ -795,13 +793,18 @@ __global__ __launch_bounds__(kBlock) void k_restrict_face_lo(const double* __res
   }
 }
 
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_restrict_face_lo(const double* __restrict__ fine, double* __restrict__ coarse,
+                                                             Geo<DIM> gc, int d) {
+  idx_t idx = logical_block() * kBlock + threadIdx.x;
+  if (idx >= gc.ncell()) return;
+  restrict_face_lo_at<DIM>(fine, coarse, gc, d, idx);
+}
+
 // coarse high-wall faces of component d from the fine high-wall faces
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_restrict_face_wall(const double* __restrict__ fine, double* __restrict__ coarse,
-                                                               int nc, int d) {
-  size_t idx = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  size_t nw = DIM == 3 ? (size_t)nc * nc : DIM == 2 ? (size_t)nc : 1;
-  if (idx >= nw) return;
+__device__ __forceinline__ void restrict_face_wall_at(const double* __restrict__ fine, double* __restrict__ coarse,
+                                                      int nc, int d, size_t idx) {
   int nf = 2 * nc;
   if (DIM == 1) {
     coarse[0] = fine[0];
@@ -817,6 +820,90 @@ __global__ __launch_bounds__(kBlock) void k_restrict_face_wall(const double* __r
     size_t su = d == 1 ? s1 : s0, sv = d == 1 ? s0 : s1;
     coarse[idx] = (fine[b] + fine[b + su] + fine[b + sv] + fine[b + su + sv]) / 4.;
   }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_restrict_face_wall(const double* __restrict__ fine, double* __restrict__ coarse,
+                                                               int nc, int d) {
+  size_t idx = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  size_t nw = DIM == 3 ? (size_t)nc * nc : DIM == 2 ? (size_t)nc : 1;
+  if (idx >= nw) return;
+  restrict_face_wall_at<DIM>(fine, coarse, nc, d, idx);
+}
+
+// every coefficient restriction from level l+1 to level l in one launch
+// (one thread per coarse cell; nwall <= ncell): lambda (restriction_average),
+// the low faces and the high walls of each component -- the arithmetic of
+// k_restrict, k_restrict_face_lo and k_restrict_face_wall, element for element
+struct CoefLevelArgs {
+  const double* flo[3];  // null: constant alpha
+  double* clo[3];
+  const double* fwall[3];
+  double* cwall[3];
+  const double* flam;  // null: constant lambda
+  double* clam;
+};
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_restrict_coef_level(CoefLevelArgs A, Geo<DIM> gc) {
+  const idx_t idx = logical_block() * kBlock + threadIdx.x;
+  if (idx >= gc.ncell()) return;
+  if (A.flam) {
+    int c[3];
+    gc.decode(idx, c);
+    Geo<DIM> gf{gc.n * 2, gc.log2n + 1};
+    int f[3] = {2 * c[0], 2 * c[1], 2 * c[2]};
+    idx_t b = gf.index(f);
+    double sum = 0.;
+    if (DIM == 1) {
+      sum += A.flam[b];
+      sum += A.flam[b + 1];
+    } else if (DIM == 2) {
+      idx_t sx = gf.stride(0);
+      sum += A.flam[b];
+      sum += A.flam[b + 1];
+      sum += A.flam[b + sx];
+      sum += A.flam[b + sx + 1];
+    } else {
+      idx_t sx = gf.stride(0), sy = gf.stride(1);
+      sum += A.flam[b];
+      sum += A.flam[b + 1];
+      sum += A.flam[b + sy];
+      sum += A.flam[b + sy + 1];
+      sum += A.flam[b + sx];
+      sum += A.flam[b + sx + 1];
+      sum += A.flam[b + sx + sy];
+      sum += A.flam[b + sx + sy + 1];
+    }
+    A.clam[idx] = sum / double(1 << DIM);
+  }
+  if (A.flo[0]) {
+    const size_t nw = DIM == 3 ? (size_t)gc.n * gc.n : DIM == 2 ? (size_t)gc.n : 1;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      restrict_face_lo_at<DIM>(A.flo[d], A.clo[d], gc, d, idx);
+      if (idx < nw) restrict_face_wall_at<DIM>(A.fwall[d], A.cwall[d], gc.n, d, idx);
+    }
+  }
+}
+
+// FP64 -> FP32 copies of many arrays in one launch: segment s covers blocks
+// first[s] .. first[s+1]-1, one element per thread
+constexpr int kConvSegs = 32;
+struct ConvBatch {
+  const double* src[kConvSegs];
+  float* dst[kConvSegs];
+  unsigned n[kConvSegs];
+  unsigned first[kConvSegs + 1];
+  int nseg;
+};
+
+__global__ __launch_bounds__(kBlock) void k_convert_batch(ConvBatch B) {
+  const unsigned blk = blockIdx.x;
+  int s = 0;
+  while (s + 1 < B.nseg && blk >= B.first[s + 1]) s++;
+  const size_t i = (size_t)(blk - B.first[s]) * kBlock + threadIdx.x;
+  if (i < B.n[s]) B.dst[s][i] = (float)B.src[s][i];
 }
 
 // wall faces of component d copied from the first plane of low faces (the
