@@ -2083,7 +2083,10 @@ int bmg_grid_dump(bmg_grid* g, const char* path, const int* fields, const char* 
   });
   ok = ok && (buf.empty() || fwrite(buf.data(), 1, buf.size(), fp) == buf.size());
   ok = (fclose(fp) == 0) && ok;
-  if (!ok) return fail(BMG_ERR_ARG, "dump: write error on %s", tmp.c_str());
+  if (!ok) {
+    remove(tmp.c_str());
+    return fail(BMG_ERR_ARG, "dump: write error on %s", tmp.c_str());
+  }
   if (rename(tmp.c_str(), path) != 0) return fail(BMG_ERR_ARG, "dump: cannot rename %s", tmp.c_str());
   return BMG_OK;
 }

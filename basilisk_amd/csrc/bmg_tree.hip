@@ -26,9 +26,11 @@
 #include <map>
 #include <memory>
 #include <atomic>
+#include <chrono>
 #include <string>
 #include <thread>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bmg.h"
@@ -443,11 +445,36 @@ struct Topo {
   bool is_refined(const Cell& c) const { return !(c.flags & C_LEAF) && c.nb && c.pid >= 0; }
   bool is_prolongation(const Cell& c) const { return !(c.flags & C_LEAF) && !c.nb && c.pid >= 0; }
 
-  void visit(int l, IJK c) {
+  // the cache entries of one stretch of the pre-order traversal; `bnd`
+  // holds candidates (a boundary cell may repeat: the merge keeps its first
+  // occurrence, as the marks of a single traversal do)
+  struct CachePart {
+    std::vector<std::vector<IJK>> active, prolong, bnd;
+    std::vector<int> leaf_level;
+    std::vector<IJK> leaf_c;
+    std::vector<Face> faces;
+    int root_level = -1;  // a subtree still to traverse from this root
+    IJK root{0, 0, 0};
+  };
+  void size_part(CachePart& P) const {
+    P.active.assign(depth + 1, {});
+    P.prolong.assign(depth + 1, {});
+    P.bnd.assign(depth + 1, {});
+  }
+  // a cell, or a default (unallocated) cell; reads only
+  const Cell& cell_or_none(int l, int i, int j, int k) {
+    static const Cell none{};
+    const Cell* c = L[l].find(i, j, k);
+    return c ? *c : none;
+  }
+
+  // the entries of cell c itself (grid/tree.h:556-684's caches); true when
+  // its children follow in the traversal.  Reads the topology only.
+  bool visit_self(int l, IJK c, CachePart& P) {
     Cell* cp = L[l].find(c.i, c.j, c.k);
-    if (!cp) return;
-    Cell& cell = *cp;
-    if (cell.pid == 0 && (cell.flags & C_ACTIVE)) active[l].push_back(c);
+    if (!cp) return false;
+    const Cell& cell = *cp;
+    if (cell.pid == 0 && (cell.flags & C_ACTIVE)) P.active[l].push_back(c);
     // only a cell next to the box has ghost cells in its ring
     const int hi = (1 << l) + GH - 1;
     const bool edge = c.i <= GH || c.i >= hi || c.j <= GH || c.j >= hi || (dim == 3 && (c.k <= GH || c.k >= hi));
@@ -456,52 +483,119 @@ struct Topo {
       for (int a = -1; a <= 1; a++)
         for (int b = -1; b <= 1; b++)
           for (int e = e0; e <= e1; e++) {
-            Cell* n = L[l].find(c.i + a, c.j + b, c.k + e);
-            if (n && is_boundary(*n) && !(n->flags & C_MARK)) {
-              bnd[l].push_back({c.i + a, c.j + b, c.k + e});
-              n->flags |= C_MARK;
-            }
+            const Cell* n = L[l].find(c.i + a, c.j + b, c.k + e);
+            if (n && is_boundary(*n)) P.bnd[l].push_back({c.i + a, c.j + b, c.k + e});
           }
     }
     if (is_leaf(cell)) {
       if (cell.pid == 0) {
-        leaf_level.push_back(l);
-        leaf_c.push_back(c);
+        P.leaf_level.push_back(l);
+        P.leaf_c.push_back(c);
         int flags = 0;
         for (int d = 0; d < dim; d++) {
           IJK m = c;
           m[d]--;
-          Cell& mc = at(l, m.i, m.j, m.k);
+          const Cell& mc = cell_or_none(l, m.i, m.j, m.k);
           if (is_boundary(mc) || is_prolongation(mc) || is_leaf(mc)) flags |= 1 << d;
         }
-        if (flags) faces.push_back({l, c, flags});
+        if (flags) P.faces.push_back({l, c, flags});
         for (int d = 0; d < dim; d++) {
           IJK p = c;
           p[d]++;
-          Cell& pc = at(l, p.i, p.j, p.k);
+          const Cell& pc = cell_or_none(l, p.i, p.j, p.k);
           if (is_boundary(pc) || is_prolongation(pc) || (pc.pid != 0 && is_leaf(pc)))
-            faces.push_back({l, p, 1 << d});
+            P.faces.push_back({l, p, 1 << d});
         }
-        if (cell.nb > 0) prolong[l].push_back(c);
+        if (cell.nb > 0) P.prolong[l].push_back(c);
       }
+      return false;
+    }
+    return l < depth;
+  }
+  IJK child(IJK c, int a, int b, int e) const {
+    return {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0};
+  }
+  // pre-order traversal of the subtree of c into P
+  void visit_part(int l, IJK c, CachePart& P) {
+    if (!visit_self(l, c, P)) return;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < kext(); e++) visit_part(l + 1, child(c, a, b, e), P);
+  }
+  // pre-order traversal above level `split`: the cells there go to the last
+  // part; each cell of level `split` gets a part of its own, traversed later,
+  // and a new part follows it
+  void visit_top(int l, IJK c, std::vector<CachePart>& parts, int split) {
+    if (l == split) {
+      if (!L[l].find(c.i, c.j, c.k)) return;
+      parts.emplace_back();
+      parts.back().root_level = l;
+      parts.back().root = c;
+      parts.emplace_back();
+      size_part(parts.back());
       return;
     }
-    if (l < depth)
-      for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
-          for (int e = 0; e < kext(); e++)
-            visit(l + 1, {2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0});
+    if (!visit_self(l, c, parts.back())) return;
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < kext(); e++) visit_top(l + 1, child(c, a, b, e), parts, split);
   }
 
+  // the caches in pre-order; the subtrees below level 3 (3D) or 4 (2D) are
+  // traversed on up to 16 host threads and concatenated in traversal order,
+  // which is the order of a single traversal
   void update() {
     if (!dirty) return;
+    const int split = dim == 3 ? 3 : 4;
+    const IJK root = {GH, GH, dim == 3 ? GH : 0};
+    std::vector<CachePart> parts(1);
+    size_part(parts[0]);
+    if (dim >= 2 && depth > split)
+      visit_top(0, root, parts, split);
+    else
+      visit_part(0, root, parts[0]);
+    std::vector<size_t> subs;
+    for (size_t q = 0; q < parts.size(); q++)
+      if (parts[q].root_level >= 0) subs.push_back(q);
+    if (!subs.empty()) {
+      std::atomic<size_t> next{0};
+      auto work = [&]() {
+        for (size_t q; (q = next.fetch_add(1)) < subs.size();) {
+          CachePart& P = parts[subs[q]];
+          size_part(P);
+          visit_part(P.root_level, P.root, P);
+        }
+      };
+      unsigned nt = std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+      nt = (unsigned)std::min<size_t>(nt, subs.size());
+      std::vector<std::thread> th;
+      for (unsigned k = 1; k < nt; k++) th.emplace_back(work);
+      work();
+      for (auto& x : th) x.join();
+    }
     active.assign(depth + 1, {});
     prolong.assign(depth + 1, {});
     bnd.assign(depth + 1, {});
     leaf_level.clear();
     leaf_c.clear();
     faces.clear();
-    visit(0, {GH, GH, dim == 3 ? GH : 0});
+    for (auto& P : parts) {
+      for (int l = 0; l <= depth; l++) {
+        active[l].insert(active[l].end(), P.active[l].begin(), P.active[l].end());
+        prolong[l].insert(prolong[l].end(), P.prolong[l].begin(), P.prolong[l].end());
+        for (auto& g : P.bnd[l]) {
+          Cell& n = at(l, g.i, g.j, g.k);
+          if (!(n.flags & C_MARK)) {
+            bnd[l].push_back(g);
+            n.flags |= C_MARK;
+          }
+        }
+      }
+      leaf_level.insert(leaf_level.end(), P.leaf_level.begin(), P.leaf_level.end());
+      leaf_c.insert(leaf_c.end(), P.leaf_c.begin(), P.leaf_c.end());
+      faces.insert(faces.end(), P.faces.begin(), P.faces.end());
+      P = CachePart();
+    }
     for (int l = 0; l <= depth; l++)
       for (auto& c : bnd[l]) at(l, c.i, c.j, c.k).flags &= (uint8_t)~C_MARK;
     dirty = false;
@@ -689,10 +783,27 @@ void par_for(size_t n, F&& f) {
   for (auto& x : th) x.join();
 }
 
+// host wall time of the phases of a rebuild, printed to stderr when
+// BMG_TREE_TIMING is set (diagnostics only; GPU phases are synchronised
+// first when it is)
+struct PhaseClock {
+  bool on = getenv("BMG_TREE_TIMING") != nullptr;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  void mark(const char* what, hipStream_t s = nullptr) {
+    if (!on) return;
+    if (s) (void)hipStreamSynchronize(s);
+    const auto t1 = std::chrono::steady_clock::now();
+    fprintf(stderr, "[bmg tree] %s: %.1f ms\n", what, std::chrono::duration<double, std::milli>(t1 - t0).count());
+    t0 = t1;
+  }
+};
+
 // compile the topology into slot tables and allocate the fields
 int compile(bmg_tree* t) {
+  PhaseClock clk;
   Topo& T = t->topo;
   T.update();
+  clk.mark("compile: caches");
   const int dim = t->dim, D = T.depth;
   if (D + 1 > kSeg) return bmg::fail(BMG_ERR_UNSUPPORTED, "tree depth %d exceeds %d levels", D, kSeg - 1);
   // keep leaf values across recompilation
@@ -776,6 +887,7 @@ int compile(bmg_tree* t) {
     if (v < 0 || v >= NS) ok.store(false, std::memory_order_relaxed);
     return v;
   };
+  clk.mark("compile: slots");
   // active neighbours and colours
   long NA = x.abase[D] + x.nA[D];
   std::vector<int> nbr((size_t)NA * 2 * dim);
@@ -796,12 +908,19 @@ int compile(bmg_tree* t) {
     });
   if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: active cell without neighbours");
   // the non-leaf active cells of each level (indices into active[l])
+  clk.mark("compile: tables: neighbours");
   std::vector<std::vector<int>> nonleaf(D + 1);
-  for (int l = 0; l <= D; l++)
-    for (int q = 0; q < x.nA[l]; q++) {
-      const IJK& c = T.active[l][q];
-      if (!T.is_leaf(*T.L[l].find(c.i, c.j, c.k))) nonleaf[l].push_back(q);
-    }
+  for (int l = 0; l <= D; l++) {
+    std::vector<uint8_t> isnl((size_t)x.nA[l]);
+    par_for((size_t)x.nA[l], [&](size_t q0, size_t q1) {
+      for (size_t q = q0; q < q1; q++) {
+        const IJK& c = T.active[l][q];
+        isnl[q] = !T.is_leaf(*T.L[l].find(c.i, c.j, c.k));
+      }
+    });
+    for (int q = 0; q < x.nA[l]; q++)
+      if (isnl[q]) nonleaf[l].push_back(q);
+  }
   // parent records: non-leaf active cells, then prolongation leaves
   const int NR = dim == 3 ? 27 : 9;
   x.rec_off.assign(D + 1, 0);
@@ -844,13 +963,16 @@ int compile(bmg_tree* t) {
   if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: parent record with an unallocated ring");
   // leaves: global cache order and per level
   x.nleaf = (long)T.leaf_c.size();
+  clk.mark("compile: tables: parent records");
   std::vector<int> leaf_gs(x.nleaf);
   std::vector<std::vector<int>> lv(D + 1);
-  for (long n = 0; n < x.nleaf; n++) {
-    IJK c = T.leaf_c[n];
-    leaf_gs[n] = chk(S(T.leaf_level[n], c.i, c.j, c.k));
-    lv[T.leaf_level[n]].push_back(leaf_gs[n]);
-  }
+  par_for((size_t)x.nleaf, [&](size_t n0, size_t n1) {
+    for (size_t n = n0; n < n1; n++) {
+      IJK c = T.leaf_c[n];
+      leaf_gs[n] = chk(S(T.leaf_level[n], c.i, c.j, c.k));
+    }
+  });
+  for (long n = 0; n < x.nleaf; n++) lv[T.leaf_level[n]].push_back(leaf_gs[n]);
   std::vector<int> lvl_leaves;
   x.leaf_off.assign(D + 1, 0);
   x.leaf_n.assign(D + 1, 0);
@@ -859,6 +981,7 @@ int compile(bmg_tree* t) {
     x.leaf_n[l] = (int)lv[l].size();
     lvl_leaves.insert(lvl_leaves.end(), lv[l].begin(), lv[l].end());
   }
+  clk.mark("compile: tables: leaves");
   // box rules (boundary cache order)
   x.bc_off.assign(D + 2, 0);
   for (int l = 0; l <= D; l++) {
@@ -937,6 +1060,7 @@ int compile(bmg_tree* t) {
     }
     x.box_n[l] = (int)box.size() - x.box_off[l];
   }
+  clk.mark("compile: tables: box rules");
   // residual face ops per level, and the face-cache value map (entry order
   // then dimension); the positions come from prefix counts of the flag bits
   const size_t NF = T.faces.size();
@@ -978,6 +1102,7 @@ int compile(bmg_tree* t) {
         }
     }
   });
+  clk.mark("compile: tables: face ops");
   // fine-face averages: halo_face (all levels in one launch) and the face
   // restriction of alpha (per level)
   auto fine_avg = [&](int l, const int c[3], int d, int hi, int target) {
@@ -992,18 +1117,48 @@ int compile(bmg_tree* t) {
                          dim == 3 ? base[2] + ftan(dim, d, m, 2) : 0));
     return fa;
   };
+  // halo faces of the prolongation leaves, levels from the depth up: per
+  // leaf a bit (2d + side) for each refined face neighbour, then the records
+  // in (leaf, d, side) order at prefix-sum positions
   std::vector<FaceAvg> hf;
-  for (int l = D - 1; l >= 0; l--)
-    for (auto& p : T.prolong[l]) {
-      const int c[3] = {p.i, p.j, p.k};
-      for (int d = 0; d < dim; d++) {
-        int m[3] = {c[0], c[1], c[2]}, q[3] = {c[0], c[1], c[2]};
-        m[d]--;
-        q[d]++;
-        if (T.is_refined(T.at(l, m[0], m[1], m[2]))) hf.push_back(fine_avg(l, c, d, 0, chk(S(l, c[0], c[1], c[2]))));
-        if (T.is_refined(T.at(l, q[0], q[1], q[2]))) hf.push_back(fine_avg(l, c, d, 1, chk(S(l, q[0], q[1], q[2]))));
+  for (int l = D - 1; l >= 0; l--) {
+    const size_t np = T.prolong[l].size();
+    std::vector<uint8_t> mask(np);
+    par_for(np, [&](size_t k0, size_t k1) {
+      for (size_t k = k0; k < k1; k++) {
+        const IJK& p = T.prolong[l][k];
+        const int c[3] = {p.i, p.j, p.k};
+        uint8_t b = 0;
+        for (int d = 0; d < dim; d++) {
+          int m[3] = {c[0], c[1], c[2]}, q[3] = {c[0], c[1], c[2]};
+          m[d]--;
+          q[d]++;
+          const Cell* cm = T.L[l].find(m[0], m[1], m[2]);
+          const Cell* cq = T.L[l].find(q[0], q[1], q[2]);
+          if (cm && T.is_refined(*cm)) b |= (uint8_t)(1u << (2 * d));
+          if (cq && T.is_refined(*cq)) b |= (uint8_t)(1u << (2 * d + 1));
+        }
+        mask[k] = b;
       }
-    }
+    });
+    std::vector<size_t> pos(np + 1);
+    pos[0] = hf.size();
+    for (size_t k = 0; k < np; k++) pos[k + 1] = pos[k] + (size_t)__builtin_popcount(mask[k]);
+    hf.resize(pos[np]);
+    par_for(np, [&](size_t k0, size_t k1) {
+      for (size_t k = k0; k < k1; k++) {
+        const IJK& p = T.prolong[l][k];
+        const int c[3] = {p.i, p.j, p.k};
+        size_t w = pos[k];
+        for (int d = 0; d < dim; d++) {
+          int q[3] = {c[0], c[1], c[2]};
+          q[d]++;
+          if (mask[k] & (1u << (2 * d))) hf[w++] = fine_avg(l, c, d, 0, chk(S(l, c[0], c[1], c[2])));
+          if (mask[k] & (1u << (2 * d + 1))) hf[w++] = fine_avg(l, c, d, 1, chk(S(l, q[0], q[1], q[2])));
+        }
+      }
+    });
+  }
   x.nhf = (int)hf.size();
   x.fr_off.assign(D + 1, 0);
   x.fr_n.assign(D + 1, 0);
@@ -1088,14 +1243,17 @@ int compile(bmg_tree* t) {
       }
     }
     });
+  clk.mark("compile: tables: face averages and refine_face");
   // ghost tangential faces: sequential semantics resolved to pre-pass sources
   std::vector<FaceCopy> fc;
   x.fc_off.assign(D + 1, 0);
   x.fc_n.assign(D + 1, 0);
   for (int l = 0; l <= D; l++) {
     x.fc_off[l] = (int)fc.size();
-    std::vector<std::vector<int>> written(dim);  // slot -> source for this pass
+    // (target, source) pairs of this pass in order of first write, and the
+    // position of each target among them
     std::vector<std::vector<std::pair<int, int>>> w(dim);
+    std::vector<std::unordered_map<int, size_t>> wpos(dim);
     for (auto& g : T.bnd[l]) {
       const int gc[3] = {g.i, g.j, g.k};
       int gs = chk(S(l, gc[0], gc[1], gc[2]));
@@ -1119,17 +1277,18 @@ int compile(bmg_tree* t) {
           }
         }
         int src = found;
-        if (found >= 0)
-          for (auto& pr : w[d])
-            if (pr.first == found) src = pr.second;  // written earlier in this pass
+        if (found >= 0) {
+          auto it = wpos[d].find(found);
+          if (it != wpos[d].end()) src = w[d][it->second].second;  // written earlier in this pass
+        }
         // later writes of the same target replace earlier ones
-        bool replaced = false;
-        for (auto& pr : w[d])
-          if (pr.first == gs) {
-            pr.second = src;
-            replaced = true;
-          }
-        if (!replaced) w[d].push_back({gs, src});
+        auto it = wpos[d].find(gs);
+        if (it != wpos[d].end()) {
+          w[d][it->second].second = src;
+        } else {
+          wpos[d].emplace(gs, w[d].size());
+          w[d].push_back({gs, src});
+        }
       }
     }
     for (int d = 0; d < dim; d++)
@@ -1138,6 +1297,7 @@ int compile(bmg_tree* t) {
     x.maxfc = std::max(x.maxfc, x.fc_n[l]);
   }
   if (!ok) return bmg::fail(BMG_ERR_ARG, "tree: index table refers to an unallocated cell");
+  clk.mark("compile: tables");
   // upload
   TRY(upload(&x.nbr, nbr));
   TRY(upload(&x.colour, colour));
@@ -1233,6 +1393,7 @@ int compile(bmg_tree* t) {
   TRY(restore(t->keep_b, t->b));
   TRY(restore(t->keep_s, t->s));
   TRY(restore(t->keep_lam, t->lam64));
+  clk.mark("compile: upload and fields", t->stream);
   return BMG_OK;
 }
 
@@ -2072,6 +2233,7 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
   TRY(ensure(t));
   if (!t->s_set) return bmg::fail(BMG_ERR_ARG, "set the estimated scalar (BMG_F_S) first");
   Topo& T = t->topo;
+  PhaseClock clk;
   std::vector<double> err;
   if (t->dim == 2) {
     adapt_boundary<2>(t);
@@ -2080,6 +2242,7 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
     adapt_boundary<3>(t);
     TRY(adapt_estimate<3>(t, err));
   }
+  clk.mark("adapt: estimate", t->stream);
   Topo::Adapt A;
   A.err = &err;
   A.max = max;
@@ -2088,6 +2251,7 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
   const IJK root = {GH, GH, t->dim == 3 ? GH : 0};
   T.adapt_refine(0, root, A);
   for (int l = T.depth; l >= 0; l--) T.adapt_coarsen(0, root, l, A);
+  clk.mark("adapt: refine and coarsen passes");
   if (!A.ok || !A.log.ok) {
     t->compiled = false;
     return bmg::fail(BMG_ERR_ARG, "tree adaptation met an inconsistent tree");
@@ -2104,12 +2268,15 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
     t->keep_s.clear();
     std::vector<int> prev;
     t->prev_slots = &prev;
+    clk.mark("adapt: before compile");
     int rc = compile(t);
     t->prev_slots = nullptr;
+    clk.mark("adapt: compile");
     if (rc == BMG_OK)
       rc = t->dim == 2 ? adapt_remap<2>(t, olds, old_nslot, prev, A.log)
                        : adapt_remap<3>(t, olds, old_nslot, prev, A.log);
     for (double* p : olds) dfree_(p);
+    clk.mark("adapt: remap", t->stream);
     if (rc != BMG_OK) return rc;
     t->coef_ready = false;
   }
@@ -2246,7 +2413,10 @@ int bmg_tree_dump(bmg_tree* t, const char* path, const int* fields, const char* 
     }
   }
   ok = (fclose(fp) == 0) && ok;
-  if (!ok) return bmg::fail(BMG_ERR_ARG, "dump: write error on %s", tmp.c_str());
+  if (!ok) {
+    remove(tmp.c_str());
+    return bmg::fail(BMG_ERR_ARG, "dump: write error on %s", tmp.c_str());
+  }
   if (rename(tmp.c_str(), path) != 0) return bmg::fail(BMG_ERR_ARG, "dump: cannot rename %s", tmp.c_str());
   return BMG_OK;
 }
