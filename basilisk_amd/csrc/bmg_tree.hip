@@ -50,6 +50,21 @@ enum : uint8_t {
   C_REFINED = 16, C_TOO_FINE = 32, C_TOO_COARSE = 64, C_JUST_FINE = 128
 };
 
+// f(begin, end) over [0, n) on up to 16 host threads, at least `grain`
+// items each; f reads the topology through find() only, which writes nothing
+template <typename F>
+void par_for(size_t n, F&& f, size_t grain = 32768) {
+  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  nt = (unsigned)std::min<size_t>(nt, n / grain);
+  if (nt <= 1) {
+    f((size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (unsigned k = 0; k < nt; k++) th.emplace_back([&, k] { f(n * k / nt, n * (k + 1) / nt); });
+  for (auto& x : th) x.join();
+}
+
 struct Cell {
   uint8_t flags = 0;
   int8_t pid = 0;
@@ -573,16 +588,43 @@ struct Topo {
       work();
       for (auto& x : th) x.join();
     }
+    // concatenation in part order: offsets first, then the copies on host
+    // threads; the boundary candidates keep their first occurrences
+    const size_t np = parts.size();
+    std::vector<std::vector<size_t>> aoff(depth + 1, std::vector<size_t>(np + 1, 0)), poff = aoff;
+    std::vector<size_t> loff(np + 1, 0), foff(np + 1, 0);
+    for (size_t q = 0; q < np; q++) {
+      for (int l = 0; l <= depth; l++) {
+        aoff[l][q + 1] = aoff[l][q] + parts[q].active[l].size();
+        poff[l][q + 1] = poff[l][q] + parts[q].prolong[l].size();
+      }
+      loff[q + 1] = loff[q] + parts[q].leaf_c.size();
+      foff[q + 1] = foff[q] + parts[q].faces.size();
+    }
     active.assign(depth + 1, {});
     prolong.assign(depth + 1, {});
     bnd.assign(depth + 1, {});
-    leaf_level.clear();
-    leaf_c.clear();
-    faces.clear();
+    for (int l = 0; l <= depth; l++) {
+      active[l].resize(aoff[l][np]);
+      prolong[l].resize(poff[l][np]);
+    }
+    leaf_level.assign(loff[np], 0);
+    leaf_c.resize(loff[np]);
+    faces.resize(foff[np]);
+    par_for(np, [&](size_t q0, size_t q1) {
+      for (size_t q = q0; q < q1; q++) {
+        const CachePart& P = parts[q];
+        for (int l = 0; l <= depth; l++) {
+          std::copy(P.active[l].begin(), P.active[l].end(), active[l].begin() + aoff[l][q]);
+          std::copy(P.prolong[l].begin(), P.prolong[l].end(), prolong[l].begin() + poff[l][q]);
+        }
+        std::copy(P.leaf_level.begin(), P.leaf_level.end(), leaf_level.begin() + loff[q]);
+        std::copy(P.leaf_c.begin(), P.leaf_c.end(), leaf_c.begin() + loff[q]);
+        std::copy(P.faces.begin(), P.faces.end(), faces.begin() + foff[q]);
+      }
+    }, 1);
     for (auto& P : parts) {
-      for (int l = 0; l <= depth; l++) {
-        active[l].insert(active[l].end(), P.active[l].begin(), P.active[l].end());
-        prolong[l].insert(prolong[l].end(), P.prolong[l].begin(), P.prolong[l].end());
+      for (int l = 0; l <= depth; l++)
         for (auto& g : P.bnd[l]) {
           Cell& n = at(l, g.i, g.j, g.k);
           if (!(n.flags & C_MARK)) {
@@ -590,10 +632,6 @@ struct Topo {
             n.flags |= C_MARK;
           }
         }
-      }
-      leaf_level.insert(leaf_level.end(), P.leaf_level.begin(), P.leaf_level.end());
-      leaf_c.insert(leaf_c.end(), P.leaf_c.begin(), P.leaf_c.end());
-      faces.insert(faces.end(), P.faces.begin(), P.faces.end());
       P = CachePart();
     }
     for (int l = 0; l <= depth; l++)
@@ -768,21 +806,6 @@ const int kFtan2[2][2][3] = {{{0, 0, 0}, {0, 1, 0}}, {{0, 0, 0}, {1, 0, 0}}};
 
 int ftan(int dim, int d, int m, int e) { return dim == 3 ? kFtan[d][m][e] : kFtan2[d][m][e]; }
 
-// f(begin, end) over [0, n) on up to 16 host threads (large n only); f
-// reads the topology through find() only, which writes nothing
-template <typename F>
-void par_for(size_t n, F&& f) {
-  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-  nt = (unsigned)std::min<size_t>(nt, n / 32768);
-  if (nt <= 1) {
-    f((size_t)0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (unsigned k = 0; k < nt; k++) th.emplace_back([&, k] { f(n * k / nt, n * (k + 1) / nt); });
-  for (auto& x : th) x.join();
-}
-
 // host wall time of the phases of a rebuild, printed to stderr when
 // BMG_TREE_TIMING is set (diagnostics only; GPU phases are synchronised
 // first when it is)
@@ -822,21 +845,46 @@ int compile(bmg_tree* t) {
     if (t->prev_slots) t->prev_slots->push_back(c->slot);
     c->slot = (int)next++;
   };
+  // n cells taking the next n slots in order (distinct cells, so the host
+  // threads write distinct slots); false when cell_of(q) finds no cell
+  auto take_range = [&](size_t n, auto&& cell_of) {
+    const long base = next;
+    if (t->prev_slots) t->prev_slots->resize((size_t)base + n);
+    std::atomic<bool> found{true};
+    par_for(n, [&](size_t q0, size_t q1) {
+      for (size_t q = q0; q < q1; q++) {
+        Cell* c = cell_of(q);
+        if (!c) {
+          found.store(false, std::memory_order_relaxed);
+          continue;
+        }
+        if (t->prev_slots) (*t->prev_slots)[(size_t)base + q] = c->slot;
+        c->slot = (int)(base + (long)q);
+      }
+    });
+    next += (long)n;
+    return found.load();
+  };
+  const int nch = 1 << dim;  // children per cell, in foreach_child order
   for (int l = 0; l <= D; l++) {
     TLevel& Lv = T.L[l];
     x.off[l] = (int)next;
     x.abase[l] = l ? x.abase[l - 1] + x.nA[l - 1] : 0;
-    for (auto& c : T.active[l]) take(Lv.find(c.i, c.j, c.k));
+    if (!take_range(T.active[l].size(), [&](size_t q) {
+          const IJK& c = T.active[l][q];
+          return Lv.find(c.i, c.j, c.k);
+        }))
+      return bmg::fail(BMG_ERR_ARG, "tree: missing active cell");
     x.nA[l] = (int)T.active[l].size();
     if (l > 0) {
-      for (auto& p : T.prolong[l - 1])
-        for (int a = 0; a < 2; a++)
-          for (int b = 0; b < 2; b++)
-            for (int e = 0; e < T.kext(); e++) {
-              Cell* ch = Lv.find(2 * p.i - GH + a, 2 * p.j - GH + b, dim == 3 ? 2 * p.k - GH + e : 0);
-              if (!ch) return bmg::fail(BMG_ERR_ARG, "tree: missing halo child");
-              take(ch);
-            }
+      const std::vector<IJK>& pl = T.prolong[l - 1];
+      if (!take_range(pl.size() * nch, [&](size_t q) {
+            const IJK& p = pl[q / nch];
+            const int m = (int)(q % nch);
+            const int a = dim == 3 ? m >> 2 : m >> 1, b = dim == 3 ? (m >> 1) & 1 : m & 1, e = dim == 3 ? m & 1 : 0;
+            return Lv.find(2 * p.i - GH + a, 2 * p.j - GH + b, dim == 3 ? 2 * p.k - GH + e : 0);
+          }))
+        return bmg::fail(BMG_ERR_ARG, "tree: missing halo child");
       for (auto& gp : ghosts[l - 1]) {
         Cell* gc = T.L[l - 1].find(gp.i, gp.j, gp.k);
         if (gc->nb == 0) continue;
@@ -863,14 +911,20 @@ int compile(bmg_tree* t) {
           }
     }
     // every allocated cell has a slot
-    long count = 0;
-    for (auto& br : Lv.bricks)
-      for (size_t q = 0; q < Lv.bsize(); q++)
-        if (br[q].flags & C_ALLOC) {
-          count++;
-          if (br[q].slot < x.off[l] || br[q].slot >= next)
-            return bmg::fail(BMG_ERR_ARG, "tree: unslotted cell at level %d", l);
-        }
+    std::atomic<long> count{0};
+    std::atomic<bool> slotted{true};
+    const long lo = x.off[l], hi = next;
+    par_for(Lv.bricks.size(), [&](size_t b0, size_t b1) {
+      long cnt = 0;
+      for (size_t b = b0; b < b1; b++)
+        for (size_t q = 0; q < Lv.bsize(); q++)
+          if (Lv.bricks[b][q].flags & C_ALLOC) {
+            cnt++;
+            if (Lv.bricks[b][q].slot < lo || Lv.bricks[b][q].slot >= hi) slotted.store(false, std::memory_order_relaxed);
+          }
+      count += cnt;
+    }, 64);
+    if (!slotted) return bmg::fail(BMG_ERR_ARG, "tree: unslotted cell at level %d", l);
     if (count != next - x.off[l]) return bmg::fail(BMG_ERR_ARG, "tree: slot count mismatch at level %d", l);
   }
   x.off[D + 1] = (int)next;
