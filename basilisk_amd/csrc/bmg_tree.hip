@@ -50,6 +50,29 @@ enum : uint8_t {
   C_REFINED = 16, C_TOO_FINE = 32, C_TOO_COARSE = 64, C_JUST_FINE = 128
 };
 
+// host tables whose every entry the loops that build them write:
+// default-initialised, so no serial zero fill precedes the host threads
+template <typename T>
+struct NoFillAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoFillAlloc<U>;
+  };
+  NoFillAlloc() = default;
+  template <typename U>
+  NoFillAlloc(const NoFillAlloc<U>&) noexcept {}
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+  template <typename U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+};
+template <typename T>
+using table = std::vector<T, NoFillAlloc<T>>;
+
 // f(begin, end) over [0, n) on up to 16 host threads, at least `grain`
 // items each; f reads the topology through find() only, which writes nothing
 template <typename F>
@@ -682,8 +705,8 @@ void dfree_(void* p) {
   if (p) (void)hipFree(p);
 }
 
-template <typename T>
-int upload(T** dst, const std::vector<T>& v) {
+template <typename T, typename A>
+int upload(T** dst, const std::vector<T, A>& v) {
   size_t bytes = std::max<size_t>(v.size() * sizeof(T), 8);
   hipError_t e = hipMalloc((void**)dst, bytes);
   if (e != hipSuccess) return bmg::fail(BMG_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -944,8 +967,8 @@ int compile(bmg_tree* t) {
   clk.mark("compile: slots");
   // active neighbours and colours
   long NA = x.abase[D] + x.nA[D];
-  std::vector<int> nbr((size_t)NA * 2 * dim);
-  std::vector<uint8_t> colour(NA);
+  table<int> nbr((size_t)NA * 2 * dim);
+  table<uint8_t> colour(NA);
   for (int l = 0; l <= D; l++)
     par_for((size_t)x.nA[l], [&](size_t q0, size_t q1) {
       for (size_t q = q0; q < q1; q++) {
@@ -987,7 +1010,7 @@ int compile(bmg_tree* t) {
     x.n_prolong[l] = (int)T.prolong[l].size();
     nrec += (size_t)x.n_nonleaf[l] + x.n_prolong[l];
   }
-  std::vector<int> rself(nrec), rchild(nrec), rring(nrec * NR);
+  table<int> rself(nrec), rchild(nrec), rring(nrec * NR);
   auto add_rec = [&](size_t r, int l, IJK c) {
     rself[r] = chk(S(l, c.i, c.j, c.k));
     int c0 = chk(S(l + 1, 2 * c.i - GH, 2 * c.j - GH, dim == 3 ? 2 * c.k - GH : 0));
@@ -1018,7 +1041,7 @@ int compile(bmg_tree* t) {
   // leaves: global cache order and per level
   x.nleaf = (long)T.leaf_c.size();
   clk.mark("compile: tables: parent records");
-  std::vector<int> leaf_gs(x.nleaf);
+  table<int> leaf_gs(x.nleaf);
   std::vector<std::vector<int>> lv(D + 1);
   par_for((size_t)x.nleaf, [&](size_t n0, size_t n1) {
     for (size_t n = n0; n < n1; n++) {
@@ -1118,7 +1141,7 @@ int compile(bmg_tree* t) {
   // residual face ops per level, and the face-cache value map (entry order
   // then dimension); the positions come from prefix counts of the flag bits
   const size_t NF = T.faces.size();
-  std::vector<size_t> gpos(NF), lpos(NF);
+  table<size_t> gpos(NF), lpos(NF);
   x.face_off.assign(D + 1, 0);
   x.face_n.assign(D + 1, 0);
   {
@@ -1137,8 +1160,8 @@ int compile(bmg_tree* t) {
       cur[T.faces[e].level] += __builtin_popcount((unsigned)T.faces[e].flags & ((1u << dim) - 1));
     }
   }
-  std::vector<int> fv_gs(x.nfaceval), fv_d(x.nfaceval);
-  std::vector<FaceOp> fops(x.nfaceval);
+  table<int> fv_gs(x.nfaceval), fv_d(x.nfaceval);
+  table<FaceOp> fops(x.nfaceval);
   par_for(NF, [&](size_t e0, size_t e1) {
     for (size_t e = e0; e < e1; e++) {
       const Face& f = T.faces[e];
@@ -1222,7 +1245,7 @@ int compile(bmg_tree* t) {
     x.fr_n[l] = (int)nonleaf[l].size() * 2 * dim;
     nfr += x.fr_n[l];
   }
-  std::vector<FaceAvg> fr(nfr);
+  table<FaceAvg> fr(nfr);
   for (int l = 0; l <= D; l++)
     par_for(nonleaf[l].size(), [&](size_t k0, size_t k1) {
       for (size_t k = k0; k < k1; k++) {
@@ -1246,7 +1269,7 @@ int compile(bmg_tree* t) {
     x.fp_n[l] = (int)T.prolong[l].size() * dim;
     nfp += x.fp_n[l];
   }
-  std::vector<FaceProlong> fp(nfp);
+  table<FaceProlong> fp(nfp);
   for (int l = 0; l < D; l++)
     par_for(T.prolong[l].size(), [&](size_t k0, size_t k1) {
     for (size_t kp = k0; kp < k1; kp++) {
