@@ -47,7 +47,8 @@ class PoissonSolver:
             self.fwd.set_bc(side, kind, values)
             self.adj.set_bc(side, kind, None)
         self.shape = self.fwd.shape()
-        self.last_stats = None
+        self.last_stats = None  # mgstats of the last forward solve
+        self.last_adjoint_stats = None  # ... and of the last adjoint solve
 
     def _coefficients(self, g, alpha, lam):
         if alpha is None:
@@ -101,7 +102,7 @@ class _PoissonFunction(torch.autograd.Function):
         alpha = saved if ctx.has_alpha else None
         if phi is None:
             phi, _ = solver._solve(solver.fwd, b, alpha, lam)
-        psi, _ = solver._solve(solver.adj, gphi.contiguous(), alpha, lam)
+        psi, solver.last_adjoint_stats = solver._solve(solver.adj, gphi.contiguous(), alpha, lam)
         g = solver.fwd
         g.set_a(phi.detach().contiguous())
         need_a = ctx.has_alpha and any(ctx.needs_input_grad[3:])

@@ -208,7 +208,7 @@ def run_precision(bmg, torch, prob, prec, args):
     visc = None
     if prob.get("inside") is not None and prec == "fp64":
         # viscosity() (viscosity.h:163-178) of a two-phase velocity field on
-        # the same grid: the second mg_solve client, FP64, Jacobi
+        # the same grid: the second mg_solve client, FP64, the grid's smoother
         from basilisk_amd import configs
         vp = configs.config2_viscous(prob)
         v0 = [torch.from_numpy(x).to(dev) for x in vp["u"]]
@@ -231,6 +231,39 @@ def run_precision(bmg, torch, prob, prec, args):
     g.close()
     return {"seconds": dt, "stats": stats[-1], "profile": kinds, "profiled_stats": st, "a": a, "project": proj,
             "viscosity": visc}
+
+
+def run_adjoint(bmg, torch, prob, args):
+    """One differentiable solve and its backward through basilisk_amd.autograd
+    (mixed precision, the problem's tolerance): phi = poisson(b, alpha),
+    J = sum(w phi) with seeded zero-mean weights w, then dJ/db and dJ/dalpha
+    from the adjoint solve and the fused face-gradient kernel."""
+    import numpy as np
+    from basilisk_amd.autograd import PoissonSolver
+    dev = torch.device("cuda:0")
+    S = PoissonSolver(prob["dim"], prob["depth"], precision="mixed", smoother=args.smoother,
+                      tolerance=prob["tolerance"])
+    b = torch.from_numpy(prob["b"]).to(dev).requires_grad_(True)
+    alpha = [torch.from_numpy(np.ascontiguousarray(a)).to(dev).requires_grad_(True) for a in prob["alpha"]]
+    w = np.random.default_rng(3).standard_normal(prob["b"].shape)
+    w = torch.from_numpy(w - w.mean()).to(dev)
+    times = []
+    for it in range(args.warmup + args.steps):
+        b.grad = None
+        for a in alpha:
+            a.grad = None
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        phi = S(b, alpha)
+        (w * phi).sum().backward()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(time.perf_counter() - t1)
+    out = {"seconds": sum(times) / len(times), "forward": S.last_stats, "adjoint": S.last_adjoint_stats,
+           "finite": bool(torch.isfinite(b.grad).all()) and all(bool(torch.isfinite(a.grad).all()) for a in alpha)}
+    S.fwd.close()
+    S.adj.close()
+    return out
 
 
 def cpu_baseline(prob, args, ncycles_target):
@@ -350,10 +383,18 @@ def main():
     rv = res.get("fp64", {}).get("viscosity")
     if rv:
         # viscosity() of a two-phase velocity (rho 1000/1, mu 1/0.01, dt 1e-3)
-        # to the same tolerance: FP64, weighted-Jacobi relax_viscosity
+        # to the same tolerance: FP64, red-black relax_viscosity (the grid's
+        # smoother)
         line["viscosity_ms"] = round(rv["seconds"] * 1e3, 3)
         line["viscosity_cycles"] = rv["stats"]["i"]
         line["viscosity_resa"] = rv["stats"]["resa"]
+    if main_prec == "mixed" and prob.get("alpha") is not None:
+        # forward solve + backward (adjoint solve, dJ/db, dJ/dalpha) through
+        # torch.autograd, on the same problem
+        adj = run_adjoint(bmg, torch, prob, args)
+        line["adjoint_ms"] = round(adj["seconds"] * 1e3, 3)
+        line["adjoint_cycles"] = {"forward": adj["forward"]["i"], "adjoint": adj["adjoint"]["i"],
+                                  "adjoint_resa": adj["adjoint"]["resa"], "finite_gradients": adj["finite"]}
     if not args.no_cpu_baseline:
         ncyc = res["fp64"]["stats"]["i"] if "fp64" in res else cycles
         line["cpu_baseline"], oracle = cpu_baseline(prob, args, ncyc)
