@@ -233,6 +233,25 @@ def run_precision(bmg, torch, prob, prec, args):
             "viscosity": visc}
 
 
+def copy_bandwidth(torch, nbytes, reps=20):
+    """GB/s (bytes read + bytes written) of torch's device-to-device copy of
+    an nbytes buffer, timed with HIP events on the current stream."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * (nbytes // 4) * 4 / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+
+
 def run_adjoint(bmg, torch, prob, args):
     """One differentiable solve and its backward through basilisk_amd.autograd
     (mixed precision, the problem's tolerance): phi = poisson(b, alpha),
@@ -363,6 +382,12 @@ def main():
                      "kernel": "fine-level relax sweep (%s, both colours)" % main_prec,
                      "sweep_ms": round(sweep_ms, 5), "algorithmic_bytes": sweep_bytes},
     }
+    copy_gbs = copy_bandwidth(torch, sweep_bytes // 2)
+    # the same box's device-to-device copy of half the sweep's bytes (read
+    # and written once): what plain streaming reaches here, and the sweep's
+    # share of it
+    line["roofline"]["copy_gbs"] = round(copy_gbs, 1)
+    line["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4) if copy_gbs > 0 else None
     if "fp64" in res and "mixed" in res:
         f, m = res["fp64"], res["mixed"]
         line["fp64_ms"] = round(f["seconds"] * 1e3, 3)
