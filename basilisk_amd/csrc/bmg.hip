@@ -1756,7 +1756,7 @@ struct Visc {
     const Level& lf = g->L[g->depth];
     Geo<DIM> geo{lf.n, lf.log2n};
     hipLaunchKernelGGL(k_stat_reset, dim3(1), dim3(64), 0, g->stream, g->stat);
-    hipLaunchKernelGGL((k_visc_residual<DIM>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream,
+    hipLaunchKernelGGL((k_visc_residual<DIM>), dim3(std::min(nblocks(lf.ncell), 8192u)), dim3(kBlock), 0, g->stream,
                        args(g->depth, u, lv(g->depth, VS_RES), g->visc_r), geo, g->stat);
     TRY(check_launch());
     Stat st;

@@ -241,11 +241,12 @@ __global__ void k_visc_one(ViscArgs A, Geo<DIM> g, int colour) {
 
 // residual_viscosity on the finest level: res = r - u + dt/rho S / D^2,
 // max |res| over the components to st (NaN counted)
+// (a grid-stride loop over a capped grid: one atomic max per block)
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_visc_residual(ViscArgs A, Geo<DIM> g, Stat* st) {
-  const idx_t idx = logical_block() * kBlock + threadIdx.x;
   double m = 0.;
-  if (idx < g.ncell()) {
+  const idx_t n = g.ncell(), stride = (idx_t)gridDim.x * kBlock;
+  for (idx_t idx = logical_block() * kBlock + threadIdx.x; idx < n; idx += stride) {
     int x[3];
     g.decode(idx, x);
     VAxes<DIM> X;
