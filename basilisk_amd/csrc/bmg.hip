@@ -818,12 +818,13 @@ int refresh_coef_copies(bmg_grid* g) {
     };
     auto add = [&](const double* src, void* dst, size_t n) {
       if (!n) return;
-      if (B.nseg == kConvSegs || blocks + nblocks(n) > 0xffffffffu) flush();
+      const size_t nb = (n + (size_t)kBlock * kConvItems - 1) / ((size_t)kBlock * kConvItems);
+      if (B.nseg == kConvSegs || blocks + nb > 0xffffffffu) flush();
       B.src[B.nseg] = src;
       B.dst[B.nseg] = (float*)dst;
       B.n[B.nseg] = (unsigned)n;
       B.first[B.nseg] = (unsigned)blocks;
-      blocks += nblocks(n);
+      blocks += nb;
       B.first[++B.nseg] = (unsigned)blocks;
     };
     for (auto& lv : g->L) {

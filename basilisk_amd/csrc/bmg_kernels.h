@@ -888,8 +888,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict_coef_level(CoefLevelArgs A,
 }
 
 // FP64 -> FP32 copies of many arrays in one launch: segment s covers blocks
-// first[s] .. first[s+1]-1, one element per thread
+// first[s] .. first[s+1]-1, kConvItems elements per thread (strided by the
+// block size, so each load instruction stays coalesced)
 constexpr int kConvSegs = 32;
+constexpr int kConvItems = 8;
 struct ConvBatch {
   const double* src[kConvSegs];
   float* dst[kConvSegs];
@@ -900,10 +902,26 @@ struct ConvBatch {
 
 __global__ __launch_bounds__(kBlock) void k_convert_batch(ConvBatch B) {
   const unsigned blk = blockIdx.x;
+  // the segment of this block: the number of later segments starting at or
+  // before it (independent loads of the increasing starts, not a chain)
   int s = 0;
-  while (s + 1 < B.nseg && blk >= B.first[s + 1]) s++;
-  const size_t i = (size_t)(blk - B.first[s]) * kBlock + threadIdx.x;
-  if (i < B.n[s]) B.dst[s][i] = (float)B.src[s][i];
+#pragma unroll
+  for (int q = 1; q < kConvSegs; q++) s += (q < B.nseg && blk >= B.first[q]) ? 1 : 0;
+  const size_t i0 = (size_t)(blk - B.first[s]) * (kBlock * kConvItems) + threadIdx.x;
+  const double* __restrict__ src = B.src[s];
+  float* __restrict__ dst = B.dst[s];
+  const size_t n = B.n[s];
+  double v[kConvItems];
+#pragma unroll
+  for (int k = 0; k < kConvItems; k++) {
+    const size_t i = i0 + (size_t)k * kBlock;
+    v[k] = i < n ? src[i] : 0.;
+  }
+#pragma unroll
+  for (int k = 0; k < kConvItems; k++) {
+    const size_t i = i0 + (size_t)k * kBlock;
+    if (i < n) dst[i] = (float)v[k];
+  }
 }
 
 // wall faces of component d copied from the first plane of low faces (the
