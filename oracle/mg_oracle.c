@@ -30,6 +30,18 @@
  * colours).  It is the order-independent smoother that the GPU path uses by
  * default, so the oracle defines its exact semantics.
  *
+ * Planted mistakes: built with -DORC_PLANT=k (k > 0), the oracle makes one
+ * plausible mistake on purpose, so that the tests can show that their
+ * mathematical checks catch it (tests/test_oracle_pins.py builds these
+ * variants into a temporary directory; the normal build has ORC_PLANT 0):
+ *   1  Jacobi update (2a + c)/3 instead of (a + 2c)/3 (poisson.h:325-328)
+ *   2  red-black colours swapped (black cells first)
+ *   3  no ghost fill of da between the two colours
+ *   4  Neumann ghost s - Delta g instead of Delta g + s (common.h:959)
+ *   5  nrelax rule with its two thresholds swapped (poisson.h:198-204)
+ *   6  relax pairs alpha.x[] with a[1] and alpha.x[1] with a[-1]
+ *   7  residual with +lambda a instead of -lambda a (poisson.h:376)
+ *
  * Build once per dimension: -DDIM=1, 2 or 3, with -ffp-contract=off so that
  * no multiply-add is fused (the reference C is compiled without FMA).
  * Built with -fopenmp, the loops are `omp for schedule(static)` over the
@@ -51,6 +63,11 @@
 #endif
 
 #define GHOSTS 2
+
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(k) (ORC_PLANT == (k))
 
 enum { F_A, F_B, F_RES, F_DA, F_LAMBDA, F_C, F_AX };
 #define NV (F_AX + DIM)
@@ -253,7 +270,7 @@ static double bc_apply(orc_grid *g, int b, int l, int homogeneous, int use_bc, d
   double expr = vals ? vals[(size_t)u * (DIM == 3 ? m : 1) + v] : 0.;
   switch (kind) {
   case BC_DIRICHLET: return homogeneous ? -s : 2. * (expr) - s;
-  case BC_NEUMANN: return homogeneous ? s : delta_l(g, l) * (expr) + s;
+  case BC_NEUMANN: return homogeneous ? s : PLANT(4) ? s - delta_l(g, l) * (expr) : delta_l(g, l) * (expr) + s;
   default: return s;
   }
 }
@@ -346,7 +363,7 @@ static void box_boundary(orc_grid *g, int f, int l, int homogeneous, int use_bc)
         double expr = vals ? vals[(size_t)u * e2 + v] : 0.;
         switch (kind) {
         case BC_DIRICHLET: r = homogeneous ? -s : 2. * (expr) - s; break;
-        case BC_NEUMANN: r = homogeneous ? s : D * (expr) + s; break;
+        case BC_NEUMANN: r = homogeneous ? s : PLANT(4) ? s - D * (expr) : D * (expr) + s; break;
         default: r = s; break;
         }
         gh[f] = r;
@@ -616,7 +633,8 @@ static inline double relax_value(const orc_grid *g, int l, const double *r, int 
   for (int c = 0; c < DIM; c++) {
     ptrdiff_t sd = dstride(l, c);
     double ah = alpha_hi(g, r, sd, c), al = alpha_lo(g, r, c);
-    n += ah * r[sd * NV + src] + al * r[-sd * NV + src];
+    if (PLANT(6)) n += al * r[sd * NV + src] + ah * r[-sd * NV + src];
+    else n += ah * r[sd * NV + src] + al * r[-sd * NV + src];
     d += ah + al;
   }
   return n / d;
@@ -649,13 +667,13 @@ void orc_relax(orc_grid *g, int l) {
     END_LOOP
     LOOP_LEVEL(l, n)
       double *r = REC(g, l, i, j, k);
-      r[F_DA] = (r[F_DA] + 2. * r[F_C]) / 3.;
+      r[F_DA] = PLANT(1) ? (2. * r[F_DA] + r[F_C]) / 3. : (r[F_DA] + 2. * r[F_C]) / 3.;
     END_LOOP
   } else {
     for (int colour = 0; colour < 2; colour++) {
-      if (colour == 1) boundary_level(g, F_DA, l, 1, 1);
+      if (colour == 1 && !PLANT(3)) boundary_level(g, F_DA, l, 1, 1);
       LOOP_LEVEL(l, n)
-        if (((i + j + k) & 1) == colour) {
+        if (((i + j + k) & 1) == (PLANT(2) ? 1 - colour : colour)) {
           double *r = REC(g, l, i, j, k);
           r[F_DA] = relax_value(g, l, r, F_DA, D2);
         }
@@ -688,7 +706,7 @@ double orc_residual(orc_grid *g) {
       for (int k = GHOSTS; k < n + GHOSTS; k++) {
 #endif
         double *r = REC(g, l, i, j, k);
-        double res = r[F_B] - lambda_at(g, r) * r[F_A];
+        double res = PLANT(7) ? r[F_B] + lambda_at(g, r) * r[F_A] : r[F_B] - lambda_at(g, r) * r[F_A];
         for (int c = 0; c < DIM; c++) {
           ptrdiff_t sd = dstride(l, c);
           double a0 = r[F_A], am = r[-sd * NV + F_A], ap = r[sd * NV + F_A];
@@ -748,6 +766,18 @@ double orc_sum_b(orc_grid *g) {
   return sum;
 }
 
+/* the nrelax adaptation of mg_solve after each cycle, poisson.h:198-204:
+   while not converged, one more sweep when the cycle reduced the residual by
+   less than 1.2 (at most 100), one fewer when by more than 10 (at least 2) */
+int orc_nrelax_update(double resb, double resa, double tolerance, int nrelax) {
+  double lo = PLANT(5) ? 10 : 1.2, hi = PLANT(5) ? 1.2 : 10;
+  if (resa > tolerance) {
+    if (resb / resa < lo && nrelax < 100) nrelax++;
+    else if (resb / resa > hi && nrelax > 2) nrelax--;
+  }
+  return nrelax;
+}
+
 int orc_mg_solve(orc_grid *g, double tolerance, int nrelax, int minlevel, orc_mgstats *out) {
   orc_mgstats s;
   memset(&s, 0, sizeof s);
@@ -759,10 +789,7 @@ int orc_mg_solve(orc_grid *g, double tolerance, int nrelax, int minlevel, orc_mg
   for (s.i = 0; s.i < g->NITERMAX && (s.i < g->NITERMIN || s.resa > tolerance); s.i++) {
     orc_mg_cycle(g, s.nrelax, minlevel, g->depth);
     s.resa = orc_residual(g);
-    if (s.resa > tolerance) {
-      if (resb / s.resa < 1.2 && s.nrelax < 100) s.nrelax++;
-      else if (resb / s.resa > 10 && s.nrelax > 2) s.nrelax--;
-    }
+    s.nrelax = orc_nrelax_update(resb, s.resa, tolerance, s.nrelax);
     resb = s.resa;
   }
   s.minlevel = minlevel;

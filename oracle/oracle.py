@@ -38,13 +38,33 @@ def build():
 _libs = {}
 
 
-def _lib(dim, omp=False):
-    key = (dim, omp)
+def _planted_path(dim, plant):
+    """Build mg_oracle.c with -DORC_PLANT=plant (one deliberate mistake, see
+    the header of mg_oracle.c) into a temporary directory, for the tests
+    that show their checks catch it."""
+    import tempfile
+    out = os.path.join(tempfile.gettempdir(), "bmg_orc_plant_%d" % os.getpid())
+    os.makedirs(out, exist_ok=True)
+    path = os.path.join(out, "liborc_mg%d_plant%d.so" % (dim, plant))
+    if not os.path.exists(path):
+        src = os.path.join(_HERE, "mg_oracle.c")
+        subprocess.run(["gcc", "-O2", "-std=gnu99", "-fPIC", "-shared", "-ffp-contract=off",
+                        "-Wno-unknown-pragmas", "-DDIM=%d" % dim, "-DORC_PLANT=%d" % plant,
+                        "-o", path + ".tmp", src, "-lm"], check=True)
+        os.replace(path + ".tmp", path)
+    return path
+
+
+def _lib(dim, omp=False, plant=0):
+    key = (dim, omp, plant)
     if key not in _libs:
-        name = "liborc_mg%d%s.so" % (dim, "_omp" if omp else "")
-        path = os.path.join(_HERE, name)
-        if not os.path.exists(path):
-            build()
+        if plant:
+            path = _planted_path(dim, plant)
+        else:
+            name = "liborc_mg%d%s.so" % (dim, "_omp" if omp else "")
+            path = os.path.join(_HERE, name)
+            if not os.path.exists(path):
+                build()
         lib = ctypes.CDLL(path)
         P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         dp = ctypes.POINTER(ctypes.c_double)
@@ -75,6 +95,7 @@ def _lib(dim, omp=False):
             "orc_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "orc_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "orc_project": (I, [P, ctypes.POINTER(dp), D, I, ctypes.POINTER(MGStats)]),
+            "orc_nrelax_update": (I, [D, D, D, I]),
             "orc_set_threads": (None, [I]),
             "orc_max_threads": (I, []),
             "orc_dimension": (I, []),
@@ -96,9 +117,9 @@ class OracleGrid:
     """A uniform multigrid hierarchy of `depth` levels in `dim` dimensions,
     holding the fields a, b, res, da, lambda and the face vector alpha."""
 
-    def __init__(self, dim, depth, omp=False, L0=1.0, origin=(0.0, 0.0, 0.0)):
+    def __init__(self, dim, depth, omp=False, L0=1.0, origin=(0.0, 0.0, 0.0), plant=0):
         self.dim, self.depth = dim, depth
-        self.lib = _lib(dim, omp)
+        self.lib = _lib(dim, omp, plant)
         self.g = self.lib.orc_new(depth)
         self.lib.orc_set_geometry(self.g, L0, *origin)
         self.L0, self.origin = L0, tuple(origin)
@@ -280,6 +301,11 @@ class OracleGrid:
         s = MGStats()
         self.lib.orc_poisson(self.g, tolerance, nrelax, minlevel, ctypes.byref(s))
         return s.as_dict()
+
+
+def nrelax_update(resb, resa, tolerance, nrelax, plant=0):
+    """The oracle's nrelax adaptation after one cycle (poisson.h:198-204)."""
+    return _lib(1, False, plant).orc_nrelax_update(resb, resa, tolerance, nrelax)
 
 
 def set_threads(n, dim=3):

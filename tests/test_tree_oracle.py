@@ -36,12 +36,15 @@ def test_circle_3d_ref(depth):
     assert tree_cases.circle_lines(t, depth) == _ref_block("circle.3D.ref", depth)
 
 
-def test_full_tree_matches_uniform_oracle():
+@pytest.mark.parametrize("smoother", ["gs", "jacobi", "rbgs"])
+def test_full_tree_matches_uniform_oracle(smoother):
     """A tree refined to one level is `full`: the tree oracle then equals the
     uniform-grid oracle in tree mode bit for bit (two restatements of
-    test/poisson.c, poisson.ref)."""
+    test/poisson.c, poisson.ref), for each smoother, so the Fourier and
+    fixed-point pins of the uniform oracle's sweeps carry over."""
     depth = 7
     t = OracleTree(2, 1.0, (-0.5, -0.5, -0.5))
+    t.set_smoother(smoother)
     t.refine(lambda x, y, z, lev: lev < depth)
     assert t.nleaves() == 4 ** depth
 
@@ -55,6 +58,7 @@ def test_full_tree_matches_uniform_oracle():
         t.set_bc_function(s, cases.BC_DIRICHLET, f)
     g = OracleGrid(2, depth, origin=(-0.5, -0.5, -0.5))
     g.set_tree()
+    g.set_smoother(smoother)
     g.set_bc_function(cases.BC_DIRICHLET, f)
     X, Y, _ = cases.mesh(g)
     g.set_b(-18. * cases.PI ** 2 * f(X, Y, 0).reshape(g.shape()))
