@@ -174,6 +174,18 @@ def _host_ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def _device_ptr(t, numel, what):
+    """Pointer of a contiguous float64 CUDA tensor of `numel` elements; any
+    other object (a CPU tensor, another dtype, a strided view, the wrong
+    size) raises ValueError before the library could write through it."""
+    if not (hasattr(t, "data_ptr") and getattr(t, "is_cuda", False)):
+        raise ValueError("%s must be a CUDA tensor" % what)
+    if str(t.dtype) != "torch.float64" or not t.is_contiguous() or t.numel() != numel:
+        raise ValueError("%s must be a contiguous float64 tensor of %d elements (got %s, %s, %d)"
+                         % (what, numel, t.dtype, "contiguous" if t.is_contiguous() else "strided", t.numel()))
+    return ctypes.c_void_p(t.data_ptr())
+
+
 class Grid:
     """A uniform multigrid hierarchy (levels 0..depth) resident on the GPU.
 
@@ -265,8 +277,9 @@ class Grid:
 
     def _get(self, field, level, shape, out=None):
         level = self.depth if level is None else level
-        if out is not None and hasattr(out, "data_ptr"):
-            _check(self.lib.bmg_get_field(self._g, field, level, ctypes.c_void_p(out.data_ptr()), 1))
+        if out is not None:
+            ptr = _device_ptr(out, int(np.prod(shape)), "out")
+            _check(self.lib.bmg_get_field(self._g, field, level, ptr, 1))
             return out
         a = np.empty(shape)
         _check(self.lib.bmg_get_field(self._g, field, level, _host_ptr(a), 0))
@@ -393,15 +406,16 @@ class Grid:
         CUDA tensors (psi and preallocated outputs) everything stays on the
         device; with numpy input new host arrays are returned."""
         if hasattr(psi, "data_ptr") and getattr(psi, "is_cuda", False):
-            import torch
-            torch.cuda.current_stream().synchronize()
+            ncell = int(np.prod(self.shape()))
+            pp = _device_ptr(psi, ncell, "psi")
             ptrs = [None, None, None]
             for c in range(self.dim):
                 if out_alpha is not None and out_alpha[c] is not None:
-                    ptrs[c] = ctypes.c_void_p(out_alpha[c].data_ptr())
-            lam = ctypes.c_void_p(out_lambda.data_ptr()) if out_lambda is not None else None
-            _check(self.lib.bmg_coef_gradient(self._g, ctypes.c_void_p(psi.data_ptr()), ptrs[0], ptrs[1],
-                                              ptrs[2], lam, 1))
+                    ptrs[c] = _device_ptr(out_alpha[c], int(np.prod(self.face_shape(c))), "out_alpha[%d]" % c)
+            lam = _device_ptr(out_lambda, ncell, "out_lambda") if out_lambda is not None else None
+            import torch
+            torch.cuda.current_stream().synchronize()
+            _check(self.lib.bmg_coef_gradient(self._g, pp, ptrs[0], ptrs[1], ptrs[2], lam, 1))
             return out_alpha, out_lambda
         p = np.ascontiguousarray(psi, dtype=np.float64)
         if p.size != int(np.prod(self.shape())):
@@ -447,7 +461,8 @@ class Grid:
         """viscosity(), viscosity.h:163-178: the implicit viscous step
         u - dt/rho div(2 mu D(u)) = u_old for the cell components `u` (one
         (n,)*dim array per component, updated in place), with relax_viscosity
-        in its JACOBI variant; mu: one face array per component (n+1 along
+        run by the grid's smoother (red-black by default, the reference's
+        JACOBI variant with smoother="jacobi"); mu: one face array per component (n+1 along
         it) or a constant; rho: a cell array or a constant; sigma: (dim, 2
         dim) wall factors (default: -1 for the normal component, +1 for the
         tangential ones).  torch CUDA tensors stay on the device, numpy
@@ -709,8 +724,9 @@ class Tree:
         _check(self.lib.bmg_tree_set_leaf_field(self._t, field, _host_ptr(a), 0))
 
     def _get(self, field, out=None):
-        if out is not None and hasattr(out, "data_ptr"):
-            _check(self.lib.bmg_tree_get_leaf_field(self._t, field, ctypes.c_void_p(out.data_ptr()), 1))
+        if out is not None:
+            ptr = _device_ptr(out, self.nleaves(), "out")
+            _check(self.lib.bmg_tree_get_leaf_field(self._t, field, ptr, 1))
             return out
         a = np.empty(self.nleaves())
         _check(self.lib.bmg_tree_get_leaf_field(self._t, field, _host_ptr(a), 0))

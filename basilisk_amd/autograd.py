@@ -19,6 +19,13 @@ psi, then
 (bmg_coef_gradient).  The gradients are FP64 whatever the solver precision.
 With ``recompute=True`` the forward solution is not kept between forward and
 backward; the backward solves the forward problem again first.
+
+With no Dirichlet side and lambda = 0 the operator is singular (phi has a
+free constant): the adjoint right-hand side then has its mean removed before
+the adjoint solve, which is exact when J does not depend on that constant
+(then sum(dJ/dphi) = 0 already) and otherwise differentiates J along
+mean-free perturbations of phi only.  An adjoint solve that does not reach
+the tolerance raises RuntimeError instead of returning a wrong gradient.
 """
 import torch
 
@@ -41,11 +48,13 @@ class PoissonSolver:
         self.adj = bmg.Grid(dim, depth, L0, origin, precision=precision, smoother=smoother)
         for g in (self.fwd, self.adj):
             g.set_niter(nitermax, 1)
+        self.has_dirichlet = False
         for side, (kind, values) in (bc or {}).items():
             if kind == bmg.BC_PERIODIC:
                 raise ValueError("periodic boundaries are not differentiable here")
             self.fwd.set_bc(side, kind, values)
             self.adj.set_bc(side, kind, None)
+            self.has_dirichlet |= kind == bmg.BC_DIRICHLET
         self.shape = self.fwd.shape()
         self.last_stats = None  # mgstats of the last forward solve
         self.last_adjoint_stats = None  # ... and of the last adjoint solve
@@ -102,7 +111,15 @@ class _PoissonFunction(torch.autograd.Function):
         alpha = saved if ctx.has_alpha else None
         if phi is None:
             phi, _ = solver._solve(solver.fwd, b, alpha, lam)
-        psi, solver.last_adjoint_stats = solver._solve(solver.adj, gphi.contiguous(), alpha, lam)
+        rhs = gphi.contiguous()
+        singular = not solver.has_dirichlet and (
+            (ctx.lam_const and not lam) or (not ctx.lam_const and not bool(torch.any(lam != 0))))
+        if singular:
+            rhs = rhs - rhs.mean()
+        psi, solver.last_adjoint_stats = solver._solve(solver.adj, rhs, alpha, lam)
+        if not solver.last_adjoint_stats["resa"] <= solver.tolerance:
+            raise RuntimeError("adjoint solve did not reach the tolerance %g: %s"
+                               % (solver.tolerance, solver.last_adjoint_stats))
         g = solver.fwd
         g.set_a(phi.detach().contiguous())
         need_a = ctx.has_alpha and any(ctx.needs_input_grad[3:])

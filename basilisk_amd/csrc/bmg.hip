@@ -1768,7 +1768,8 @@ struct Visc {
     return BMG_OK;
   }
 
-  // mg_cycle (poisson.h:32-90) with minlevel 0 and relax_viscosity (JACOBI)
+  // mg_cycle (poisson.h:32-90) with minlevel 0 and relax_viscosity in the
+  // grid's smoother: red-black (default) or the reference's JACOBI variant
   int cycle(int nrelax) {
     const int D = g->depth;
     for (int l = D - 1; l >= 0; l--) {

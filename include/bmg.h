@@ -159,7 +159,8 @@ int bmg_project(bmg_grid *g, double *uf_x, double *uf_y, double *uf_z, double dt
    mg_solve's loop (poisson.h:139-235: minlevel 0, tolerance TOLERANCE,
    NITERMAX/NITERMIN, nrelax = 0 meaning 4) around residual_viscosity (the
    multigrid grid's "naive" form, viscosity.h:134-157) and relax_viscosity
-   (viscosity.h:27-96) in its JACOBI variant, non-AXI, after restriction
+   (viscosity.h:27-96) run by the grid's smoother (red-black by default,
+   the reference's JACOBI variant with BMG_RELAX_JACOBI), non-AXI, after restriction
    ({mu, rho}).  u_x/u_y/u_z: the cell components (n^dim, C order), updated
    in place; mu_x/y/z: face fields in the layout of bmg_set_field (n+1 along
    their component), or all NULL for the constant mu_c; rho: a cell field,

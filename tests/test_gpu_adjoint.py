@@ -118,3 +118,30 @@ def test_adjoint_workload_finite_differences():
     err = np.abs(fds - wants).max() / np.abs(wants).max()
     print("FD error %.3g of the largest directional derivative %.3g" % (err, np.abs(wants).max()), flush=True)
     assert err <= 1e-6
+
+
+def test_neumann_lambda0_adjoint_removes_mean():
+    """Symmetry walls and lambda = 0: phi has a free constant, so the
+    adjoint operator is singular.  J = sum(w phi) with weights w whose mean
+    is not zero: the backward pass removes the mean of dJ/dphi, so the
+    gradients equal those for w - mean(w), and the adjoint solve converges
+    instead of stalling at NITERMAX."""
+    print("adjoint neumann lambda0", flush=True)
+    dim, depth = 2, 6
+    n = 1 << depth
+    rng = np.random.default_rng(31)
+    b0 = rng.standard_normal((n, n))
+    b0 -= b0.mean()
+    alpha0 = [np.exp(0.5 * rng.standard_normal((n + 1, n) if c == 0 else (n, n + 1))) for c in range(2)]
+    w = rng.standard_normal((n, n)) + 0.7
+    grads = []
+    for weights in (w, w - w.mean()):
+        solver = PoissonSolver(dim, depth, precision="fp64", tolerance=1e-10, nitermax=200)
+        b = _torch(b0)
+        alpha = [_torch(a) for a in alpha0]
+        phi = solver(b, alpha)
+        (torch.tensor(weights, device=DEV) * phi).sum().backward()
+        assert solver.last_adjoint_stats["resa"] <= 1e-10
+        grads.append([b.grad.cpu().numpy()] + [a.grad.cpu().numpy() for a in alpha])
+    for g1, g2 in zip(*grads):
+        assert np.abs(g1 - g2).max() <= 1e-12 * np.abs(g2).max()
