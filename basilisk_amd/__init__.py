@@ -50,6 +50,7 @@ EXPORTS = [
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
     "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
     "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
+    "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
 ]
 
 
@@ -120,6 +121,8 @@ def library():
         "bmg_profile_enable": (I, [P, I]),
         "bmg_profile_get": (I, [P, I, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_long)]),
         "bmg_poisson_view": (I, [P, P, P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_view_configure": (I, [I, I]),
+        "bmg_view_release": (I, []),
         "bmg_tree_new": (I, [ctypes.POINTER(P), I, D, D, D, D]),
         "bmg_tree_free": (I, [P]),
         "bmg_tree_info": (I, [P, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_long),
@@ -145,6 +148,9 @@ def library():
         "bmg_tree_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_tree_synchronize": (I, [P]),
+        "bmg_tree_profile_enable": (I, [P, I]),
+        "bmg_tree_profile_get": (I, [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_long),
+                                     ctypes.POINTER(ctypes.c_long)]),
         "bmg_tree_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "bmg_tree_dump": (I, [P, ctypes.c_char_p, dp, dp, I, D, I]),
         "bmg_tree_nonleaf": (I, [P, ctypes.POINTER(ctypes.c_long), dp, dp]),
@@ -192,7 +198,9 @@ class Grid:
     ``dim``: 1, 2 or 3; ``depth``: finest level (2^depth cells per
     dimension); ``L0``/``origin``: the box, like size() and origin()
     (common.h:857-866).  ``precision``: "fp64" or "mixed";
-    ``smoother``: "rbgs" (default) or "jacobi"."""
+    ``smoother``: "rbgs" (default), "jacobi" (the reference's -DJACOBI
+    relax) or "gs" (the reference's default in-place Gauss-Seidel, i-major
+    order, run by hyperplanes; multigrid grids only)."""
 
     def __init__(self, dim, depth, L0=1.0, origin=(0.0, 0.0, 0.0),
                  precision="fp64", smoother="rbgs", kind="multigrid"):
@@ -867,3 +875,15 @@ class Tree:
 
     def synchronize(self):
         _check(self.lib.bmg_tree_synchronize(self._t))
+
+    def profile(self, enable):
+        """Time every relax launch of more than 2048 cells with HIP events
+        (the cycle runs without its graph meanwhile); enabling resets."""
+        _check(self.lib.bmg_tree_profile_enable(self._t, 1 if enable else 0))
+
+    def profile_get(self):
+        """(total ms, launches, cells relaxed) of the relax launches timed
+        since profiling was enabled."""
+        ms, n, cells = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
+        _check(self.lib.bmg_tree_profile_get(self._t, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(cells)))
+        return ms.value, n.value, cells.value

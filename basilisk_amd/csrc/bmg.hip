@@ -230,8 +230,10 @@ int alloc_levels(bmg_grid* g) {
   return BMG_OK;
 }
 
+// the second correction buffer of the ping-pong smoothers (Jacobi and the
+// hyperplane Gauss-Seidel)
 int ensure_jacobi_buffers(bmg_grid* g) {
-  if (g->smoother != BMG_RELAX_JACOBI) return BMG_OK;
+  if (g->smoother != BMG_RELAX_JACOBI && g->smoother != BMG_RELAX_GS) return BMG_OK;
   size_t es = esize(g);
   for (auto& lv : g->L)
     if (!lv.da[1]) TRY(dalloc(&lv.da[1], lv.ncell * es));
@@ -293,7 +295,7 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // relax kernel of a level: two-launch in-place red-black (tiny levels),
 // one-launch fused red-black, single-pass streaming red-black (3D,
 // n >= 128) or Jacobi
-enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI };
+enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI, RK_GS };
 
 // a cell-count threshold with an environment override (tuning knob)
 size_t size_knob(const char* name, size_t dflt) {
@@ -336,6 +338,7 @@ int stream3_min() {
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
   if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
+  if (g->smoother == BMG_RELAX_GS) return RK_GS;
   // the column-streamed single-pass kernel: 3D levels of 128^3 cells and
   // more (fewer planes give too few blocks), walls in every direction
   const bool walls = !g->periodic[0] && !g->periodic[1] && !g->periodic[2];
@@ -400,6 +403,25 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
                            (const T*)lv.da[src], (T*)lv.da[src ^ 1], (const T*)lv.res, k, geo, bc, D2);
         out = src ^ 1;
         break;
+      case RK_GS: {
+        // hyperplanes i+j+k = s in order: one workgroup for the whole sweep
+        // while a hyperplane has at most 4096 cells (BMG_GS_BLOCK_MAX
+        // overrides), else one launch each
+        const T* din = (const T*)lv.da[src];
+        T* dout = (T*)lv.da[src ^ 1];
+        const int smax = DIM * (lv.n - 1);
+        if (powd(lv.n, DIM - 1) <= size_knob("BMG_GS_BLOCK_MAX", 4096)) {
+          hipLaunchKernelGGL((k_gs_block<DIM, T>), dim3(1), dim3(1024), 0, g->stream, din, dout,
+                             (const T*)lv.res, k, geo, bc, D2);
+        } else {
+          const size_t rows = DIM == 3 ? (size_t)lv.n * lv.n : (size_t)lv.n;
+          for (int s = 0; s <= smax; s++)
+            hipLaunchKernelGGL((k_gs_plane<DIM, T>), dim3(nblocks(rows)), dim3(kBlock), 0, g->stream, din, dout,
+                               (const T*)lv.res, k, geo, bc, D2, s);
+        }
+        out = src ^ 1;
+        break;
+      }
       case RK_FUSED_RB:
         hipLaunchKernelGGL((k_relax_rb_fused<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
                            (const T*)lv.da[src], (T*)lv.da[src ^ 1], (const T*)lv.res, k, geo, bc, D2);
@@ -508,8 +530,7 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
 // the public single relax (bmg_relax): result kept in da[0]
 template <int DIM, typename T>
 int relax_level(bmg_grid* g, int l) {
-  if (g->smoother != BMG_RELAX_JACOBI && g->smoother != BMG_RELAX_RBGS)
-    return fail(BMG_ERR_UNSUPPORTED, "lexicographic Gauss-Seidel is not available on the GPU path");
+  TRY(ensure_jacobi_buffers(g));
   int rc = BMG_OK;
   int out = sweep<DIM, T>(g, l, 0, &rc);
   if (out == 1) std::swap(g->L[l].da[0], g->L[l].da[1]);
@@ -539,6 +560,7 @@ int coarse_block(bmg_grid* g, int nrelax, int minlevel, int top) {
   A.top = top;
   A.nrelax = nrelax;
   A.jacobi = g->smoother == BMG_RELAX_JACOBI;
+  A.gs = g->smoother == BMG_RELAX_GS;
   ProfScope ps(g, PROF_RELAX_COARSE);
   // threads of the workgroup (BMG_COARSE_BLOCK: a multiple of 64 up to 1024)
   size_t nt = size_knob("BMG_COARSE_BLOCK", 1024);
@@ -1211,6 +1233,10 @@ int bmg_set_grid_kind(bmg_grid* g, int kind) {
   if (kind != BMG_GRID_MULTIGRID && kind != BMG_GRID_TREE) return fail(BMG_ERR_ARG, "unknown grid kind %d", kind);
   if (kind == BMG_GRID_TREE && (g->periodic[0] || g->periodic[1] || g->periodic[2]))
     return fail(BMG_ERR_UNSUPPORTED, "periodic trees are not supported");
+  // a full tree's Gauss-Seidel follows foreach_cell's Morton order
+  // (grid/foreach_cell.h:66-123), not the multigrid's i-major one
+  if (kind == BMG_GRID_TREE && g->smoother == BMG_RELAX_GS)
+    return fail(BMG_ERR_UNSUPPORTED, "Gauss-Seidel in the Morton order of a full tree is not built; use JACOBI or RBGS");
   g->grid_kind = kind;
   drop_graphs(g);
   return BMG_OK;
@@ -1218,9 +1244,10 @@ int bmg_set_grid_kind(bmg_grid* g, int kind) {
 
 int bmg_set_smoother(bmg_grid* g, int smoother) {
   TRY(check_grid(g));
-  if (smoother == BMG_RELAX_GS)
-    return fail(BMG_ERR_UNSUPPORTED, "lexicographic Gauss-Seidel is not available on the GPU path; use JACOBI or RBGS");
-  if (smoother != BMG_RELAX_JACOBI && smoother != BMG_RELAX_RBGS) return fail(BMG_ERR_ARG, "unknown smoother %d", smoother);
+  if (smoother != BMG_RELAX_GS && smoother != BMG_RELAX_JACOBI && smoother != BMG_RELAX_RBGS)
+    return fail(BMG_ERR_ARG, "unknown smoother %d", smoother);
+  if (smoother == BMG_RELAX_GS && g->grid_kind == BMG_GRID_TREE)
+    return fail(BMG_ERR_UNSUPPORTED, "Gauss-Seidel in the Morton order of a full tree is not built; use JACOBI or RBGS");
   g->smoother = smoother;
   drop_graphs(g);
   return ensure_jacobi_buffers(g);
@@ -1522,11 +1549,13 @@ int bmg_profile_get(bmg_grid* g, int kind, double* total_ms, long* count) {
 }
 
 // --------------------------------------------------------------------------
-// poisson() on the reference's own storage.  The view is read once per call
-// (a, b, alpha, lambda on the finest level), the whole solve runs on the
-// GPU, and a (and res when asked for) is written back once.  Ghost cells of
-// the host storage are not written: the caller marks `a` dirty so that the
-// reference refills them on its next stencil.
+// poisson() on the reference's own storage.  The finest level's record
+// array is uploaded once per call and a, b, alpha and lambda are gathered
+// from it on the device; the whole solve runs on the GPU; a (and res when
+// asked for) is scattered into the device copy, which goes back to the host
+// in one copy (ghost cells and the other fields come back as they were: the
+// caller marks `a` dirty so that the reference refills its ghosts on the
+// next stencil).
 
 }  // extern "C"
 
@@ -1536,29 +1565,42 @@ struct ViewCache {
   bmg_grid* g = nullptr;
   int dim = -1, depth = -1;
   double L0 = 0., X0 = 0., Y0 = 0., Z0 = 0.;
+  int precision = BMG_FP64, smoother = BMG_RELAX_RBGS;  // bmg_view_configure
+  // device copy of the finest level's record array, and a dense box of
+  // one field (both grown on demand, kept across calls)
+  void* rec = nullptr;
+  size_t rec_bytes = 0;
+  double* box = nullptr;
+  size_t box_n = 0;
 };
 ViewCache g_view_cache;
 
-inline const char* view_cell(const bmg_multigrid_view* v, int l, const int ijk[3]) {
-  size_t m = (size_t)(1 << l) + 4, idx;
-  if (v->dimension == 3) idx = ((size_t)(ijk[0] + 2) * m + (size_t)(ijk[1] + 2)) * m + (size_t)(ijk[2] + 2);
-  else if (v->dimension == 2) idx = (size_t)(ijk[0] + 2) * m + (size_t)(ijk[1] + 2);
-  else idx = (size_t)(ijk[0] + 2);
-  return v->d[l] + idx * v->datasize;
+// a dense box of one field to or from the device record copy
+int view_move(bmg_grid* g, ViewCache& vc, const bmg_multigrid_view* v, int h, const int ext[3], bool gather) {
+  ViewBox B;
+  B.nrec = v->datasize / sizeof(double);
+  B.h = h;
+  B.m = (1 << v->depth) + 4;
+  size_t total = 1;
+  for (int d = 0; d < 3; d++) {
+    B.e[d] = d < v->dimension ? ext[d] : 1;
+    total *= (size_t)B.e[d];
+  }
+  const unsigned nb = (unsigned)((total + kBlock - 1) / kBlock);
+  double* rec = (double*)vc.rec;
+  if (gather) {
+    if (v->dimension == 3) hipLaunchKernelGGL((k_view_gather<3>), dim3(nb), dim3(kBlock), 0, g->stream, rec, B, vc.box);
+    else if (v->dimension == 2) hipLaunchKernelGGL((k_view_gather<2>), dim3(nb), dim3(kBlock), 0, g->stream, rec, B, vc.box);
+    else hipLaunchKernelGGL((k_view_gather<1>), dim3(nb), dim3(kBlock), 0, g->stream, rec, B, vc.box);
+  } else {
+    if (v->dimension == 3) hipLaunchKernelGGL((k_view_scatter<3>), dim3(nb), dim3(kBlock), 0, g->stream, rec, B, vc.box);
+    else if (v->dimension == 2) hipLaunchKernelGGL((k_view_scatter<2>), dim3(nb), dim3(kBlock), 0, g->stream, rec, B, vc.box);
+    else hipLaunchKernelGGL((k_view_scatter<1>), dim3(nb), dim3(kBlock), 0, g->stream, rec, B, vc.box);
+  }
+  return check_launch();
 }
 
-// visit the cells of a box with extents ext (C order)
-template <typename F>
-void for_box(int dim, const int ext[3], F f) {
-  int e0 = ext[0], e1 = dim >= 2 ? ext[1] : 1, e2 = dim >= 3 ? ext[2] : 1;
-  size_t p = 0;
-  for (int i = 0; i < e0; i++)
-    for (int j = 0; j < e1; j++)
-      for (int k = 0; k < e2; k++, p++) {
-        int c[3] = {i, j, k};
-        f(c, p);
-      }
-}
+
 
 }  // namespace
 
@@ -1580,17 +1622,31 @@ int bmg_poisson_view(const bmg_multigrid_view* v, const bmg_poisson_args* p, con
     vc.dim = dim; vc.depth = depth; vc.L0 = v->L0; vc.X0 = v->X0; vc.Y0 = v->Y0; vc.Z0 = v->Z0;
   }
   bmg_grid* g = vc.g;
-  int n = 1 << depth;
+  if (g->precision != vc.precision) TRY(bmg_set_precision(g, vc.precision));
+  if (g->smoother != vc.smoother) TRY(bmg_set_smoother(g, vc.smoother));
+  const int n = 1 << depth;
   int ext[3] = {n, n, n};
-  size_t ncell = powd(n, dim);
-  std::vector<double> buf(ncell);
-  auto gather = [&](int h, double* out, const int* e) {
-    for_box(dim, e, [&](const int c[3], size_t q) { out[q] = ((const double*)view_cell(v, depth, c))[h]; });
-  };
-  gather(p->a, buf.data(), ext);
-  TRY(bmg_set_field(g, BMG_F_A, depth, buf.data(), 0));
-  gather(p->b, buf.data(), ext);
-  TRY(bmg_set_field(g, BMG_F_B, depth, buf.data(), 0));
+  // the finest level's records, uploaded once; every field the solve reads
+  // is gathered from them on the device
+  const size_t rec_bytes = powd((size_t)n + 4, dim) * v->datasize;
+  const size_t box_n = powd(n, dim - 1) * (size_t)(n + 1);
+  if (vc.rec_bytes < rec_bytes) {
+    dfree(vc.rec);
+    vc.rec_bytes = 0;
+    TRY(dalloc(&vc.rec, rec_bytes));
+    vc.rec_bytes = rec_bytes;
+  }
+  if (vc.box_n < box_n) {
+    dfreet(vc.box);
+    vc.box_n = 0;
+    TRY(dalloc((void**)&vc.box, box_n * sizeof(double)));
+    vc.box_n = box_n;
+  }
+  HIPCHECK(hipMemcpyAsync(vc.rec, v->d[depth], rec_bytes, hipMemcpyHostToDevice, g->stream));
+  TRY(view_move(g, vc, v, p->a, ext, true));
+  TRY(bmg_set_field(g, BMG_F_A, depth, vc.box, 1));
+  TRY(view_move(g, vc, v, p->b, ext, true));
+  TRY(bmg_set_field(g, BMG_F_B, depth, vc.box, 1));
   // alpha: unset (0) or constant handles -> constant; otherwise face fields
   bool aconst = true;
   for (int c = 0; c < dim; c++) aconst &= p->alpha[c] == 0 || p->alpha[c] >= v->nvarmax;
@@ -1604,17 +1660,16 @@ int bmg_poisson_view(const bmg_multigrid_view* v, const bmg_poisson_args* p, con
         return fail(BMG_ERR_UNSUPPORTED, "alpha mixes constant and variable components");
       int fe[3] = {n, n, n};
       fe[c] = n + 1;
-      std::vector<double> fb(powd(n, dim) / n * (n + 1));
-      gather(p->alpha[c], fb.data(), fe);
-      TRY(bmg_set_field(g, BMG_F_ALPHA + c, depth, fb.data(), 0));
+      TRY(view_move(g, vc, v, p->alpha[c], fe, true));
+      TRY(bmg_set_field(g, BMG_F_ALPHA + c, depth, vc.box, 1));
     }
   }
   if (p->lambda <= 0 || p->lambda >= v->nvarmax) {
     double val = p->lambda >= v->nvarmax ? v->constant[p->lambda - v->nvarmax] : 0.;  // zeroc
     TRY(bmg_set_lambda_const(g, val));
   } else {
-    gather(p->lambda, buf.data(), ext);
-    TRY(bmg_set_field(g, BMG_F_LAMBDA, depth, buf.data(), 0));
+    TRY(view_move(g, vc, v, p->lambda, ext, true));
+    TRY(bmg_set_field(g, BMG_F_LAMBDA, depth, vc.box, 1));
   }
   for (int b = 0; b < 2 * dim; b++) {
     if (bc->kind[b] == BMG_BC_PERIODIC) TRY(bmg_set_bc(g, b, BMG_BC_PERIODIC, nullptr));
@@ -1627,16 +1682,38 @@ int bmg_poisson_view(const bmg_multigrid_view* v, const bmg_poisson_args* p, con
   int r = bmg_poisson(g, p->tolerance, p->nrelax, p->minlevel, &s);
   if (r != BMG_OK && r != BMG_ERR_NAN) return r;
   if (stats) *stats = s;
-  auto scatter = [&](int h, const double* in) {
-    for_box(dim, ext, [&](const int c[3], size_t q) { ((double*)view_cell(v, depth, c))[h] = in[q]; });
-  };
-  TRY(bmg_get_field(g, BMG_F_A, depth, buf.data(), 0));
-  scatter(p->a, buf.data());
+  // a (and res) into the device records, then the records back to the
+  // host in one copy: every other byte goes back as it came
+  TRY(bmg_get_field(g, BMG_F_A, depth, vc.box, 1));
+  TRY(view_move(g, vc, v, p->a, ext, false));
   if (p->res >= 0 && p->res < v->nvarmax) {
-    TRY(bmg_get_field(g, BMG_F_RES, depth, buf.data(), 0));
-    scatter(p->res, buf.data());
+    TRY(bmg_get_field(g, BMG_F_RES, depth, vc.box, 1));
+    TRY(view_move(g, vc, v, p->res, ext, false));
   }
+  HIPCHECK(hipMemcpyAsync(v->d[depth], vc.rec, rec_bytes, hipMemcpyDeviceToHost, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
   return r;
+}
+
+int bmg_view_configure(int precision, int smoother) {
+  if (precision != BMG_FP64 && precision != BMG_MIXED) return fail(BMG_ERR_ARG, "unknown precision %d", precision);
+  if (smoother != BMG_RELAX_GS && smoother != BMG_RELAX_JACOBI && smoother != BMG_RELAX_RBGS)
+    return fail(BMG_ERR_ARG, "unknown smoother %d", smoother);
+  g_view_cache.precision = precision;
+  g_view_cache.smoother = smoother;
+  return BMG_OK;
+}
+
+int bmg_view_release(void) {
+  ViewCache& vc = g_view_cache;
+  if (vc.g) bmg_grid_free(vc.g);
+  vc.g = nullptr;
+  vc.dim = vc.depth = -1;
+  dfree(vc.rec);
+  vc.rec_bytes = 0;
+  dfreet(vc.box);
+  vc.box_n = 0;
+  return BMG_OK;
 }
 
 }  // extern "C"
@@ -1912,6 +1989,8 @@ int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const doub
   if (g->dim < 2) return fail(BMG_ERR_UNSUPPORTED, "viscosity() is built for 2D and 3D grids");
   for (int d = 0; d < g->dim; d++)
     if (g->periodic[d]) return fail(BMG_ERR_UNSUPPORTED, "viscosity() on periodic grids is not built");
+  if (g->smoother == BMG_RELAX_GS)
+    return fail(BMG_ERR_UNSUPPORTED, "viscosity() runs red-black or Jacobi relaxation; set one of them");
   double* u[3] = {u_x, u_y, u_z};
   const double* mu[3] = {mu_x, mu_y, mu_z};
   int nmu = 0;

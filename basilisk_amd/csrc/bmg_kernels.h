@@ -186,6 +186,115 @@ __global__ __launch_bounds__(kBlock) void k_relax_jacobi(const T* __restrict__ d
   out[idx] = (self + T(2) * cval) / T(3);
 }
 
+// --- the reference's default smoother: in-place Gauss-Seidel ------------
+//
+// relax() without -DJACOBI (poisson.h:296, 303-320) overwrites each cell as
+// foreach_level visits it, i-major with j and then k inside
+// (grid/multigrid.h:191-215), after one ghost fill (the ghosts keep the
+// values of the sweep's start).  A cell therefore reads the new values of
+// its neighbours at a lower index along any dimension and the old values of
+// the others and of every ghost.  Those dependencies only link cells of
+// neighbouring hyperplanes i+j+k = s-1 and s+1, so the sweep runs hyperplane
+// by hyperplane, dold -> dnew, with the result of the serial loop bit for
+// bit: lower neighbours come from dnew, upper ones and ghosts (mirrored or
+// periodic) from dold.
+
+// n/d of cell c in that order
+template <int DIM, typename T>
+__device__ __forceinline__ T gs_value(const T* __restrict__ dold, const T* dnew, const T* __restrict__ res,
+                                      const Coef<T>& k, const Geo<DIM>& g, const BCH& bc, idx_t idx,
+                                      const int c[3], T D2) {
+  const T self = dold[idx];
+  T lam = k.lam ? k.lam[idx] : k.lam_c;
+  T nn = -D2 * res[idx], dd = -lam * D2;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    T ah, al;
+    if (k.lo[0]) {
+      ah = alpha_hi<DIM, T>(k.lo[d], k.wall[d], g, idx, c, d);
+      al = k.lo[d][idx];
+    } else {
+      ah = al = k.alpha;
+    }
+    T ap = nbr_h<DIM, T>(dold, g, bc, idx, c, d, +1, self);
+    T am = c[d] > 0 ? dnew[idx - g.stride(d)] : nbr_h<DIM, T>(dold, g, bc, idx, c, d, -1, self);
+    nn += ah * ap + al * am;
+    dd += ah + al;
+  }
+  return nn / dd;
+}
+
+// cells of hyperplane s: first index i in [i0, i1], (i - i0, j) flattened
+// over rows of n in 3D (k = s - i - j, masked when outside the level)
+template <int DIM>
+__device__ __forceinline__ void gs_plane_range(const Geo<DIM>& g, int s, int& i0, idx_t& count) {
+  const int hi = (DIM - 1) * (g.n - 1);
+  i0 = s - hi > 0 ? s - hi : 0;
+  const int i1 = s < g.n - 1 ? s : g.n - 1;
+  count = i1 < i0 ? 0u : (idx_t)(i1 - i0 + 1) * (DIM == 3 ? (idx_t)g.n : 1u);
+}
+
+template <int DIM>
+__device__ __forceinline__ bool gs_plane_cell(const Geo<DIM>& g, int s, int i0, idx_t t, int c[3]) {
+  if (DIM == 3) {
+    c[0] = i0 + (int)(t >> g.log2n);
+    c[1] = (int)(t & (idx_t)(g.n - 1));
+    c[2] = s - c[0] - c[1];
+    return c[2] >= 0 && c[2] < g.n;
+  }
+  if (DIM == 2) {
+    c[0] = i0 + (int)t;
+    c[1] = s - c[0];
+    c[2] = 0;
+    return true;
+  }
+  c[0] = s;
+  c[1] = c[2] = 0;
+  return true;
+}
+
+// one hyperplane per launch (levels whose hyperplanes fill many workgroups)
+template <int DIM, typename T>
+__global__ __launch_bounds__(kBlock) void k_gs_plane(const T* __restrict__ dold, T* dnew, const T* __restrict__ res,
+                                                     Coef<T> k, Geo<DIM> g, BCH bc, T D2, int s) {
+  int i0;
+  idx_t count;
+  gs_plane_range<DIM>(g, s, i0, count);
+  const idx_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  int c[3];
+  if (!gs_plane_cell<DIM>(g, s, i0, t, c)) return;
+  const idx_t idx = g.index(c);
+  dnew[idx] = gs_value<DIM, T>(dold, dnew, res, k, g, bc, idx, c, D2);
+}
+
+// the whole sweep in one workgroup, a barrier between hyperplanes (all
+// waves of a workgroup share the CU's vector L1, so the barrier makes the
+// new values of hyperplane s visible to hyperplane s+1)
+template <int DIM, typename T>
+__device__ __forceinline__ void gs_sweep_block(const T* __restrict__ dold, T* dnew, const T* __restrict__ res,
+                                               const Coef<T>& k, const Geo<DIM>& g, const BCH& bc, T D2) {
+  const int smax = DIM * (g.n - 1);
+  for (int s = 0; s <= smax; s++) {
+    int i0;
+    idx_t count;
+    gs_plane_range<DIM>(g, s, i0, count);
+    for (idx_t t = threadIdx.x; t < count; t += (idx_t)blockDim.x) {
+      int c[3];
+      if (!gs_plane_cell<DIM>(g, s, i0, t, c)) continue;
+      const idx_t idx = g.index(c);
+      dnew[idx] = gs_value<DIM, T>(dold, dnew, res, k, g, bc, idx, c, D2);
+    }
+    __syncthreads();
+  }
+}
+
+template <int DIM, typename T>
+__global__ __launch_bounds__(1024) void k_gs_block(const T* __restrict__ dold, T* dnew, const T* __restrict__ res,
+                                                   Coef<T> k, Geo<DIM> g, BCH bc, T D2) {
+  gs_sweep_block<DIM, T>(dold, dnew, res, k, g, bc, D2);
+}
+
 // a full red-black sweep in one launch, dold -> dnew: a red cell is relaxed
 // from the old field; a black cell relaxes its red neighbours on the fly
 // (from the old field, exactly as the red pass does) and then itself from
@@ -500,6 +609,50 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ f, size_t n, T v) {
   size_t idx = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (idx < n) f[idx] = v;
+}
+
+// --- the reference's own storage (grid/multigrid.h:16-73) ------------------
+//
+// Level l of the reference holds (n+4)^DIM cell records (GHOSTS = 2) of
+// `nrec` doubles each, in C order; field handle h is double h of a record.
+// These kernels move one field between a device copy of such a record
+// array and a dense box of extents e[0..DIM) (a cell field: n per
+// dimension; a face component c: n+1 along c, face i of a row being the
+// low face of cell i, so face n reads the record of the ghost cell n).
+struct ViewBox {
+  size_t nrec;   // doubles per record (datasize / 8)
+  int h;         // field handle
+  int m;         // n + 4
+  int e[3];      // box extents (1 beyond DIM)
+};
+
+__device__ __forceinline__ size_t view_record(const ViewBox& B, int DIMS, size_t q, int c[3]) {
+  const size_t e1 = (size_t)B.e[1], e2 = (size_t)B.e[2];
+  c[2] = (int)(q % e2);
+  c[1] = (int)((q / e2) % e1);
+  c[0] = (int)(q / (e1 * e2));
+  const size_t m = (size_t)B.m;
+  if (DIMS == 3) return ((size_t)(c[0] + 2) * m + (size_t)(c[1] + 2)) * m + (size_t)(c[2] + 2);
+  if (DIMS == 2) return (size_t)(c[0] + 2) * m + (size_t)(c[1] + 2);
+  return (size_t)(c[0] + 2);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_view_gather(const double* __restrict__ rec, ViewBox B,
+                                                        double* __restrict__ out) {
+  const size_t q = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= (size_t)B.e[0] * B.e[1] * B.e[2]) return;
+  int c[3];
+  out[q] = rec[view_record(B, DIM, q, c) * B.nrec + (size_t)B.h];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_view_scatter(double* __restrict__ rec, ViewBox B,
+                                                         const double* __restrict__ in) {
+  const size_t q = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= (size_t)B.e[0] * B.e[1] * B.e[2]) return;
+  int c[3];
+  rec[view_record(B, DIM, q, c) * B.nrec + (size_t)B.h] = in[q];
 }
 
 template <typename TI, typename TO>
@@ -2048,7 +2201,7 @@ struct CoarseLevel {
 template <typename T, int DIM>
 struct CoarseArgs {
   CoarseLevel<T, DIM> L[kCoarseLevels];
-  int minlevel, top, nrelax, jacobi;
+  int minlevel, top, nrelax, jacobi, gs;  // gs: the in-place Gauss-Seidel order, through da2
 };
 
 template <int DIM, typename T>
@@ -2114,7 +2267,11 @@ __global__ __launch_bounds__(1024) void k_coarse_cycle(CoarseArgs<T, DIM> A) {
     }
     __syncthreads();
     for (int it = 0; it < A.nrelax; it++) {
-      if (A.jacobi) {
+      if (A.gs) {
+        gs_sweep_block<DIM, T>(L.da, L.da2, L.res, L.k, L.g, L.bc, L.D2);
+        for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) L.da[idx] = L.da2[idx];
+        __syncthreads();
+      } else if (A.jacobi) {
         for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) coarse_relax_cell<DIM, T>(L, idx, -1);
         __syncthreads();
         for (idx_t idx = t; idx < nl; idx += (idx_t)blockDim.x) L.da[idx] = L.da2[idx];

@@ -718,6 +718,7 @@ unsigned nb_(long n) { return (unsigned)((n + kB - 1) / kB); }
 
 }  // namespace
 
+
 struct bmg_tree {
   int dim = 2;
   double L0 = 1., X0[3] = {0., 0., 0.};
@@ -755,7 +756,62 @@ struct bmg_tree {
   bool batching = true;  // small operations run batched in one workgroup (BMG_TREE_BATCH=0: off)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> cycle_graphs;
   hipGraphExec_t residual_graph = nullptr;
+  // relax-kernel timing (bmg_tree_profile_*): HIP events around every
+  // k_t_relax launch on the tree's stream, with the cells each relaxes;
+  // the cycle runs without graphs meanwhile
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<long, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+  double prof_ms = 0.;
+  long prof_n = 0, prof_cells = 0;
 };
+
+namespace {
+
+long seg_cells(const Segs& S) {
+  long n = 0;
+  for (int i = 0; i < S.n; i++) n += S.count[i];
+  return n;
+}
+
+// events around one relax launch of `cells` cells (no-op unless profiling)
+struct TProf {
+  bmg_tree* t;
+  hipEvent_t e1 = nullptr;
+  TProf(bmg_tree* t_, long cells) : t(t_) {
+    if (!t->prof) return;
+    hipEvent_t e0;
+    if (t->ev_pool.size() >= 2) {
+      e0 = t->ev_pool.back(); t->ev_pool.pop_back();
+      e1 = t->ev_pool.back(); t->ev_pool.pop_back();
+    } else {
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+    }
+    (void)hipEventRecord(e0, t->stream);
+    t->ev_used.push_back({cells, {e0, e1}});
+  }
+  ~TProf() {
+    if (t->prof) (void)hipEventRecord(e1, t->stream);
+  }
+};
+
+void tprof_collect(bmg_tree* t) {
+  if (t->ev_used.empty()) return;
+  (void)hipStreamSynchronize(t->stream);
+  for (auto& u : t->ev_used) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, u.second.first, u.second.second);
+    t->prof_ms += ms;
+    t->prof_n += 1;
+    t->prof_cells += u.first;
+    t->ev_pool.push_back(u.second.first);
+    t->ev_pool.push_back(u.second.second);
+  }
+  t->ev_used.clear();
+}
+
+}  // namespace
 
 namespace {
 
@@ -1623,14 +1679,20 @@ int relax(bmg_tree* t, int l) {
   T* da = F<T>(t->da);
   T* res = F<T>(t->res);
   if (t->smoother == BMG_RELAX_JACOBI) {
-    k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, F<T>(t->tmp), res, x.nbr, x.colour, k, -1);
+    {
+      TProf tp(t, seg_cells(S));
+      k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, F<T>(t->tmp), res, x.nbr, x.colour, k, -1);
+    }
     k_t_jacobi<T><<<nbk, kB, 0, t->stream>>>(S, da, F<T>(t->tmp));
   } else if (t->smoother == BMG_RELAX_RBGS) {
     for (int c = 0; c < 2; c++) {
       if (c) for (int m = 0; m <= l; m++) box<T>(t, da, BK_HOM, m);
       Segs SC = level_or_leaf_colour(t, l, c);
       unsigned nbc = (unsigned)SC.first_block[SC.n];
-      if (nbc) k_t_relax<T, DIM><<<nbc, kB, 0, t->stream>>>(SC, da, nullptr, res, x.nbr, x.colour, k, 2);
+      if (nbc) {
+        TProf tp(t, seg_cells(SC));
+        k_t_relax<T, DIM><<<nbc, kB, 0, t->stream>>>(SC, da, nullptr, res, x.nbr, x.colour, k, 2);
+      }
     }
   } else
     return bmg::fail(BMG_ERR_UNSUPPORTED, "the in-place Gauss-Seidel smoother is sequential; use JACOBI or RBGS");
@@ -1747,7 +1809,10 @@ struct Batcher {
     if (!nbk) return BMG_OK;
     T* da = A.fld[0];
     if (jac) {
-      k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, A.fld[2], A.res, x.nbr, x.colour, A.k, -1);
+      {
+        TProf tp(t, seg_cells(S));
+        k_t_relax<T, DIM><<<nbk, kB, 0, t->stream>>>(S, da, A.fld[2], A.res, x.nbr, x.colour, A.k, -1);
+      }
       k_t_jacobi<T><<<nbk, kB, 0, t->stream>>>(S, da, A.fld[2]);
     } else {
       for (int c = 0; c < 2; c++) {
@@ -1757,7 +1822,10 @@ struct Batcher {
         }
         Segs SC = level_or_leaf_colour(t, l, c);
         unsigned nbc = (unsigned)SC.first_block[SC.n];
-        if (nbc) k_t_relax<T, DIM><<<nbc, kB, 0, t->stream>>>(SC, da, nullptr, A.res, x.nbr, x.colour, A.k, 2);
+        if (nbc) {
+          TProf tp(t, seg_cells(SC));
+          k_t_relax<T, DIM><<<nbc, kB, 0, t->stream>>>(SC, da, nullptr, A.res, x.nbr, x.colour, A.k, 2);
+        }
       }
     }
     return BMG_OK;
@@ -1929,7 +1997,11 @@ int cycle_kernels(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
 template <typename T, int DIM>
 int mg_cycle(bmg_tree* t, int nrelax, int minlevel, int maxlevel) {
   TRY(coefficients<T, DIM>(t));
-  if (!t->use_graphs) return cycle_kernels<T, DIM>(t, nrelax, minlevel, maxlevel);
+  if (!t->use_graphs || t->prof) {
+    TRY((cycle_kernels<T, DIM>(t, nrelax, minlevel, maxlevel)));
+    tprof_collect(t);
+    return BMG_OK;
+  }
   hipGraphExec_t& slot = t->cycle_graphs[std::make_tuple(nrelax, minlevel, maxlevel)];
   return run_graph(t, &slot, [&]() { return cycle_kernels<T, DIM>(t, nrelax, minlevel, maxlevel); });
 }
@@ -2233,6 +2305,11 @@ int bmg_tree_free(bmg_tree* t) {
   if (!t) return BMG_OK;
   if (t->stream) (void)hipStreamSynchronize(t->stream);
   free_device(t);
+  for (auto& u : t->ev_used) {
+    (void)hipEventDestroy(u.second.first);
+    (void)hipEventDestroy(u.second.second);
+  }
+  for (hipEvent_t e : t->ev_pool) (void)hipEventDestroy(e);
   if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
   (void)hipGetLastError();
   delete t;
@@ -2813,6 +2890,24 @@ int bmg_tree_poisson(bmg_tree* t, double tolerance, int nrelax, int minlevel, bm
 int bmg_tree_synchronize(bmg_tree* t) {
   if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
   HIPCHECK(hipStreamSynchronize(t->stream));
+  return BMG_OK;
+}
+
+int bmg_tree_profile_enable(bmg_tree* t, int enable) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  tprof_collect(t);
+  t->prof = enable != 0;
+  t->prof_ms = 0.;
+  t->prof_n = t->prof_cells = 0;
+  return BMG_OK;
+}
+
+int bmg_tree_profile_get(bmg_tree* t, double* total_ms, long* launches, long* cells) {
+  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
+  tprof_collect(t);
+  if (total_ms) *total_ms = t->prof_ms;
+  if (launches) *launches = t->prof_n;
+  if (cells) *cells = t->prof_cells;
   return BMG_OK;
 }
 

@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--smoother", default="rbgs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--no-cpu-serial", action="store_true",
+                    help="skip the one-thread oracle solve (about a minute at 256^3)")
+    ap.add_argument("--no-gs", action="store_true", help="skip the GPU solve with the reference's default GS")
     ap.add_argument("--precisions", default="fp64,mixed")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
                     help="2: the 256^3 multigrid problem (default); 3: 2D adaptive quadtree; "
@@ -81,8 +83,14 @@ def tree_bench(args):
         t.synchronize()
         dt = (time.perf_counter() - t1) / args.steps
         d, nl, nc = t.info()
+        # one more solve with HIP events around every relax launch
+        t.profile(True)
+        t.set_a(zeros)
+        t.poisson(tolerance=info["tolerance"])
+        relax_ms, relax_n, relax_cells = t.profile_get()
+        t.profile(False)
         res[prec] = {"seconds": dt, "stats": st, "a": t.get_a(), "build_s": build, "depth": d, "cells": nc,
-                     "leaves": nl}
+                     "leaves": nl, "relax": (relax_ms, relax_n, relax_cells)}
         print("# %s: %.2f ms, %d cycles, resa %.3g, %d leaves, setup %.1f s" % (prec, dt * 1e3, st["i"], st["resa"],
               nl, build), file=sys.stderr, flush=True)
         if prec == args.precisions.split(",")[-1]:
@@ -137,8 +145,30 @@ def tree_bench(args):
         "resa": r["stats"]["resa"],
         "setup_s": round(r["build_s"], 2),
         "adapt_wavelet": adapt,
-        "roofline": None,
     }
+    # the dominant kernel, k_t_relax: per relaxed cell the algorithmic bytes
+    # of the sweep (da, res and the dim low faces read, da written: (dim+3)w)
+    # plus the 2*dim 4-byte neighbour indices it gathers through
+    w = 4 if main_prec == "mixed" else 8
+    per_cell = (dim + 3) * w + 4 * 2 * dim
+    rms, rn, rcells = r["relax"]
+    if rms > 0 and rn > 0:
+        achieved = per_cell * rcells / (rms * 1e-3) / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                            "kernel": "k_t_relax (%s), %d launches of more than 2048 cells per solve"
+                                      % (main_prec, rn),
+                            "launch_us": round(rms / rn * 1e3, 2), "cells_per_launch": round(rcells / rn),
+                            "bytes_per_cell": per_cell}
+    else:
+        line["roofline"] = None
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"], oref = tree_cpu_baseline(args, r)
+        if oref is not None and len(oref) == len(r["a"]):
+            scale = float(np.abs(oref - oref.mean()).max())
+            for prec, rr in res.items():
+                a = rr["a"] - rr["a"].mean()
+                line["%s_vs_oracle_rel_maxnorm" % prec] = float(np.abs(a - (oref - oref.mean())).max() / scale)
     if "fp64" in res and "mixed" in res:
         f, m = res["fp64"], res["mixed"]
         line["fp64_ms"] = round(f["seconds"] * 1e3, 3)
@@ -147,6 +177,42 @@ def tree_bench(args):
         af, am = f["a"] - f["a"].mean(), m["a"] - m["a"].mean()
         line["mixed_vs_fp64_rel_maxnorm"] = float(np.abs(am - af).max() / np.abs(af).max())
     print(json.dumps(line), flush=True)
+
+
+def tree_cpu_baseline(args, r):
+    """The tree oracle (one thread, the reference's default in-place GS in
+    foreach_cell's Morton order) on the same generator.  Config 3: the whole
+    solve to 1e-9, timed, on the benchmark's own tree.  Config 5: the same
+    generator at depth 8 (1.1M leaves; the depth-10 tree takes the oracle
+    tens of minutes to build), solved to 1e-9 and scaled by leaf-cycles to
+    the GPU run's leaves and cycles.  Tree construction is not timed."""
+    from oracle.tree import OracleTree
+    from basilisk_amd import configs
+    dim = 2 if args.config == 3 else 3
+    depth = None if args.config == 3 else 8
+    o = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    o.set_smoother("gs")
+    if args.config == 3:
+        info = configs.config3_tree(o)
+    else:
+        info = configs.config5_tree(o, depth=depth)
+    o.set_niter(300, 1)
+    t0 = time.perf_counter()
+    st = o.poisson(tolerance=info["tolerance"])
+    sec = time.perf_counter() - t0
+    leaves = o.nleaves()
+    base = {"unit": "ms", "cores": 1, "kind": "port", "cpu": cpu_model(), "cycles": st["i"], "resa": st["resa"],
+            "leaves": leaves}
+    if args.config == 3:
+        base["value"] = round(sec * 1e3, 1)
+        base["sample"] = "the whole poisson() solve to 1e-9 on the same quadtree (%d leaves), GS, one thread" % leaves
+        return base, o.get_a()
+    per = sec / (leaves * st["i"])
+    base["value"] = round(per * r["leaves"] * r["stats"]["i"] * 1e3, 1)
+    base["sample"] = ("poisson() to 1e-9 of the same generator at depth 8 (%d leaves, %d cycles, %.1f s, GS, one "
+                      "thread), %.3g s per leaf-cycle scaled to the GPU run's %d leaves x %d cycles"
+                      % (leaves, st["i"], sec, per, r["leaves"], r["stats"]["i"]))
+    return base, None
 
 
 def solve_once(g, zeros, tol):
@@ -285,38 +351,82 @@ def run_adjoint(bmg, torch, prob, args):
     return out
 
 
-def cpu_baseline(prob, args, ncycles_target):
-    """The oracle's OpenMP build (reference-default lexicographic GS,
-    AoS storage, schedule(static)) on the host cores: a bounded sample of
-    `--cpu-cycles` V-cycles of the same problem, extrapolated to the GPU
-    FP64 run's cycle count."""
+def cpu_model():
+    """The host CPU's model name (the "model name" of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prob, args):
+    """The oracle's build of the reference's default solver (in-place
+    lexicographic GS, nrelax 4, AoS storage, schedule(static) loops) on the
+    host: poisson() from a = 0 to the problem's tolerance, timed whole, on
+    min(nproc, 16) OpenMP threads (the reference's OpenMP path, whose GS
+    races between threads like the reference's) and, unless
+    --no-cpu-serial, on one thread (the deterministic serial order: its
+    cycle count and residual are what the GPU's GS mode reproduces).  The
+    threaded solution is then carried on to 1e-11: the reference solution
+    the GPU solutions are measured against."""
+    import numpy as np
     from oracle.oracle import OracleGrid, set_threads
     threads = min(os.cpu_count() or 1, 16)
+    tol = prob["tolerance"]
     set_threads(threads, prob["dim"])
     o = OracleGrid(prob["dim"], prob["depth"], omp=True)
     o.set_b(prob["b"])
     o.set_alpha(prob["alpha"])
     o.set_a(0 * prob["b"])
     t0 = time.perf_counter()
-    o.restriction_alpha_lambda()
-    o.residual()
-    for _ in range(args.cpu_cycles):
-        o.mg_cycle(4, 1)
-        o.residual()
-    dt = (time.perf_counter() - t0) / args.cpu_cycles
-    base = {"value": dt * ncycles_target * 1e3, "unit": "ms", "cores": threads, "kind": "port",
-            "sample": "%d V-cycles (GS, nrelax 4) + residuals of the %d^3 problem on %d OpenMP threads, "
-                      "%.3f s per cycle, scaled to the GPU FP64 run's %d cycles"
-                      % (args.cpu_cycles, 1 << prob["depth"], threads, dt, ncycles_target)}
-    # the oracle's own solve to 1e-11: the reference solution the GPU
-    # solutions are measured against (relative max norm after mean removal)
-    o.set_a(0 * prob["b"])
-    t1 = time.perf_counter()
-    st = o.poisson(tolerance=1e-11)
+    st = o.poisson(tolerance=tol)
+    dt = time.perf_counter() - t0
+    base = {"value": round(dt * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "cycles": st["i"], "resa": st["resa"],
+            "sample": "the whole solve: poisson() of the %d^3 problem from a = 0 to %g with the reference's "
+                      "default relax (GS, nrelax 4) on %d OpenMP threads, restriction of alpha included"
+                      % (1 << prob["depth"], tol, threads)}
+    serial = None
+    if not args.no_cpu_serial:
+        s1 = OracleGrid(prob["dim"], prob["depth"], omp=False)
+        s1.set_b(prob["b"])
+        s1.set_alpha(prob["alpha"])
+        s1.set_a(0 * prob["b"])
+        t1 = time.perf_counter()
+        ss = s1.poisson(tolerance=tol)
+        base["one_thread"] = {"ms": round((time.perf_counter() - t1) * 1e3, 1), "cycles": ss["i"],
+                              "resa": ss["resa"]}
+        serial = {"stats": ss, "a": s1.get_a()}
+        del s1
+    t2 = time.perf_counter()
+    st2 = o.poisson(tolerance=1e-11)
     ref = o.get_a()
-    oracle = {"tolerance": 1e-11, "cycles": st["i"], "resa": st["resa"],
-              "seconds": round(time.perf_counter() - t1, 2), "a": ref - ref.mean()}
-    return base, oracle
+    oracle = {"tolerance": 1e-11, "cycles": st["i"] + st2["i"], "resa": st2["resa"],
+              "seconds": round(dt + time.perf_counter() - t2, 2), "a": ref - ref.mean()}
+    return base, oracle, serial
+
+
+def gs_gpu(bmg, torch, prob, args):
+    """The GPU's GS mode (the reference's default relax in its serial order,
+    FP64) on the same problem: one timed poisson() and its solution."""
+    g = bmg.Grid(prob["dim"], prob["depth"], precision="fp64", smoother="gs")
+    dev = torch.device("cuda:0")
+    b = torch.from_numpy(prob["b"]).to(dev)
+    g.set_b(b)
+    g.set_alpha([torch.from_numpy(a).to(dev) for a in prob["alpha"]])
+    zeros = torch.zeros(b.shape, dtype=torch.float64, device=dev)
+    solve_once(g, zeros, prob["tolerance"])  # graphs of the cycle built here
+    g.synchronize()
+    t0 = time.perf_counter()
+    st = solve_once(g, zeros, prob["tolerance"])
+    g.synchronize()
+    out = {"seconds": time.perf_counter() - t0, "stats": st, "a": g.get_a()}
+    g.close()
+    return out
 
 
 def main():
@@ -420,15 +530,40 @@ def main():
         line["adjoint_ms"] = round(adj["seconds"] * 1e3, 3)
         line["adjoint_cycles"] = {"forward": adj["forward"]["i"], "adjoint": adj["adjoint"]["i"],
                                   "adjoint_resa": adj["adjoint"]["resa"], "finite_gradients": adj["finite"]}
+    gs = None
+    if not args.no_gs:
+        gs = gs_gpu(bmg, torch, prob, args)
+        line["gs_gpu"] = {"ms": round(gs["seconds"] * 1e3, 3), "cycles": gs["stats"]["i"],
+                          "resa": gs["stats"]["resa"]}
+        print("# gs (fp64): %.2f ms, %d cycles, resa %.3g" % (gs["seconds"] * 1e3, gs["stats"]["i"],
+              gs["stats"]["resa"]), file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
-        ncyc = res["fp64"]["stats"]["i"] if "fp64" in res else cycles
-        line["cpu_baseline"], oracle = cpu_baseline(prob, args, ncyc)
+        line["cpu_baseline"], oracle, serial = cpu_baseline(prob, args)
         ref = oracle.pop("a")
         scale = float(np.abs(ref).max())
         for prec, rr in res.items():
             a = rr["a"] - rr["a"].mean()
             oracle["%s_rel_maxnorm" % prec] = float(np.abs(a.reshape(ref.shape) - ref).max() / scale)
+        if gs is not None:
+            a = gs["a"] - gs["a"].mean()
+            oracle["gs_rel_maxnorm"] = float(np.abs(a.reshape(ref.shape) - ref).max() / scale)
         line["error_vs_oracle"] = oracle
+        # V-cycles to the tolerance against the reference's default relax in
+        # its serial order (the oracle on one thread): the GPU's GS mode
+        # must give the same count and bit-identical iterates, the red-black
+        # default within a cycle or two
+        if serial is not None:
+            sc = serial["stats"]["i"]
+            cyc = {"serial_gs_oracle": sc}
+            for prec, rr in res.items():
+                cyc["rbgs_%s" % prec] = rr["stats"]["i"]
+                cyc["rbgs_%s_minus_serial_gs" % prec] = rr["stats"]["i"] - sc
+            if gs is not None:
+                cyc["gs_gpu"] = gs["stats"]["i"]
+                cyc["gs_gpu_resa_equals_oracle"] = gs["stats"]["resa"] == serial["stats"]["resa"]
+                cyc["gs_gpu_a_equals_oracle"] = bool(np.array_equal(gs["a"].reshape(serial["a"].shape),
+                                                                    serial["a"]))
+            line["cycles_vs_reference_default"] = cyc
     print(json.dumps(line), flush=True)
 
 

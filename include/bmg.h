@@ -64,8 +64,12 @@ enum {
   BMG_BC_AFFINE = 4     /* ghost = c + sigma s, c given per face (see bmg_set_bc_affine) */
 };
 
-/* smoothers: the reference's in-place Gauss-Seidel (poisson.h:296), its
-   -DJACOBI weighted Jacobi (poisson.h:293-294, 325-328), and red-black GS */
+/* smoothers: the reference's in-place Gauss-Seidel (poisson.h:296) in the
+   i-major order of foreach_level (grid/multigrid.h:191-215), run by
+   hyperplanes i+j+k = s with the serial loop's iterates (multigrid grids;
+   refused for the full-tree kind, whose order is Morton, and by
+   bmg_viscosity), its -DJACOBI weighted Jacobi (poisson.h:293-294,
+   325-328), and red-black GS (the default: one pass over memory per sweep) */
 enum { BMG_RELAX_GS = 0, BMG_RELAX_JACOBI = 1, BMG_RELAX_RBGS = 2 };
 
 /* arithmetic of the cycle: plain FP64, or FP32 smoothing / restriction /
@@ -239,9 +243,22 @@ typedef struct {
   const double *c[6];
 } bmg_view_bc;
 
+/* poisson() on the reference's storage: the finest level's record array
+   ((2^depth+4)^dim records of datasize bytes) is uploaded once, the fields
+   are gathered from it on the device, the whole solve runs on the GPU, and
+   the records come back in one copy with `a` (and `res` when p->res >= 0)
+   updated; every other byte comes back as it was.  Not re-entrant: the
+   device grid and staging buffers are one per process, kept between
+   calls (bmg_view_release frees them). */
 int bmg_poisson_view(const bmg_multigrid_view *v, const bmg_poisson_args *p,
                      const bmg_view_bc *bc, double TOLERANCE, int NITERMAX,
                      int NITERMIN, bmg_mgstats *stats);
+/* precision (BMG_FP64, BMG_MIXED) and smoother (BMG_RELAX_*) of the
+   following bmg_poisson_view calls; the default is BMG_FP64 with red-black
+   GS, BMG_RELAX_GS gives the reference's own relax bit for bit (no
+   counterpart in the reference, whose struct Poisson has no such field) */
+int bmg_view_configure(int precision, int smoother);
+int bmg_view_release(void);
 
 /* ---------------------------------------------------------------- adaptive
    quadtrees and octrees (grid/tree.h, grid/tree-common.h)
@@ -345,6 +362,13 @@ int bmg_tree_mg_solve(bmg_tree *t, double tolerance, int nrelax, int minlevel,
 int bmg_tree_poisson(bmg_tree *t, double tolerance, int nrelax, int minlevel,
                      bmg_mgstats *stats);
 int bmg_tree_synchronize(bmg_tree *t);
+/* timing of the relax kernel (no counterpart in the reference): while
+   enabled, HIP events on the tree's stream bracket every relax launch of
+   more than 2048 cells (the cycle then runs without its graph);
+   bmg_tree_profile_get returns their total time, the number of launches
+   and the cells they relaxed since enabling.  Enabling resets the sums. */
+int bmg_tree_profile_enable(bmg_tree *t, int enable);
+int bmg_tree_profile_get(bmg_tree *t, double *total_ms, long *launches, long *cells);
 
 #ifdef __cplusplus
 }

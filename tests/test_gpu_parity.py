@@ -343,11 +343,11 @@ def test_nan_is_reported():
     assert e.value.code == 5
 
 
-def test_lexicographic_gs_refused():
+def test_unknown_smoother_refused():
     g = bmg.Grid(2, 3)
     with pytest.raises(bmg.BMGError) as e:
-        g.set_smoother("gs")
-    assert e.value.code == 3
+        bmg._check(g.lib.bmg_set_smoother(g._g, 7))
+    assert e.value.code == 1
 
 
 @pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
