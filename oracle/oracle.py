@@ -215,6 +215,10 @@ class OracleGrid:
             self.lib.orc_set_bc(self.g, direction, kind, None)
         else:
             v = np.ascontiguousarray(values, dtype=np.float64)
+            m = self.n() + 2 * GHOSTS
+            if v.size != m ** (self.dim - 1):
+                raise ValueError("expected %d boundary values (ghost-inclusive extent), got %d"
+                                 % (m ** (self.dim - 1), v.size))
             self.lib.orc_set_bc(self.g, direction, kind, _dptr(v))
 
     def set_bc_function(self, kind, fn, directions=None):

@@ -116,13 +116,16 @@ def test_view_against_oracle(dim, depth, smoother):
     o.set_b(b)
     o.set_alpha(alpha)
     o.set_lambda(lam)
-    o.set_bc(bmg.RIGHT, bmg.BC_DIRICHLET, vals[0])
-    o.set_bc(bmg.LEFT, bmg.BC_DIRICHLET, vals[1])
+    # the oracle takes BC values over the ghost-inclusive tangential extent
+    n = 1 << depth
+    for side, v in ((bmg.RIGHT, vals[0]), (bmg.LEFT, vals[1])):
+        full = np.zeros((n + 4,) * (dim - 1))
+        full[(slice(2, n + 2),) * (dim - 1)] = v.reshape((n,) * (dim - 1))
+        o.set_bc(side, bmg.BC_DIRICHLET, full.reshape(-1))
     o.set_a(np.zeros(o.shape()))
     so = o.poisson(tolerance=1e-9)
     assert stats.i == so["i"] and stats.resa == so["resa"] and stats.nrelax == so["nrelax"]
     assert np.array_equal(interior(levels, depth, dim, A), o.get_a())
-    n = 1 << depth
     for l in range(depth + 1):
         if l < depth:
             assert np.array_equal(levels[l], before[l])
