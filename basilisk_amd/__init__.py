@@ -51,6 +51,7 @@ EXPORTS = [
     "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
     "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
     "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
+    "bmg_poisson_noalloc",
 ]
 
 

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -21,6 +22,10 @@
 #include "bmg_visc_kernels.h"
 
 using namespace bmg;
+
+// the scratch of bmg_poisson_noalloc (see dalloc below)
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 thread_local std::string g_err;
@@ -117,21 +122,71 @@ namespace {
 
 size_t esize(const bmg_grid* g) { return g->precision == BMG_MIXED ? sizeof(float) : sizeof(double); }
 
+// --- device memory --------------------------------------------------------
+//
+// Every device array of a grid comes from dalloc.  Normally that is
+// hipMalloc; inside bmg_poisson_noalloc it carves 256-byte aligned pieces
+// out of `scratch`, a static device array of kScratchBytes that each GPU
+// holds from the library's load until the process ends, and dfree leaves
+// such pieces alone.
+
+struct Arena {
+  char* base;
+  size_t size, used;
+};
+thread_local Arena* t_arena = nullptr;
+
+// the device address of `scratch` on each device, resolved once per device
+// with that device current (the name is only a host-side stand-in)
+std::mutex g_scratch_mu;
+std::vector<char*> g_scratch_addr;
+
+int scratch_address(char** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize(dev + 1, nullptr);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(BMG_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_addr[dev] = (char*)p;
+  }
+  *out = g_scratch_addr[dev];
+  return BMG_OK;
+}
+
+bool in_scratch(const void* p) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  for (char* b : g_scratch_addr)
+    if (b && (const char*)p >= b && (const char*)p < b + kScratchBytes) return true;
+  return false;
+}
+
 int dalloc(void** p, size_t bytes) {
   if (bytes == 0) bytes = 8;
+  if (Arena* A = t_arena) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > A->size - A->used)
+      return fail(BMG_ERR_ARG, "the problem needs more than the %zu-byte scratch of bmg_poisson_noalloc", A->size);
+    *p = A->base + A->used;
+    A->used += need;
+    return BMG_OK;
+  }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(BMG_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   return BMG_OK;
 }
 
 void dfree(void*& p) {
-  if (p) (void)hipFree(p);
+  if (p && !in_scratch(p)) (void)hipFree(p);
   p = nullptr;
 }
 
 template <typename T>
 void dfreet(T*& p) {
-  if (p) (void)hipFree((void*)p);
+  if (p && !in_scratch((const void*)p)) (void)hipFree((void*)p);
   p = nullptr;
 }
 
@@ -1093,7 +1148,14 @@ int bmg_device_count(int* count) {
   return BMG_OK;
 }
 
-int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0, double Y0, double Z0) {
+}  // extern "C"
+
+namespace {
+
+// a grid of the given precision on its own stream (borrowed = null) or on
+// the caller's stream `borrowed` (which it never destroys)
+int grid_create(bmg_grid** out, int dimension, int depth, double L0, double X0, double Y0, double Z0,
+                int precision, bool use_borrowed, hipStream_t borrowed) {
   if (!out) return fail(BMG_ERR_ARG, "null output pointer");
   if (dimension < 1 || dimension > 3) return fail(BMG_ERR_ARG, "dimension %d not in 1..3", dimension);
   int maxdepth = dimension == 3 ? 10 : dimension == 2 ? 15 : 28;
@@ -1102,9 +1164,11 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(BMG_ERR_HIP, "no HIP device available: libbmg runs on the GPU only");
+  if (precision != BMG_FP64 && precision != BMG_MIXED) return fail(BMG_ERR_ARG, "unknown precision %d", precision);
   bmg_grid* g = new bmg_grid();
   g->dim = dimension;
   g->depth = depth;
+  g->precision = precision;
   g->L0 = L0;
   g->X0[0] = X0; g->X0[1] = Y0; g->X0[2] = Z0;
   g->L.resize(depth + 1);
@@ -1122,11 +1186,15 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
   int r = BMG_OK;
   // a blocking stream: it orders itself after work on the legacy default
   // stream, where callers (PyTorch included) usually prepare device inputs
-  if (hipStreamCreateWithFlags(&g->own_stream, hipStreamDefault) != hipSuccess) {
-    delete g;
-    return fail(BMG_ERR_HIP, "hipStreamCreate failed");
+  if (use_borrowed) {
+    g->stream = borrowed;
+  } else {
+    if (hipStreamCreateWithFlags(&g->own_stream, hipStreamDefault) != hipSuccess) {
+      delete g;
+      return fail(BMG_ERR_HIP, "hipStreamCreate failed");
+    }
+    g->stream = g->own_stream;
   }
-  g->stream = g->own_stream;
   for (int l = 0; l <= depth && r == BMG_OK; l++) {
     Level& lv = g->L[l];
     lv.n = 1 << l;
@@ -1174,6 +1242,14 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
   }
   *out = g;
   return BMG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0, double Y0, double Z0) {
+  return grid_create(out, dimension, depth, L0, X0, Y0, Z0, BMG_FP64, false, nullptr);
 }
 
 int bmg_grid_free(bmg_grid* g) {
@@ -1702,6 +1778,60 @@ int bmg_view_configure(int precision, int smoother) {
   g_view_cache.precision = precision;
   g_view_cache.smoother = smoother;
   return BMG_OK;
+}
+
+int bmg_poisson_noalloc(const bmg_noalloc_problem* q, double tolerance, int nrelax, int minlevel,
+                        bmg_mgstats* stats) {
+  if (!q || !q->a || !q->b) return fail(BMG_ERR_ARG, "null problem, a or b");
+  char* base = nullptr;
+  TRY(scratch_address(&base));
+  // every device array of the call comes out of the scratch; the previous
+  // call's contents are overwritten field by field before they are read
+  Arena arena{base, kScratchBytes, 0};
+  struct Scope {
+    Arena* prev;
+    explicit Scope(Arena* a) : prev(t_arena) { t_arena = a; }
+    ~Scope() { t_arena = prev; }
+  } scope(&arena);
+  bmg_grid* g = nullptr;
+  TRY(grid_create(&g, q->dimension, q->depth, q->L0, q->X0, q->Y0, q->Z0, q->precision, true,
+                  (hipStream_t)q->stream));
+  struct Free {
+    bmg_grid* g;
+    ~Free() { bmg_grid_free(g); }
+  } release{g};
+  g->use_graphs = false;  // instantiating a graph may allocate
+  const int dim = q->dimension, n = 1 << q->depth;
+  TRY(bmg_set_smoother(g, q->smoother));
+  for (int b = 0; b < 2 * dim; b++) {
+    if (q->bc_kind[b] == BMG_BC_PERIODIC) TRY(bmg_set_bc(g, b, BMG_BC_PERIODIC, nullptr));
+    else if (q->bc_kind[b] == BMG_BC_AFFINE) TRY(bmg_set_bc_affine(g, b, q->bc_sigma[b], q->bc_c[b]));
+    else return fail(BMG_ERR_ARG, "bc_kind[%d] must be BMG_BC_AFFINE or BMG_BC_PERIODIC", b);
+  }
+  TRY(bmg_set_field(g, BMG_F_A, q->depth, q->a, 1));
+  TRY(bmg_set_field(g, BMG_F_B, q->depth, q->b, 1));
+  int nalpha = 0;
+  for (int c = 0; c < dim; c++) nalpha += q->alpha[c] != nullptr;
+  if (nalpha != 0 && nalpha != dim) return fail(BMG_ERR_ARG, "alpha: every component or none");
+  if (nalpha)
+    for (int c = 0; c < dim; c++) TRY(bmg_set_field(g, BMG_F_ALPHA + c, q->depth, q->alpha[c], 1));
+  else
+    TRY(bmg_set_alpha_const(g, q->alpha_c));
+  if (q->lambda) TRY(bmg_set_field(g, BMG_F_LAMBDA, q->depth, q->lambda, 1));
+  else TRY(bmg_set_lambda_const(g, q->lambda_c));
+  // the fused residual's second buffer of a, which bmg_poisson would
+  // otherwise allocate during the first cycle
+  if (dim == 3 && fused_residual(g) && !g->a2)
+    TRY(dalloc((void**)&g->a2, g->L[g->depth].ncell * sizeof(double)));
+  TRY(bmg_set_tolerance(g, q->TOLERANCE));
+  TRY(bmg_set_niter(g, q->NITERMAX, q->NITERMIN));
+  bmg_mgstats st;
+  const int r = bmg_poisson(g, tolerance, nrelax, minlevel, &st);
+  if (r != BMG_OK && r != BMG_ERR_NAN) return r;
+  if (stats) *stats = st;
+  HIPCHECK(hipMemcpyAsync(q->a, g->a, powd(n, dim) * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return r;
 }
 
 int bmg_view_release(void) {

@@ -260,6 +260,43 @@ int bmg_poisson_view(const bmg_multigrid_view *v, const bmg_poisson_args *p,
 int bmg_view_configure(int precision, int smoother);
 int bmg_view_release(void);
 
+/* poisson() with no device allocation (no counterpart in the reference: it
+   is for callers that cannot allocate -- code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  The same solve as
+   bmg_grid_new + bmg_set_* + bmg_poisson + bmg_get_field(BMG_F_A), with the
+   whole hierarchy (levels, coefficients, residuals, partial norms) carved
+   from one static device scratch of 64 MiB that each GPU holds from the
+   library's load until the process ends.  Limit: a problem whose arrays
+   need more than that (about 3D 64^3 or 2D 512^2 in FP64) is refused with
+   BMG_ERR_ARG before any kernel runs.  The caller passes the inputs and the
+   output: a (initial guess in, solution out), b, alpha and lambda are
+   device arrays in the layout of bmg_set_field; the BC values bc_c are
+   host arrays as for bmg_set_bc_affine.  Every kernel and copy runs on
+   `stream` (NULL: the null stream); the call returns after it is done.
+   All calls on one GPU share its scratch, which keeps the last call's
+   contents: calls that overlap on one GPU (two host threads, two streams)
+   race and must be serialised by the caller. */
+typedef struct {
+  int dimension, depth;
+  double L0, X0, Y0, Z0;
+  int precision;            /* BMG_FP64 or BMG_MIXED */
+  int smoother;             /* BMG_RELAX_* */
+  int bc_kind[6];           /* BMG_BC_AFFINE or BMG_BC_PERIODIC per direction */
+  int bc_sigma[6];          /* affine: ghost = c + sigma s, sigma = +1 or -1 */
+  const double *bc_c[6];    /* affine: n^(dim-1) host values per side, or NULL for 0 */
+  double *a;                /* device, n^dim */
+  const double *b;          /* device, n^dim */
+  const double *alpha[3];   /* device face components (n+1 along c), or all NULL */
+  double alpha_c;           /* the constant alpha when alpha[] are NULL (1: unityf) */
+  const double *lambda;     /* device, n^dim, or NULL */
+  double lambda_c;          /* the constant lambda when NULL (0: zeroc) */
+  double TOLERANCE;
+  int NITERMAX, NITERMIN;
+  void *stream;
+} bmg_noalloc_problem;
+int bmg_poisson_noalloc(const bmg_noalloc_problem *q, double tolerance, int nrelax,
+                        int minlevel, bmg_mgstats *stats);
+
 /* ---------------------------------------------------------------- adaptive
    quadtrees and octrees (grid/tree.h, grid/tree-common.h)
 
