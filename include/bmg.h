@@ -20,7 +20,10 @@
  * Reference interfaces replaced (paths relative to the reference's src/):
  *   mgstats                        poisson.h:108-114   -> bmg_mgstats
  *   mg_cycle(...)                  poisson.h:32-90     -> bmg_mg_cycle
- *   mg_solve(struct MGSolve)       poisson.h:125-235   -> bmg_mg_solve
+ *   mg_solve(struct MGSolve)       poisson.h:125-235   -> bmg_mg_solve with poisson()'s own
+ *                                  relax/residual pair (poisson.h:271-391); bmg_viscosity for
+ *                                  viscosity()'s pair (viscosity.h:176); other callbacks have
+ *                                  no device counterpart
  *   relax / residual callbacks     poisson.h:271-391   -> bmg_relax, bmg_residual
  *   poisson(struct Poisson)        poisson.h:258-265, 402-441 -> bmg_poisson, bmg_poisson_view
  *   NITERMAX, NITERMIN, TOLERANCE  poisson.h:102-103   -> bmg_set_niter, bmg_set_tolerance
