@@ -51,7 +51,7 @@ EXPORTS = [
     "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
     "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
     "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
-    "bmg_poisson_noalloc",
+    "bmg_poisson_noalloc", "bmg_centered_gradient", "bmg_correction", "bmg_tree_project",
 ]
 
 
@@ -69,6 +69,63 @@ class MGStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MultigridView(ctypes.Structure):
+    """bmg_multigrid_view: the reference's multigrid storage
+    (grid/multigrid.h:16-73), one array of (2^l+4)^dim records per level."""
+    _fields_ = [("dimension", ctypes.c_int), ("depth", ctypes.c_int),
+                ("d", ctypes.POINTER(ctypes.c_void_p)), ("datasize", ctypes.c_size_t),
+                ("nvarmax", ctypes.c_int), ("constant", ctypes.POINTER(ctypes.c_double)),
+                ("L0", ctypes.c_double), ("X0", ctypes.c_double), ("Y0", ctypes.c_double),
+                ("Z0", ctypes.c_double)]
+
+
+class PoissonArgs(ctypes.Structure):
+    """bmg_poisson_args: struct Poisson (poisson.h:258-265) with handles."""
+    _fields_ = [("a", ctypes.c_int), ("b", ctypes.c_int), ("alpha", ctypes.c_int * 3),
+                ("lambda_", ctypes.c_int), ("tolerance", ctypes.c_double),
+                ("nrelax", ctypes.c_int), ("minlevel", ctypes.c_int), ("res", ctypes.c_int)]
+
+
+class ViewBC(ctypes.Structure):
+    """bmg_view_bc: per direction, kind, sigma and the face values c."""
+    _fields_ = [("kind", ctypes.c_int * 6), ("sigma", ctypes.c_int * 6),
+                ("c", ctypes.POINTER(ctypes.c_double) * 6)]
+
+
+def view_records(dim, depth, nvar):
+    """Zeroed record arrays laid out as the reference's multigrid storage:
+    level l is ((2^l+4)^dim, nvar) doubles (GHOSTS = 2, datasize 8 nvar)."""
+    return [np.zeros((((1 << l) + 4) ** dim, nvar)) for l in range(depth + 1)]
+
+
+def poisson_view(levels, dim, depth, nvar, a, b, alpha, lam, tolerance, bc_sigma=None, TOLERANCE=1e-3,
+                 NITERMAX=100, NITERMIN=1, res=-1):
+    """bmg_poisson_view on record arrays from view_records(): field handles
+    a, b, alpha (dim handles or 0), lam (handle or 0), res (-1: none);
+    walls are affine with c = 0 and sigma bc_sigma[side] (default +1,
+    symmetry).  Returns the mgstats dict."""
+    lib = library()
+    ptrs = (ctypes.c_void_p * (depth + 1))(*[lv.ctypes.data for lv in levels])
+    const = np.zeros(4)
+    v = MultigridView(dim, depth, ptrs, nvar * 8, 1 << 16, const.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                      1.0, 0.0, 0.0, 0.0)
+    al = list(alpha) + [0] * (3 - len(alpha))
+    args = PoissonArgs(a, b, (ctypes.c_int * 3)(*al), lam, tolerance, 0, 0, res)
+    bc = ViewBC()
+    for side in range(2 * dim):
+        bc.kind[side] = BC_AFFINE
+        bc.sigma[side] = 1 if bc_sigma is None else bc_sigma[side]
+    st = MGStats()
+    _check(lib.bmg_poisson_view(ctypes.byref(v), ctypes.byref(args), ctypes.byref(bc), TOLERANCE, NITERMAX,
+                                NITERMIN, ctypes.byref(st)), allow_nan=True)
+    return st.as_dict()
+
+
+def view_configure(precision="fp64", smoother="rbgs"):
+    """Precision and smoother of the following poisson_view calls."""
+    _check(library().bmg_view_configure(PRECISIONS[precision], SMOOTHERS[smoother]))
 
 
 _lib = None
@@ -123,6 +180,9 @@ def library():
         "bmg_profile_get": (I, [P, I, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_long)]),
         "bmg_poisson_view": (I, [P, P, P, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_view_configure": (I, [I, I]),
+        "bmg_centered_gradient": (I, [P, P, P, P, P, P, P, I]),
+        "bmg_tree_project": (I, [P, P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_correction": (I, [P, P, P, P, P, P, P, D, I]),
         "bmg_view_release": (I, []),
         "bmg_tree_new": (I, [ctypes.POINTER(P), I, D, D, D, D]),
         "bmg_tree_free": (I, [P]),
@@ -465,6 +525,47 @@ class Grid:
         ptrs = [_host_ptr(u) for u in uf] + [None] * (3 - self.dim)
         _check(self.lib.bmg_project(self._g, ptrs[0], ptrs[1], ptrs[2], float(dt), int(nrelax), 0, ctypes.byref(s)))
         return s.as_dict()
+
+    def _arrays(self, arrs, shapes, what):
+        """(pointers, on_device) of a list of per-component arrays: all
+        contiguous float64 CUDA tensors, or all C-contiguous float64 numpy
+        arrays, with the given shapes."""
+        for c, x in enumerate(arrs):
+            if tuple(x.shape) != tuple(shapes[c]):
+                raise ValueError("%s component %d: shape %s, expected %s" % (what, c, tuple(x.shape), shapes[c]))
+        if all(getattr(x, "is_cuda", False) for x in arrs):
+            import torch
+            for x in arrs:
+                if x.dtype != torch.float64 or not x.is_contiguous():
+                    raise ValueError("%s must be contiguous float64 tensors" % what)
+            torch.cuda.current_stream().synchronize()
+            return [ctypes.c_void_p(x.data_ptr()) for x in arrs] + [None] * (3 - len(arrs)), 1
+        for x in arrs:
+            if not isinstance(x, np.ndarray) or x.dtype != np.float64 or not x.flags.c_contiguous:
+                raise ValueError("%s must be C-contiguous float64 numpy arrays (or CUDA tensors)" % what)
+        return [_host_ptr(x) for x in arrs] + [None] * (3 - len(arrs)), 0
+
+    def centered_gradient(self, g, acc=None):
+        """centered_gradient (p, g), navier-stokes/centered.h:387-406, with
+        p = the grid's `a` and its BCs, the grid's alpha, fm = 1: writes the
+        cell field g (dim arrays of shape (n,)*dim) from the face
+        acceleration `acc` (dim face arrays, or None for zero)."""
+        gp, dev = self._arrays(g, [self.shape()] * self.dim, "g")
+        if acc is None:
+            ap = [None] * 3
+        else:
+            ap, adev = self._arrays(acc, [self.face_shape(c) for c in range(self.dim)], "acc")
+            if adev != dev:
+                raise ValueError("acc and g must both be on the device or both on the host")
+        _check(self.lib.bmg_centered_gradient(self._g, ap[0], ap[1], ap[2], gp[0], gp[1], gp[2], dev))
+
+    def correction(self, u, g, dt):
+        """correction (dt), navier-stokes/centered.h:322-327: u += dt g."""
+        up, dev = self._arrays(u, [self.shape()] * self.dim, "u")
+        gp, gdev = self._arrays(g, [self.shape()] * self.dim, "g")
+        if gdev != dev:
+            raise ValueError("u and g must both be on the device or both on the host")
+        _check(self.lib.bmg_correction(self._g, up[0], up[1], up[2], gp[0], gp[1], gp[2], float(dt), dev))
 
     def viscosity(self, u, mu=1.0, rho=1.0, dt=1.0, sigma=None, nrelax=0):
         """viscosity(), viscosity.h:163-178: the implicit viscous step
@@ -842,6 +943,29 @@ class Tree:
         if a.size != n:
             raise ValueError("one alpha value per face value (%d), got %d" % (n, a.size))
         _check(self.lib.bmg_tree_set_alpha_faces(self._t, _host_ptr(a), 0))
+
+    def nfacevalues(self):
+        f = self.faces()
+        return int(sum(((f[:, 4] >> d) & 1).sum() for d in range(self.dim)))
+
+    def project(self, uf, dt=1.0, nrelax=0):
+        """project(), poisson.h:469-509, on the tree with p = a: `uf` holds
+        the face velocity on the face cache (one value per set flag bit, as
+        set_alpha_faces) -- a float64 numpy array updated in place, or a
+        contiguous float64 CUDA tensor updated on the device; returns the
+        poisson() statistics."""
+        n = self.nfacevalues()
+        s = MGStats()
+        if getattr(uf, "is_cuda", False):
+            ptr = _device_ptr(uf, n, "uf")
+            import torch
+            torch.cuda.current_stream().synchronize()
+            _check(self.lib.bmg_tree_project(self._t, ptr, float(dt), int(nrelax), 1, ctypes.byref(s)))
+            return s.as_dict()
+        if not isinstance(uf, np.ndarray) or uf.dtype != np.float64 or not uf.flags.c_contiguous or uf.size != n:
+            raise ValueError("uf must be a C-contiguous float64 array of %d face values" % n)
+        _check(self.lib.bmg_tree_project(self._t, _host_ptr(uf), float(dt), int(nrelax), 0, ctypes.byref(s)))
+        return s.as_dict()
 
     def set_alpha_function(self, f):
         """alpha.x[] = f(x, y, z, d) on every face of the face cache."""

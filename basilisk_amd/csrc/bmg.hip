@@ -1586,6 +1586,124 @@ int bmg_project(bmg_grid* g, double* uf_x, double* uf_y, double* uf_z, double dt
   return rc;
 }
 
+}  // extern "C"
+
+namespace {
+
+// device copies of up to three host arrays of n doubles (or the device
+// pointers themselves when on_device); copied back on request
+struct DevTriple {
+  double* p[3] = {nullptr, nullptr, nullptr};
+  double* host[3] = {nullptr, nullptr, nullptr};
+  bool owned = false;
+  size_t n = 0;
+  int take(double* const* in, int count, size_t n_, int on_device, bool copy_in) {
+    n = n_;
+    owned = !on_device;
+    for (int d = 0; d < count; d++) {
+      if (!in[d]) continue;
+      if (on_device) {
+        p[d] = in[d];
+        continue;
+      }
+      host[d] = in[d];
+      TRY(dalloc((void**)&p[d], n * sizeof(double)));
+      if (copy_in) HIPCHECK(hipMemcpy(p[d], in[d], n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return BMG_OK;
+  }
+  int give_back() {
+    if (!owned) return BMG_OK;
+    for (int d = 0; d < 3; d++)
+      if (p[d] && host[d]) HIPCHECK(hipMemcpy(host[d], p[d], n * sizeof(double), hipMemcpyDeviceToHost));
+    return BMG_OK;
+  }
+  ~DevTriple() {
+    if (owned)
+      for (double*& q : p) dfreet(q);
+  }
+};
+
+BCA fine_bca(const bmg_grid* g) {
+  BCA bc;
+  memset(&bc, 0, sizeof bc);
+  for (int d = 0; d < 3; d++) bc.periodic[d] = d < g->dim ? g->periodic[d] : 0;
+  for (int b = 0; b < 6; b++) {
+    bc.sigma[b] = g->bc_sigma[b];
+    bc.c[b] = g->bc_c[b];
+  }
+  return bc;
+}
+
+template <int DIM>
+int centered_gradient_t(bmg_grid* g, double* const* acc, double* const* gout) {
+  Level& lf = g->L[g->depth];
+  Geo<DIM> geo{lf.n, lf.log2n};
+  CenteredArgs A;
+  memset(&A, 0, sizeof A);
+  A.p = g->a;
+  for (int d = 0; d < 3; d++) {
+    A.alo[d] = g->alpha_field && d < DIM ? lf.alo64[d] : nullptr;
+    A.awall[d] = g->alpha_field && d < DIM ? lf.awall64[d] : nullptr;
+    A.acc[d] = d < DIM ? acc[d] : nullptr;
+    A.gout[d] = d < DIM ? gout[d] : nullptr;
+  }
+  if (!g->alpha_field) A.alo[0] = nullptr;
+  A.alpha_c = g->alpha_c;
+  hipLaunchKernelGGL((k_centered_gradient<DIM>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, A, geo,
+                     fine_bca(g), delta(g, g->depth));
+  TRY(check_launch());
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return BMG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_centered_gradient(bmg_grid* g, const double* a_x, const double* a_y, const double* a_z, double* g_x,
+                          double* g_y, double* g_z, int on_device) {
+  TRY(check_grid(g));
+  double* out[3] = {g_x, g_y, g_z};
+  double* acc[3] = {(double*)a_x, (double*)a_y, (double*)a_z};
+  int nacc = 0;
+  for (int d = 0; d < g->dim; d++) {
+    if (!out[d]) return fail(BMG_ERR_ARG, "null gradient component %d", d);
+    nacc += acc[d] != nullptr;
+  }
+  if (nacc != 0 && nacc != g->dim) return fail(BMG_ERR_ARG, "acceleration: every component or none");
+  const Level& lf = g->L[g->depth];
+  const size_t nface = (size_t)(lf.n + 1) * powd(lf.n, g->dim - 1);
+  DevTriple da, dg;
+  TRY(da.take(acc, g->dim, nface, on_device, true));
+  TRY(dg.take(out, g->dim, lf.ncell, on_device, false));
+  switch (g->dim) {
+    case 1: TRY(centered_gradient_t<1>(g, da.p, dg.p)); break;
+    case 2: TRY(centered_gradient_t<2>(g, da.p, dg.p)); break;
+    default: TRY(centered_gradient_t<3>(g, da.p, dg.p)); break;
+  }
+  return dg.give_back();
+}
+
+int bmg_correction(bmg_grid* g, double* u_x, double* u_y, double* u_z, const double* g_x, const double* g_y,
+                   const double* g_z, double dt, int on_device) {
+  TRY(check_grid(g));
+  double* u[3] = {u_x, u_y, u_z};
+  double* gv[3] = {(double*)g_x, (double*)g_y, (double*)g_z};
+  for (int d = 0; d < g->dim; d++)
+    if (!u[d] || !gv[d]) return fail(BMG_ERR_ARG, "null component %d", d);
+  const size_t n = g->L[g->depth].ncell;
+  DevTriple du, dgv;
+  TRY(du.take(u, g->dim, n, on_device, true));
+  TRY(dgv.take(gv, g->dim, n, on_device, true));
+  for (int d = 0; d < g->dim; d++)
+    hipLaunchKernelGGL(k_correction, dim3(nblocks(n)), dim3(kBlock), 0, g->stream, du.p[d], (const double*)dgv.p[d],
+                       dt, n);
+  TRY(check_launch());
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  return du.give_back();
+}
+
 int bmg_coef_gradient(bmg_grid* g, const double* psi, double* galpha_x, double* galpha_y, double* galpha_z,
                       double* glambda, int on_device) {
   TRY(check_grid(g));

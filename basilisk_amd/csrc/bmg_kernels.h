@@ -917,6 +917,100 @@ __global__ __launch_bounds__(kBlock) void k_face_update(double* __restrict__ u, 
   u[f] = u[f] - dt * alpha * ((ph - pl) / D);
 }
 
+// --- the projection event of navier-stokes/centered.h ------------------
+//
+// centered_gradient (centered.h:387-406): on every face
+//   gf.x[] = fm.x[]*a.x[] - alpha.x[]*(p[] - p[-1])/Delta
+// (fm = unityf, a the face acceleration, p the grid's `a` with its
+// boundary conditions), then in every cell
+//   g.x[] = (gf.x[] + gf.x[1])/(fm.x[] + fm.x[1] + SEPS)
+// with SEPS = 0 (common.h:1287).  A cell evaluates its two faces with the
+// expression the face loop uses, so shared faces get identical values.
+
+// p on both sides of the face with index i (0..n) along d of the row of
+// cell c: the ghost values at the walls (c + sigma s) or the periodic copy
+template <int DIM>
+__device__ __forceinline__ void face_sides(const double* __restrict__ p, const Geo<DIM>& g, const BCA& bc, int d,
+                                           const int c[3], int i, double& ph, double& pl) {
+  const int n = g.n;
+  int q[3] = {c[0], c[1], c[2]};
+  q[d] = i < n ? i : n - 1;
+  const double ahi = p[g.index(q)];
+  q[d] = i > 0 ? i - 1 : 0;
+  const double alow = p[g.index(q)];
+  const idx_t w = g.wall(d, c);
+  if (i < n) {
+    ph = ahi;
+  } else if (bc.periodic[d]) {
+    q[d] = 0;
+    ph = p[g.index(q)];
+  } else {
+    const double cv = bc.c[2 * d] ? bc.c[2 * d][w] : 0.;
+    ph = cv + double(bc.sigma[2 * d]) * alow;
+  }
+  if (i > 0) {
+    pl = alow;
+  } else if (bc.periodic[d]) {
+    q[d] = n - 1;
+    pl = p[g.index(q)];
+  } else {
+    const double cv = bc.c[2 * d + 1] ? bc.c[2 * d + 1][w] : 0.;
+    pl = cv + double(bc.sigma[2 * d + 1]) * ahi;
+  }
+}
+
+// index of face i along d in the user face layout (n+1 along d)
+template <int DIM>
+__device__ __forceinline__ size_t uface_at(int n, int d, const int c[3], int i) {
+  int q[3] = {c[0], c[1], c[2]};
+  q[d] = i;
+  size_t e1 = DIM >= 2 ? (size_t)n + (d == 1) : 1, e2 = DIM >= 3 ? (size_t)n + (d == 2) : 1;
+  return ((size_t)q[0] * e1 + (size_t)(DIM >= 2 ? q[1] : 0)) * e2 + (size_t)(DIM >= 3 ? q[2] : 0);
+}
+
+struct CenteredArgs {
+  const double* p;
+  const double* alo[3];
+  const double* awall[3];
+  double alpha_c;
+  const double* acc[3];  // face acceleration (user face layout) or null for zerof
+  double* gout[3];
+};
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_centered_gradient(CenteredArgs A, Geo<DIM> g, BCA bc, double D) {
+  const idx_t idx = logical_block() * kBlock + threadIdx.x;
+  if (idx >= g.ncell()) return;
+  int c[3];
+  g.decode(idx, c);
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    double gf[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int i = c[d] + h;
+      double ph, pl;
+      face_sides<DIM>(A.p, g, bc, d, c, i, ph, pl);
+      double alpha = A.alpha_c;
+      if (A.alo[0]) {
+        int q[3] = {c[0], c[1], c[2]};
+        q[d] = i;
+        alpha = i < g.n ? A.alo[d][g.index(q)] : A.awall[d][g.wall(d, c)];
+      }
+      const double acc = A.acc[d] ? A.acc[d][uface_at<DIM>(g.n, d, c, i)] : 0.;
+      gf[h] = 1. * acc - alpha * (ph - pl) / D;
+    }
+    A.gout[d][idx] = (gf[0] + gf[1]) / (1. + 1. + 0.);
+  }
+}
+
+// correction (centered.h:322-327): u.x[] += dt*g.x[]
+__global__ __launch_bounds__(kBlock) void k_correction(double* __restrict__ u, const double* __restrict__ gx,
+                                                       double dt, size_t n) {
+  const size_t idx = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (idx < n) u[idx] += dt * gx[idx];
+}
+
 // --- face coefficients (FP64, once per poisson() call) ------------------
 
 // coarse low faces of component d: face_average, grid/multigrid-common.h:42-58,

@@ -736,6 +736,7 @@ struct bmg_tree {
   DevIndex ix;
   double *a = nullptr, *b = nullptr, *res64 = nullptr, *g[3] = {nullptr, nullptr, nullptr};
   double* s = nullptr;  // the user scalar adapt_wavelet estimates (BMG_F_S)
+  double* uf[3] = {nullptr, nullptr, nullptr};  // project()'s face velocity per slot (allocated on use)
   bool s_set = false;
   double *al64[3] = {nullptr, nullptr, nullptr}, *lam64 = nullptr;
   void *da = nullptr, *res = nullptr, *tmp = nullptr, *alT[3] = {nullptr, nullptr, nullptr}, *lamT = nullptr;
@@ -844,6 +845,10 @@ void free_device(bmg_tree* t) {
     t->alT[d] = nullptr;
   }
   dfree_(t->lam64);
+  for (int d = 0; d < 3; d++) {
+    dfree_(t->uf[d]);
+    t->uf[d] = nullptr;
+  }
   if (mixed) { dfree_(t->lamT); dfree_(t->res); }
   dfree_(t->da); dfree_(t->tmp);
   for (int s = 0; s < 6; s++) { dfree_(t->bcv[s]); t->bcv[s] = nullptr; }
@@ -2786,6 +2791,82 @@ int bmg_tree_get_leaf_field(bmg_tree* t, int field, double* values, int on_devic
     case BMG_F_LAMBDA: return get_leaf_values<double>(t, t->lam64, values, on_device);
   }
   return bmg::fail(BMG_ERR_ARG, "field %d cannot be read on a tree", field);
+}
+
+}  // extern "C"
+
+namespace {
+
+// project() (poisson.h:469-509) on the tree: vals are the face velocity's
+// values on the face cache (device memory), updated in place
+template <int DIM>
+int tree_project_t(bmg_tree* t, double* vals, double dt, int nrelax, bmg_mgstats* stats) {
+  const DevIndex& x = t->ix;
+  for (int d = 0; d < DIM; d++)
+    if (!t->uf[d]) TRY(dalloc_((void**)&t->uf[d], std::max<long>(1, x.nslot) * sizeof(double)));
+  double* u1 = DIM > 1 ? t->uf[1] : nullptr;
+  double* u2 = DIM > 2 ? t->uf[2] : nullptr;
+  if (x.nfaceval)
+    k_t_face_set<<<nb_(x.nfaceval), kB, 0, t->stream>>>(x.fv_gs, x.fv_d, x.nfaceval, vals, t->uf[0], u1, u2);
+  // the divergence's stencil reads uf.x[1]: halo_face first (the fine
+  // fluxes averaged onto the coarse faces of resolution jumps)
+  if (x.nhf) k_t_face_avg<DIM><<<nb_(x.nhf), kB, 0, t->stream>>>(x.hf, x.nhf, t->uf[0], u1, u2);
+  Segs S = all_leaves(t);
+  if (S.first_block[S.n])
+    k_t_leaf_div<DIM><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(S, t->uf[0], u1, u2, x.nbr, dt, t->b);
+  HIPCHECK(hipGetLastError());
+  const int rc = bmg_tree_poisson(t, t->TOLERANCE / (dt * dt), nrelax, 0, stats);
+  if (rc != BMG_OK && rc != BMG_ERR_NAN) return rc;
+  // the face update's stencil reads p[-1]: the full tree boundary of p
+  {
+    Batcher<double, DIM> B(t, t->a, nullptr, nullptr);
+    B.tree_boundary(0, BK_A, -1);
+    B.flush();
+  }
+  const double* al[3] = {nullptr, nullptr, nullptr};
+  if (t->alpha_field)
+    for (int d = 0; d < DIM; d++) al[d] = t->al64[d];
+  for (int l = 0; l <= x.depth; l++) {
+    const int n = x.face_n[l];
+    if (n)
+      k_t_face_update<<<nb_(n), kB, 0, t->stream>>>(x.fops + x.face_off[l], n, t->a, t->uf[0], u1, u2, al[0], al[1],
+                                                    al[2], t->alpha_c, dt, delta(t, l));
+  }
+  if (x.nfaceval)
+    k_t_face_get<<<nb_(x.nfaceval), kB, 0, t->stream>>>(x.fv_gs, x.fv_d, x.nfaceval, vals, t->uf[0], u1, u2);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_tree_project(bmg_tree* t, double* uf, double dt, int nrelax, int on_device, bmg_mgstats* stats) {
+  if (!t || !uf) return bmg::fail(BMG_ERR_ARG, "null argument");
+  if (dt == 0.) dt = 1.;
+  if (nrelax == 0) nrelax = 4;
+  if (!(dt > 0.) || nrelax < 0) return bmg::fail(BMG_ERR_ARG, "dt must be positive and nrelax non-negative");
+  TRY(ensure(t));
+  const DevIndex& x = t->ix;
+  double* vals = uf;
+  double* tmp = nullptr;
+  if (!on_device) {
+    TRY(dalloc_((void**)&tmp, std::max<long>(1, x.nfaceval) * 8));
+    HIPCHECK(hipMemcpy(tmp, uf, x.nfaceval * 8, hipMemcpyHostToDevice));
+    vals = tmp;
+  }
+  const int rc = t->dim == 3 ? tree_project_t<3>(t, vals, dt, nrelax, stats)
+                             : tree_project_t<2>(t, vals, dt, nrelax, stats);
+  if (!on_device) {
+    if ((rc == BMG_OK || rc == BMG_ERR_NAN) && hipMemcpy(uf, tmp, x.nfaceval * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+      dfree_(tmp);
+      return bmg::fail(BMG_ERR_HIP, "copy of the face velocity failed");
+    }
+    dfree_(tmp);
+  }
+  return rc;
 }
 
 int bmg_tree_set_alpha_faces(bmg_tree* t, const double* values, int on_device) {

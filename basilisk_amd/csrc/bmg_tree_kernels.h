@@ -350,6 +350,52 @@ __global__ __launch_bounds__(kB) void k_t_face_scatter(const FaceCopy* __restric
 }
 
 // values given in face-cache order (a foreach_face assignment)
+// --- project() on the tree (poisson.h:469-509) ---------------------------
+
+// face-cache values out of the per-dimension slot arrays (k_t_face_set's
+// inverse)
+__global__ __launch_bounds__(kB) void k_t_face_get(const int* __restrict__ gs, const int* __restrict__ dd, long n,
+                                                   double* __restrict__ v, const double* f0, const double* f1,
+                                                   const double* f2) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const double* f = dd[r] == 0 ? f0 : dd[r] == 1 ? f1 : f2;
+  v[r] = f[gs[r]];
+}
+
+// divergence at the leaves: div = 0; div += uf.x[1] - uf.x[] per
+// dimension; div /= dt*Delta -> b
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_t_leaf_div(Segs S, const double* u0, const double* u1, const double* u2,
+                                                   const int* __restrict__ nbr, double dt, double* __restrict__ b) {
+  int s = seg_of(S);
+  int gs = seg_slot(S, s);
+  if (gs < 0) return;
+  int ai = S.abase[s] + (gs - S.lvoff[s]);
+  const int* nb = nbr + (size_t)ai * 2 * DIM;
+  const double* u[3] = {u0, u1, u2};
+  double div = 0.;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) div += u[d][nb[2 * d + 1]] - u[d][gs];
+  div /= dt * S.D[s];
+  b[gs] = div;
+}
+
+// the face update on the faces of one level:
+// uf.x[] -= dt*alpha.x[]*face_gradient_x (p, 0)
+__global__ __launch_bounds__(kB) void k_t_face_update(const FaceOp* __restrict__ F, int n,
+                                                      const double* __restrict__ p, double* u0, double* u1,
+                                                      double* u2, const double* al0, const double* al1,
+                                                      const double* al2, double alpha_c, double dt, double D) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  FaceOp q = F[r];
+  double* u = q.d == 0 ? u0 : q.d == 1 ? u1 : u2;
+  const double* al = q.d == 0 ? al0 : q.d == 1 ? al1 : al2;
+  double alpha = al ? al[q.gs] : alpha_c;
+  u[q.gs] -= dt * alpha * ((p[q.gs] - p[q.gm]) / D);
+}
+
 __global__ __launch_bounds__(kB) void k_t_face_set(const int* __restrict__ gs, const int* __restrict__ dd, long n,
                                                    const double* __restrict__ v, double* f0, double* f1, double* f2) {
   long r = (long)blockIdx.x * kB + threadIdx.x;

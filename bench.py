@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu-serial", action="store_true",
                     help="skip the one-thread oracle solve (about a minute at 256^3)")
     ap.add_argument("--no-gs", action="store_true", help="skip the GPU solve with the reference's default GS")
+    ap.add_argument("--no-view", action="store_true", help="skip poisson() through the reference-storage view")
     ap.add_argument("--precisions", default="fp64,mixed")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
                     help="2: the 256^3 multigrid problem (default); 3: 2D adaptive quadtree; "
@@ -299,6 +300,39 @@ def run_precision(bmg, torch, prob, prec, args):
             "viscosity": visc}
 
 
+def run_view(bmg, prob, args, nvar=8):
+    """poisson() through the drop-in boundary: the problem laid out in the
+    reference's multigrid storage (records of nvar doubles: a, b, res,
+    alpha.x/y/z and two unrelated fields, two ghost layers) and solved by
+    bmg_poisson_view in mixed precision with red-black relax -- the upload
+    and download of the record array included in the time."""
+    import numpy as np
+    dim, depth = prob["dim"], prob["depth"]
+    n = 1 << depth
+    levels = bmg.view_records(dim, depth, nvar)
+    fin = levels[depth].reshape((n + 4,) * dim + (nvar,))
+    fin[(slice(2, n + 2),) * dim + (1,)] = prob["b"]
+    for c in range(dim):
+        shape = list(prob["b"].shape)
+        shape[c] += 1
+        fin[tuple(slice(2, 2 + e) for e in shape) + (3 + c,)] = prob["alpha"][c]
+    bmg.view_configure("mixed", "rbgs")
+    times = []
+    try:
+        for it in range(args.warmup + args.steps):
+            fin[..., 0] = 0.0
+            t0 = time.perf_counter()
+            st = bmg.poisson_view(levels, dim, depth, nvar, 0, 1, (3, 4, 5)[:dim], 0, prob["tolerance"], res=2)
+            if it >= args.warmup:
+                times.append(time.perf_counter() - t0)
+    finally:
+        bmg.view_configure("fp64", "rbgs")
+        bmg.library().bmg_view_release()
+    rec_bytes = levels[depth].nbytes
+    return {"ms": round(sum(times) / len(times) * 1e3, 3), "cycles": st["i"], "resa": st["resa"],
+            "record_bytes": rec_bytes, "datasize": nvar * 8}
+
+
 def copy_bandwidth(torch, nbytes, reps=20):
     """GB/s (bytes read + bytes written) of torch's device-to-device copy of
     an nbytes buffer, timed with HIP events on the current stream."""
@@ -530,6 +564,11 @@ def main():
         line["adjoint_ms"] = round(adj["seconds"] * 1e3, 3)
         line["adjoint_cycles"] = {"forward": adj["forward"]["i"], "adjoint": adj["adjoint"]["i"],
                                   "adjoint_resa": adj["adjoint"]["resa"], "finite_gradients": adj["finite"]}
+    if not args.no_view:
+        v = run_view(bmg, prob, args)
+        line["view"] = v
+        print("# view (mixed): %.2f ms, %d cycles, %.2f GB of records each way" % (v["ms"], v["cycles"],
+              v["record_bytes"] / 1e9), file=sys.stderr, flush=True)
     gs = None
     if not args.no_gs:
         gs = gs_gpu(bmg, torch, prob, args)

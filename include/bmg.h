@@ -161,6 +161,21 @@ int bmg_synchronize(bmg_grid *g);
 int bmg_project(bmg_grid *g, double *uf_x, double *uf_y, double *uf_z, double dt, int nrelax, int on_device,
                 bmg_mgstats *stats);
 
+/* centered_gradient (p, g), navier-stokes/centered.h:387-406, with p the
+   grid's `a` (its boundary conditions give the wall ghosts), alpha the
+   grid's face coefficients, fm = unityf and SEPS = 0: on every face
+   gf = a_f - alpha (p[] - p[-1])/Delta, then per cell
+   g = (gf[] + gf[1])/2.  a_x/a_y/a_z: the face acceleration in the face
+   layout of bmg_set_field (n+1 along its component), all NULL for zerof;
+   g_x/g_y/g_z: the cell outputs (n^dim).  Device pointers when on_device,
+   else host arrays. */
+int bmg_centered_gradient(bmg_grid *g, const double *a_x, const double *a_y, const double *a_z,
+                          double *g_x, double *g_y, double *g_z, int on_device);
+/* correction (dt), navier-stokes/centered.h:322-327: u += dt g per cell
+   and component (n^dim values each). */
+int bmg_correction(bmg_grid *g, double *u_x, double *u_y, double *u_z, const double *g_x,
+                   const double *g_y, const double *g_z, double dt, int on_device);
+
 /* viscosity(), viscosity.h:163-178 (struct Viscosity, viscosity.h:3-10):
    the implicit viscous step u - dt/rho div(2 mu D(u)) = u_old, solved by
    mg_solve's loop (poisson.h:139-235: minlevel 0, tolerance TOLERANCE,
@@ -401,6 +416,16 @@ int bmg_tree_mg_solve(bmg_tree *t, double tolerance, int nrelax, int minlevel,
                       bmg_mgstats *stats);
 int bmg_tree_poisson(bmg_tree *t, double tolerance, int nrelax, int minlevel,
                      bmg_mgstats *stats);
+/* project(), poisson.h:469-509, on the tree: uf holds the face velocity on
+   the face cache (one value per set flag bit, entry then dimension order,
+   as bmg_tree_set_alpha_faces) and is updated in place; p is the tree's
+   `a` (initial guess in, pressure out) with its boundary conditions.  The
+   divergence (per leaf, over dt*Delta, after halo_face of uf) becomes b,
+   poisson() runs with TOLERANCE/dt^2, then every face of the cache gets
+   uf -= dt alpha (p[] - p[-1])/Delta after the full tree boundary of p.
+   dt = 0 means 1, nrelax = 0 means 4.  Device memory when on_device, else
+   a host array. */
+int bmg_tree_project(bmg_tree *t, double *uf, double dt, int nrelax, int on_device, bmg_mgstats *stats);
 int bmg_tree_synchronize(bmg_tree *t);
 /* timing of the relax kernel (no counterpart in the reference): while
    enabled, HIP events on the tree's stream bracket every relax launch of

@@ -307,6 +307,43 @@ class OracleGrid:
         return s.as_dict()
 
 
+def centered_gradient(o, alpha=None, acc=None, plant=0):
+    """centered_gradient (p, g) of navier-stokes/centered.h:387-406 with
+    p = the oracle grid's `a` (ghosts filled with its boundary conditions,
+    grid/multigrid.h:543-594), fm = unityf and SEPS = 0 (common.h:1281,
+    1287): on every face gf.x[] = fm.x[]*a.x[] - alpha.x[]*(p[] - p[-1])/Delta,
+    then g.x[] = (gf.x[] + gf.x[1])/(fm.x[] + fm.x[1] + SEPS) per cell.
+    alpha, acc: dim face arrays (n+1 along their component) or None for
+    unityf / zerof.  plant (tests only): 1 flips the sign of the pressure
+    term, 2 averages a cell's low face twice."""
+    dim, n, G = o.dim, o.n(), GHOSTS
+    o.boundary(F_A, o.depth, False)
+    p = o.raw_level(o.depth)[..., F_A]
+    D = o.delta()
+    g = []
+    for d in range(dim):
+        hi = [slice(G, G + n)] * dim
+        lo = [slice(G, G + n)] * dim
+        hi[d] = slice(G, G + n + 1)
+        lo[d] = slice(G - 1, G + n)
+        ph, pl = p[tuple(hi)], p[tuple(lo)]
+        al = 1.0 if alpha is None else alpha[d]
+        ac = 0.0 if acc is None else acc[d]
+        gf = 1.0 * ac + al * (ph - pl) / D if plant == 1 else 1.0 * ac - al * (ph - pl) / D
+        c0 = [slice(None)] * dim
+        c1 = [slice(None)] * dim
+        c0[d] = slice(0, n)
+        c1[d] = slice(0, n) if plant == 2 else slice(1, n + 1)
+        g.append((gf[tuple(c0)] + gf[tuple(c1)]) / (1.0 + 1.0 + 0.0))
+    return g
+
+
+def correction(u, g, dt):
+    """correction (dt), navier-stokes/centered.h:322-327: u.x[] += dt*g.x[]
+    per cell and component (returns new arrays)."""
+    return [x + dt * y for x, y in zip(u, g)]
+
+
 def nrelax_update(resb, resa, tolerance, nrelax, plant=0):
     """The oracle's nrelax adaptation after one cycle (poisson.h:198-204)."""
     return _lib(1, False, plant).orc_nrelax_update(resb, resa, tolerance, nrelax)

@@ -57,6 +57,7 @@ def _lib(dim):
             "ot_mg_cycle": (None, [P, I, I, I]), "ot_sum_b": (D, [P]),
             "ot_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "ot_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
+            "ot_project": (I, [P, vp, D, I, ctypes.POINTER(MGStats)]),
             "ot_boundary": (None, [P, I, I, I]),
             "ot_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I)]),
             "ot_refine_cell": (I, [P, I, I, I, I]),
@@ -261,6 +262,22 @@ class OracleTree:
         s = MGStats()
         self.lib.ot_mg_solve(self.t, tolerance, nrelax, minlevel, ctypes.byref(s))
         return s.as_dict()
+
+    def project(self, uf, dt=1.0, nrelax=0):
+        """project(), poisson.h:469-509, with p = a: uf holds the face
+        velocity on the face cache (one value per set flag bit, like
+        set_alpha_faces) and is updated in place; returns the stats."""
+        v = np.ascontiguousarray(uf, dtype=np.float64)
+        if v.size != self.nfacevalues():
+            raise ValueError("expected %d face values" % self.nfacevalues())
+        s = MGStats()
+        self.lib.ot_project(self.t, _ptr(v), float(dt), int(nrelax), ctypes.byref(s))
+        uf[...] = v
+        return s.as_dict()
+
+    def nfacevalues(self):
+        f = self.faces()
+        return int(sum(((f[:, 4] >> d) & 1).sum() for d in range(self.dim)))
 
     def poisson(self, tolerance=0.0, nrelax=4, minlevel=0):
         s = MGStats()

@@ -85,7 +85,9 @@ enum { C_ACTIVE = 1, C_LEAF = 2, C_FB = 4, C_REFINED = 8, C_TOO_FINE = 16, C_TOO
 enum { T_A, T_B, T_RES, T_DA, T_LAM, T_C, T_G };
 #define T_AL (T_G + DIM)
 #define T_S (T_AL + DIM)
-#define NVT (T_S + 1)
+/* T_U + d: the face velocity of project() */
+#define T_U (T_S + 1)
+#define NVT (T_U + DIM)
 
 enum { BC_SYMMETRY = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2 };
 enum { SM_GS = 0, SM_JACOBI = 1, SM_RBGS = 2 };
@@ -1166,6 +1168,48 @@ int ot_poisson(ot_tree *t, double tolerance, int nrelax, int minlevel, ot_mgstat
   if (tolerance) t->TOLERANCE = tolerance;
   int r = ot_mg_solve(t, 0., nrelax, minlevel > 1 ? minlevel : 1, out);
   if (tolerance) t->TOLERANCE = defaultol;
+  return r;
+}
+
+/* project(), poisson.h:469-509, on the tree.  uf: the face velocity on the
+   face cache (one value per set flag bit, entry order then dimension, as
+   ot_set_faces), updated in place; p is T_A.  The divergence's stencil
+   reads uf.x[1], so uf first gets its automatic boundary, halo_face (the
+   fine fluxes averaged onto the coarse faces of resolution jumps); the
+   face update's stencil reads p[-1], so p first gets the full tree
+   boundary with its own conditions. */
+int ot_project(ot_tree *t, double *uf, double dt, int nrelax, ot_mgstats *out) {
+  ot_update(t);
+  if (dt == 0.) dt = 1.;
+  if (nrelax == 0) nrelax = 4;
+  long v = 0;
+  for (int n = 0; n < t->nfaces; n++)
+    for (int d = 0; d < DIM; d++)
+      if (t->faces[n].flags & (1 << d)) t->L[t->faces[n].level].f[T_U + d][t->faces[n].idx] = uf[v++];
+  halo_face(t, T_U);
+  for (long n = 0; n < t->nleaves; n++) {
+    int l = t->leaves_l.p[n];
+    otl *L = &t->L[l];
+    size_t q = t->leaves_i.p[n];
+    double div = 0.;
+    for (int d = 0; d < DIM; d++) div += L->f[T_U + d][q + lstride(L, d)] - L->f[T_U + d][q];
+    div /= dt * delta_l(t, l);
+    L->f[T_B][q] = div;
+  }
+  int r = ot_poisson(t, t->TOLERANCE / (dt * dt), nrelax, 0, out);
+  tree_boundary_scalar(t, T_A, BK_A, -1);
+  v = 0;
+  for (int n = 0; n < t->nfaces; n++) {
+    otl *L = &t->L[t->faces[n].level];
+    double D = delta_l(t, t->faces[n].level);
+    size_t q = t->faces[n].idx;
+    for (int d = 0; d < DIM; d++)
+      if (t->faces[n].flags & (1 << d)) {
+        const double *p = L->f[T_A];
+        L->f[T_U + d][q] -= dt * alpha_at(t, L, d, q) * ((p[q] - p[q - lstride(L, d)]) / D);
+        uf[v++] = L->f[T_U + d][q];
+      }
+  }
   return r;
 }
 

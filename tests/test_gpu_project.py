@@ -127,3 +127,87 @@ def test_project_time_steps_with_nrelax_feedback(dim, depth):
         assert np.array_equal(g.get_a(), o.get_a())
         for c in range(dim):
             assert np.array_equal(ug[c], uo[c]), (step, c)
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 7), (3, 5)])
+@pytest.mark.parametrize("bcs", sorted(KINDS))
+@pytest.mark.parametrize("acc", [False, True])
+def test_centered_gradient_bitexact(dim, depth, bcs, acc):
+    """centered_gradient (navier-stokes/centered.h:387-406) of a random
+    pressure with every boundary kind, a variable alpha and a face
+    acceleration: the GPU's g equals the oracle's bit for bit, from host
+    arrays and from device tensors."""
+    import torch
+    from oracle import oracle as orc
+    print("case centered_gradient", dim, depth, bcs, acc, flush=True)
+    g = bmg.Grid(dim, depth)
+    o = OracleGrid(dim, depth)
+    for grid in (g, o):
+        _setup(grid, dim, bcs, 29, True)
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(g.shape())
+    g.set_a(p)
+    o.set_a(p)
+    af = [np.ascontiguousarray(rng.standard_normal(g.face_shape(c))) for c in range(dim)] if acc else None
+    alpha = [o.get_face(c) for c in range(dim)]
+    want = orc.centered_gradient(o, alpha, af)
+    out = [np.zeros(g.shape()) for _ in range(dim)]
+    g.centered_gradient(out, af)
+    for c in range(dim):
+        assert np.array_equal(out[c], want[c]), c
+    dev = [torch.zeros(g.shape(), dtype=torch.float64, device="cuda:0") for _ in range(dim)]
+    g.centered_gradient(dev, None if af is None else [torch.from_numpy(x).cuda() for x in af])
+    for c in range(dim):
+        assert np.array_equal(dev[c].cpu().numpy(), want[c]), c
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 7), (3, 5)])
+def test_centered_time_step_bitexact(dim, depth):
+    """The solver's part of a navier-stokes/centered.h time step: the
+    half-step project (uf, pf, alpha, dt/2.) of advection_term
+    (centered.h:307-313) on the pressure pf, then the projection event
+    (centered.h:412-420): project (uf, p, alpha, dt), centered_gradient (p,
+    g), correction (dt) -- twice, each with its own pressure field and nrelax
+    fed back as the reference does (mgpf, mgp).  Every face velocity,
+    pressure, gradient and cell velocity equals the oracle's bit for bit."""
+    from oracle import oracle as orc
+    print("case centered time step", dim, depth, flush=True)
+    dt = 0.3
+    grids = {}
+    for name in ("p", "pf"):
+        g = bmg.Grid(dim, depth, smoother="rbgs")
+        o = OracleGrid(dim, depth)
+        o.set_smoother("rbgs")
+        for grid in (g, o):
+            _setup(grid, dim, "symmetry", 31, True)
+            grid.set_tolerance(1e-6)
+        grids[name] = (g, o)
+    rng = np.random.default_rng(8)
+    ug = [np.ascontiguousarray(x) for x in _setup(OracleGrid(dim, depth), dim, "symmetry", 37, False)]
+    uo = [x.copy() for x in ug]
+    u_g = [rng.standard_normal(grids["p"][0].shape()) for _ in range(dim)]
+    u_o = [x.copy() for x in u_g]
+    alpha = [grids["p"][1].get_face(c) for c in range(dim)]
+    nr = {"p": {"g": 0, "o": 0}, "pf": {"g": 0, "o": 0}}
+    for step in range(2):
+        gpf, opf = grids["pf"]
+        s1 = gpf.project(ug, dt=dt / 2.0, nrelax=nr["pf"]["g"])
+        s2 = opf.project(uo, dt=dt / 2.0, nrelax=nr["pf"]["o"])
+        assert s1["i"] == s2["i"] and s1["resa"] == s2["resa"]
+        nr["pf"] = {"g": s1["nrelax"], "o": s2["nrelax"]}
+        gp, op = grids["p"]
+        s1 = gp.project(ug, dt=dt, nrelax=nr["p"]["g"])
+        s2 = op.project(uo, dt=dt, nrelax=nr["p"]["o"])
+        assert s1["i"] == s2["i"] and s1["resa"] == s2["resa"]
+        nr["p"] = {"g": s1["nrelax"], "o": s2["nrelax"]}
+        gg = [np.zeros(gp.shape()) for _ in range(dim)]
+        gp.centered_gradient(gg)
+        go = orc.centered_gradient(op, alpha)
+        gp.correction(u_g, gg, dt)
+        u_o = orc.correction(u_o, go, dt)
+        for c in range(dim):
+            assert np.array_equal(ug[c], uo[c]), (step, c)
+            assert np.array_equal(gg[c], go[c]), (step, c)
+            assert np.array_equal(u_g[c], u_o[c]), (step, c)
+        assert np.array_equal(gp.get_a(), op.get_a())
+        assert np.array_equal(gpf.get_a(), opf.get_a())
