@@ -44,6 +44,7 @@ def parse():
                     help="skip the one-thread oracle solve (about a minute at 256^3)")
     ap.add_argument("--no-gs", action="store_true", help="skip the GPU solve with the reference's default GS")
     ap.add_argument("--no-view", action="store_true", help="skip poisson() through the reference-storage view")
+    ap.add_argument("--no-adapt", action="store_true", help="tree workloads: skip adapt_wavelet, dump and restore")
     ap.add_argument("--precisions", default="fp64,mixed")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 5),
                     help="2: the 256^3 multigrid problem (default); 3: 2D adaptive quadtree; "
@@ -94,7 +95,7 @@ def tree_bench(args):
                      "leaves": nl, "relax": (relax_ms, relax_n, relax_cells)}
         print("# %s: %.2f ms, %d cycles, resa %.3g, %d leaves, setup %.1f s" % (prec, dt * 1e3, st["i"], st["resa"],
               nl, build), file=sys.stderr, flush=True)
-        if prec == args.precisions.split(",")[-1]:
+        if prec == args.precisions.split(",")[-1] and not args.no_adapt:
             # one adapt_wavelet (grid/tree-common.h:165-303) on the solution:
             # estimate, host passes, recompilation and field remap
             a = res[prec]["a"]
@@ -127,6 +128,8 @@ def tree_bench(args):
             print("# dump: %.1f ms, restore %.1f ms, %.1f MB" % (adapt["dump_ms"], adapt["restore_ms"],
                   adapt["dump_mb"]), file=sys.stderr, flush=True)
         t.close()
+    if args.no_adapt:
+        adapt = None
     main_prec = "mixed" if "mixed" in res else args.precisions.split(",")[0]
     r = res[main_prec]
     name = {3: "2D quadtree levels 5-11 around 40 circles", 5: "3D octree levels 5-10 around 200 spheres"}
@@ -163,6 +166,12 @@ def tree_bench(args):
                             "bytes_per_cell": per_cell}
     else:
         line["roofline"] = None
+    if line["roofline"] is not None:
+        # a device-to-device copy of 256 MB on the same box (its counters
+        # also calibrate FETCH_SIZE / WRITE_SIZE in a profiled run)
+        copy_gbs = copy_bandwidth(torch, 1 << 28)
+        line["roofline"]["copy_gbs"] = round(copy_gbs, 1)
+        line["roofline"]["frac_of_copy"] = round(line["roofline"]["achieved"] / copy_gbs, 4)
     if not args.no_cpu_baseline:
         line["cpu_baseline"], oref = tree_cpu_baseline(args, r)
         if oref is not None and len(oref) == len(r["a"]):

@@ -58,6 +58,9 @@ def _lib(dim):
             "ot_mg_solve": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "ot_poisson": (I, [P, D, I, I, ctypes.POINTER(MGStats)]),
             "ot_project": (I, [P, vp, D, I, ctypes.POINTER(MGStats)]),
+            "ot_residual_transpose": (None, [P, vp, vp]),
+            "ot_mg_cycle_T": (None, [P, vp, vp, I, I, I]),
+            "ot_adjoint": (I, [P, vp, D, I, I, vp, vp, vp, ctypes.POINTER(MGStats)]),
             "ot_boundary": (None, [P, I, I, I]),
             "ot_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I)]),
             "ot_refine_cell": (I, [P, I, I, I, I]),
@@ -274,6 +277,36 @@ class OracleTree:
         self.lib.ot_project(self.t, _ptr(v), float(dt), int(nrelax), ctypes.byref(s))
         uf[...] = v
         return s.as_dict()
+
+    def residual_transpose(self, y):
+        """r^T y for the leaf operator r(x) = -lambda x - div(alpha grad x)
+        with homogeneous boundary conditions (leaf order in and out)."""
+        yv = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(self.nleaves())
+        self.lib.ot_residual_transpose(self.t, _ptr(yv), _ptr(out))
+        return out
+
+    def mg_cycle_transpose(self, y, nrelax=4, minlevel=1):
+        """The transpose of one V-cycle from da = 0 (res -> da on the leaves,
+        red-black relax) applied to y."""
+        yv = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(self.nleaves())
+        self.lib.ot_mg_cycle_T(self.t, _ptr(yv), _ptr(out), int(nrelax), int(minlevel), self.depth())
+        return out
+
+    def adjoint(self, g, tolerance=1e-10, nitermax=100, nrelax=4, alpha=True, lam=False):
+        """The adjoint of the last poisson() solve: psi with L^T psi = g
+        (L = -r), then dJ/db = psi, dJ/dalpha on the face values and
+        dJ/dlambda at the leaves.  Returns (psi, galpha or None, glambda
+        or None, stats)."""
+        gv = np.ascontiguousarray(g, dtype=np.float64)
+        psi = np.zeros(self.nleaves())
+        ga = np.zeros(self.nfacevalues()) if alpha else None
+        gl = np.zeros(self.nleaves()) if lam else None
+        s = MGStats()
+        self.lib.ot_adjoint(self.t, _ptr(gv), float(tolerance), int(nitermax), int(nrelax), _ptr(psi),
+                            _ptr(ga) if alpha else None, _ptr(gl) if lam else None, ctypes.byref(s))
+        return psi, ga, gl, s.as_dict()
 
     def nfacevalues(self):
         f = self.faces()

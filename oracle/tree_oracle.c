@@ -85,9 +85,12 @@ enum { C_ACTIVE = 1, C_LEAF = 2, C_FB = 4, C_REFINED = 8, C_TOO_FINE = 16, C_TOO
 enum { T_A, T_B, T_RES, T_DA, T_LAM, T_C, T_G };
 #define T_AL (T_G + DIM)
 #define T_S (T_AL + DIM)
-/* T_U + d: the face velocity of project() */
+/* T_U + d: the face velocity of project(); T_DB, T_RB: the adjoints of
+   da and res in the transposed V-cycle */
 #define T_U (T_S + 1)
-#define NVT (T_U + DIM)
+#define T_DB (T_U + DIM)
+#define T_RB (T_DB + 1)
+#define NVT (T_RB + 1)
 
 enum { BC_SYMMETRY = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2 };
 enum { SM_GS = 0, SM_JACOBI = 1, SM_RBGS = 2 };
@@ -1211,6 +1214,380 @@ int ot_project(ot_tree *t, double *uf, double dt, int nrelax, ot_mgstats *out) {
       }
   }
   return r;
+}
+
+/* ------------------------------------------------ the transposed operator
+
+   The adjoint of a tree solve needs the transpose of the leaf operator
+   r(x) = -lambda x - sum_d (g_d[hi] - g_d[])/Delta, g = halo_face(alpha
+   (x[] - x[-1])/Delta) on the face cache, x after its tree boundary with the
+   homogeneous conditions (the residual of poisson.h:355-372 with b = 0 and
+   zero boundary values).  The tree operator is not symmetric (the halo
+   values are bilinear prolongations of restrictions), so its transpose is
+   built stage by stage, each stage of the forward chain transposed in
+   reverse order:
+
+     residual_T    y (leaves) -> xbar[q] = -lambda y, gbar[q] += y/Delta,
+                   gbar[hi] -= y/Delta
+     halo_face_T   each coarse target face's adjoint goes to its fine faces
+                   (divided by 2^(dim-1)) and the target's is cleared
+     faces_T       c = (alpha gbar)/Delta of each face-cache face in cache
+                   order: xbar[] += c, xbar[-1] -= c
+     boundary_T    tree_boundary_level's stages in reverse: box ghosts to
+                   their sources (the homogeneous rule's factor), halo
+                   children to the coarse ring (the bilinear weights, over
+                   16 in 2D and 64 in 3D), coarse cells to their children
+                   (divided by 2^dim), each written cell's adjoint cleared.
+
+   xbar lives in T_C, gbar in T_G + d.  Every accumulation runs in the
+   order written here (the order the GPU's gathers reproduce). */
+
+/* the homogeneous (or, sym = 1, symmetry) box rule of ghost q:
+   ghost = k * s[src]; src < 0: none */
+#define HOMF(b) (sym ? 1. : (t->bc[b] == BC_DIRICHLET ? -1. : 1.))
+static long box_rule_hom(ot_tree *t, int l, size_t q, double *k, int sym) {
+  otl *L = &t->L[l];
+  int g[3];
+  lijk(L, q, g);
+  for (int d = 0; d < DIM; d++)
+    for (int i = -1; i <= 1; i += 2) {
+      size_t p = q + i * lstride(L, d);
+      if (is_nbr(L, p)) {
+        *k = HOMF(bid_of(L, q));
+        return (long)p;
+      }
+    }
+  for (int pl = 0; pl < (DIM == 3 ? 3 : 1); pl++) {
+    int da = pl, db = (pl + 1) % DIM;
+    for (int i = -1; i <= 1; i += 2)
+      for (int j = -1; j <= 1; j += 2) {
+        size_t nn = q + i * lstride(L, da) + j * lstride(L, db);
+        size_t n1 = q + i * lstride(L, da), n2 = q + j * lstride(L, db);
+        if (L->alloc[nn] && is_nbr(L, nn) && L->alloc[n1] && is_boundary(L, n1) && L->alloc[n2] &&
+            is_boundary(L, n2)) {
+          double h1 = HOMF(bid_of(L, n1));
+          double h2 = HOMF(bid_of(L, n2));
+          *k = h1 + h2 - 1.;
+          return (long)nn;
+        }
+      }
+  }
+#if DIM == 3
+  for (int i = -1; i <= 1; i += 2)
+    for (int j = -1; j <= 1; j += 2)
+      for (int kk = -1; kk <= 1; kk += 2) {
+        size_t n0 = q + i * lstride(L, 0) + j * lstride(L, 1) + kk * lstride(L, 2);
+        size_t n1 = q + i * lstride(L, 0) + j * lstride(L, 1);
+        size_t n2 = q + i * lstride(L, 0) + kk * lstride(L, 2);
+        size_t n3 = q + j * lstride(L, 1) + kk * lstride(L, 2);
+        if (is_nbr(L, n0) && is_boundary(L, n1) && is_boundary(L, n2) && is_boundary(L, n3)) {
+          double h1 = HOMF(bid_of(L, n1));
+          double h2 = HOMF(bid_of(L, n2));
+          double h3 = HOMF(bid_of(L, n3));
+          *k = h1 + h2 + h3 - 2.;
+          return (long)n0;
+        }
+      }
+#endif
+  return -1;
+}
+
+#undef HOMF
+
+static void box_T(ot_tree *t, int f, int l, int sym) {
+  otl *L = &t->L[l];
+  double *x = L->f[f];
+  for (int n = 0; n < t->bnd[l].n; n++) {
+    size_t q = t->bnd[l].p[n];
+    double k;
+    long src = box_rule_hom(t, l, q, &k, sym);
+    if (src >= 0) x[src] += k * x[q];
+    x[q] = 0.;
+  }
+}
+
+/* bilinear (multigrid-common.h:124-140) of fine cell fc at level l+1
+   transposed: its adjoint onto the coarse ring at level l, then cleared */
+static void bilinear_T(ot_tree *t, int f, int l, const int fc[3]) {
+  otl *L = &t->L[l], *F = &t->L[l + 1];
+  double *xc = L->f[f], *xf = F->f[f];
+  size_t fq = lidx(F, fc[0], fc[1], fc[2]);
+  double cb = xf[fq];
+  int pc[3], ch[3];
+  for (int d = 0; d < 3; d++) {
+    pc[d] = (fc[d] + GH) / 2;
+    ch[d] = 2 * ((fc[d] + GH) % 2) - 1;
+  }
+#if DIM == 2
+  xc[lidx(L, pc[0], pc[1], 0)] += (9. * cb) / 16.;
+  xc[lidx(L, pc[0] + ch[0], pc[1], 0)] += (3. * cb) / 16.;
+  xc[lidx(L, pc[0], pc[1] + ch[1], 0)] += (3. * cb) / 16.;
+  xc[lidx(L, pc[0] + ch[0], pc[1] + ch[1], 0)] += cb / 16.;
+#else
+  xc[lidx(L, pc[0], pc[1], pc[2])] += (27. * cb) / 64.;
+  xc[lidx(L, pc[0] + ch[0], pc[1], pc[2])] += (9. * cb) / 64.;
+  xc[lidx(L, pc[0], pc[1] + ch[1], pc[2])] += (9. * cb) / 64.;
+  xc[lidx(L, pc[0], pc[1], pc[2] + ch[2])] += (9. * cb) / 64.;
+  xc[lidx(L, pc[0] + ch[0], pc[1] + ch[1], pc[2])] += (3. * cb) / 64.;
+  xc[lidx(L, pc[0] + ch[0], pc[1], pc[2] + ch[2])] += (3. * cb) / 64.;
+  xc[lidx(L, pc[0], pc[1] + ch[1], pc[2] + ch[2])] += (3. * cb) / 64.;
+  xc[lidx(L, pc[0] + ch[0], pc[1] + ch[1], pc[2] + ch[2])] += cb / 64.;
+#endif
+  xf[fq] = 0.;
+}
+
+/* the children of prolongation leaf q at level l back onto the coarse ring */
+static void halo_bilinear_T(ot_tree *t, int f, int l, size_t q) {
+  otl *L = &t->L[l];
+  int c[3];
+  lijk(L, q, c);
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
+        int fc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
+        bilinear_T(t, f, l, fc);
+      }
+}
+
+/* coarse (non-leaf active) cells of level l onto their children */
+static void restrict_T(ot_tree *t, int f, int l) {
+  otl *L = &t->L[l], *F = &t->L[l + 1];
+  olist *A = &t->active[l];
+  for (int n = 0; n < A->n; n++) {
+    size_t q = A->p[n];
+    if (is_leaf(L, q)) continue;
+    int c[3];
+    lijk(L, q, c);
+    double pb = L->f[f][q];
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++)
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++)
+          F->f[f][lidx(F, 2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0)] += pb / NCH;
+    L->f[f][q] = 0.;
+  }
+}
+
+/* tree_boundary_scalar (f, BK_HOM or BK_SYM, lev) transposed (lev < 0:
+   depth) */
+static void tree_boundary_T(ot_tree *t, int f, int lev, int sym) {
+  int depth = lev < 0 ? t->depth : lev;
+  if (is_full(t)) {
+    box_T(t, f, depth, sym);
+    return;
+  }
+  for (int i = depth - 1; i >= 0; i--) {
+    box_T(t, f, i + 1, sym);
+    olist *P = &t->prolong[i];
+    for (int n = 0; n < P->n; n++) halo_bilinear_T(t, f, i, P->p[n]);
+  }
+  box_T(t, f, 0, sym);
+  for (int m = 0; m < depth; m++) restrict_T(t, f, m);
+}
+
+/* halo_face (T_G) transposed */
+static void halo_face_T(ot_tree *t) {
+  for (int l = 0; l < t->depth; l++) {
+    otl *L = &t->L[l], *F = &t->L[l + 1];
+    olist *P = &t->prolong[l];
+    for (int n = 0; n < P->n; n++) {
+      size_t q = P->p[n];
+      int c[3];
+      lijk(L, q, c);
+      for (int d = 0; d < DIM; d++) {
+        ptrdiff_t sd = lstride(L, d);
+        for (int hi = 0; hi <= 1; hi++) {
+          size_t tq = hi ? q + sd : q;
+          if (!is_refined(L, hi ? q + sd : q - sd)) continue;
+          double v = L->f[T_G + d][tq];
+          int base[3] = {2 * c[0] - GH, 2 * c[1] - GH, 2 * c[2] - GH};
+          base[d] += 2 * hi;
+#if DIM == 2
+          for (int m = 0; m < 2; m++)
+            F->f[T_G + d][lidx(F, base[0] + ftan[d][m][0], base[1] + ftan[d][m][1], 0)] += v / 2.;
+#else
+          for (int m = 0; m < 4; m++)
+            F->f[T_G + d][lidx(F, base[0] + ftan[d][m][0], base[1] + ftan[d][m][1], base[2] + ftan[d][m][2])] +=
+                v / 4.;
+#endif
+          L->f[T_G + d][tq] = 0.;
+        }
+      }
+    }
+  }
+}
+
+/* residual_T and halo_face_T: gbar of y (leaf cache order), xbar's leaf
+   values -lambda y; everything else of T_C and T_G cleared first */
+static void adjoint_faces(ot_tree *t, const double *y) {
+  for (int l = 0; l <= t->depth; l++) {
+    otl *L = &t->L[l];
+    memset(L->f[T_C], 0, L->ncell * sizeof(double));
+    for (int d = 0; d < DIM; d++) memset(L->f[T_G + d], 0, L->ncell * sizeof(double));
+  }
+  for (long n = 0; n < t->nleaves; n++) {
+    int l = t->leaves_l.p[n];
+    otl *L = &t->L[l];
+    size_t q = t->leaves_i.p[n];
+    double D = delta_l(t, l);
+    L->f[T_C][q] = -lambda_at(t, L, q) * y[n];
+    for (int d = 0; d < DIM; d++) {
+      double tt = y[n] / D;
+      L->f[T_G + d][q] += tt;
+      L->f[T_G + d][q + lstride(L, d)] -= tt;
+    }
+  }
+  halo_face_T(t);
+}
+
+/* out = r^T y (leaf cache order) */
+void ot_residual_transpose(ot_tree *t, const double *y, double *out) {
+  ot_update(t);
+  adjoint_faces(t, y);
+  for (int n = 0; n < t->nfaces; n++) {
+    otl *L = &t->L[t->faces[n].level];
+    double D = delta_l(t, t->faces[n].level);
+    size_t q = t->faces[n].idx;
+    for (int d = 0; d < DIM; d++)
+      if (t->faces[n].flags & (1 << d)) {
+        double c = (alpha_at(t, L, d, q) * L->f[T_G + d][q]) / D;
+        L->f[T_C][q] += c;
+        L->f[T_C][q - lstride(L, d)] -= c;
+      }
+  }
+  tree_boundary_T(t, T_C, -1, 0);
+  for (long n = 0; n < t->nleaves; n++) out[n] = t->L[t->leaves_l.p[n]].f[T_C][t->leaves_i.p[n]];
+}
+
+/* one red-black pass of relax (level l, colour) transposed: each relaxed
+   cell's adjoint goes to res (-Delta^2/d) and to its neighbours (alpha/d),
+   then is cleared (the pass overwrote the cell without reading it) */
+static void relax_pass_T(ot_tree *t, int l, int colour) {
+  FOREACH_LEVEL_OR_LEAF(t, l, {
+    int c[3];
+    lijk(L, q, c);
+    if (((c[0] + c[1] + c[2]) & 1) == colour) {
+      double *db = L->f[T_DB];
+      double d = -lambda_at(t, L, q) * D2;
+      for (int cc = 0; cc < DIM; cc++) {
+        ptrdiff_t sd = lstride(L, cc);
+        d += alpha_at(t, L, cc, q + sd) + alpha_at(t, L, cc, q);
+      }
+      double v = db[q];
+      L->f[T_RB][q] += (-D2 * v) / d;
+      for (int cc = 0; cc < DIM; cc++) {
+        ptrdiff_t sd = lstride(L, cc);
+        db[q + sd] += (alpha_at(t, L, cc, q + sd) * v) / d;
+        db[q - sd] += (alpha_at(t, L, cc, q) * v) / d;
+      }
+      db[q] = 0.;
+    }
+  })
+}
+
+/* mg_cycle (poisson.h:32-90 on the tree, red-black relax) transposed: the
+   map res (leaves) -> da (leaves) of one V-cycle from da = 0, applied to
+   y (leaf order) in reverse, giving out (leaf order) */
+void ot_mg_cycle_T(ot_tree *t, const double *y, double *out, int nrelax, int minlevel, int maxlevel) {
+  ot_update(t);
+  for (int l = 0; l <= t->depth; l++) {
+    otl *L = &t->L[l];
+    memset(L->f[T_DB], 0, L->ncell * sizeof(double));
+    memset(L->f[T_RB], 0, L->ncell * sizeof(double));
+  }
+  for (long n = 0; n < t->nleaves; n++) t->L[t->leaves_l.p[n]].f[T_DB][t->leaves_i.p[n]] = y[n];
+  minlevel = minlevel < maxlevel ? minlevel : maxlevel;
+  for (int l = maxlevel; l >= minlevel; l--) {
+    for (int it = 0; it < nrelax; it++) {
+      tree_boundary_T(t, T_DB, l, 0);
+      relax_pass_T(t, l, 1);
+      for (int m = l; m >= 0; m--) box_T(t, T_DB, m, 0);
+      relax_pass_T(t, l, 0);
+    }
+    tree_boundary_T(t, T_DB, l, 0);
+    if (l == minlevel) {
+      FOREACH_LEVEL_OR_LEAF(t, l, { L->f[T_DB][q] = 0.; })
+    } else {
+      otl *L = &t->L[l];
+      olist *A = &t->active[l];
+      for (int n = 0; n < A->n; n++) {
+        int c[3];
+        lijk(L, A->p[n], c);
+        bilinear_T(t, T_DB, l - 1, c);
+      }
+    }
+  }
+  /* restriction (res): tree_restriction with the symmetry rules */
+  if (is_full(t)) {
+    for (int l = 0; l < t->depth; l++) {
+      box_T(t, T_RB, l, 1);
+      restrict_T(t, T_RB, l);
+    }
+    box_T(t, T_RB, t->depth, 1);
+  } else
+    tree_boundary_T(t, T_RB, -1, 1);
+  for (long n = 0; n < t->nleaves; n++) out[n] = t->L[t->leaves_l.p[n]].f[T_RB][t->leaves_i.p[n]];
+}
+
+/* The adjoint of poisson() on the tree, about the state a of the last
+   solve: psi solves L^T psi = g for L x = -r(x), by defect correction with
+   the transposed V-cycle M^T (M: the cycle mg_solve runs, red-black,
+   minlevel 1, nrelax fixed), which contracts L^T's error exactly as M does
+   L's (I - M^T L^T is the transpose of I - L M):
+     psi = 0, rho = g;  repeat psi += M^T rho, rho = g + r^T psi
+   until max|rho| <= tolerance (at least one cycle, at most nitermax).
+   Then dJ/db = psi, dJ/dalpha on every face value of the face cache
+   (entry order, then dimension) = gbar(psi) (a[] - a[-1])/Delta with a
+   after its tree boundary, and dJ/dlambda = -psi a at the leaves.
+   galpha / glambda may be NULL.  stats: i, resb (max|g|), resa. */
+int ot_adjoint(ot_tree *t, const double *g, double tolerance, int nitermax, int nrelax, double *psi,
+               double *galpha, double *glambda, ot_mgstats *st) {
+  ot_update(t);
+  long N = t->nleaves;
+  double *rho = malloc(N * sizeof(double)), *tr = malloc(N * sizeof(double));
+  double resb = 0., resa;
+  for (long n = 0; n < N; n++) {
+    rho[n] = g[n];
+    psi[n] = 0.;
+    if (fabs(rho[n]) > resb) resb = fabs(rho[n]);
+  }
+  resa = resb;
+  int it;
+  for (it = 0; it < nitermax && (it < 1 || resa > tolerance); it++) {
+    ot_mg_cycle_T(t, rho, tr, nrelax, 1, t->depth);
+    for (long n = 0; n < N; n++) psi[n] += tr[n];
+    ot_residual_transpose(t, psi, tr);
+    resa = 0.;
+    for (long n = 0; n < N; n++) {
+      rho[n] = g[n] + tr[n];
+      if (fabs(rho[n]) > resa) resa = fabs(rho[n]);
+    }
+  }
+  if (galpha) {
+    tree_boundary_scalar(t, T_A, BK_A, -1);
+    adjoint_faces(t, psi);
+    long v = 0;
+    for (int n = 0; n < t->nfaces; n++) {
+      otl *L = &t->L[t->faces[n].level];
+      double D = delta_l(t, t->faces[n].level);
+      size_t q = t->faces[n].idx;
+      const double *a = L->f[T_A];
+      for (int d = 0; d < DIM; d++)
+        if (t->faces[n].flags & (1 << d)) galpha[v++] = L->f[T_G + d][q] * ((a[q] - a[q - lstride(L, d)]) / D);
+    }
+  }
+  if (glambda)
+    for (long n = 0; n < N; n++) glambda[n] = -psi[n] * t->L[t->leaves_l.p[n]].f[T_A][t->leaves_i.p[n]];
+  if (st) {
+    memset(st, 0, sizeof *st);
+    st->i = it;
+    st->resb = resb;
+    st->resa = resa;
+    st->nrelax = nrelax;
+    st->minlevel = 1;
+  }
+  free(rho);
+  free(tr);
+  return 0;
 }
 
 /* the whole tree boundary of a field (for tests): l < 0 = depth */

@@ -83,7 +83,8 @@ def test_mixed_gradient_close_to_fp64(dim, depth):
 def test_adjoint_workload_finite_differences():
     """The 1024^2 adjoint workload: log-normal alpha, 32-Gaussian source,
     Dirichlet walls; gradients from the mixed-precision solver, J from FP64
-    solves; 64 seeded directions in (b, alpha)."""
+    solves; 64 seeded directions in (b, alpha), each checked against its
+    own central difference (eps = 1e-6) to 1e-6 relative."""
     print("adjoint workload fd", flush=True)
     p = configs.config4(depth=10, seed=4)
     src = torch.tensor(p["source"], device=DEV)
@@ -116,8 +117,12 @@ def test_adjoint_workload_finite_differences():
         fds.append((jp - jm) / (2 * eps))
     fds, wants = np.array(fds), np.array(wants)
     err = np.abs(fds - wants).max() / np.abs(wants).max()
-    print("FD error %.3g of the largest directional derivative %.3g" % (err, np.abs(wants).max()), flush=True)
-    assert err <= 1e-6
+    rel = np.abs(fds - wants) / np.abs(wants)
+    print("FD error %.3g of the largest directional derivative %.3g; per direction: max %.3g, median %.3g, "
+          "smallest |dJ| %.3g" % (err, np.abs(wants).max(), rel.max(), np.median(rel), np.abs(wants).min()),
+          flush=True)
+    # every one of the 64 directions to 1e-6 of its own derivative
+    assert rel.max() <= 1e-6
 
 
 def test_neumann_lambda0_adjoint_removes_mean():

@@ -150,8 +150,6 @@ def test_large_levels_bitexact(kernel, bcs, depth, monkeypatch):
     oracle's two-colour iterates bit for bit, periodic wrap included."""
     if kernel != "default":
         monkeypatch.setenv("BMG_RB_KERNEL", kernel)
-    if kernel == "fused" and depth == 8:
-        pytest.skip("covered at 128^3")
     print("case large levels", kernel, bcs, depth, flush=True)
     g, o = pair(3, depth, "rbgs")
     kinds = BC_MIXES[bcs]
