@@ -29,7 +29,7 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 F_A, F_B, F_RES, F_DA, F_LAMBDA, F_ALPHA = range(6)
 F_S = 8  # trees: the scalar adapt_wavelet estimates
 PROFILE_KINDS = {"relax_fine": 0, "residual": 1, "restriction": 2,
-                 "prolongation": 3, "correction": 4, "relax_coarse": 5}
+                 "prolongation": 3, "correction": 4, "relax_coarse": 5, "relax_fine2": 6}
 
 # every symbol declared in include/bmg.h
 EXPORTS = [
@@ -52,6 +52,7 @@ EXPORTS = [
     "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
     "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
     "bmg_poisson_noalloc", "bmg_centered_gradient", "bmg_correction", "bmg_tree_project",
+    "bmg_tree_residual_transpose", "bmg_tree_cycle_transpose", "bmg_tree_adjoint",
 ]
 
 
@@ -182,6 +183,9 @@ def library():
         "bmg_view_configure": (I, [I, I]),
         "bmg_centered_gradient": (I, [P, P, P, P, P, P, P, I]),
         "bmg_tree_project": (I, [P, P, D, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_tree_residual_transpose": (I, [P, P, P, I]),
+        "bmg_tree_cycle_transpose": (I, [P, P, P, I, I, I]),
+        "bmg_tree_adjoint": (I, [P, P, D, I, I, P, P, P, I, ctypes.POINTER(MGStats)]),
         "bmg_correction": (I, [P, P, P, P, P, P, P, D, I]),
         "bmg_view_release": (I, []),
         "bmg_tree_new": (I, [ctypes.POINTER(P), I, D, D, D, D]),
@@ -947,6 +951,59 @@ class Tree:
     def nfacevalues(self):
         f = self.faces()
         return int(sum(((f[:, 4] >> d) & 1).sum() for d in range(self.dim)))
+
+    def _leafvec(self, x, n, what):
+        """(pointer, on_device, keep-alive) of a leaf-order or face-value
+        vector: a contiguous float64 CUDA tensor or a host array"""
+        if getattr(x, "is_cuda", False):
+            return _device_ptr(x, n, what), 1, x
+        v = np.ascontiguousarray(x, dtype=np.float64)
+        if v.size != n:
+            raise ValueError("%s: expected %d values, got %d" % (what, n, v.size))
+        return _host_ptr(v), 0, v
+
+    def residual_transpose(self, y):
+        """r^T y for the leaf operator r(x) = -lambda x - div(alpha grad x)
+        with homogeneous boundary conditions (leaf order; host arrays)."""
+        n = self.nleaves()
+        yp, _, keep = self._leafvec(y, n, "y")
+        out = np.zeros(n)
+        _check(self.lib.bmg_tree_residual_transpose(self._t, yp, _host_ptr(out), 0))
+        return out
+
+    def cycle_transpose(self, y, nrelax=4, minlevel=1):
+        """the transpose of one V-cycle from da = 0 (red-black relax)
+        applied to y (leaf order; host arrays)."""
+        n = self.nleaves()
+        yp, _, keep = self._leafvec(y, n, "y")
+        out = np.zeros(n)
+        _check(self.lib.bmg_tree_cycle_transpose(self._t, yp, _host_ptr(out), int(nrelax), int(minlevel), 0))
+        return out
+
+    def adjoint(self, g, tolerance=1e-10, nitermax=100, nrelax=4, alpha=True, lam=False):
+        """The adjoint of the last poisson() solve: psi with L^T psi = g (L
+        the leaf operator), dJ/db = psi, dJ/dalpha on the face values and
+        dJ/dlambda at the leaves.  g: numpy (results numpy) or a CUDA tensor
+        (results CUDA tensors).  Returns (psi, galpha or None, glambda or
+        None, stats)."""
+        n, nv = self.nleaves(), self.nfacevalues()
+        gp, dev, keep = self._leafvec(g, n, "g")
+        if dev:
+            import torch
+            mk = lambda m: torch.zeros(m, dtype=torch.float64, device=g.device)
+            ptr = lambda x: ctypes.c_void_p(x.data_ptr())
+            torch.cuda.current_stream().synchronize()
+        else:
+            mk = np.zeros
+            ptr = _host_ptr
+        psi = mk(n)
+        ga = mk(nv) if alpha else None
+        gl = mk(n) if lam else None
+        s = MGStats()
+        _check(self.lib.bmg_tree_adjoint(self._t, gp, float(tolerance), int(nitermax), int(nrelax), ptr(psi),
+                                         ptr(ga) if alpha else None, ptr(gl) if lam else None, dev,
+                                         ctypes.byref(s)))
+        return psi, ga, gl, s.as_dict()
 
     def project(self, uf, dt=1.0, nrelax=0):
         """project(), poisson.h:469-509, on the tree with p = a: `uf` holds

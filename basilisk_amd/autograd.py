@@ -132,3 +132,87 @@ class _PoissonFunction(torch.autograd.Function):
             torch.cuda.synchronize()
         grads_alpha = tuple(ga) if need_a else tuple(None for _ in range(len(alpha) if alpha else 0))
         return (None, psi if ctx.needs_input_grad[1] else None, gl) + grads_alpha
+
+
+class TreePoissonSolver:
+    """Differentiable poisson() on an adaptive tree (a basilisk_amd.Tree
+    whose mesh, boundary conditions and settings the caller prepared).
+    Calling it with b (leaf order), alpha (face values, the order of
+    Tree.set_alpha_faces, or None for unity) and lambda (a leaf tensor, a
+    constant or None) solves the tree problem and returns the leaf values
+    of the solution.  The tree operator is not symmetric, so the backward
+    pass solves the transposed system L^T psi = dJ/da (Tree.adjoint: the
+    transposed V-cycle as defect correction) and returns dJ/db = psi,
+    dJ/dalpha per face value and dJ/dlambda = -psi a.  ``recompute=True``
+    solves the forward problem again in backward (the adjoint linearises
+    about the tree's current solution)."""
+
+    def __init__(self, tree, tolerance=1e-9, adjoint_tolerance=None, nitermax=100, recompute=False):
+        self.tree = tree
+        self.tolerance = tolerance
+        self.adjoint_tolerance = tolerance if adjoint_tolerance is None else adjoint_tolerance
+        self.nitermax = nitermax
+        self.recompute = recompute
+        tree.set_niter(nitermax, 1)
+        self.last_stats = None
+        self.last_adjoint_stats = None
+
+    def _coefficients(self, alpha, lam):
+        t = self.tree
+        if alpha is None:
+            t.set_alpha_const(1.0)
+        else:
+            t.set_alpha_faces(alpha.detach().cpu().numpy())
+        if lam is None or not torch.is_tensor(lam):
+            t.set_lambda(0.0 if lam is None else float(lam))
+        else:
+            t.set_lambda(lam.detach().contiguous())
+
+    def _solve(self, b, alpha, lam):
+        t = self.tree
+        self._coefficients(alpha, lam)
+        t.set_b(b.detach().contiguous())
+        t.set_a(torch.zeros(t.nleaves(), dtype=torch.float64, device=b.device))
+        st = t.poisson(tolerance=self.tolerance)
+        a = torch.empty(t.nleaves(), dtype=torch.float64, device=b.device)
+        t.get_a(a)
+        return a, st
+
+    def __call__(self, b, alpha=None, lam=None):
+        return _TreePoissonFunction.apply(self, b, alpha, lam)
+
+
+class _TreePoissonFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, solver, b, alpha, lam):
+        a, st = solver._solve(b, alpha, lam)
+        solver.last_stats = st
+        ctx.solver = solver
+        ctx.alpha, ctx.lam = alpha, lam
+        if solver.recompute:
+            ctx.save_for_backward(b)
+        else:
+            ctx.save_for_backward(b, a)
+        return a
+
+    @staticmethod
+    def backward(ctx, ga_out):
+        solver = ctx.solver
+        saved = ctx.saved_tensors
+        if solver.recompute:
+            _, solver.last_stats = solver._solve(saved[0], ctx.alpha, ctx.lam)
+        else:
+            # the tree may have solved something else since: linearise about
+            # this call's coefficients and solution
+            solver._coefficients(ctx.alpha, ctx.lam)
+            solver.tree.set_a(saved[1].detach().contiguous())
+        lam_field = torch.is_tensor(ctx.lam)
+        need_alpha = ctx.alpha is not None and ctx.needs_input_grad[2]
+        need_lam = lam_field and ctx.needs_input_grad[3]
+        psi, galpha, glam, st = solver.tree.adjoint(ga_out.contiguous(), tolerance=solver.adjoint_tolerance,
+                                                    nitermax=solver.nitermax, alpha=need_alpha, lam=need_lam)
+        solver.last_adjoint_stats = st
+        if not st["resa"] <= solver.adjoint_tolerance:
+            raise RuntimeError("tree adjoint did not reach the tolerance %g: %s" % (solver.adjoint_tolerance, st))
+        return (None, psi if ctx.needs_input_grad[1] else None, galpha if need_alpha else None,
+                glam if need_lam else None)

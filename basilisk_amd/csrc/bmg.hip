@@ -66,7 +66,16 @@ struct Level {
   double* lam64 = nullptr;
 };
 
-enum { PROF_RELAX_FINE = 0, PROF_RESIDUAL, PROF_RESTRICT, PROF_PROLONG, PROF_CORRECT, PROF_RELAX_COARSE, PROF_N };
+enum {
+  PROF_RELAX_FINE = 0,
+  PROF_RESIDUAL,
+  PROF_RESTRICT,
+  PROF_PROLONG,
+  PROF_CORRECT,
+  PROF_RELAX_COARSE,
+  PROF_RELAX_FINE2,
+  PROF_N
+};
 
 }  // namespace
 
@@ -101,6 +110,7 @@ struct bmg_grid {
   bool prof = false;
   int rb_override = -1;  // BMG_RB_KERNEL: kernel for red-black levels of >= 4096 cells
   int rb_reverse = 1;    // black passes walk the blocks backwards (BMG_RB_REVERSE=0: forwards)
+  int sweep2_min = 256;  // BMG_SWEEP2_MIN: smallest level edge of the two-sweep kernel (0: off)
   int grid_kind = BMG_GRID_MULTIGRID;  // ghost rule of edges and corners
   // the cycle as a HIP graph per (nrelax, minlevel, maxlevel, deferred
   // correction, fresh coarse res); dropped whenever a kernel argument of the
@@ -582,6 +592,61 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
   return out;
 }
 
+// FP32 levels that take the two-sweep kernel: the column-streamed ones of
+// at least sweep2_min cells per edge
+template <int DIM, typename T>
+bool use_sweep2(const bmg_grid* g, int l) {
+  return DIM == 3 && sizeof(T) == 4 && g->sweep2_min > 0 && g->L[l].n >= g->sweep2_min &&
+         level_kernel<DIM>(g, l) == RK_STREAM3_RB;
+}
+
+// plane chunks of the two-sweep kernel (BMG_SWEEP2_CHUNKS overrides,
+// 1..n/8): about one block per CU
+int sweep2_chunks(int n) {
+  if (const char* e = getenv("BMG_SWEEP2_CHUNKS")) {
+    const int c = atoi(e);
+    if (c >= 1 && c <= n / 8) return c;
+  }
+  const int tiles = (n / kTTK) * (n / kTTJ);
+  return std::max(1, std::min(n / 8, 256 / tiles));
+}
+
+// two red-black sweeps of level l in one pass, da[src] -> da[src^1]
+template <int DIM, typename T>
+int sweep2(bmg_grid* g, int l, int src, int* rc) {
+  if constexpr (DIM == 3 && sizeof(T) == 4) {
+    Level& lv = g->L[l];
+    Geo<DIM> geo{lv.n, lv.log2n};
+    const BCH bc = level_bc<DIM>(g, l);
+    const Coef<T> k = level_coef<T>(g, l);
+    const double D = delta(g, l);
+    const T D2 = (T)(D * D);
+    if (lv.n % kTTK || lv.ncell > (1u << 27)) {
+      *rc = fail(BMG_ERR_ARG, "internal: two-sweep kernel on level %d", l);
+      return src;
+    }
+    const int nch = sweep2_chunks(lv.n);
+    const unsigned nb = (unsigned)((lv.n / kTTK) * (lv.n / kTTJ) * nch);
+    const T* din = (const T*)lv.da[src];
+    T* dout = (T*)lv.da[src ^ 1];
+    const T* r = (const T*)lv.res;
+    const bool af = k.lo[0] != nullptr, lf = k.lam != nullptr;
+    ProfScope ps(g, l == g->depth ? PROF_RELAX_FINE2 : PROF_RELAX_COARSE);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(kTBlock), 0, g->stream, din, dout, r, k, geo, bc, D2, nch);
+    };
+    if (af && lf) go(k_rb_stream3t<T, true, true>);
+    else if (af) go(k_rb_stream3t<T, true, false>);
+    else if (lf) go(k_rb_stream3t<T, false, true>);
+    else go(k_rb_stream3t<T, false, false>);
+    *rc = check_launch();
+    return src ^ 1;
+  } else {
+    *rc = fail(BMG_ERR_ARG, "internal: two-sweep kernel is 3D FP32");
+    return src;
+  }
+}
+
 // the public single relax (bmg_relax): result kept in da[0]
 template <int DIM, typename T>
 int relax_level(bmg_grid* g, int l) {
@@ -659,9 +724,12 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) 
   if (use_block) TRY((coarse_block<DIM, T>(g, nrelax, minlevel, top)));
   for (int l = use_block ? top + 1 : minlevel; l <= maxlevel; l++) {
     Level& lv = g->L[l];
-    // ping-pong sweeps start in da[nrelax & 1] so that they end in da[0]
+    // ping-pong sweeps start in da[flips & 1] so that they end in da[0]
+    // (a two-sweep launch flips the buffers once)
     const int rk = level_kernel<DIM>(g, l);
-    int dst = (rk == RK_INPLACE_RB || rk == RK_PAIR_RB) ? 0 : (nrelax & 1);
+    const bool two = use_sweep2<DIM, T>(g, l);
+    const int flips = two ? nrelax - nrelax / 2 : nrelax;
+    int dst = (rk == RK_INPLACE_RB || rk == RK_PAIR_RB) ? 0 : (flips & 1);
     if (l == minlevel) {
       hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, (T*)lv.da[dst],
                          lv.ncell, T(0));
@@ -679,8 +747,14 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) 
     }
     TRY(check_launch());
     int cur = dst, rc = BMG_OK;
-    for (int it = 0; it < nrelax; it++) {
-      cur = sweep<DIM, T>(g, l, cur, &rc);
+    for (int it = 0; it < nrelax;) {
+      if (two && nrelax - it >= 2) {
+        cur = sweep2<DIM, T>(g, l, cur, &rc);
+        it += 2;
+      } else {
+        cur = sweep<DIM, T>(g, l, cur, &rc);
+        it++;
+      }
       TRY(rc);
     }
     if (cur != 0) return fail(BMG_ERR_ARG, "internal: sweep parity");
@@ -1182,6 +1256,10 @@ int grid_create(bmg_grid** out, int dimension, int depth, double L0, double X0, 
     else if (!strcmp(e, "stream3") && dimension == 3) g->rb_override = RK_STREAM3_RB;
   }
   if (const char* e = getenv("BMG_RB_REVERSE")) g->rb_reverse = e[0] != '0';
+  if (const char* e = getenv("BMG_SWEEP2_MIN")) {
+    const int v = atoi(e);
+    g->sweep2_min = v >= 128 ? v : 0;
+  }
   if (const char* e = getenv("BMG_GRAPHS")) g->use_graphs = e[0] != '0';
   int r = BMG_OK;
   // a blocking stream: it orders itself after work on the legacy default
@@ -2050,6 +2128,16 @@ struct Visc {
       }
       for (int b = 0; b < 6; b++) A.sig[c][b] = sig[c][b];
     }
+    // periodic dimensions wrap; on level 0 the periodic copy gives every
+    // ghost the root value (grid/multigrid.h:655-664)
+    bool anyp = false;
+    for (int d = 0; d < 3; d++) {
+      A.periodic[d] = d < DIM ? g->periodic[d] : 0;
+      anyp |= A.periodic[d] != 0;
+    }
+    if (l == 0 && anyp)
+      for (int c = 0; c < 3; c++)
+        for (int b = 0; b < 6; b++) A.sig[c][b] = 1;
     A.mu_c = mu_c;
     A.rho = rho_field ? vbuf(g, l, VS_RHO) : nullptr;
     A.rho_c = rho_c;
@@ -2113,7 +2201,12 @@ struct Visc {
         } else {
           BCH bcc;
           memset(&bcc, 0, sizeof bcc);
-          for (int b = 0; b < 6; b++) bcc.sigma[b] = sig[c][b];
+          bool anyp = false;
+          for (int d = 0; d < 3; d++) {
+            bcc.periodic[d] = d < DIM ? g->periodic[d] : 0;
+            anyp |= bcc.periodic[d] != 0;
+          }
+          for (int b = 0; b < 6; b++) bcc.sigma[b] = (l - 1 == 0 && anyp) ? 1 : sig[c][b];
           hipLaunchKernelGGL((k_prolong<DIM, double>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream,
                              vbuf(g, l - 1, VS_DA + c), vbuf(g, l, VS_DA + c), gf, bcc);
         }
@@ -2182,7 +2275,8 @@ struct Visc {
 };
 
 // copy of a face field in the user layout (n+1 along d) into low faces and
-// wall faces (as set_face does for alpha)
+// wall faces (as set_face does for alpha); along a periodic dimension face n
+// is face 0, so the wall faces take face 0 and face n is not read
 int visc_set_face(bmg_grid* g, int d, const double* src, int on_device) {
   const int l = g->depth;
   const Level& lv = g->L[l];
@@ -2196,7 +2290,8 @@ int visc_set_face(bmg_grid* g, int d, const double* src, int on_device) {
   HIPCHECK(hipMemcpy2DAsync(vbuf(g, l, VS_MLO + d), (size_t)n * inner * sizeof(double), src,
                             (size_t)(n + 1) * inner * sizeof(double), (size_t)n * inner * sizeof(double), outer, kind,
                             g->stream));
-  HIPCHECK(hipMemcpy2DAsync(vbuf(g, l, VS_MWALL + d), inner * sizeof(double), src + (size_t)n * inner,
+  const double* wall = g->periodic[d] ? src : src + (size_t)n * inner;
+  HIPCHECK(hipMemcpy2DAsync(vbuf(g, l, VS_MWALL + d), inner * sizeof(double), wall,
                             (size_t)(n + 1) * inner * sizeof(double), inner * sizeof(double), outer, kind, g->stream));
   return BMG_OK;
 }
@@ -2235,8 +2330,6 @@ int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const doub
                   int nrelax, int on_device, bmg_mgstats* stats) {
   TRY(check_grid(g));
   if (g->dim < 2) return fail(BMG_ERR_UNSUPPORTED, "viscosity() is built for 2D and 3D grids");
-  for (int d = 0; d < g->dim; d++)
-    if (g->periodic[d]) return fail(BMG_ERR_UNSUPPORTED, "viscosity() on periodic grids is not built");
   if (g->smoother == BMG_RELAX_GS)
     return fail(BMG_ERR_UNSUPPORTED, "viscosity() runs red-black or Jacobi relaxation; set one of them");
   double* u[3] = {u_x, u_y, u_z};

@@ -2009,6 +2009,253 @@ void k_rb_stream3c(const T* __restrict__ dold, T* __restrict__ dnew, const T* __
   }
 }
 
+// --- two red-black sweeps in one pass (temporal blocking), 3D, FP32 ------
+//
+// Two consecutive red-black sweeps, A then B, of a level with walls in
+// every direction, dold -> dnew, in one pass over the coefficients: each
+// plane's res, lambda and faces are loaded once and serve four half-sweeps.
+// Step s of the stream along i relaxes, for the cells whose column parity is
+// red on plane s, A's red cells of plane s, A's black cells of plane s-1,
+// B's red cells of plane s-2 and B's black cells of plane s-3.  Each reads
+// its plane neighbours in its own column, written by the same thread earlier
+// in the step or two steps before, and its in-plane neighbours, which have
+// the other parity and were written during the step before (ordered by the
+// step's one barrier).  The coefficients of a pair of cells on plane p serve
+// the four stages of steps p .. p+3, so a thread keeps the records of planes
+// s+1 .. s-3 of its pair and rotates their roles through a loop unrolled by
+// five.
+//
+// B's black cells on the tile's edge need B's red cells one cell outside the
+// tile, which need A's black cells two cells out, which need A's red cells
+// three cells out: 5 extra waves own the pairs of a ring of width 3 around
+// the 16 x 64 tile and relax, per step, the stages their cell's distance to
+// the tile allows; and a chunk of planes [i0, i1) runs its stages from three
+// planes below to three above it.  Blocks recompute these rings instead of
+// exchanging data.  The field region in LDS has a halo of 4 cells.
+//
+// Ghosts: a sweep reads, across a wall, sigma times the value its cell had
+// before the sweep.  The planes hold sigma times dold there when loaded, so
+// an A stage that writes a cell next to a wall also writes sigma times the
+// new value into the ghost position, which no other cell reads.
+//
+// The arithmetic is relax_nd's: the result equals two sweeps of
+// k_rb_stream3c bit for bit.
+constexpr int kTTJ = 16, kTTK = 64, kTH = 4;                      // tile rows, columns, LDS halo
+constexpr int kTInner = kTTJ * kTTK / 2;                          // pairs of the tile
+constexpr int kTRingPairs = 6 * ((kTTK + 2 * kTH) / 2) + 4 * kTTJ;  // pairs of the ring
+constexpr int kTBlock = (kTInner + kTRingPairs + 63) / 64 * 64;
+
+template <typename T, bool ALPHAF, bool LAMF>
+__global__ __launch_bounds__(kTBlock) void k_rb_stream3t(const T* __restrict__ dold, T* __restrict__ dnew,
+                                                         const T* __restrict__ res, Coef<T> k, Geo<3> g, BCH bc,
+                                                         T D2, int nchunks) {
+  using V2 = typename Vec2<T>::type;
+  constexpr int RJ = kTTJ + 2 * kTH, RK = kTTK + 2 * kTH;  // field region in LDS
+  constexpr int HP = RK / 2;                               // pairs per region row
+  constexpr int Region2 = RJ * HP;
+  constexpr int NLoad = (Region2 + kTBlock - 1) / kTBlock;
+  __shared__ __align__(16) T buf[8][RJ][RK];
+  const int n = g.n, tid = threadIdx.x;
+  const size_t b = logical_block();
+  const int nkt = n / kTTK, njt = n / kTTJ;
+  const int kt = (int)(b % nkt), jt = (int)((b / nkt) % njt), ct = (int)(b / ((size_t)nkt * njt));
+  const int i0 = (int)((size_t)ct * n / nchunks), i1 = (int)((size_t)(ct + 1) * n / nchunks);
+  const int j0 = jt * kTTJ, k0 = kt * kTTK;
+
+  // the pair of this thread, (pj, pk) and (pj, pk+1): the tile's pairs, then
+  // the ring's (three rows below and above the tile over the region's width,
+  // and the two pairs left and right of each tile row)
+  int pj = j0, pk = k0;
+  const bool valid = tid < kTInner + kTRingPairs;
+  if (tid < kTInner) {
+    pj = j0 + (tid >> 5);
+    pk = k0 + 2 * (tid & 31);
+  } else if (valid) {
+    const int u = tid - kTInner;
+    if (u < 6 * HP) {
+      const int r = u / HP, c = u - r * HP;
+      pj = r < 3 ? j0 - 3 + r : j0 + kTTJ + r - 3;
+      pk = k0 - kTH + 2 * c;
+    } else {
+      const int v = u - 6 * HP, side = v & 3;
+      pj = j0 + (v >> 2);
+      pk = side < 2 ? k0 - kTH + 2 * side : k0 + kTTK + 2 * (side - 2);
+    }
+  }
+  const bool pair_in = valid && pj >= 0 && pj < n && pk >= 0 && pk < n;
+  // how far each cell of the pair lies outside the tile (more than 3: idle)
+  auto dist = [](int x, int lo, int len) { return x < lo ? lo - x : (x >= lo + len ? x - (lo + len - 1) : 0); };
+  const int wj = dist(pj, j0, kTTJ);
+  const int w0 = pair_in ? max(wj, dist(pk, k0, kTTK)) : 99;
+  const int w1 = pair_in ? max(wj, dist(pk + 1, k0, kTTK)) : 99;
+  const int qj = pj - j0 + kTH, qk0 = pk - k0 + kTH;  // LDS position of cell pk
+
+  const int s0 = i0 - kTH;  // lowest plane loaded
+  auto slot = [&](int p) { return (p - s0 + 16) & 7; };
+  auto clampn = [&](int v) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); };
+  constexpr uint32_t es = sizeof(T);
+  const size_t plane = (size_t)n * n;
+  const uint32_t pbytes = (uint32_t)(plane * es), rbytes = (uint32_t)n * es;
+  const uint32_t voff = pair_in ? ((uint32_t)pj * n + pk) * es : kOOB;
+  const uint32_t vbytes = (uint32_t)((size_t)n * plane * es);
+  const Rsrc Rold = make_rsrc(dold, vbytes), Rnew = make_rsrc(dnew, vbytes), Rres = make_rsrc(res, vbytes);
+  const Rsrc Rlam = make_rsrc(LAMF ? k.lam : nullptr, LAMF ? vbytes : 0u);
+  const Rsrc Rl0 = make_rsrc(ALPHAF ? k.lo[0] : nullptr, ALPHAF ? vbytes : 0u);
+  const Rsrc Rl1 = make_rsrc(ALPHAF ? k.lo[1] : nullptr, ALPHAF ? vbytes : 0u);
+  const Rsrc Rl2 = make_rsrc(ALPHAF ? k.lo[2] : nullptr, ALPHAF ? vbytes : 0u);
+  const Rsrc Rw0 = make_rsrc(ALPHAF ? k.wall[0] : nullptr, ALPHAF ? pbytes : 0u);
+  const Rsrc Rw1 = make_rsrc(ALPHAF ? k.wall[1] : nullptr, ALPHAF ? pbytes : 0u);
+  const Rsrc Rw2 = make_rsrc(ALPHAF ? k.wall[2] : nullptr, ALPHAF ? pbytes : 0u);
+  auto pso = [&](int p) { return (uint32_t)clampn(p) * pbytes; };
+  auto wso = [&](int p) { return (uint32_t)clampn(p) * rbytes; };
+
+  // field plane p over the region; outside the level, the cells the ghosts
+  // mirror (plane, row and pair clamped into the level)
+  auto fetch_plane = [&](int p, V2* v) {
+    const uint32_t so = pso(p);
+#pragma unroll
+    for (int m = 0; m < NLoad; m++) {
+      const int q = tid + m * kTBlock;
+      const int row = q / HP, c2 = q - row * HP;
+      const int gj = clampn(j0 - kTH + row), gk = k0 - kTH + 2 * c2;
+      const int mk = gk < 0 ? 0 : (gk > n - 2 ? n - 2 : gk);
+      v[m] = bld2<T>(Rold, q < Region2 ? ((uint32_t)gj * n + mk) * es : kOOB, so);
+    }
+  };
+  // ... stored with the ghost rule: sigma times the mirrored cell
+  auto commit_plane = [&](int p, const V2* v) {
+    T(*P)[RK] = buf[slot(p)];
+#pragma unroll
+    for (int m = 0; m < NLoad; m++) {
+      const int q = tid + m * kTBlock;
+      if (q < Region2) {
+        const int row = q / HP, c2 = q - row * HP;
+        const int gj = j0 - kTH + row, gk = k0 - kTH + 2 * c2;
+        V2 w = v[m];
+        if (p < 0 || p >= n) {
+          const T sx = T(bc.sigma[p < 0 ? 1 : 0]);
+          w.x = sx * w.x;
+          w.y = sx * w.y;
+        } else if (gj < 0 || gj >= n) {
+          const T sy = T(bc.sigma[gj < 0 ? 3 : 2]);
+          w.x = sy * w.x;
+          w.y = sy * w.y;
+        } else if (gk < 0) {
+          w.y = T(bc.sigma[5]) * w.x;  // cell -1 mirrors cell 0
+        } else if (gk >= n) {
+          w.x = T(bc.sigma[4]) * w.y;  // cell n mirrors cell n-1
+        }
+        *reinterpret_cast<V2*>(&P[row][2 * c2]) = w;
+      }
+    }
+  };
+  // coefficients of the pair on plane p (clamped into the level)
+  auto fetch_pair = [&](int p, PairCoef<T>& c) {
+    const uint32_t so = pso(p), sw = wso(p);
+    c.res = bld2<T>(Rres, voff, so);
+    if (LAMF) c.lam = bld2<T>(Rlam, voff, so);
+    if (ALPHAF) {
+      const bool lastj = pair_in && pj == n - 1, lastk = pair_in && pk + 2 == n, lastp = clampn(p) == n - 1;
+      c.ay = bld2<T>(Rl1, voff, so);
+      const V2 yl = bld2<T>(Rl1, lastj ? kOOB : voff + rbytes, so), yw = bld2<T>(Rw1, lastj ? pk * es : kOOB, sw);
+      c.ayh.x = bor(yl.x, yw.x);
+      c.ayh.y = bor(yl.y, yw.y);
+      c.az = bld2<T>(Rl2, voff, so);
+      c.azh = bor(bld1<T>(Rl2, lastk ? kOOB : voff + 2 * es, so), bld1<T>(Rw2, lastk ? pj * es : kOOB, sw));
+      c.axl = bld2<T>(Rl0, voff, so);
+      c.axh = bld2<T>(lastp ? Rw0 : Rl0, voff, lastp ? 0u : so + pbytes);
+    }
+  };
+  auto pick = [](V2 v, int e) { return e ? v.y : v.x; };
+  // relax cell e of the pair on plane p with the pair's coefficients cp
+  auto relax_at = [&](int p, int e, const PairCoef<T>& cp) {
+    const int qk = qk0 + e, sl = slot(p);
+    T ap[3], am[3], al[3], ah[3];
+    ap[0] = buf[slot(p + 1)][qj][qk];
+    am[0] = buf[slot(p - 1)][qj][qk];
+    ap[1] = buf[sl][qj + 1][qk];
+    am[1] = buf[sl][qj - 1][qk];
+    ap[2] = buf[sl][qj][qk + 1];
+    am[2] = buf[sl][qj][qk - 1];
+    if (ALPHAF) {
+      al[0] = pick(cp.axl, e);
+      ah[0] = pick(cp.axh, e);
+      al[1] = pick(cp.ay, e);
+      ah[1] = pick(cp.ayh, e);
+      al[2] = pick(cp.az, e);
+      ah[2] = e ? cp.azh : cp.az.y;
+    } else {
+      al[0] = al[1] = al[2] = ah[0] = ah[1] = ah[2] = k.alpha;
+    }
+    return relax_nd<T>(D2, pick(cp.res, e), LAMF ? pick(cp.lam, e) : k.lam_c, al, ah, ap, am);
+  };
+  // an A value: the cell, and sigma times it in the ghosts that mirror it
+  auto put_a = [&](int p, int e, T v) {
+    const int qk = qk0 + e, kc = pk + e;
+    T(*P)[RK] = buf[slot(p)];
+    P[qj][qk] = v;
+    if (pj == 0) P[qj - 1][qk] = T(bc.sigma[3]) * v;
+    if (pj == n - 1) P[qj + 1][qk] = T(bc.sigma[2]) * v;
+    if (kc == 0) P[qj][qk - 1] = T(bc.sigma[5]) * v;
+    if (kc == n - 1) P[qj][qk + 1] = T(bc.sigma[4]) * v;
+    if (p == 0) buf[slot(-1)][qj][qk] = T(bc.sigma[1]) * v;
+    if (p == n - 1) buf[slot(n)][qj][qk] = T(bc.sigma[0]) * v;
+  };
+  // the planes each stage covers, clipped to the level
+  const int lo3 = max(0, i0 - 3), hi3 = min(n - 1, i1 + 2);
+  const int lo2 = max(0, i0 - 2), hi2 = min(n - 1, i1 + 1);
+  const int lo1 = max(0, i0 - 1), hi1 = min(n - 1, i1);
+
+  V2 dreg[NLoad];
+  // step s with the records of planes s (r0), s-1 (r1), s-2 (r2), s-3 (r3);
+  // loads plane s+2 of the field and the record of plane s+1 (nx)
+  auto step = [&](int s, const PairCoef<T>& r3, const PairCoef<T>& r2, const PairCoef<T>& r1,
+                  const PairCoef<T>& r0, PairCoef<T>& nx) {
+    commit_plane(s + 1, dreg);
+    fetch_plane(s + 2, dreg);
+    fetch_pair(s + 1, nx);
+    __syncthreads();
+    const int e = (s + pj + pk) & 1;  // the cell of the pair that is red on plane s
+    const int w = e ? w1 : w0;
+    if (w <= 3 && s >= lo3 && s <= hi3) put_a(s, e, relax_at(s, e, r0));
+    if (w <= 2 && s - 1 >= lo2 && s - 1 <= hi2) put_a(s - 1, e, relax_at(s - 1, e, r1));
+    if (w <= 1 && s - 2 >= lo1 && s - 2 <= hi1) buf[slot(s - 2)][qj][qk0 + e] = relax_at(s - 2, e, r2);
+    if (w == 0 && s - 3 >= i0 && s - 3 < i1) {
+      // B's black cell of plane s-3, written out with B's red one
+      const int q = s - 3;
+      const T v = relax_at(q, e, r3);
+      const T r = buf[slot(q)][qj][qk0 + 1 - e];
+      V2 out;
+      out.x = e ? r : v;
+      out.y = e ? v : r;
+      bst2(Rnew, voff, (uint32_t)q * pbytes, out);
+    }
+  };
+
+  {
+    V2 tmp[NLoad];
+    fetch_plane(i0 - 4, tmp);
+    commit_plane(i0 - 4, tmp);
+    fetch_plane(i0 - 3, tmp);
+    commit_plane(i0 - 3, tmp);
+  }
+  fetch_plane(i0 - 2, dreg);
+  PairCoef<T> R0{}, R1{}, R2{}, R3{}, R4{};
+  fetch_pair(i0 - 3, R0);
+  // steps i0-3 .. i1+2; the record of plane s is R[(s - i0 + 3) % 5]
+  for (int s = i0 - 3; s <= i1 + 2; s += 5) {
+    step(s, R2, R3, R4, R0, R1);
+    if (s + 1 > i1 + 2) break;
+    step(s + 1, R3, R4, R0, R1, R2);
+    if (s + 2 > i1 + 2) break;
+    step(s + 2, R4, R0, R1, R2, R3);
+    if (s + 3 > i1 + 2) break;
+    step(s + 3, R0, R1, R2, R3, R4);
+    if (s + 4 > i1 + 2) break;
+    step(s + 4, R1, R2, R3, R4, R0);
+  }
+}
+
 // --- correction, residual and first restriction in one pass, 3D --------
 //
 // a_new = a + da on the finest level (the correction that ends mg_cycle,

@@ -737,6 +737,18 @@ struct bmg_tree {
   double *a = nullptr, *b = nullptr, *res64 = nullptr, *g[3] = {nullptr, nullptr, nullptr};
   double* s = nullptr;  // the user scalar adapt_wavelet estimates (BMG_F_S)
   double* uf[3] = {nullptr, nullptr, nullptr};  // project()'s face velocity per slot (allocated on use)
+  // the adjoint's inverse tables and scratch (built on first use per mesh)
+  struct Adj {
+    bool built = false;
+    int *fop_gs = nullptr, *fop_gm = nullptr;      // per slot and dimension: face op with that gs / gm
+    int *box_off = nullptr, *box_inv = nullptr;    // per source slot: its box rules, rule order
+    int* pro_off = nullptr;                        // per ring slot: bilinear terms naming it
+    bmgt::ProEnt* pro_inv = nullptr;
+    int *nbr_off = nullptr, *nbr_inv = nullptr;    // per slot: (active index, direction) reading it
+    uint8_t* slot_leaf = nullptr;
+    double *xb = nullptr, *gb[3] = {nullptr, nullptr, nullptr}, *db = nullptr, *rb = nullptr;
+    double *lv0 = nullptr, *lv1 = nullptr, *lv2 = nullptr, *lv3 = nullptr, *lv4 = nullptr;  // leaf-order vectors
+  } adj;
   bool s_set = false;
   double *al64[3] = {nullptr, nullptr, nullptr}, *lam64 = nullptr;
   void *da = nullptr, *res = nullptr, *tmp = nullptr, *alT[3] = {nullptr, nullptr, nullptr}, *lamT = nullptr;
@@ -848,6 +860,15 @@ void free_device(bmg_tree* t) {
   for (int d = 0; d < 3; d++) {
     dfree_(t->uf[d]);
     t->uf[d] = nullptr;
+  }
+  {
+    auto& A = t->adj;
+    for (void* p : {(void*)A.fop_gs, (void*)A.fop_gm, (void*)A.box_off, (void*)A.box_inv, (void*)A.pro_off,
+                    (void*)A.pro_inv, (void*)A.nbr_off, (void*)A.nbr_inv, (void*)A.slot_leaf, (void*)A.xb,
+                    (void*)A.gb[0], (void*)A.gb[1], (void*)A.gb[2], (void*)A.db, (void*)A.rb, (void*)A.lv0,
+                    (void*)A.lv1, (void*)A.lv2, (void*)A.lv3, (void*)A.lv4})
+      dfree_(p);
+    A = bmg_tree::Adj();
   }
   if (mixed) { dfree_(t->lamT); dfree_(t->res); }
   dfree_(t->da); dfree_(t->tmp);
@@ -2842,6 +2863,402 @@ int tree_project_t(bmg_tree* t, double* vals, double dt, int nrelax, bmg_mgstats
 }  // namespace
 
 extern "C" {
+
+}  // extern "C"
+
+namespace {
+
+// --- the adjoint of poisson() on the tree --------------------------------
+
+template <typename T>
+int download(std::vector<T>& v, const T* dev, size_t n) {
+  v.resize(n);
+  if (n) HIPCHECK(hipMemcpy(v.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost));
+  return BMG_OK;
+}
+
+// counting sort of (key, value) pairs by key, stable: CSR offsets over
+// nkeys and the values in their original order per key
+template <typename V>
+void csr(int nkeys, const std::vector<int>& key, const std::vector<V>& val, std::vector<int>& off,
+         std::vector<V>& out) {
+  off.assign(nkeys + 1, 0);
+  for (int k : key) off[k + 1]++;
+  for (int i = 0; i < nkeys; i++) off[i + 1] += off[i];
+  out.resize(val.size());
+  std::vector<int> cur(off.begin(), off.end() - 1);
+  for (size_t i = 0; i < key.size(); i++) out[cur[key[i]]++] = val[i];
+}
+
+int nslot_of(const DevIndex& x, int l) { return (int)((l < x.depth ? x.off[l + 1] : x.nslot) - x.off[l]); }
+
+AdjLv adj_levels(const bmg_tree* t) {
+  const DevIndex& x = t->ix;
+  AdjLv L;
+  memset(&L, 0, sizeof L);
+  L.nl = x.depth + 1;
+  for (int l = 0; l <= x.depth; l++) {
+    L.off[l] = x.off[l];
+    L.nslot[l] = nslot_of(x, l);
+    L.nA[l] = x.nA[l];
+    L.abase[l] = x.abase[l];
+    L.D[l] = t->L0 * (1. / (1 << l));
+  }
+  return L;
+}
+
+// the inverse tables of the transposed stages, from the compiled tree
+int adj_build(bmg_tree* t) {
+  auto& A = t->adj;
+  if (A.built) return BMG_OK;
+  const DevIndex& x = t->ix;
+  const int dim = t->dim, D = x.depth, NR = dim == 3 ? 27 : 9, NCH = 1 << dim;
+  if (D + 1 > kAdjLevels) return bmg::fail(BMG_ERR_UNSUPPORTED, "the adjoint handles trees of depth < %d", kAdjLevels);
+  if (x.nslot >= (1L << 30)) return bmg::fail(BMG_ERR_UNSUPPORTED, "too many slots for the adjoint tables");
+  const int NS = (int)x.nslot;
+  // face ops: which op has a slot as gs / gm, per dimension
+  std::vector<FaceOp> F;
+  TRY(download(F, x.fops, (size_t)x.nfaceval));
+  std::vector<int> fgs((size_t)NS * dim, -1), fgm((size_t)NS * dim, -1);
+  for (size_t o = 0; o < F.size(); o++) {
+    int& a = fgs[(size_t)F[o].gs * dim + F[o].d];
+    int& b = fgm[(size_t)F[o].gm * dim + F[o].d];
+    if (a >= 0 || b >= 0) return bmg::fail(BMG_ERR_ARG, "internal: a slot is the face of two ops");
+    a = b = (int)o;
+  }
+  // box rules by source (rule order)
+  const int nrules = x.box_off[D] + x.box_n[D];
+  std::vector<BoxRule> R;
+  TRY(download(R, x.box, (size_t)nrules));
+  std::vector<int> rk, rv, roff, rinv;
+  for (int r = 0; r < nrules; r++)
+    if (R[r].nterm) {
+      rk.push_back(R[r].src);
+      rv.push_back(r);
+    }
+  csr(NS, rk, rv, roff, rinv);
+  // bilinear terms by ring slot (record, child, term order)
+  const int nrec = x.rec_off[D] + x.n_nonleaf[D] + x.n_prolong[D];
+  std::vector<int> ring;
+  TRY(download(ring, x.rec_ring, (size_t)nrec * NR));
+  std::vector<int> pk;
+  std::vector<bmgt::ProEnt> pv;
+  pk.reserve((size_t)nrec * NCH * NCH);
+  pv.reserve((size_t)nrec * NCH * NCH);
+  for (int r = 0; r < nrec; r++)
+    for (int m = 0; m < NCH; m++) {
+      int a = dim == 3 ? m >> 2 : m >> 1, b = dim == 3 ? (m >> 1) & 1 : m & 1, e = dim == 3 ? m & 1 : 0;
+      int cx = 2 * a - 1, cy = 2 * b - 1, cz = 2 * e - 1;
+      static const int T3[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+      static const int T2[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+      for (int k = 0; k < NCH; k++) {
+        int pos;
+        if (dim == 3)
+          pos = ((1 + T3[k][0] * cx) * 3 + 1 + T3[k][1] * cy) * 3 + 1 + T3[k][2] * cz;
+        else
+          pos = (1 + T2[k][0] * cx) * 3 + 1 + T2[k][1] * cy;
+        const int slot = ring[(size_t)r * NR + pos];
+        if (slot < 0 || slot >= NS) return bmg::fail(BMG_ERR_ARG, "internal: ring slot out of range");
+        pk.push_back(slot);
+        pv.push_back(bmgt::ProEnt{r, (unsigned char)m, (unsigned char)k, 0, 0});
+      }
+    }
+  std::vector<int> poff;
+  std::vector<bmgt::ProEnt> pinv;
+  csr(NS, pk, pv, poff, pinv);
+  // neighbour reads of relax by receiving slot (active order)
+  const int nact = x.abase[D] + x.nA[D];
+  if ((long)nact * 8 >= (1L << 31)) return bmg::fail(BMG_ERR_UNSUPPORTED, "too many cells for the adjoint tables");
+  std::vector<int> nb;
+  TRY(download(nb, x.nbr, (size_t)nact * 2 * dim));
+  std::vector<int> nk, nv, noff, ninv;
+  nk.reserve(nb.size());
+  nv.reserve(nb.size());
+  for (int ai = 0; ai < nact; ai++)
+    for (int dir = 0; dir < 2 * dim; dir++) {
+      nk.push_back(nb[(size_t)ai * 2 * dim + dir]);
+      nv.push_back(ai * 8 + dir);
+    }
+  csr(NS, nk, nv, noff, ninv);
+  TRY(upload(&A.fop_gs, fgs));
+  TRY(upload(&A.fop_gm, fgm));
+  TRY(upload(&A.box_off, roff));
+  TRY(upload(&A.box_inv, rinv));
+  TRY(upload(&A.pro_off, poff));
+  TRY(upload(&A.pro_inv, pinv));
+  TRY(upload(&A.nbr_off, noff));
+  TRY(upload(&A.nbr_inv, ninv));
+  TRY(dalloc_((void**)&A.slot_leaf, std::max(1, NS)));
+  HIPCHECK(hipMemset(A.slot_leaf, 0, std::max(1, NS)));
+  if (x.nleaf) k_ta_leaf_mark<<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, A.slot_leaf);
+  for (double** p : {&A.xb, &A.gb[0], &A.gb[1], &A.gb[2], &A.db, &A.rb})
+    TRY(dalloc_((void**)p, std::max(1, NS) * sizeof(double)));
+  for (double** p : {&A.lv0, &A.lv1, &A.lv2, &A.lv3, &A.lv4})
+    TRY(dalloc_((void**)p, std::max<long>(1, x.nleaf) * sizeof(double)));
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  A.built = true;
+  return BMG_OK;
+}
+
+template <int DIM>
+struct AdjOps {
+  bmg_tree* t;
+  const DevIndex& x;
+  AdjLv L;
+  const double* al[3] = {nullptr, nullptr, nullptr};
+  const double* lam;
+  BCT bc;
+  explicit AdjOps(bmg_tree* t_) : t(t_), x(t_->ix), L(adj_levels(t_)), lam(t_->lambda_field ? t_->lam64 : nullptr),
+                                  bc(bct(t_)) {
+    if (t->alpha_field)
+      for (int d = 0; d < DIM; d++) al[d] = t->al64[d];
+  }
+  void box_T(double* f, int l, int sym) {
+    const int n = nslot_of(x, l);
+    if (n) k_ta_box_T<<<nb_(n), kB, 0, t->stream>>>(x.off[l], n, t->adj.box_off, t->adj.box_inv, x.box, bc, sym, f);
+    if (x.box_n[l]) k_ta_box_zero<<<nb_(x.box_n[l]), kB, 0, t->stream>>>(x.box + x.box_off[l], x.box_n[l], f);
+  }
+  // bilinear transposed for the records [r0, r0 + nr) of level l
+  void bilinear_T(double* f, int l, int r0, int nr) {
+    if (!nr) return;
+    const int n = nslot_of(x, l);
+    k_ta_bilinear_T<DIM><<<nb_(n), kB, 0, t->stream>>>(x.off[l], n, t->adj.pro_off, t->adj.pro_inv, x.rec_child0,
+                                                       r0, r0 + nr, f);
+    k_ta_children_zero<DIM><<<nb_(nr), kB, 0, t->stream>>>(x.rec_child0, r0, nr, f);
+  }
+  void restrict_T(double* f, int m) {
+    const int n = x.n_nonleaf[m];
+    if (n)
+      k_ta_restrict_T<DIM><<<nb_(n), kB, 0, t->stream>>>(x.rec_self + x.rec_off[m], x.rec_child0 + x.rec_off[m], n,
+                                                         f);
+  }
+  // tree_boundary_level (f, lev) transposed (lev < 0: depth)
+  void boundary_T(double* f, int lev, int sym) {
+    const int depth = lev < 0 ? x.depth : lev;
+    if (t->topo.full()) {
+      box_T(f, depth, sym);
+      return;
+    }
+    for (int i = depth - 1; i >= 0; i--) {
+      box_T(f, i + 1, sym);
+      bilinear_T(f, i, x.rec_off[i] + x.n_nonleaf[i], x.n_prolong[i]);
+    }
+    box_T(f, 0, sym);
+    for (int m = 0; m < depth; m++) restrict_T(f, m);
+  }
+  // y (leaf order) -> gb (residual_T and halo_face_T), xb[leaves] = -lambda y
+  void faces_of(const double* y) {
+    auto& A = t->adj;
+    const size_t bytes = (size_t)std::max<long>(1, x.nslot) * sizeof(double);
+    (void)hipMemsetAsync(A.xb, 0, bytes, t->stream);
+    for (int d = 0; d < 3; d++) (void)hipMemsetAsync(A.gb[d], 0, bytes, t->stream);
+    if (x.nleaf)
+      k_ta_div_T<DIM><<<nb_(x.nleaf), kB, 0, t->stream>>>(L, x.leaf_gs, x.nleaf, y, x.nbr, lam, t->lambda_c, A.xb,
+                                                          A.gb[0], A.gb[1], A.gb[2]);
+    if (x.nhf) k_ta_halo_face_T<DIM><<<nb_(x.nhf), kB, 0, t->stream>>>(x.hf, x.nhf, A.gb[0], A.gb[1], A.gb[2]);
+  }
+  // out = r^T y (leaf order, device)
+  void residual_T(const double* y, double* out) {
+    auto& A = t->adj;
+    faces_of(y);
+    if (x.nslot)
+      k_ta_faces_T<DIM><<<nb_(x.nslot), kB, 0, t->stream>>>(L, (int)x.nslot, x.fops, A.fop_gs, A.fop_gm, al[0], al[1],
+                                                            al[2], t->alpha_c, A.gb[0], A.gb[1], A.gb[2], A.xb);
+    boundary_T(A.xb, -1, 0);
+    if (x.nleaf) k_ta_leaf_get<<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, A.xb, out);
+  }
+  // one red-black pass of relax (level l, colour) transposed
+  void relax_T(int l, int col) {
+    auto& A = t->adj;
+    const int nrecv = x.off[l] + nslot_of(x, l);
+    if (nrecv)
+      k_ta_relax_T<DIM><<<nb_(nrecv), kB, 0, t->stream>>>(L, nrecv, A.nbr_off, A.nbr_inv, x.nbr, x.colour,
+                                                          A.slot_leaf, l, col, al[0], al[1], al[2], t->alpha_c, lam,
+                                                          t->lambda_c, A.db);
+    Segs S = level_or_leaf_colour(t, l, col);
+    if (S.first_block[S.n])
+      k_ta_relax_cells_T<DIM><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(S, x.nbr, al[0], al[1], al[2],
+                                                                                  t->alpha_c, lam, t->lambda_c,
+                                                                                  A.db, A.rb);
+  }
+  // out = M^T y: the transposed V-cycle (red-black relax) from da = 0
+  void cycle_T(const double* y, double* out, int nrelax, int minlevel, int maxlevel) {
+    auto& A = t->adj;
+    const size_t bytes = (size_t)std::max<long>(1, x.nslot) * sizeof(double);
+    (void)hipMemsetAsync(A.db, 0, bytes, t->stream);
+    (void)hipMemsetAsync(A.rb, 0, bytes, t->stream);
+    if (x.nleaf) k_ta_leaf_set<<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, y, A.db);
+    minlevel = std::min(minlevel, maxlevel);
+    for (int l = maxlevel; l >= minlevel; l--) {
+      for (int it = 0; it < nrelax; it++) {
+        boundary_T(A.db, l, 0);
+        relax_T(l, 1);
+        for (int m = l; m >= 0; m--) box_T(A.db, m, 0);
+        relax_T(l, 0);
+      }
+      boundary_T(A.db, l, 0);
+      if (l == minlevel) {
+        Segs S = level_or_leaf(t, l);
+        if (S.first_block[S.n]) k_t_fill<double><<<(unsigned)S.first_block[S.n], kB, 0, t->stream>>>(S, A.db, 0.);
+      } else {
+        bilinear_T(A.db, l - 1, x.rec_off[l - 1], x.n_nonleaf[l - 1]);
+      }
+    }
+    if (t->topo.full()) {
+      for (int l = 0; l < x.depth; l++) {
+        box_T(A.rb, l, 1);
+        restrict_T(A.rb, l);
+      }
+      box_T(A.rb, x.depth, 1);
+    } else {
+      boundary_T(A.rb, -1, 1);
+    }
+    if (x.nleaf) k_ta_leaf_get<<<nb_(x.nleaf), kB, 0, t->stream>>>(x.leaf_gs, x.nleaf, A.rb, out);
+  }
+};
+
+template <int DIM>
+int tree_adjoint_t(bmg_tree* t, const double* g, double tolerance, int nitermax, int nrelax, double* psi,
+                   double* galpha, double* glambda, bmg_mgstats* st) {
+  const DevIndex& x = t->ix;
+  auto& A = t->adj;
+  AdjOps<DIM> O(t);
+  const long N = x.nleaf;
+  const unsigned nbl = nb_(std::max<long>(1, N));
+  double *rho = A.lv0, *tmp = A.lv1;
+  HIPCHECK(hipMemcpyAsync(rho, g, N * sizeof(double), hipMemcpyDeviceToDevice, t->stream));
+  HIPCHECK(hipMemsetAsync(psi, 0, N * sizeof(double), t->stream));
+  // resb = max|g|
+  k_t_stat_reset<<<1, 64, 0, t->stream>>>(t->stat);
+  HIPCHECK(hipMemsetAsync(tmp, 0, N * sizeof(double), t->stream));
+  k_ta_update<<<nbl, kB, 0, t->stream>>>(N, psi, tmp, g, tmp, rho, t->stat, 1);
+  TStat ts;
+  HIPCHECK(hipMemcpyAsync(&ts, t->stat, sizeof ts, hipMemcpyDeviceToHost, t->stream));
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  double resb, resa;
+  memcpy(&resb, &ts.maxbits, sizeof resb);
+  resa = resb;
+  int it;
+  for (it = 0; it < nitermax && (it < 1 || resa > tolerance); it++) {
+    O.cycle_T(rho, tmp, nrelax, 1, x.depth);
+    k_ta_update<<<nbl, kB, 0, t->stream>>>(N, psi, tmp, g, tmp, rho, t->stat, 0);
+    O.residual_T(psi, tmp);
+    k_t_stat_reset<<<1, 64, 0, t->stream>>>(t->stat);
+    k_ta_update<<<nbl, kB, 0, t->stream>>>(N, psi, tmp, g, tmp, rho, t->stat, 1);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(&ts, t->stat, sizeof ts, hipMemcpyDeviceToHost, t->stream));
+    HIPCHECK(hipStreamSynchronize(t->stream));
+    if (ts.nans) return bmg::fail(BMG_ERR_NAN, "adjoint: NaN in the residual");
+    memcpy(&resa, &ts.maxbits, sizeof resa);
+  }
+  if (galpha && x.nfaceval) {
+    {
+      Batcher<double, DIM> B(t, t->a, nullptr, nullptr);
+      B.tree_boundary(0, BK_A, -1);
+      B.flush();
+    }
+    O.faces_of(psi);
+    k_ta_galpha<DIM><<<nb_(x.nfaceval), kB, 0, t->stream>>>(O.L, x.nfaceval, x.fv_gs, x.fv_d, A.fop_gs, x.fops, t->a,
+                                                            A.gb[0], A.gb[1], A.gb[2], galpha);
+  }
+  if (glambda && N) k_ta_glambda<<<nb_(N), kB, 0, t->stream>>>(x.leaf_gs, N, psi, t->a, glambda);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  if (st) {
+    memset(st, 0, sizeof *st);
+    st->i = it;
+    st->resb = resb;
+    st->resa = resa;
+    st->nrelax = nrelax;
+    st->minlevel = 1;
+  }
+  return BMG_OK;
+}
+
+// a leaf-order (or face-value) array in device memory: the pointer itself
+// when on_device, else a device copy (copied back by give)
+struct LeafBuf {
+  double* dev = nullptr;
+  double* host = nullptr;
+  bool owned = false;
+  long n = 0;
+  int take(double* p, long n_, int on_device, bool copy_in) {
+    n = n_;
+    if (!p) return BMG_OK;
+    if (on_device) {
+      dev = p;
+      return BMG_OK;
+    }
+    host = p;
+    owned = true;
+    TRY(dalloc_((void**)&dev, std::max<long>(1, n) * sizeof(double)));
+    if (copy_in && n) HIPCHECK(hipMemcpy(dev, p, n * sizeof(double), hipMemcpyHostToDevice));
+    return BMG_OK;
+  }
+  int give() {
+    if (owned && host && n) HIPCHECK(hipMemcpy(host, dev, n * sizeof(double), hipMemcpyDeviceToHost));
+    return BMG_OK;
+  }
+  ~LeafBuf() {
+    if (owned) dfree_(dev);
+  }
+};
+
+int adj_ready(bmg_tree* t) {
+  TRY(ensure(t));
+  TRY(coefs(t));
+  return adj_build(t);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_tree_residual_transpose(bmg_tree* t, const double* y, double* out, int on_device) {
+  if (!t || !y || !out) return bmg::fail(BMG_ERR_ARG, "null argument");
+  TRY(adj_ready(t));
+  LeafBuf Y, O;
+  TRY(Y.take((double*)y, t->ix.nleaf, on_device, true));
+  TRY(O.take(out, t->ix.nleaf, on_device, false));
+  if (t->dim == 3) AdjOps<3>(t).residual_T(Y.dev, O.dev);
+  else AdjOps<2>(t).residual_T(Y.dev, O.dev);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  return O.give();
+}
+
+int bmg_tree_cycle_transpose(bmg_tree* t, const double* y, double* out, int nrelax, int minlevel, int on_device) {
+  if (!t || !y || !out || nrelax < 0 || minlevel < 0) return bmg::fail(BMG_ERR_ARG, "bad argument");
+  TRY(adj_ready(t));
+  LeafBuf Y, O;
+  TRY(Y.take((double*)y, t->ix.nleaf, on_device, true));
+  TRY(O.take(out, t->ix.nleaf, on_device, false));
+  if (t->dim == 3) AdjOps<3>(t).cycle_T(Y.dev, O.dev, nrelax, minlevel, t->ix.depth);
+  else AdjOps<2>(t).cycle_T(Y.dev, O.dev, nrelax, minlevel, t->ix.depth);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(t->stream));
+  return O.give();
+}
+
+int bmg_tree_adjoint(bmg_tree* t, const double* g, double tolerance, int nitermax, int nrelax, double* psi,
+                     double* galpha, double* glambda, int on_device, bmg_mgstats* stats) {
+  if (!t || !g || !psi) return bmg::fail(BMG_ERR_ARG, "null argument");
+  if (nitermax < 0) return bmg::fail(BMG_ERR_ARG, "nitermax must be non-negative");
+  if (nrelax <= 0) nrelax = 4;
+  TRY(adj_ready(t));
+  const DevIndex& x = t->ix;
+  LeafBuf G, P, GA, GL;
+  TRY(G.take((double*)g, x.nleaf, on_device, true));
+  TRY(P.take(psi, x.nleaf, on_device, false));
+  TRY(GA.take(galpha, x.nfaceval, on_device, false));
+  TRY(GL.take(glambda, x.nleaf, on_device, false));
+  const int rc = t->dim == 3 ? tree_adjoint_t<3>(t, G.dev, tolerance, nitermax, nrelax, P.dev, GA.dev, GL.dev, stats)
+                             : tree_adjoint_t<2>(t, G.dev, tolerance, nitermax, nrelax, P.dev, GA.dev, GL.dev, stats);
+  if (rc != BMG_OK) return rc;
+  TRY(P.give());
+  TRY(GA.give());
+  return GL.give();
+}
 
 int bmg_tree_project(bmg_tree* t, double* uf, double dt, int nrelax, int on_device, bmg_mgstats* stats) {
   if (!t || !uf) return bmg::fail(BMG_ERR_ARG, "null argument");

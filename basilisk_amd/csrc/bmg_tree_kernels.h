@@ -666,3 +666,345 @@ __global__ __launch_bounds__(kB) void k_t_refine_remap(const int* __restrict__ r
 }
 
 }  // namespace bmgt
+
+// --- the adjoint of poisson() on the tree ----------------------------------
+//
+// The tree operator r(x) = -lambda x - sum_d (g_d[hi] - g_d[])/Delta (g =
+// halo_face of alpha (x[] - x[-1])/Delta, x after its tree boundary) is not
+// symmetric, so its adjoint runs the transposes of the forward stages in
+// reverse order: each stage that wrote y = f(x) becomes xbar += (df/dx)^T
+// ybar, ybar = 0.  Where several writes reach one value, the kernels gather
+// them through inverse tables in the forward order (face ops in face-cache
+// order, box rules in boundary-cache order, bilinear terms in record,
+// child, term order, relaxed cells in active-cache order), so every sum is
+// accumulated in one fixed order; where at most two terms reach a value
+// (the leaf divergence's two faces) an atomic add is exact in either order.
+namespace bmgt {
+
+constexpr int kAdjLevels = 24;
+
+struct AdjLv {
+  int nl;                   // levels 0 .. nl-1
+  int off[kAdjLevels];      // first slot of the level
+  int nslot[kAdjLevels];    // slots of the level
+  int nA[kAdjLevels];       // active cells of the level (first in its slots)
+  int abase[kAdjLevels];    // active index of the level's first active cell
+  double D[kAdjLevels];     // Delta of the level
+};
+
+__device__ __forceinline__ int adj_level_of_slot(const AdjLv& L, int s) {
+  int l = 0;
+  while (l + 1 < L.nl && s >= L.off[l + 1]) l++;
+  return l;
+}
+__device__ __forceinline__ int adj_level_of_ai(const AdjLv& L, int ai) {
+  int l = 0;
+  while (l + 1 < L.nl && ai >= L.abase[l + 1]) l++;
+  return l;
+}
+
+// one bilinear term of a record's child: ring position and weight numerator
+struct ProEnt {
+  int rec;
+  unsigned char child, term, pad0, pad1;
+};
+
+// homogeneous (sym = 0) or symmetry (sym = 1) factor of a box rule:
+// ghost = k * src
+__device__ __forceinline__ double box_factor(const BoxRule& q, const BCT& bc, int sym) {
+  double h[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) h[m] = sym ? 1. : (bc.kind[q.side[m]] == K_DIRICHLET ? -1. : 1.);
+  if (q.nterm == 1) return h[0];
+  if (q.nterm == 2) return h[0] + h[1] - 1.;
+  return h[0] + h[1] + h[2] - 2.;
+}
+
+// y (leaf order) -> xb[leaf] = -lambda y, gb_d[leaf] += y/Delta,
+// gb_d[hi] -= y/Delta
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_div_T(AdjLv L, const int* __restrict__ leaf_gs, long nleaf,
+                                                 const double* __restrict__ y, const int* __restrict__ nbr,
+                                                 const double* __restrict__ lam, double lam_c, double* xb,
+                                                 double* g0, double* g1, double* g2) {
+  long n = (long)blockIdx.x * kB + threadIdx.x;
+  if (n >= nleaf) return;
+  const int gs = leaf_gs[n];
+  const int l = adj_level_of_slot(L, gs);
+  const int ai = L.abase[l] + (gs - L.off[l]);
+  const int* nb = nbr + (size_t)ai * 2 * DIM;
+  const double yv = y[n], D = L.D[l];
+  xb[gs] = -(lam ? lam[gs] : lam_c) * yv;
+  double* g[3] = {g0, g1, g2};
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const double tt = yv / D;
+    atomicAdd(&g[d][gs], tt);
+    atomicAdd(&g[d][nb[2 * d + 1]], -tt);
+  }
+}
+
+// halo_face transposed: each target's adjoint to its fine faces, then 0
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_halo_face_T(const FaceAvg* __restrict__ F, int n, double* f0, double* f1,
+                                                       double* f2) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  FaceAvg q = F[r];
+  double* f = q.d == 0 ? f0 : q.d == 1 ? f1 : f2;
+  const double v = f[q.target];
+#pragma unroll
+  for (int m = 0; m < (1 << (DIM - 1)); m++) f[q.fine[m]] += v / (double)(1 << (DIM - 1));
+  f[q.target] = 0.;
+}
+
+// the face fluxes transposed: per slot, c = (alpha gb)/Delta of each face op
+// whose gs (+c) or gm (-c) it is, in face op order
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_faces_T(AdjLv L, int nslot, const FaceOp* __restrict__ fops,
+                                                   const int* __restrict__ fop_gs, const int* __restrict__ fop_gm,
+                                                   const double* al0, const double* al1, const double* al2,
+                                                   double alpha_c, const double* g0, const double* g1,
+                                                   const double* g2, double* xb) {
+  int s = blockIdx.x * kB + threadIdx.x;
+  if (s >= nslot) return;
+  int op[2 * DIM];
+  int k = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const int a = fop_gs[s * DIM + d], b = fop_gm[s * DIM + d];
+    if (a >= 0) op[k++] = a * 2;        // +c
+    if (b >= 0) op[k++] = b * 2 + 1;    // -c
+  }
+  if (!k) return;
+  // face op order (insertion sort of at most 2 dim entries)
+  for (int i = 1; i < k; i++) {
+    const int v = op[i];
+    int j = i - 1;
+    while (j >= 0 && op[j] > v) {
+      op[j + 1] = op[j];
+      j--;
+    }
+    op[j + 1] = v;
+  }
+  const double D = L.D[adj_level_of_slot(L, s)];
+  const double* al[3] = {al0, al1, al2};
+  const double* g[3] = {g0, g1, g2};
+  double v = xb[s];
+  for (int i = 0; i < k; i++) {
+    const FaceOp q = fops[op[i] >> 1];
+    const double alpha = al[q.d] ? al[q.d][q.gs] : alpha_c;
+    const double c = (alpha * g[q.d][q.gs]) / D;
+    if (op[i] & 1)
+      v -= c;
+    else
+      v += c;
+  }
+  xb[s] = v;
+}
+
+// box rules of one level transposed, gather part: each source slot of the
+// level adds k * x[ghost] of the rules naming it, in rule order
+__global__ __launch_bounds__(kB) void k_ta_box_T(int s0, int n, const int* __restrict__ inv_off,
+                                                 const int* __restrict__ inv, const BoxRule* __restrict__ R, BCT bc,
+                                                 int sym, double* x) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const int s = s0 + r;
+  const int b = inv_off[s], e = inv_off[s + 1];
+  if (b == e) return;
+  double v = x[s];
+  for (int i = b; i < e; i++) {
+    const BoxRule q = R[inv[i]];
+    v += box_factor(q, bc, sym) * x[q.ghost];
+  }
+  x[s] = v;
+}
+
+__global__ __launch_bounds__(kB) void k_ta_box_zero(const BoxRule* __restrict__ R, int n, double* x) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  x[R[r].ghost] = 0.;
+}
+
+// bilinear transposed, gather part: each ring slot of the level adds
+// (w * x[child])/den over the terms naming it, of the records in [r0, r1)
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_bilinear_T(int s0, int n, const int* __restrict__ inv_off,
+                                                      const ProEnt* __restrict__ inv,
+                                                      const int* __restrict__ child0, int r0, int r1, double* x) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const int s = s0 + r;
+  const int b = inv_off[s], e = inv_off[s + 1];
+  if (b == e) return;
+  double v = x[s];
+  bool any = false;
+  for (int i = b; i < e; i++) {
+    const ProEnt p = inv[i];
+    if (p.rec < r0 || p.rec >= r1) continue;
+    const double cb = x[child0[p.rec] + p.child];
+    double w;
+    if (DIM == 2)
+      w = p.term == 0 ? 9. : (p.term == 3 ? 1. : 3.);
+    else
+      w = p.term == 0 ? 27. : (p.term <= 3 ? 9. : (p.term <= 6 ? 3. : 1.));
+    v += (w * cb) / (DIM == 2 ? 16. : 64.);
+    any = true;
+  }
+  if (any) x[s] = v;
+}
+
+// the children of the records [r0, r1) cleared (after their bilinear_T)
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_children_zero(const int* __restrict__ child0, int r0, int n, double* x) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const int c = child0[r0 + r];
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); m++) x[c + m] = 0.;
+}
+
+// restriction transposed: each parent's adjoint to its children, then 0
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_restrict_T(const int* __restrict__ self, const int* __restrict__ child0,
+                                                      int n, double* x) {
+  int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= n) return;
+  const double pb = x[self[r]];
+  const int c = child0[r];
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); m++) x[c + m] += pb / (double)(1 << DIM);
+  x[self[r]] = 0.;
+}
+
+// the relax denominator of the cell at active index ai (slot gs)
+template <int DIM>
+__device__ __forceinline__ double adj_relax_den(const int* __restrict__ nbr, int ai, int gs, const double* const* al,
+                                                double alpha_c, const double* lam, double lam_c, double D2) {
+  const int* nb = nbr + (size_t)ai * 2 * DIM;
+  double d = -(lam ? lam[gs] : lam_c) * D2;
+#pragma unroll
+  for (int c = 0; c < DIM; c++) {
+    const double ah = al[c] ? al[c][nb[2 * c + 1]] : alpha_c, alo = al[c] ? al[c][gs] : alpha_c;
+    d += ah + alo;
+  }
+  return d;
+}
+
+// one red-black pass of relax (level lrel, colour) transposed, gather part:
+// each slot of levels 0..lrel adds (alpha v)/d of the relaxing cells that
+// read it, in active order
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_relax_T(AdjLv L, int nrecv, const int* __restrict__ inv_off,
+                                                   const int* __restrict__ inv, const int* __restrict__ nbr,
+                                                   const uint8_t* __restrict__ colour,
+                                                   const uint8_t* __restrict__ slot_leaf, int lrel, int col,
+                                                   const double* al0, const double* al1, const double* al2,
+                                                   double alpha_c, const double* lam, double lam_c, double* db) {
+  int s = blockIdx.x * kB + threadIdx.x;
+  if (s >= nrecv) return;
+  const int b = inv_off[s], e = inv_off[s + 1];
+  if (b == e) return;
+  const double* al[3] = {al0, al1, al2};
+  double v = db[s];
+  bool any = false;
+  for (int i = b; i < e; i++) {
+    const int ai = inv[i] >> 3, dir = inv[i] & 7;
+    if (colour[ai] != col) continue;
+    const int m = adj_level_of_ai(L, ai);
+    const int gs = L.off[m] + (ai - L.abase[m]);
+    if (m > lrel || (m < lrel && !slot_leaf[gs])) continue;
+    const double D2 = L.D[m] * L.D[m];
+    const double den = adj_relax_den<DIM>(nbr, ai, gs, al, alpha_c, lam, lam_c, D2);
+    const int c = dir >> 1;
+    const double a = (dir & 1) ? (al[c] ? al[c][nbr[(size_t)ai * 2 * DIM + dir]] : alpha_c)
+                               : (al[c] ? al[c][gs] : alpha_c);
+    v += (a * db[gs]) / den;
+    any = true;
+  }
+  if (any) db[s] = v;
+}
+
+// ... and the relaxing cells themselves: rb += (-Delta^2 v)/d, db = 0
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_relax_cells_T(Segs S, const int* __restrict__ nbr, const double* al0,
+                                                         const double* al1, const double* al2, double alpha_c,
+                                                         const double* lam, double lam_c, double* db, double* rb) {
+  int sg = seg_of(S);
+  int gs = seg_slot(S, sg);
+  if (gs < 0) return;
+  const int ai = S.abase[sg] + (gs - S.lvoff[sg]);
+  const double* al[3] = {al0, al1, al2};
+  const double D2 = S.D[sg] * S.D[sg];
+  const double den = adj_relax_den<DIM>(nbr, ai, gs, al, alpha_c, lam, lam_c, D2);
+  rb[gs] += (-D2 * db[gs]) / den;
+  db[gs] = 0.;
+}
+
+// values at the leaves, leaf order <-> slots
+__global__ __launch_bounds__(kB) void k_ta_leaf_get(const int* __restrict__ leaf_gs, long n, const double* x,
+                                                    double* out) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r < n) out[r] = x[leaf_gs[r]];
+}
+__global__ __launch_bounds__(kB) void k_ta_leaf_set(const int* __restrict__ leaf_gs, long n, const double* in,
+                                                    double* x) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r < n) x[leaf_gs[r]] = in[r];
+}
+__global__ __launch_bounds__(kB) void k_ta_leaf_mark(const int* __restrict__ leaf_gs, long n, uint8_t* mark) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r < n) mark[leaf_gs[r]] = 1;
+}
+
+// psi += m; rho = g + t; max |rho| into st
+__global__ __launch_bounds__(kB) void k_ta_update(long n, double* psi, const double* m, const double* g,
+                                                  const double* tt, double* rho, TStat* st, int step) {
+  __shared__ double smax[kB / 64];
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  double v = 0.;
+  if (r < n) {
+    if (step == 0) {
+      psi[r] += m[r];
+    } else {
+      const double x = g[r] + tt[r];
+      rho[r] = x;
+      v = fabs(x);
+    }
+  }
+  if (step == 0) return;
+  v = t_wave_max(v);
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = smax[0];
+    for (int i = 1; i < kB / 64; i++) mm = fmax(mm, smax[i]);
+    if (mm > 0.) atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(mm));
+    if (mm != mm) atomicAdd(&st->nans, 1ull);
+  }
+}
+
+// dJ/dalpha per face value: gb_d[gs] (a[gs] - a[gm])/Delta
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_ta_galpha(AdjLv L, long nv, const int* __restrict__ fv_gs,
+                                                  const int* __restrict__ fv_d, const int* __restrict__ fop_gs,
+                                                  const FaceOp* __restrict__ fops, const double* a, const double* g0,
+                                                  const double* g1, const double* g2, double* out) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r >= nv) return;
+  const int gs = fv_gs[r], d = fv_d[r];
+  const FaceOp q = fops[fop_gs[gs * DIM + d]];
+  const double D = L.D[adj_level_of_slot(L, gs)];
+  const double* g = d == 0 ? g0 : d == 1 ? g1 : g2;
+  out[r] = g[gs] * ((a[gs] - a[q.gm]) / D);
+}
+
+// dJ/dlambda = -psi a at the leaves
+__global__ __launch_bounds__(kB) void k_ta_glambda(const int* __restrict__ leaf_gs, long n, const double* psi,
+                                                   const double* a, double* out) {
+  long r = (long)blockIdx.x * kB + threadIdx.x;
+  if (r < n) out[r] = -psi[r] * a[leaf_gs[r]];
+}
+
+}  // namespace bmgt

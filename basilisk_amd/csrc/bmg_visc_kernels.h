@@ -40,6 +40,7 @@ struct ViscArgs {
   const double* rho;      // null: rho_c
   double rho_c;
   int sig[3][6];          // per component: right, left, top, bottom, front, back
+  int periodic[3];        // periodic dimensions (their neighbours wrap around)
   double dt, D;
 };
 
@@ -72,13 +73,14 @@ template <int DIM>
 struct VAxes {
   idx_t off[3][3];
   int lo, hi;  // bit d: the -1 / +1 neighbour along d is a ghost
-  __device__ __forceinline__ void init(const Geo<DIM>& g, const int x[3]) {
+  __device__ __forceinline__ void init(const Geo<DIM>& g, const int x[3], const int* periodic) {
     lo = hi = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++)
 #pragma unroll
       for (int o = -1; o <= 1; o++) {
         int q = d < DIM ? x[d] + o : 0;
+        if (d < DIM && periodic[d]) q = q < 0 ? q + g.n : (q >= g.n ? q - g.n : q);
         const bool l = d < DIM && q < 0, h = d < DIM && q >= g.n;
         if (l) lo |= 1 << d;
         if (h) hi |= 1 << d;
@@ -176,7 +178,7 @@ template <int DIM>
 __device__ __forceinline__ void visc_target(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3],
                                             double w[3]) {
   VAxes<DIM> X;
-  X.init(g, x);
+  X.init(g, x, A.periodic);
   VNb<DIM> N;
   visc_load<DIM>(A, X, N);
   double mu[3][2];
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_residual(ViscArgs A, Geo<DIM> g
     int x[3];
     g.decode(idx, x);
     VAxes<DIM> X;
-    X.init(g, x);
+    X.init(g, x, A.periodic);
     VNb<DIM> N;
     visc_load<DIM>(A, X, N);
     double mu[3][2];

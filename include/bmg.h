@@ -189,9 +189,12 @@ int bmg_correction(bmg_grid *g, double *u_x, double *u_y, double *u_z, const dou
    or NULL for the constant rho_c.  Walls at rest: each component's ghost is
    sigma times the mirrored cell, sigma[6*c + side] = +1 or -1 (NULL: the
    reference's default vector BCs, -1 for the normal component and +1 for the
-   tangential ones; -1 everywhere is no slip).  2D and 3D; periodic grids are
-   refused.  Device pointers when on_device, else host arrays.  FP64,
-   whatever the grid's precision. */
+   tangential ones; -1 everywhere is no slip).  Periodic dimensions
+   (grid/multigrid.h:600-701) wrap: the ghosts copy the opposite cells, on
+   level 0 every ghost is the root value, and face n of mu along a periodic
+   dimension is face 0 (the value given for face n is not read).  2D and 3D;
+   the GS smoother is refused.  Device pointers when on_device, else host
+   arrays.  FP64, whatever the grid's precision. */
 int bmg_viscosity(bmg_grid *g, double *u_x, double *u_y, double *u_z, const double *mu_x, const double *mu_y,
                   const double *mu_z, double mu_c, const double *rho, double rho_c, double dt, const int *sigma,
                   int nrelax, int on_device, bmg_mgstats *stats);
@@ -223,8 +226,10 @@ int bmg_coef_gradient(bmg_grid *g, const double *psi, double *galpha_x, double *
                       double *galpha_z, double *glambda, int on_device);
 
 /* kernel timing with HIP events on the grid's stream; kind 0 = fine-level
-   relax, 1 = residual, 2 = restriction, 3 = prolongation, 4 = correction,
-   5 = coarse-level relax */
+   relax (one sweep per launch), 1 = residual, 2 = restriction,
+   3 = prolongation, 4 = correction, 5 = coarse-level relax, 6 = fine-level
+   relax of the two-sweep kernel (two sweeps per launch; FP32 levels of
+   256^3 cells and more) */
 int bmg_profile_enable(bmg_grid *g, int enable);
 int bmg_profile_get(bmg_grid *g, int kind, double *total_ms, long *count);
 
@@ -426,6 +431,28 @@ int bmg_tree_poisson(bmg_tree *t, double tolerance, int nrelax, int minlevel,
    dt = 0 means 1, nrelax = 0 means 4.  Device memory when on_device, else
    a host array. */
 int bmg_tree_project(bmg_tree *t, double *uf, double dt, int nrelax, int on_device, bmg_mgstats *stats);
+/* The adjoint of poisson() on the tree (no counterpart in the reference).
+   The leaf operator r(x) = -lambda x - div(alpha grad x) (the TREE
+   residual, poisson.h:355-372, with b = 0 and homogeneous boundary
+   conditions) is not symmetric on adaptive trees (halo values are bilinear
+   prolongations of restrictions), so its adjoint applies the transposes of
+   the forward stages in reverse order.
+   bmg_tree_residual_transpose: out = r^T y (leaf order).
+   bmg_tree_cycle_transpose: out = M^T y with M one V-cycle from da = 0 with
+   red-black relax (res -> da on the leaves; minlevel as for mg_cycle).
+   bmg_tree_adjoint: psi solves L^T psi = g (L = -r, about the state `a` of
+   the last solve) by defect correction with the transposed V-cycle,
+   psi += M^T rho, rho = g + r^T psi, until max|rho| <= tolerance (at least
+   one, at most nitermax cycles; nrelax <= 0 means 4); then dJ/db = psi,
+   galpha (face values, order of bmg_tree_set_alpha_faces) = the adjoint
+   flux of psi times (a[] - a[-1])/Delta, glambda (leaves) = -psi a.
+   galpha and glambda may be NULL; stats: i, resb = max|g|, resa.  FP64;
+   device pointers when on_device, else host arrays. */
+int bmg_tree_residual_transpose(bmg_tree *t, const double *y, double *out, int on_device);
+int bmg_tree_cycle_transpose(bmg_tree *t, const double *y, double *out, int nrelax, int minlevel,
+                             int on_device);
+int bmg_tree_adjoint(bmg_tree *t, const double *g, double tolerance, int nitermax, int nrelax,
+                     double *psi, double *galpha, double *glambda, int on_device, bmg_mgstats *stats);
 int bmg_tree_synchronize(bmg_tree *t);
 /* timing of the relax kernel (no counterpart in the reference): while
    enabled, HIP events on the tree's stream bracket every relax launch of

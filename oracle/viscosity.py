@@ -33,8 +33,16 @@ round it), of
                          (antisymmetry -1 / symmetry +1: the reference's
                          vector BCs at walls at rest)
 
+  periodic               grid/multigrid.h:600-701: a periodic dimension's
+                         ghost layers are copies of the opposite interior
+                         layer (filled in the same direction order, so an
+                         edge ghost across a periodic and a wall direction is
+                         the wall's factor times the wrapped cell); on level
+                         0 every ghost is the root value (655-664)
+
 Fields are (n,)*dim arrays in C order (i slowest); face field d has n+1
-entries along d.  Periodic boundaries are not covered.
+entries along d (on a periodic dimension face n is face 0: the value given
+for it is replaced by face 0's).
 """
 import numpy as np
 
@@ -49,9 +57,13 @@ def default_sigma(dim):
     return s
 
 
-def _pad(f, sig, dim):
-    """One ghost layer, filled direction by direction."""
+def _pad(f, sig, dim, periodic=None):
+    """One ghost layer, filled direction by direction; a periodic dimension
+    copies the opposite interior layer, and on level 0 of a grid with a
+    periodic dimension every ghost is the root value."""
     n = f.shape[0]
+    periodic = periodic or [False] * dim
+    root = n == 1 and any(periodic)
     p = np.zeros((n + 2,) * dim)
     p[(slice(1, n + 1),) * dim] = f
     for d in range(dim):
@@ -59,10 +71,16 @@ def _pad(f, sig, dim):
         lo = [slice(None)] * dim
         hi_src = [slice(None)] * dim
         lo_src = [slice(None)] * dim
+        if periodic[d]:
+            hi[d], hi_src[d] = n + 1, 1
+            lo[d], lo_src[d] = 0, n
+            p[tuple(hi)] = p[tuple(hi_src)]
+            p[tuple(lo)] = p[tuple(lo_src)]
+            continue
         hi[d], hi_src[d] = n + 1, n
         lo[d], lo_src[d] = 0, 1
-        p[tuple(hi)] = float(sig[2 * d]) * p[tuple(hi_src)]
-        p[tuple(lo)] = float(sig[2 * d + 1]) * p[tuple(lo_src)]
+        p[tuple(hi)] = (1.0 if root else float(sig[2 * d])) * p[tuple(hi_src)]
+        p[tuple(lo)] = (1.0 if root else float(sig[2 * d + 1])) * p[tuple(lo_src)]
     return p
 
 
@@ -97,6 +115,7 @@ class Viscosity:
         self.TOLERANCE = 1e-3
         self.NITERMAX, self.NITERMIN = 100, 1
         self.smoother = "jacobi"
+        self.periodic = [False] * dim
 
     def delta(self, l):
         return self.L0 / (1 << l)
@@ -111,7 +130,13 @@ class Viscosity:
         self.mu = [None] * (D + 1)
         self.rho = [None] * (D + 1)
         if self.mu_c is None:
-            self.mu[D] = [np.asarray(m, np.float64) for m in mu]
+            self.mu[D] = [np.array(m, np.float64) for m in mu]
+            for d in range(dim):
+                if self.periodic[d]:
+                    # face n of a periodic dimension is face 0
+                    hi, lo = [slice(None)] * dim, [slice(None)] * dim
+                    hi[d], lo[d] = -1, 0
+                    self.mu[D][d][tuple(hi)] = self.mu[D][d][tuple(lo)]
             for l in range(D - 1, -1, -1):
                 self.mu[l] = [self._restrict_face(self.mu[l + 1][d], d) for d in range(dim)]
         if self.rho_c is None:
@@ -195,7 +220,7 @@ class Viscosity:
         n = 1 << l
         D = self.delta(l)
         D2 = D * D
-        P = [_pad(da[c], self.sigma[c], dim) for c in range(dim)]
+        P = [_pad(da[c], self.sigma[c], dim, self.periodic) for c in range(dim)]
         dtr = self._dtr(l, dt)
         return [(dtr * self._terms(P, c, l, n, False) + res[c] * D2) / (D2 * 1. + dtr * self._diag(c, l, n))
                 for c in range(dim)]
@@ -216,7 +241,7 @@ class Viscosity:
         dim, l = self.dim, self.depth
         n = 1 << l
         D = self.delta(l)
-        P = [_pad(u[c], self.sigma[c], dim) for c in range(dim)]
+        P = [_pad(u[c], self.sigma[c], dim, self.periodic) for c in range(dim)]
         dtr = self._dtr(l, dt)
         res = []
         m = 0.
@@ -229,7 +254,7 @@ class Viscosity:
     def _bilinear(self, coarse, sig):
         dim = self.dim
         nc = coarse.shape[0]
-        P = _pad(coarse, sig, dim)
+        P = _pad(coarse, sig, dim, self.periodic)
         n = 2 * nc
         idx = np.indices((n,) * dim)
         p = [(idx[d] >> 1) + 1 for d in range(dim)]

@@ -3,8 +3,8 @@ oracle's restatement (oracle/viscosity.py): with the Jacobi relax_viscosity
 (the reference's JACOBI variant) and with the red-black Gauss-Seidel
 extension, the solver statistics and every velocity component equal the
 oracle's bit for bit, for variable and constant mu and rho, the default wall
-BCs and no slip, in 2D and 3D.  The oracle is pinned by
-test_viscosity_oracle.py."""
+BCs and no slip, periodic dimensions, in 2D and 3D.  The oracle is pinned
+by test_viscosity_oracle.py."""
 import numpy as np
 import pytest
 
@@ -85,11 +85,40 @@ def test_viscosity_zero_dt_is_identity_and_errors():
         g.viscosity([x.copy() for x in u], mu=mu, rho=rho, dt=0.1, sigma=[[2, 1, 1, 1], [1, 1, 1, 1]])
     with pytest.raises(bmg.BMGError):
         g.viscosity([x.copy() for x in u], mu=1.0, rho=-1.0, dt=0.1)
-    p = bmg.Grid(2, 5)
-    p.set_bc(bmg.LEFT, bmg.BC_PERIODIC)
-    p.set_bc(bmg.RIGHT, bmg.BC_PERIODIC)
+    p = bmg.Grid(2, 5, smoother="gs")
     with pytest.raises(bmg.BMGError):
         p.viscosity([x.copy() for x in u], mu=1.0, rho=1.0, dt=0.1)
+
+
+@pytest.mark.parametrize("dim,depth,periodic", [(2, 6, (True, False)), (2, 6, (False, True)), (2, 5, (True, True)),
+                                                (3, 4, (True, False, True)), (3, 4, (True, True, True)),
+                                                (2, 1, (True, True)), (3, 2, (False, True, False))])
+@pytest.mark.parametrize("variable", [True, False])
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+def test_viscosity_periodic_bitexact(dim, depth, periodic, variable, smoother):
+    """Periodic dimensions (grid/multigrid.h:600-701): wrapped ghosts, the
+    level-0 root rule, and face n of mu replaced by face 0 (random values
+    are given for face n, so reading it would show)."""
+    print("case viscosity periodic", dim, depth, periodic, variable, smoother, flush=True)
+    u, mu, rho = _problem(dim, depth, 31 * dim + depth + 7 * sum(periodic), variable)
+    dt = 0.004
+    g = bmg.Grid(dim, depth, smoother=smoother)
+    for d in range(dim):
+        if periodic[d]:
+            g.set_periodic(d)
+    g.set_tolerance(1e-8)
+    o = Viscosity(dim, depth)
+    o.smoother = smoother
+    o.TOLERANCE = 1e-8
+    o.periodic = list(periodic)
+    o.set_coefficients(mu, rho)
+    uo, so, _ = o.solve(u, dt)
+    ug = [x.copy() for x in u]
+    sg = g.viscosity(ug, mu=mu, rho=rho, dt=dt)
+    for k in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert sg[k] == so[k], (k, sg, so)
+    for c in range(dim):
+        assert np.array_equal(ug[c], uo[c]), c
 
 
 @pytest.mark.parametrize("dim,depth", [(2, 1), (2, 2), (3, 1), (3, 2)])
