@@ -20,13 +20,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# HBM bytes moved per fine-level red-black sweep, measured with rocprofv3
+# HBM bytes moved per fine-level relax launch, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE on this workload (KiB per dispatch) and scaled by
 # the counters' calibration on a copy kernel of known size (fetch x2.0,
 # write x1.0); the derivation is in profiles/hbm_traffic_poisson256_mixed.md.
-# Keyed by (precision, depth, red-black kernel): "stream3" is the default
-# single-pass sweep at 256^3, "inplace" the two launches of the colour kernel.
+# Keyed by (precision, depth, red-black kernel): "stream3t" is the default
+# two-sweep launch of the FP32 256^3 level (per launch, two sweeps),
+# "stream3" the single-pass sweep, "inplace" the two launches of the colour
+# kernel (per sweep).
 MEASURED_SWEEP_TRAFFIC = {
+    ("mixed", 8, "stream3t"): (2 * 183990.1 + 65536.0) * 1024,
     ("mixed", 8, "stream3"): (2 * 181527.1 + 65536.0) * 1024,
     ("mixed", 8, "inplace"): 2 * (333867.0 + 65536.0) * 1024,
 }
