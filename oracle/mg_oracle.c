@@ -41,6 +41,11 @@
  *   5  nrelax rule with its two thresholds swapped (poisson.h:198-204)
  *   6  relax pairs alpha.x[] with a[1] and alpha.x[1] with a[-1]
  *   7  residual with +lambda a instead of -lambda a (poisson.h:376)
+ *   8  embedded flux with its value's sign flipped (embed.h:700)
+ *   9  embedded face gradient with the weights (1 + fs) and (1 - fs)
+ *      swapped (embed.h:73-74)
+ *  10  dirichlet_gradient's quadratic interpolation with a1 and a3
+ *      swapped (embed.h:369-370)
  *
  * Build once per dimension: -DDIM=1, 2 or 3, with -ffp-contract=off so that
  * no multiply-add is fused (the reference C is compiled without FMA).
@@ -72,7 +77,10 @@
 enum { F_A, F_B, F_RES, F_DA, F_LAMBDA, F_C, F_AX };
 #define NV (F_AX + DIM)
 
-enum { BC_SYMMETRY = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2, BC_PERIODIC = 3 };
+/* BC_VALUE: the ghost takes the expression itself (`a[left] = expr;`, the
+   BC function returning expr; homogeneous form 0), as the reference's
+   embedded-boundary test cases set their box conditions */
+enum { BC_SYMMETRY = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2, BC_PERIODIC = 3, BC_VALUE = 5 };
 enum { SM_GS = 0, SM_JACOBI = 1, SM_RBGS = 2 };
 
 typedef struct {
@@ -80,6 +88,8 @@ typedef struct {
   double resb, resa, sum;
   int nrelax, minlevel;
 } orc_mgstats;
+
+#include "embed_oracle.h"
 
 typedef struct {
   int depth;
@@ -96,6 +106,10 @@ typedef struct {
   int tree; /* full quadtree/octree semantics: Morton GS, tree corner ghosts */
   double TOLERANCE;
   int NITERMAX, NITERMIN;
+#if DIM == 2
+  orc_embed emb; /* embedded boundary (emb.kind 0: none) */
+  int emb_bad;   /* a cut cell's stencil left the level's interior */
+#endif
 } orc_grid;
 
 #if DIM == 1
@@ -168,6 +182,19 @@ void orc_free(orc_grid *g) {
   for (int l = 0; l <= g->depth; l++) free(g->d[l]);
   free(g->d);
   for (int b = 0; b < 2 * DIM; b++) free(g->bcv[b]);
+#if DIM == 2
+  if (g->emb.cs) {
+    for (int l = 0; l <= g->depth; l++) {
+      free(g->emb.cs[l]);
+      free(g->emb.fsx[l]);
+      free(g->emb.fsy[l]);
+    }
+    free(g->emb.cs);
+    free(g->emb.fsx);
+    free(g->emb.fsy);
+    free(g->emb.bc);
+  }
+#endif
   free(g);
 }
 
@@ -271,6 +298,7 @@ static double bc_apply(orc_grid *g, int b, int l, int homogeneous, int use_bc, d
   switch (kind) {
   case BC_DIRICHLET: return homogeneous ? -s : 2. * (expr) - s;
   case BC_NEUMANN: return homogeneous ? s : PLANT(4) ? s - delta_l(g, l) * (expr) : delta_l(g, l) * (expr) + s;
+  case BC_VALUE: return homogeneous ? 0. : expr;
   default: return s;
   }
 }
@@ -364,6 +392,7 @@ static void box_boundary(orc_grid *g, int f, int l, int homogeneous, int use_bc)
         switch (kind) {
         case BC_DIRICHLET: r = homogeneous ? -s : 2. * (expr) - s; break;
         case BC_NEUMANN: r = homogeneous ? s : PLANT(4) ? s - D * (expr) : D * (expr) + s; break;
+        case BC_VALUE: r = homogeneous ? 0. : expr; break;
         default: r = s; break;
         }
         gh[f] = r;
@@ -640,9 +669,292 @@ static inline double relax_value(const orc_grid *g, int l, const double *r, int 
   return n / d;
 }
 
+#if DIM == 2
+/* --------------------------------------------- embedded boundaries, 2D */
+
+/* cs / fs of level l at interior cell (i,j) or face; a read outside the
+   interior marks the grid (the reference would read a ghost there) */
+static inline double emb_cs(orc_grid *g, int l, int i, int j) {
+  int n = 1 << l;
+  if (i < 0 || i >= n || j < 0 || j >= n) {
+    g->emb_bad = 1;
+    return 0.;
+  }
+  return E_CS(&g->emb, l, i, j);
+}
+static inline double emb_fx(orc_grid *g, int l, int i, int j) {
+  int n = 1 << l;
+  if (i < 0 || i > n || j < 0 || j >= n) {
+    g->emb_bad = 1;
+    return 0.;
+  }
+  return E_FX(&g->emb, l, i, j);
+}
+static inline double emb_fy(orc_grid *g, int l, int i, int j) {
+  int n = 1 << l;
+  if (i < 0 || i >= n || j < 0 || j > n) {
+    g->emb_bad = 1;
+    return 0.;
+  }
+  return E_FY(&g->emb, l, i, j);
+}
+/* the face fraction normal to direction d (rot: y) and the tangential one,
+   at offsets (u along the rotated x, v along the rotated y) */
+static inline double emb_fn(orc_grid *g, int l, int ci, int cj, int rot, int u, int v) {
+  return rot ? emb_fy(g, l, ci + v, cj + u) : emb_fx(g, l, ci + u, cj + v);
+}
+static inline double emb_ft(orc_grid *g, int l, int ci, int cj, int rot, int u, int v) {
+  return rot ? emb_fx(g, l, ci + v, cj + u) : emb_fy(g, l, ci + u, cj + v);
+}
+static inline double emb_csr(orc_grid *g, int l, int ci, int cj, int rot, int u, int v) {
+  return rot ? emb_cs(g, l, ci + v, cj + u) : emb_cs(g, l, ci + u, cj + v);
+}
+/* field f at offsets (u, v) of interior cell (ci, cj), ghosts included */
+static inline double emb_s(orc_grid *g, int l, int ci, int cj, int rot, int u, int v, int f) {
+  int di = rot ? v : u, dj = rot ? u : v;
+  return REC(g, l, ci + di + GHOSTS, cj + dj + GHOSTS, 0)[f];
+}
+
+#define QUADRATIC(x, a1, a2, a3) \
+  (((a1) * ((x) - 1.) + (a3) * ((x) + 1.)) * (x) / 2. - (a2) * ((x) - 1.) * ((x) + 1.))
+
+/* dirichlet_gradient_x (embed.h:374-433); rot = 1: the y variant */
+static double dirichlet_gradient_r(orc_grid *g, int l, int ci, int cj, int f, int rot, coord2 n, coord2 p,
+                                   double bc, double *coef) {
+  double D = delta_l(g, l);
+  /* the rotated frame: x is the direction of the larger normal component */
+  double nx = rot ? n.y : n.x, ny = rot ? n.x : n.y, px = rot ? p.y : p.x, py = rot ? p.x : p.y;
+  nx = -nx;
+  ny = -ny;
+  double d[2], v[2] = {NODATA, NODATA};
+  int defined = 1;
+  if (defined && !emb_fn(g, l, ci, cj, rot, (nx > 0.), 0)) defined = 0;
+  if (defined && !emb_ft(g, l, ci, cj, rot, 0, (ny > 0.))) defined = 0;
+  if (defined)
+    for (int m = 0; m <= 1; m++) {
+      int i = (m + 1) * isign(nx);
+      d[m] = (i - px) / nx;
+      double y1 = py + d[m] * ny;
+      int j = y1 > 0.5 ? 1 : y1 < -0.5 ? -1 : 0;
+      y1 -= j;
+      if (emb_fn(g, l, ci, cj, rot, i + (i < 0), j) && emb_ft(g, l, ci, cj, rot, i, j) &&
+          emb_ft(g, l, ci, cj, rot, i, j + 1) && emb_csr(g, l, ci, cj, rot, i, j - 1) &&
+          emb_csr(g, l, ci, cj, rot, i, j) && emb_csr(g, l, ci, cj, rot, i, j + 1))
+        v[m] = PLANT(10) ? QUADRATIC(y1, (emb_s(g, l, ci, cj, rot, i, j + 1, f)), (emb_s(g, l, ci, cj, rot, i, j, f)),
+                                     (emb_s(g, l, ci, cj, rot, i, j - 1, f)))
+                         : QUADRATIC(y1, (emb_s(g, l, ci, cj, rot, i, j - 1, f)), (emb_s(g, l, ci, cj, rot, i, j, f)),
+                                     (emb_s(g, l, ci, cj, rot, i, j + 1, f)));
+      else
+        break;
+    }
+  if (v[0] == NODATA) {
+    double q = fabs(px / nx);
+    d[0] = 1e-3 > q ? 1e-3 : q;
+    *coef = -1. / (d[0] * D);
+    return bc / (d[0] * D);
+  }
+  *coef = 0.;
+  if (v[1] != NODATA) return (d[1] * (bc - v[0]) / d[0] - d[0] * (bc - v[1]) / d[1]) / ((d[1] - d[0]) * D);
+  return (bc - v[0]) / (d[0] * D);
+}
+
+/* embed_flux (embed.h:654-702) of field f at interior cell (ci, cj) of
+   level l: returns the coefficient of the cell's own value, *val the rest;
+   bc / dirichlet: the condition's value there and its kind */
+static double embed_flux2(orc_grid *g, int l, int ci, int cj, int f, double bc, int dirichlet, double *val) {
+  const orc_embed *E = &g->emb;
+  *val = 0.;
+  double c = E_CS(E, l, ci, cj);
+  if (c >= 1. || c <= 0.) return 0.;
+  double grad = bc;
+  if (!grad && !dirichlet) return 0.;
+  coord2 n = facet_normal(E, l, ci, cj), p;
+  double alpha = orc_line_alpha(c, n.x, n.y);
+  double area = line_length_center(n, alpha, &p);
+  double coef = 0.;
+  if (dirichlet) {
+    normalize2(&n);
+    grad = dirichlet_gradient_r(g, l, ci, cj, f, fabs(n.x) >= fabs(n.y) ? 0 : 1, n, p, grad, &coef);
+  }
+  const double *r = REC(g, l, ci + GHOSTS, cj + GHOSTS, 0);
+  double mua = 0., fa = 0.;
+  for (int d = 0; d < 2; d++) {
+    mua += alpha_lo(g, r, d) + alpha_hi(g, r, dstride(l, d), d);
+    fa += (d ? E_FY(E, l, ci, cj) : E_FX(E, l, ci, cj)) + (d ? E_FY(E, l, ci, cj + 1) : E_FX(E, l, ci + 1, cj));
+  }
+  double D = delta_l(g, l);
+  *val = PLANT(8) ? mua / (fa + SEPS) * grad * area / D : -mua / (fa + SEPS) * grad * area / D;
+  return -mua / (fa + SEPS) * coef * area / D;
+}
+
+/* face_gradient_x (a, h) of the residual at interior cell (ci, cj) of the
+   finest level, direction d, h = 0 (low face) or 1 (high face): the
+   embedded form on partial faces when `third` is set (embed.h:61-76,
+   179-195) */
+static double emb_face_gradient(orc_grid *g, int ci, int cj, int d, int h, int f) {
+  int l = g->depth, rot = d;
+  double D = delta_l(g, l);
+  double fsf = emb_fn(g, l, ci, cj, rot, h, 0);
+  double am = emb_s(g, l, ci, cj, rot, h - 1, 0, f), a0 = emb_s(g, l, ci, cj, rot, h, 0, f);
+  if (!(g->emb.third && fsf < 1. && fsf > 0.)) return (a0 - am) / D;
+  int j = isign(emb_fn(g, l, ci, cj, rot, h, 1) - emb_fn(g, l, ci, cj, rot, h, -1));
+  if (emb_fn(g, l, ci, cj, rot, h, j) > 0.5 && emb_ft(g, l, ci, cj, rot, h, j + (j < 0)) &&
+      emb_ft(g, l, ci, cj, rot, h - 1, j + (j < 0)) && emb_csr(g, l, ci, cj, rot, h, j) &&
+      emb_csr(g, l, ci, cj, rot, h - 1, j))
+    return ((PLANT(9) ? 1. - fsf : 1. + fsf) * (a0 - am) +
+            (PLANT(9) ? 1. + fsf : 1. - fsf) *
+                (emb_s(g, l, ci, cj, rot, h, j, f) - emb_s(g, l, ci, cj, rot, h - 1, j, f))) /
+           (2. * D);
+  return (a0 - am) / D;
+}
+
+/* the relax value with the EMBED branch (poisson.h:303-320): returns n/d,
+   or 0 and *zero = 1 when d is zero (then b is zeroed too) */
+static double relax_value_embed(orc_grid *g, int l, int i, int j, int src, double D2, int *zero) {
+  double *r = REC(g, l, i, j, 0);
+  double n = -D2 * r[F_RES], d = -lambda_at(g, r) * D2;
+  for (int c = 0; c < DIM; c++) {
+    ptrdiff_t sd = dstride(l, c);
+    double ah = alpha_hi(g, r, sd, c), al = alpha_lo(g, r, c);
+    n += ah * r[sd * NV + src] + al * r[-sd * NV + src];
+    d += ah + al;
+  }
+  /* da carries the homogeneous condition: value 0, of the same kind */
+  double c, e = embed_flux2(g, l, i - GHOSTS, j - GHOSTS, src, 0., g->emb.kind == 1, &c);
+  n -= c * D2;
+  d += e * D2;
+  *zero = !d;
+  return d ? n / d : 0.;
+}
+
+/* bilinear_embed (embed.h:738-757): injection when a coarse cell that
+   bilinear would use is empty; coarse cs ghosts are the symmetric mirror */
+static inline double emb_cs_ghost(orc_grid *g, int l, int i, int j) {
+  int n = 1 << l;
+  i = i < 0 ? 0 : i >= n ? n - 1 : i;
+  j = j < 0 ? 0 : j >= n ? n - 1 : j;
+  return E_CS(&g->emb, l, i, j);
+}
+static double bilinear_embed(orc_grid *g, int f, int l, int i, int j) {
+  int pi = (i + GHOSTS) / 2, cx = 2 * ((i + GHOSTS) % 2) - 1;
+  int pj = (j + GHOSTS) / 2, cy = 2 * ((j + GHOSTS) % 2) - 1;
+  int qi = pi - GHOSTS, qj = pj - GHOSTS;
+  if (!emb_cs_ghost(g, l - 1, qi, qj) || !emb_cs_ghost(g, l - 1, qi + cx, qj) ||
+      !emb_cs_ghost(g, l - 1, qi, qj + cy) || !emb_cs_ghost(g, l - 1, qi + cx, qj + cy))
+    return REC(g, l - 1, pi, pj, 0)[f];
+  return bilinear(g, f, l, i, j, 0);
+}
+
+/* set the embedded boundary: cs (n^2), fsx ((n+1) x n), fsy (n x (n+1)) of
+   the finest level, restricted to every level (restriction_average,
+   face_average); kind 1 Dirichlet / 2 Neumann with the condition's value
+   per finest cell (at the fragment's centroid); third: embedded face
+   gradients in the residual */
+void orc_set_embed(orc_grid *g, const double *cs, const double *fsx, const double *fsy, int kind,
+                   const double *bc, int third) {
+  orc_embed *E = &g->emb;
+  int D = g->depth;
+  if (!E->cs) {
+    E->cs = calloc(D + 1, sizeof(double *));
+    E->fsx = calloc(D + 1, sizeof(double *));
+    E->fsy = calloc(D + 1, sizeof(double *));
+    for (int l = 0; l <= D; l++) {
+      size_t n = (size_t)1 << l;
+      E->cs[l] = calloc(n * n, sizeof(double));
+      E->fsx[l] = calloc((n + 1) * n, sizeof(double));
+      E->fsy[l] = calloc(n * (n + 1), sizeof(double));
+    }
+    E->bc = calloc(((size_t)1 << D) * ((size_t)1 << D), sizeof(double));
+  }
+  size_t nf = (size_t)1 << D;
+  memcpy(E->cs[D], cs, nf * nf * sizeof(double));
+  memcpy(E->fsx[D], fsx, (nf + 1) * nf * sizeof(double));
+  memcpy(E->fsy[D], fsy, nf * (nf + 1) * sizeof(double));
+  if (bc) memcpy(E->bc, bc, nf * nf * sizeof(double));
+  else memset(E->bc, 0, nf * nf * sizeof(double));
+  for (int l = D - 1; l >= 0; l--) {
+    int n = 1 << l, m = 2 * n;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        double sum = 0.;
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++) sum += E->cs[l + 1][(size_t)(2 * i + a) * m + 2 * j + b];
+        E->cs[l][(size_t)i * n + j] = sum / (1 << DIM);
+      }
+    for (int i = 0; i <= n; i++)
+      for (int j = 0; j < n; j++) {
+        double sum = 0.;
+        for (int t = 0; t < 2; t++) sum += E->fsx[l + 1][(size_t)(2 * i) * m + 2 * j + t];
+        E->fsx[l][(size_t)i * n + j] = sum / 2.;
+      }
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j <= n; j++) {
+        double sum = 0.;
+        for (int t = 0; t < 2; t++) sum += E->fsy[l + 1][(size_t)(2 * i + t) * (m + 1) + 2 * j];
+        E->fsy[l][(size_t)i * (n + 1) + j] = sum / 2.;
+      }
+  }
+  E->kind = kind;
+  E->third = third;
+  g->emb_bad = 0;
+}
+
+/* nonzero when a cut cell's stencil left a level's interior since the
+   embedded boundary was set */
+int orc_embed_bad(orc_grid *g) { return g->emb_bad; }
+#endif /* DIM == 2 */
+
 void orc_relax(orc_grid *g, int l) {
   int n = 1 << l;
   double D = delta_l(g, l), D2 = D * D;
+#if DIM == 2
+  if (g->emb.kind) {
+    /* the EMBED branch: a cut cell also reads cells of its own colour
+       (dirichlet_gradient), so each red-black pass computes its cells from
+       the state before the pass; GS in place, Jacobi as usual */
+    if (g->smoother == SM_GS) {
+      LOOP_LEVEL(l, n)
+        int z;
+        double v = relax_value_embed(g, l, i, j, F_DA, D2, &z);
+        double *r = REC(g, l, i, j, k);
+        if (z) r[F_DA] = r[F_RES] = 0.;
+        else r[F_DA] = v;
+      END_LOOP
+    } else if (g->smoother == SM_JACOBI) {
+      LOOP_LEVEL(l, n)
+        int z;
+        double v = relax_value_embed(g, l, i, j, F_DA, D2, &z);
+        double *r = REC(g, l, i, j, k);
+        if (z) r[F_C] = r[F_RES] = 0.;
+        else r[F_C] = v;
+      END_LOOP
+      LOOP_LEVEL(l, n)
+        double *r = REC(g, l, i, j, k);
+        r[F_DA] = (r[F_DA] + 2. * r[F_C]) / 3.;
+      END_LOOP
+    } else {
+      for (int colour = 0; colour < 2; colour++) {
+        if (colour == 1) boundary_level(g, F_DA, l, 1, 1);
+        LOOP_LEVEL(l, n)
+          if (((i + j + k) & 1) == colour) {
+            int z;
+            double v = relax_value_embed(g, l, i, j, F_DA, D2, &z);
+            double *r = REC(g, l, i, j, k);
+            if (z) r[F_C] = r[F_RES] = 0.;
+            else r[F_C] = v;
+          }
+        END_LOOP
+        LOOP_LEVEL(l, n)
+          if (((i + j + k) & 1) == colour) {
+            double *r = REC(g, l, i, j, k);
+            r[F_DA] = r[F_C];
+          }
+        END_LOOP
+      }
+    }
+    return;
+  }
+#endif
   if (g->smoother == SM_GS && g->tree) {
     /* a full tree visits level l in the pre-order of foreach_cell
        (grid/foreach_cell.h:66-123): Morton order, x the high bit */
@@ -707,6 +1019,24 @@ double orc_residual(orc_grid *g) {
 #endif
         double *r = REC(g, l, i, j, k);
         double res = PLANT(7) ? r[F_B] + lambda_at(g, r) * r[F_A] : r[F_B] - lambda_at(g, r) * r[F_A];
+#if DIM == 2
+        if (g->emb.kind) {
+          /* the EMBED branch (poisson.h:373-385): embedded face gradients,
+             then the embedded flux with the inhomogeneous condition */
+          int ci = i - GHOSTS, cj = j - GHOSTS;
+          for (int c = 0; c < DIM; c++) {
+            ptrdiff_t sd = dstride(l, c);
+            res += (alpha_lo(g, r, c) * emb_face_gradient(g, ci, cj, c, 0, F_A) -
+                    alpha_hi(g, r, sd, c) * emb_face_gradient(g, ci, cj, c, 1, F_A)) /
+                   D;
+          }
+          double cv, e = embed_flux2(g, l, ci, cj, F_A, g->emb.bc[(size_t)ci * n + cj], g->emb.kind == 1, &cv);
+          res += cv - e * r[F_A];
+          r[F_RES] = res;
+          if (fabs(res) > maxres) maxres = fabs(res);
+          continue;
+        }
+#endif
         for (int c = 0; c < DIM; c++) {
           ptrdiff_t sd = dstride(l, c);
           double a0 = r[F_A], am = r[-sd * NV + F_A], ap = r[sd * NV + F_A];
@@ -730,6 +1060,15 @@ void orc_mg_cycle(orc_grid *g, int nrelax, int minlevel, int maxlevel) {
         REC(g, l, i, j, k)[F_DA] = 0.;
       END_LOOP
     } else {
+#if DIM == 2
+      /* bilinear_embed is the multigrid grid's (embed.h:738-757); a full
+         tree keeps bilinear */
+      if (g->emb.kind && !g->tree) {
+        LOOP_LEVEL(l, n)
+          REC(g, l, i, j, k)[F_DA] = bilinear_embed(g, F_DA, l, i, j);
+        END_LOOP
+      } else
+#endif
       LOOP_LEVEL(l, n)
         REC(g, l, i, j, k)[F_DA] = bilinear(g, F_DA, l, i, j, k);
       END_LOOP

@@ -16,6 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 F_A, F_B, F_RES, F_DA, F_LAMBDA, F_C, F_AX = range(7)
 BC_SYMMETRY, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
+BC_VALUE = 5  # the ghost is the expression itself (a[left] = expr)
 SMOOTHERS = {"gs": 0, "jacobi": 1, "rbgs": 2}
 GHOSTS = 2
 
@@ -101,6 +102,17 @@ def _lib(dim, omp=False, plant=0):
             "orc_dimension": (I, []),
             "orc_nvar": (I, []),
         }
+        if dim == 2:
+            # embedded boundaries (embed.h), 2D only
+            sigs.update({
+                "orc_set_embed": (None, [P, dp, dp, dp, I, dp, I]),
+                "orc_embed_bad": (I, [P]),
+                "orc_fractions": (None, [I, dp, D, dp, dp, dp]),
+                "orc_embed_geometry": (None, [I, dp, dp, dp, dp, dp, dp, dp, dp]),
+                "orc_cut_centroids": (None, [I, dp, dp, dp, dp, dp]),
+                "orc_line_alpha": (D, [D, D, D]),
+                "orc_line_area": (D, [D, D, D]),
+            })
         for name_, (res, args) in sigs.items():
             fn = getattr(lib, name_)
             fn.restype = res
@@ -306,6 +318,20 @@ class OracleGrid:
         self.lib.orc_poisson(self.g, tolerance, nrelax, minlevel, ctypes.byref(s))
         return s.as_dict()
 
+    def set_embed(self, cs, fx, fy, kind, bc=None, third=False):
+        """The embedded boundary (2D): finest-level cs, fs_x, fs_y; kind
+        EMBED_DIRICHLET or EMBED_NEUMANN with the condition's value per
+        cell (at the fragment centroid; None: homogeneous); third: embedded
+        face gradients in the residual (embed.h:179-195)."""
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (cs, fx, fy)]
+        bcv = None if bc is None else np.ascontiguousarray(bc, dtype=np.float64)
+        self._emb = args + [bcv]
+        self.lib.orc_set_embed(self.g, *[_dptr(a) for a in args], int(kind),
+                               None if bcv is None else _dptr(bcv), 1 if third else 0)
+
+    def embed_bad(self):
+        return bool(self.lib.orc_embed_bad(self.g))
+
 
 def centered_gradient(o, alpha=None, acc=None, plant=0):
     """centered_gradient (p, g) of navier-stokes/centered.h:387-406 with
@@ -351,3 +377,45 @@ def nrelax_update(resb, resa, tolerance, nrelax, plant=0):
 
 def set_threads(n, dim=3):
     _lib(dim, True).orc_set_threads(n)
+
+
+# --- embedded boundaries (embed.h), 2D -----------------------------------
+
+EMBED_DIRICHLET, EMBED_NEUMANN = 1, 2
+
+
+def fractions(phi, val=0.0):
+    """fractions() (fractions.h:129-290) in 2D from the vertex level set
+    phi ((n+1, n+1), vertex (i, j) the lower-left corner of cell (i, j)):
+    returns cs (n, n), fs_x (n+1, n) and fs_y (n, n+1)."""
+    lib = _lib(2)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    n = phi.shape[0] - 1
+    cs = np.zeros((n, n))
+    fx = np.zeros((n + 1, n))
+    fy = np.zeros((n, n + 1))
+    lib.orc_fractions(n, _dptr(phi), float(val), _dptr(cs), _dptr(fx), _dptr(fy))
+    return cs, fx, fy
+
+
+def embed_geometry(cs, fx, fy):
+    """Per cut cell (0 < cs < 1): the fragment's centroid (px, py, relative
+    to the cell centre in cell units, as embed_area_center shifts x and y),
+    its length and its Euclidean unit normal (nx, ny); zero elsewhere."""
+    lib = _lib(2)
+    n = cs.shape[0]
+    out = [np.zeros((n, n)) for _ in range(5)]
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (cs, fx, fy)]
+    lib.orc_embed_geometry(n, *[_dptr(a) for a in args], *[_dptr(o) for o in out])
+    return tuple(out)
+
+
+def cut_centroids(cs, fx, fy):
+    """Per cut cell, the centroid of its fluid part (line_center), relative
+    to the cell centre in cell units."""
+    lib = _lib(2)
+    n = cs.shape[0]
+    px, py = np.zeros((n, n)), np.zeros((n, n))
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (cs, fx, fy)]
+    lib.orc_cut_centroids(n, *[_dptr(a) for a in args], _dptr(px), _dptr(py))
+    return px, py
