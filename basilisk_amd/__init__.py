@@ -53,7 +53,24 @@ EXPORTS = [
     "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
     "bmg_poisson_noalloc", "bmg_centered_gradient", "bmg_correction", "bmg_tree_project",
     "bmg_tree_residual_transpose", "bmg_tree_cycle_transpose", "bmg_tree_adjoint",
+    "bmg_set_embed", "bmg_embed_geometry",
 ]
+
+
+EMBED_NONE, EMBED_DIRICHLET, EMBED_NEUMANN = 0, 1, 2
+
+
+def embed_geometry(cs, fs_x, fs_y):
+    """Per cut cell of an n x n level: the fragment's centroid (px, py,
+    relative to the cell centre in cell sizes), its length and its unit
+    normal (nx, ny), as embed_geometry (embed.h:221-230) gives them; zero
+    elsewhere."""
+    lib = library()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (cs, fs_x, fs_y)]
+    n = arrs[0].shape[0]
+    out = [np.zeros((n, n)) for _ in range(5)]
+    _check(lib.bmg_embed_geometry(n, *[_host_ptr(a) for a in arrs], *[_host_ptr(o) for o in out]))
+    return tuple(out)
 
 
 class BMGError(RuntimeError):
@@ -166,6 +183,8 @@ def library():
         "bmg_set_stream": (I, [P, P]),
         "bmg_set_bc": (I, [P, I, I, dp]),
         "bmg_set_bc_affine": (I, [P, I, I, dp]),
+        "bmg_set_embed": (I, [P, P, P, P, P, I, P, I]),
+        "bmg_embed_geometry": (I, [I, P, P, P, P, P, P, P, P]),
         "bmg_set_field": (I, [P, I, I, dp, I]),
         "bmg_get_field": (I, [P, I, I, dp, I]),
         "bmg_set_alpha_const": (I, [P, D]),
@@ -418,8 +437,28 @@ class Grid:
         _check(self.lib.bmg_set_bc(self._g, direction, kind, _host_ptr(v)))
 
     def set_bc_affine(self, direction, sigma, c):
+        """ghost = c + sigma s on that side (sigma +1, -1, or 0: the ghost
+        is c itself); c: n^(dim-1) values of the finest level."""
         v = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
         _check(self.lib.bmg_set_bc_affine(self._g, direction, int(sigma), _host_ptr(v)))
+
+    def set_embed(self, cs, fs_x, fs_y, kind, bc=None, third=False):
+        """An embedded boundary (embed.h, 2D): the finest level's volume
+        fractions cs (n, n) and face fractions fs_x (n+1, n), fs_y (n, n+1);
+        kind EMBED_DIRICHLET or EMBED_NEUMANN with the condition's value in
+        each cut cell at the fragment's centroid (None: 0); third: embedded
+        face gradients in the residual.  kind EMBED_NONE removes it."""
+        if kind == EMBED_NONE:
+            _check(self.lib.bmg_set_embed(self._g, None, None, None, None, 0, None, 0))
+            return
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (cs, fs_x, fs_y)]
+        n = self.n()
+        for a, shp in zip(arrs, ((n, n), (n + 1, n), (n, n + 1))):
+            if a.shape != shp:
+                raise ValueError("embed fractions: shape %s, expected %s" % (a.shape, shp))
+        b = None if bc is None else np.ascontiguousarray(bc, dtype=np.float64).reshape(n, n)
+        _check(self.lib.bmg_set_embed(self._g, *[_host_ptr(a) for a in arrs], None, int(kind),
+                                      None if b is None else _host_ptr(b), 1 if third else 0))
 
     def set_bc_function(self, kind, fn, directions=None):
         """Evaluate fn(x, y, z) at the centre of each boundary face (the

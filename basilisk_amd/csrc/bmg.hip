@@ -18,6 +18,7 @@
 
 #include "../../include/bmg.h"
 #include "bmg_internal.h"
+#include "bmg_embed_kernels.h"
 #include "bmg_kernels.h"
 #include "bmg_visc_kernels.h"
 
@@ -126,6 +127,27 @@ struct bmg_grid {
   // faces per component, rho), and r = u_old on the finest level
   std::vector<double*> visc;
   double* visc_r[3] = {nullptr, nullptr, nullptr};
+  // embedded boundary (bmg_set_embed), null when none
+  struct EmbState* emb = nullptr;
+};
+
+// the embedded boundary of a 2D grid: per level the cut cells' records on
+// the device, and on the finest level the condition's values and the face
+// codes of the residual's embedded gradients
+struct EmbState {
+  int kind = 0, third = 0;
+  struct Lev {
+    int* cell = nullptr;
+    EmbRec* rec = nullptr;
+    double* cs = nullptr;
+    int nrec = 0;
+    bool bad = false;  // a cut cell's stencil leaves the level's interior
+  };
+  std::vector<Lev> L;
+  double* bc = nullptr;
+  signed char* fcode[2] = {nullptr, nullptr};
+  double* fs[2] = {nullptr, nullptr};
+  bool fine_bad = false;  // an embedded face gradient leaves the interior
 };
 
 namespace {
@@ -353,6 +375,265 @@ Coef<double> fine_coef64(const bmg_grid* g) {
 
 double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l)); }
 
+// --- embedded boundaries, host side (2D) -----------------------------------
+//
+// The geometry of the cut cells, from the volume and face fractions, in the
+// reference's expressions and order: facet_normal from the face fractions
+// (fractions.h:414-431), line_alpha (geometry.h:28-50), line_length_center
+// (geometry.h:352-404), normalize (common.h:922-930), and which points
+// dirichlet_gradient interpolates (embed.h:374-433).
+
+namespace embgeo {
+
+struct V2 {
+  double x, y;
+};
+
+double line_alpha(double c, V2 n) {
+  double alpha, n1 = fabs(n.x), n2 = fabs(n.y);
+  if (n1 > n2) std::swap(n1, n2);
+  c = c < 0. ? 0. : (c > 1. ? 1. : c);
+  const double v1 = n1 / 2.;
+  if (c <= v1 / n2)
+    alpha = sqrt(2. * c * n1 * n2);
+  else if (c <= 1. - v1 / n2)
+    alpha = c * n2 + v1;
+  else
+    alpha = n1 + n2 - sqrt(2. * n1 * n2 * (1. - c));
+  if (n.x < 0.) alpha += n.x;
+  if (n.y < 0.) alpha += n.y;
+  return alpha - (n.x + n.y) / 2.;
+}
+
+double clamp01(double v) { return v < 0. ? 0. : (v > 1. ? 1. : v); }
+
+// fragment length; p: its centroid relative to the cell centre (cell units)
+double line_length_center(V2 m, double alpha, V2* p) {
+  alpha += (m.x + m.y) / 2.;
+  V2 n = m;
+  if (n.x < 0.) {
+    alpha -= n.x;
+    n.x = -n.x;
+  }
+  if (n.y < 0.) {
+    alpha -= n.y;
+    n.y = -n.y;
+  }
+  p->x = p->y = 0.;
+  if (alpha <= 0. || alpha >= n.x + n.y) return 0.;
+  if (n.x < 1e-4) {
+    p->x = 0.;
+    p->y = (m.y < 0. ? 1. - alpha : alpha) - 0.5;
+    return 1.;
+  }
+  if (n.y < 1e-4) {
+    p->y = 0.;
+    p->x = (m.x < 0. ? 1. - alpha : alpha) - 0.5;
+    return 1.;
+  }
+  if (alpha >= n.x) {
+    p->x += 1.;
+    p->y += (alpha - n.x) / n.y;
+  } else {
+    p->x += alpha / n.x;
+  }
+  double ax = p->x, ay = p->y;
+  if (alpha >= n.y) {
+    p->y += 1.;
+    ay -= 1.;
+    p->x += (alpha - n.y) / n.x;
+    ax -= (alpha - n.y) / n.x;
+  } else {
+    p->y += alpha / n.y;
+    ay -= alpha / n.y;
+  }
+  p->x = clamp01(p->x / 2.);
+  if (m.x < 0.) p->x = 1. - p->x;
+  p->x -= 0.5;
+  p->y = clamp01(p->y / 2.);
+  if (m.y < 0.) p->y = 1. - p->y;
+  p->y -= 0.5;
+  return sqrt(ax * ax + ay * ay);
+}
+
+// one level's fractions: cs (n x n), fx ((n+1) x n, left faces), fy
+// (n x (n+1), bottom faces), interior only; a read outside marks `bad`
+struct Frac {
+  int n = 0;
+  std::vector<double> cs, fx, fy;
+  mutable bool bad = false;
+  double c(int i, int j) const {
+    if (i < 0 || i >= n || j < 0 || j >= n) return bad = true, 0.;
+    return cs[(size_t)i * n + j];
+  }
+  double x(int i, int j) const {
+    if (i < 0 || i > n || j < 0 || j >= n) return bad = true, 0.;
+    return fx[(size_t)i * n + j];
+  }
+  double y(int i, int j) const {
+    if (i < 0 || i >= n || j < 0 || j > n) return bad = true, 0.;
+    return fy[(size_t)i * (n + 1) + j];
+  }
+  // along direction d (0: x, 1: y) at offsets (u along d, v across) from
+  // cell (i, j): the face normal to d, the face across, the cell
+  double fn(int d, int i, int j, int u, int v) const { return d ? y(i + v, j + u) : x(i + u, j + v); }
+  double ft(int d, int i, int j, int u, int v) const { return d ? x(i + v, j + u) : y(i + u, j + v); }
+  double cr(int d, int i, int j, int u, int v) const { return d ? c(i + v, j + u) : c(i + u, j + v); }
+};
+
+V2 facet_normal(const Frac& F, int i, int j) {
+  V2 n;
+  double nn = 0.;
+  n.x = F.x(i, j) - F.x(i + 1, j);
+  nn += fabs(n.x);
+  n.y = F.y(i, j) - F.y(i, j + 1);
+  nn += fabs(n.y);
+  if (nn > 0.) {
+    n.x /= nn;
+    n.y /= nn;
+  } else {
+    n.x = n.y = 1. / 2;
+  }
+  return n;
+}
+
+void normalize(V2* n) {
+  double norm = 0.;
+  norm += n->x * n->x;
+  norm += n->y * n->y;
+  norm = sqrt(norm);
+  n->x /= norm;
+  n->y /= norm;
+}
+
+int isign(double v) { return v > 0 ? 1 : -1; }
+
+// the record of cut cell (i, j) of a level; dirichlet: work out the
+// interpolation of dirichlet_gradient too
+EmbRec cut_record(const Frac& F, int i, int j, bool dirichlet, V2* centroid, V2* unit_normal) {
+  EmbRec R;
+  memset(&R, 0, sizeof R);
+  const double c = F.c(i, j);
+  V2 n = facet_normal(F, i, j), p;
+  const double alpha = line_alpha(c, n);
+  R.area = line_length_center(n, alpha, &p);
+  double fa = 0.;
+  fa += F.x(i, j) + F.x(i + 1, j);
+  fa += F.y(i, j) + F.y(i, j + 1);
+  R.fa = fa;
+  normalize(&n);
+  if (centroid) *centroid = p;
+  if (unit_normal) *unit_normal = n;
+  if (!dirichlet) return R;
+  const int rot = fabs(n.x) >= fabs(n.y) ? 0 : 1;
+  double nx = rot ? n.y : n.x, ny = rot ? n.x : n.y;
+  const double px = rot ? p.y : p.x, py = rot ? p.x : p.y;
+  nx = -nx;
+  ny = -ny;
+  bool defined = true;
+  if (defined && F.fn(rot, i, j, (nx > 0.), 0) == 0.) defined = false;
+  if (defined && F.ft(rot, i, j, 0, (ny > 0.)) == 0.) defined = false;
+  int ok = 0;
+  double d[2] = {0., 0.};
+  if (defined)
+    for (int m = 0; m <= 1; m++) {
+      const int ii = (m + 1) * isign(nx);
+      d[m] = (ii - px) / nx;
+      double y1 = py + d[m] * ny;
+      const int jj = y1 > 0.5 ? 1 : y1 < -0.5 ? -1 : 0;
+      y1 -= jj;
+      if (F.fn(rot, i, j, ii + (ii < 0), jj) && F.ft(rot, i, j, ii, jj) && F.ft(rot, i, j, ii, jj + 1) &&
+          F.cr(rot, i, j, ii, jj - 1) && F.cr(rot, i, j, ii, jj) && F.cr(rot, i, j, ii, jj + 1)) {
+        for (int t = 0; t < 3; t++) {
+          const int u = ii, v = jj - 1 + t;  // along the direction, across it
+          const int di = rot ? v : u, dj = rot ? u : v;
+          R.off[m][t] = di * F.n + dj;
+        }
+        R.y1[m] = y1;
+        ok++;
+      } else {
+        break;
+      }
+    }
+  R.mode = ok;
+  if (ok == 0) {
+    const double q = fabs(px / nx);
+    d[0] = 1e-3 > q ? 1e-3 : q;
+  }
+  R.d[0] = d[0];
+  R.d[1] = d[1];
+  return R;
+}
+
+// the code of a partial face of the finest level (embed_face_gradient's
+// choice, embed.h:61-76): face (fi, ft) normal to d; 0 plain, else j
+signed char face_code(const Frac& F, int d, int fi, int ft) {
+  // offsets are taken from the cell (fi, ft) on the face's high side
+  const int i = d ? ft : fi, j = d ? fi : ft;
+  const double fsf = F.fn(d, i, j, 0, 0);
+  if (!(fsf < 1. && fsf > 0.)) return 0;
+  const int jj = isign(F.fn(d, i, j, 0, 1) - F.fn(d, i, j, 0, -1));
+  if (F.fn(d, i, j, 0, jj) > 0.5 && F.ft(d, i, j, 0, jj + (jj < 0)) && F.ft(d, i, j, -1, jj + (jj < 0)) &&
+      F.cr(d, i, j, 0, jj) && F.cr(d, i, j, -1, jj))
+    return (signed char)jj;
+  return 0;
+}
+
+}  // namespace embgeo
+
+// the device view of level l's embedded boundary
+EmbLevel emb_level(const bmg_grid* g, int l) {
+  EmbLevel E;
+  memset(&E, 0, sizeof E);
+  const EmbState* S = g->emb;
+  E.cell = S->L[l].cell;
+  E.rec = S->L[l].rec;
+  E.cs = S->L[l].cs;
+  E.dirichlet = S->kind == BMG_EMBED_DIRICHLET;
+  if (l == g->depth) {
+    E.bc = S->bc;
+    for (int d = 0; d < 2; d++) {
+      E.fcode[d] = S->fcode[d];
+      E.fs[d] = S->fs[d];
+    }
+  }
+  return E;
+}
+
+void emb_free(bmg_grid* g) {
+  EmbState* S = g->emb;
+  if (!S) return;
+  for (auto& lv : S->L) {
+    dfreet(lv.cell);
+    dfreet(lv.rec);
+    dfreet(lv.cs);
+  }
+  dfreet(S->bc);
+  for (int d = 0; d < 2; d++) {
+    dfreet(S->fcode[d]);
+    dfreet(S->fs[d]);
+  }
+  delete S;
+  g->emb = nullptr;
+}
+
+// an embedded boundary is usable from level minlevel up
+int emb_check(const bmg_grid* g, int minlevel) {
+  const EmbState* S = g->emb;
+  if (!S) return BMG_OK;
+  if (g->precision != BMG_FP64) return fail(BMG_ERR_UNSUPPORTED, "embedded boundaries run in FP64");
+  if (g->smoother == BMG_RELAX_GS)
+    return fail(BMG_ERR_UNSUPPORTED, "embedded boundaries run red-black or Jacobi relaxation");
+  for (int l = std::max(minlevel, 0); l <= g->depth; l++)
+    if (S->L[l].bad)
+      return fail(BMG_ERR_UNSUPPORTED,
+                  "the embedded boundary reaches outside level %d's interior (raise minlevel or move it away from "
+                  "the box)",
+                  l);
+  if (S->fine_bad) return fail(BMG_ERR_UNSUPPORTED, "an embedded face gradient reaches outside the finest level");
+  return BMG_OK;
+}
+
 // --- kernels per dimension / precision --------------------------------
 
 // relax kernel of a level: the two-launch in-place red-black, the
@@ -360,7 +641,7 @@ double delta(const bmg_grid* g, int l) { return g->L0 * (1. / (double)(1 << l));
 // relax kernel of a level: two-launch in-place red-black (tiny levels),
 // one-launch fused red-black, single-pass streaming red-black (3D,
 // n >= 128) or Jacobi
-enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI, RK_GS };
+enum { RK_INPLACE_RB, RK_FUSED_RB, RK_STREAM_RB, RK_STREAM2_RB, RK_STREAM3_RB, RK_PAIR_RB, RK_JACOBI, RK_GS, RK_EMBED };
 
 // a cell-count threshold with an environment override (tuning knob)
 size_t size_knob(const char* name, size_t dflt) {
@@ -402,6 +683,7 @@ int stream3_min() {
 
 template <int DIM>
 int level_kernel(const bmg_grid* g, int l) {
+  if (g->emb) return RK_EMBED;
   if (g->smoother == BMG_RELAX_JACOBI) return RK_JACOBI;
   if (g->smoother == BMG_RELAX_GS) return RK_GS;
   // the column-streamed single-pass kernel: 3D levels of 128^3 cells and
@@ -463,6 +745,30 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
   {
     ProfScope ps(g, l == g->depth ? PROF_RELAX_FINE : PROF_RELAX_COARSE);
     switch (level_kernel<DIM>(g, l)) {
+      case RK_EMBED:
+        // the EMBED branch (2D, FP64): Jacobi in one pass; red-black as two
+        // passes, each from the state before it, that end in da[src]
+        if constexpr (DIM == 2 && sizeof(T) == 8) {
+          const EmbLevel E = emb_level(g, l);
+          Geo<2> g2{lv.n, lv.log2n};
+          const Coef<double> k2 = level_coef<double>(g, l);
+          double* d0 = (double*)lv.da[src];
+          double* d1 = (double*)lv.da[src ^ 1];
+          double* r = (double*)lv.res;
+          if (g->smoother == BMG_RELAX_JACOBI) {
+            hipLaunchKernelGGL(k_relax_embed, dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, d0, d1, r, k2,
+                               g2, bc, D * D, D, -1, E);
+            out = src ^ 1;
+          } else {
+            hipLaunchKernelGGL(k_relax_embed, dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, d0, d1, r, k2,
+                               g2, bc, D * D, D, 0, E);
+            hipLaunchKernelGGL(k_relax_embed, dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, d1, d0, r, k2,
+                               g2, bc, D * D, D, 1, E);
+          }
+        } else {
+          *rc = fail(BMG_ERR_UNSUPPORTED, "embedded boundaries are 2D FP64");
+        }
+        break;
       case RK_JACOBI:
         hipLaunchKernelGGL((k_relax_jacobi<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
                            (const T*)lv.da[src], (T*)lv.da[src ^ 1], (const T*)lv.res, k, geo, bc, D2);
@@ -705,7 +1011,8 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) 
   // by the cycle and are not computed
   minlevel = std::min(minlevel, maxlevel);
   if (minlevel < 0) minlevel = 0;
-  int top = std::min(coarse_top(g), maxlevel);
+  // (no single-workgroup coarse levels with an embedded boundary)
+  int top = g->emb ? -1 : std::min(coarse_top(g), maxlevel);
   bool use_block = top >= minlevel;
   int lowest = use_block ? top : minlevel;
   int lstart = g->coarse_res_fresh ? g->depth - 2 : g->depth - 1;
@@ -729,7 +1036,9 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) 
     const int rk = level_kernel<DIM>(g, l);
     const bool two = use_sweep2<DIM, T>(g, l);
     const int flips = two ? nrelax - nrelax / 2 : nrelax;
-    int dst = (rk == RK_INPLACE_RB || rk == RK_PAIR_RB) ? 0 : (flips & 1);
+    const bool inplace = rk == RK_INPLACE_RB || rk == RK_PAIR_RB ||
+                         (rk == RK_EMBED && g->smoother != BMG_RELAX_JACOBI);
+    int dst = inplace ? 0 : (flips & 1);
     if (l == minlevel) {
       hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, (T*)lv.da[dst],
                          lv.ncell, T(0));
@@ -737,13 +1046,25 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) 
       ProfScope ps(g, PROF_PROLONG);
       Geo<DIM> gf{lv.n, lv.log2n};
       BCH bcc = level_bc<DIM>(g, l - 1);
+      bool done = false;
+      if constexpr (DIM == 2 && sizeof(T) == 8) {
+        if (g->emb) {
+          // bilinear_embed: injection next to empty coarse cells
+          Geo<2> g2{lv.n, lv.log2n};
+          hipLaunchKernelGGL(k_prolong_embed, dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
+                             (const double*)g->L[l - 1].da[0], (double*)lv.da[dst], g2, bcc, g->emb->L[l - 1].cs);
+          done = true;
+        }
+      }
       // one thread per coarse cell on the large 2D/3D levels
-      if (DIM >= 2 && lv.ncell >= (1u << 15))
+      if (done) {
+      } else if (DIM >= 2 && lv.ncell >= (1u << 15)) {
         hipLaunchKernelGGL((k_prolong_children<DIM, T>), dim3(nblocks(g->L[l - 1].ncell)), dim3(kBlock), 0,
                            g->stream, (const T*)g->L[l - 1].da[0], (T*)lv.da[dst], gf, bcc);
-      else
+      } else {
         hipLaunchKernelGGL((k_prolong<DIM, T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream,
                            (const T*)g->L[l - 1].da[0], (T*)lv.da[dst], gf, bcc);
+      }
     }
     TRY(check_launch());
     int cur = dst, rc = BMG_OK;
@@ -814,6 +1135,26 @@ int residual_t(bmg_grid* g, double* maxres, bool force64, unsigned long long* na
   bc.pow2 = std::frexp(D, &ex) == 0.5;
   bc.invD = 1. / D;
   bool mixed = g->precision == BMG_MIXED && !force64;
+  if constexpr (DIM == 2) {
+    if (g->emb) {
+      // the EMBED branch, FP64 (the solve refuses the mixed precision); it
+      // does not restrict, so the cycle restricts res from the finest level
+      ProfScope ps(g, PROF_RESIDUAL);
+      g->coarse_res_fresh = false;
+      hipLaunchKernelGGL(k_stat_reset, dim3(1), dim3(64), 0, g->stream, g->stat);
+      hipLaunchKernelGGL(k_residual_embed, dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, g->a, g->b,
+                         g->res64, k, geo, bc, D, emb_level(g, g->depth), g->stat);
+      TRY(check_launch());
+      Stat st;
+      HIPCHECK(hipMemcpyAsync(&st, g->stat, sizeof st, hipMemcpyDeviceToHost, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      double m;
+      memcpy(&m, &st.maxbits, sizeof m);
+      *maxres = m;
+      if (nans) *nans += st.nans;
+      return BMG_OK;
+    }
+  }
   if constexpr (DIM == 3) {
     if (fused_residual(g)) {
       // the pending correction, the residual and the first restriction in
@@ -1112,6 +1453,7 @@ int get_face(bmg_grid* g, int d, int l, double* dst, int on_device) {
 }
 
 int mg_solve_impl(bmg_grid* g, double tolerance, int nrelax, int minlevel, bmg_mgstats* out) {
+  TRY(emb_check(g, minlevel));
   bmg_mgstats s;
   memset(&s, 0, sizeof s);
   unsigned long long nans = 0;
@@ -1333,6 +1675,7 @@ int bmg_grid_new(bmg_grid** out, int dimension, int depth, double L0, double X0,
 int bmg_grid_free(bmg_grid* g) {
   if (!g) return BMG_OK;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  emb_free(g);
   for (auto& lv : g->L) {
     for (int s = 0; s < 2; s++) dfree(lv.da[s]);
     if (lv.res != (void*)g->res64) dfree(lv.res);
@@ -1472,8 +1815,160 @@ int bmg_set_bc(bmg_grid* g, int b, int kind, const double* values) {
 
 int bmg_set_bc_affine(bmg_grid* g, int b, int sigma, const double* c) {
   TRY(check_grid(g));
-  if (sigma != 1 && sigma != -1) return fail(BMG_ERR_ARG, "sigma must be +1 or -1");
+  if (sigma != 1 && sigma != -1 && sigma != 0) return fail(BMG_ERR_ARG, "sigma must be +1, -1 or 0");
   return set_bc_common(g, b, BMG_BC_AFFINE, sigma, c, 1.);
+}
+
+int bmg_set_embed(bmg_grid* g, const double* cs, const double* fs_x, const double* fs_y, const double* fs_z,
+                  int kind, const double* bc, int third) {
+  TRY(check_grid(g));
+  (void)fs_z;
+  if (kind == 0) {
+    drop_graphs(g);
+    emb_free(g);
+    return BMG_OK;
+  }
+  if (g->dim != 2) return fail(BMG_ERR_UNSUPPORTED, "embedded boundaries are built for 2D grids");
+  if (kind != BMG_EMBED_DIRICHLET && kind != BMG_EMBED_NEUMANN) return fail(BMG_ERR_ARG, "unknown embed kind %d", kind);
+  if (!cs || !fs_x || !fs_y) return fail(BMG_ERR_ARG, "cs and fs are required");
+  if (g->periodic[0] || g->periodic[1]) return fail(BMG_ERR_UNSUPPORTED, "embedded boundaries on periodic grids");
+  HIPCHECK(hipStreamSynchronize(g->stream));
+  drop_graphs(g);
+  emb_free(g);
+  EmbState* S = new EmbState();
+  g->emb = S;
+  S->kind = kind;
+  S->third = third ? 1 : 0;
+  const int D = g->depth;
+  // the fractions of every level: restriction_average for cs, face_average
+  // for fs (multigrid-common.h:26-63), children in foreach_child order
+  std::vector<embgeo::Frac> F(D + 1);
+  {
+    embgeo::Frac& f = F[D];
+    f.n = 1 << D;
+    const size_t n = f.n;
+    f.cs.assign(cs, cs + n * n);
+    f.fx.assign(fs_x, fs_x + (n + 1) * n);
+    f.fy.assign(fs_y, fs_y + n * (n + 1));
+  }
+  for (int l = D - 1; l >= 0; l--) {
+    embgeo::Frac& f = F[l];
+    const embgeo::Frac& h = F[l + 1];
+    f.n = 1 << l;
+    const int n = f.n, m = h.n;
+    f.cs.resize((size_t)n * n);
+    f.fx.resize((size_t)(n + 1) * n);
+    f.fy.resize((size_t)n * (n + 1));
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        double sum = 0.;
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++) sum += h.cs[(size_t)(2 * i + a) * m + 2 * j + b];
+        f.cs[(size_t)i * n + j] = sum / 4;
+      }
+    for (int i = 0; i <= n; i++)
+      for (int j = 0; j < n; j++) {
+        double sum = 0.;
+        for (int t = 0; t < 2; t++) sum += h.fx[(size_t)(2 * i) * m + 2 * j + t];
+        f.fx[(size_t)i * n + j] = sum / 2.;
+      }
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j <= n; j++) {
+        double sum = 0.;
+        for (int t = 0; t < 2; t++) sum += h.fy[(size_t)(2 * i + t) * (m + 1) + 2 * j];
+        f.fy[(size_t)i * (n + 1) + j] = sum / 2.;
+      }
+  }
+  S->L.resize(D + 1);
+  const bool dirichlet = kind == BMG_EMBED_DIRICHLET;
+  int rc = BMG_OK;
+  for (int l = 0; l <= D && rc == BMG_OK; l++) {
+    const embgeo::Frac& f = F[l];
+    const int n = f.n;
+    std::vector<int> cell((size_t)n * n, -1);
+    std::vector<EmbRec> rec;
+    f.bad = false;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        const double c = f.cs[(size_t)i * n + j];
+        if (c > 0. && c < 1.) {
+          cell[(size_t)i * n + j] = (int)rec.size();
+          rec.push_back(embgeo::cut_record(f, i, j, dirichlet, nullptr, nullptr));
+        }
+      }
+    EmbState::Lev& lv = S->L[l];
+    lv.bad = f.bad;
+    lv.nrec = (int)rec.size();
+    rc = dalloc((void**)&lv.cell, cell.size() * sizeof(int));
+    if (rc == BMG_OK) rc = dalloc((void**)&lv.rec, std::max<size_t>(rec.size(), 1) * sizeof(EmbRec));
+    if (rc == BMG_OK) rc = dalloc((void**)&lv.cs, f.cs.size() * sizeof(double));
+    if (rc == BMG_OK && (hipMemcpy(lv.cell, cell.data(), cell.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                         (rec.size() && hipMemcpy(lv.rec, rec.data(), rec.size() * sizeof(EmbRec),
+                                                  hipMemcpyHostToDevice) != hipSuccess) ||
+                         hipMemcpy(lv.cs, f.cs.data(), f.cs.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+      rc = fail(BMG_ERR_HIP, "upload of the embedded boundary failed");
+  }
+  // the finest level: the condition's values and the partial faces' codes
+  const embgeo::Frac& f = F[D];
+  const size_t n = f.n;
+  std::vector<signed char> code[2] = {std::vector<signed char>((n + 1) * n, 0),
+                                      std::vector<signed char>(n * (n + 1), 0)};
+  f.bad = false;
+  if (S->third)
+    for (int d = 0; d < 2; d++)
+      for (size_t fi = 0; fi <= n; fi++)
+        for (size_t ft = 0; ft < n; ft++) {
+          const size_t q = d == 0 ? fi * n + ft : ft * (n + 1) + fi;
+          code[d][q] = embgeo::face_code(f, d, (int)fi, (int)ft);
+        }
+  S->fine_bad = f.bad;
+  std::vector<double> bcv(n * n, 0.);
+  if (bc) bcv.assign(bc, bc + n * n);
+  if (rc == BMG_OK) rc = dalloc((void**)&S->bc, n * n * sizeof(double));
+  for (int d = 0; d < 2 && rc == BMG_OK; d++) {
+    rc = dalloc((void**)&S->fcode[d], code[d].size());
+    if (rc == BMG_OK) rc = dalloc((void**)&S->fs[d], code[d].size() * sizeof(double));
+    const std::vector<double>& fsv = d == 0 ? f.fx : f.fy;
+    if (rc == BMG_OK && (hipMemcpy(S->fcode[d], code[d].data(), code[d].size(), hipMemcpyHostToDevice) != hipSuccess ||
+                         hipMemcpy(S->fs[d], fsv.data(), fsv.size() * sizeof(double), hipMemcpyHostToDevice) !=
+                             hipSuccess))
+      rc = fail(BMG_ERR_HIP, "upload of the embedded boundary failed");
+  }
+  if (rc == BMG_OK &&
+      hipMemcpy(S->bc, bcv.data(), bcv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(BMG_ERR_HIP, "upload of the embedded boundary failed");
+  if (rc != BMG_OK) emb_free(g);
+  return rc;
+}
+
+int bmg_embed_geometry(int n, const double* cs, const double* fs_x, const double* fs_y, double* px, double* py,
+                       double* area, double* nx, double* ny) {
+  if (n <= 0 || !cs || !fs_x || !fs_y) return fail(BMG_ERR_ARG, "bad arguments");
+  embgeo::Frac f;
+  f.n = n;
+  const size_t m = n;
+  f.cs.assign(cs, cs + m * m);
+  f.fx.assign(fs_x, fs_x + (m + 1) * m);
+  f.fy.assign(fs_y, fs_y + m * (m + 1));
+  for (size_t q = 0; q < m * m; q++) {
+    double v[5] = {0., 0., 0., 0., 0.};
+    const double c = f.cs[q];
+    if (c > 0. && c < 1.) {
+      embgeo::V2 p, un;
+      const EmbRec R = embgeo::cut_record(f, (int)(q / m), (int)(q % m), false, &p, &un);
+      v[0] = p.x;
+      v[1] = p.y;
+      v[2] = R.area;
+      v[3] = un.x;
+      v[4] = un.y;
+    }
+    if (px) px[q] = v[0];
+    if (py) py[q] = v[1];
+    if (area) area[q] = v[2];
+    if (nx) nx[q] = v[3];
+    if (ny) ny[q] = v[4];
+  }
+  return BMG_OK;
 }
 
 int bmg_set_alpha_const(bmg_grid* g, double alpha) {
@@ -1580,6 +2075,7 @@ int bmg_restriction_alpha_lambda(bmg_grid* g) {
 
 int bmg_residual(bmg_grid* g, double* maxres) {
   TRY(check_grid(g));
+  TRY(emb_check(g, g->depth));
   double m = 0.;
   unsigned long long nans = 0;
   TRY(dispatch_residual(g, &m, false, &nans, true));
@@ -1591,6 +2087,7 @@ int bmg_residual(bmg_grid* g, double* maxres) {
 int bmg_relax(bmg_grid* g, int level) {
   TRY(check_grid(g));
   if (level < 0 || level > g->depth) return fail(BMG_ERR_ARG, "level %d out of range", level);
+  TRY(emb_check(g, level));
   TRY(dispatch_relax(g, level));
   HIPCHECK(hipStreamSynchronize(g->stream));
   return BMG_OK;
@@ -1599,6 +2096,7 @@ int bmg_relax(bmg_grid* g, int level) {
 int bmg_mg_cycle(bmg_grid* g, int nrelax, int minlevel, int maxlevel) {
   TRY(check_grid(g));
   if (maxlevel < 0 || maxlevel > g->depth) return fail(BMG_ERR_ARG, "maxlevel %d out of range", maxlevel);
+  TRY(emb_check(g, minlevel));
   if (nrelax < 0) return fail(BMG_ERR_ARG, "nrelax must be >= 0");
   TRY(dispatch_cycle(g, nrelax, minlevel, maxlevel));
   HIPCHECK(hipStreamSynchronize(g->stream));

@@ -75,6 +75,9 @@ enum {
    325-328), and red-black GS (the default: one pass over memory per sweep) */
 enum { BMG_RELAX_GS = 0, BMG_RELAX_JACOBI = 1, BMG_RELAX_RBGS = 2 };
 
+/* the a[embed] condition of an embedded boundary (embed.h) */
+enum { BMG_EMBED_NONE = 0, BMG_EMBED_DIRICHLET = 1, BMG_EMBED_NEUMANN = 2 };
+
 /* arithmetic of the cycle: plain FP64, or FP32 smoothing / restriction /
    prolongation inside FP64 residual, correction and norm */
 enum { BMG_FP64 = 0, BMG_MIXED = 1 };
@@ -124,8 +127,39 @@ int bmg_set_stream(bmg_grid *g, void *hip_stream);              /* NULL: the gri
    n^(dim-1) values in C order over the tangential indices.  The correction
    field uses the homogeneous version (poisson.h:155-157). */
 int bmg_set_bc(bmg_grid *g, int direction, int kind, const double *values);
-/* general affine BC: ghost = c + sigma * s, with sigma = +1 or -1 */
+/* general affine BC: ghost = c + sigma * s, with sigma = +1, -1 or 0 (0:
+   the ghost is the expression itself, `a[left] = expr;`, with the
+   homogeneous form 0) */
 int bmg_set_bc_affine(bmg_grid *g, int direction, int sigma, const double *c);
+
+/* Embedded boundary (embed.h), 2D grids: the EMBED branch of relax and
+   residual (poisson.h:277-279, 309-318, 351-352, 380-385) with embed_flux
+   (embed.h:654-702), dirichlet_gradient (embed.h:371-457), the multigrid
+   prolongation bilinear_embed (embed.h:738-757) and, with `third`, the
+   embedded face gradients of the residual (embed.h:61-76, 179-195).
+   cs: the finest level's volume fractions (n x n, C order); fs_x: the face
+   fractions of the x faces ((n+1) x n, face (i, j) the left face of cell
+   (i, j)), fs_y of the y faces (n x (n+1)); fs_z unused (NULL).  Coarser
+   levels take restriction_average of cs and face_average of fs.  kind:
+   BMG_EMBED_DIRICHLET or BMG_EMBED_NEUMANN, the a[embed] condition, with
+   bc its value in every cut cell (0 < cs < 1) at the fragment's centroid
+   (the value of the expression, or of the normal gradient; NULL: 0;
+   bmg_embed_geometry gives the centroids); BMG_EMBED_NONE removes the
+   boundary.  alpha is the caller's (usually fs times a coefficient).  The
+   grid must be FP64, its smoother red-black or Jacobi (a red-black pass
+   computes its cells from the state before the pass: a cut cell's stencil
+   reaches cells of its own parity), and every cut cell's stencil must stay
+   inside the levels the solve relaxes (minlevel and up): otherwise the
+   solve returns BMG_ERR_UNSUPPORTED.  Host arrays. */
+int bmg_set_embed(bmg_grid *g, const double *cs, const double *fs_x, const double *fs_y, const double *fs_z,
+                  int kind, const double *bc, int third);
+/* the geometry of the cut cells of an n x n level (embed_geometry,
+   embed.h:221-230): per cell the fragment's centroid (px, py, relative to
+   the cell centre in cell sizes: embed_area_center adds px Delta to x), its
+   length and its unit normal (nx, ny); zero outside cut cells.  Any output
+   may be NULL.  Host arrays. */
+int bmg_embed_geometry(int n, const double *cs, const double *fs_x, const double *fs_y, double *px, double *py,
+                       double *area, double *nx, double *ny);
 
 /* fields: `ptr` is host memory (on_device = 0) or device memory (1).
    Cell fields are n^dim values; face component c is n+1 along c and n

@@ -74,3 +74,16 @@ def test_planted_embed_mistakes_fail(plant, dirichlet):
     ref = cases.read_ref("dirichlet.ref" if dirichlet else "neumann.ref")
     lines, _, _ = _run(64, dirichlet, plant=plant)
     assert lines[0] != ref[2] or lines[1].split()[:7] != ref[3].split()[:7]
+
+
+def test_library_geometry_equals_the_oracle():
+    """bmg_embed_geometry (the library's host code, which also builds the
+    cut-cell records) gives the oracle's centroids, lengths and normals."""
+    import basilisk_amd as bmg
+    from oracle.oracle import embed_geometry
+    for dirichlet in (True, False):
+        p = ec.problem(128, dirichlet)
+        a = bmg.embed_geometry(p["cs"], p["fx"], p["fy"])
+        b = embed_geometry(p["cs"], p["fx"], p["fy"])
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
