@@ -27,7 +27,8 @@ RIGHT, LEFT, TOP, BOTTOM, FRONT, BACK = range(6)
 SMOOTHERS = {"gs": 0, "jacobi": 1, "rbgs": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 F_A, F_B, F_RES, F_DA, F_LAMBDA, F_ALPHA = range(6)
-F_S = 8  # trees: the scalar adapt_wavelet estimates
+F_S = 8  # trees: user scalar k is F_S + k, k < USER_SCALARS (adapt_wavelet's scalars)
+USER_SCALARS = 4
 PROFILE_KINDS = {"relax_fine": 0, "residual": 1, "restriction": 2,
                  "prolongation": 3, "correction": 4, "relax_coarse": 5, "relax_fine2": 6}
 
@@ -48,7 +49,7 @@ EXPORTS = [
     "bmg_tree_set_alpha_faces", "bmg_tree_set_alpha_const", "bmg_tree_set_lambda_const",
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
-    "bmg_tree_adapt_wavelet", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
+    "bmg_tree_adapt_wavelet", "bmg_tree_adapt_wavelet_list", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
     "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
     "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
     "bmg_poisson_noalloc", "bmg_centered_gradient", "bmg_correction", "bmg_tree_project",
@@ -236,6 +237,7 @@ def library():
         "bmg_tree_profile_get": (I, [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_long),
                                      ctypes.POINTER(ctypes.c_long)]),
         "bmg_tree_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "bmg_tree_adapt_wavelet_list": (I, [P, I, P, P, I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "bmg_tree_dump": (I, [P, ctypes.c_char_p, dp, dp, I, D, I]),
         "bmg_tree_nonleaf": (I, [P, ctypes.POINTER(ctypes.c_long), dp, dp]),
         "bmg_tree_unrefine": (I, [P, dp, ctypes.POINTER(ctypes.c_long)]),
@@ -904,12 +906,17 @@ class Tree:
         return self._get(F_DA)
 
     # -- adaptation (grid/tree-common.h)
-    def set_s(self, v):
-        """Leaf values of the scalar that adapt_wavelet estimates."""
-        self._put(F_S, v)
+    def set_s(self, v, k=0):
+        """Leaf values of user scalar k (0..USER_SCALARS-1): a field that
+        adapt_wavelet may estimate and that follows the mesh."""
+        if not 0 <= k < USER_SCALARS:
+            raise ValueError("user scalar %d outside 0..%d" % (k, USER_SCALARS - 1))
+        self._put(F_S + k, v)
 
-    def get_s(self, out=None):
-        return self._get(F_S, out)
+    def get_s(self, out=None, k=0):
+        if not 0 <= k < USER_SCALARS:
+            raise ValueError("user scalar %d outside 0..%d" % (k, USER_SCALARS - 1))
+        return self._get(F_S + k, out)
 
     def nonleaf(self):
         """Levels and centres of the non-leaf cells foreach_cell() visits."""
@@ -935,18 +942,21 @@ class Tree:
     @staticmethod
     def _dump_fields(names):
         fmap = {"a": F_A, "b": F_B, "s": F_S}
+        for k in range(1, USER_SCALARS):
+            fmap["s%d" % k] = F_S + k
         items = list(names.items())
         for k, _ in items:
             if k not in fmap:
-                raise ValueError("dump fields are a, b and s (got %r)" % k)
+                raise ValueError("dump fields are a, b, s and s1..s%d (got %r)" % (USER_SCALARS - 1, k))
         ids = (ctypes.c_int * max(1, len(items)))(*[fmap[k] for k, _ in items])
         nm = (ctypes.c_char_p * max(1, len(items)))(*[str(v).encode() for _, v in items])
         return ids, nm, len(items)
 
     def dump(self, path, names=None, t=0.0, i=0):
         """dump() (output.h:1043-1118): the tree and the leaf fields in the
-        reference's file format; `names` maps "a", "b", "s" to the names
-        written (default: a and b under their own names)."""
+        reference's file format; `names` maps "a", "b", "s" (user scalar 0)
+        and "s1".."s3" to the names written (default: a and b under their
+        own names)."""
         ids, nm, n = self._dump_fields(names if names is not None else {"a": "a", "b": "b"})
         _check(self.lib.bmg_tree_dump(self._t, str(path).encode(), ctypes.cast(ids, ctypes.c_void_p),
                                       ctypes.cast(nm, ctypes.c_void_p), n, float(t), int(i)))
@@ -961,13 +971,26 @@ class Tree:
                                          ctypes.cast(nm, ctypes.c_void_p), n, ctypes.byref(tt), ctypes.byref(ii)))
         return tt.value, ii.value
 
-    def adapt_wavelet(self, max, maxlevel, minlevel=1):
-        """adapt_wavelet ({s}, {max}, maxlevel, minlevel) with s, a and b
-        following the mesh; returns (nf, nc), the refined and coarsened cell
-        counts.  alpha and a lambda field must be set again afterwards."""
+    def adapt_wavelet(self, max, maxlevel, minlevel=1, scalars=None):
+        """adapt_wavelet (slist, max, maxlevel, minlevel)
+        (grid/tree-common.h:165-303): slist is the user scalars `scalars`
+        (default [0]) with one tolerance each in `max` (or one number for
+        all); every user scalar that is set, a and b follow the mesh.
+        Returns (nf, nc), the refined and coarsened cell counts.  alpha and
+        a lambda field must be set again afterwards."""
         nf, nc = ctypes.c_int(0), ctypes.c_int(0)
-        _check(self.lib.bmg_tree_adapt_wavelet(self._t, float(max), int(maxlevel), int(minlevel),
-                                               ctypes.byref(nf), ctypes.byref(nc)))
+        if scalars is None and np.isscalar(max):
+            _check(self.lib.bmg_tree_adapt_wavelet(self._t, float(max), int(maxlevel), int(minlevel),
+                                                   ctypes.byref(nf), ctypes.byref(nc)))
+            return nf.value, nc.value
+        us = [0] if scalars is None else list(scalars)
+        mx = [float(max)] * len(us) if np.isscalar(max) else [float(m) for m in max]
+        if len(mx) != len(us):
+            raise ValueError("one tolerance per estimated scalar")
+        ids = np.array([F_S + k for k in us], np.int32)
+        ma = np.array(mx, np.float64)
+        _check(self.lib.bmg_tree_adapt_wavelet_list(self._t, len(us), _host_ptr(ids), _host_ptr(ma), int(maxlevel),
+                                                    int(minlevel), ctypes.byref(nf), ctypes.byref(nc)))
         return nf.value, nc.value
 
     def set_lambda(self, v):

@@ -341,10 +341,11 @@ struct Topo {
   }
 
   // adapt_wavelet's passes over the topology (grid/tree-common.h:196-294);
-  // err[slot] is the wavelet estimate of each child of a refined cell
+  // err[i][slot] is the wavelet estimate of estimated scalar i at each
+  // child of a refined cell, max[i] its tolerance
   struct Adapt {
-    const std::vector<double>* err = nullptr;
-    double max = 0.;
+    std::vector<const std::vector<double>*> err;
+    std::vector<double> max;
     int maxlevel = 0, minlevel = 1;
     long nf = 0, nc = 0;
     RefineLog log;
@@ -375,22 +376,27 @@ struct Topo {
       for (int b = 0; b < 2; b++)
         for (int e = 0; e < kext(); e++, m++) {
           ch[m] = L[l + 1].find(2 * c.i - GH + a, 2 * c.j - GH + b, dim == 3 ? 2 * c.k - GH + e : 0);
-          if (!ch[m] || ch[m]->slot < 0 || ch[m]->slot >= (long)A.err->size()) {
+          if (!ch[m] || ch[m]->slot < 0 || ch[m]->slot >= (long)A.err[0]->size()) {
             A.ok = false;
             return;
           }
         }
-    for (m = 0; m < (1 << dim); m++) {
-      uint8_t& fl = ch[m]->flags;
-      const double e = (*A.err)[ch[m]->slot];
-      if (e > A.max && lev < A.maxlevel) {
-        fl &= (uint8_t)~C_TOO_FINE;
-        fl |= C_TOO_COARSE;
-      } else if ((e <= A.max / 1.5 || lev > A.maxlevel) && !(fl & (C_TOO_COARSE | C_JUST_FINE))) {
-        if (lev >= A.minlevel) fl |= C_TOO_FINE;
-      } else if (!(fl & C_TOO_COARSE)) {
-        fl &= (uint8_t)~C_TOO_FINE;
-        fl |= C_JUST_FINE;
+    // each estimated scalar in turn updates the children's flags
+    for (size_t i = 0; i < A.err.size(); i++) {
+      const std::vector<double>& err = *A.err[i];
+      const double max = A.max[i];
+      for (m = 0; m < (1 << dim); m++) {
+        uint8_t& fl = ch[m]->flags;
+        const double e = err[ch[m]->slot];
+        if (e > max && lev < A.maxlevel) {
+          fl &= (uint8_t)~C_TOO_FINE;
+          fl |= C_TOO_COARSE;
+        } else if ((e <= max / 1.5 || lev > A.maxlevel) && !(fl & (C_TOO_COARSE | C_JUST_FINE))) {
+          if (lev >= A.minlevel) fl |= C_TOO_FINE;
+        } else if (!(fl & C_TOO_COARSE)) {
+          fl &= (uint8_t)~C_TOO_FINE;
+          fl |= C_JUST_FINE;
+        }
       }
     }
     for (m = 0; m < (1 << dim); m++) {
@@ -718,6 +724,8 @@ unsigned nb_(long n) { return (unsigned)((n + kB - 1) / kB); }
 
 }  // namespace
 
+// user scalars of a tree (BMG_F_S .. BMG_F_S + kUserScalars - 1)
+constexpr int kUserScalars = 4;
 
 struct bmg_tree {
   int dim = 2;
@@ -735,7 +743,9 @@ struct bmg_tree {
   bool compiled = false;
   DevIndex ix;
   double *a = nullptr, *b = nullptr, *res64 = nullptr, *g[3] = {nullptr, nullptr, nullptr};
-  double* s = nullptr;  // the user scalar adapt_wavelet estimates (BMG_F_S)
+  // user scalars BMG_F_S + k, which adapt_wavelet may estimate and which
+  // follow the mesh; us[0] always exists, the others once set
+  double* us[kUserScalars] = {};
   double* uf[3] = {nullptr, nullptr, nullptr};  // project()'s face velocity per slot (allocated on use)
   // the adjoint's inverse tables and scratch (built on first use per mesh)
   struct Adj {
@@ -749,7 +759,7 @@ struct bmg_tree {
     double *xb = nullptr, *gb[3] = {nullptr, nullptr, nullptr}, *db = nullptr, *rb = nullptr;
     double *lv0 = nullptr, *lv1 = nullptr, *lv2 = nullptr, *lv3 = nullptr, *lv4 = nullptr;  // leaf-order vectors
   } adj;
-  bool s_set = false;
+  bool us_set[kUserScalars] = {};
   double *al64[3] = {nullptr, nullptr, nullptr}, *lam64 = nullptr;
   void *da = nullptr, *res = nullptr, *tmp = nullptr, *alT[3] = {nullptr, nullptr, nullptr}, *lamT = nullptr;
   double* bcv[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -758,7 +768,7 @@ struct bmg_tree {
   double *part = nullptr, *scalar = nullptr;
   hipStream_t own_stream = nullptr, stream = nullptr;
   // field values kept across a recompilation, in leaf order
-  std::vector<double> keep_a, keep_b, keep_lam, keep_s;
+  std::vector<double> keep_a, keep_b, keep_lam, keep_us[kUserScalars];
   // when set, compile() records each new slot's slot in the previous layout
   std::vector<int>* prev_slots = nullptr;
   // HIP graphs of the cycle (per nrelax, minlevel, maxlevel) and of the
@@ -847,8 +857,11 @@ void free_device(bmg_tree* t) {
     dfree_(p);
   x = DevIndex();
   bool mixed = t->precision == BMG_MIXED;
-  dfree_(t->a); dfree_(t->b); dfree_(t->res64); dfree_(t->s);
-  t->s = nullptr;
+  dfree_(t->a); dfree_(t->b); dfree_(t->res64);
+  for (auto& p : t->us) {
+    dfree_(p);
+    p = nullptr;
+  }
   for (int d = 0; d < 3; d++) {
     dfree_(t->g[d]);
     dfree_(t->al64[d]);
@@ -1507,7 +1520,8 @@ int compile(bmg_tree* t) {
   size_t ts = tsize(t);
   TRY(dalloc_((void**)&t->a, NS * 8));
   TRY(dalloc_((void**)&t->b, NS * 8));
-  TRY(dalloc_((void**)&t->s, NS * 8));
+  for (int k = 0; k < kUserScalars; k++)
+    if (k == 0 || t->us_set[k]) TRY(dalloc_((void**)&t->us[k], NS * 8));
   TRY(dalloc_((void**)&t->res64, NS * 8));
   TRY(dalloc_((void**)&t->lam64, NS * 8));
   for (int d = 0; d < dim; d++) {
@@ -1550,7 +1564,8 @@ int compile(bmg_tree* t) {
   };
   TRY(restore(t->keep_a, t->a));
   TRY(restore(t->keep_b, t->b));
-  TRY(restore(t->keep_s, t->s));
+  for (int k = 0; k < kUserScalars; k++)
+    if (t->us[k]) TRY(restore(t->keep_us[k], t->us[k]));
   TRY(restore(t->keep_lam, t->lam64));
   clk.mark("compile: upload and fields", t->stream);
   return BMG_OK;
@@ -2128,32 +2143,73 @@ int get_leaf_values(bmg_tree* t, const T* f, double* v, int on_device) {
 
 // ------------------------------------------------------------ adapt_wavelet
 
-// adapt_wavelet's boundary (list) and restriction ({s}) for list = {s, a, b}:
-// every level of the three fields restricted and its ghosts and halos filled
-// (a and b are also restricted on a full tree, where the reference leaves
-// their coarse levels alone: coarsen_cell recomputes the same averages)
-template <int DIM>
-void adapt_boundary(bmg_tree* t) {
-  Batcher<double, DIM> B(t, t->s, t->a, t->b);
-  const int bk[3] = {BK_SYM, BK_A, BK_SYM};
-  if (t->topo.full()) {
-    const int D = t->ix.depth;
-    for (int f = 0; f < 3; f++) {
-      B.box(f, bk[f], D);
-      for (int l = D - 1; l >= 0; l--) {
-        B.restrict_level(f, l);
-        B.box(f, bk[f], l);
-      }
-    }
-  } else {
-    for (int f = 0; f < 3; f++) B.tree_boundary(f, bk[f], -1);
+// the fields that follow the mesh (adapt_wavelet's default list, all
+// fields): the user scalars that exist, a and b, in groups of three for the
+// batcher, with their boundary kinds
+struct FieldGroup {
+  double* f[3];
+  int bk[3];
+};
+std::vector<FieldGroup> mesh_fields(double* const us[kUserScalars], double* a, double* b) {
+  std::vector<FieldGroup> g(1);
+  g[0] = FieldGroup{{us[0], a, b}, {BK_SYM, BK_A, BK_SYM}};
+  bool extra = false;
+  for (int k = 1; k < kUserScalars; k++) extra |= us[k] != nullptr;
+  if (extra) g.push_back(FieldGroup{{us[1], us[2], us[3]}, {BK_SYM, BK_SYM, BK_SYM}});
+  static_assert(kUserScalars == 4, "one group of three extra user scalars");
+  return g;
+}
+double* const* user_scalars(bmg_tree* t) { return t->us; }
+
+// the arrays of the fields that follow the mesh (user scalars, then a and
+// b), taken out of the tree before a recompilation, which allocates new
+// ones (user scalars that are set); the leaf values kept for a plain
+// recompilation are dropped
+void take_mesh_fields(bmg_tree* t, double* olds[kUserScalars + 2]) {
+  for (int k = 0; k < kUserScalars; k++) {
+    olds[k] = t->us_set[k] ? t->us[k] : nullptr;
+    if (!olds[k]) dfree_(t->us[k]);
+    t->us[k] = nullptr;
+    t->keep_us[k].clear();
   }
-  B.flush();
+  olds[kUserScalars] = t->a;
+  olds[kUserScalars + 1] = t->b;
+  t->a = t->b = nullptr;
+  t->keep_a.clear();
+  t->keep_b.clear();
+  t->keep_lam.clear();
 }
 
-// the wavelet estimate of every child of a refined cell, by slot
+// adapt_wavelet's boundary (list) and restriction (slist) for the fields
+// that follow the mesh: every level restricted and its ghosts and halos
+// filled (a, b and the scalars not estimated are also restricted on a full
+// tree, where the reference leaves their coarse levels alone: coarsen_cell
+// recomputes the same averages)
 template <int DIM>
-int adapt_estimate(bmg_tree* t, std::vector<double>& err_h) {
+void adapt_boundary(bmg_tree* t) {
+  for (const FieldGroup& G : mesh_fields(user_scalars(t), t->a, t->b)) {
+    Batcher<double, DIM> B(t, G.f[0], G.f[1], G.f[2]);
+    if (t->topo.full()) {
+      const int D = t->ix.depth;
+      for (int f = 0; f < 3; f++) {
+        if (!G.f[f]) continue;
+        B.box(f, G.bk[f], D);
+        for (int l = D - 1; l >= 0; l--) {
+          B.restrict_level(f, l);
+          B.box(f, G.bk[f], l);
+        }
+      }
+    } else {
+      for (int f = 0; f < 3; f++)
+        if (G.f[f]) B.tree_boundary(f, G.bk[f], -1);
+    }
+    B.flush();
+  }
+}
+
+// the wavelet estimate of scalar s at every child of a refined cell, by slot
+template <int DIM>
+int adapt_estimate(bmg_tree* t, const double* s, std::vector<double>& err_h) {
   const DevIndex& x = t->ix;
   double* err = nullptr;
   TRY(dalloc_((void**)&err, x.nslot * 8));
@@ -2162,7 +2218,7 @@ int adapt_estimate(bmg_tree* t, std::vector<double>& err_h) {
     int n = x.n_nonleaf[l];
     if (n)
       k_t_wavelet<DIM><<<nb_(n), kB, 0, t->stream>>>(x.rec_ring + (size_t)x.rec_off[l] * NR,
-                                                     x.rec_child0 + x.rec_off[l], n, t->s, err);
+                                                     x.rec_child0 + x.rec_off[l], n, s, err);
   }
   err_h.resize(x.nslot);
   hipError_t e = hipMemcpyAsync(err_h.data(), err, x.nslot * 8, hipMemcpyDeviceToHost, t->stream);
@@ -2172,11 +2228,12 @@ int adapt_estimate(bmg_tree* t, std::vector<double>& err_h) {
   return BMG_OK;
 }
 
-// s, a and b in the new layout: the old value of every cell that stays, the
-// bilinear of the old ring in the children of the refined cells, then the
-// tree boundary (mpi_boundary_update (list))
+// the fields that follow the mesh in the new layout: the old value of every
+// cell that stays, the bilinear of the old ring in the children of the
+// refined cells, then the tree boundary (mpi_boundary_update (list)).
+// olds: the old arrays of the user scalars, then a and b (null: none)
 template <int DIM>
-int adapt_remap(bmg_tree* t, double* const olds[3], long old_nslot, const std::vector<int>& prev,
+int adapt_remap(bmg_tree* t, double* const olds[kUserScalars + 2], long old_nslot, const std::vector<int>& prev,
                 const RefineLog& log) {
   const DevIndex& x = t->ix;
   Topo& T = t->topo;
@@ -2206,16 +2263,21 @@ int adapt_remap(bmg_tree* t, double* const olds[3], long old_nslot, const std::v
   if (rc == BMG_OK) rc = upload(&dring, log.ring);
   if (rc == BMG_OK) rc = upload(&dchild, child0);
   if (rc == BMG_OK) {
-    double* news[3] = {t->s, t->a, t->b};
-    for (int f = 0; f < 3; f++) {
+    double* news[kUserScalars + 2];
+    for (int k = 0; k < kUserScalars; k++) news[k] = t->us[k];
+    news[kUserScalars] = t->a;
+    news[kUserScalars + 1] = t->b;
+    for (int f = 0; f < kUserScalars + 2; f++) {
+      if (!olds[f] || !news[f]) continue;
       k_t_remap<<<nb_(x.nslot), kB, 0, t->stream>>>(dprev, x.nslot, olds[f], news[f]);
       if (nr) k_t_refine_remap<DIM><<<nb_((long)nr), kB, 0, t->stream>>>(dring, dchild, (int)nr, olds[f], news[f]);
     }
-    Batcher<double, DIM> B(t, t->s, t->a, t->b);
-    B.tree_boundary(0, BK_SYM, -1);
-    B.tree_boundary(1, BK_A, -1);
-    B.tree_boundary(2, BK_SYM, -1);
-    B.flush();
+    for (const FieldGroup& G : mesh_fields(user_scalars(t), t->a, t->b)) {
+      Batcher<double, DIM> B(t, G.f[0], G.f[1], G.f[2]);
+      for (int f = 0; f < 3; f++)
+        if (G.f[f]) B.tree_boundary(f, G.bk[f], -1);
+      B.flush();
+    }
     hipError_t e = hipStreamSynchronize(t->stream);
     if (e != hipSuccess) rc = bmg::fail(BMG_ERR_HIP, "tree adaptation: %s", hipGetErrorString(e));
   }
@@ -2269,7 +2331,8 @@ size_t dump_sizes(const std::vector<Cell*>& cells, int dim, size_t q, std::vecto
 }
 
 double* field_ptr(bmg_tree* t, int field) {
-  return field == BMG_F_A ? t->a : field == BMG_F_B ? t->b : field == BMG_F_S ? t->s : nullptr;
+  if (field >= BMG_F_S && field < BMG_F_S + kUserScalars) return t->us[field - BMG_F_S];
+  return field == BMG_F_A ? t->a : field == BMG_F_B ? t->b : nullptr;
 }
 
 // restore(): the pre-order traversal of the growing tree, one record per
@@ -2398,34 +2461,47 @@ int bmg_tree_refine(bmg_tree* t, const unsigned char* mark, long* nrefined) {
     t->keep_a.clear();
     t->keep_b.clear();
     t->keep_lam.clear();
-    t->keep_s.clear();
-    t->s_set = false;
+    for (int k = 0; k < kUserScalars; k++) {
+      t->keep_us[k].clear();
+      t->us_set[k] = false;
+    }
   }
   if (nrefined) *nrefined = nr;
   return BMG_OK;
 }
 
-int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, int* nf, int* nc) {
-  if (!t) return bmg::fail(BMG_ERR_ARG, "null tree");
-  if (!(max >= 0.)) return bmg::fail(BMG_ERR_ARG, "max must be a non-negative number");
+int bmg_tree_adapt_wavelet_list(bmg_tree* t, int n, const int* fields, const double* max, int maxlevel,
+                                int minlevel, int* nf, int* nc) {
+  if (!t || !fields || !max) return bmg::fail(BMG_ERR_ARG, "null argument");
+  if (n < 1 || n > kUserScalars) return bmg::fail(BMG_ERR_ARG, "1 to %d estimated scalars (got %d)", kUserScalars, n);
+  for (int i = 0; i < n; i++) {
+    if (fields[i] < BMG_F_S || fields[i] >= BMG_F_S + kUserScalars)
+      return bmg::fail(BMG_ERR_ARG, "field %d is not a user scalar", fields[i]);
+    if (!(max[i] >= 0.)) return bmg::fail(BMG_ERR_ARG, "max must be a non-negative number");
+  }
   const int lim = std::min(kSeg - 1, kMaxDepth);
   if (maxlevel < 0 || maxlevel > lim) return bmg::fail(BMG_ERR_ARG, "maxlevel %d outside 0..%d", maxlevel, lim);
   TRY(ensure(t));
-  if (!t->s_set) return bmg::fail(BMG_ERR_ARG, "set the estimated scalar (BMG_F_S) first");
+  for (int i = 0; i < n; i++)
+    if (!t->us_set[fields[i] - BMG_F_S])
+      return bmg::fail(BMG_ERR_ARG, "set the estimated scalar (field %d) first", fields[i]);
   Topo& T = t->topo;
   PhaseClock clk;
-  std::vector<double> err;
-  if (t->dim == 2) {
+  std::vector<std::vector<double>> err(n);
+  if (t->dim == 2)
     adapt_boundary<2>(t);
-    TRY(adapt_estimate<2>(t, err));
-  } else {
+  else
     adapt_boundary<3>(t);
-    TRY(adapt_estimate<3>(t, err));
+  for (int i = 0; i < n; i++) {
+    const double* s = t->us[fields[i] - BMG_F_S];
+    TRY(t->dim == 2 ? adapt_estimate<2>(t, s, err[i]) : adapt_estimate<3>(t, s, err[i]));
   }
   clk.mark("adapt: estimate", t->stream);
   Topo::Adapt A;
-  A.err = &err;
-  A.max = max;
+  for (int i = 0; i < n; i++) {
+    A.err.push_back(&err[i]);
+    A.max.push_back(max[i]);
+  }
   A.maxlevel = maxlevel;
   A.minlevel = minlevel < 1 ? 1 : minlevel;
   const IJK root = {GH, GH, t->dim == 3 ? GH : 0};
@@ -2438,14 +2514,11 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
   }
   if (A.nf || A.nc) {
     T.dirty = true;
-    // the three fields survive the recompilation in their old layout
-    double* olds[3] = {t->s, t->a, t->b};
+    // the fields that follow the mesh survive the recompilation in their
+    // old layout
+    double* olds[kUserScalars + 2];
+    take_mesh_fields(t, olds);
     const long old_nslot = t->ix.nslot;
-    t->s = t->a = t->b = nullptr;
-    t->keep_a.clear();
-    t->keep_b.clear();
-    t->keep_lam.clear();
-    t->keep_s.clear();
     std::vector<int> prev;
     t->prev_slots = &prev;
     clk.mark("adapt: before compile");
@@ -2463,6 +2536,11 @@ int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, 
   if (nf) *nf = (int)A.nf;
   if (nc) *nc = (int)A.nc;
   return BMG_OK;
+}
+
+int bmg_tree_adapt_wavelet(bmg_tree* t, double max, int maxlevel, int minlevel, int* nf, int* nc) {
+  const int s = BMG_F_S;
+  return bmg_tree_adapt_wavelet_list(t, 1, &s, &max, maxlevel, minlevel, nf, nc);
 }
 
 int bmg_tree_nonleaf(bmg_tree* t, long* n, int* level, double* xyz) {
@@ -2509,13 +2587,9 @@ int bmg_tree_unrefine(bmg_tree* t, const unsigned char* mark, long* ncoarsened) 
   }
   if (nc) {
     T.dirty = true;
-    double* olds[3] = {t->s, t->a, t->b};
+    double* olds[kUserScalars + 2];
+    take_mesh_fields(t, olds);
     const long old_nslot = t->ix.nslot;
-    t->s = t->a = t->b = nullptr;
-    t->keep_a.clear();
-    t->keep_b.clear();
-    t->keep_lam.clear();
-    t->keep_s.clear();
     std::vector<int> prev;
     t->prev_slots = &prev;
     int rc = compile(t);
@@ -2535,10 +2609,10 @@ int bmg_tree_dump(bmg_tree* t, const char* path, const int* fields, const char* 
                   double time, int iter) {
   if (!t || !path || (nfields && (!fields || !names)) || nfields < 0)
     return bmg::fail(BMG_ERR_ARG, "null argument");
-  for (int m = 0; m < nfields; m++)
-    if (!names[m] || (fields[m] != BMG_F_A && fields[m] != BMG_F_B && fields[m] != BMG_F_S))
-      return bmg::fail(BMG_ERR_ARG, "dump: field %d is not a, b or s", m);
   TRY(ensure(t));
+  for (int m = 0; m < nfields; m++)
+    if (!names[m] || !field_ptr(t, fields[m]))
+      return bmg::fail(BMG_ERR_ARG, "dump: field %d is not a, b or a user scalar that is set", m);
   // non-leaf cells hold the restriction of their leaves
   if (t->dim == 2)
     adapt_boundary<2>(t);
@@ -2606,8 +2680,9 @@ int bmg_tree_restore(bmg_tree* t, const char* path, const int* fields, const cha
   if (!t || !path || (nfields && (!fields || !names)) || nfields < 0)
     return bmg::fail(BMG_ERR_ARG, "null argument");
   for (int m = 0; m < nfields; m++)
-    if (!names[m] || (fields[m] != BMG_F_A && fields[m] != BMG_F_B && fields[m] != BMG_F_S))
-      return bmg::fail(BMG_ERR_ARG, "restore: field %d is not a, b or s", m);
+    if (!names[m] || (fields[m] != BMG_F_A && fields[m] != BMG_F_B &&
+                      !(fields[m] >= BMG_F_S && fields[m] < BMG_F_S + kUserScalars)))
+      return bmg::fail(BMG_ERR_ARG, "restore: field %d is not a, b or a user scalar", m);
   FILE* fp = fopen(path, "rb");
   if (!fp) return bmg::fail(BMG_ERR_ARG, "restore: cannot open %s", path);
   std::unique_ptr<FILE, int (*)(FILE*)> guard(fp, fclose);
@@ -2641,7 +2716,10 @@ int bmg_tree_restore(bmg_tree* t, const char* path, const int* fields, const cha
   t->keep_a.clear();
   t->keep_b.clear();
   t->keep_lam.clear();
-  t->keep_s.clear();
+  for (int k = 0; k < kUserScalars; k++) {
+    t->keep_us[k].clear();
+    t->us_set[k] = false;
+  }
   t->coef_ready = false;
   Topo& T = t->topo;
   std::vector<double> rec(h.len);
@@ -2665,12 +2743,10 @@ int bmg_tree_restore(bmg_tree* t, const char* path, const int* fields, const cha
     recs[n] = T.at(T.leaf_level[n], c.i, c.j, c.k).slot;
   }
   TRY(ensure(t));
-  t->s_set = false;
   std::vector<double> leafv(recs.size());
   for (int m = 0; m < nfields; m++) {
     for (size_t n = 0; n < recs.size(); n++) leafv[n] = vals[m][recs[n]];
-    TRY(set_leaf_values(t, field_ptr(t, fields[m]), leafv.data(), 0));
-    if (fields[m] == BMG_F_S) t->s_set = true;
+    TRY(bmg_tree_set_leaf_field(t, fields[m], leafv.data(), 0));
   }
   if (time) *time = h.t;
   if (iter) *iter = h.i;
@@ -2718,10 +2794,11 @@ int bmg_tree_set_precision(bmg_tree* t, int precision) {
       t->keep_b.resize(t->ix.nleaf);
       TRY(get_leaf_values<double>(t, t->a, t->keep_a.data(), 0));
       TRY(get_leaf_values<double>(t, t->b, t->keep_b.data(), 0));
-      if (t->s_set) {
-        t->keep_s.resize(t->ix.nleaf);
-        TRY(get_leaf_values<double>(t, t->s, t->keep_s.data(), 0));
-      }
+      for (int k = 0; k < kUserScalars; k++)
+        if (t->us_set[k]) {
+          t->keep_us[k].resize(t->ix.nleaf);
+          TRY(get_leaf_values<double>(t, t->us[k], t->keep_us[k].data(), 0));
+        }
       if (t->lambda_field) {
         t->keep_lam.resize(t->ix.nleaf);
         TRY(get_leaf_values<double>(t, t->lam64, t->keep_lam.data(), 0));
@@ -2784,9 +2861,11 @@ int bmg_tree_set_leaf_field(bmg_tree* t, int field, const double* values, int on
   TRY(ensure(t));
   if (field == BMG_F_A) return set_leaf_values(t, t->a, values, on_device);
   if (field == BMG_F_B) return set_leaf_values(t, t->b, values, on_device);
-  if (field == BMG_F_S) {
-    t->s_set = true;
-    return set_leaf_values(t, t->s, values, on_device);
+  if (field >= BMG_F_S && field < BMG_F_S + kUserScalars) {
+    const int k = field - BMG_F_S;
+    if (!t->us[k]) TRY(dalloc_((void**)&t->us[k], std::max<long>(1, t->ix.nslot) * 8));
+    t->us_set[k] = true;
+    return set_leaf_values(t, t->us[k], values, on_device);
   }
   if (field == BMG_F_LAMBDA) {
     t->lambda_field = true;
@@ -2801,10 +2880,13 @@ int bmg_tree_get_leaf_field(bmg_tree* t, int field, double* values, int on_devic
   if (!t || !values) return bmg::fail(BMG_ERR_ARG, "null argument");
   TRY(ensure(t));
   bool mixed = t->precision == BMG_MIXED;
+  if (field >= BMG_F_S && field < BMG_F_S + kUserScalars) {
+    if (!t->us[field - BMG_F_S]) return bmg::fail(BMG_ERR_ARG, "user scalar %d is not set", field - BMG_F_S);
+    return get_leaf_values<double>(t, t->us[field - BMG_F_S], values, on_device);
+  }
   switch (field) {
     case BMG_F_A: return get_leaf_values<double>(t, t->a, values, on_device);
     case BMG_F_B: return get_leaf_values<double>(t, t->b, values, on_device);
-    case BMG_F_S: return get_leaf_values<double>(t, t->s, values, on_device);
     case BMG_F_RES: return get_leaf_values<double>(t, t->res64, values, on_device);
     case BMG_F_DA:
       return mixed ? get_leaf_values<float>(t, F<float>(t->da), values, on_device)

@@ -82,6 +82,9 @@ enum { BMG_EMBED_NONE = 0, BMG_EMBED_DIRICHLET = 1, BMG_EMBED_NEUMANN = 2 };
    prolongation inside FP64 residual, correction and norm */
 enum { BMG_FP64 = 0, BMG_MIXED = 1 };
 
+/* user scalars of a tree */
+enum { BMG_USER_SCALARS = 4 };
+
 /* field identifiers */
 enum {
   BMG_F_A = 0,       /* unknown (finest level) */
@@ -90,7 +93,8 @@ enum {
   BMG_F_DA = 3,      /* correction (any level) */
   BMG_F_LAMBDA = 4,  /* Helmholtz coefficient field (any level) */
   BMG_F_ALPHA = 5,   /* face coefficient: BMG_F_ALPHA + c for component c (any level) */
-  BMG_F_S = 8        /* trees: the user scalar adapt_wavelet estimates (leaf field) */
+  BMG_F_S = 8        /* trees: user scalar k is BMG_F_S + k, k < BMG_USER_SCALARS (leaf
+                        fields that adapt_wavelet may estimate and that follow the mesh) */
 };
 
 typedef struct bmg_grid bmg_grid;
@@ -386,12 +390,22 @@ int bmg_tree_refine(bmg_tree *t, const unsigned char *mark, long *nrefined);
    maxlevel), cells whose children all stay within max/1.5 are coarsened (at
    minlevel and above; minlevel < 1 counts as 1), keeping 2:1 balance.  The
    estimate and the field updates run on the GPU, the passes over the tree
-   on the host.  s, a and b follow the mesh (refine_bilinear into new leaves,
-   the children's average into coarsened ones); alpha and a lambda field must
+   on the host.  The user scalars that are set, a and b follow the mesh
+   (refine_bilinear into new leaves, the children's average into coarsened
+   ones); alpha and a lambda field must
    be set again, and boundary values of `a` for levels the tree grew into
    read zero until set again.  nf and nc (may be NULL) receive the numbers of
    refined and coarsened cells (astats). */
 int bmg_tree_adapt_wavelet(bmg_tree *t, double max, int maxlevel, int minlevel, int *nf, int *nc);
+/* adapt_wavelet (slist, max, maxlevel, minlevel) (grid/tree-common.h:165-303,
+   struct Adapt at 157-163) with slist the n user scalars fields[]
+   (BMG_F_S + k, each set first) and max[] their tolerances: each scalar in
+   turn updates the children's flags, so a cell is refined when any scalar
+   asks for it and coarsened when all allow it.  Every user scalar that is
+   set, a and b follow the mesh (the reference's default list, all fields);
+   the rest as bmg_tree_adapt_wavelet, which is this call with {BMG_F_S}. */
+int bmg_tree_adapt_wavelet_list(bmg_tree *t, int n, const int *fields, const double *max, int maxlevel,
+                                int minlevel, int *nf, int *nc);
 /* the non-leaf cells that foreach_cell() visits, in its pre-order: level
    and centre (3 doubles) each; level/xyz may be NULL to get the count */
 int bmg_tree_nonleaf(bmg_tree *t, long *n, int *level, double *xyz);

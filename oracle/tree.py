@@ -21,20 +21,47 @@ GH = 2
 T_A, T_B, T_RES, T_DA, T_LAM, T_C, T_G = range(7)
 
 
-def t_s(dim):
-    """Index of the user scalar field (after the face fields g and alpha)."""
-    return T_G + 2 * dim
+def t_s(dim, k=0):
+    """Field index of user scalar k (0..NUS-1)."""
+    i = _lib(dim).ot_user_scalar(int(k))  # the same in every build
+    if i < 0:
+        raise ValueError("user scalar %d outside 0..%d" % (k, NUS - 1))
+    return i
+
+
+NUS = 4  # user scalars
 BK_A, BK_HOM, BK_SYM = range(3)
 RIGHT, LEFT, TOP, BOTTOM, FRONT, BACK = range(6)
 
 _libs = {}
 
 
-def _lib(dim):
-    if dim not in _libs:
-        path = os.path.join(_HERE, "liborc_tree%d.so" % dim)
-        if not os.path.exists(path):
-            build()
+def _planted_path(dim, plant):
+    """Build tree_oracle.c with -DORC_PLANT=plant (one deliberate mistake,
+    see the header of tree_oracle.c) into a temporary directory, for the
+    tests that show their checks catch it."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.gettempdir(), "bmg_orc_plant_%d" % os.getpid())
+    os.makedirs(out, exist_ok=True)
+    path = os.path.join(out, "liborc_tree%d_plant%d.so" % (dim, plant))
+    if not os.path.exists(path):
+        src = os.path.join(_HERE, "tree_oracle.c")
+        subprocess.run(["gcc", "-O2", "-std=gnu99", "-fPIC", "-shared", "-ffp-contract=off",
+                        "-Wno-unknown-pragmas", "-DDIM=%d" % dim, "-DORC_PLANT=%d" % plant,
+                        "-o", path + ".tmp", src, "-lm"], check=True)
+        os.replace(path + ".tmp", path)
+    return path
+
+
+def _lib(dim, plant=0):
+    if (dim, plant) not in _libs:
+        if plant:
+            path = _planted_path(dim, plant)
+        else:
+            path = os.path.join(_HERE, "liborc_tree%d.so" % dim)
+            if not os.path.exists(path):
+                build()
         lib = ctypes.CDLL(path)
         P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
         vp = ctypes.c_void_p
@@ -63,6 +90,8 @@ def _lib(dim):
             "ot_adjoint": (I, [P, vp, D, I, I, vp, vp, vp, ctypes.POINTER(MGStats)]),
             "ot_boundary": (None, [P, I, I, I]),
             "ot_adapt_wavelet": (I, [P, D, I, I, ctypes.POINTER(I)]),
+            "ot_adapt_wavelet_list": (I, [P, I, vp, vp, I, I, ctypes.POINTER(I)]),
+            "ot_user_scalar": (I, [I]),
             "ot_refine_cell": (I, [P, I, I, I, I]),
             "ot_nonleaf": (L, [P, vp, vp]), "ot_unrefine": (I, [P, vp]),
         }
@@ -70,8 +99,8 @@ def _lib(dim):
             f = getattr(lib, name)
             f.restype = res
             f.argtypes = args
-        _libs[dim] = lib
-    return _libs[dim]
+        _libs[(dim, plant)] = lib
+    return _libs[(dim, plant)]
 
 
 def _ptr(a):
@@ -86,9 +115,10 @@ class OracleTree:
     satisfies it (the refine() macro, grid/tree-common.h:308-326).
     """
 
-    def __init__(self, dim, L0=1.0, origin=(0.0, 0.0, 0.0)):
+    def __init__(self, dim, L0=1.0, origin=(0.0, 0.0, 0.0), plant=0):
         self.dim = dim
-        self.lib = _lib(dim)
+        self.plant = plant
+        self.lib = _lib(dim, plant)
         o = list(origin) + [0.0] * (3 - len(origin))
         self.L0, self.origin = L0, o[:3]
         self.t = self.lib.ot_new(L0, *o[:3])
@@ -318,21 +348,33 @@ class OracleTree:
         return s.as_dict()
 
     # -- adaptation
-    def set_s(self, v):
-        """Leaf values of the user scalar that adapt_wavelet estimates."""
-        self.set_leaf(t_s(self.dim), v)
+    def set_s(self, v, k=0):
+        """Leaf values of user scalar k (0..NUS-1), which adapt_wavelet may
+        estimate and which follows the mesh."""
+        self.set_leaf(t_s(self.dim, k), v)
 
-    def get_s(self):
-        return self.get_leaf(t_s(self.dim))
+    def get_s(self, k=0):
+        return self.get_leaf(t_s(self.dim, k))
 
     def get_b(self):
         return self.get_leaf(T_B)
 
-    def adapt_wavelet(self, max, maxlevel, minlevel=1):
-        """adapt_wavelet ({s}, {max}, maxlevel, minlevel), updating s, a and
-        b (grid/tree-common.h:165-303); returns (nf, nc)."""
+    def adapt_wavelet(self, max, maxlevel, minlevel=1, scalars=None):
+        """adapt_wavelet (slist, max, maxlevel, minlevel) with slist the
+        user scalars `scalars` (default [0]) and `max` one tolerance per
+        scalar (or one number), every user scalar, a and b following the
+        mesh (grid/tree-common.h:165-303); returns (nf, nc)."""
+        us = [0] if scalars is None else list(scalars)
+        mx = [float(max)] * len(us) if np.isscalar(max) else [float(m) for m in max]
+        if len(mx) != len(us):
+            raise ValueError("one tolerance per estimated scalar")
+        ua = np.array(us, np.int32)
+        ma = np.array(mx, np.float64)
         nf = ctypes.c_int(0)
-        nc = self.lib.ot_adapt_wavelet(self.t, float(max), int(maxlevel), int(minlevel), ctypes.byref(nf))
+        nc = self.lib.ot_adapt_wavelet_list(self.t, len(us), ua.ctypes.data, ma.ctypes.data, int(maxlevel),
+                                            int(minlevel), ctypes.byref(nf))
+        if nc < 0:
+            raise ValueError("bad user scalar index")
         return nf.value, nc
 
     def refine_cell_at(self, level, c):

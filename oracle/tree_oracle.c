@@ -57,6 +57,14 @@
  * face between the interior cell and the ghost; the values are given per
  * level and side on every boundary face of that level.
  *
+ * Planted mistakes: built with -DORC_PLANT=k (k > 0), the oracle makes one
+ * plausible mistake on purpose, so that the tests can show that their checks
+ * catch it (oracle/tree.py builds these variants into a temporary directory;
+ * the normal build has ORC_PLANT 0):
+ *  11  adapt_wavelet clears the children's flags before each estimated
+ *      scalar (only the last scalar decides)
+ *  12  adapt_wavelet estimates only the first scalar of the list
+ *
  * Build with -ffp-contract=off (the reference is compiled without FMA).
  */
 #include <math.h>
@@ -79,6 +87,11 @@
 /* cell flags; the last four are adapt_wavelet's user flags
    (grid/tree-common.h:197-198, 201, 218: refined, too_fine, too_coarse,
    just_fine) */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(k) (ORC_PLANT == (k))
+
 enum { C_ACTIVE = 1, C_LEAF = 2, C_FB = 4, C_REFINED = 8, C_TOO_FINE = 16, C_TOO_COARSE = 32, C_JUST_FINE = 64 };
 /* fields, per level, structure of arrays; T_S is a user scalar (the one
    adapt_wavelet estimates) */
@@ -90,7 +103,10 @@ enum { T_A, T_B, T_RES, T_DA, T_LAM, T_C, T_G };
 #define T_U (T_S + 1)
 #define T_DB (T_U + DIM)
 #define T_RB (T_DB + 1)
-#define NVT (T_RB + 1)
+/* user scalars 1..NUS-1 (user scalar 0 is T_S): T_SX + k - 1 */
+#define NUS 4
+#define T_SX (T_RB + 1)
+#define NVT (T_SX + NUS - 1)
 
 enum { BC_SYMMETRY = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2 };
 enum { SM_GS = 0, SM_JACOBI = 1, SM_RBGS = 2 };
@@ -1598,21 +1614,33 @@ void ot_boundary(ot_tree *t, int f, int bk, int l) {
 
 /* ---------------------------------------------------------- adaptation */
 
-/* the fields adapt_wavelet keeps up to date (its `list`): the estimated
-   scalar s and the solver's a and b, with their boundary conditions */
-static const int adapt_list[3] = {T_S, T_A, T_B};
-static const int adapt_bk[3] = {BK_SYM, BK_A, BK_SYM};
+/* the fields adapt_wavelet keeps up to date (its default `list`, all
+   fields): the user scalars and the solver's a and b, with their boundary
+   conditions */
+#define NADAPT (NUS + 2)
+static const int adapt_list[NADAPT] = {T_S, T_A, T_B, T_SX, T_SX + 1, T_SX + 2};
+static const int adapt_bk[NADAPT] = {BK_SYM, BK_A, BK_SYM, BK_SYM, BK_SYM, BK_SYM};
+static int user_scalar(int k) { return k == 0 ? T_S : T_SX + k - 1; }
+/* the field index of user scalar k, or -1 */
+int ot_user_scalar(int k) { return k >= 0 && k < NUS ? user_scalar(k) : -1; }
+
+/* the estimated scalars of one adaptation and their tolerances */
+typedef struct {
+  int n, f[NUS];
+  double max[NUS];
+} adapt_slist;
 
 /* the refinement pass of adapt_wavelet (grid/tree-common.h:196-265): a
    pre-order foreach_cell() in which `continue` skips a cell's children */
-static void adapt_refine_visit(ot_tree *t, int l, const int c[3], double max, int maxlevel, int minlevel, int *nf) {
+static void adapt_refine_visit(ot_tree *t, int l, const int c[3], const adapt_slist *S, int maxlevel, int minlevel,
+                               int *nf) {
   otl *L = &t->L[l];
   size_t q = lidx(L, c[0], c[1], c[2]);
   if (!L->alloc[q] || !(L->flags[q] & C_ACTIVE)) return;
   if (L->flags[q] & C_LEAF) {
     if (L->flags[q] & C_TOO_COARSE) {
       L->flags[q] &= ~C_TOO_COARSE;
-      refine_cell_f(t, l, c, C_REFINED, adapt_list, 3);
+      refine_cell_f(t, l, c, C_REFINED, adapt_list, NADAPT);
       (*nf)++;
     }
     return;
@@ -1628,25 +1656,31 @@ static void adapt_refine_visit(ot_tree *t, int l, const int c[3], double max, in
     int m = 0;
     for (int a = 0; a < 2; a++)
       for (int b = 0; b < 2; b++)
-        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++, m++) {
+        for (int e = 0; e < (DIM == 3 ? 2 : 1); e++, m++)
           ch[m] = lidx(F, 2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0);
-          sc[m] = F->f[T_S][ch[m]];
-        }
-    halo_bilinear(t, T_S, l, q); /* s.prolongation (point, s) */
     const int lev = l + 1;
-    for (m = 0; m < NCH; m++) {
-      unsigned char *fl = &F->flags[ch[m]];
-      double e = fabs(sc[m] - F->f[T_S][ch[m]]);
-      if (e > max && lev < maxlevel) {
-        *fl &= ~C_TOO_FINE;
-        *fl |= C_TOO_COARSE;
-      } else if ((e <= max / 1.5 || lev > maxlevel) && !(*fl & (C_TOO_COARSE | C_JUST_FINE))) {
-        if (lev >= minlevel) *fl |= C_TOO_FINE;
-      } else if (!(*fl & C_TOO_COARSE)) {
-        *fl &= ~C_TOO_FINE;
-        *fl |= C_JUST_FINE;
+    /* each estimated scalar in turn updates the children's flags */
+    for (int k = 0; k < (PLANT(12) ? 1 : S->n); k++) {
+      const int f = S->f[k];
+      const double max = S->max[k];
+      if (PLANT(11))
+        for (m = 0; m < NCH; m++) F->flags[ch[m]] &= ~(C_TOO_COARSE | C_TOO_FINE | C_JUST_FINE);
+      for (m = 0; m < NCH; m++) sc[m] = F->f[f][ch[m]];
+      halo_bilinear(t, f, l, q); /* s.prolongation (point, s) */
+      for (m = 0; m < NCH; m++) {
+        unsigned char *fl = &F->flags[ch[m]];
+        double e = fabs(sc[m] - F->f[f][ch[m]]);
+        if (e > max && lev < maxlevel) {
+          *fl &= ~C_TOO_FINE;
+          *fl |= C_TOO_COARSE;
+        } else if ((e <= max / 1.5 || lev > maxlevel) && !(*fl & (C_TOO_COARSE | C_JUST_FINE))) {
+          if (lev >= minlevel) *fl |= C_TOO_FINE;
+        } else if (!(*fl & C_TOO_COARSE)) {
+          *fl &= ~C_TOO_FINE;
+          *fl |= C_JUST_FINE;
+        }
+        F->f[f][ch[m]] = sc[m];
       }
-      F->f[T_S][ch[m]] = sc[m];
     }
     for (m = 0; m < NCH; m++) {
       unsigned char *fl = &F->flags[ch[m]];
@@ -1663,7 +1697,7 @@ static void adapt_refine_visit(ot_tree *t, int l, const int c[3], double max, in
       for (int b = 0; b < 2; b++)
         for (int e = 0; e < (DIM == 3 ? 2 : 1); e++) {
           int cc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
-          adapt_refine_visit(t, l + 1, cc, max, maxlevel, minlevel, nf);
+          adapt_refine_visit(t, l + 1, cc, S, maxlevel, minlevel, nf);
         }
 }
 
@@ -1679,7 +1713,7 @@ static void adapt_coarsen_visit(ot_tree *t, int m, const int c[3], int l, int *n
         if (L->flags[q] & C_REFINED)
           L->flags[q] &= ~(C_REFINED | C_TOO_FINE);
         else if (L->flags[q] & C_TOO_FINE) {
-          if (coarsen_cell(t, m, c, adapt_list, 3)) (*nc)++;
+          if (coarsen_cell(t, m, c, adapt_list, NADAPT)) (*nc)++;
           L = &t->L[m];
           L->flags[q] &= ~C_TOO_FINE; /* do not coarsen the parent */
         }
@@ -1704,31 +1738,49 @@ static void adapt_coarsen_visit(ot_tree *t, int m, const int c[3], int l, int *n
         }
 }
 
-/* adapt_wavelet ({s}, {max}, maxlevel, minlevel) on a Cartesian tree
-   (grid/tree-common.h:165-303, no MPI): s is the field T_S, and the fields
-   it refines and coarsens are s, a and b.  Returns the number of coarsened
-   cells; *nf receives the number of refined ones. */
-int ot_adapt_wavelet(ot_tree *t, double max, int maxlevel, int minlevel, int *nf) {
+/* adapt_wavelet (slist, max, maxlevel, minlevel) on a Cartesian tree
+   (grid/tree-common.h:165-303, no MPI) for the user scalars us[0..n-1]
+   (indices 0..NUS-1) with tolerances max[]; every user scalar, a and b are
+   refined and coarsened with the mesh (the default list, all fields).
+   Returns the number of coarsened cells (-1 for a bad scalar index); *nf
+   receives the number of refined ones. */
+int ot_adapt_wavelet_list(ot_tree *t, int n, const int *us, const double *max, int maxlevel, int minlevel,
+                          int *nf) {
+  adapt_slist S;
+  if (n < 1 || n > NUS) return -1;
+  S.n = n;
+  for (int k = 0; k < n; k++) {
+    if (us[k] < 0 || us[k] >= NUS) return -1;
+    S.f[k] = user_scalar(us[k]);
+    S.max[k] = max[k];
+  }
   ot_update(t);
-  /* boundary (list); restriction ({s}) */
+  /* boundary (list); restriction (slist) */
   if (is_full(t)) {
-    for (int m = 0; m < 3; m++) box_scalar(t, adapt_list[m], adapt_bk[m], t->depth);
-    for (int l = t->depth - 1; l >= 0; l--) { /* multigrid_restriction */
-      restrict_level(t, T_S, l);
-      box_scalar(t, T_S, BK_SYM, l);
-    }
+    for (int m = 0; m < NADAPT; m++) box_scalar(t, adapt_list[m], adapt_bk[m], t->depth);
+    for (int k = 0; k < n; k++)
+      for (int l = t->depth - 1; l >= 0; l--) { /* multigrid_restriction */
+        restrict_level(t, S.f[k], l);
+        box_scalar(t, S.f[k], BK_SYM, l);
+      }
   } else
-    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+    for (int m = 0; m < NADAPT; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
   if (minlevel < 1) minlevel = 1;
   int nrf = 0, ncs = 0;
   const int root[3] = {GH, GH, DIM == 3 ? GH : 0};
-  adapt_refine_visit(t, 0, root, max, maxlevel, minlevel, &nrf);
+  adapt_refine_visit(t, 0, root, &S, maxlevel, minlevel, &nrf);
   for (int l = t->depth; l >= 0; l--) adapt_coarsen_visit(t, 0, root, l, &ncs);
   t->dirty = 1;
   if (nrf || ncs) /* mpi_boundary_update (list) */
-    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+    for (int m = 0; m < NADAPT; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
   if (nf) *nf = nrf;
   return ncs;
+}
+
+/* adapt_wavelet ({s}, {max}, maxlevel, minlevel): user scalar 0 */
+int ot_adapt_wavelet(ot_tree *t, double max, int maxlevel, int minlevel, int *nf) {
+  const int us = 0;
+  return ot_adapt_wavelet_list(t, 1, &us, &max, maxlevel, minlevel, nf);
 }
 
 /* refine one leaf (refine_cell with the 2:1 cascade, no fields): restore()
@@ -1797,7 +1849,7 @@ static void unrefine_pass(ot_tree *t, int m, const int c[3], int l, int *nc) {
   if (!L->alloc[q] || (L->flags[q] & C_LEAF)) return;
   if (m == l) {
     if (L->flags[q] & C_TOO_FINE) {
-      if (coarsen_cell(t, m, c, adapt_list, 3)) (*nc)++;
+      if (coarsen_cell(t, m, c, adapt_list, NADAPT)) (*nc)++;
       L = &t->L[m];
       L->flags[q] &= ~C_TOO_FINE;
     }
@@ -1814,23 +1866,23 @@ static void unrefine_pass(ot_tree *t, int m, const int c[3], int l, int *nc) {
 
 /* unrefine (cond), grid/tree-common.h:348-371, with cond given as one mark
    per cell of ot_nonleaf's list: marked cells are coarsened from the depth
-   up (coarsen_cell, which the 2:1 rule may refuse), s, a and b taking the
-   children's average; then the tree boundary of the three fields.  Returns
-   the number of coarsened cells. */
+   up (coarsen_cell, which the 2:1 rule may refuse), the user scalars, a
+   and b taking the children's average; then the tree boundary of those
+   fields.  Returns the number of coarsened cells. */
 int ot_unrefine(ot_tree *t, const unsigned char *mark) {
   ot_update(t);
   /* the fields hold their restriction where coarsen_cell reads it */
   if (is_full(t)) {
-    for (int m = 0; m < 3; m++)
+    for (int m = 0; m < NADAPT; m++)
       for (int l = t->depth - 1; l >= 0; l--) restrict_level(t, adapt_list[m], l);
   } else
-    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+    for (int m = 0; m < NADAPT; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
   const int root[3] = {GH, GH, DIM == 3 ? GH : 0};
   unrefine_mark(t, 0, root, mark, 0);
   int nc = 0;
   for (int l = t->depth; l >= 0; l--) unrefine_pass(t, 0, root, l, &nc);
   t->dirty = 1;
   if (nc)
-    for (int m = 0; m < 3; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+    for (int m = 0; m < NADAPT; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
   return nc;
 }

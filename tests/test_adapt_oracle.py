@@ -182,3 +182,136 @@ def test_unrefine_region_keeps_tree_consistent(dim):
     h = 0.5 ** lev.astype(float)
     ok = np.all((xyz[:, :dim] > 4 * h[:, None]) & (xyz[:, :dim] < 1 - 4 * h[:, None]), axis=1)
     np.testing.assert_allclose(a[ok], 0.5 + xyz[ok][:, :dim] @ coef, rtol=0, atol=1e-12)
+
+
+# -- adapt_wavelet with a list of scalars (struct Adapt's slist and max[])
+
+def _leafset(t):
+    lev, xyz = t.leaves()
+    return lev, xyz
+
+
+def _level_at(t, pts):
+    """Level of the leaf of t containing each point."""
+    lev, xyz = t.leaves()
+    dim = t.dim
+    keys = {}
+    for l, x in zip(lev, xyz[:, :dim]):
+        keys[(int(l),) + tuple(np.floor(x * (1 << int(l))).astype(int))] = int(l)
+    out = np.zeros(len(pts), int)
+    for n, p in enumerate(pts[:, :dim]):
+        for l in range(int(lev.max()), -1, -1):
+            k = (l,) + tuple(np.floor(p * (1 << l)).astype(int))
+            if k in keys:
+                out[n] = l
+                break
+        else:
+            raise AssertionError("no leaf holds %s" % p)
+    return out
+
+
+def _two_bumps_tree(dim, base, fields, plant=0):
+    """A tree refined to `base` with the user scalars `fields` (k -> function
+    of the leaf centres) set."""
+    t = OracleTree(dim, 1.0, (0.0, 0.0, 0.0), plant=plant)
+    t.refine(lambda x, y, z, lev: lev < base)
+    _, xyz = t.leaves()
+    for k, f in fields.items():
+        t.set_s(f(xyz[:, :dim]), k)
+    return t
+
+
+def _union_levels(dim, plant=0):
+    """Levels of one call with {s0, s1} and of {s0} and {s1} alone (from
+    the same tree, coarsening off) at every leaf centre of the three."""
+    base, maxlevel = (4, 7) if dim == 2 else (3, 5)
+    c0, c1 = np.full(dim, 0.35), np.full(dim, 0.62)
+    f0 = lambda p: _gauss_field(p, [c0], 0.07)
+    f1 = lambda p: _gauss_field(p, [c1], 0.05)
+    both = _two_bumps_tree(dim, base, {0: f0, 1: f1}, plant)
+    only0 = _two_bumps_tree(dim, base, {0: f0, 1: f1})
+    only1 = _two_bumps_tree(dim, base, {0: f0, 1: f1})
+    mx = 4e-3
+    nf = both.adapt_wavelet([mx, mx], maxlevel, maxlevel + 1, scalars=[0, 1])[0]
+    n0 = only0.adapt_wavelet(mx, maxlevel, maxlevel + 1, scalars=[0])[0]
+    n1 = only1.adapt_wavelet(mx, maxlevel, maxlevel + 1, scalars=[1])[0]
+    assert n0 > 0 and n1 > 0
+    if not plant:
+        assert nf >= max(n0, n1)
+        tree_check(both)
+    pts = np.concatenate([t.leaves()[1] for t in (both, only0, only1)])
+    return _level_at(both, pts), _level_at(only0, pts), _level_at(only1, pts)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_adapt_list_refines_the_union(dim):
+    """With coarsening off (minlevel above maxlevel), one call with {s0, s1}
+    leaves every point at the finer of the levels {s0} and {s1} reach alone:
+    each scalar can only add the too_coarse flag."""
+    lb, l0, l1 = _union_levels(dim)
+    assert np.array_equal(lb, np.maximum(l0, l1))
+
+
+@pytest.mark.parametrize("plant", [11, 12])
+def test_adapt_list_union_catches_planted_mistakes(plant):
+    """Flags cleared before each scalar (only the last one decides), or only
+    the first scalar estimated: the union check fails."""
+    lb, l0, l1 = _union_levels(2, plant)
+    assert not np.array_equal(lb, np.maximum(l0, l1))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_adapt_list_order_and_null_scalar(dim):
+    """Refinement and coarsening together: {s0, s1} and {s1, s0} give the
+    same tree and fields (refine when any scalar asks, just-fine when any
+    says so, coarsen only when all allow it), and adding a scalar that is
+    zero everywhere (its estimate always allows coarsening) changes
+    nothing."""
+    base, maxlevel = (5, 7) if dim == 2 else (3, 5)
+    c0, c1 = np.full(dim, 0.3), np.full(dim, 0.66)
+    f0 = lambda p: _gauss_field(p, [c0], 0.08)
+    f1 = lambda p: _gauss_field(p, [c1], 0.06)
+    zero = lambda p: np.zeros(len(p))
+    a = _two_bumps_tree(dim, base, {0: f0, 1: f1, 2: zero})
+    b = _two_bumps_tree(dim, base, {0: f0, 1: f1, 2: zero})
+    c = _two_bumps_tree(dim, base, {0: f0, 1: f1, 2: zero})
+    d = _two_bumps_tree(dim, base, {0: f0, 1: f1, 2: zero})
+    for rnd in range(2):
+        sa = a.adapt_wavelet([3e-3, 2e-3], maxlevel, 2, scalars=[0, 1])
+        sb = b.adapt_wavelet([2e-3, 3e-3], maxlevel, 2, scalars=[1, 0])
+        sc = c.adapt_wavelet([3e-3, 1e-3, 2e-3], maxlevel, 2, scalars=[0, 2, 1])
+        assert sa == sb == sc and sa[0] > 0
+        for t in (b, c):
+            assert np.array_equal(t.leaves()[0], a.leaves()[0])
+            assert np.array_equal(t.leaves()[1], a.leaves()[1])
+            for k in range(3):
+                assert np.array_equal(t.get_s(k), a.get_s(k))
+    # {s0} alone (s1 ignored) gives another tree
+    for rnd in range(2):
+        d.adapt_wavelet(3e-3, maxlevel, 2, scalars=[0])
+    assert d.nleaves() != a.nleaves()
+    tree_check(a)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_every_user_scalar_follows_the_mesh(dim):
+    """A user scalar that is not estimated is carried like a: a linear
+    function stays exact away from the walls (refine_bilinear and the
+    children's average reproduce linear data) in user scalars 1..3."""
+    coef = np.array([2.0, -3.0, 0.5])[:dim]
+    t = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    t.refine(lambda x, y, z, lev: lev < (4 if dim == 2 else 3))
+    for rnd in range(3):
+        _, xyz = t.leaves()
+        p = xyz[:, :dim]
+        if rnd == 0:
+            for k in (1, 2, 3):
+                t.set_s(k + p @ coef * k, k)
+        t.set_s(_gauss_field(p, [np.full(dim, 0.3 + 0.2 * rnd)], 0.07), 0)
+        nf, nc = t.adapt_wavelet(2e-3, 7 if dim == 2 else 5)
+        lev, xyz = t.leaves()
+        h = 0.5 ** lev.astype(float)
+        ok = np.all((xyz[:, :dim] > 4 * h[:, None]) & (xyz[:, :dim] < 1 - 4 * h[:, None]), axis=1)
+        assert ok.sum() > 100
+        for k in (1, 2, 3):
+            np.testing.assert_allclose(t.get_s(k)[ok], (k + xyz[:, :dim] @ coef * k)[ok], rtol=0, atol=1e-12)

@@ -202,3 +202,50 @@ def test_unrefine_region_bitexact(dim):
     for k in ("i", "resb", "resa", "nrelax"):
         assert sg[k] == so[k], k
     assert np.array_equal(g.get_a(), o.get_a())
+
+
+@pytest.mark.parametrize("dim,base,maxlevel,minlevel", [(2, 4, 8, 2), (3, 3, 6, 2)])
+def test_adapt_list_bitexact(dim, base, maxlevel, minlevel):
+    """adapt_wavelet with two estimated user scalars (struct Adapt's slist
+    and max[]) and a third user scalar carried along, then unrefine(): the
+    tree, the counts and every field (user scalars 0, 1, 3, a and b) equal
+    the oracle's, whose list semantics test_adapt_oracle.py pins."""
+    print("adapt list", dim, base, maxlevel, minlevel, flush=True)
+    rng = np.random.default_rng(7 * dim + base)
+    g = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0))
+    o = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: lev < base)
+    tot = np.zeros(2, int)
+    for rnd in range(4):
+        c0, c1 = rng.uniform(0.2, 0.8, (2, 3))[:, :dim]
+        for t in (g, o):
+            _, xyz = t.leaves()
+            p = xyz[:, :dim]
+            t.set_s(_bumps(p, [c0], 0.06), k=0)
+            t.set_s(_bumps(p, [c1], 0.05), k=1)
+            if rnd == 0:
+                t.set_s(np.sin(4 * p[:, 0]) * np.cos(3 * p[:, 1]), k=3)
+                t.set_a(np.sin(3 * p[:, 0]) + p[:, 1] ** 2)
+                t.set_b(np.cos(2 * p.sum(1)))
+        sg = g.adapt_wavelet([3e-3, 2e-3], maxlevel, minlevel, scalars=[0, 1])
+        so = o.adapt_wavelet([3e-3, 2e-3], maxlevel, minlevel, scalars=[0, 1])
+        print("  round", rnd, sg, g.nleaves(), flush=True)
+        assert sg == so
+        tot += sg
+        _same_tree(g, o)
+        for k in (0, 1, 3):
+            assert np.array_equal(g.get_s(k=k), o.get_s(k)), k
+        assert np.array_equal(g.get_a(), o.get_a())
+        assert np.array_equal(g.get_b(), o.get_b())
+    assert tot[0] > 0 and tot[1] > 0
+    cond = lambda x, y, z, lev: (lev > maxlevel - 2) & (x < 0.5)
+    assert g.unrefine(cond) == o.unrefine(cond)
+    _same_tree(g, o)
+    for k in (0, 1, 3):
+        assert np.array_equal(g.get_s(k=k), o.get_s(k)), k
+    # a scalar not set cannot be estimated; indices outside the list fail
+    with pytest.raises(bmg.BMGError):
+        g.adapt_wavelet([1e-2], maxlevel, scalars=[2])
+    with pytest.raises(ValueError):
+        g.set_s(np.zeros(g.nleaves()), k=bmg.USER_SCALARS)
