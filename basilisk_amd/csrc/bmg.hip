@@ -622,8 +622,6 @@ int emb_check(const bmg_grid* g, int minlevel) {
   const EmbState* S = g->emb;
   if (!S) return BMG_OK;
   if (g->precision != BMG_FP64) return fail(BMG_ERR_UNSUPPORTED, "embedded boundaries run in FP64");
-  if (g->smoother == BMG_RELAX_GS)
-    return fail(BMG_ERR_UNSUPPORTED, "embedded boundaries run red-black or Jacobi relaxation");
   for (int l = std::max(minlevel, 0); l <= g->depth; l++)
     if (S->L[l].bad)
       return fail(BMG_ERR_UNSUPPORTED,
@@ -746,8 +744,9 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
     ProfScope ps(g, l == g->depth ? PROF_RELAX_FINE : PROF_RELAX_COARSE);
     switch (level_kernel<DIM>(g, l)) {
       case RK_EMBED:
-        // the EMBED branch (2D, FP64): Jacobi in one pass; red-black as two
-        // passes, each from the state before it, that end in da[src]
+        // the EMBED branch (2D, FP64): Jacobi in one pass; the reference's
+        // Gauss-Seidel order as hyperplanes in one workgroup; red-black as
+        // two passes, each from the state before it, that end in da[src]
         if constexpr (DIM == 2 && sizeof(T) == 8) {
           const EmbLevel E = emb_level(g, l);
           Geo<2> g2{lv.n, lv.log2n};
@@ -758,6 +757,10 @@ int sweep(bmg_grid* g, int l, int src, int* rc) {
           if (g->smoother == BMG_RELAX_JACOBI) {
             hipLaunchKernelGGL(k_relax_embed, dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, d0, d1, r, k2,
                                g2, bc, D * D, D, -1, E);
+            out = src ^ 1;
+          } else if (g->smoother == BMG_RELAX_GS) {
+            hipLaunchKernelGGL(k_gs_embed_block, dim3(1), dim3(1024), 0, g->stream, d0, d1, r, k2, g2, bc, D * D, D,
+                               E);
             out = src ^ 1;
           } else {
             hipLaunchKernelGGL(k_relax_embed, dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, d0, d1, r, k2,
@@ -1037,7 +1040,7 @@ int mg_cycle_t(bmg_grid* g, int nrelax, int minlevel, int maxlevel, bool defer) 
     const bool two = use_sweep2<DIM, T>(g, l);
     const int flips = two ? nrelax - nrelax / 2 : nrelax;
     const bool inplace = rk == RK_INPLACE_RB || rk == RK_PAIR_RB ||
-                         (rk == RK_EMBED && g->smoother != BMG_RELAX_JACOBI);
+                         (rk == RK_EMBED && g->smoother == BMG_RELAX_RBGS);
     int dst = inplace ? 0 : (flips & 1);
     if (l == minlevel) {
       hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(lv.ncell)), dim3(kBlock), 0, g->stream, (T*)lv.da[dst],

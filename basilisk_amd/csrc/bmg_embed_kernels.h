@@ -42,10 +42,12 @@ __device__ __forceinline__ double emb_quadratic(double x, double a1, double a2, 
   return ((a1) * ((x)-1.) + (a3) * ((x) + 1.)) * (x) / 2. - (a2) * ((x)-1.) * ((x) + 1.);
 }
 
-// embed_flux of field s at cell idx (record R): returns the coefficient of
-// the cell's own value and sets *val; bc, dirichlet: the condition there
-__device__ __forceinline__ double embed_flux_dev(const EmbRec& R, const double* __restrict__ s, idx_t idx,
-                                                 double bc, int dirichlet, double mua, double D, double* val) {
+// embed_flux of field s at a cell (record R; s(o) is the value at flat
+// offset o from the cell): returns the coefficient of the cell's own value
+// and sets *val; bc, dirichlet: the condition there
+template <class S>
+__device__ __forceinline__ double embed_flux_dev(const EmbRec& R, const S& s, double bc, int dirichlet,
+                                                 double mua, double D, double* val) {
   *val = 0.;
   double grad = bc;
   if (!grad && !dirichlet) return 0.;
@@ -55,11 +57,9 @@ __device__ __forceinline__ double embed_flux_dev(const EmbRec& R, const double* 
       coef = -1. / (R.d[0] * D);
       grad = bc / (R.d[0] * D);
     } else {
-      const double v0 =
-          emb_quadratic(R.y1[0], s[idx + R.off[0][0]], s[idx + R.off[0][1]], s[idx + R.off[0][2]]);
+      const double v0 = emb_quadratic(R.y1[0], s(R.off[0][0]), s(R.off[0][1]), s(R.off[0][2]));
       if (R.mode == 2) {
-        const double v1 =
-            emb_quadratic(R.y1[1], s[idx + R.off[1][0]], s[idx + R.off[1][1]], s[idx + R.off[1][2]]);
+        const double v1 = emb_quadratic(R.y1[1], s(R.off[1][0]), s(R.off[1][1]), s(R.off[1][2]));
         grad = (R.d[1] * (bc - v0) / R.d[0] - R.d[0] * (bc - v1) / R.d[1]) / ((R.d[1] - R.d[0]) * D);
       } else {
         grad = (bc - v0) / (R.d[0] * D);
@@ -124,7 +124,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_embed(const double* __restrict
   const int r = E.cell[idx];
   if (r >= 0) {
     double cv;
-    const double e = embed_flux_dev(E.rec[r], src, idx, 0., E.dirichlet, emb_mua(k, g, idx, c), D, &cv);
+    const double e = embed_flux_dev(E.rec[r], [&](int o) { return src[idx + o]; }, 0., E.dirichlet,
+                                    emb_mua(k, g, idx, c), D, &cv);
     nn -= cv * D2;
     dd += e * D2;
   }
@@ -136,6 +137,62 @@ __global__ __launch_bounds__(kBlock) void k_relax_embed(const double* __restrict
     v = nn / dd;
   }
   dst[idx] = colour >= 0 ? v : (self + 2. * v) / 3.;
+}
+
+// relax() without -DJACOBI (poisson.h:296, 303-320) with the EMBED branch:
+// each cell is overwritten as the i-major loop visits it, so a cell reads
+// the new values of the cells before it in that order and the old values of
+// the others, of itself and of the ghosts.  The stencil of a cut cell
+// (embed_flux's interpolation) reaches two cells along each axis, so the
+// cells of the hyperplane w = 3i + j depend only on hyperplanes below w
+// (a cell before (i, j) has w' <= w - 1) and the sweep runs hyperplane by
+// hyperplane in one workgroup, dold -> dnew, equal to the serial loop bit
+// for bit: a flat offset o < 0 is a cell before this one (dnew), o >= 0 one
+// after it or itself (dold).  A cell whose d is zero gets 0 and its res 0.
+__global__ __launch_bounds__(1024) void k_gs_embed_block(const double* __restrict__ dold, double* dnew,
+                                                         double* __restrict__ res, Coef<double> k, Geo<2> g,
+                                                         BCH bc, double D2, double D, EmbLevel E) {
+  const int n = g.n, smax = 4 * (n - 1);
+  for (int s = 0; s <= smax; s++) {
+    // the cells (i, s - 3i) with 0 <= s - 3i < n
+    const int ilo = s - (n - 1) <= 0 ? 0 : (s - (n - 1) + 2) / 3, ihi = min(n - 1, s / 3);
+    for (int i = ilo + (int)threadIdx.x; i <= ihi; i += (int)blockDim.x) {
+      const int c[3] = {i, s - 3 * i, 0};
+      const idx_t idx = g.index(c);
+      auto at = [&](int o) { return o < 0 ? dnew[(int)idx + o] : dold[(int)idx + o]; };
+      const double self = dold[idx];
+      const double lam = k.lam ? k.lam[idx] : k.lam_c;
+      double nn = -D2 * res[idx], dd = -lam * D2;
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        double ah, al;
+        if (k.lo[0]) {
+          ah = alpha_hi<2, double>(k.lo[d], k.wall[d], g, idx, c, d);
+          al = k.lo[d][idx];
+        } else {
+          ah = al = k.alpha;
+        }
+        const double ap = nbr_h<2, double>(dold, g, bc, idx, c, d, +1, self);
+        const double am = c[d] > 0 ? dnew[idx - g.stride(d)] : nbr_h<2, double>(dold, g, bc, idx, c, d, -1, self);
+        nn += ah * ap + al * am;
+        dd += ah + al;
+      }
+      const int r = E.cell[idx];
+      if (r >= 0) {
+        double cv;
+        const double e = embed_flux_dev(E.rec[r], at, 0., E.dirichlet, emb_mua(k, g, idx, c), D, &cv);
+        nn -= cv * D2;
+        dd += e * D2;
+      }
+      if (!dd) {
+        dnew[idx] = 0.;
+        res[idx] = 0.;
+      } else {
+        dnew[idx] = nn / dd;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // face_gradient_x (a, h) at cell c along d, h = 0 low face, 1 high face:
@@ -200,7 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_residual_embed(const double* __restr
     const int q = E.cell[idx];
     if (q >= 0) {
       double cv;
-      const double e = embed_flux_dev(E.rec[q], a, idx, E.bc[idx], E.dirichlet, emb_mua(k, g, idx, c), D, &cv);
+      const double e = embed_flux_dev(E.rec[q], [&](int o) { return a[idx + o]; }, E.bc[idx], E.dirichlet,
+                                    emb_mua(k, g, idx, c), D, &cv);
       r += cv - e * a0;
     }
     res[idx] = r;

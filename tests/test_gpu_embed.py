@@ -4,9 +4,11 @@ bilinear_embed; 2D, FP64) against the oracle (oracle/embed_oracle.h, pinned
 by the reference's test/neumann.ref and dirichlet.ref in
 test_embed_oracle.py): the reference's Johansen-Colella problems with
 Dirichlet and Neumann embedded conditions, the residual of the exact
-solution, whole poisson() solves from minlevel 4 with the red-black and
-Jacobi smoothers -- statistics and solution bit for bit -- and the error
-norms of the reference's logs."""
+solution, whole poisson() solves from minlevel 4 with the reference's
+Gauss-Seidel order, red-black and Jacobi -- statistics and solution bit for
+bit -- and the reference's logs: with Gauss-Seidel, every line of
+neumann.ref (residuals, error norms, cycle count, nrelax) for N = 32 .. 512
+and dirichlet.ref's residuals and error norms."""
 import numpy as np
 import pytest
 
@@ -48,7 +50,7 @@ def test_residual_of_exact_solution_bitexact(N, dirichlet):
 
 @pytest.mark.parametrize("N", [64, 128])
 @pytest.mark.parametrize("dirichlet", [True, False])
-@pytest.mark.parametrize("smoother", ["rbgs", "jacobi"])
+@pytest.mark.parametrize("smoother", ["gs", "rbgs", "jacobi"])
 def test_poisson_bitexact(N, dirichlet, smoother):
     print("case embed poisson", N, dirichlet, smoother, flush=True)
     p = ec.problem(N, dirichlet)
@@ -80,10 +82,49 @@ def test_error_norms_of_the_reference_logs(dirichlet):
         assert line.split() == ref[2 * k + 1].split()[:7], (line, ref[2 * k + 1])
 
 
+def _log_lines(N, dirichlet):
+    """The two lines test/neumann.c and test/dirichlet.c print for N, from
+    the GPU: the residual of the exact solution (full cells, all cells) and,
+    after poisson() to 1e-6 from minlevel 4 with Gauss-Seidel, the six error
+    norms, the cycle count and nrelax."""
+    p = ec.problem(N, dirichlet)
+    g = _gpu(p, "gs")
+    maxp = g.residual()
+    maxf = np.abs(g.get_res()[p["cs"] == 1.0]).max()
+    st = g.poisson(tolerance=1e-6, minlevel=4)
+    nn = ec.norms(p, g.get_a())
+    return ["maxres %d %.3g %.3g" % (N, maxf, maxp),
+            "%d %.3g %.3g %.3g %.3g %.3g %.3g %d %d" % ((N,) + tuple(nn) + (st["i"], st["nrelax"]))]
+
+
+def test_neumann_ref_on_the_gpu():
+    got = []
+    for N in (32, 64, 128, 256, 512):
+        print("case neumann.ref", N, flush=True)
+        got += _log_lines(N, False)
+    assert got == cases.read_ref("neumann.ref")
+
+
+def test_dirichlet_ref_on_the_gpu():
+    """The residual lines and the six norms; the log's cycle counts at N =
+    32 .. 128 come from the reference's quadtree (plain bilinear
+    prolongation), this is the multigrid grid (bilinear_embed)."""
+    ref = cases.read_ref("dirichlet.ref")
+    got = []
+    for N in (32, 64, 128, 256, 512):
+        print("case dirichlet.ref", N, flush=True)
+        got += _log_lines(N, True)
+    for g, r in zip(got, ref):
+        if g.startswith("maxres"):
+            assert g == r
+        else:
+            assert g.split()[:7] == r.split()[:7], (g, r)
+
+
 def test_refusals():
     print("case embed refusals", flush=True)
     p = ec.problem(64, True)
-    for kw in ({"smoother": "gs"}, {"precision": "mixed"}):
+    for kw in ({"precision": "mixed"},):
         g = bmg.Grid(2, 6, L0=1.0, origin=(-0.5, -0.5, 0.0), **kw)
         g.set_alpha([p["fx"], p["fy"]])
         g.set_embed(p["cs"], p["fx"], p["fy"], p["kind"], p["bc"], third=True)
