@@ -150,9 +150,10 @@ int bmg_set_bc_affine(bmg_grid *g, int direction, int sigma, const double *c);
    (the value of the expression, or of the normal gradient; NULL: 0;
    bmg_embed_geometry gives the centroids); BMG_EMBED_NONE removes the
    boundary.  alpha is the caller's (usually fs times a coefficient).  The
-   grid must be FP64, its smoother red-black or Jacobi (a red-black pass
-   computes its cells from the state before the pass: a cut cell's stencil
-   reaches cells of its own parity), and every cut cell's stencil must stay
+   grid must be FP64 (any smoother: GS in the reference's order; a
+   red-black pass computes its cells from the state before the pass, since
+   a cut cell's stencil reaches cells of its own parity), and every cut
+   cell's stencil must stay
    inside the levels the solve relaxes (minlevel and up): otherwise the
    solve returns BMG_ERR_UNSUPPORTED.  Host arrays. */
 int bmg_set_embed(bmg_grid *g, const double *cs, const double *fs_x, const double *fs_y, const double *fs_z,
