@@ -50,7 +50,7 @@ EXPORTS = [
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
     "bmg_tree_adapt_wavelet", "bmg_tree_adapt_wavelet_list", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
-    "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_grid_dump", "bmg_grid_restore",
+    "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_set_axi", "bmg_grid_dump", "bmg_grid_restore",
     "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
     "bmg_poisson_noalloc", "bmg_centered_gradient", "bmg_correction", "bmg_tree_project",
     "bmg_tree_residual_transpose", "bmg_tree_cycle_transpose", "bmg_tree_adjoint",
@@ -245,6 +245,7 @@ def library():
         "bmg_coef_gradient": (I, [P, dp, dp, dp, dp, dp, I]),
         "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_viscosity": (I, [P, dp, dp, dp, dp, dp, dp, D, dp, D, D, dp, I, I, ctypes.POINTER(MGStats)]),
+        "bmg_set_axi": (I, [P, I]),
         "bmg_grid_dump": (I, [P, ctypes.c_char_p, dp, dp, I, D, I]),
         "bmg_grid_restore": (I, [P, ctypes.c_char_p, dp, dp, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
     }
@@ -611,6 +612,13 @@ class Grid:
         if gdev != dev:
             raise ValueError("u and g must both be on the device or both on the host")
         _check(self.lib.bmg_correction(self._g, up[0], up[1], up[2], gp[0], gp[1], gp[2], float(dt), dev))
+
+    def set_axi(self, axi=True):
+        """viscosity() in its AXI form (2D: x the axis, y the radius; the y
+        component's lambda gains dt/rho (sum of the cell's four mu)/2/y^2).
+        mu and rho are the caller's metric-weighted fields, as axi.h gives
+        them (mu times the face's y, rho times the cell's y)."""
+        _check(self.lib.bmg_set_axi(self._g, 1 if axi else 0))
 
     def viscosity(self, u, mu=1.0, rho=1.0, dt=1.0, sigma=None, nrelax=0):
         """viscosity(), viscosity.h:163-178: the implicit viscous step

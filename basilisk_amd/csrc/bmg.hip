@@ -127,6 +127,8 @@ struct bmg_grid {
   // faces per component, rho), and r = u_old on the finest level
   std::vector<double*> visc;
   double* visc_r[3] = {nullptr, nullptr, nullptr};
+  // viscosity(): the AXI form (2D; x the axis, y the radius)
+  bool axi = false;
   // embedded boundary (bmg_set_embed), null when none
   struct EmbState* emb = nullptr;
 };
@@ -2644,6 +2646,8 @@ struct Visc {
     A.rho_c = rho_c;
     A.dt = dt;
     A.D = delta(g, l);
+    A.axi = DIM == 2 && g->axi;
+    A.y0 = g->X0[1];
     return A;
   }
   double* const* lv(int l, int slot) const { return &g->visc[(size_t)l * kViscSlots + slot]; }
@@ -2825,6 +2829,13 @@ int viscosity_t(bmg_grid* g, double* const* u, const double* const* mu, double m
 }  // namespace
 
 extern "C" {
+
+int bmg_set_axi(bmg_grid* g, int axi) {
+  TRY(check_grid(g));
+  if (axi && g->dim != 2) return fail(BMG_ERR_UNSUPPORTED, "AXI is axisymmetric 2D");
+  g->axi = axi != 0;
+  return BMG_OK;
+}
 
 int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const double* mu_x, const double* mu_y,
                   const double* mu_z, double mu_c, const double* rho, double rho_c, double dt, const int* sigma,

@@ -1,7 +1,9 @@
 // libbmg: kernels of viscosity() on the uniform multigrid grid, the second
 // client of mg_solve (viscosity.h:27-178), FP64.
 //
-//   relax_viscosity     viscosity.h:27-96 (non-AXI, lambda = 1): the JACOBI
+//   relax_viscosity     viscosity.h:27-96 (lambda = 1; with AXI, 2D, the y
+//                       component's lambda = 1 + dt/rho (mu.x[] + mu.x[1] +
+//                       mu.y[] + mu.y[0,1])/2/y^2, viscosity.h:12-14): the JACOBI
 //                       variant, w from the old iterate for every component,
 //                       then u = (u + 2 w)/3; or red-black Gauss-Seidel (an
 //                       extension standing in for the sequential in-place
@@ -42,6 +44,8 @@ struct ViscArgs {
   int sig[3][6];          // per component: right, left, top, bottom, front, back
   int periodic[3];        // periodic dimensions (their neighbours wrap around)
   double dt, D;
+  int axi;                // 2D axisymmetric: x the axis, y the radius
+  double y0;              // the box's lower y (cell centres at (j + 1/2) D + y0)
 };
 
 // the values a cell's stencils read: component m at offset o (|o_d| <= 1),
@@ -162,6 +166,17 @@ __device__ __forceinline__ double visc_terms(const VNb<DIM>& N, const double mu[
   return S;
 }
 
+// lambda of component c (viscosity.h:12-23): 1, or with AXI for the y
+// component 1 + dt/rho (mu.x[] + mu.x[1] + mu.y[] + mu.y[0,1])/2/y^2 at the
+// cell's centre, (j + 1/2) Delta + Y0 as the reference's `y`
+template <int DIM>
+__device__ __forceinline__ double visc_lambda(const ViscArgs& A, const double mu[3][2], double dtr, const int x[3],
+                                              int c) {
+  if (DIM != 2 || !A.axi || c != 1) return 1.;
+  const double y = ((double)x[1] + 0.5) * A.D + A.y0;
+  return 1. + dtr * (mu[0][0] + mu[0][1] + mu[1][0] + mu[1][1]) / 2. / (y * y);
+}
+
 template <int DIM>
 __device__ __forceinline__ double visc_diag(const double mu[3][2], int c) {
   double s = 2. * mu[c][1] + 2. * mu[c][0];
@@ -187,7 +202,8 @@ __device__ __forceinline__ void visc_target(const ViscArgs& A, const Geo<DIM>& g
   const double dtr = A.dt / (A.rho ? A.rho[idx] : A.rho_c);
 #pragma unroll
   for (int c = 0; c < DIM; c++)
-    w[c] = (dtr * visc_terms<DIM, false>(N, mu, c) + A.r[c][idx] * D2) / (D2 * 1. + dtr * visc_diag<DIM>(mu, c));
+    w[c] = (dtr * visc_terms<DIM, false>(N, mu, c) + A.r[c][idx] * D2) /
+           (D2 * visc_lambda<DIM>(A, mu, dtr, x, c) + dtr * visc_diag<DIM>(mu, c));
 }
 
 // one Jacobi sweep of relax_viscosity over a level: out = (u + 2 w)/3
@@ -261,7 +277,8 @@ __global__ __launch_bounds__(kBlock) void k_visc_residual(ViscArgs A, Geo<DIM> g
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
       const double u0 = N.v[c][VNb<DIM>::slot(0, 0, 0)];
-      const double v = (A.r[c][idx] - 1. * u0) + dtr * visc_terms<DIM, true>(N, mu, c) / (A.D * A.D);
+      const double v =
+          (A.r[c][idx] - visc_lambda<DIM>(A, mu, dtr, x, c) * u0) + dtr * visc_terms<DIM, true>(N, mu, c) / (A.D * A.D);
       A.out[c][idx] = v;
       if (v != v) atomicAdd(&st->nans, 1ull);
       else m = fmax(m, fabs(v));

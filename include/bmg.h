@@ -221,7 +221,7 @@ int bmg_correction(bmg_grid *g, double *u_x, double *u_y, double *u_z, const dou
    NITERMAX/NITERMIN, nrelax = 0 meaning 4) around residual_viscosity (the
    multigrid grid's "naive" form, viscosity.h:134-157) and relax_viscosity
    (viscosity.h:27-96) run by the grid's smoother (red-black by default,
-   the reference's JACOBI variant with BMG_RELAX_JACOBI), non-AXI, after restriction
+   the reference's JACOBI variant with BMG_RELAX_JACOBI), after restriction
    ({mu, rho}).  u_x/u_y/u_z: the cell components (n^dim, C order), updated
    in place; mu_x/y/z: face fields in the layout of bmg_set_field (n+1 along
    their component), or all NULL for the constant mu_c; rho: a cell field,
@@ -237,6 +237,14 @@ int bmg_correction(bmg_grid *g, double *u_x, double *u_y, double *u_z, const dou
 int bmg_viscosity(bmg_grid *g, double *u_x, double *u_y, double *u_z, const double *mu_x, const double *mu_y,
                   const double *mu_z, double mu_c, const double *rho, double rho_c, double dt, const int *sigma,
                   int nrelax, int on_device, bmg_mgstats *stats);
+/* the AXI form of viscosity() (axi.h, viscosity.h:12-14) on a 2D grid: x is
+   the axis and y the radius, and the y component's lambda is
+   1 + dt/rho (mu.x[] + mu.x[1] + mu.y[] + mu.y[0,1])/2/y^2 with y the cell
+   centre, (j + 1/2) Delta + Y0.  The caller passes the metric-weighted
+   fields axi.h gives the solvers (mu times fm: the cell's y on x faces, the
+   face's y on y faces; rho times cm = y); poisson()'s coefficients carry the
+   metric the same way and need no switch.  axi = 0 turns it off. */
+int bmg_set_axi(bmg_grid *g, int axi);
 
 /* dump() / restore() (output.h:1043-1390, non-MPI, file version 170901) on
    the multigrid grid: foreach_cell() visits every cell of every level in

@@ -8,7 +8,10 @@ round it), of
 
   viscosity()            viscosity.h:163-178 (r = u, restriction ({mu, rho}),
                          mg_solve with minlevel 0 and TOLERANCE)
-  relax_viscosity        viscosity.h:27-96, non-AXI, lambda = 1: the JACOBI
+  relax_viscosity        viscosity.h:27-96, lambda = 1 (or, with AXI, the
+                         y component's lambda = 1 + dt/rho (mu.x[] + mu.x[1]
+                         + mu.y[] + mu.y[0,1])/2/y^2, viscosity.h:12-14, y the
+                         cell centre (j + 1/2) Delta + Y0): the JACOBI
                          variant (w from the old iterate, then
                          u = (u + 2 w)/3), or red-black Gauss-Seidel (an
                          extension standing in for the sequential in-place
@@ -16,7 +19,8 @@ round it), of
                          ones, take w computed from the state before their
                          pass (the cross terms read same-colour diagonal
                          neighbours, which keep their old values)
-  residual_viscosity     viscosity.h:98-160, the "naive" (!TREE) form
+  residual_viscosity     viscosity.h:98-160, the "naive" (!TREE) form, with
+                         the same lambda
   mg_solve, mg_cycle     poisson.h:32-90, 139-235 (nrelax default 4, the
                          nrelax adaptation, NITERMIN/NITERMAX)
   restriction            restriction_average and face_average,
@@ -109,8 +113,12 @@ def _others(c, dim):
 class Viscosity:
     """The viscous problem on a 2^depth grid of the box [0, L0]^dim."""
 
-    def __init__(self, dim, depth, L0=1.0):
+    def __init__(self, dim, depth, L0=1.0, Y0=0.0, axi=False, plant=0):
         self.dim, self.depth, self.L0 = dim, depth, L0
+        # AXI (2D): x is the axis, y the radius; Y0 the box's lower y
+        self.axi, self.Y0 = axi, Y0
+        # planted mistake (tests only): 1 = the AXI term with its sign flipped
+        self.plant = plant
         self.sigma = default_sigma(dim)
         self.TOLERANCE = 1e-3
         self.NITERMAX, self.NITERMIN = 100, 1
@@ -180,6 +188,17 @@ class Viscosity:
     def _dtr(self, l, dt):
         return dt / (self.rho_c if self.rho_c is not None else self.rho[l])
 
+    def _lambda(self, c, l, n, dt):
+        """lambda of component c: 1, or with AXI for the y component
+        1 + dt/rho (mu.x[] + mu.x[1] + mu.y[] + mu.y[0,1])/2/y^2."""
+        if not (self.axi and c == 1):
+            return 1.
+        y = (np.arange(n) + 0.5) * self.delta(l) + self.Y0
+        y = y[None, :]
+        s = self._mu(l, 0, False, n) + self._mu(l, 0, True, n) + self._mu(l, 1, False, n) + self._mu(l, 1, True, n)
+        t = self._dtr(l, dt) * s / 2. / (y * y)
+        return 1. - t if self.plant == 1 else 1. + t
+
     # -- the stencils
     def _terms(self, P, c, l, n, residual):
         """The bracket of relax_viscosity (numerator terms, residual=False)
@@ -222,8 +241,8 @@ class Viscosity:
         D2 = D * D
         P = [_pad(da[c], self.sigma[c], dim, self.periodic) for c in range(dim)]
         dtr = self._dtr(l, dt)
-        return [(dtr * self._terms(P, c, l, n, False) + res[c] * D2) / (D2 * 1. + dtr * self._diag(c, l, n))
-                for c in range(dim)]
+        return [(dtr * self._terms(P, c, l, n, False) + res[c] * D2) /
+                (D2 * self._lambda(c, l, n, dt) + dtr * self._diag(c, l, n)) for c in range(dim)]
 
     def relax(self, da, res, l, dt):
         """relax_viscosity: returns the new components."""
@@ -246,7 +265,7 @@ class Viscosity:
         res = []
         m = 0.
         for c in range(dim):
-            rc = (r[c] - 1. * u[c]) + dtr * self._terms(P, c, l, n, True) / (D * D)
+            rc = (r[c] - self._lambda(c, l, n, dt) * u[c]) + dtr * self._terms(P, c, l, n, True) / (D * D)
             res.append(rc)
             m = max(m, float(np.abs(rc).max()))
         return res, m

@@ -141,3 +141,48 @@ def test_viscosity_tiny_grids_bitexact(dim, depth, smoother):
         assert sg[k] == so[k], (k, sg, so)
     for c in range(dim):
         assert np.array_equal(ug[c], uo[c]), c
+
+
+@pytest.mark.parametrize("depth", [5, 7])
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+@pytest.mark.parametrize("y0", [0.0, 0.5])
+def test_viscosity_axi_bitexact(depth, smoother, y0):
+    """The AXI form (viscosity.h:12-14; pinned in test_viscosity_oracle.py
+    by second-order convergence to the axisymmetric operator): mu times the
+    metric (the cell's y on x faces, the face's y on y faces, with a random
+    variation), rho times the cell's y, no slip on the walls; statistics and
+    both components equal the oracle's bit for bit, the box from y0."""
+    print("case viscosity axi", depth, smoother, y0, flush=True)
+    rng = np.random.default_rng(17 + depth)
+    n = 1 << depth
+    D = 1.0 / n
+    yc = (np.arange(n) + 0.5) * D + y0
+    yf = np.arange(n + 1) * D + y0
+    mux = np.ascontiguousarray(np.tile(yc[None, :], (n + 1, 1)) * rng.uniform(0.5, 2.0, (n + 1, n)))
+    muy = np.ascontiguousarray(np.tile(yf[None, :], (n, 1)) * rng.uniform(0.5, 2.0, (n, n + 1)))
+    rho = np.ascontiguousarray(np.tile(yc[None, :], (n, 1)) * np.where(rng.uniform(size=(n, n)) < 0.3, 5.0, 1.0))
+    u = [np.ascontiguousarray(rng.standard_normal((n, n))) for _ in range(2)]
+    sigma = -np.ones((2, 4), np.int64)
+    dt = 0.02
+    g = bmg.Grid(2, depth, origin=(0.0, y0, 0.0), smoother=smoother)
+    g.set_axi(True)
+    g.set_tolerance(1e-9)
+    o = Viscosity(2, depth, Y0=y0, axi=True)
+    o.smoother = smoother
+    o.TOLERANCE = 1e-9
+    o.sigma = sigma
+    o.set_coefficients([mux, muy], rho)
+    uo, so, _ = o.solve(u, dt)
+    ug = [x.copy() for x in u]
+    sg = g.viscosity(ug, mu=[mux, muy], rho=rho, dt=dt, sigma=sigma)
+    for k in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert sg[k] == so[k], (k, sg, so)
+    for c in range(2):
+        assert np.array_equal(ug[c], uo[c]), c
+    # AXI off again: the plain operator (a different solution)
+    g.set_axi(False)
+    ug2 = [x.copy() for x in u]
+    g.viscosity(ug2, mu=[mux, muy], rho=rho, dt=dt, sigma=sigma)
+    assert not np.array_equal(ug2[1], ug[1])
+    with pytest.raises(bmg.BMGError):
+        bmg.Grid(3, 3).set_axi(True)

@@ -249,3 +249,79 @@ def test_periodic_residual_second_order_everywhere(dim, periodic):
         errs.append(m)
     for a, b in zip(errs, errs[1:]):
         assert 3.5 < a / b < 4.5, errs
+
+
+# -- AXI (viscosity.h:12-14): x the axis, y the radius, with the metric
+# axi.h gives the caller's fields: mu on x faces times the cell's y, on y
+# faces times the face's y; rho times the cell's y
+
+def _axi_problem(depth, dt, plant=0):
+    """u = sin(a x) y^2, v = cos(a x) y^3 (mu = 1) and the right-hand side
+    r = u - dt div(2 D(u)) of the axisymmetric operator, worked out by hand:
+      (div tau)_x = sin(a x) (4 - 4 a y^2 - 2 a^2 y^2)
+      (div tau)_r = cos(a x) (16 y + 2 a y - a^2 y^3)
+    (tau_xx = 2 a cos y^2, tau_rr = 6 cos y^2, tau_xr = sin (2 y - a y^3),
+    tau_thetatheta = 2 v / y: the hoop term -tau_thetatheta / y)."""
+    n = 1 << depth
+    D = 1.0 / n
+    V = Viscosity(2, depth, axi=True, plant=plant)
+    xc = (np.arange(n) + 0.5) * D
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    yf = np.arange(n + 1) * D
+    mux = np.tile(xc[None, :], (n + 1, 1))    # x faces: the cell's y
+    muy = np.tile(yf[None, :], (n, 1))        # y faces: the face's y
+    V.set_coefficients([mux, muy], Y.copy())  # rho = cm = y
+    a = 2.0
+    u = np.sin(a * X) * Y ** 2
+    v = np.cos(a * X) * Y ** 3
+    divx = np.sin(a * X) * (4 - 4 * a * Y ** 2 - 2 * a * a * Y ** 2)
+    divr = np.cos(a * X) * (16 * Y + 2 * a * Y - a * a * Y ** 3)
+    r = [u - dt * divx, v - dt * divr]
+    return V, [u, v], r, X, Y
+
+
+@pytest.mark.parametrize("plant", [0, 1])
+def test_axi_residual_second_order(plant):
+    """The residual of the exact solution against the hand-derived
+    axisymmetric operator falls at second order in the cells away from the
+    walls and the axis; the AXI term with its sign flipped (plant 1) leaves
+    an O(1) residual."""
+    dt = 0.01
+    errs = []
+    for depth in (5, 6, 7):
+        V, U, r, X, Y = _axi_problem(depth, dt, plant)
+        res, _ = V.residual(U, r, dt)
+        n = 1 << depth
+        m = (X > 2.5 / n) & (X < 1 - 2.5 / n) & (Y > 0.25) & (Y < 1 - 2.5 / n)
+        errs.append(max(np.abs(res[c][m]).max() for c in range(2)))
+    if plant:
+        assert errs[-1] > 1e-4
+    else:
+        assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5, errs
+        assert errs[-1] < 1e-5
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+def test_axi_relax_matches_residual_and_solve_matches_direct(smoother):
+    """relax and residual share the AXI lambda (the Jacobi target satisfies
+    w - u = Delta^2 res / diag with diag = Delta^2 lambda + dt/rho sum mu),
+    and mg_solve converges to the direct solution of the same operator."""
+    dt = 0.01
+    V, U, r, X, Y = _axi_problem(5, dt)
+    n = 32
+    D2 = (1.0 / n) ** 2
+    res, _ = V.residual(U, r, dt)
+    V.smoother = "jacobi"
+    new = V.relax(U, r, 5, dt)
+    for c in range(2):
+        w = (3.0 * new[c] - U[c]) / 2.0
+        diag = D2 * V._lambda(c, 5, n, dt) + dt / V.rho[5] * V._diag(c, 5, n)
+        np.testing.assert_allclose(w - U[c], D2 * res[c] / diag, rtol=1e-9, atol=1e-12)
+    V.smoother = smoother
+    V.TOLERANCE = 1e-12
+    u, s, hist = V.solve(r, dt)
+    assert s["resa"] <= 1e-12
+    J = _operator(V, dt)
+    direct = scipy.sparse.linalg.spsolve(scipy.sparse.csr_matrix(J), np.concatenate([q.reshape(-1) for q in r]))
+    got = np.concatenate([q.reshape(-1) for q in u])
+    assert np.abs(got - direct).max() < 1e-10 * max(1.0, np.abs(direct).max())
