@@ -625,7 +625,8 @@ class Grid:
         u - dt/rho div(2 mu D(u)) = u_old for the cell components `u` (one
         (n,)*dim array per component, updated in place), with relax_viscosity
         run by the grid's smoother (red-black by default, the reference's
-        JACOBI variant with smoother="jacobi"); mu: one face array per component (n+1 along
+        JACOBI variant with smoother="jacobi", its default in-place
+        Gauss-Seidel with smoother="gs"); mu: one face array per component (n+1 along
         it) or a constant; rho: a cell array or a constant; sigma: (dim, 2
         dim) wall factors (default: -1 for the normal component, +1 for the
         tangential ones).  torch CUDA tensors stay on the device, numpy

@@ -2716,13 +2716,28 @@ struct Visc {
                              vbuf(g, l - 1, VS_DA + c), vbuf(g, l, VS_DA + c), gf, bcc);
         }
       }
-      const bool jacobi = g->smoother == BMG_RELAX_JACOBI;
+      // passes per sweep: Jacobi one (colour -1), Gauss-Seidel one (the
+      // single cell of level 0 as colour 0), red-black two
+      const bool jacobi = g->smoother == BMG_RELAX_JACOBI, gs = g->smoother == BMG_RELAX_GS;
+      const int c0 = jacobi ? -1 : 0, c1 = jacobi || gs ? c0 + 1 : 2;
       for (int i = 0; i < nrelax; i++)
-        for (int colour = jacobi ? -1 : 0; colour < (jacobi ? 0 : 2); colour++) {
+        for (int colour = c0; colour < c1; colour++) {
           const ViscArgs A = args(l, lv(l, VS_DA), lv(l, VS_DN), lv(l, VS_RES));
-          if (lv_.n < 2)
+          if (lv_.n < 2) {
             hipLaunchKernelGGL((k_visc_one<DIM>), dim3(1), dim3(64), 0, g->stream, A, gf, colour);
-          else if (jacobi)
+          } else if (gs) {
+            // hyperplanes in one workgroup up to 16384 cells per
+            // hyperplane's index range, else one launch each
+            const size_t count = DIM == 3 ? (size_t)lv_.n * lv_.n : (size_t)lv_.n;
+            if (count <= 16384) {
+              hipLaunchKernelGGL((k_visc_gs_block<DIM>), dim3(1), dim3(1024), 0, g->stream, A, gf);
+            } else {
+              const int smax = (DIM == 3 ? 7 : 3) * (lv_.n - 1);
+              for (int s = 0; s <= smax; s++)
+                hipLaunchKernelGGL((k_visc_gs_plane<DIM>), dim3(nblocks(count)), dim3(kBlock), 0, g->stream, A,
+                                   gf, s);
+            }
+          } else if (jacobi)
             hipLaunchKernelGGL((k_visc_jacobi<DIM>), dim3(nblocks(lv_.ncell)), dim3(kBlock), 0, g->stream, A, gf);
           else
             hipLaunchKernelGGL((k_visc_rb<DIM>), dim3(nblocks(lv_.ncell / 2)), dim3(kBlock), 0, g->stream, A, gf,
@@ -2842,8 +2857,6 @@ int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const doub
                   int nrelax, int on_device, bmg_mgstats* stats) {
   TRY(check_grid(g));
   if (g->dim < 2) return fail(BMG_ERR_UNSUPPORTED, "viscosity() is built for 2D and 3D grids");
-  if (g->smoother == BMG_RELAX_GS)
-    return fail(BMG_ERR_UNSUPPORTED, "viscosity() runs red-black or Jacobi relaxation; set one of them");
   double* u[3] = {u_x, u_y, u_z};
   const double* mu[3] = {mu_x, mu_y, mu_z};
   int nmu = 0;

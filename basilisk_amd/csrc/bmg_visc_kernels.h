@@ -3,7 +3,9 @@
 //
 //   relax_viscosity     viscosity.h:27-96 (lambda = 1; with AXI, 2D, the y
 //                       component's lambda = 1 + dt/rho (mu.x[] + mu.x[1] +
-//                       mu.y[] + mu.y[0,1])/2/y^2, viscosity.h:12-14): the JACOBI
+//                       mu.y[] + mu.y[0,1])/2/y^2, viscosity.h:12-14): the
+//                       default in-place Gauss-Seidel in the reference's
+//                       i-major order (hyperplanes, see k_visc_gs_block), the JACOBI
 //                       variant, w from the old iterate for every component,
 //                       then u = (u + 2 w)/3; or red-black Gauss-Seidel (an
 //                       extension standing in for the sequential in-place
@@ -115,6 +117,36 @@ __device__ __forceinline__ void visc_load(const ViscArgs& A, const VAxes<DIM>& X
           }
 }
 
+// the same values for the Gauss-Seidel sweep (relax_viscosity without
+// -DJACOBI: w = u, each cell overwritten as the i-major loop visits it): a
+// neighbour before this cell in that order (the first non-zero offset is
+// -1) that lies inside the level has its new value (A.out), every other one
+// and every ghost (wrapped or mirrored) the sweep's start value (A.u)
+template <int DIM>
+__device__ __forceinline__ void visc_load_gs(const ViscArgs& A, const VAxes<DIM>& X, const int x[3], int n,
+                                             VNb<DIM>& N) {
+#pragma unroll
+  for (int m = 0; m < DIM; m++)
+#pragma unroll
+    for (int ox = -1; ox <= 1; ox++)
+#pragma unroll
+      for (int oy = -1; oy <= 1; oy++)
+#pragma unroll
+        for (int oz = (DIM == 3 ? -1 : 0); oz <= (DIM == 3 ? 1 : 0); oz++)
+          if (VNb<DIM>::needed(m, ox, oy, oz)) {
+            const int o[3] = {ox, oy, oz};
+            const int first = ox ? ox : oy ? oy : oz;
+            bool inside = true;
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+              if (o[d]) inside = inside && x[d] + o[d] >= 0 && x[d] + o[d] < n;
+            const idx_t a = X.off[0][ox + 1] + X.off[1][oy + 1] + X.off[2][oz + 1];
+            const int s = X.factor(A, m, 0, ox) * X.factor(A, m, 1, oy) * X.factor(A, m, 2, oz);
+            const double* src = first < 0 && inside ? A.out[m] : A.u[m];
+            N.v[m][VNb<DIM>::slot(ox, oy, oz)] = (double)s * src[a];
+          }
+}
+
 // mu on the low (0) and high (1) face of cell idx along each axis
 template <int DIM>
 __device__ __forceinline__ void visc_mu(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3],
@@ -188,14 +220,18 @@ __device__ __forceinline__ double visc_diag(const double mu[3][2], int c) {
   return s;
 }
 
-// w of relax_viscosity for every component of cell idx
-template <int DIM>
+// w of relax_viscosity for every component of cell idx (GS: the values of
+// the Gauss-Seidel sweep)
+template <int DIM, bool GS = false>
 __device__ __forceinline__ void visc_target(const ViscArgs& A, const Geo<DIM>& g, idx_t idx, const int x[3],
                                             double w[3]) {
   VAxes<DIM> X;
   X.init(g, x, A.periodic);
   VNb<DIM> N;
-  visc_load<DIM>(A, X, N);
+  if (GS)
+    visc_load_gs<DIM>(A, X, x, g.n, N);
+  else
+    visc_load<DIM>(A, X, N);
   double mu[3][2];
   visc_mu<DIM>(A, g, idx, x, mu);
   const double D2 = A.D * A.D;
@@ -241,6 +277,59 @@ __global__ __launch_bounds__(kBlock) void k_visc_rb(ViscArgs A, Geo<DIM> g, int 
     A.out[c][idx] = w[c];
     A.out[c][other] = A.u[c][other];
   }
+}
+
+// The Gauss-Seidel sweep: a cell reads the new values of the cells before
+// it in the i-major loop and the old values of the others; its stencils
+// reach at most one cell along two axes, so with w = 4i + 2j + k (3D) or
+// 2i + j (2D) every cell before (i, j, k) in that order has a smaller w and
+// every cell after it a larger one: the cells of one hyperplane w = s are
+// independent and the sweep runs hyperplane by hyperplane, A.u -> A.out,
+// equal to the serial loop bit for bit.  Thread t of hyperplane s: (i, j) =
+// (t / n, t % n) and k = s - 4i - 2j in 3D, i = t and j = s - 2i in 2D.
+template <int DIM>
+__device__ __forceinline__ bool visc_gs_cell(const Geo<DIM>& g, int s, idx_t t, int x[3]) {
+  if (DIM == 3) {
+    x[0] = (int)(t >> g.log2n);
+    x[1] = (int)(t & (idx_t)(g.n - 1));
+    x[2] = s - 4 * x[0] - 2 * x[1];
+    return x[2] >= 0 && x[2] < g.n;
+  }
+  x[0] = (int)t;
+  x[1] = s - 2 * x[0];
+  x[2] = 0;
+  return x[1] >= 0 && x[1] < g.n;
+}
+
+template <int DIM>
+__device__ __forceinline__ void visc_gs_at(const ViscArgs& A, const Geo<DIM>& g, int s, idx_t t) {
+  int x[3];
+  if (!visc_gs_cell<DIM>(g, s, t, x)) return;
+  const idx_t idx = g.index(x);
+  double w[3];
+  visc_target<DIM, true>(A, g, idx, x, w);
+#pragma unroll
+  for (int c = 0; c < DIM; c++) A.out[c][idx] = w[c];
+}
+
+// the hyperplanes (7 (n-1) + 1 in 3D, 3 (n-1) + 1 in 2D) in one workgroup
+// with a barrier between them
+template <int DIM>
+__global__ __launch_bounds__(1024) void k_visc_gs_block(ViscArgs A, Geo<DIM> g) {
+  const int smax = (DIM == 3 ? 7 : 3) * (g.n - 1);
+  const idx_t count = DIM == 3 ? (idx_t)g.n * g.n : (idx_t)g.n;
+  for (int s = 0; s <= smax; s++) {
+    for (idx_t t = threadIdx.x; t < count; t += blockDim.x) visc_gs_at<DIM>(A, g, s, t);
+    __syncthreads();
+  }
+}
+
+// one hyperplane per launch (large levels)
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_visc_gs_plane(ViscArgs A, Geo<DIM> g, int s) {
+  const idx_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (DIM == 3 ? (idx_t)g.n * g.n : (idx_t)g.n)) return;
+  visc_gs_at<DIM>(A, g, s, t);
 }
 
 // the single cell of level 0: either parity, the same arithmetic

@@ -221,7 +221,8 @@ int bmg_correction(bmg_grid *g, double *u_x, double *u_y, double *u_z, const dou
    NITERMAX/NITERMIN, nrelax = 0 meaning 4) around residual_viscosity (the
    multigrid grid's "naive" form, viscosity.h:134-157) and relax_viscosity
    (viscosity.h:27-96) run by the grid's smoother (red-black by default,
-   the reference's JACOBI variant with BMG_RELAX_JACOBI), after restriction
+   the reference's JACOBI variant with BMG_RELAX_JACOBI, its default in-place
+   Gauss-Seidel in the i-major order with BMG_RELAX_GS), after restriction
    ({mu, rho}).  u_x/u_y/u_z: the cell components (n^dim, C order), updated
    in place; mu_x/y/z: face fields in the layout of bmg_set_field (n+1 along
    their component), or all NULL for the constant mu_c; rho: a cell field,
@@ -231,9 +232,9 @@ int bmg_correction(bmg_grid *g, double *u_x, double *u_y, double *u_z, const dou
    tangential ones; -1 everywhere is no slip).  Periodic dimensions
    (grid/multigrid.h:600-701) wrap: the ghosts copy the opposite cells, on
    level 0 every ghost is the root value, and face n of mu along a periodic
-   dimension is face 0 (the value given for face n is not read).  2D and 3D;
-   the GS smoother is refused.  Device pointers when on_device, else host
-   arrays.  FP64, whatever the grid's precision. */
+   dimension is face 0 (the value given for face n is not read).  2D and 3D.
+   Device pointers when on_device, else host arrays.  FP64, whatever the
+   grid's precision. */
 int bmg_viscosity(bmg_grid *g, double *u_x, double *u_y, double *u_z, const double *mu_x, const double *mu_y,
                   const double *mu_z, double mu_c, const double *rho, double rho_c, double dt, const int *sigma,
                   int nrelax, int on_device, bmg_mgstats *stats);

@@ -11,14 +11,17 @@ round it), of
   relax_viscosity        viscosity.h:27-96, lambda = 1 (or, with AXI, the
                          y component's lambda = 1 + dt/rho (mu.x[] + mu.x[1]
                          + mu.y[] + mu.y[0,1])/2/y^2, viscosity.h:12-14, y the
-                         cell centre (j + 1/2) Delta + Y0): the JACOBI
-                         variant (w from the old iterate, then
-                         u = (u + 2 w)/3), or red-black Gauss-Seidel (an
-                         extension standing in for the sequential in-place
-                         default): the cells with i+j+k even, then the odd
-                         ones, take w computed from the state before their
-                         pass (the cross terms read same-colour diagonal
-                         neighbours, which keep their old values)
+                         cell centre (j + 1/2) Delta + Y0): the default
+                         in-place Gauss-Seidel (a serial loop over the cells
+                         in i-major order, each component overwritten as it
+                         is computed; the ghosts keep the sweep's start
+                         values), the JACOBI variant (w from the old
+                         iterate, then u = (u + 2 w)/3), or red-black
+                         Gauss-Seidel (an extension): the cells with i+j+k
+                         even, then the odd ones, take w computed from the
+                         state before their pass (the cross terms read
+                         same-colour diagonal neighbours, which keep their
+                         old values)
   residual_viscosity     viscosity.h:98-160, the "naive" (!TREE) form, with
                          the same lambda
   mg_solve, mg_cycle     poisson.h:32-90, 139-235 (nrelax default 4, the
@@ -180,40 +183,48 @@ class Viscosity:
             s = term if s is None else s + term
         return s / float(len(order))
 
-    def _mu(self, l, d, hi, n):
+    def _mu(self, l, d, hi, n, at=None):
         if self.mu_c is not None:
             return self.mu_c
-        return _face(self.mu[l][d], d, hi, n, self.dim)
+        f = _face(self.mu[l][d], d, hi, n, self.dim)
+        return f if at is None else f[at]
 
-    def _dtr(self, l, dt):
-        return dt / (self.rho_c if self.rho_c is not None else self.rho[l])
+    def _dtr(self, l, dt, at=None):
+        if self.rho_c is not None:
+            return dt / self.rho_c
+        return dt / (self.rho[l] if at is None else self.rho[l][at])
 
-    def _lambda(self, c, l, n, dt):
+    def _lambda(self, c, l, n, dt, at=None):
         """lambda of component c: 1, or with AXI for the y component
-        1 + dt/rho (mu.x[] + mu.x[1] + mu.y[] + mu.y[0,1])/2/y^2."""
+        1 + dt/rho (mu.x[] + mu.x[1] + mu.y[] + mu.y[0,1])/2/y^2 (at: one
+        cell, else every cell)."""
         if not (self.axi and c == 1):
             return 1.
         y = (np.arange(n) + 0.5) * self.delta(l) + self.Y0
-        y = y[None, :]
-        s = self._mu(l, 0, False, n) + self._mu(l, 0, True, n) + self._mu(l, 1, False, n) + self._mu(l, 1, True, n)
-        t = self._dtr(l, dt) * s / 2. / (y * y)
+        y = y[None, :] if at is None else y[at[1]]
+        s = (self._mu(l, 0, False, n, at) + self._mu(l, 0, True, n, at) + self._mu(l, 1, False, n, at) +
+             self._mu(l, 1, True, n, at))
+        t = self._dtr(l, dt, at) * s / 2. / (y * y)
         return 1. - t if self.plant == 1 else 1. + t
 
     # -- the stencils
-    def _terms(self, P, c, l, n, residual):
+    def _terms(self, P, c, l, n, residual, at=None):
         """The bracket of relax_viscosity (numerator terms, residual=False)
         or of residual_viscosity (residual=True) for component c; P are the
-        padded components."""
+        padded components (at: one cell, else every cell)."""
         dim = self.dim
-        U = lambda comp, *pairs: _sh(P[comp], _unit(dim, *pairs), n, dim)
+        if at is None:
+            U = lambda comp, *pairs: _sh(P[comp], _unit(dim, *pairs), n, dim)
+        else:
+            U = lambda comp, *pairs: P[comp][tuple(1 + i + o for i, o in zip(at, _unit(dim, *pairs)))]
         u0 = U(c)
-        mch, mcl = self._mu(l, c, True, n), self._mu(l, c, False, n)
+        mch, mcl = self._mu(l, c, True, n, at), self._mu(l, c, False, n, at)
         if residual:
             S = 2. * mch * (U(c, (c, 1)) - u0) - 2. * mcl * (u0 - U(c, (c, -1)))
         else:
             S = 2. * mch * U(c, (c, 1)) + 2. * mcl * U(c, (c, -1))
         for e in _others(c, dim):
-            meh, mel = self._mu(l, e, True, n), self._mu(l, e, False, n)
+            meh, mel = self._mu(l, e, True, n, at), self._mu(l, e, False, n, at)
             up = (U(e, (c, 1)) + U(e, (c, 1), (e, 1))) / 4.
             um = (U(e, (c, -1)) + U(e, (c, -1), (e, 1))) / 4.
             lp = (U(e, (c, 1), (e, -1)) + U(e, (c, 1))) / 4.
@@ -226,11 +237,11 @@ class Viscosity:
                 S = S - mel * (-U(c, (e, -1)) + lp - lm)
         return S
 
-    def _diag(self, c, l, n):
+    def _diag(self, c, l, n, at=None):
         dim = self.dim
-        s = 2. * self._mu(l, c, True, n) + 2. * self._mu(l, c, False, n)
+        s = 2. * self._mu(l, c, True, n, at) + 2. * self._mu(l, c, False, n, at)
         for e in _others(c, dim):
-            s = s + self._mu(l, e, True, n) + self._mu(l, e, False, n)
+            s = s + self._mu(l, e, True, n, at) + self._mu(l, e, False, n, at)
         return s
 
     def _target(self, da, res, l, dt):
@@ -246,6 +257,22 @@ class Viscosity:
 
     def relax(self, da, res, l, dt):
         """relax_viscosity: returns the new components."""
+        if self.smoother == "gs":
+            # the serial loop of foreach_level (l), i-major, w = u: each
+            # component is overwritten as it is computed; the ghosts keep
+            # the values of the sweep's start
+            dim = self.dim
+            n = 1 << l
+            D = self.delta(l)
+            D2 = D * D
+            P = [_pad(da[c], self.sigma[c], dim, self.periodic) for c in range(dim)]
+            for at in np.ndindex(*(n,) * dim):
+                dtr = self._dtr(l, dt, at)
+                for c in range(dim):
+                    w = (dtr * self._terms(P, c, l, n, False, at) + res[c][at] * D2) / \
+                        (D2 * self._lambda(c, l, n, dt, at) + dtr * self._diag(c, l, n, at))
+                    P[c][tuple(1 + i for i in at)] = w
+            return [P[c][(slice(1, n + 1),) * dim].copy() for c in range(dim)]
         if self.smoother == "jacobi":
             w = self._target(da, res, l, dt)
             return [(da[c] + 2. * w[c]) / 3. for c in range(self.dim)]

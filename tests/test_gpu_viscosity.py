@@ -1,7 +1,8 @@
 """viscosity() on the GPU (bmg_viscosity, viscosity.h:163-178) against the
-oracle's restatement (oracle/viscosity.py): with the Jacobi relax_viscosity
-(the reference's JACOBI variant) and with the red-black Gauss-Seidel
-extension, the solver statistics and every velocity component equal the
+oracle's restatement (oracle/viscosity.py): with the reference's default
+in-place Gauss-Seidel (hyperplanes on the GPU, the serial loop in the
+oracle), the Jacobi relax_viscosity (the reference's JACOBI variant) and
+the red-black Gauss-Seidel extension, the solver statistics and every velocity component equal the
 oracle's bit for bit, for variable and constant mu and rho, the default wall
 BCs and no slip, periodic dimensions, in 2D and 3D.  The oracle is pinned
 by test_viscosity_oracle.py."""
@@ -85,9 +86,6 @@ def test_viscosity_zero_dt_is_identity_and_errors():
         g.viscosity([x.copy() for x in u], mu=mu, rho=rho, dt=0.1, sigma=[[2, 1, 1, 1], [1, 1, 1, 1]])
     with pytest.raises(bmg.BMGError):
         g.viscosity([x.copy() for x in u], mu=1.0, rho=-1.0, dt=0.1)
-    p = bmg.Grid(2, 5, smoother="gs")
-    with pytest.raises(bmg.BMGError):
-        p.viscosity([x.copy() for x in u], mu=1.0, rho=1.0, dt=0.1)
 
 
 @pytest.mark.parametrize("dim,depth,periodic", [(2, 6, (True, False)), (2, 6, (False, True)), (2, 5, (True, True)),
@@ -186,3 +184,40 @@ def test_viscosity_axi_bitexact(depth, smoother, y0):
     assert not np.array_equal(ug2[1], ug[1])
     with pytest.raises(bmg.BMGError):
         bmg.Grid(3, 3).set_axi(True)
+
+
+@pytest.mark.parametrize("dim,depth,case", [(2, 5, "walls"), (2, 4, "noslip"), (2, 4, "periodic"), (2, 4, "axi"),
+                                            (3, 3, "walls"), (3, 3, "periodic"), (2, 1, "walls"), (3, 2, "noslip")])
+def test_viscosity_gauss_seidel_bitexact(dim, depth, case):
+    """relax_viscosity without -DJACOBI, the reference's default: each cell
+    overwritten as the i-major loop visits it.  The GPU sweeps hyperplanes
+    (4i + 2j + k, or 2i + j); the oracle runs the serial loop.  Statistics
+    and every component equal bit for bit (sizes the serial loop finishes
+    in seconds)."""
+    print("case viscosity gs", dim, depth, case, flush=True)
+    u, mu, rho = _problem(dim, depth, 53 * dim + depth, True)
+    dt = 0.004
+    g = bmg.Grid(dim, depth, smoother="gs")
+    g.set_tolerance(1e-8)
+    o = Viscosity(dim, depth, axi=case == "axi")
+    o.smoother = "gs"
+    o.TOLERANCE = 1e-8
+    sigma = None
+    if case == "noslip" or case == "axi":
+        sigma = -np.ones((dim, 2 * dim), np.int64)
+        o.sigma = sigma
+    if case == "periodic":
+        o.periodic = [True] + [False] * (dim - 1)
+        g.set_periodic(0)
+        mu = [m.copy() for m in mu]
+        mu[0][-1] = mu[0][0]
+    if case == "axi":
+        g.set_axi(True)
+    o.set_coefficients(mu, rho)
+    uo, so, _ = o.solve(u, dt)
+    ug = [x.copy() for x in u]
+    sg = g.viscosity(ug, mu=mu, rho=rho, dt=dt, sigma=sigma)
+    for k in ("i", "resb", "resa", "nrelax", "minlevel"):
+        assert sg[k] == so[k], (k, sg, so)
+    for c in range(dim):
+        assert np.array_equal(ug[c], uo[c]), c

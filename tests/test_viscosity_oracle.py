@@ -9,7 +9,9 @@ logs come from whole Navier-Stokes runs), so:
 - relax_viscosity and residual_viscosity describe the same operator: the
   Jacobi target satisfies w - u = Delta^2 res / diag;
 - mg_solve converges to the direct sparse solution of the same operator,
-  with the Jacobi relax and with the red-black Gauss-Seidel extension.
+  with the reference's in-place Gauss-Seidel (the serial i-major loop), the
+  Jacobi relax and the red-black Gauss-Seidel extension; the serial loop's
+  per-cell arithmetic equals the vectorised stencil's bit for bit.
 - on periodic dimensions (grid/multigrid.h:600-701) the residual commutes
   with a cyclic shift of the fields, converges at second order to the
   continuous operator in every cell, corners between a periodic and a wall
@@ -115,7 +117,7 @@ def _periodic_faces(mu, periodic):
     (2, 5, None, None), (2, 4, "noslip", None), (3, 3, None, None),
     (2, 4, None, (True, False)), (2, 4, "noslip", (False, True)), (2, 4, None, (True, True)),
     (3, 3, None, (True, False, True)), (3, 3, None, (True, True, True))])
-@pytest.mark.parametrize("smoother", ["jacobi", "rbgs"])
+@pytest.mark.parametrize("smoother", ["jacobi", "rbgs", "gs"])
 def test_solve_matches_direct(dim, depth, sigma, periodic, smoother):
     V = Viscosity(dim, depth)
     V.smoother = smoother
@@ -325,3 +327,36 @@ def test_axi_relax_matches_residual_and_solve_matches_direct(smoother):
     direct = scipy.sparse.linalg.spsolve(scipy.sparse.csr_matrix(J), np.concatenate([q.reshape(-1) for q in r]))
     got = np.concatenate([q.reshape(-1) for q in u])
     assert np.abs(got - direct).max() < 1e-10 * max(1.0, np.abs(direct).max())
+
+
+@pytest.mark.parametrize("dim,depth,axi", [(2, 4, False), (2, 4, True), (3, 3, False)])
+def test_gs_cell_arithmetic_equals_the_vectorised_stencil(dim, depth, axi):
+    """The serial Gauss-Seidel loop evaluates relax_viscosity one cell at a
+    time; on a state it has not yet modified (the first cell) and on every
+    cell of a fixed state its w equals the vectorised Jacobi target bit for
+    bit, so the loop differs from the pinned stencil only in which values it
+    reads."""
+    rng = np.random.default_rng(dim + depth)
+    n = 1 << depth
+    V = Viscosity(dim, depth, axi=axi, Y0=0.3 if axi else 0.0)
+    V.set_coefficients(_faces(rng, dim, n), rng.uniform(0.5, 2.0, (n,) * dim))
+    dt = 0.01
+    u = [rng.standard_normal((n,) * dim) for _ in range(dim)]
+    r = [rng.standard_normal((n,) * dim) for _ in range(dim)]
+    from oracle.viscosity import _pad
+    P = [_pad(u[c], V.sigma[c], dim, V.periodic) for c in range(dim)]
+    w = V._target(u, r, depth, dt)
+    D2 = V.delta(depth) ** 2
+    for at in np.ndindex(*(n,) * dim):
+        dtr = V._dtr(depth, dt, at)
+        for c in range(dim):
+            wc = (dtr * V._terms(P, c, depth, n, False, at) + r[c][at] * D2) / \
+                (D2 * V._lambda(c, depth, n, dt, at) + dtr * V._diag(c, depth, n, at))
+            assert wc == w[c][at]
+    # one GS sweep: the first cell in i-major order takes w, the last one
+    # differs from it (it reads updated neighbours)
+    V.smoother = "gs"
+    new = V.relax(u, r, depth, dt)
+    first, last = (0,) * dim, (n - 1,) * dim
+    assert all(new[c][first] == w[c][first] for c in range(dim))
+    assert any(new[c][last] != w[c][last] for c in range(dim))
