@@ -50,7 +50,7 @@ EXPORTS = [
     "bmg_tree_restriction_alpha_lambda", "bmg_tree_residual", "bmg_tree_relax",
     "bmg_tree_mg_cycle", "bmg_tree_mg_solve", "bmg_tree_poisson", "bmg_tree_synchronize",
     "bmg_tree_adapt_wavelet", "bmg_tree_adapt_wavelet_list", "bmg_tree_dump", "bmg_tree_restore", "bmg_tree_nonleaf", "bmg_tree_unrefine",
-    "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_set_axi", "bmg_grid_dump", "bmg_grid_restore",
+    "bmg_coef_gradient", "bmg_project", "bmg_viscosity", "bmg_viscosity_explicit", "bmg_set_axi", "bmg_grid_dump", "bmg_grid_restore",
     "bmg_tree_profile_enable", "bmg_tree_profile_get", "bmg_view_configure", "bmg_view_release",
     "bmg_poisson_noalloc", "bmg_centered_gradient", "bmg_correction", "bmg_tree_project",
     "bmg_tree_residual_transpose", "bmg_tree_cycle_transpose", "bmg_tree_adjoint",
@@ -246,6 +246,7 @@ def library():
         "bmg_project": (I, [P, dp, dp, dp, D, I, I, ctypes.POINTER(MGStats)]),
         "bmg_viscosity": (I, [P, dp, dp, dp, dp, dp, dp, D, dp, D, D, dp, I, I, ctypes.POINTER(MGStats)]),
         "bmg_set_axi": (I, [P, I]),
+        "bmg_viscosity_explicit": (I, [P, dp, dp, dp, dp, dp, dp, D, dp, D, D, dp, I, ctypes.POINTER(MGStats)]),
         "bmg_grid_dump": (I, [P, ctypes.c_char_p, dp, dp, I, D, I]),
         "bmg_grid_restore": (I, [P, ctypes.c_char_p, dp, dp, I, ctypes.POINTER(D), ctypes.POINTER(I)]),
     }
@@ -620,7 +621,7 @@ class Grid:
         them (mu times the face's y, rho times the cell's y)."""
         _check(self.lib.bmg_set_axi(self._g, 1 if axi else 0))
 
-    def viscosity(self, u, mu=1.0, rho=1.0, dt=1.0, sigma=None, nrelax=0):
+    def viscosity(self, u, mu=1.0, rho=1.0, dt=1.0, sigma=None, nrelax=0, explicit=False):
         """viscosity(), viscosity.h:163-178: the implicit viscous step
         u - dt/rho div(2 mu D(u)) = u_old for the cell components `u` (one
         (n,)*dim array per component, updated in place), with relax_viscosity
@@ -631,7 +632,9 @@ class Grid:
         dim) wall factors (default: -1 for the normal component, +1 for the
         tangential ones).  torch CUDA tensors stay on the device, numpy
         float64 arrays are updated through host copies.  Returns the
-        mg_solve statistics."""
+        mg_solve statistics.  explicit=True: viscosity_explicit()
+        (viscosity.h:180-190), u += residual_viscosity (u, u), returning
+        resb only."""
         if len(u) != self.dim:
             raise ValueError("need %d velocity components" % self.dim)
         shape = self.shape()
@@ -682,8 +685,12 @@ class Grid:
             import torch
             torch.cuda.current_stream().synchronize()
         s = MGStats()
-        _check(self.lib.bmg_viscosity(self._g, up[0], up[1], up[2], mp[0], mp[1], mp[2], mu_c, rp, rho_c,
-                                      float(dt), sp, int(nrelax), 1 if dev else 0, ctypes.byref(s)))
+        if explicit:
+            _check(self.lib.bmg_viscosity_explicit(self._g, up[0], up[1], up[2], mp[0], mp[1], mp[2], mu_c, rp, rho_c,
+                                                   float(dt), sp, 1 if dev else 0, ctypes.byref(s)))
+        else:
+            _check(self.lib.bmg_viscosity(self._g, up[0], up[1], up[2], mp[0], mp[1], mp[2], mu_c, rp, rho_c,
+                                          float(dt), sp, int(nrelax), 1 if dev else 0, ctypes.byref(s)))
         return s.as_dict()
 
     @staticmethod

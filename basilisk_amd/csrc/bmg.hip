@@ -2686,6 +2686,24 @@ struct Visc {
     return BMG_OK;
   }
 
+  // viscosity_explicit (viscosity.h:180-190): resb = residual_viscosity
+  // with b = u, then u += res
+  int explicit_step(bmg_mgstats* out) {
+    const Level& lf = g->L[g->depth];
+    for (int c = 0; c < DIM; c++)
+      HIPCHECK(hipMemcpyAsync(g->visc_r[c], u[c], lf.ncell * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+    bmg_mgstats s;
+    memset(&s, 0, sizeof s);
+    unsigned long long nans = 0;
+    TRY(residual(&s.resb, &nans));
+    for (int c = 0; c < DIM; c++)
+      hipLaunchKernelGGL((k_correct<double>), dim3(nblocks(lf.ncell)), dim3(kBlock), 0, g->stream, u[c],
+                         (const double*)vbuf(g, g->depth, VS_RES + c), lf.ncell);
+    TRY(check_launch());
+    if (out) *out = s;
+    return nans ? fail(BMG_ERR_NAN, "viscous residual is NaN") : BMG_OK;
+  }
+
   // mg_cycle (poisson.h:32-90) with minlevel 0 and relax_viscosity in the
   // grid's smoother: red-black (default) or the reference's JACOBI variant
   int cycle(int nrelax) {
@@ -2818,7 +2836,7 @@ int visc_set_face(bmg_grid* g, int d, const double* src, int on_device) {
 
 template <int DIM>
 int viscosity_t(bmg_grid* g, double* const* u, const double* const* mu, double mu_c, const double* rho, double rho_c,
-                double dt, const int (*sig)[6], int nrelax, int on_device, bmg_mgstats* stats) {
+                double dt, const int (*sig)[6], int nrelax, int on_device, bmg_mgstats* stats, bool expl) {
   TRY(visc_alloc(g));
   Visc<DIM> V;
   V.g = g;
@@ -2836,7 +2854,7 @@ int viscosity_t(bmg_grid* g, double* const* u, const double* const* mu, double m
   if (V.rho_field)
     HIPCHECK(hipMemcpyAsync(vbuf(g, g->depth, VS_RHO), rho, lf.ncell * sizeof(double),
                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, g->stream));
-  const int rc = V.solve(nrelax, stats);
+  const int rc = expl ? V.explicit_step(stats) : V.solve(nrelax, stats);
   HIPCHECK(hipStreamSynchronize(g->stream));
   return rc;
 }
@@ -2852,9 +2870,14 @@ int bmg_set_axi(bmg_grid* g, int axi) {
   return BMG_OK;
 }
 
-int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const double* mu_x, const double* mu_y,
-                  const double* mu_z, double mu_c, const double* rho, double rho_c, double dt, const int* sigma,
-                  int nrelax, int on_device, bmg_mgstats* stats) {
+}  // extern "C"
+
+namespace {
+
+// bmg_viscosity and bmg_viscosity_explicit: checks, host copies, the step
+int viscosity_entry(bmg_grid* g, double* u_x, double* u_y, double* u_z, const double* mu_x, const double* mu_y,
+                    const double* mu_z, double mu_c, const double* rho, double rho_c, double dt, const int* sigma,
+                    int nrelax, int on_device, bmg_mgstats* stats, bool expl) {
   TRY(check_grid(g));
   if (g->dim < 2) return fail(BMG_ERR_UNSUPPORTED, "viscosity() is built for 2D and 3D grids");
   double* u[3] = {u_x, u_y, u_z};
@@ -2889,8 +2912,8 @@ int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const doub
     for (int d = 0; d < g->dim; d++) dev[d] = u[d];
   }
   if (rc == BMG_OK)
-    rc = g->dim == 2 ? viscosity_t<2>(g, dev, mu, mu_c, rho, rho_c, dt, sig, nrelax, on_device, stats)
-                     : viscosity_t<3>(g, dev, mu, mu_c, rho, rho_c, dt, sig, nrelax, on_device, stats);
+    rc = g->dim == 2 ? viscosity_t<2>(g, dev, mu, mu_c, rho, rho_c, dt, sig, nrelax, on_device, stats, expl)
+                     : viscosity_t<3>(g, dev, mu, mu_c, rho, rho_c, dt, sig, nrelax, on_device, stats, expl);
   if (!on_device) {
     for (int d = 0; d < g->dim; d++) {
       if (dev[d] && (rc == BMG_OK || rc == BMG_ERR_NAN) &&
@@ -2900,6 +2923,23 @@ int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const doub
     }
   }
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmg_viscosity(bmg_grid* g, double* u_x, double* u_y, double* u_z, const double* mu_x, const double* mu_y,
+                  const double* mu_z, double mu_c, const double* rho, double rho_c, double dt, const int* sigma,
+                  int nrelax, int on_device, bmg_mgstats* stats) {
+  return viscosity_entry(g, u_x, u_y, u_z, mu_x, mu_y, mu_z, mu_c, rho, rho_c, dt, sigma, nrelax, on_device, stats,
+                         false);
+}
+
+int bmg_viscosity_explicit(bmg_grid* g, double* u_x, double* u_y, double* u_z, const double* mu_x,
+                           const double* mu_y, const double* mu_z, double mu_c, const double* rho, double rho_c,
+                           double dt, const int* sigma, int on_device, bmg_mgstats* stats) {
+  return viscosity_entry(g, u_x, u_y, u_z, mu_x, mu_y, mu_z, mu_c, rho, rho_c, dt, sigma, 0, on_device, stats, true);
 }
 
 }  // extern "C"

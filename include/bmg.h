@@ -238,6 +238,13 @@ int bmg_correction(bmg_grid *g, double *u_x, double *u_y, double *u_z, const dou
 int bmg_viscosity(bmg_grid *g, double *u_x, double *u_y, double *u_z, const double *mu_x, const double *mu_y,
                   const double *mu_z, double mu_c, const double *rho, double rho_c, double dt, const int *sigma,
                   int nrelax, int on_device, bmg_mgstats *stats);
+/* viscosity_explicit(), viscosity.h:180-190: one explicit step, res =
+   residual_viscosity with the right-hand side u, then u += res; stats get
+   resb = max |res| (the rest zero).  Arguments as bmg_viscosity (no
+   nrelax); the AXI form follows bmg_set_axi. */
+int bmg_viscosity_explicit(bmg_grid *g, double *u_x, double *u_y, double *u_z, const double *mu_x,
+                           const double *mu_y, const double *mu_z, double mu_c, const double *rho, double rho_c,
+                           double dt, const int *sigma, int on_device, bmg_mgstats *stats);
 /* the AXI form of viscosity() (axi.h, viscosity.h:12-14) on a 2D grid: x is
    the axis and y the radius, and the y component's lambda is
    1 + dt/rho (mu.x[] + mu.x[1] + mu.y[] + mu.y[0,1])/2/y^2 with y the cell

@@ -24,6 +24,8 @@ round it), of
                          old values)
   residual_viscosity     viscosity.h:98-160, the "naive" (!TREE) form, with
                          the same lambda
+  viscosity_explicit     viscosity.h:180-190: resb = residual_viscosity
+                         with b = u, then u += res (no restriction)
   mg_solve, mg_cycle     poisson.h:32-90, 139-235 (nrelax default 4, the
                          nrelax adaptation, NITERMIN/NITERMAX)
   restriction            restriction_average and face_average,
@@ -328,6 +330,13 @@ class Viscosity:
             for _ in range(nrelax):
                 da = self.relax(da, rl[l], l, dt)
         return [u[c] + da[c] for c in range(dim)]
+
+    def explicit(self, u, dt):
+        """viscosity_explicit(): returns (u + res, stats with only resb)."""
+        u = [np.array(x, np.float64) for x in u]
+        res, resb = self.residual(u, u, dt)
+        s = {"i": 0, "resb": resb, "resa": 0.0, "sum": 0.0, "nrelax": 0, "minlevel": 0}
+        return [u[c] + res[c] for c in range(self.dim)], s
 
     def solve(self, u, dt, nrelax=0, tolerance=0.0):
         """viscosity(): returns (u, stats, residuals after each cycle)."""

@@ -126,8 +126,9 @@ def test_gs_config2_bitexact():
 
 
 def test_gs_refused_where_the_order_differs():
-    """A full tree's GS runs in Morton order and viscosity() has no GS
-    relax_viscosity here: both are refused, not silently replaced."""
+    """A full tree's GS runs in Morton order, not the multigrid grid's
+    i-major order: refused, not silently replaced.  (viscosity() runs GS
+    in the i-major order: tests/test_gpu_viscosity.py.)"""
     g = bmg.Grid(2, 4, smoother="gs")
     with pytest.raises(bmg.BMGError):
         g.set_kind("tree")
@@ -135,7 +136,3 @@ def test_gs_refused_where_the_order_differs():
     g.set_kind("tree")
     with pytest.raises(bmg.BMGError):
         g.set_smoother("gs")
-    g2 = bmg.Grid(2, 4, smoother="gs")
-    u = [np.zeros((16, 16)), np.zeros((16, 16))]
-    with pytest.raises(bmg.BMGError):
-        g2.viscosity(u, dt=1e-3)

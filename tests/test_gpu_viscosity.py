@@ -221,3 +221,22 @@ def test_viscosity_gauss_seidel_bitexact(dim, depth, case):
         assert sg[k] == so[k], (k, sg, so)
     for c in range(dim):
         assert np.array_equal(ug[c], uo[c]), c
+
+
+@pytest.mark.parametrize("dim,depth,axi", [(2, 6, False), (2, 6, True), (3, 5, False)])
+def test_viscosity_explicit_bitexact(dim, depth, axi):
+    """viscosity_explicit(): u += residual_viscosity (u, u), resb."""
+    print("case viscosity explicit", dim, depth, axi, flush=True)
+    u, mu, rho = _problem(dim, depth, 71 * dim + depth, True)
+    dt = 0.003
+    g = bmg.Grid(dim, depth)
+    if axi:
+        g.set_axi(True)
+    o = Viscosity(dim, depth, axi=axi)
+    o.set_coefficients(mu, rho)
+    uo, so = o.explicit(u, dt)
+    ug = [x.copy() for x in u]
+    sg = g.viscosity(ug, mu=mu, rho=rho, dt=dt, explicit=True)
+    assert sg["resb"] == so["resb"] and sg["i"] == 0
+    for c in range(dim):
+        assert np.array_equal(ug[c], uo[c]), c

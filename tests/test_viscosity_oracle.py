@@ -360,3 +360,25 @@ def test_gs_cell_arithmetic_equals_the_vectorised_stencil(dim, depth, axi):
     first, last = (0,) * dim, (n - 1,) * dim
     assert all(new[c][first] == w[c][first] for c in range(dim))
     assert any(new[c][last] != w[c][last] for c in range(dim))
+
+
+@pytest.mark.parametrize("dim,depth", [(2, 5), (3, 4)])
+def test_explicit_step_of_a_quadratic(dim, depth):
+    """viscosity_explicit (viscosity.h:180-190) adds residual_viscosity
+    (u, u): for a quadratic field with constant mu and rho, in the interior
+    exactly dt/rho mu (lap u + grad div u)."""
+    V = Viscosity(dim, depth)
+    n = 1 << depth
+    mu, rho, dt = 0.7, 1.3, 0.01
+    V.set_coefficients(mu, rho)
+    x = _cells(dim, n)
+    rng = np.random.default_rng(10 + dim)
+    A = rng.standard_normal((dim, dim, dim))
+    u = [sum(A[c, a, b] * x[a] * x[b] for a in range(dim) for b in range(dim)) for c in range(dim)]
+    H = A + A.transpose(0, 2, 1)
+    new, st = V.explicit(u, dt)
+    inner = tuple(slice(2, n - 2) for _ in range(dim))
+    for c in range(dim):
+        du = dt / rho * mu * (sum(H[c, a, a] for a in range(dim)) + sum(H[a, a, c] for a in range(dim)))
+        assert np.abs((new[c] - u[c] - du)[inner]).max() < 1e-11
+    assert st["resb"] > 0 and st["i"] == 0
