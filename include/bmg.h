@@ -428,12 +428,13 @@ int bmg_tree_adapt_wavelet_list(bmg_tree *t, int n, const int *fields, const dou
 int bmg_tree_nonleaf(bmg_tree *t, long *n, int *level, double *xyz);
 /* unrefine (cond) (grid/tree-common.h:348-371) with cond given as one mark
    per cell of bmg_tree_nonleaf's list: the marked cells are coarsened from
-   the depth up (coarsen_cell, which the 2:1 rule may refuse); s, a and b
-   take the children's average; alpha and a lambda field must be set again.
+   the depth up (coarsen_cell, which the 2:1 rule may refuse); the user
+   scalars that are set, a and b take the children's average; alpha and a
+   lambda field must be set again.
    ncoarsened (may be NULL) receives the number of coarsened cells. */
 int bmg_tree_unrefine(bmg_tree *t, const unsigned char *mark, long *ncoarsened);
 /* dump() (output.h:1043-1118, the non-MPI branch, file version 170901) of
-   the leaf fields `fields` (BMG_F_A, BMG_F_B, BMG_F_S) under `names`: the
+   the leaf fields `fields` (BMG_F_A, BMG_F_B, BMG_F_S + k) under `names`: the
    header (struct DumpHeader: t, the field count with the subtree "size"
    first, iter, depth, npe = 1, version), the names, {X0, Y0, Z0, L0}, then
    every active cell in foreach_cell() order: unsigned flags (2 for a leaf),

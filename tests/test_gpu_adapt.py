@@ -4,7 +4,7 @@ The wavelet estimate, the field updates (refine_bilinear into new leaves, the
 children's average into coarsened cells) and the tree boundary run on the
 GPU; the refinement and coarsening passes over the topology on the host.
 After every call the adapted tree (depth, leaves, every cache) and the leaf
-values of s, a and b equal the oracle's bit for bit, and so do the refined /
+values of the user scalars, a and b equal the oracle's bit for bit, and so do the refined /
 coarsened counts.  The oracle's adaptation is pinned by halo.ref
 (test_adapt_oracle.py); the GPU tree reproduces its x, y, level columns too.
 """
