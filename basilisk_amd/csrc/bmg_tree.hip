@@ -694,6 +694,7 @@ struct DevIndex {
   // per level and colour: the active slots, then the leaf slots (red-black
   // passes launch over one colour only)
   int* col_list = nullptr;
+  int* col_nbr = nullptr;  // the neighbour-table rows of col_list's entries, in its order
   std::vector<int> col_off, col_n, lcol_off, lcol_n;  // [2 * level + colour]
   BoxRule* box = nullptr;
   FaceOp* fops = nullptr;
@@ -852,7 +853,7 @@ void free_device(bmg_tree* t) {
   drop_graphs(t);
   DevIndex& x = t->ix;
   for (void* p : {(void*)x.nbr, (void*)x.rec_self, (void*)x.rec_child0, (void*)x.rec_ring, (void*)x.colour,
-                  (void*)x.leaf_gs, (void*)x.lvl_leaves, (void*)x.col_list, (void*)x.box, (void*)x.fops, (void*)x.hf, (void*)x.fr,
+                  (void*)x.leaf_gs, (void*)x.lvl_leaves, (void*)x.col_list, (void*)x.col_nbr, (void*)x.box, (void*)x.fops, (void*)x.hf, (void*)x.fr,
                   (void*)x.fp, (void*)x.fc, (void*)x.fv_gs, (void*)x.fv_d, (void*)x.fc_tmp, (void*)x.lvinfo})
     dfree_(p);
   x = DevIndex();
@@ -1498,6 +1499,26 @@ int compile(bmg_tree* t) {
         x.lcol_n[2 * l + c] = (int)cl.size() - x.lcol_off[2 * l + c];
       }
     TRY(upload(&x.col_list, cl));
+    // the neighbour rows of the list entries (level m of slot gs: the one
+    // whose slot range holds it), so that a colour pass reads them in order
+    std::vector<int> cnb(cl.size() * 2 * dim);
+    std::vector<int8_t> slot_level(x.nslot, -1);
+    for (int l = 0; l <= D; l++)
+      for (int q = 0; q < x.nA[l]; q++) slot_level[(size_t)x.off[l] + q] = (int8_t)l;
+    std::atomic<bool> bad{false};
+    par_for(cl.size(), [&](size_t e0, size_t e1) {
+      for (size_t e = e0; e < e1; e++) {
+        const int gs = cl[e], m = gs >= 0 && gs < (int)x.nslot ? slot_level[gs] : -1;
+        if (m < 0) {
+          bad = true;
+          return;
+        }
+        const size_t ai = (size_t)x.abase[m] + (gs - x.off[m]);
+        for (int k = 0; k < 2 * dim; k++) cnb[e * 2 * dim + k] = nbr[ai * 2 * dim + k];
+      }
+    });
+    if (bad) return bmg::fail(BMG_ERR_ARG, "tree: colour list entry outside the active cells");
+    TRY(upload(&x.col_nbr, cnb));
   }
   TRY(upload(&x.box, box));
   TRY(upload(&x.fops, fops));
@@ -1630,13 +1651,10 @@ Segs level_or_leaf_colour(const bmg_tree* t, int l, int c) {
     S.abase[s] = x.abase[m];
     S.lvoff[s] = x.off[m];
     S.D[s] = t->L0 * (1. / (1 << m));
-    if (m == l) {
-      S.count[s] = x.col_n[2 * m + c];
-      S.list[s] = x.col_list + x.col_off[2 * m + c];
-    } else {
-      S.count[s] = x.lcol_n[2 * m + c];
-      S.list[s] = x.col_list + x.lcol_off[2 * m + c];
-    }
+    const int off = m == l ? x.col_off[2 * m + c] : x.lcol_off[2 * m + c];
+    S.count[s] = m == l ? x.col_n[2 * m + c] : x.lcol_n[2 * m + c];
+    S.list[s] = x.col_list + off;
+    S.nbl[s] = x.col_nbr + (size_t)off * 2 * t->dim;
     blocks += (int)nb_(S.count[s]);
   }
   S.first_block[S.n] = blocks;

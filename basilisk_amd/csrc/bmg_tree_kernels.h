@@ -36,6 +36,9 @@ struct Segs {
   int lvoff[kSeg];
   double D[kSeg];   // Delta of the segment's level
   const int* list[kSeg];
+  // colour lists: the 2 dim neighbour slots of list entry t at nbl + 2 dim t
+  // (the rows of the neighbour table in list order, read contiguously)
+  const int* nbl[kSeg];
 };
 
 __device__ __forceinline__ int seg_of(const Segs& S) {
@@ -187,9 +190,7 @@ struct RelaxCoef {
 // the value n/d of poisson.h:303-320 at slot gs (active index ai)
 template <typename T, int DIM>
 __device__ __forceinline__ T t_relax_value(const T* __restrict__ da, const T* __restrict__ res,
-                                           const int* __restrict__ nbr, const RelaxCoef<T>& k, int gs, int ai,
-                                           T D2) {
-  const int* nb = nbr + (size_t)ai * 2 * DIM;
+                                           const int* __restrict__ nb, const RelaxCoef<T>& k, int gs, T D2) {
   T n = -D2 * res[gs], d = -(k.lam ? k.lam[gs] : k.lam_c) * D2;
 #pragma unroll
   for (int c = 0; c < DIM; c++) {
@@ -215,7 +216,9 @@ __global__ __launch_bounds__(kB) void k_t_relax(Segs S, T* __restrict__ da, T* _
   int ai = S.abase[s] + (gs - S.lvoff[s]);
   if ((sel == 0 || sel == 1) && colour[ai] != sel) return;
   T D = (T)S.D[s];
-  T v = t_relax_value<T, DIM>(da, res, nbr, k, gs, ai, D * D);
+  const int t = ((int)blockIdx.x - S.first_block[s]) * kB + (int)threadIdx.x;
+  const int* nb = S.nbl[s] ? S.nbl[s] + (size_t)t * 2 * DIM : nbr + (size_t)ai * 2 * DIM;
+  T v = t_relax_value<T, DIM>(da, res, nb, k, gs, D * D);
   if (sel < 0)
     out[gs] = v;
   else
@@ -577,7 +580,7 @@ __device__ __forceinline__ void t_ops_cells(const TreeArgs<T>& A, const TOp& op)
       } else {
         if (op.arg >= 0 && A.colour[ai] != op.arg) continue;
         const T D = (T)Dm;
-        T v = t_relax_value<T, DIM>(da, A.res, A.nbr, A.k, gs, ai, D * D);
+        T v = t_relax_value<T, DIM>(da, A.res, A.nbr + (size_t)ai * 2 * DIM, A.k, gs, D * D);
         if (op.arg < 0)
           A.fld[2][gs] = v;
         else
