@@ -793,7 +793,8 @@ class Tree:
 
     def refine(self, cond):
         """refine(cond): passes until no leaf satisfies
-        ``cond(x, y, z, level) -> bool array`` (vectorised over the leaves)."""
+        ``cond(x, y, z, level) -> bool array`` (vectorised over the leaves);
+        the user scalars that are set, a and b follow the mesh."""
         while True:
             lev, xyz = self.leaves()
             mark = np.asarray(cond(xyz[:, 0], xyz[:, 1], xyz[:, 2], lev), dtype=np.uint8)

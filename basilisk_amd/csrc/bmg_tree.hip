@@ -2459,9 +2459,19 @@ int bmg_tree_refine(bmg_tree* t, const unsigned char* mark, long* nrefined) {
   if (!t || !mark) return bmg::fail(BMG_ERR_ARG, "null argument");
   Topo& T = t->topo;
   T.update();
-  // keep the leaf fields across the mesh change only when nothing changes
+  // on a compiled tree the fields follow the mesh: boundary (all), then
+  // refine_cell (point, all) logs each refined cell's ring for the remap
+  const bool carry = t->compiled && !T.dirty;
+  if (carry) {
+    if (t->dim == 2)
+      adapt_boundary<2>(t);
+    else
+      adapt_boundary<3>(t);
+    HIPCHECK(hipStreamSynchronize(t->stream));
+  }
   std::vector<int> ll = T.leaf_level;
   std::vector<IJK> lc = T.leaf_c;
+  RefineLog log;
   long nr = 0;
   for (size_t q = 0; q < lc.size(); q++) {
     if (!mark[q]) continue;
@@ -2469,19 +2479,37 @@ int bmg_tree_refine(bmg_tree* t, const unsigned char* mark, long* nrefined) {
     if (!(c.flags & C_LEAF)) continue;
     if (ll[q] + 1 >= kSeg || ll[q] + 1 > kMaxDepth)
       return bmg::fail(BMG_ERR_UNSUPPORTED, "refinement beyond level %d", std::min(kSeg - 1, kMaxDepth));
-    T.refine_cell(ll[q], lc[q]);
+    T.refine_cell(ll[q], lc[q], 0, carry ? &log : nullptr);
     nr++;
   }
   if (nr) {
     T.dirty = true;
-    t->compiled = false;
     t->coef_ready = false;
-    t->keep_a.clear();
-    t->keep_b.clear();
-    t->keep_lam.clear();
-    for (int k = 0; k < kUserScalars; k++) {
-      t->keep_us[k].clear();
-      t->us_set[k] = false;
+    if (carry) {
+      if (!log.ok) {
+        t->compiled = false;
+        return bmg::fail(BMG_ERR_ARG, "tree refinement met an inconsistent tree");
+      }
+      double* olds[kUserScalars + 2];
+      take_mesh_fields(t, olds);
+      const long old_nslot = t->ix.nslot;
+      std::vector<int> prev;
+      t->prev_slots = &prev;
+      int rc = compile(t);
+      t->prev_slots = nullptr;
+      if (rc == BMG_OK)
+        rc = t->dim == 2 ? adapt_remap<2>(t, olds, old_nslot, prev, log) : adapt_remap<3>(t, olds, old_nslot, prev, log);
+      for (double* p : olds) dfree_(p);
+      if (rc != BMG_OK) return rc;
+    } else {
+      t->compiled = false;
+      t->keep_a.clear();
+      t->keep_b.clear();
+      t->keep_lam.clear();
+      for (int k = 0; k < kUserScalars; k++) {
+        t->keep_us[k].clear();
+        t->us_set[k] = false;
+      }
     }
   }
   if (nrefined) *nrefined = nr;

@@ -399,7 +399,12 @@ int bmg_tree_info(bmg_tree *t, int *depth, long *nleaves, long *ncells);
 int bmg_tree_leaves(bmg_tree *t, int *level, double *xyz);
 /* one pass of refine() (grid/tree-common.h:308-326): refine every leaf
    whose mark (leaf order) is non-zero, with 2:1 balancing; repeat passes
-   until no leaf is marked.  Fields set before are discarded. */
+   until no leaf is marked.  Once the tree holds fields (it has been
+   compiled by a field or solver call), each pass starts with boundary (all)
+   and the user scalars that are set, a and b follow the mesh
+   (refine_bilinear into the new leaves, refine_cell (point, all)); alpha
+   and a lambda field must be set again.  Before that there is nothing to
+   carry. */
 int bmg_tree_refine(bmg_tree *t, const unsigned char *mark, long *nrefined);
 /* adapt_wavelet ({s}, {max}, maxlevel, minlevel) (grid/tree-common.h:165-303)
    for the leaf scalar s (BMG_F_S, set first): the children of every refined

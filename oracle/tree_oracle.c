@@ -578,8 +578,15 @@ void ot_leaves(ot_tree *t, int *level, double *xyz) {
    marked in cache order; the fixed point reached by repeated passes does not
    depend on the visiting order, since refine_cell only refines what 2:1
    balance requires */
+static void mesh_fields_boundary(ot_tree *t);
+static void refine_cell_fields(ot_tree *t, int l, const int c[3]);
+
+/* one pass of refine (cond) (grid/tree-common.h:308-326): boundary (all),
+   then refine_cell (point, all) on every marked leaf in foreach_leaf()
+   order; the user scalars, a and b get refine_bilinear in the new cells */
 int ot_refine_marked(ot_tree *t, const unsigned char *mark) {
   ot_update(t);
+  mesh_fields_boundary(t);
   long n = t->nleaves;
   int *ll = malloc(sizeof(int) * n), *li = malloc(sizeof(int) * n);
   memcpy(ll, t->leaves_l.p, sizeof(int) * n);
@@ -591,7 +598,7 @@ int ot_refine_marked(ot_tree *t, const unsigned char *mark) {
     otl *L = &t->L[l];
     if (!(L->flags[li[q]] & C_LEAF)) continue; /* refined by 2:1 meanwhile */
     lijk(L, li[q], c);
-    refine_cell(t, l, c);
+    refine_cell_fields(t, l, c);
     refined++;
   }
   free(ll); free(li);
@@ -1736,6 +1743,19 @@ static void adapt_coarsen_visit(ot_tree *t, int m, const int c[3], int l, int *n
           int cc[3] = {2 * c[0] - GH + a, 2 * c[1] - GH + b, DIM == 3 ? 2 * c[2] - GH + e : 0};
           adapt_coarsen_visit(t, m + 1, cc, l, nc);
         }
+}
+
+/* boundary (all) for the fields that follow the mesh: box ghosts of the
+   depth on a full tree (every leaf is there), else the tree boundary */
+static void mesh_fields_boundary(ot_tree *t) {
+  if (is_full(t))
+    for (int m = 0; m < NADAPT; m++) box_scalar(t, adapt_list[m], adapt_bk[m], t->depth);
+  else
+    for (int m = 0; m < NADAPT; m++) tree_boundary_scalar(t, adapt_list[m], adapt_bk[m], -1);
+}
+
+static void refine_cell_fields(ot_tree *t, int l, const int c[3]) {
+  refine_cell_f(t, l, c, 0, adapt_list, NADAPT);
 }
 
 /* adapt_wavelet (slist, max, maxlevel, minlevel) on a Cartesian tree

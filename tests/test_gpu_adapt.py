@@ -249,3 +249,31 @@ def test_adapt_list_bitexact(dim, base, maxlevel, minlevel):
         g.adapt_wavelet([1e-2], maxlevel, scalars=[2])
     with pytest.raises(ValueError):
         g.set_s(np.zeros(g.nleaves()), k=bmg.USER_SCALARS)
+
+
+@pytest.mark.parametrize("dim,base,maxlevel", [(2, 4, 7), (3, 3, 5)])
+def test_refine_carries_fields_bitexact(dim, base, maxlevel):
+    """refine (cond) (grid/tree-common.h:308-326) on a tree that holds
+    fields: each pass applies boundary (all) and refine_cell (point, all),
+    so the user scalars, a and b get refine_bilinear in the new leaves (2:1
+    cascades included); tree and fields equal the oracle's."""
+    print("refine carries fields", dim, base, maxlevel, flush=True)
+    g = bmg.Tree(dim, 1.0, (0.0, 0.0, 0.0))
+    o = OracleTree(dim, 1.0, (0.0, 0.0, 0.0))
+    for t in (g, o):
+        t.refine(lambda x, y, z, lev: lev < base)
+        _, xyz = t.leaves()
+        p = xyz[:, :dim]
+        t.set_a(np.sin(3 * p[:, 0]) + p[:, 1] ** 2)
+        t.set_b(np.cos(2 * p.sum(1)))
+        t.set_s(_bumps(p, [np.full(dim, 0.4)], 0.1), k=0)
+        t.set_s(p[:, 0] - 2 * p[:, 1], k=2)
+    for rnd, (c, r) in enumerate(((0.3, 0.15), (0.62, 0.1))):
+        cond = lambda x, y, z, lev: ((x - c) ** 2 + (y - c) ** 2 < r * r) & (lev < maxlevel - rnd)
+        for t in (g, o):
+            t.refine(cond)
+        _same_tree(g, o)
+        for k in (0, 2):
+            assert np.array_equal(g.get_s(k=k), o.get_s(k)), k
+        assert np.array_equal(g.get_a(), o.get_a())
+        assert np.array_equal(g.get_b(), o.get_b())
