@@ -727,6 +727,8 @@ unsigned nb_(long n) { return (unsigned)((n + kB - 1) / kB); }
 
 // user scalars of a tree (BMG_F_S .. BMG_F_S + kUserScalars - 1)
 constexpr int kUserScalars = 4;
+// largest operation the batched small-operation kernel takes by default
+constexpr long kSmallOp = 1024;
 
 struct bmg_tree {
   int dim = 2;
@@ -778,6 +780,7 @@ struct bmg_tree {
   bool use_graphs = true;
   int ops_block = kOpsBlock;  // threads of the batched small-operation kernel (BMG_OPS_BLOCK)
   bool batching = true;  // small operations run batched in one workgroup (BMG_TREE_BATCH=0: off)
+  long small_op = kSmallOp;  // largest batched operation (BMG_TREE_SMALL_OP overrides)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> cycle_graphs;
   hipGraphExec_t residual_graph = nullptr;
   // relax-kernel timing (bmg_tree_profile_*): HIP events around every
@@ -1758,10 +1761,9 @@ int relax(bmg_tree* t, int l) {
   return BMG_OK;
 }
 
-// Operations of at most kSmallOp elements are queued and run by one
-// workgroup (k_t_ops); a larger one first flushes the queue, then runs as its
-// own kernel, so the order of the chain is kept.
-constexpr long kSmallOp = 2048;
+// Operations of at most small_op elements (default kSmallOp) are queued and
+// run by one workgroup (k_t_ops); a larger one first flushes the queue, then
+// runs as its own kernel, so the order of the chain is kept.
 
 template <typename T, int DIM>
 struct Batcher {
@@ -1788,7 +1790,7 @@ struct Batcher {
     A.bc = bct(t);
     A.L0 = t->L0;
   }
-  bool small(long n) const { return t->batching && n <= kSmallOp; }
+  bool small(long n) const { return t->batching && n <= t->small_op; }
   void flush() {
     if (P.n) k_t_ops<T, DIM><<<1, t->ops_block, 0, t->stream>>>(P, A);
     P.n = 0;
@@ -2404,6 +2406,10 @@ int bmg_tree_new(bmg_tree** out, int dimension, double L0, double X0, double Y0,
   }
   const char* tb = getenv("BMG_TREE_BATCH");
   t->batching = !(tb && tb[0] == '0');
+  if (const char* so = getenv("BMG_TREE_SMALL_OP")) {
+    const long v = atol(so);
+    if (v >= 64 && v <= (1L << 20)) t->small_op = v;
+  }
   *out = t;
   return BMG_OK;
 }
