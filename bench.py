@@ -163,7 +163,7 @@ def tree_bench(args):
         achieved = per_cell * rcells / (rms * 1e-3) / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                            "kernel": "k_t_relax (%s), %d launches of more than 2048 cells per solve"
+                            "kernel": "k_t_relax (%s), %d launches of more than 1024 cells per solve"
                                       % (main_prec, rn),
                             "launch_us": round(rms / rn * 1e3, 2), "cells_per_launch": round(rcells / rn),
                             "bytes_per_cell": per_cell}
